@@ -1,0 +1,286 @@
+"""Scene descriptions for the projective-dynamics engine.
+
+A ``SceneDesc`` holds everything ``Dfobj::SetDfobj`` + ``Environment``'s JSON
+parameters carry for the hot path (sim/Environment.h:109-210,
+sim/Dfobj.h:142-314): rest vertices, tets (unoriented; orientation is fixed at
+setup as Dfobj.h:474-487 does), total mass, material, attachments, muscle
+segments, contact obstacles and the stepping parameters.
+
+Two sources build one:
+  * ``box_grid`` / ``make_config`` — seeded Kuhn-split box grids (edge 0.05 m)
+    for the five benchmark workloads;
+  * ``load_json_scene`` — the reference's JSON scene + ``v``/``t`` OBJ tet mesh
+    format (sim/fem/Mesh/OBJMesh.h:25-97), missing keys read as the reference's
+    defaults.
+"""
+from __future__ import annotations
+
+import dataclasses
+import json
+import math
+import os
+from typing import List, Optional
+
+import numpy as np
+
+EDGE = 0.05
+
+
+@dataclasses.dataclass
+class SceneDesc:
+    x_rest: np.ndarray                 # (N,3) float64
+    tets: np.ndarray                   # (T,4) int32, as read (unoriented)
+    total_mass: float = 10.0           # Dfobj.h:114 default
+    mass_same: bool = True             # "Is Mass Same"
+    E: np.ndarray = None               # (T,) Young's modulus per tet
+    nu_fwd: float = 0.4                # value entering mMu = E/(1+nu)
+    nu_ctor: np.ndarray = None         # (T,) nu of the volume-term weight
+    attach_idx: np.ndarray = None      # (A,) int32
+    attach_w: float = 1e20
+    # muscles: per segment start/end points and its activation group
+    seg_start: np.ndarray = None       # (S,3)
+    seg_end: np.ndarray = None         # (S,3)
+    seg_group: np.ndarray = None       # (S,) int32
+    n_groups: int = 0
+    k_muscle: float = 5e5              # Environment.h:185 Dfobj(5E5, ...)
+    # stepping
+    h: float = 1.0 / 200
+    max_iter: int = 30
+    damping: float = 0.9999
+    gravity: tuple = (0.0, 0.0, 0.0)
+    gravity_on: bool = True
+    contact: bool = False
+    max_iter1: int = 30
+    alpha: float = 1.0
+    mu: float = 1.0
+    planes: np.ndarray = None          # (P,6): point, normal
+    spheres: np.ndarray = None         # (Q,4): centre, radius
+    name: str = "scene"
+
+    def __post_init__(self):
+        self.x_rest = np.ascontiguousarray(self.x_rest, dtype=np.float64).reshape(-1, 3)
+        self.tets = np.ascontiguousarray(self.tets, dtype=np.int32).reshape(-1, 4)
+        T = self.tets.shape[0]
+        if self.E is None:
+            self.E = np.full(T, 2e5)
+        self.E = np.ascontiguousarray(np.broadcast_to(np.asarray(self.E, dtype=np.float64), (T,)))
+        if self.nu_ctor is None:
+            self.nu_ctor = np.full(T, 0.4)
+        self.nu_ctor = np.ascontiguousarray(np.broadcast_to(np.asarray(self.nu_ctor, dtype=np.float64), (T,)))
+        if self.attach_idx is None:
+            self.attach_idx = np.zeros(0, dtype=np.int32)
+        self.attach_idx = np.ascontiguousarray(self.attach_idx, dtype=np.int32)
+        if self.seg_start is None:
+            self.seg_start = np.zeros((0, 3))
+            self.seg_end = np.zeros((0, 3))
+            self.seg_group = np.zeros(0, dtype=np.int32)
+        self.seg_start = np.ascontiguousarray(self.seg_start, dtype=np.float64).reshape(-1, 3)
+        self.seg_end = np.ascontiguousarray(self.seg_end, dtype=np.float64).reshape(-1, 3)
+        self.seg_group = np.ascontiguousarray(self.seg_group, dtype=np.int32)
+        if self.planes is None:
+            self.planes = np.zeros((0, 6))
+        if self.spheres is None:
+            self.spheres = np.zeros((0, 4))
+        self.planes = np.ascontiguousarray(self.planes, dtype=np.float64).reshape(-1, 6)
+        self.spheres = np.ascontiguousarray(self.spheres, dtype=np.float64).reshape(-1, 4)
+
+    @property
+    def N(self) -> int:
+        return self.x_rest.shape[0]
+
+    @property
+    def T(self) -> int:
+        return self.tets.shape[0]
+
+
+def box_grid(nx: int, ny: int, nz: int, edge: float = EDGE, origin=(0.0, 0.0, 0.0)):
+    """Axis-aligned grid of nx*ny*nz cubes, each split into 6 tets around the
+    (0,0,0)-(1,1,1) diagonal (Kuhn split). Vertex (i,j,k) has index
+    (i*(ny+1)+j)*(nz+1)+k."""
+    ii, jj, kk = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), np.arange(nz + 1), indexing="ij")
+    x = np.stack([ii, jj, kk], -1).reshape(-1, 3).astype(np.float64) * edge + np.asarray(origin, dtype=np.float64)
+
+    def vid(i, j, k):
+        return (i * (ny + 1) + j) * (nz + 1) + k
+
+    perms = [(0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)]
+    ci, cj, ck = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    ci, cj, ck = ci.ravel(), cj.ravel(), ck.ravel()
+    tets = []
+    for (a, b, _c) in perms:
+        e = np.zeros((4, 3), dtype=np.int64)
+        e[1, a] = 1
+        e[2, a] = 1
+        e[2, b] = 1
+        e[3, :] = 1
+        tets.append(np.stack([vid(ci + e[q, 0], cj + e[q, 1], ck + e[q, 2]) for q in range(4)], -1))
+    tets = np.stack(tets, 1).reshape(-1, 4).astype(np.int32)
+    return x, tets
+
+
+def straight_fibres(nfib_y: int, nfib_z: int, lo, hi, nseg: int = 50, jitter: float = 1.3e-3):
+    """Straight fibres along +x on an nfib_y x nfib_z lattice of the box
+    cross-section [lo, hi], each split into ``nseg`` equal segments (the 50
+    segments of Dfobj::MakeMuscles, Dfobj.h:767). A small offset keeps the
+    fibres off mesh faces and edges."""
+    lo = np.asarray(lo, dtype=np.float64)
+    hi = np.asarray(hi, dtype=np.float64)
+    starts, ends, groups = [], [], []
+    g = 0
+    x0 = lo[0] + 0.25 * EDGE
+    x1 = hi[0] - 0.25 * EDGE
+    for a in range(nfib_y):
+        for b in range(nfib_z):
+            y = lo[1] + (a + 0.5) / nfib_y * (hi[1] - lo[1]) + jitter
+            z = lo[2] + (b + 0.5) / nfib_z * (hi[2] - lo[2]) + 0.7 * jitter
+            xs = np.linspace(x0, x1, nseg + 1)
+            for s in range(nseg):
+                starts.append((xs[s], y, z))
+                ends.append((xs[s + 1], y, z))
+                groups.append(g)
+            g += 1
+    return np.array(starts), np.array(ends), np.array(groups, dtype=np.int32), g
+
+
+def config1() -> SceneDesc:
+    """12x4x4 cantilever bar, E=1e5, nu=0.4, rho=1000 (Mass = 24 kg), x=0 face
+    pinned, gravity -z, h=0.01, 10 iterations, no contact."""
+    nx, ny, nz = 12, 4, 4
+    x, t = box_grid(nx, ny, nz)
+    pinned = np.nonzero(np.isclose(x[:, 0], 0.0))[0].astype(np.int32)
+    vol = nx * ny * nz * EDGE ** 3
+    return SceneDesc(x_rest=x, tets=t, total_mass=1000.0 * vol, E=np.full(t.shape[0], 1e5), nu_fwd=0.4,
+                     nu_ctor=np.full(t.shape[0], 0.4), attach_idx=pinned, attach_w=1e20, h=0.01, max_iter=10,
+                     damping=0.9999, gravity=(0.0, 0.0, -9.81), contact=False, name="c1_bar_12x4x4")
+
+
+def _muscle_box(nx, ny, nz, seed, drop=0.1, mu=0.5, fibres=(8, 8), name="box"):
+    rng = np.random.default_rng(seed)
+    x, t = box_grid(nx, ny, nz, origin=(0.0, 0.0, drop))
+    T = t.shape[0]
+    E = rng.uniform(5e4, 2e5, size=T)
+    lo = np.array([0.0, 0.0, drop])
+    hi = np.array([nx * EDGE, ny * EDGE, nz * EDGE + drop])
+    s0, s1, grp, ng = straight_fibres(fibres[0], fibres[1], lo, hi)
+    vol = nx * ny * nz * EDGE ** 3
+    return SceneDesc(x_rest=x, tets=t, total_mass=1000.0 * vol, E=E, nu_fwd=0.4, nu_ctor=np.full(T, 0.4),
+                     seg_start=s0, seg_end=s1, seg_group=grp, n_groups=ng, k_muscle=5e5, h=0.01, max_iter=10,
+                     damping=0.9999, gravity=(0.0, 0.0, -9.81), contact=True, max_iter1=10, alpha=1.0, mu=mu,
+                     planes=np.array([[0.0, 0.0, 0.0, 0.0, 0.0, 1.0]]), name=name)
+
+
+def config2(seed: int = 2) -> SceneDesc:
+    """30x10x10 muscle body (18,000 tets) with per-tet E ~ U[5e4,2e5], 64
+    straight fibres along x, ground z=0 with mu=0.5, dropped from 0.1 m."""
+    return _muscle_box(30, 10, 10, seed, name="c2_muscle_30x10x10")
+
+
+def config4(seed: int = 4) -> SceneDesc:
+    """40x12x12 muscle body (34,560 tets), 64 fibres, ground mu=0.5."""
+    return _muscle_box(40, 12, 12, seed, name="c4_muscle_40x12x12")
+
+
+def activations_uniform(seed: int, steps: int, n_groups: int, R: int = 1) -> np.ndarray:
+    """Config-2 activations: uniform [0,1] per (rollout, step, fibre)."""
+    rng = np.random.default_rng(seed)
+    return rng.uniform(0.0, 1.0, size=(R, steps, n_groups))
+
+
+def activations_sinusoid(seed: int, steps: int, n_groups: int, R: int, h: float) -> np.ndarray:
+    """Config-4 activations: a(t) = (1 + sin(2 pi f t + phi))/2 with
+    f ~ U[0.5, 4] Hz and phi ~ U[0, 2 pi) per (rollout, fibre)."""
+    rng = np.random.default_rng(seed)
+    f = rng.uniform(0.5, 4.0, size=(R, 1, n_groups))
+    ph = rng.uniform(0.0, 2 * math.pi, size=(R, 1, n_groups))
+    t = (np.arange(steps, dtype=np.float64) * h)[None, :, None]
+    return 0.5 * (1.0 + np.sin(2 * math.pi * f * t + ph))
+
+
+def load_obj_tet(path: str, scale: float = 0.05):
+    """``v x y z`` / ``t i0 i1 i2 i3`` (0-based) tet OBJ, vertices scaled by
+    ``scale`` (OBJMesh.h:40-97; Dfobj.h:163-190 default Scale 0.05)."""
+    vs, ts = [], []
+    with open(path) as f:
+        for line in f:
+            p = line.split()
+            if not p:
+                continue
+            if p[0] == "v":
+                vs.append([float(p[1]), float(p[2]), float(p[3])])
+            elif p[0] == "t":
+                ts.append([int(p[1]), int(p[2]), int(p[3]), int(p[4])])
+    return np.asarray(vs, dtype=np.float64) * scale, np.asarray(ts, dtype=np.int32)
+
+
+def load_json_scene(config_path: str, data_root: Optional[str] = None) -> SceneDesc:
+    """Reference JSON scene (first mesh), keys and defaults of
+    Environment.h:109-181 and Dfobj.h:142-250. ``config_path`` is resolved
+    against ``data_root`` (the reference prefixes SOFTCON_DIR, Environment.h:90);
+    mesh files are resolved against ``<data_root>/data``."""
+    root = data_root if data_root is not None else os.environ.get("PDM_DATA_ROOT", "")
+    path = os.path.join(root, config_path.lstrip("/")) if root else config_path
+    if not os.path.exists(path):
+        raise ValueError("Couldn't open file " + path)
+    with open(path) as f:
+        js = json.load(f)
+    para = js["Parameters"][0]
+    hz = int(para.get("Simulation HZ", 200))
+    max_iter = int(para.get("Max Iteration", 30))
+    damping = float(para.get("Damping coefficient", 0.9999))
+    contact = bool(para.get("Handle collision", False))
+    gravity = tuple(float(g) for g in para.get("Gravity", [0.0, 0.0, 0.0]))
+    alpha = float(para.get("World Alpha", 1.0))
+    meshes = js.get("Meshes", [])
+    if len(meshes) != 1:
+        raise ValueError("exactly one mesh per scene is supported (got %d)" % len(meshes))
+    m = meshes[0]
+    scale = float(m.get("Scale", 0.05))
+    data_dir = os.path.join(root, "data") if root else os.path.dirname(path)
+    mesh_path = os.path.join(data_dir, m["Obj filename"])
+    if not os.path.exists(mesh_path):
+        mesh_path = os.path.join(os.path.dirname(path), os.path.basename(m["Obj filename"]))
+    x, t = load_obj_tet(mesh_path, scale)
+    T = t.shape[0]
+    E = float(m.get("Youngs Modulus", 2e5))
+    nu = float(m.get("Poisson Ratio", 0.4))
+    attach = np.asarray(sorted(set(int(a) for a in m.get("Attach Verts", []))), dtype=np.int32)
+    return SceneDesc(x_rest=x, tets=t, total_mass=float(m.get("Mass", 10.0)),
+                     mass_same=bool(m.get("Is Mass Same", True)), E=np.full(T, E), nu_fwd=nu,
+                     nu_ctor=np.full(T, nu), attach_idx=attach, attach_w=float(m.get("Attach Stiffness", 0.0)),
+                     k_muscle=float(m.get("Muscle Stiffness", 5e5)), h=1.0 / hz, max_iter=max_iter,
+                     damping=damping, gravity=gravity, gravity_on=bool(m.get("Enable Gravity", 1)),
+                     contact=contact, max_iter1=max_iter, alpha=alpha, mu=1.0,
+                     name=os.path.splitext(os.path.basename(config_path))[0])
+
+
+def config5_meshes(seed: int = 5, n_meshes: int = 64, tmin: int = 2000, tmax: int = 60000) -> List[SceneDesc]:
+    """64 seeded box meshes with T ~ logU[2k,60k], E ~ logU[5e4,2e5],
+    nu ~ U[0.3,0.45]; even-indexed meshes carry 16 fibres, odd ones are pure
+    elastic; ground plane plus a rigid sphere r=0.2 m."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n_meshes):
+        target = math.exp(rng.uniform(math.log(tmin), math.log(tmax)))
+        cubes = max(1, int(round(target / 6)))
+        # aspect 3:1:1
+        ny = max(2, int(round((cubes / 3.0) ** (1.0 / 3.0))))
+        nz = ny
+        nx = max(2, int(round(cubes / (ny * nz))))
+        x, t = box_grid(nx, ny, nz, origin=(-0.5 * nx * EDGE, -0.5 * ny * EDGE, 0.35))
+        T = t.shape[0]
+        E = math.exp(rng.uniform(math.log(5e4), math.log(2e5)))
+        nu = rng.uniform(0.3, 0.45)
+        sph = np.array([[rng.uniform(-0.2, 0.2), rng.uniform(-0.1, 0.1), 0.1, 0.2]])
+        kw = {}
+        if i % 2 == 0:
+            lo = x.min(0)
+            hi = x.max(0)
+            s0, s1, grp, ng = straight_fibres(4, 4, lo, hi)
+            kw = dict(seg_start=s0, seg_end=s1, seg_group=grp, n_groups=ng)
+        vol = nx * ny * nz * EDGE ** 3
+        out.append(SceneDesc(x_rest=x, tets=t, total_mass=1000.0 * vol, E=np.full(T, E), nu_fwd=nu,
+                             nu_ctor=np.full(T, nu), h=0.01, max_iter=10, damping=0.9999,
+                             gravity=(0.0, 0.0, -9.81), contact=True, max_iter1=10, mu=0.5,
+                             planes=np.array([[0.0, 0.0, 0.0, 0.0, 0.0, 1.0]]), spheres=sph,
+                             name="c5_mesh%02d_%dx%dx%d" % (i, nx, ny, nz), **kw))
+    return out

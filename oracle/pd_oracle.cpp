@@ -1,0 +1,1364 @@
+// pd_oracle.cpp — CPU FP64 restatement of diff_fem's corotated-tet projective
+// dynamics step and of its gradient.
+//
+// TEST INFRASTRUCTURE ONLY. This file is the checker: tests, the smoke check and
+// the benchmark's CPU-baseline leg load it; the product library never does.
+//
+// It restates, in plain C++ with AoS state x[3i+c] like the reference:
+//   * Dfobj::Initialize tet loop (sim/Dfobj.h:455-529): orientation fix i1<->i2,
+//     vol = det(Dm)/6, invDm, volume-lumped masses scaled to the total mass.
+//   * CorotateFEMConstraint (sim/fem/Constraint/CorotateFEMConstraint.h:62-77,
+//     85-102, 185-205, 221-392): mMu = E/(1+nu), weights mMu*vol and mMu*vol*nu,
+//     polar factor with the det<0 column flip, 5-step volume projection.
+//   * LinearMuscleConstraint (sim/fem/Constraint/LinearMuscleConstraint.h:89-271)
+//     and the per-segment tet selection of Muscle::Initialize (sim/Muscle.h:109-215).
+//   * AttachmentConstraint (sim/fem/Constraint/AttachmentConstraint.h:70-99).
+//   * World::PreComputation / FactorizeLDLT (sim/fem/World.h:1089-1139, 1212-1232)
+//     as the scalar N x N system K_s (H2ML = K_s (x) I3), solved by a skyline
+//     Cholesky under reverse Cuthill-McKee ordering, with the reference's
+//     regularisation ladder (reg = 1e-5, 1e-4, ...).
+//   * Environment::Step gravity kick (sim/Environment.h:228-259),
+//     World::TimeStepping (sim/fem/World.h:673-720) and the PD local/global loop
+//     with its 5e-7 early exit (sim/fem/World.h:835-919, 1601-1670).
+//   * The contact phase (sim/fem/World.h:921-1023, 1317-1436;
+//     sim/utils/collisionBrutal.h:281-426, 484-503) against analytic static
+//     obstacles (plane, sphere) using the vertex-vs-static-face CCD rule of
+//     sim/utils/arcsim/collision.cpp:127-278 (crossing time or end-of-step
+//     distance below thickness 1e-4; normal opposed to the motion).
+//   * The gradient the reference obtains by CppAD taping (python/pydifem.cc:
+//     261-301): here a hand-derived reverse sweep over the recorded iterates,
+//     control flow (iteration counts) frozen as executed.
+//
+// Parity pins live in tests/ (closed forms, numpy SVD, brute-force 3N assembly,
+// dense solves, invariants, central finite differences).
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <unordered_map>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+constexpr double kEps = 5e-7;          // World.h:214
+constexpr double kThickness = 1e-4;    // arcsim collision.hpp:51
+
+thread_local std::string g_err;
+
+// ---------------------------------------------------------------- 3x3 helpers
+// Matrices are row-major a[r*3+c].
+inline double det3(const double* a) {
+  return a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) +
+         a[2] * (a[3] * a[7] - a[4] * a[6]);
+}
+inline void inv3(const double* a, double* o) {
+  double d = det3(a);
+  o[0] = (a[4] * a[8] - a[5] * a[7]) / d;
+  o[1] = (a[2] * a[7] - a[1] * a[8]) / d;
+  o[2] = (a[1] * a[5] - a[2] * a[4]) / d;
+  o[3] = (a[5] * a[6] - a[3] * a[8]) / d;
+  o[4] = (a[0] * a[8] - a[2] * a[6]) / d;
+  o[5] = (a[2] * a[3] - a[0] * a[5]) / d;
+  o[6] = (a[3] * a[7] - a[4] * a[6]) / d;
+  o[7] = (a[1] * a[6] - a[0] * a[7]) / d;
+  o[8] = (a[0] * a[4] - a[1] * a[3]) / d;
+}
+inline void matmul3(const double* a, const double* b, double* o) {
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c)
+      o[r * 3 + c] = a[r * 3 + 0] * b[0 * 3 + c] + a[r * 3 + 1] * b[1 * 3 + c] + a[r * 3 + 2] * b[2 * 3 + c];
+}
+inline void matmul3_bt(const double* a, const double* b, double* o) {  // a * b^T
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c)
+      o[r * 3 + c] = a[r * 3 + 0] * b[c * 3 + 0] + a[r * 3 + 1] * b[c * 3 + 1] + a[r * 3 + 2] * b[c * 3 + 2];
+}
+inline void matmul3_at(const double* a, const double* b, double* o) {  // a^T * b
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c)
+      o[r * 3 + c] = a[0 * 3 + r] * b[0 * 3 + c] + a[1 * 3 + r] * b[1 * 3 + c] + a[2 * 3 + r] * b[2 * 3 + c];
+}
+
+// One-sided (Hestenes) Jacobi SVD of a 3x3 matrix: F = U diag(S) V^T with
+// S >= 0 sorted descending, U and V orthogonal. An independent route to the
+// same factors Eigen::JacobiSVD returns (CorotateFEMConstraint.h:191); the
+// quantities used downstream (U V^T and U diag(.) V^T) do not depend on the
+// sign/ordering freedom of the SVD for nonsingular F.
+void svd3(const double* F, double* U, double* S, double* V) {
+  double A[9];
+  std::memcpy(A, F, sizeof(A));
+  for (int i = 0; i < 9; ++i) V[i] = (i % 4 == 0) ? 1.0 : 0.0;
+  const int pairs[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+  for (int sweep = 0; sweep < 60; ++sweep) {
+    double off = 0.0;
+    for (auto& pq : pairs) {
+      int p = pq[0], q = pq[1];
+      double alpha = 0, beta = 0, gamma = 0;
+      for (int r = 0; r < 3; ++r) {
+        alpha += A[r * 3 + p] * A[r * 3 + p];
+        beta += A[r * 3 + q] * A[r * 3 + q];
+        gamma += A[r * 3 + p] * A[r * 3 + q];
+      }
+      double scale = std::sqrt(alpha * beta);
+      if (scale == 0.0 || std::fabs(gamma) <= 1e-300) continue;
+      double rel = std::fabs(gamma) / scale;
+      if (rel <= 1e-17) continue;
+      off = std::max(off, rel);
+      double zeta = (beta - alpha) / (2.0 * gamma);
+      double t = (zeta >= 0 ? 1.0 : -1.0) / (std::fabs(zeta) + std::sqrt(1.0 + zeta * zeta));
+      double c = 1.0 / std::sqrt(1.0 + t * t);
+      double s = c * t;
+      for (int r = 0; r < 3; ++r) {
+        double ap = A[r * 3 + p], aq = A[r * 3 + q];
+        A[r * 3 + p] = c * ap - s * aq;
+        A[r * 3 + q] = s * ap + c * aq;
+        double vp = V[r * 3 + p], vq = V[r * 3 + q];
+        V[r * 3 + p] = c * vp - s * vq;
+        V[r * 3 + q] = s * vp + c * vq;
+      }
+    }
+    if (off <= 1e-16) break;
+  }
+  double sig[3];
+  for (int c = 0; c < 3; ++c)
+    sig[c] = std::sqrt(A[0 * 3 + c] * A[0 * 3 + c] + A[1 * 3 + c] * A[1 * 3 + c] + A[2 * 3 + c] * A[2 * 3 + c]);
+  int ord[3] = {0, 1, 2};
+  std::sort(ord, ord + 3, [&](int a, int b) { return sig[a] > sig[b]; });
+  double A2[9], V2[9];
+  for (int k = 0; k < 3; ++k) {
+    S[k] = sig[ord[k]];
+    for (int r = 0; r < 3; ++r) {
+      A2[r * 3 + k] = A[r * 3 + ord[k]];
+      V2[r * 3 + k] = V[r * 3 + ord[k]];
+    }
+  }
+  std::memcpy(V, V2, sizeof(V2));
+  double tiny = 1e-300 + 1e-15 * S[0];
+  for (int k = 0; k < 3; ++k) {
+    if (S[k] > tiny) {
+      for (int r = 0; r < 3; ++r) U[r * 3 + k] = A2[r * 3 + k] / S[k];
+    } else {
+      // complete an orthonormal basis (rank-deficient F)
+      double u[3] = {0, 0, 0};
+      if (k == 2) {
+        u[0] = U[1] * U[5] - U[2] * U[4];
+        u[1] = U[2] * U[3] - U[0] * U[5];
+        u[2] = U[0] * U[4] - U[1] * U[3];
+      } else {
+        // pick the axis least aligned with the previous columns
+        for (int ax = 0; ax < 3; ++ax) {
+          double e[3] = {0, 0, 0};
+          e[ax] = 1;
+          for (int j = 0; j < k; ++j) {
+            double dot = e[0] * U[0 * 3 + j] + e[1] * U[1 * 3 + j] + e[2] * U[2 * 3 + j];
+            for (int r = 0; r < 3; ++r) e[r] -= dot * U[r * 3 + j];
+          }
+          double n = std::sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+          if (n > 0.5) {
+            for (int r = 0; r < 3; ++r) u[r] = e[r] / n;
+            break;
+          }
+        }
+      }
+      for (int r = 0; r < 3; ++r) U[r * 3 + k] = u[r];
+    }
+  }
+}
+
+// 5-step linearised volume projection, CorotateFEMConstraint.h:362-374.
+void volume_projection(const double* S, double* D) {
+  D[0] = D[1] = D[2] = 0.0;
+  for (int it = 0; it < 5; ++it) {
+    double a0 = S[0] + D[0], a1 = S[1] + D[1], a2 = S[2] + D[2];
+    double CD = a0 * a1 * a2 - 1.0;
+    double g0 = a1 * a2, g1 = a0 * a2, g2 = a0 * a1;
+    double gg = g0 * g0 + g1 * g1 + g2 * g2;
+    double coef = (g0 * D[0] + g1 * D[1] + g2 * D[2] - CD) / gg;
+    D[0] = coef * g0;
+    D[1] = coef * g1;
+    D[2] = coef * g2;
+  }
+}
+
+// Local projection of one corotated tet at deformation gradient F:
+// md (polar factor, third column negated when det F < 0) and
+// md_volume = U diag(S+D) V^T (CorotateFEMConstraint.h:352-377).
+void corotate_project(const double* F, double* md, double* mdv) {
+  double U[9], S[3], V[9];
+  svd3(F, U, S, V);
+  matmul3_bt(U, V, md);
+  if (det3(F) < 0) {
+    md[2] = -md[2];
+    md[5] = -md[5];
+    md[8] = -md[8];
+  }
+  double D[3];
+  volume_projection(S, D);
+  double US[9];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) US[r * 3 + c] = U[r * 3 + c] * (S[c] + D[c]);
+  matmul3_bt(US, V, mdv);
+}
+
+// Jacobian-transpose of phi(S) = S + D(S) (the 5-step projection), by forward
+// mode over the 3 input directions: returns J[i][j] = d phi_i / d S_j.
+void volume_projection_jac(const double* S, double* Dout, double J[3][3]) {
+  double D[3] = {0, 0, 0};
+  double dD[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};  // dD[i][j] = dD_i/dS_j
+  for (int it = 0; it < 5; ++it) {
+    double a[3] = {S[0] + D[0], S[1] + D[1], S[2] + D[2]};
+    double da[3][3];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) da[i][j] = (i == j ? 1.0 : 0.0) + dD[i][j];
+    double CD = a[0] * a[1] * a[2] - 1.0;
+    double g[3] = {a[1] * a[2], a[0] * a[2], a[0] * a[1]};
+    double dCD[3], dg[3][3];
+    for (int j = 0; j < 3; ++j) {
+      dCD[j] = da[0][j] * a[1] * a[2] + a[0] * da[1][j] * a[2] + a[0] * a[1] * da[2][j];
+      dg[0][j] = da[1][j] * a[2] + a[1] * da[2][j];
+      dg[1][j] = da[0][j] * a[2] + a[0] * da[2][j];
+      dg[2][j] = da[0][j] * a[1] + a[0] * da[1][j];
+    }
+    double gg = g[0] * g[0] + g[1] * g[1] + g[2] * g[2];
+    double num = g[0] * D[0] + g[1] * D[1] + g[2] * D[2] - CD;
+    double coef = num / gg;
+    double dcoef[3];
+    for (int j = 0; j < 3; ++j) {
+      double dnum = dg[0][j] * D[0] + g[0] * dD[0][j] + dg[1][j] * D[1] + g[1] * dD[1][j] + dg[2][j] * D[2] +
+                    g[2] * dD[2][j] - dCD[j];
+      double dgg = 2.0 * (g[0] * dg[0][j] + g[1] * dg[1][j] + g[2] * dg[2][j]);
+      dcoef[j] = (dnum * gg - num * dgg) / (gg * gg);
+    }
+    double nD[3], ndD[3][3];
+    for (int i = 0; i < 3; ++i) {
+      nD[i] = coef * g[i];
+      for (int j = 0; j < 3; ++j) ndD[i][j] = dcoef[j] * g[i] + coef * dg[i][j];
+    }
+    std::memcpy(D, nD, sizeof(D));
+    std::memcpy(dD, ndD, sizeof(dD));
+  }
+  for (int i = 0; i < 3; ++i) {
+    Dout[i] = D[i];
+    for (int j = 0; j < 3; ++j) J[i][j] = (i == j ? 1.0 : 0.0) + dD[i][j];
+  }
+}
+
+// Vector-Jacobian product of the corotated projection:
+//   Fbar = wr * (d md/dF)^T [Zr] + wv * (d mdv/dF)^T [Zv]
+// using the SVD-based derivative of spectral matrix functions
+// G(F) = U diag(phi(S)) V^T: with P = U^T dF V,
+//   (U^T dG V)_ii = sum_j dphi_i/dS_j P_jj
+//   (U^T dG V)_ij = s_ij (phi_i-phi_j)/(S_i-S_j) + a_ij (phi_i+phi_j)/(S_i+S_j), i != j,
+// s = sym part, a = skew part of P. The off-diagonal map is self-adjoint.
+void corotate_vjp(const double* F, double wr, double wv, const double* Z, double* Fbar) {
+  double U[9], S[3], V[9];
+  svd3(F, U, S, V);
+  bool flip = det3(F) < 0;
+  double D[3], J[3][3];
+  volume_projection_jac(S, D, J);
+  double phi[3] = {S[0] + D[0], S[1] + D[1], S[2] + D[2]};
+  // Rotation part: md = (U V^T) * Df, Df = diag(1,1,-1) if flipped.
+  // d md = dR * Df  ->  adjoint input Zr = Z * Df.
+  double Zr[9];
+  std::memcpy(Zr, Z, sizeof(Zr));
+  if (flip) {
+    Zr[2] = -Zr[2];
+    Zr[5] = -Zr[5];
+    Zr[8] = -Zr[8];
+  }
+  double t1[9], Hr[9], Hv[9];
+  matmul3_at(U, Zr, t1);
+  matmul3(t1, V, Hr);  // U^T Zr V
+  matmul3_at(U, Z, t1);
+  matmul3(t1, V, Hv);  // U^T Z V
+  double Out[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  // diagonal: rotation part has phi = 1 (zero Jacobian); volume part J^T
+  for (int i = 0; i < 3; ++i) {
+    double acc = 0;
+    for (int j = 0; j < 3; ++j) acc += J[j][i] * Hv[j * 3 + j];
+    Out[i * 3 + i] = wv * acc;
+  }
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      if (i == j) continue;
+      double ssum = S[i] + S[j];
+      double beta_r = ssum > 0 ? 2.0 / ssum : 0.0;
+      double alpha_v, beta_v;
+      double sd = S[i] - S[j];
+      if (std::fabs(sd) > 1e-10 * std::max(1.0, std::fabs(S[i]))) {
+        alpha_v = (phi[i] - phi[j]) / sd;
+      } else {
+        alpha_v = J[i][i] - J[i][j];
+      }
+      beta_v = ssum > 0 ? (phi[i] + phi[j]) / ssum : 0.0;
+      // Forward map on the (ij, ji) pair is [[(a+b)/2, (a-b)/2], [(a-b)/2, (a+b)/2]]
+      // (symmetric), so the adjoint is the same map.
+      double hr_ij = Hr[i * 3 + j], hr_ji = Hr[j * 3 + i];
+      double hv_ij = Hv[i * 3 + j], hv_ji = Hv[j * 3 + i];
+      double rot = 0.5 * (0.0 + beta_r) * hr_ij + 0.5 * (0.0 - beta_r) * hr_ji;
+      double vol = 0.5 * (alpha_v + beta_v) * hv_ij + 0.5 * (alpha_v - beta_v) * hv_ji;
+      Out[i * 3 + j] = wr * rot + wv * vol;
+    }
+  // Fbar = U Out V^T
+  double t2[9];
+  matmul3(U, Out, t2);
+  matmul3_bt(t2, V, Fbar);
+}
+
+struct Tet {
+  int v[4];
+  double vol;
+  double G[4][3];  // F[:,j] = sum_k x_k * G[k][j]
+};
+
+struct Muscle {
+  int v[4];
+  double g[4];   // F f = sum_k x_k g[k]
+  double w;      // k_muscle * weight * vol
+  int group;     // activation group
+};
+
+struct Plane { double p[3], n[3]; };
+struct Sphere { double c[3], r; };
+
+struct Impact { int vtx; int obs; double n[3]; };
+
+// Skyline Cholesky of the scalar K_s in RCM order.
+struct Skyline {
+  int n = 0;
+  std::vector<int> perm, iperm;  // perm[new] = old
+  std::vector<int> first;        // first column of row i (new order)
+  std::vector<int64_t> start;    // offset of row i
+  std::vector<double> val;       // L entries, row i: columns first[i]..i
+  bool factor(const std::vector<std::map<int, double>>& K, double reg) {
+    for (int i = 0; i < n; ++i) {
+      int oi = perm[i];
+      double* Li = &val[start[i]];
+      int fi = first[i];
+      std::fill(Li, Li + (i - fi + 1), 0.0);
+      for (auto& kv : K[oi]) {
+        int j = iperm[kv.first];
+        if (j <= i) Li[j - fi] += kv.second;
+      }
+      Li[i - fi] += reg;
+      for (int j = fi; j < i; ++j) {
+        const double* Lj = &val[start[j]];
+        int fj = first[j];
+        int k0 = std::max(fi, fj);
+        double s = Li[j - fi];
+        for (int k = k0; k < j; ++k) s -= Li[k - fi] * Lj[k - fj];
+        Li[j - fi] = s / Lj[j - fj];
+      }
+      double s = Li[i - fi];
+      for (int k = fi; k < i; ++k) s -= Li[k - fi] * Li[k - fi];
+      if (!(s > 0.0)) return false;
+      Li[i - fi] = std::sqrt(s);
+    }
+    return true;
+  }
+  // Solve K x = b in place on a 3-component AoS vector (one column per component).
+  void solve3(double* x) const {
+    std::vector<double> y(3 * (size_t)n);
+    for (int i = 0; i < n; ++i)
+      for (int c = 0; c < 3; ++c) y[3 * i + c] = x[3 * perm[i] + c];
+    for (int i = 0; i < n; ++i) {
+      const double* Li = &val[start[i]];
+      int fi = first[i];
+      for (int c = 0; c < 3; ++c) {
+        double s = y[3 * i + c];
+        for (int k = fi; k < i; ++k) s -= Li[k - fi] * y[3 * k + c];
+        y[3 * i + c] = s / Li[i - fi];
+      }
+    }
+    for (int i = n - 1; i >= 0; --i) {
+      const double* Li = &val[start[i]];
+      int fi = first[i];
+      for (int c = 0; c < 3; ++c) {
+        y[3 * i + c] /= Li[i - fi];
+        double yi = y[3 * i + c];
+        for (int k = fi; k < i; ++k) y[3 * k + c] -= Li[k - fi] * yi;
+      }
+    }
+    for (int i = 0; i < n; ++i)
+      for (int c = 0; c < 3; ++c) x[3 * perm[i] + c] = y[3 * i + c];
+  }
+};
+
+struct StepTape {
+  std::vector<double> xn, vstar;           // start-of-step state (3N)
+  std::vector<std::vector<double>> ys;     // PD iterates y_0..y_K
+  std::vector<double> act;                 // activations used this step
+  bool contact_active = false;
+};
+
+struct Scene {
+  int N = 0;
+  std::vector<double> xrest;  // 3N
+  std::vector<double> mass;   // N
+  std::vector<Tet> tets;
+  std::vector<double> E;      // per tet
+  std::vector<double> nu_ctor;  // per tet (volume-term weight factor)
+  double nu_fwd = 0.4;          // enters mMu = E/(1+nu)
+  std::vector<double> wr, wv;   // per tet weights
+  std::vector<int> attach_idx;
+  std::vector<double> attach_w;
+  std::vector<Muscle> muscles;
+  int n_groups = 0;
+  // sim
+  double h = 0.01, damping = 0.9999;
+  double g[3] = {0, 0, 0};
+  int gravity_on = 1;
+  int max_iter = 30, max_iter1 = 30;
+  int contact = 0;
+  double alpha = 1.0, mu = 1.0;
+  int outer_cap = 64;
+  std::vector<Plane> planes;
+  std::vector<Sphere> spheres;
+  std::vector<int> contact_vtx;  // surface vertices
+  // factor
+  std::vector<std::map<int, double>> K;  // scalar K_s rows (original order)
+  std::vector<double> Kdiag;
+  Skyline sky;
+  double reg_used = 0.0;
+  bool factored = false;
+  // tape
+  std::vector<StepTape> tape;
+  std::vector<int> iters_pd, iters_contact, outer_count;
+};
+
+void build_tet(const double* xr, const int* t, Tet& out) {
+  int i0 = t[0], i1 = t[1], i2 = t[2], i3 = t[3];
+  auto Dm_of = [&](int a, int b, int c, int d, double* Dm) {
+    for (int r = 0; r < 3; ++r) {
+      Dm[r * 3 + 0] = xr[3 * b + r] - xr[3 * a + r];
+      Dm[r * 3 + 1] = xr[3 * c + r] - xr[3 * a + r];
+      Dm[r * 3 + 2] = xr[3 * d + r] - xr[3 * a + r];
+    }
+  };
+  double Dm[9];
+  Dm_of(i0, i1, i2, i3, Dm);
+  if (det3(Dm) < 0) {  // Dfobj.h:474-487
+    std::swap(i1, i2);
+    Dm_of(i0, i1, i2, i3, Dm);
+  }
+  out.v[0] = i0; out.v[1] = i1; out.v[2] = i2; out.v[3] = i3;
+  out.vol = det3(Dm) / 6.0;
+  double iD[9];
+  inv3(Dm, iD);
+  for (int j = 0; j < 3; ++j) {
+    out.G[0][j] = -(iD[0 * 3 + j] + iD[1 * 3 + j] + iD[2 * 3 + j]);
+    for (int m = 0; m < 3; ++m) out.G[m + 1][j] = iD[m * 3 + j];
+  }
+}
+
+inline void tet_F(const Tet& t, const double* x, double* F) {
+  for (int r = 0; r < 3; ++r)
+    for (int j = 0; j < 3; ++j) {
+      double s = 0;
+      for (int k = 0; k < 4; ++k) s += x[3 * t.v[k] + r] * t.G[k][j];
+      F[r * 3 + j] = s;
+    }
+}
+
+void update_weights(Scene& s) {
+  size_t T = s.tets.size();
+  s.wr.resize(T);
+  s.wv.resize(T);
+  for (size_t t = 0; t < T; ++t) {
+    double mMu = s.E[t] / (1.0 + s.nu_fwd);  // CorotateFEMConstraint.h:69, pydifem.cc:217
+    s.wr[t] = mMu * s.tets[t].vol;           // :247
+    s.wv[t] = mMu * s.tets[t].vol * s.nu_ctor[t];  // :267
+  }
+}
+
+void assemble_K(Scene& s) {
+  int N = s.N;
+  s.K.assign(N, {});
+  double ih2 = 1.0 / (s.h * s.h);
+  for (int i = 0; i < N; ++i) s.K[i][i] += s.mass[i] * ih2;
+  for (size_t t = 0; t < s.tets.size(); ++t) {
+    const Tet& te = s.tets[t];
+    double w = s.wr[t] + s.wv[t];
+    for (int a = 0; a < 4; ++a)
+      for (int b = 0; b < 4; ++b) {
+        double gg = te.G[a][0] * te.G[b][0] + te.G[a][1] * te.G[b][1] + te.G[a][2] * te.G[b][2];
+        s.K[te.v[a]][te.v[b]] += w * gg;
+      }
+  }
+  for (const Muscle& m : s.muscles)
+    for (int a = 0; a < 4; ++a)
+      for (int b = 0; b < 4; ++b) s.K[m.v[a]][m.v[b]] += m.w * m.g[a] * m.g[b];
+  for (size_t k = 0; k < s.attach_idx.size(); ++k) s.K[s.attach_idx[k]][s.attach_idx[k]] += s.attach_w[k];
+  s.Kdiag.assign(N, 0.0);
+  for (int i = 0; i < N; ++i) s.Kdiag[i] = s.K[i][i];
+}
+
+void rcm_order(const Scene& s, std::vector<int>& perm) {
+  int N = s.N;
+  std::vector<int> deg(N);
+  for (int i = 0; i < N; ++i) deg[i] = (int)s.K[i].size();
+  std::vector<int> order;
+  order.reserve(N);
+  std::vector<char> seen(N, 0);
+  auto bfs_last = [&](int root, std::vector<int>* levels_out) {
+    std::vector<int> lev(N, -1), q{root};
+    lev[root] = 0;
+    for (size_t h = 0; h < q.size(); ++h) {
+      int u = q[h];
+      for (auto& kv : s.K[u])
+        if (lev[kv.first] < 0) {
+          lev[kv.first] = lev[u] + 1;
+          q.push_back(kv.first);
+        }
+    }
+    if (levels_out) *levels_out = lev;
+    return q.back();
+  };
+  for (int comp = 0; comp < N; ++comp) {
+    if (seen[comp]) continue;
+    int root = comp;
+    for (int k = 0; k < 3; ++k) root = bfs_last(root, nullptr);
+    std::vector<int> q{root};
+    seen[root] = 1;
+    for (size_t hd = 0; hd < q.size(); ++hd) {
+      int u = q[hd];
+      std::vector<int> nb;
+      for (auto& kv : s.K[u])
+        if (!seen[kv.first]) {
+          seen[kv.first] = 1;
+          nb.push_back(kv.first);
+        }
+      std::sort(nb.begin(), nb.end(), [&](int a, int b) { return deg[a] != deg[b] ? deg[a] < deg[b] : a < b; });
+      for (int v : nb) q.push_back(v);
+    }
+    for (int v : q) order.push_back(v);
+  }
+  perm.assign(order.rbegin(), order.rend());
+}
+
+bool factorize(Scene& s) {
+  update_weights(s);
+  assemble_K(s);
+  Skyline& L = s.sky;
+  L.n = s.N;
+  rcm_order(s, L.perm);
+  L.iperm.assign(s.N, 0);
+  for (int i = 0; i < s.N; ++i) L.iperm[L.perm[i]] = i;
+  L.first.assign(s.N, 0);
+  L.start.assign(s.N + 1, 0);
+  for (int i = 0; i < s.N; ++i) {
+    int f = i;
+    for (auto& kv : s.K[L.perm[i]]) f = std::min(f, L.iperm[kv.first]);
+    L.first[i] = f;
+    L.start[i + 1] = L.start[i] + (i - f + 1);
+  }
+  L.val.assign(L.start[s.N], 0.0);
+  // FactorizeLDLT regularisation ladder, World.h:1212-1232
+  double reg = 1e-6;
+  bool ok = L.factor(s.K, 0.0);
+  s.reg_used = 0.0;
+  int tries = 0;
+  while (!ok && tries < 40) {
+    reg *= 10;
+    ok = L.factor(s.K, reg);
+    s.reg_used = reg;
+    ++tries;
+  }
+  s.factored = ok;
+  if (!ok) g_err = "factorization failed";
+  return ok;
+}
+
+// d-term: accumulate J*d into out (3N), evaluated at positions x.
+void add_Jd(const Scene& s, const double* x, const double* act, double* out) {
+  for (size_t t = 0; t < s.tets.size(); ++t) {
+    const Tet& te = s.tets[t];
+    double F[9], md[9], mdv[9], P[9];
+    tet_F(te, x, F);
+    corotate_project(F, md, mdv);
+    for (int i = 0; i < 9; ++i) P[i] = s.wr[t] * md[i] + s.wv[t] * mdv[i];
+    for (int k = 0; k < 4; ++k)
+      for (int r = 0; r < 3; ++r)
+        out[3 * te.v[k] + r] += P[r * 3 + 0] * te.G[k][0] + P[r * 3 + 1] * te.G[k][1] + P[r * 3 + 2] * te.G[k][2];
+  }
+  for (const Muscle& m : s.muscles) {
+    double a = act ? act[m.group] : 0.0;
+    double Ff[3] = {0, 0, 0};
+    for (int k = 0; k < 4; ++k)
+      for (int r = 0; r < 3; ++r) Ff[r] += x[3 * m.v[k] + r] * m.g[k];
+    double sc = m.w * (1.0 - a);  // p0 = (1-a) F f, LinearMuscleConstraint.h:269
+    for (int k = 0; k < 4; ++k)
+      for (int r = 0; r < 3; ++r) out[3 * m.v[k] + r] += sc * Ff[r] * m.g[k];
+  }
+  for (size_t k = 0; k < s.attach_idx.size(); ++k) {
+    int i = s.attach_idx[k];
+    for (int r = 0; r < 3; ++r) out[3 * i + r] += s.attach_w[k] * s.xrest[3 * i + r];
+  }
+}
+
+// out += scale * L_full x   (L_full includes attachments; excludes masses)
+void add_Lx(const Scene& s, const double* x, double scale, double* out) {
+  for (size_t t = 0; t < s.tets.size(); ++t) {
+    const Tet& te = s.tets[t];
+    double F[9];
+    tet_F(te, x, F);
+    double w = s.wr[t] + s.wv[t];
+    for (int k = 0; k < 4; ++k)
+      for (int r = 0; r < 3; ++r)
+        out[3 * te.v[k] + r] +=
+            scale * w * (F[r * 3 + 0] * te.G[k][0] + F[r * 3 + 1] * te.G[k][1] + F[r * 3 + 2] * te.G[k][2]);
+  }
+  for (const Muscle& m : s.muscles) {
+    double Ff[3] = {0, 0, 0};
+    for (int k = 0; k < 4; ++k)
+      for (int r = 0; r < 3; ++r) Ff[r] += x[3 * m.v[k] + r] * m.g[k];
+    for (int k = 0; k < 4; ++k)
+      for (int r = 0; r < 3; ++r) out[3 * m.v[k] + r] += scale * m.w * Ff[r] * m.g[k];
+  }
+  for (size_t k = 0; k < s.attach_idx.size(); ++k) {
+    int i = s.attach_idx[k];
+    for (int r = 0; r < 3; ++r) out[3 * i + r] += scale * s.attach_w[k] * x[3 * i + r];
+  }
+}
+
+// out += scale * offdiag(K_s) (x) I3 applied to y
+void add_Cy(const Scene& s, const double* y, double scale, double* out) {
+  for (int i = 0; i < s.N; ++i)
+    for (auto& kv : s.K[i]) {
+      if (kv.first == i) continue;
+      for (int r = 0; r < 3; ++r) out[3 * i + r] += scale * kv.second * y[3 * kv.first + r];
+    }
+}
+
+// Static-obstacle CCD, restating arcsim collision_test for a vertex against a
+// static face (collision.cpp:211-278): candidate times are the crossing roots
+// in [0,1] then t = 1; an impact needs |signed distance| < thickness there.
+// The contact normal opposes the vertex motion (collision.cpp:269-273).
+bool detect_plane(const Plane& pl, const double* x0, const double* x1, double* n) {
+  double ds = 0, de = 0, vn = 0;
+  for (int r = 0; r < 3; ++r) {
+    ds += (x0[r] - pl.p[r]) * pl.n[r];
+    de += (x1[r] - pl.p[r]) * pl.n[r];
+    vn += (x1[r] - x0[r]) * pl.n[r];
+  }
+  bool hit = false;
+  if (ds != de) {
+    double t = ds / (ds - de);
+    if (t >= 0 && t <= 1) hit = true;  // distance at the crossing is 0
+  }
+  if (!hit && std::fabs(de) < kThickness) hit = true;
+  if (!hit) return false;
+  double sg = vn > 0 ? -1.0 : 1.0;
+  for (int r = 0; r < 3; ++r) n[r] = sg * pl.n[r];
+  return true;
+}
+
+bool detect_sphere(const Sphere& sp, const double* x0, const double* x1, double* n) {
+  double d[3], e[3];
+  for (int r = 0; r < 3; ++r) {
+    d[r] = x1[r] - x0[r];
+    e[r] = x0[r] - sp.c[r];
+  }
+  double a = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+  double b = 2 * (d[0] * e[0] + d[1] * e[1] + d[2] * e[2]);
+  double c = e[0] * e[0] + e[1] * e[1] + e[2] * e[2] - sp.r * sp.r;
+  double ts[3];
+  int nt = 0;
+  if (a > 0) {
+    double disc = b * b - 4 * a * c;
+    if (disc >= 0) {
+      double sq = std::sqrt(disc);
+      double q = -0.5 * (b + (b >= 0 ? sq : -sq));
+      double r1 = q / a, r2 = q != 0 ? c / q : r1;
+      if (r1 > r2) std::swap(r1, r2);
+      ts[nt++] = r1;
+      ts[nt++] = r2;
+    }
+  }
+  ts[nt++] = 1.0;
+  for (int k = 0; k < nt; ++k) {
+    double t = ts[k];
+    if (t < 0 || t > 1) continue;
+    double p[3];
+    for (int r = 0; r < 3; ++r) p[r] = x0[r] + t * d[r] - sp.c[r];
+    double len = std::sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+    double dist = len - sp.r;
+    if (std::fabs(dist) < kThickness && len > 0) {
+      double nf[3] = {p[0] / len, p[1] / len, p[2] / len};
+      double vn = nf[0] * d[0] + nf[1] * d[1] + nf[2] * d[2];
+      double sg = vn > 0 ? -1.0 : 1.0;
+      for (int r = 0; r < 3; ++r) n[r] = sg * nf[r];
+      return true;
+    }
+  }
+  return false;
+}
+
+// computeBasis, collisionBrutal.h:408-426: columns [n, t, b].
+void contact_basis(const double* n, double* B) {
+  double t[3] = {0, n[2], -n[1]};  // n x (1,0,0)
+  double tn = std::sqrt(t[1] * t[1] + t[2] * t[2]);
+  if (tn < 1e-15) {  // n x (0,1,0)
+    t[0] = -n[2]; t[1] = 0; t[2] = n[0];
+    tn = std::sqrt(t[0] * t[0] + t[2] * t[2]);
+  }
+  for (int r = 0; r < 3; ++r) t[r] /= tn;
+  double b[3] = {n[1] * t[2] - n[2] * t[1], n[2] * t[0] - n[0] * t[2], n[0] * t[1] - n[1] * t[0]};
+  double bn = std::sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+  double nn = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+  for (int r = 0; r < 3; ++r) {
+    B[r * 3 + 0] = n[r] / nn;
+    B[r * 3 + 1] = t[r];
+    B[r * 3 + 2] = b[r] / bn;
+  }
+}
+
+// Cone projection of World::CollisionHandling (World.h:1363-1436): the rhs of a
+// contact vertex is replaced by the reaction B r.
+void project_contact(const double* B, double mu, const double* f_in, double* out) {
+  double fl[3];
+  for (int c = 0; c < 3; ++c) fl[c] = B[0 * 3 + c] * f_in[0] + B[1 * 3 + c] * f_in[1] + B[2 * 3 + c] * f_in[2];
+  if (fl[0] > 0.0) {  // take-off: unchanged
+    for (int r = 0; r < 3; ++r) out[r] = f_in[r];
+    return;
+  }
+  double rr[3] = {-fl[0], -fl[1], -fl[2]};
+  double rt = std::sqrt(rr[1] * rr[1] + rr[2] * rr[2]);
+  if (rt > mu * rr[0]) {
+    rr[1] *= mu * rr[0] / rt;
+    rr[2] *= mu * rr[0] / rt;
+  }
+  for (int r = 0; r < 3; ++r) out[r] = B[r * 3 + 0] * rr[0] + B[r * 3 + 1] * rr[1] + B[r * 3 + 2] * rr[2];
+}
+
+// One time step. x,v in/out (3N). act: activations per group (or null).
+void step(Scene& s, double* x, double* v, const double* act, StepTape* tape, int* it_pd, int* it_c, int* outer) {
+  int N = s.N, n3 = 3 * N;
+  // Environment::Step gravity kick (Environment.h:236-245) + TimeStepping (World.h:692-699)
+  std::vector<double> vstar(v, v + n3);
+  if (s.gravity_on)
+    for (int i = 0; i < N; ++i)
+      for (int r = 0; r < 3; ++r) vstar[3 * i + r] += s.g[r] * s.h;
+  std::vector<double> xn(x, x + n3), qn(n3);
+  for (int k = 0; k < n3; ++k) qn[k] = xn[k] + s.h * vstar[k];
+  // constant part c = M v*/h - L_full xn ; per iteration rhs = c + J d(y)
+  std::vector<double> cvec(n3, 0.0);
+  for (int i = 0; i < N; ++i)
+    for (int r = 0; r < 3; ++r) cvec[3 * i + r] = s.mass[i] * vstar[3 * i + r] / s.h;
+  add_Lx(s, xn.data(), -1.0, cvec.data());
+  if (tape) {
+    tape->xn = xn;
+    tape->vstar = vstar;
+    tape->ys.clear();
+    tape->ys.push_back(qn);
+    if (act) tape->act.assign(act, act + s.n_groups);
+    else tape->act.assign(s.n_groups, 0.0);
+  }
+  std::vector<double> y = qn, yold(n3, 0.0), rhs(n3);
+  int iters = 0;
+  for (int i = 0; i < s.max_iter; ++i) {  // World.h:885-916
+    double e2 = 0;
+    for (int k = 0; k < n3; ++k) e2 += (y[k] - yold[k]) * (y[k] - yold[k]);
+    if (std::sqrt(e2) / n3 < kEps) break;
+    yold = y;
+    rhs = cvec;
+    add_Jd(s, y.data(), act, rhs.data());
+    s.sky.solve3(rhs.data());
+    for (int k = 0; k < n3; ++k) y[k] = xn[k] + rhs[k];
+    ++iters;
+    if (tape) tape->ys.push_back(y);
+  }
+  if (it_pd) *it_pd = iters;
+  std::vector<double> ori_x = y;
+  int cit = 0, outer_n = 0;
+  if (s.contact && !s.contact_vtx.empty() && (!s.planes.empty() || !s.spheres.empty())) {
+    // Contact phase, World.h:921-1023
+    std::vector<Impact> impacts;
+    std::vector<double> xit = y, oold(n3, 0.0), prev(n3, 0.0), rc(n3);
+    while (true) {
+      std::vector<Impact> fresh;
+      for (int ci = 0; ci < (int)s.contact_vtx.size(); ++ci) {
+        int vtx = s.contact_vtx[ci];
+        double nrm[3];
+        int ob = 0;
+        for (const Plane& pl : s.planes) {
+          if (detect_plane(pl, &xn[3 * vtx], &ori_x[3 * vtx], nrm)) fresh.push_back({vtx, ob, {nrm[0], nrm[1], nrm[2]}});
+          ++ob;
+        }
+        for (const Sphere& sp : s.spheres) {
+          if (detect_sphere(sp, &xn[3 * vtx], &ori_x[3 * vtx], nrm)) fresh.push_back({vtx, ob, {nrm[0], nrm[1], nrm[2]}});
+          ++ob;
+        }
+      }
+      for (auto& f : fresh) {  // add-or-replace by (vertex, face) key (collisionBrutal.h:312-327)
+        bool found = false;
+        for (auto& im : impacts)
+          if (im.vtx == f.vtx && im.obs == f.obs) {
+            im = f;
+            found = true;
+            break;
+          }
+        if (!found) impacts.push_back(f);
+      }
+      if (fresh.empty()) break;
+      if (++outer_n > s.outer_cap) break;
+      if (tape) tape->contact_active = true;
+      std::vector<double> Bs(impacts.size() * 9);
+      for (size_t k = 0; k < impacts.size(); ++k) contact_basis(impacts[k].n, &Bs[9 * k]);
+      for (int i = 0; i < s.max_iter1; ++i) {
+        ori_x = xit;
+        double err = 0;
+        for (int k = 0; k < n3; ++k) err = std::max(err, std::fabs(ori_x[k] - oold[k]));
+        if (err < kEps) break;
+        // rhs_speed = M v* + h (J d - L xn) - h C (ori_x - xn)
+        rc = cvec;
+        add_Jd(s, ori_x.data(), act, rc.data());
+        std::vector<double> dy(n3);
+        for (int k = 0; k < n3; ++k) dy[k] = ori_x[k] - xn[k];
+        add_Cy(s, dy.data(), -1.0, rc.data());
+        for (int k = 0; k < n3; ++k) rc[k] *= s.h;
+        if (i == 0) prev = rc;
+        std::vector<double> orig = rc;
+        for (size_t k = 0; k < impacts.size(); ++k) {
+          int vtx = impacts[k].vtx;
+          project_contact(&Bs[9 * k], s.mu, &orig[3 * vtx], &rc[3 * vtx]);
+        }
+        for (int k = 0; k < n3; ++k) prev[k] = rc[k] * s.alpha + prev[k] * (1.0 - s.alpha);
+        for (int vtx = 0; vtx < N; ++vtx)
+          for (int r = 0; r < 3; ++r) xit[3 * vtx + r] = xn[3 * vtx + r] + (prev[3 * vtx + r] / s.h) / s.Kdiag[vtx];
+        oold = ori_x;
+        ++cit;
+      }
+    }
+  }
+  if (it_c) *it_c = cit;
+  if (outer) *outer = outer_n;
+  // UpdatePositionsAndVelocities + damping (World.h:757-766, 708)
+  for (int k = 0; k < n3; ++k) {
+    v[k] = (ori_x[k] - xn[k]) * (1.0 / s.h);
+    x[k] = ori_x[k];
+    v[k] *= s.damping;
+  }
+}
+
+// Reverse sweep of one step (no contact). Inputs: gx, gv = dL/dx_{n+1}, dL/dv_{n+1}.
+// Outputs (accumulated): gx_n, gv_n, gE (per tet), gnu, gact (per group).
+void step_backward(Scene& s, const StepTape& tp, const double* gx, const double* gv, double* gxn, double* gvn,
+                   double* gE, double* gnu, double* gact) {
+  int N = s.N, n3 = 3 * N;
+  double gam = s.damping, h = s.h;
+  int K = (int)tp.ys.size() - 1;
+  std::vector<double> ybar(n3), xbar(n3, 0.0), zsum(n3, 0.0), z(n3);
+  for (int k = 0; k < n3; ++k) {
+    ybar[k] = gx[k] + gam / h * gv[k];
+    xbar[k] = -gam / h * gv[k];
+  }
+  const double* act = tp.act.empty() ? nullptr : tp.act.data();
+  for (int it = K - 1; it >= 0; --it) {
+    const double* yk = tp.ys[it].data();
+    const double* yk1 = tp.ys[it + 1].data();
+    z = ybar;
+    s.sky.solve3(z.data());
+    for (int k = 0; k < n3; ++k) {
+      xbar[k] += ybar[k];
+      zsum[k] += z[k];
+    }
+    std::fill(ybar.begin(), ybar.end(), 0.0);
+    for (size_t t = 0; t < s.tets.size(); ++t) {
+      const Tet& te = s.tets[t];
+      double F[9], Fz[9], F1[9], md[9], mdv[9], Fbar[9];
+      tet_F(te, yk, F);
+      tet_F(te, z.data(), Fz);
+      tet_F(te, yk1, F1);
+      corotate_project(F, md, mdv);
+      // parameter gradient: (A z).(d - A y_{k+1}) per weight
+      double gr = 0, gvv = 0;
+      for (int i = 0; i < 9; ++i) {
+        gr += Fz[i] * (md[i] - F1[i]);
+        gvv += Fz[i] * (mdv[i] - F1[i]);
+      }
+      double opn = 1.0 + s.nu_fwd;
+      double vol = te.vol;
+      // wr = E vol/(1+nu), wv = E vol nu_c/(1+nu)
+      if (gE) gE[t] += gr * vol / opn + gvv * vol * s.nu_ctor[t] / opn;
+      if (gnu) *gnu += -(gr * s.wr[t] + gvv * s.wv[t]) / opn;
+      corotate_vjp(F, s.wr[t], s.wv[t], Fz, Fbar);
+      for (int k = 0; k < 4; ++k)
+        for (int r = 0; r < 3; ++r)
+          ybar[3 * te.v[k] + r] += Fbar[r * 3 + 0] * te.G[k][0] + Fbar[r * 3 + 1] * te.G[k][1] + Fbar[r * 3 + 2] * te.G[k][2];
+    }
+    for (const Muscle& m : s.muscles) {
+      double a = act ? act[m.group] : 0.0;
+      double Ff[3] = {0, 0, 0}, Fz[3] = {0, 0, 0};
+      for (int k = 0; k < 4; ++k)
+        for (int r = 0; r < 3; ++r) {
+          Ff[r] += yk[3 * m.v[k] + r] * m.g[k];
+          Fz[r] += z[3 * m.v[k] + r] * m.g[k];
+        }
+      if (gact) gact[m.group] += -m.w * (Fz[0] * Ff[0] + Fz[1] * Ff[1] + Fz[2] * Ff[2]);
+      double sc = m.w * (1.0 - a);
+      for (int k = 0; k < 4; ++k)
+        for (int r = 0; r < 3; ++r) ybar[3 * m.v[k] + r] += sc * Fz[r] * m.g[k];
+    }
+  }
+  // y_0 = xn + h v*  ;  c = M v*/h - L_full xn
+  std::vector<double> vsb(n3);
+  for (int k = 0; k < n3; ++k) {
+    xbar[k] += ybar[k];
+    vsb[k] = h * ybar[k];
+  }
+  for (int i = 0; i < N; ++i)
+    for (int r = 0; r < 3; ++r) vsb[3 * i + r] += s.mass[i] / h * zsum[3 * i + r];
+  add_Lx(s, zsum.data(), -1.0, xbar.data());
+  // c depends on the weights through -L xn; that term is already folded into
+  // the per-iteration (A z).(d - A y_{k+1}) expression above.
+  for (int k = 0; k < n3; ++k) {
+    gxn[k] += xbar[k];
+    gvn[k] += vsb[k];
+  }
+}
+
+std::vector<int> surface_vertices(int N, const std::vector<Tet>& tets) {
+  std::map<std::array<int, 3>, int> cnt;
+  for (const Tet& t : tets) {
+    int sv[4] = {t.v[0], t.v[1], t.v[2], t.v[3]};
+    std::sort(sv, sv + 4);
+    const int fc[4][3] = {{0, 1, 2}, {0, 1, 3}, {0, 2, 3}, {1, 2, 3}};
+    for (auto& f : fc) cnt[{sv[f[0]], sv[f[1]], sv[f[2]]}]++;
+  }
+  std::vector<char> on(N, 0);
+  for (auto& kv : cnt)
+    if (kv.second == 1)
+      for (int k = 0; k < 3; ++k) on[kv.first[k]] = 1;
+  std::vector<int> out;
+  for (int i = 0; i < N; ++i)
+    if (on[i]) out.push_back(i);
+  return out;
+}
+
+// ---- Muscle::Initialize per-segment tet selection (Muscle.h:109-215)
+inline double signed_vol(const double* p0, const double* p1, const double* p2, const double* p3) {
+  double a[3], b[3], c[3];
+  for (int r = 0; r < 3; ++r) {
+    a[r] = p1[r] - p0[r];
+    b[r] = p2[r] - p0[r];
+    c[r] = p3[r] - p0[r];
+  }
+  double cx = a[1] * b[2] - a[2] * b[1], cy = a[2] * b[0] - a[0] * b[2], cz = a[0] * b[1] - a[1] * b[0];
+  return (cx * c[0] + cy * c[1] + cz * c[2]) / 6.0;
+}
+bool line_tri(const double* p0, const double* p1, const double* p2, const double* s, const double* e) {
+  if (signed_vol(s, p0, p1, p2) * signed_vol(e, p0, p1, p2) < 0.0) {
+    double a = signed_vol(s, e, p0, p1), b = signed_vol(s, e, p1, p2), c = signed_vol(s, e, p2, p0);
+    if (a > 0 && b > 0 && c > 0) return true;
+    if (a < 0 && b < 0 && c < 0) return true;
+  }
+  return false;
+}
+double pos_det(const double* q, const double* p1, const double* p2, const double* p3) {
+  // GetDm (Muscle.h:273-290): determinant made non-negative by swapping columns
+  double Dm[9];
+  for (int r = 0; r < 3; ++r) {
+    Dm[r * 3 + 0] = p1[r] - q[r];
+    Dm[r * 3 + 1] = p2[r] - q[r];
+    Dm[r * 3 + 2] = p3[r] - q[r];
+  }
+  double d = det3(Dm);
+  return d < 0 ? -d : d;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* orc_last_error() { return g_err.c_str(); }
+
+void* orc_scene_new(int N, const double* xrest, int T, const int* tets, int mass_same, double total_mass) {
+  Scene* s = new Scene();
+  s->N = N;
+  s->xrest.assign(xrest, xrest + 3 * (size_t)N);
+  s->tets.resize(T);
+  std::vector<double> m(N, 0.0);
+  double totvol = 0;
+  for (int t = 0; t < T; ++t) {
+    build_tet(xrest, tets + 4 * t, s->tets[t]);
+    double vol = s->tets[t].vol;
+    totvol += vol;
+    for (int k = 0; k < 4; ++k) m[tets[4 * t + k]] += 0.25 * vol;  // Dfobj.h:506-512
+  }
+  for (int i = 0; i < N; ++i) m[i] = mass_same ? total_mass / totvol * m[i] : total_mass / N;  // :515-525
+  s->mass = m;
+  s->E.assign(T, 2e5);
+  s->nu_ctor.assign(T, 0.4);
+  s->contact_vtx = surface_vertices(N, s->tets);
+  return s;
+}
+
+void orc_scene_free(void* p) { delete (Scene*)p; }
+
+int orc_num_tets(void* p) { return (int)((Scene*)p)->tets.size(); }
+
+void orc_get_masses(void* p, double* m) {
+  Scene* s = (Scene*)p;
+  std::copy(s->mass.begin(), s->mass.end(), m);
+}
+
+void orc_get_tets(void* p, int* v, double* vol, double* G) {
+  Scene* s = (Scene*)p;
+  for (size_t t = 0; t < s->tets.size(); ++t) {
+    for (int k = 0; k < 4; ++k) v[4 * t + k] = s->tets[t].v[k];
+    vol[t] = s->tets[t].vol;
+    for (int k = 0; k < 4; ++k)
+      for (int j = 0; j < 3; ++j) G[12 * t + 3 * k + j] = s->tets[t].G[k][j];
+  }
+}
+
+int orc_num_contact_vertices(void* p) { return (int)((Scene*)p)->contact_vtx.size(); }
+void orc_get_contact_vertices(void* p, int* out) {
+  Scene* s = (Scene*)p;
+  std::copy(s->contact_vtx.begin(), s->contact_vtx.end(), out);
+}
+
+void orc_set_material(void* p, const double* E_tet, double nu_fwd, const double* nu_ctor_tet) {
+  Scene* s = (Scene*)p;
+  size_t T = s->tets.size();
+  s->E.assign(E_tet, E_tet + T);
+  s->nu_fwd = nu_fwd;
+  s->nu_ctor.assign(nu_ctor_tet, nu_ctor_tet + T);
+  s->factored = false;
+}
+
+void orc_set_attachments(void* p, int n, const int* idx, double w) {
+  Scene* s = (Scene*)p;
+  s->attach_idx.assign(idx, idx + n);
+  s->attach_w.assign(n, w);
+  s->factored = false;
+}
+
+// Explicit muscle constraints: indices (4 per constraint, orientation is fixed
+// here as Muscle.h:143-154 does), fibre direction, weight, group.
+void orc_set_muscles(void* p, int n, const int* idx4, const double* fiber, const double* weight, double k_muscle,
+                     const int* group, int n_groups) {
+  Scene* s = (Scene*)p;
+  s->muscles.clear();
+  s->n_groups = n_groups;
+  for (int c = 0; c < n; ++c) {
+    Tet te;
+    build_tet(s->xrest.data(), idx4 + 4 * c, te);
+    Muscle m;
+    for (int k = 0; k < 4; ++k) m.v[k] = te.v[k];
+    // v = invDm f ; A = [-(a+b+c), a, b, c] (LinearMuscleConstraint.h:156-166)
+    const double* f = fiber + 3 * c;
+    double vv[3];
+    // invDm rows are G[1..3]: invDm(m, j) = G[m+1][j]
+    for (int mm = 0; mm < 3; ++mm) vv[mm] = te.G[mm + 1][0] * f[0] + te.G[mm + 1][1] * f[1] + te.G[mm + 1][2] * f[2];
+    m.g[0] = -(vv[0] + vv[1] + vv[2]);
+    m.g[1] = vv[0];
+    m.g[2] = vv[1];
+    m.g[3] = vv[2];
+    m.w = te.vol * k_muscle * weight[c];  // mStiffness *= mWeight (:109); J = vol*k*A^T (:168)
+    m.group = group[c];
+    s->muscles.push_back(m);
+  }
+  s->factored = false;
+}
+
+// Per-segment tet selection of Muscle::Initialize for one segment [start,end].
+// Writes up to cap constraints (tet index, weight); returns the count.
+int orc_muscle_segment_select(void* p, const int* tets, int T, const double* start, const double* end, int cap,
+                              int* out_tet, double* out_w) {
+  Scene* s = (Scene*)p;
+  const double* X = s->xrest.data();
+  int n = 0;
+  double dir[3] = {end[0] - start[0], end[1] - start[1], end[2] - start[2]};
+  double len = std::sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+  for (int r = 0; r < 3; ++r) dir[r] /= len;
+  auto push = [&](int t, double w) {
+    if (n < cap) {
+      out_tet[n] = t;
+      out_w[n] = w;
+    }
+    ++n;
+  };
+  for (int j = 0; j < T; ++j) {
+    int i0 = tets[4 * j], i1 = tets[4 * j + 1], i2 = tets[4 * j + 2], i3 = tets[4 * j + 3];
+    double Dm[9];
+    for (int r = 0; r < 3; ++r) {
+      Dm[r * 3 + 0] = X[3 * i1 + r] - X[3 * i0 + r];
+      Dm[r * 3 + 1] = X[3 * i2 + r] - X[3 * i0 + r];
+      Dm[r * 3 + 2] = X[3 * i3 + r] - X[3 * i0 + r];
+    }
+    if (det3(Dm) < 0) std::swap(i1, i2);
+    const double *p0 = &X[3 * i0], *p1 = &X[3 * i1], *p2 = &X[3 * i2], *p3 = &X[3 * i3];
+    double center[3], v[3];
+    for (int r = 0; r < 3; ++r) {
+      center[r] = (p0[r] + p1[r] + p2[r] + p3[r]) / 4.0;
+      v[r] = center[r] - start[r];
+    }
+    double t = v[0] * dir[0] + v[1] * dir[1] + v[2] * dir[2];
+    double pp[3];
+    for (int r = 0; r < 3; ++r) pp[r] = start[r] + t * dir[r];
+    auto dist3 = [](const double* a, const double* b) {
+      return std::sqrt((a[0] - b[0]) * (a[0] - b[0]) + (a[1] - b[1]) * (a[1] - b[1]) + (a[2] - b[2]) * (a[2] - b[2]));
+    };
+    double dist;
+    if (std::max(dist3(pp, start), dist3(pp, end)) / dist3(end, start) < 1.0)
+      dist = dist3(pp, center);
+    else
+      dist = 1e5;
+    dist = std::min(dist, std::min(dist3(center, start), dist3(center, end)));
+    double weight = std::exp(-std::fabs(dist) / 0.005);
+    bool inside = line_tri(p0, p1, p2, start, end) || line_tri(p0, p1, p3, start, end) ||
+                  line_tri(p0, p2, p3, start, end) || line_tri(p1, p2, p3, start, end);
+    if (inside)
+      push(j, weight);
+    else if (weight > 0.2)
+      push(j, weight);
+    double D = det3(Dm) < 0 ? -det3(Dm) : det3(Dm);
+    double S = (pos_det(start, p1, p2, p3) + pos_det(start, p2, p3, p0) + pos_det(start, p3, p0, p1) +
+                pos_det(start, p0, p1, p2)) / D;
+    if (std::fabs(S - 1.0) < 1e-4) push(j, weight);
+    double Ee = (pos_det(end, p1, p2, p3) + pos_det(end, p2, p3, p0) + pos_det(end, p3, p0, p1) +
+                 pos_det(end, p0, p1, p2)) / D;
+    if (std::fabs(Ee - 1.0) < 1e-4) push(j, weight);
+  }
+  return n;
+}
+
+void orc_set_sim(void* p, double h, int max_iter, double damping, const double* g, int gravity_on, int contact,
+                 int max_iter1, double alpha, double mu) {
+  Scene* s = (Scene*)p;
+  s->h = h;
+  s->max_iter = max_iter;
+  s->damping = damping;
+  for (int r = 0; r < 3; ++r) s->g[r] = g[r];
+  s->gravity_on = gravity_on;
+  s->contact = contact;
+  s->max_iter1 = max_iter1;
+  s->alpha = alpha;
+  s->mu = mu;
+  s->factored = false;
+}
+
+void orc_set_obstacles(void* p, int n_planes, const double* planes /*6 each*/, int n_spheres,
+                       const double* spheres /*4 each*/) {
+  Scene* s = (Scene*)p;
+  s->planes.clear();
+  s->spheres.clear();
+  for (int k = 0; k < n_planes; ++k) {
+    Plane pl;
+    double nn = 0;
+    for (int r = 0; r < 3; ++r) {
+      pl.p[r] = planes[6 * k + r];
+      pl.n[r] = planes[6 * k + 3 + r];
+      nn += pl.n[r] * pl.n[r];
+    }
+    nn = std::sqrt(nn);
+    for (int r = 0; r < 3; ++r) pl.n[r] /= nn;
+    s->planes.push_back(pl);
+  }
+  for (int k = 0; k < n_spheres; ++k) {
+    Sphere sp;
+    for (int r = 0; r < 3; ++r) sp.c[r] = spheres[4 * k + r];
+    sp.r = spheres[4 * k + 3];
+    s->spheres.push_back(sp);
+  }
+}
+
+int orc_factor(void* p) { return factorize(*(Scene*)p) ? 0 : 1; }
+double orc_reg_used(void* p) { return ((Scene*)p)->reg_used; }
+
+// Dense scalar K_s (N x N, row-major) for tests.
+void orc_dense_K(void* p, double* out) {
+  Scene* s = (Scene*)p;
+  if (!s->factored) factorize(*s);
+  std::fill(out, out + (size_t)s->N * s->N, 0.0);
+  for (int i = 0; i < s->N; ++i)
+    for (auto& kv : s->K[i]) out[(size_t)i * s->N + kv.first] = kv.second;
+}
+
+// Brute-force 3N x 3N H2ML = M/h^2 + L from the reference's per-constraint
+// 9x12 / 3x12 A_i stencils (CorotateFEMConstraint.h:221-350,
+// LinearMuscleConstraint.h:150-247, AttachmentConstraint.h:77-84), dense.
+void orc_dense_H2ML_bruteforce(void* p, double* out) {
+  Scene* s = (Scene*)p;
+  update_weights(*s);
+  int n3 = 3 * s->N;
+  std::fill(out, out + (size_t)n3 * n3, 0.0);
+  for (int i = 0; i < s->N; ++i)
+    for (int c = 0; c < 3; ++c) out[(size_t)(3 * i + c) * n3 + 3 * i + c] += s->mass[i] / (s->h * s->h);
+  for (size_t t = 0; t < s->tets.size(); ++t) {
+    const Tet& te = s->tets[t];
+    // invDm(m,j) = G[m+1][j]
+    double d[3][3];
+    for (int m = 0; m < 3; ++m)
+      for (int j = 0; j < 3; ++j) d[m][j] = te.G[m + 1][j];
+    double Ai[9][12];
+    std::memset(Ai, 0, sizeof(Ai));
+    for (int j = 0; j < 3; ++j)      // F column j
+      for (int c = 0; c < 3; ++c) {  // component
+        int row = 3 * j + c;
+        Ai[row][0 + c] = -d[0][j] - d[1][j] - d[2][j];
+        Ai[row][3 + c] = d[0][j];
+        Ai[row][6 + c] = d[1][j];
+        Ai[row][9 + c] = d[2][j];
+      }
+    for (int pass = 0; pass < 2; ++pass) {
+      double w = pass == 0 ? s->wr[t] : s->wv[t];
+      for (int a = 0; a < 12; ++a)
+        for (int b = 0; b < 12; ++b) {
+          double acc = 0;
+          for (int r = 0; r < 9; ++r) acc += Ai[r][a] * Ai[r][b];
+          int ga = 3 * te.v[a / 3] + a % 3, gb = 3 * te.v[b / 3] + b % 3;
+          out[(size_t)ga * n3 + gb] += w * acc;
+        }
+    }
+  }
+  for (const Muscle& m : s->muscles) {
+    double Ai[3][12];
+    std::memset(Ai, 0, sizeof(Ai));
+    for (int c = 0; c < 3; ++c)
+      for (int k = 0; k < 4; ++k) Ai[c][3 * k + c] = m.g[k];
+    for (int a = 0; a < 12; ++a)
+      for (int b = 0; b < 12; ++b) {
+        double acc = 0;
+        for (int r = 0; r < 3; ++r) acc += Ai[r][a] * Ai[r][b];
+        int ga = 3 * m.v[a / 3] + a % 3, gb = 3 * m.v[b / 3] + b % 3;
+        out[(size_t)ga * n3 + gb] += m.w * acc;
+      }
+  }
+  for (size_t k = 0; k < s->attach_idx.size(); ++k)
+    for (int c = 0; c < 3; ++c) out[(size_t)(3 * s->attach_idx[k] + c) * n3 + 3 * s->attach_idx[k] + c] += s->attach_w[k];
+}
+
+// Solve K_s X = B for a 3-component AoS vector with the oracle's factor.
+int orc_solve3(void* p, double* x) {
+  Scene* s = (Scene*)p;
+  if (!s->factored && !factorize(*s)) return 1;
+  s->sky.solve3(x);
+  return 0;
+}
+
+// J*d - L_full*x at positions x (the per-iteration force term) for tests.
+void orc_force_terms(void* p, const double* x, const double* act, double* Jd, double* Lx) {
+  Scene* s = (Scene*)p;
+  update_weights(*s);
+  std::fill(Jd, Jd + 3 * s->N, 0.0);
+  std::fill(Lx, Lx + 3 * s->N, 0.0);
+  add_Jd(*s, x, act, Jd);
+  add_Lx(*s, x, 1.0, Lx);
+}
+
+// Rollout of `steps` steps. act: steps x n_groups (or null). iters: 3 ints per
+// step (pd iterations, contact inner iterations, contact outer passes), may be null.
+int orc_rollout(void* p, int steps, const double* x0, const double* v0, const double* act, double* xT, double* vT,
+                int* iters, int record) {
+  Scene* s = (Scene*)p;
+  if (!s->factored && !factorize(*s)) return 1;
+  int n3 = 3 * s->N;
+  std::vector<double> x(x0, x0 + n3), v(v0, v0 + n3);
+  if (record) s->tape.assign(steps, StepTape());
+  else s->tape.clear();
+  for (int n = 0; n < steps; ++n) {
+    int a = 0, b = 0, c = 0;
+    step(*s, x.data(), v.data(), act ? act + (size_t)n * s->n_groups : nullptr, record ? &s->tape[n] : nullptr, &a,
+         &b, &c);
+    if (iters) {
+      iters[3 * n] = a;
+      iters[3 * n + 1] = b;
+      iters[3 * n + 2] = c;
+    }
+  }
+  std::copy(x.begin(), x.end(), xT);
+  std::copy(v.begin(), v.end(), vT);
+  return 0;
+}
+
+// Several independent rollouts (shared factor), OpenMP over rollouts when
+// built with it. x0,v0,xT,vT: R x 3N; act: R x steps x n_groups (or null).
+int orc_rollout_batch(void* p, int R, int steps, const double* x0, const double* v0, const double* act, double* xT,
+                      double* vT, int nthreads) {
+  Scene* s = (Scene*)p;
+  if (!s->factored && !factorize(*s)) return 1;
+  int n3 = 3 * s->N;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+  for (int r = 0; r < R; ++r) {
+    std::vector<double> x(x0 + (size_t)r * n3, x0 + (size_t)(r + 1) * n3), v(v0 + (size_t)r * n3, v0 + (size_t)(r + 1) * n3);
+    for (int n = 0; n < steps; ++n)
+      step(*s, x.data(), v.data(), act ? act + ((size_t)r * steps + n) * s->n_groups : nullptr, nullptr, nullptr,
+           nullptr, nullptr);
+    std::copy(x.begin(), x.end(), xT + (size_t)r * n3);
+    std::copy(v.begin(), v.end(), vT + (size_t)r * n3);
+  }
+  return 0;
+}
+
+int orc_max_threads() {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+// Gradient of <gx, x_T> + <gv, v_T> w.r.t. x0, v0, per-tet E, nu (the
+// forward value entering mMu) and activations (steps x n_groups), over the tape
+// of the last recorded rollout. Returns 2 if a step had active contacts (the
+// contact-phase reverse sweep is not restated here).
+int orc_backward(void* p, const double* gxT, const double* gvT, double* gx0, double* gv0, double* gE, double* gnu,
+                 double* gact) {
+  Scene* s = (Scene*)p;
+  int n3 = 3 * s->N;
+  int steps = (int)s->tape.size();
+  std::vector<double> gx(gxT, gxT + n3), gv(gvT, gvT + n3);
+  size_t T = s->tets.size();
+  if (gE) std::fill(gE, gE + T, 0.0);
+  if (gnu) *gnu = 0.0;
+  if (gact) std::fill(gact, gact + (size_t)steps * s->n_groups, 0.0);
+  int rc = 0;
+  for (int n = steps - 1; n >= 0; --n) {
+    if (s->tape[n].contact_active) rc = 2;
+    std::vector<double> gxn(n3, 0.0), gvn(n3, 0.0);
+    step_backward(*s, s->tape[n], gx.data(), gv.data(), gxn.data(), gvn.data(), gE, gnu,
+                  gact ? gact + (size_t)n * s->n_groups : nullptr);
+    gx.swap(gxn);
+    gv.swap(gvn);
+  }
+  std::copy(gx.begin(), gx.end(), gx0);
+  std::copy(gv.begin(), gv.end(), gv0);
+  return rc;
+}
+
+// Unit-level access to the local projection (for closed-form / numpy pins).
+void orc_corotate_project(const double* F, double* md, double* mdv) { corotate_project(F, md, mdv); }
+void orc_corotate_vjp(const double* F, double wr, double wv, const double* Z, double* Fbar) {
+  corotate_vjp(F, wr, wv, Z, Fbar);
+}
+void orc_svd3(const double* F, double* U, double* S, double* V) { svd3(F, U, S, V); }
+void orc_volume_projection(const double* S, double* D) { volume_projection(S, D); }
+int orc_detect_plane(const double* p, const double* n, const double* x0, const double* x1, double* nout) {
+  Plane pl;
+  double nn = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+  for (int r = 0; r < 3; ++r) {
+    pl.p[r] = p[r];
+    pl.n[r] = n[r] / nn;
+  }
+  return detect_plane(pl, x0, x1, nout) ? 1 : 0;
+}
+void orc_project_contact(const double* n, double mu, const double* f, double* out) {
+  double B[9];
+  contact_basis(n, B);
+  project_contact(B, mu, f, out);
+}
+
+}  // extern "C"

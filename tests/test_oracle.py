@@ -1,0 +1,327 @@
+"""Pins of the CPU FP64 oracle against mathematics, independent library
+routines and brute force (the reference ships no golden vectors for this path;
+its compiled build needs Eigen/CGAL/Boost/CppAD, absent here)."""
+import os
+
+import numpy as np
+import pytest
+
+from diff_fem_amd import scene
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def rand_rot(rng):
+    q, r = np.linalg.qr(rng.standard_normal((3, 3)))
+    q = q * np.sign(np.diag(r))
+    if np.linalg.det(q) < 0:
+        q[:, 0] = -q[:, 0]
+    return q
+
+
+def polar_numpy(F):
+    u, s, vt = np.linalg.svd(F)
+    return u @ vt, u, s, vt
+
+
+# ---------------------------------------------------------------- local step
+def test_svd_matches_numpy(oracle_mod):
+    rng = np.random.default_rng(0)
+    for _ in range(200):
+        F = rng.standard_normal((3, 3))
+        U, S, V = oracle_mod.svd3(F)
+        _, _, s_np, _ = polar_numpy(F)
+        assert np.allclose(S, s_np, rtol=1e-13, atol=1e-14)
+        assert np.allclose(U @ np.diag(S) @ V.T, F, atol=1e-13)
+        assert np.allclose(U.T @ U, np.eye(3), atol=1e-13)
+        assert np.allclose(V.T @ V, np.eye(3), atol=1e-13)
+
+
+def test_projection_closed_forms(oracle_mod):
+    rng = np.random.default_rng(1)
+    md, mdv = oracle_mod.corotate_project(np.eye(3))
+    assert np.allclose(md, np.eye(3), atol=1e-15) and np.allclose(mdv, np.eye(3), atol=1e-15)
+    for _ in range(20):
+        Q = rand_rot(rng)
+        md, mdv = oracle_mod.corotate_project(Q)
+        assert np.allclose(md, Q, atol=1e-13) and np.allclose(mdv, Q, atol=1e-13)
+        s = rng.uniform(0.6, 1.5, 3)
+        F = Q @ np.diag(s) @ rand_rot(rng).T
+        md, mdv = oracle_mod.corotate_project(F)
+        R, u, sv, vt = polar_numpy(F)
+        assert np.allclose(md, R, atol=1e-12)  # polar factor is unique: numpy agrees
+        D = oracle_mod.volume_projection(sv)
+        assert np.allclose(mdv, u @ np.diag(sv + D) @ vt, atol=1e-12)
+        # the 5 linearised steps land on the volume-preserving manifold
+        assert abs(np.prod(sv + D) - 1.0) < 1e-6
+
+
+def test_volume_projection_properties(oracle_mod):
+    rng = np.random.default_rng(2)
+    for _ in range(50):
+        S = rng.uniform(0.7, 1.4, 3)
+        D = oracle_mod.volume_projection(S)
+        g = np.array([S[1] * S[2], S[0] * S[2], S[0] * S[1]])
+        # isotropic stretch: correction is isotropic and exact to 1e-9
+        a = rng.uniform(0.8, 1.2)
+        Da = oracle_mod.volume_projection(np.array([a, a, a]))
+        assert np.allclose(Da, Da[0]) and abs((a + Da[0]) ** 3 - 1) < 1e-9
+        # permutation equivariance
+        p = rng.permutation(3)
+        assert np.allclose(oracle_mod.volume_projection(S[p]), D[p], atol=1e-15)
+        assert abs(np.prod(S + D) - 1.0) < 1e-6
+        # identity -> no correction
+    assert np.allclose(oracle_mod.volume_projection(np.ones(3)), 0)
+
+
+def test_inverted_tet_flip(oracle_mod):
+    rng = np.random.default_rng(3)
+    for _ in range(20):
+        F = rand_rot(rng) @ np.diag(rng.uniform(0.5, 1.5, 3)) @ rand_rot(rng).T
+        F[:, 2] *= -1  # det < 0
+        md, _ = oracle_mod.corotate_project(F)
+        R, *_ = polar_numpy(F)
+        assert np.linalg.det(R) < 0
+        assert np.allclose(md, R @ np.diag([1, 1, -1]), atol=1e-12)
+        assert np.linalg.det(md) > 0
+
+
+def test_rotation_equivariance(oracle_mod):
+    rng = np.random.default_rng(4)
+    for _ in range(20):
+        F = np.eye(3) + 0.3 * rng.standard_normal((3, 3))
+        Q = rand_rot(rng)
+        a, b = oracle_mod.corotate_project(F)
+        qa, qb = oracle_mod.corotate_project(Q @ F)
+        assert np.allclose(qa, Q @ a, atol=1e-12) and np.allclose(qb, Q @ b, atol=1e-12)
+
+
+def test_vjp_matches_finite_differences(oracle_mod):
+    rng = np.random.default_rng(5)
+    for trial in range(30):
+        F = np.eye(3) + 0.35 * rng.standard_normal((3, 3))
+        if trial % 5 == 0:
+            F[:, 2] *= -1
+        if trial % 7 == 1:  # repeated singular values
+            F = rand_rot(rng) @ np.diag([1.1, 1.1, 0.9]) @ rand_rot(rng).T
+        wr, wv = 1.3, 0.7
+        Z = rng.standard_normal((3, 3))
+        dF = rng.standard_normal((3, 3))
+        Fb = oracle_mod.corotate_vjp(F, wr, wv, Z)
+        h = 1e-6
+
+        def f(M):
+            a, b = oracle_mod.corotate_project(M)
+            return np.sum(Z * (wr * a + wv * b))
+
+        fd = (f(F + h * dF) - f(F - h * dF)) / (2 * h)
+        assert abs(fd - np.sum(Fb * dF)) < 1e-6 * max(1.0, abs(fd)), trial
+
+
+# ---------------------------------------------------------------- setup + assembly
+def small_muscle_scene(contact=False):
+    x, t = scene.box_grid(4, 2, 2)
+    s0, s1, g, ng = scene.straight_fibres(1, 2, x.min(0), x.max(0), nseg=6)
+    rng = np.random.default_rng(7)
+    return scene.SceneDesc(x_rest=x, tets=t, total_mass=5.0, E=rng.uniform(5e4, 2e5, t.shape[0]), nu_fwd=0.4,
+                           nu_ctor=np.full(t.shape[0], 0.4), attach_idx=np.array([0, 1, 2], np.int32),
+                           attach_w=1e7, seg_start=s0, seg_end=s1, seg_group=g, n_groups=ng, k_muscle=5e5,
+                           h=0.01, max_iter=10, gravity=(0, 0, -9.81), contact=contact, mu=0.5,
+                           planes=np.array([[0, 0, -0.01, 0, 0, 1.0]]))
+
+
+def test_masses_and_orientation(oracle_mod):
+    x, t = scene.box_grid(1, 1, 1)
+    d = scene.SceneDesc(x_rest=x, tets=t, total_mass=6.0)
+    s = oracle_mod.OracleScene(d)
+    m = s.masses()
+    assert abs(m.sum() - 6.0) < 1e-14
+    v, vol, G = s.tets()
+    assert np.all(vol > 0) and abs(vol.sum() - 0.05 ** 3) < 1e-18
+    # m_i proportional to incident volume / 4 (Kuhn split: diagonal corners in all 6 tets)
+    inc = np.zeros(8)
+    for tt in t:
+        inc[tt] += 1
+    assert np.allclose(m, 6.0 * inc / 24.0)
+
+
+def test_kronecker_assembly_matches_bruteforce(oracle_mod):
+    d = small_muscle_scene()
+    s = oracle_mod.OracleScene(d)
+    assert s.muscles[0].shape[0] > 0
+    K = s.dense_K()
+    H = s.dense_H2ML_bruteforce()
+    assert np.allclose(np.kron(K, np.eye(3)), H, rtol=1e-13, atol=1e-9)
+
+
+def test_solve_matches_dense(oracle_mod):
+    d = scene.config1()
+    s = oracle_mod.OracleScene(d)
+    K = s.dense_K()
+    rng = np.random.default_rng(8)
+    b = rng.standard_normal((s.N, 3))
+    x = s.solve3(b)
+    xr = np.linalg.solve(K, b)
+    assert np.allclose(x, xr, rtol=1e-10, atol=1e-30)
+    assert np.linalg.norm(K @ x - b) < 1e-8 * np.linalg.norm(b) * 1e6
+
+
+# ---------------------------------------------------------------- PD loop invariants
+def test_rest_state_is_fixed_point(oracle_mod):
+    d = scene.config1()
+    d.gravity = (0.0, 0.0, 0.0)
+    s = oracle_mod.OracleScene(d)
+    x0 = d.x_rest.ravel().copy()
+    xT, vT, its = s.rollout(x0, np.zeros_like(x0), 5)
+    assert np.abs(xT - x0).max() < 1e-13 and np.abs(vT).max() < 1e-11
+
+
+def test_free_fall_velocity_recursion(oracle_mod):
+    d = scene.config1()
+    d.attach_idx = np.zeros(0, np.int32)
+    s = oracle_mod.OracleScene(d)
+    x0 = d.x_rest.ravel().copy()
+    v = np.zeros(3)
+    for n in range(1, 6):
+        xT, vT, _ = s.rollout(x0, np.zeros_like(x0), n)
+    vz = 0.0
+    for _ in range(5):
+        vz = (vz - 9.81 * 0.01) * 0.9999
+    assert np.allclose(vT.reshape(-1, 3)[:, 2], vz, rtol=1e-10)
+    assert np.allclose(vT.reshape(-1, 3)[:, :2], 0, atol=1e-12)
+
+
+def test_early_exit_counts(oracle_mod):
+    d = scene.config1()
+    s = oracle_mod.OracleScene(d)
+    x0 = d.x_rest.ravel().copy()
+    _, _, its = s.rollout(x0, np.zeros_like(x0), 20)
+    assert np.all(its[:, 0] >= 1) and np.all(its[:, 0] <= 10)
+
+
+# ---------------------------------------------------------------- gradient vs FD
+def _fd_check(s, x0, v0, steps, act, gx, gv, param_eval, analytic, h):
+    fd = (param_eval(+h) - param_eval(-h)) / (2 * h)
+    assert abs(fd - analytic) <= 1e-5 * max(abs(fd), 1e-12), (fd, analytic)
+
+
+def test_gradient_vs_central_fd_config1(oracle_mod):
+    d = scene.config1()
+    s = oracle_mod.OracleScene(d)
+    x0 = d.x_rest.ravel().copy()
+    v0 = np.zeros_like(x0)
+    steps = 15
+    rng = np.random.default_rng(9)
+    gx = rng.standard_normal(x0.shape)
+    gv = 0.01 * rng.standard_normal(x0.shape)
+    s.rollout(x0, v0, steps, record=True)
+    g, rc = s.backward(gx, gv)
+    assert rc == 0
+
+    def loss(E=1e5, nu=0.4, dx=None, dv=None):
+        s.set_material(np.full(s.T, E), nu)
+        xT, vT, _ = s.rollout(x0 + (0 if dx is None else dx), v0 + (0 if dv is None else dv), steps)
+        return gx @ xT + gv @ vT
+
+    hE = 1e5 * 1e-6
+    fd = (loss(E=1e5 + hE) - loss(E=1e5 - hE)) / (2 * hE)
+    assert abs(fd - g["E"].sum()) < 1e-5 * abs(fd)
+    hn = 1e-6
+    fd = (loss(nu=0.4 + hn) - loss(nu=0.4 - hn)) / (2 * hn)
+    assert abs(fd - g["nu"]) < 1e-5 * abs(fd)
+    dirx = rng.standard_normal(x0.shape) * 1e-7
+    fd = (loss(dx=dirx) - loss(dx=-dirx)) / 2
+    assert abs(fd - g["x0"] @ dirx) < 1e-5 * abs(fd)
+    dirv = rng.standard_normal(x0.shape) * 1e-6
+    fd = (loss(dv=dirv) - loss(dv=-dirv)) / 2
+    assert abs(fd - g["v0"] @ dirv) < 1e-5 * abs(fd)
+    s.set_material(np.full(s.T, 1e5), 0.4)
+
+
+def test_gradient_per_tet_E_and_activations(oracle_mod):
+    d = small_muscle_scene()
+    s = oracle_mod.OracleScene(d)
+    x0 = d.x_rest.ravel().copy()
+    v0 = np.zeros_like(x0)
+    steps = 8
+    rng = np.random.default_rng(10)
+    act = rng.uniform(0, 1, (steps, d.n_groups))
+    gx = rng.standard_normal(x0.shape)
+    gv = 0.01 * rng.standard_normal(x0.shape)
+    s.rollout(x0, v0, steps, act=act, record=True)
+    g, rc = s.backward(gx, gv)
+    assert rc == 0
+    E0 = d.E.copy()
+
+    def loss(E=E0, a=act):
+        s.set_material(E, d.nu_fwd)
+        xT, vT, _ = s.rollout(x0, v0, steps, act=a)
+        return gx @ xT + gv @ vT
+
+    for t in [0, 5, 17]:
+        e = np.zeros_like(E0)
+        e[t] = E0[t] * 1e-5
+        fd = (loss(E=E0 + e) - loss(E=E0 - e)) / (2 * e[t])
+        assert abs(fd - g["E"][t]) < 1e-4 * abs(fd) + 1e-12, (t, fd, g["E"][t])
+    for (n, k) in [(0, 0), (3, 1), (7, 0)]:
+        da = np.zeros_like(act)
+        da[n, k] = 1e-6
+        fd = (loss(a=act + da) - loss(a=act - da)) / 2e-6
+        assert abs(fd - g["act"][n, k]) < 1e-4 * abs(fd) + 1e-12, (n, k, fd, g["act"][n, k])
+    s.set_material(E0, d.nu_fwd)
+
+
+# ---------------------------------------------------------------- contact pins
+def test_plane_detection_table(oracle_mod):
+    p, n = np.zeros(3), np.array([0, 0, 1.0])
+    cases = [
+        ((0, 0, 0.1), (0, 0, -0.05), True, +1),    # crossing downwards: normal +z
+        ((0, 0, 0.1), (0, 0, 0.05), False, 0),     # above, no crossing
+        ((0, 0, 0.1), (0, 0, 5e-5), True, +1),     # ends inside thickness
+        ((0, 0, -0.1), (0, 0, -0.05), False, 0),   # below, no crossing
+        ((0, 0, -0.1), (0, 0, 0.05), True, -1),    # crossing upwards: normal opposes motion
+        ((0, 0, -5e-5), (0, 0, 2e-5), True, -1),
+    ]
+    for x0, x1, hit, sg in cases:
+        h, nn = oracle_mod.detect_plane(p, n, np.array(x0, float), np.array(x1, float))
+        assert h == hit, (x0, x1)
+        if hit:
+            assert np.allclose(nn, sg * n)
+
+
+def test_cone_projection_cases(oracle_mod):
+    n = np.array([0, 0, 1.0])
+    # take-off: f_n > 0 -> unchanged
+    f = np.array([0.3, -0.2, 1.0])
+    assert np.allclose(oracle_mod.project_contact(n, 0.5, f), f)
+    # stick: |f_t| <= mu |f_n| -> reaction -f
+    f = np.array([0.1, 0.05, -1.0])
+    assert np.allclose(oracle_mod.project_contact(n, 0.5, f), -f)
+    # slide: tangential clamped to the cone
+    f = np.array([2.0, 0.0, -1.0])
+    r = oracle_mod.project_contact(n, 0.5, f)
+    assert np.allclose(r, [-0.5, 0.0, 1.0])
+    r = oracle_mod.project_contact(n, 1.0, np.array([0.0, 3.0, -1.0]))
+    assert np.allclose(r, [0.0, -1.0, 1.0])
+
+
+def test_contact_rollout_keeps_body_above_ground(oracle_mod):
+    x, t = scene.box_grid(4, 2, 2, origin=(0, 0, 0.02))
+    d = scene.SceneDesc(x_rest=x, tets=t, total_mass=2.0, E=np.full(t.shape[0], 1e5), h=0.01, max_iter=10,
+                        gravity=(0, 0, -9.81), contact=True, max_iter1=10, mu=0.5,
+                        planes=np.array([[0, 0, 0, 0, 0, 1.0]]))
+    s = oracle_mod.OracleScene(d)
+    x0 = x.ravel().copy()
+    xT, vT, its = s.rollout(x0, np.zeros_like(x0), 40)
+    assert its[:, 2].max() >= 1  # contact phase ran
+    assert xT.reshape(-1, 3)[:, 2].min() > -0.02
+
+
+def test_bridgesus_fixture_runs(oracle_mod):
+    d = scene.load_json_scene(os.path.join(GOLD, "bridgesus.json"), data_root="")
+    assert d.N == 212 and d.T == 668
+    s = oracle_mod.OracleScene(d)
+    x0 = d.x_rest.ravel().copy()
+    xT, vT, its = s.rollout(x0, np.zeros_like(x0), 3)
+    assert np.all(np.isfinite(xT))
+    assert np.abs(xT.reshape(-1, 3)[d.attach_idx] - d.x_rest[d.attach_idx]).max() < 1e-12
