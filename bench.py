@@ -1,0 +1,162 @@
+"""Benchmark: tet-iterations/s of forward + adjoint, batched mixed precision
+vs the plain FP64 GPU path, on one MI355X.
+
+Default workload (the batched-rollout configuration): 40x12x12 box
+(34,560 tets, 6,929 vertices), 64 straight fibres, R = 256 rollouts sharing one
+Cholesky factor, seeded sinusoidal activations, 100 steps x 10 iterations,
+forward + adjoint with checkpoints every 20 steps, FP32 local step with FP64
+factor/solves/adjoint accumulation.
+
+One bench "step" = one full forward + adjoint pass over the batch (inputs
+already resident in HBM). value = sum over rollouts of T * executed PD
+iterations * steps / (t_forward + t_adjoint).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_workload(args):
+    from diff_fem_amd import scene
+    d = scene.config4(seed=4)
+    d.contact = bool(args.contact)
+    if not d.contact:
+        d.planes = np.zeros((0, 6))
+    return d
+
+
+def cpu_baseline(d, steps, nthreads):
+    """Oracle (C++ FP64 restatement, single thread) on a bounded sample: one
+    rollout, `steps` steps forward + adjoint."""
+    from oracle import oracle as O
+    from diff_fem_amd import scene
+    o = O.OracleScene(d)
+    acts = scene.activations_sinusoid(4, steps, d.n_groups, 1, d.h)[0]
+    x0 = d.x_rest.ravel().copy()
+    v0 = np.zeros_like(x0)
+    t0 = time.perf_counter()
+    xT, vT, its = o.rollout(x0, v0, steps, act=acts, record=True)
+    g, _ = o.backward(np.ones_like(xT), np.zeros_like(vT))
+    dt = time.perf_counter() - t0
+    tet_it = float(d.T * its[:, 0].sum())
+    return dict(value=tet_it / dt, unit="tet-iterations/s", cores=1, kind="port",
+                sample="oracle FP64 C++, 1 rollout x %d steps fwd+adjoint of the 40x12x12 muscle box (%.1f s)" % (steps, dt))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2, help="timed forward+adjoint passes")
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--rollouts", type=int, default=256)
+    ap.add_argument("--sim-steps", type=int, default=100)
+    ap.add_argument("--ckpt", type=int, default=20)
+    ap.add_argument("--plain-steps", type=int, default=10, help="sim steps of the plain FP64 comparator")
+    ap.add_argument("--contact", type=int, default=0)
+    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: the rollout batch is not sharded across GPUs in this build "
+              "(run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from diff_fem_amd import engine, scene
+
+    d = build_workload(args)
+    R, S = args.rollouts, args.sim_steps
+    log("setup: T=%d N=%d R=%d steps=%d" % (d.T, d.N, R, S))
+    t0 = time.perf_counter()
+    sc = engine.Scene(d, R=R, precision=engine.PDM_MIXED, flags=0, checkpoint_every=args.ckpt)
+    info = sc.info()
+    log("setup done in %.1fs: %s" % (time.perf_counter() - t0, info))
+    dev = torch.device("cuda:0")
+    acts = torch.from_numpy(scene.activations_sinusoid(4, S, d.n_groups, R, d.h)).to(dev).contiguous()
+    x0 = torch.from_numpy(np.tile(d.x_rest.ravel(), (R, 1))).to(dev).contiguous()
+    v0 = torch.zeros_like(x0)
+    gx = torch.ones_like(x0) / x0.numel()
+    gv = torch.zeros_like(x0)
+    xT = torch.empty_like(x0)
+    vT = torch.empty_like(x0)
+
+    def one_pass():
+        sc.forward_device(x0, v0, S, act=acts, xT=xT, vT=vT, record=True)
+        g = sc.backward_device(gx, gv)
+        return g
+
+    for w in range(args.warmup):
+        one_pass()
+        torch.cuda.synchronize()
+        log("warmup %d done" % w)
+    times, tet_its, fwd_ms, bwd_ms = [], [], [], []
+    for k in range(args.steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        one_pass()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        its = sc.iterations()
+        tet_its.append(float(d.T) * float(its.sum()))
+        times.append(dt)
+        tm = sc.timing()
+        fwd_ms.append(tm["forward_ms"])
+        bwd_ms.append(tm["backward_ms"])
+        log("pass %d: %.3fs fwd %.1fms bwd %.1fms mean iters %.2f" % (k, dt, tm["forward_ms"], tm["backward_ms"],
+                                                                      its.mean()))
+    total_t = sum(times)
+    value = sum(tet_its) / total_t
+    out = {
+        "metric": "tet-iterations/sec fwd+adjoint, batched mixed vs plain FP64 on GPU at stated err",
+        "value": value, "unit": "tet-iterations/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1000.0 * total_t / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32-local/fp64-solve", "data": "synthetic (seeded box mesh, seeded activations)",
+        "config": {"workload": "c4_muscle_40x12x12_R%d_%dsteps_fwd+adjoint%s" % (R, S, "" if d.contact else "_nocontact"),
+                   "tets": d.T, "vertices": d.N, "rollouts": R, "sim_steps": S, "max_iter": d.max_iter,
+                   "checkpoint_every": args.ckpt, "contact": bool(d.contact), "nnz_factor": info["nnz_factor"],
+                   "forward_ms": float(np.mean(fwd_ms)), "adjoint_ms": float(np.mean(bwd_ms))},
+    }
+    sc.close()
+    del sc
+    if not args.no_plain:
+        ps = args.plain_steps
+        pl = engine.Scene(d, R=1, precision=engine.PDM_FP64, flags=engine.PDM_HOST_EARLY_EXIT,
+                          checkpoint_every=args.ckpt)
+        px0 = x0[:1].contiguous()
+        pv0 = v0[:1].contiguous()
+        pa = acts[:1, :ps].contiguous()
+        pl.forward_device(px0, pv0, ps, act=pa, record=True)
+        pl.backward_device(gx[:1].contiguous(), gv[:1].contiguous())
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        pl.forward_device(px0, pv0, ps, act=pa, record=True)
+        pl.backward_device(gx[:1].contiguous(), gv[:1].contiguous())
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        pv = d.T * float(pl.iterations().sum()) / dt
+        out["plain_fp64"] = {"value": pv, "unit": "tet-iterations/s", "rollouts": 1, "sim_steps": ps,
+                             "host_early_exit": True}
+        out["speedup_vs_plain_fp64"] = value / pv
+        pl.close()
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(d, args.cpu_steps, 1)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

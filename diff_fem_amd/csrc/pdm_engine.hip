@@ -1,0 +1,1482 @@
+// libpdm engine: device data, HIP kernels of the PD step / adjoint, and the
+// C ABI declared in include/pdm.h.
+//
+// Layout in HBM. Vertices are in the engine's nested-dissection order. Every
+// per-vertex state array is row-major [N][C] with C = 3R columns
+// (column 3r+c = component c of rollout r), so rollouts are the contiguous
+// dimension: a wavefront of the local step reads 64 rollouts of one tet with
+// unit stride, and the supernodal solves see R*3 right-hand sides per row.
+// Tet data is SoA ([k][T]); tet->vertex forces go through per-(element,corner)
+// slots grouped by vertex ([slot][C]), summed per vertex in a fixed order
+// (deterministic segmented reduction, no atomics).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/pdm.h"
+#include "pdm_internal.h"
+#include "pdm_kernels.h"
+
+using namespace pdm;
+
+namespace {
+
+thread_local std::string g_err;
+
+#define HIPX(expr)                                                                       \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess) {                                                              \
+      g_err = std::string(#expr " failed: ") + hipGetErrorString(e_);                    \
+      return 1;                                                                          \
+    }                                                                                    \
+  } while (0)
+
+constexpr double kEps = 5e-7;  // World.h:214
+constexpr int kThreads = 256;
+
+inline int cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+// ------------------------------------------------------------------ kernels
+__global__ void k_caller_to_engine(const double* __restrict__ src, double* __restrict__ dst,
+                                   const int* __restrict__ perm, int N, int R) {
+  long long C = 3LL * R;
+  long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx >= (long long)N * C) return;
+  int v = (int)(idx / C);
+  int j = (int)(idx % C);
+  int r = j / 3, c = j % 3;
+  dst[idx] = src[(long long)r * 3 * N + 3LL * perm[v] + c];
+}
+
+__global__ void k_engine_to_caller(const double* __restrict__ src, double* __restrict__ dst,
+                                   const int* __restrict__ perm, int N, int R) {
+  long long C = 3LL * R;
+  long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx >= (long long)N * C) return;
+  int v = (int)(idx / C);
+  int j = (int)(idx % C);
+  int r = j / 3, c = j % 3;
+  dst[(long long)r * 3 * N + 3LL * perm[v] + c] = src[idx];
+}
+
+// Y = A X for the scalar CSR A applied to every column.
+__global__ void k_spmv(const int* __restrict__ ptr, const int* __restrict__ col, const double* __restrict__ val,
+                       const double* __restrict__ X, double* __restrict__ Y, int N, int C) {
+  long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx >= (long long)N * C) return;
+  int v = (int)(idx / C);
+  int j = (int)(idx % C);
+  double s = 0.0;
+  for (int q = ptr[v]; q < ptr[v + 1]; ++q) s += val[q] * X[(long long)col[q] * C + j];
+  Y[idx] = s;
+}
+
+// Start of a step: gravity kick (Environment.h:236-245), qn = x + h v*
+// (World.h:692-699) and the iteration-invariant rhs c = M v*/h - L x + W p.
+__global__ void k_step_prep(const double* __restrict__ X, const double* __restrict__ V, const double* __restrict__ Lx,
+                            const double* __restrict__ mass, const double* __restrict__ attw,
+                            const double* __restrict__ xrest, double* __restrict__ Xn, double* __restrict__ Vs,
+                            double* __restrict__ Cc, double* __restrict__ Xit, int N, int C, double h, double g0,
+                            double g1, double g2, int gravity_on) {
+  long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx >= (long long)N * C) return;
+  int v = (int)(idx / C);
+  int c = (int)(idx % C) % 3;
+  double g = c == 0 ? g0 : (c == 1 ? g1 : g2);
+  double vs = V[idx] + (gravity_on ? g * h : 0.0);
+  double x = X[idx];
+  Xn[idx] = x;
+  Vs[idx] = vs;
+  Xit[idx] = x + h * vs;
+  Cc[idx] = mass[v] * vs / h - Lx[idx] + attw[v] * xrest[3 * v + c];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_local_tet(const double* __restrict__ X, const int* __restrict__ tv,
+                                                   const double* __restrict__ tG, const double* __restrict__ wr,
+                                                   const double* __restrict__ wv, const int* __restrict__ tslot,
+                                                   const int* __restrict__ active, T* __restrict__ corner, int Tn,
+                                                   int R) {
+  long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx >= (long long)Tn * R) return;
+  int t = (int)(idx / R);
+  int r = (int)(idx % R);
+  if (!active[r]) return;
+  int C = 3 * R;
+  int v0 = tv[t], v1 = tv[Tn + t], v2 = tv[2 * Tn + t], v3 = tv[3 * Tn + t];
+  const double* x0 = X + (long long)v0 * C + 3 * r;
+  const double* x1 = X + (long long)v1 * C + 3 * r;
+  const double* x2 = X + (long long)v2 * C + 3 * r;
+  const double* x3 = X + (long long)v3 * C + 3 * r;
+  double b0 = x0[0], b1 = x0[1], b2 = x0[2];
+  // Ds = [x1-x0, x2-x0, x3-x0] in FP64, then F = Ds invDm in T
+  T ds[9];
+  ds[0] = (T)(x1[0] - b0); ds[1] = (T)(x2[0] - b0); ds[2] = (T)(x3[0] - b0);
+  ds[3] = (T)(x1[1] - b1); ds[4] = (T)(x2[1] - b1); ds[5] = (T)(x3[1] - b1);
+  ds[6] = (T)(x1[2] - b2); ds[7] = (T)(x2[2] - b2); ds[8] = (T)(x3[2] - b2);
+  T G[12];
+#pragma unroll
+  for (int q = 0; q < 12; ++q) G[q] = (T)tG[(long long)q * Tn + t];
+  T F[9];
+#pragma unroll
+  for (int rr = 0; rr < 3; ++rr)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      F[rr * 3 + j] = ds[rr * 3 + 0] * G[3 + j] + ds[rr * 3 + 1] * G[6 + j] + ds[rr * 3 + 2] * G[9 + j];
+  T P[9];
+  corotate_P<T>(F, (T)wr[t], (T)wv[t], P);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    T* o = corner + (long long)tslot[k * Tn + t] * C + 3 * r;
+#pragma unroll
+    for (int rr = 0; rr < 3; ++rr) o[rr] = P[rr * 3 + 0] * G[3 * k + 0] + P[rr * 3 + 1] * G[3 * k + 1] + P[rr * 3 + 2] * G[3 * k + 2];
+  }
+}
+
+// Linear muscle: d = (1 - a) F f (LinearMuscleConstraint.h:265-271).
+template <typename T>
+__global__ void k_local_muscle(const double* __restrict__ X, const int* __restrict__ mv, const double* __restrict__ mg,
+                               const double* __restrict__ mw, const int* __restrict__ mgroup,
+                               const int* __restrict__ mslot, const double* __restrict__ act, int act_stride_r,
+                               int act_off, const int* __restrict__ active, T* __restrict__ corner, int M, int R) {
+  long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx >= (long long)M * R) return;
+  int m = (int)(idx / R);
+  int r = (int)(idx % R);
+  if (!active[r]) return;
+  int C = 3 * R;
+  double a = act ? act[(long long)r * act_stride_r + act_off + mgroup[m]] : 0.0;
+  int v[4];
+  T g[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    v[k] = mv[k * M + m];
+    g[k] = (T)mg[k * M + m];
+  }
+  const double* xb = X + (long long)v[0] * C + 3 * r;
+  T ff[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    double base = xb[c];
+    ff[c] = g[1] * (T)(X[(long long)v[1] * C + 3 * r + c] - base) + g[2] * (T)(X[(long long)v[2] * C + 3 * r + c] - base) +
+            g[3] * (T)(X[(long long)v[3] * C + 3 * r + c] - base);
+  }
+  T sc = (T)(mw[m] * (1.0 - a));
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    T* o = corner + (long long)mslot[k * M + m] * C + 3 * r;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) o[c] = sc * ff[c] * g[k];
+  }
+}
+
+// B = Cc + sum of the vertex's corner slots (fixed order).
+template <typename T>
+__global__ void k_gather(const double* __restrict__ Cc, const T* __restrict__ corner, const int* __restrict__ vptr,
+                         double* __restrict__ B, int N, int C) {
+  long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx >= (long long)N * C) return;
+  int v = (int)(idx / C);
+  int j = (int)(idx % C);
+  double s = Cc ? Cc[idx] : 0.0;
+  for (int q = vptr[v]; q < vptr[v + 1]; ++q) s += (double)corner[(long long)q * C + j];
+  B[idx] = s;
+}
+
+// Adjoint gather: Ybar = sum of corners for rollouts active at this
+// iteration; others keep their value.
+template <typename T>
+__global__ void k_gather_masked(const T* __restrict__ corner, const int* __restrict__ vptr,
+                                const int* __restrict__ active, double* __restrict__ B, int N, int C) {
+  long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx >= (long long)N * C) return;
+  int v = (int)(idx / C);
+  int j = (int)(idx % C);
+  if (!active[j / 3]) return;
+  double s = 0.0;
+  for (int q = vptr[v]; q < vptr[v + 1]; ++q) s += (double)corner[(long long)q * C + j];
+  B[idx] = s;
+}
+
+struct FactorView {
+  const int* first;
+  const long long* rowptr;
+  const int* rows;
+  const long long* lptr;
+  const double* L;
+  const int* child_ptr;
+  const int* child;
+  const long long* relptr;
+  const int* rel;
+  const long long* uptr;
+};
+
+// Forward substitution (L y = b) of one elimination-tree level. Block =
+// (supernode, column tile). The front T (m x Ct) is staged in LDS: own rows
+// from B, children's update blocks added in child order, then the triangular
+// solve of the diagonal block and the update block U_s = T_below - L_below y.
+__global__ __launch_bounds__(256) void k_fwd_level(FactorView f, const int* __restrict__ lev, int ntile, int Ct, int C,
+                                                   double* __restrict__ B, double* __restrict__ U) {
+  extern __shared__ double sh[];
+  int s = lev[blockIdx.x / ntile];
+  int col0 = (blockIdx.x % ntile) * Ct;
+  int ct = min(Ct, C - col0);
+  int c0 = f.first[s];
+  int w = f.first[s + 1] - c0;
+  long long rp = f.rowptr[s];
+  int m = (int)(f.rowptr[s + 1] - rp);
+  const double* Ls = f.L + f.lptr[s];
+  int tid = threadIdx.x, nt = blockDim.x;
+  for (int i = tid; i < m * ct; i += nt) {
+    int row = i / ct, c = i % ct;
+    sh[row * Ct + c] = row < w ? B[(long long)(c0 + row) * C + col0 + c] : 0.0;
+  }
+  __syncthreads();
+  for (int q = f.child_ptr[s]; q < f.child_ptr[s + 1]; ++q) {
+    int ch = f.child[q];
+    int wc = f.first[ch + 1] - f.first[ch];
+    int mbc = (int)(f.rowptr[ch + 1] - f.rowptr[ch]) - wc;
+    const int* rel = f.rel + f.relptr[ch];
+    const double* Uc = U + f.uptr[ch] * C;
+    for (int i = tid; i < mbc * ct; i += nt) {
+      int a = i / ct, c = i % ct;
+      sh[rel[a] * Ct + c] += Uc[(long long)a * C + col0 + c];
+    }
+    __syncthreads();
+  }
+  // triangular solve, one thread per column
+  if (tid < ct) {
+    for (int k = 0; k < w; ++k) {
+      double yk = sh[k * Ct + tid] / Ls[(long long)k * m + k];
+      sh[k * Ct + tid] = yk;
+      for (int i = k + 1; i < w; ++i) sh[i * Ct + tid] -= Ls[(long long)k * m + i] * yk;
+    }
+  }
+  __syncthreads();
+  for (int i = tid; i < w * ct; i += nt) {
+    int row = i / ct, c = i % ct;
+    B[(long long)(c0 + row) * C + col0 + c] = sh[row * Ct + c];
+  }
+  int mb = m - w;
+  if (mb > 0) {
+    double* Us = U + f.uptr[s] * C;
+    for (int i = tid; i < mb * ct; i += nt) {
+      int a = i / ct, c = i % ct;
+      double acc = sh[(w + a) * Ct + c];
+      for (int k = 0; k < w; ++k) acc -= Ls[(long long)k * m + w + a] * sh[k * Ct + c];
+      Us[(long long)a * C + col0 + c] = acc;
+    }
+  }
+}
+
+// Backward substitution (L^T x = y) of one level: t = y_s - L_below^T x_below,
+// then the transposed triangular solve of the diagonal block.
+__global__ __launch_bounds__(256) void k_bwd_level(FactorView f, const int* __restrict__ lev, int ntile, int Ct, int C,
+                                                   double* __restrict__ B) {
+  extern __shared__ double sh[];
+  int s = lev[blockIdx.x / ntile];
+  int col0 = (blockIdx.x % ntile) * Ct;
+  int ct = min(Ct, C - col0);
+  int c0 = f.first[s];
+  int w = f.first[s + 1] - c0;
+  long long rp = f.rowptr[s];
+  int m = (int)(f.rowptr[s + 1] - rp);
+  const int* rows = f.rows + rp;
+  const double* Ls = f.L + f.lptr[s];
+  int tid = threadIdx.x, nt = blockDim.x;
+  for (int i = tid; i < m * ct; i += nt) {
+    int row = i / ct, c = i % ct;
+    sh[row * Ct + c] = B[(long long)rows[row] * C + col0 + c];
+  }
+  __syncthreads();
+  int mb = m - w;
+  if (mb > 0) {
+    for (int i = tid; i < w * ct; i += nt) {
+      int k = i / ct, c = i % ct;
+      double acc = sh[k * Ct + c];
+      const double* lk = Ls + (long long)k * m + w;
+      for (int a = 0; a < mb; ++a) acc -= lk[a] * sh[(w + a) * Ct + c];
+      sh[k * Ct + c] = acc;
+    }
+    __syncthreads();
+  }
+  if (tid < ct) {
+    for (int k = w - 1; k >= 0; --k) {
+      const double* lk = Ls + (long long)k * m;
+      double acc = sh[k * Ct + tid];
+      for (int i = k + 1; i < w; ++i) acc -= lk[i] * sh[i * Ct + tid];
+      sh[k * Ct + tid] = acc / lk[k];
+    }
+  }
+  __syncthreads();
+  for (int i = tid; i < w * ct; i += nt) {
+    int row = i / ct, c = i % ct;
+    B[(long long)(c0 + row) * C + col0 + c] = sh[row * Ct + c];
+  }
+}
+
+// x_new = x_n + B for active rollouts; per-(chunk, column) partial sums of
+// (x_new - x_old)^2 for the early-exit test. Block (64 columns, 4 rows).
+constexpr int kChunk = 64;
+__global__ void k_update(const double* __restrict__ Xn, const double* __restrict__ B, double* __restrict__ Xit,
+                         const int* __restrict__ active, double* __restrict__ part, double* __restrict__ tape, int N,
+                         int C) {
+  __shared__ double red[4][64];
+  int col = blockIdx.x * 64 + threadIdx.x;
+  int chunk = blockIdx.y;
+  double acc = 0.0;
+  if (col < C) {
+    bool act = active[col / 3] != 0;
+    int vend = min(N, (chunk + 1) * kChunk);
+    for (int v = chunk * kChunk + threadIdx.y; v < vend; v += 4) {
+      long long i = (long long)v * C + col;
+      double old = Xit[i];
+      double nw = act ? Xn[i] + B[i] : old;
+      double d = nw - old;
+      acc += d * d;
+      Xit[i] = nw;
+      if (tape) tape[i] = nw;
+    }
+  }
+  red[threadIdx.y][threadIdx.x] = acc;
+  __syncthreads();
+  if (threadIdx.y == 0 && col < C)
+    part[(long long)chunk * C + col] = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+}
+
+// Per rollout: err = ||x_new - x_old||_2 / (3N) (World.h:890-895); a rollout
+// whose err falls under 5e-7 stops iterating. iters counts executed iterations.
+__global__ void k_err_final(const double* __restrict__ part, int nchunk, int R, int N, int* __restrict__ active,
+                            int* __restrict__ iters, double* __restrict__ err) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= R) return;
+  int C = 3 * R;
+  if (!active[r]) return;
+  double s = 0.0;
+  for (int ch = 0; ch < nchunk; ++ch)
+    for (int c = 0; c < 3; ++c) s += part[(long long)ch * C + 3 * r + c];
+  double e = sqrt(s) / (3.0 * N);
+  err[r] = e;
+  iters[r] += 1;
+  if (e < kEps) active[r] = 0;
+}
+
+// First check of the loop (iteration 0 compares qn with zero).
+__global__ void k_err_init(const double* __restrict__ part, int nchunk, int R, int N, int* __restrict__ active,
+                           int* __restrict__ iters) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= R) return;
+  int C = 3 * R;
+  double s = 0.0;
+  for (int ch = 0; ch < nchunk; ++ch)
+    for (int c = 0; c < 3; ++c) s += part[(long long)ch * C + 3 * r + c];
+  active[r] = (sqrt(s) / (3.0 * N) < kEps) ? 0 : 1;
+  iters[r] = 0;
+}
+
+__global__ void k_sqsum_chunks(const double* __restrict__ X, double* __restrict__ part, int N, int C) {
+  __shared__ double red[4][64];
+  int col = blockIdx.x * 64 + threadIdx.x;
+  int chunk = blockIdx.y;
+  double acc = 0.0;
+  if (col < C) {
+    int vend = min(N, (chunk + 1) * kChunk);
+    for (int v = chunk * kChunk + threadIdx.y; v < vend; v += 4) {
+      double x = X[(long long)v * C + col];
+      acc += x * x;
+    }
+  }
+  red[threadIdx.y][threadIdx.x] = acc;
+  __syncthreads();
+  if (threadIdx.y == 0 && col < C)
+    part[(long long)chunk * C + col] = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+}
+
+// Constant attachment part of J d: W p.
+__global__ void k_attach_rhs(const double* __restrict__ attw, const double* __restrict__ xrest, double* __restrict__ Cc,
+                             int N, int C) {
+  long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx >= (long long)N * C) return;
+  int v = (int)(idx / C);
+  int c = (int)(idx % C) % 3;
+  Cc[idx] = attw[v] * xrest[3 * v + c];
+}
+
+// End of step: v = damping (x+ - x)/h ; x = x+ (World.h:757-766, 708).
+__global__ void k_step_finish(const double* __restrict__ Xit, const double* __restrict__ Xn, double* __restrict__ X,
+                              double* __restrict__ V, long long n, double h, double damping) {
+  long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx >= n) return;
+  double x1 = Xit[idx];
+  V[idx] = (x1 - Xn[idx]) * (1.0 / h) * damping;
+  X[idx] = x1;
+}
+
+// ------------------------------------------------------------ adjoint kernels
+// ybar = gx + (gamma/h) gv ; xbar = -(gamma/h) gv ; zsum = 0
+__global__ void k_adj_seed(const double* __restrict__ gX, const double* __restrict__ gV, double* __restrict__ Yb,
+                           double* __restrict__ Xb, double* __restrict__ Zs, long long n, double gh) {
+  long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx >= n) return;
+  Yb[idx] = gX[idx] + gh * gV[idx];
+  Xb[idx] = -gh * gV[idx];
+  Zs[idx] = 0.0;
+}
+
+__global__ void k_copy_masked(const double* __restrict__ src, double* __restrict__ dst, const int* __restrict__ active,
+                              long long n, int C) {
+  long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx >= n) return;
+  int j = (int)(idx % C);
+  dst[idx] = active[j / 3] ? src[idx] : 0.0;
+}
+
+// After z = K^-1 ybar: xbar += ybar, zsum += z for active rollouts.
+__global__ void k_adj_post_solve(const double* __restrict__ Z, const double* __restrict__ Yb, double* __restrict__ Xb,
+                                 double* __restrict__ Zs, const int* __restrict__ active, long long n, int C) {
+  long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx >= n) return;
+  int j = (int)(idx % C);
+  if (!active[j / 3]) return;
+  Xb[idx] += Yb[idx];
+  Zs[idx] += Z[idx];
+}
+
+// Local-step VJP of one tet at iterate y_k with z = K^-1 ybar_{k+1}:
+// corners get Fbar G_k; the per-(tet,rollout) E-gradient accumulates
+// (A z).(d_c(y_k) - A y_{k+1}) dw_c/dE over the two weights.
+template <typename T>
+__global__ __launch_bounds__(256) void k_vjp_tet(const double* __restrict__ Yk, const double* __restrict__ Yk1,
+                                                 const double* __restrict__ Z, const int* __restrict__ tv,
+                                                 const double* __restrict__ tG, const double* __restrict__ wr,
+                                                 const double* __restrict__ wv, const double* __restrict__ dEr,
+                                                 const double* __restrict__ dEv, const int* __restrict__ tslot,
+                                                 const int* __restrict__ active, T* __restrict__ corner,
+                                                 double* __restrict__ gEpart, int Tn, int R) {
+  long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx >= (long long)Tn * R) return;
+  int t = (int)(idx / R);
+  int r = (int)(idx % R);
+  if (!active[r]) return;
+  int C = 3 * R;
+  int v[4] = {tv[t], tv[Tn + t], tv[2 * Tn + t], tv[3 * Tn + t]};
+  T G[12];
+#pragma unroll
+  for (int q = 0; q < 12; ++q) G[q] = (T)tG[(long long)q * Tn + t];
+  double Gd[12];
+#pragma unroll
+  for (int q = 0; q < 12; ++q) Gd[q] = tG[(long long)q * Tn + t];
+  T ds[9];
+  double F1[9], Fz[9];
+  {
+    const double* b = Yk + (long long)v[0] * C + 3 * r;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int k = 1; k < 4; ++k) ds[c * 3 + (k - 1)] = (T)(Yk[(long long)v[k] * C + 3 * r + c] - b[c]);
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      double a = 0, z = 0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        a += Yk1[(long long)v[k] * C + 3 * r + c] * Gd[3 * k + j];
+        z += Z[(long long)v[k] * C + 3 * r + c] * Gd[3 * k + j];
+      }
+      F1[c * 3 + j] = a;
+      Fz[c * 3 + j] = z;
+    }
+  T F[9], Zt[9];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      F[c * 3 + j] = ds[c * 3 + 0] * G[3 + j] + ds[c * 3 + 1] * G[6 + j] + ds[c * 3 + 2] * G[9 + j];
+      Zt[c * 3 + j] = (T)Fz[c * 3 + j];
+    }
+  T Fb[9], md[9], mdv[9];
+  corotate_vjp<T>(F, (T)wr[t], (T)wv[t], Zt, Fb, md, mdv);
+  double gr = 0, gv = 0;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    gr += Fz[i] * ((double)md[i] - F1[i]);
+    gv += Fz[i] * ((double)mdv[i] - F1[i]);
+  }
+  gEpart[idx] += gr * dEr[t] + gv * dEv[t];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    T* o = corner + (long long)tslot[k * Tn + t] * C + 3 * r;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) o[c] = Fb[c * 3 + 0] * G[3 * k + 0] + Fb[c * 3 + 1] * G[3 * k + 1] + Fb[c * 3 + 2] * G[3 * k + 2];
+  }
+}
+
+// Muscle VJP: ybar += w (1-a) A^T (A z); dL/da += -w (A z).(F(y_k) f).
+template <typename T>
+__global__ void k_vjp_muscle(const double* __restrict__ Yk, const double* __restrict__ Z, const int* __restrict__ mv,
+                             const double* __restrict__ mg, const double* __restrict__ mw,
+                             const int* __restrict__ mgroup, const int* __restrict__ mslot,
+                             const double* __restrict__ act, int act_stride_r, int act_off,
+                             const int* __restrict__ active, T* __restrict__ corner, double* __restrict__ gMpart,
+                             int M, int R) {
+  long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx >= (long long)M * R) return;
+  int m = (int)(idx / R);
+  int r = (int)(idx % R);
+  if (!active[r]) {
+    gMpart[idx] = 0.0;
+    return;
+  }
+  int C = 3 * R;
+  double a = act ? act[(long long)r * act_stride_r + act_off + mgroup[m]] : 0.0;
+  double fz[3] = {0, 0, 0}, ff[3] = {0, 0, 0};
+  int v[4];
+  double g[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    v[k] = mv[k * M + m];
+    g[k] = mg[k * M + m];
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    double b = Yk[(long long)v[0] * C + 3 * r + c];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) fz[c] += Z[(long long)v[k] * C + 3 * r + c] * g[k];
+#pragma unroll
+    for (int k = 1; k < 4; ++k) ff[c] += (Yk[(long long)v[k] * C + 3 * r + c] - b) * g[k];
+  }
+  gMpart[idx] = -mw[m] * (fz[0] * ff[0] + fz[1] * ff[1] + fz[2] * ff[2]);
+  double sc = mw[m] * (1.0 - a);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    T* o = corner + (long long)mslot[k * M + m] * C + 3 * r;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) o[c] = (T)(sc * fz[c] * g[k]);
+  }
+}
+
+// gact[r, step, g] += sum over the group's muscles (fixed order).
+__global__ void k_reduce_groups(const double* __restrict__ gMpart, const int* __restrict__ gptr,
+                                const int* __restrict__ gmus, double* __restrict__ gact, int G, int R, int steps,
+                                int step) {
+  int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= G * R) return;
+  int g = idx / R, r = idx % R;
+  double s = 0.0;
+  for (int q = gptr[g]; q < gptr[g + 1]; ++q) s += gMpart[(long long)gmus[q] * R + r];
+  gact[((long long)r * steps + step) * G + g] += s;
+}
+
+// End of a step's reverse sweep:
+// gx_n = xbar + ybar_0 - L zsum ; gv_n = h ybar_0 + (M/h) zsum.
+__global__ void k_adj_step_end(const double* __restrict__ Xb, const double* __restrict__ Yb,
+                               const double* __restrict__ LZ, const double* __restrict__ Zs,
+                               const double* __restrict__ mass, double* __restrict__ gX, double* __restrict__ gV,
+                               int N, int C, double h) {
+  long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx >= (long long)N * C) return;
+  int v = (int)(idx / C);
+  gX[idx] = Xb[idx] + Yb[idx] - LZ[idx];
+  gV[idx] = h * Yb[idx] + mass[v] / h * Zs[idx];
+}
+
+__global__ void k_active_rev(const int* __restrict__ iters, int k, int R, int* __restrict__ active) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < R) active[r] = k < iters[r] ? 1 : 0;
+}
+
+__global__ void k_reduce_rows(const double* __restrict__ part, const int* __restrict__ tet_order, double* __restrict__ out,
+                              int Tn, int R) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= Tn) return;
+  double s = 0.0;
+  for (int r = 0; r < R; ++r) s += part[(long long)t * R + r];
+  out[tet_order[t]] = s;
+}
+
+// ------------------------------------------------------------------ engine
+template <typename X>
+struct DevBuf {
+  X* p = nullptr;
+  size_t n = 0;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  int alloc(size_t count) {
+    if (count <= n && p) return 0;
+    release();
+    if (count == 0) return 0;
+    hipError_t e = hipMalloc((void**)&p, count * sizeof(X));
+    if (e != hipSuccess) {
+      g_err = std::string("hipMalloc failed: ") + hipGetErrorString(e);
+      p = nullptr;
+      return 1;
+    }
+    n = count;
+    return 0;
+  }
+  int upload(const X* h, size_t count) {
+    if (alloc(count)) return 1;
+    if (count) {
+      hipError_t e = hipMemcpy(p, h, count * sizeof(X), hipMemcpyHostToDevice);
+      if (e != hipSuccess) {
+        g_err = std::string("hipMemcpy failed: ") + hipGetErrorString(e);
+        return 1;
+      }
+    }
+    return 0;
+  }
+};
+
+template <typename X>
+std::vector<X> soa(const std::vector<X>& aos, int n, int k) {
+  std::vector<X> o((size_t)n * k);
+  for (int i = 0; i < n; ++i)
+    for (int q = 0; q < k; ++q) o[(size_t)q * n + i] = aos[(size_t)i * k + q];
+  return o;
+}
+
+}  // namespace
+
+struct pdm_scene_s {
+  HostScene S;
+  Csr Ls;
+  Factor F;
+  Corners Cn;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  int R = 1, C = 3, precision = PDM_FP64, flags = PDM_HOST_EARLY_EXIT, ckpt = 20;
+  // static device data
+  DevBuf<int> perm, tv, tslot, vptr, mv, mslot, mgroup, gptr, gmus, tet_order;
+  DevBuf<double> mass, attw, xrest, tG, wr, wv, dEr, dEv, mg, mw;
+  DevBuf<int> Lptr, Lcol;
+  DevBuf<double> Lval;
+  // factor
+  DevBuf<int> f_first, f_rows, f_child_ptr, f_child, f_rel, f_lev;
+  DevBuf<long long> f_rowptr, f_lptr, f_relptr, f_uptr;
+  DevBuf<double> f_L;
+  std::vector<int> lev_ptr, lev_maxm;
+  // dynamic
+  DevBuf<double> X, V, Xn, Vs, Cc, Xit, B, Lx, U, part, err;
+  DevBuf<char> corner;
+  DevBuf<int> active, iters_dev;
+  std::vector<int> iters_host;  // R*steps of the last forward
+  int last_steps = 0;
+  // tape
+  DevBuf<double> ckX, ckV;      // checkpoints
+  DevBuf<double> tY, tXn, tVs;  // segment tape
+  DevBuf<int> tIters;           // per step of segment: R
+  DevBuf<double> act_dev;       // activations of the recorded forward
+  bool have_tape = false;
+  int tape_steps = 0;
+  bool act_null = true;
+  // adjoint buffers
+  DevBuf<double> gX, gV, Yb, Xb, Zs, Zb, LZ, gEpart, gMpart, gact;
+  DevBuf<int> ractive;
+  double t_fwd = 0, t_bwd = 0, t_factor = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+  FactorView view() const {
+    return FactorView{f_first.p, f_rowptr.p, f_rows.p, f_lptr.p, f_L.p, f_child_ptr.p, f_child.p, f_relptr.p, f_rel.p,
+                      f_uptr.p};
+  }
+  size_t NC() const { return (size_t)S.N * C; }
+};
+
+namespace {
+
+int upload_static(pdm_scene s) {
+  HostScene& S = s->S;
+  if (s->perm.upload(S.perm.data(), S.N)) return 1;
+  if (s->tet_order.upload(S.tet_order.data(), S.T)) return 1;
+  if (s->tv.upload(soa(S.tv, S.T, 4).data(), 4 * (size_t)S.T)) return 1;
+  if (s->tG.upload(soa(S.tG, S.T, 12).data(), 12 * (size_t)S.T)) return 1;
+  if (s->tslot.upload(soa(s->Cn.tet_slot, S.T, 4).data(), 4 * (size_t)S.T)) return 1;
+  if (s->vptr.upload(s->Cn.vptr.data(), S.N + 1)) return 1;
+  if (s->mass.upload(S.mass.data(), S.N)) return 1;
+  std::vector<double> aw(S.N, 0.0);
+  for (size_t k = 0; k < S.att_idx.size(); ++k) aw[S.att_idx[k]] += S.att_w[k];
+  if (s->attw.upload(aw.data(), S.N)) return 1;
+  if (s->xrest.upload(S.xrest.data(), 3 * (size_t)S.N)) return 1;
+  if (S.M > 0) {
+    if (s->mv.upload(soa(S.mv, S.M, 4).data(), 4 * (size_t)S.M)) return 1;
+    if (s->mg.upload(soa(S.mg, S.M, 4).data(), 4 * (size_t)S.M)) return 1;
+    if (s->mw.upload(S.mw.data(), S.M)) return 1;
+    if (s->mgroup.upload(S.mgroup.data(), S.M)) return 1;
+    if (s->mslot.upload(soa(s->Cn.mus_slot, S.M, 4).data(), 4 * (size_t)S.M)) return 1;
+    std::vector<int> gp(S.n_groups + 1, 0), gm(S.M);
+    for (int m = 0; m < S.M; ++m) gp[S.mgroup[m] + 1]++;
+    for (int g = 0; g < S.n_groups; ++g) gp[g + 1] += gp[g];
+    std::vector<int> fill(gp.begin(), gp.end() - 1);
+    for (int m = 0; m < S.M; ++m) gm[fill[S.mgroup[m]]++] = m;
+    if (s->gptr.upload(gp.data(), gp.size())) return 1;
+    if (s->gmus.upload(gm.data(), gm.size())) return 1;
+  }
+  return 0;
+}
+
+int upload_material(pdm_scene s) {
+  HostScene& S = s->S;
+  if (s->wr.upload(S.wr.data(), S.T)) return 1;
+  if (s->wv.upload(S.wv.data(), S.T)) return 1;
+  std::vector<double> der(S.T), dev(S.T);
+  for (int t = 0; t < S.T; ++t) {
+    der[t] = S.tvol[t] / (1.0 + S.nu_fwd);
+    dev[t] = S.tvol[t] * S.nu_ctor[t] / (1.0 + S.nu_fwd);
+  }
+  if (s->dEr.upload(der.data(), S.T)) return 1;
+  if (s->dEv.upload(dev.data(), S.T)) return 1;
+  if (s->Lptr.upload(s->Ls.ptr.data(), s->Ls.ptr.size())) return 1;
+  if (s->Lcol.upload(s->Ls.col.data(), s->Ls.col.size())) return 1;
+  if (s->Lval.upload(s->Ls.val.data(), s->Ls.val.size())) return 1;
+  return 0;
+}
+
+int upload_factor(pdm_scene s) {
+  Factor& F = s->F;
+  if (s->f_first.upload(F.first.data(), F.first.size())) return 1;
+  if (s->f_rows.upload(F.rows.data(), F.rows.size())) return 1;
+  if (s->f_child_ptr.upload(F.child_ptr.data(), F.child_ptr.size())) return 1;
+  if (s->f_child.upload(F.child.data(), std::max<size_t>(1, F.child.size()))) return 1;
+  if (s->f_rel.upload(F.rel.data(), std::max<size_t>(1, F.rel.size()))) return 1;
+  if (s->f_lev.upload(F.level_sn.data(), F.level_sn.size())) return 1;
+  std::vector<long long> tmp(F.rowptr.begin(), F.rowptr.end());
+  if (s->f_rowptr.upload(tmp.data(), tmp.size())) return 1;
+  tmp.assign(F.lptr.begin(), F.lptr.end());
+  if (s->f_lptr.upload(tmp.data(), tmp.size())) return 1;
+  tmp.assign(F.relptr.begin(), F.relptr.end());
+  if (s->f_relptr.upload(tmp.data(), tmp.size())) return 1;
+  tmp.assign(F.uptr.begin(), F.uptr.end());
+  if (s->f_uptr.upload(tmp.data(), tmp.size())) return 1;
+  if (s->f_L.upload(F.L.data(), F.L.size())) return 1;
+  int nlev = (int)F.level_ptr.size() - 1;
+  s->lev_ptr = F.level_ptr;
+  s->lev_maxm.assign(nlev, 0);
+  for (int l = 0; l < nlev; ++l)
+    for (int q = F.level_ptr[l]; q < F.level_ptr[l + 1]; ++q) {
+      int sn = F.level_sn[q];
+      s->lev_maxm[l] = std::max(s->lev_maxm[l], (int)(F.rowptr[sn + 1] - F.rowptr[sn]));
+    }
+  return 0;
+}
+
+int host_factor(pdm_scene s);
+
+// Re-factorisation after a material change, then upload.
+int refactor(pdm_scene s) {
+  auto t0 = std::chrono::steady_clock::now();
+  if (host_factor(s)) return 1;
+  if (upload_material(s) || upload_factor(s)) return 1;
+  s->t_factor = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return 0;
+}
+
+
+// Scene construction shared by pdm_scene_create and the host-only diagnostics.
+int host_setup(const pdm_scene_desc* d, pdm_scene s) {
+  HostScene& S = s->S;
+  std::string err;
+  if (!build_scene(S, d->n_vertices, d->x_rest, d->n_tets, d->tets, d->mass_same, d->total_mass, err)) {
+    g_err = err;
+    return 1;
+  }
+  for (int t = 0; t < S.T; ++t) {
+    int ct = S.tet_order[t];
+    S.E[t] = d->E ? d->E[ct] : 2e5;
+    S.nu_ctor[t] = d->nu_ctor ? d->nu_ctor[ct] : 0.4;
+  }
+  S.nu_fwd = d->nu_fwd;
+  for (int k = 0; k < d->n_attach; ++k) {
+    int a = d->attach_idx[k];
+    if (a < 0 || a >= S.N) {
+      g_err = "attachment index out of range";
+      return 1;
+    }
+    S.att_idx.push_back(S.iperm[a]);
+    S.att_w.push_back(d->attach_w);
+  }
+  if (d->n_segments > 0 &&
+      !add_muscles(S, d->x_rest, d->tets, d->n_segments, d->seg_start, d->seg_end, d->seg_group, d->n_groups,
+                   d->k_muscle, err)) {
+    g_err = err;
+    return 1;
+  }
+  S.n_groups = std::max(S.n_groups, d->n_groups);
+  S.h = d->h;
+  S.max_iter = d->max_iter;
+  S.damping = d->damping;
+  for (int r = 0; r < 3; ++r) S.g[r] = d->gravity[r];
+  S.gravity_on = d->gravity_on;
+  S.contact = d->contact;
+  S.max_iter1 = d->max_iter1;
+  S.alpha = d->alpha;
+  S.mu = d->mu;
+  S.planes.assign(d->planes, d->planes + 6 * (size_t)d->n_planes);
+  S.spheres.assign(d->spheres, d->spheres + 4 * (size_t)d->n_spheres);
+  if (S.contact && (d->n_planes + d->n_spheres) > 0) {
+    g_err = "contact with obstacles is not available in this build of the engine";
+    return 1;
+  }
+  if (!(S.h > 0) || S.max_iter < 0) {
+    g_err = "invalid stepping parameters";
+    return 1;
+  }
+  update_weights(S);
+  assemble_L(S, s->Ls);
+  if (!symbolic(s->Ls, s->F, err)) {
+    g_err = "symbolic analysis failed: " + err;
+    return 1;
+  }
+  build_corners(S, s->Cn);
+  return 0;
+}
+
+// Numeric factorisation with the regularisation ladder of
+// World::FactorizeLDLT (World.h:1212-1232): reg = 1e-5, 1e-4, ... on failure.
+int host_factor(pdm_scene s) {
+  HostScene& S = s->S;
+  update_weights(S);
+  assemble_L(S, s->Ls);
+  std::vector<double> dadd(S.N);
+  for (int i = 0; i < S.N; ++i) dadd[i] = S.mass[i] / (S.h * S.h);
+  std::string err;
+  bool ok = numeric(s->Ls, dadd, s->F, err);
+  double reg = 1e-6;
+  s->F.reg_used = 0;
+  int tries = 0;
+  while (!ok && tries < 40) {
+    reg *= 10;
+    for (int i = 0; i < S.N; ++i) dadd[i] = S.mass[i] / (S.h * S.h) + reg;
+    ok = numeric(s->Ls, dadd, s->F, err);
+    s->F.reg_used = reg;
+    ++tries;
+  }
+  if (!ok) {
+    g_err = "factorisation failed: " + err;
+    return 1;
+  }
+  return 0;
+}
+
+int ensure_batch_buffers(pdm_scene s) {
+  size_t nc = s->NC();
+  int N = s->S.N;
+  if (s->X.alloc(nc) || s->V.alloc(nc) || s->Xn.alloc(nc) || s->Vs.alloc(nc) || s->Cc.alloc(nc) || s->Xit.alloc(nc) ||
+      s->B.alloc(nc) || s->Lx.alloc(nc))
+    return 1;
+  if (s->U.alloc(std::max<long long>(1, s->F.uptr.back()) * (size_t)s->C)) return 1;
+  size_t esz = s->precision == PDM_MIXED ? sizeof(float) : sizeof(double);
+  if (s->corner.alloc((size_t)s->Cn.nslot * s->C * esz)) return 1;
+  if (s->part.alloc((size_t)cdiv(N, kChunk) * s->C)) return 1;
+  if (s->err.alloc(s->R) || s->active.alloc(s->R) || s->iters_dev.alloc(s->R)) return 1;
+  return 0;
+}
+
+int launch_solve(pdm_scene s, double* Bp) {
+  FactorView fv = s->view();
+  int nlev = (int)s->lev_ptr.size() - 1;
+  const size_t kLds = 64 * 1024;
+  for (int l = 0; l < nlev; ++l) {
+    int cnt = s->lev_ptr[l + 1] - s->lev_ptr[l];
+    int m = std::max(1, s->lev_maxm[l]);
+    int Ct = (int)std::min<size_t>((size_t)s->C, std::max<size_t>(1, kLds / (8 * (size_t)m)));
+    if ((size_t)m * Ct * 8 > 160 * 1024) {
+      g_err = "supernode front exceeds LDS";
+      return 1;
+    }
+    int ntile = cdiv(s->C, Ct);
+    hipLaunchKernelGGL(k_fwd_level, dim3(cnt * ntile), dim3(kThreads), (size_t)m * Ct * 8, s->stream, fv,
+                       s->f_lev.p + s->lev_ptr[l], ntile, Ct, s->C, Bp, s->U.p);
+  }
+  for (int l = nlev - 1; l >= 0; --l) {
+    int cnt = s->lev_ptr[l + 1] - s->lev_ptr[l];
+    int m = std::max(1, s->lev_maxm[l]);
+    int Ct = (int)std::min<size_t>((size_t)s->C, std::max<size_t>(1, kLds / (8 * (size_t)m)));
+    int ntile = cdiv(s->C, Ct);
+    hipLaunchKernelGGL(k_bwd_level, dim3(cnt * ntile), dim3(kThreads), (size_t)m * Ct * 8, s->stream, fv,
+                       s->f_lev.p + s->lev_ptr[l], ntile, Ct, s->C, Bp);
+  }
+  HIPX(hipGetLastError());
+  return 0;
+}
+
+template <typename T>
+void launch_local(pdm_scene s, const double* Xsrc, const double* act, int act_steps, int step) {
+  HostScene& S = s->S;
+  T* corner = (T*)s->corner.p;
+  long long nt = (long long)S.T * s->R;
+  hipLaunchKernelGGL(k_local_tet<T>, dim3(cdiv(nt, kThreads)), dim3(kThreads), 0, s->stream, Xsrc, s->tv.p, s->tG.p,
+                     s->wr.p, s->wv.p, s->tslot.p, s->active.p, corner, S.T, s->R);
+  if (S.M > 0) {
+    long long nm = (long long)S.M * s->R;
+    hipLaunchKernelGGL(k_local_muscle<T>, dim3(cdiv(nm, kThreads)), dim3(kThreads), 0, s->stream, Xsrc, s->mv.p,
+                       s->mg.p, s->mw.p, s->mgroup.p, s->mslot.p, act, act_steps * S.n_groups, step * S.n_groups,
+                       s->active.p, corner, S.M, s->R);
+  }
+  hipLaunchKernelGGL(k_gather<T>, dim3(cdiv(s->NC(), kThreads)), dim3(kThreads), 0, s->stream, s->Cc.p, corner,
+                     s->vptr.p, s->B.p, S.N, s->C);
+}
+
+// One time step on device. tapeY: optional (max_iter+1) x NC buffer for the
+// iterates; tIters: optional R ints receiving the executed iterations.
+int run_step(pdm_scene s, const double* act, int act_steps, int step, double* tapeY, double* tapeXn, double* tapeVs,
+             int* tIters) {
+  HostScene& S = s->S;
+  int N = S.N, C = s->C;
+  size_t nc = s->NC();
+  int nchunk = cdiv(N, kChunk);
+  dim3 ugrid(cdiv(C, 64), nchunk), ublock(64, 4);
+  hipLaunchKernelGGL(k_spmv, dim3(cdiv(nc, kThreads)), dim3(kThreads), 0, s->stream, s->Lptr.p, s->Lcol.p, s->Lval.p,
+                     s->X.p, s->Lx.p, N, C);
+  hipLaunchKernelGGL(k_step_prep, dim3(cdiv(nc, kThreads)), dim3(kThreads), 0, s->stream, s->X.p, s->V.p, s->Lx.p,
+                     s->mass.p, s->attw.p, s->xrest.p, s->Xn.p, s->Vs.p, s->Cc.p, s->Xit.p, N, C, S.h, S.g[0], S.g[1],
+                     S.g[2], S.gravity_on);
+  if (tapeXn) HIPX(hipMemcpyAsync(tapeXn, s->Xn.p, nc * 8, hipMemcpyDeviceToDevice, s->stream));
+  if (tapeVs) HIPX(hipMemcpyAsync(tapeVs, s->Vs.p, nc * 8, hipMemcpyDeviceToDevice, s->stream));
+  if (tapeY) HIPX(hipMemcpyAsync(tapeY, s->Xit.p, nc * 8, hipMemcpyDeviceToDevice, s->stream));
+  hipLaunchKernelGGL(k_sqsum_chunks, ugrid, ublock, 0, s->stream, s->Xit.p, s->part.p, N, C);
+  hipLaunchKernelGGL(k_err_init, dim3(cdiv(s->R, 64)), dim3(64), 0, s->stream, s->part.p, nchunk, s->R, N,
+                     s->active.p, s->iters_dev.p);
+  bool host_exit = (s->flags & PDM_HOST_EARLY_EXIT) != 0;
+  std::vector<int> act_h(s->R);
+  for (int it = 0; it < S.max_iter; ++it) {
+    if (host_exit) {
+      HIPX(hipMemcpyAsync(act_h.data(), s->active.p, s->R * sizeof(int), hipMemcpyDeviceToHost, s->stream));
+      HIPX(hipStreamSynchronize(s->stream));
+      bool any = false;
+      for (int r = 0; r < s->R; ++r) any |= act_h[r] != 0;
+      if (!any) break;
+    }
+    if (s->precision == PDM_MIXED)
+      launch_local<float>(s, s->Xit.p, act, act_steps, step);
+    else
+      launch_local<double>(s, s->Xit.p, act, act_steps, step);
+    if (launch_solve(s, s->B.p)) return 1;
+    hipLaunchKernelGGL(k_update, ugrid, ublock, 0, s->stream, s->Xn.p, s->B.p, s->Xit.p, s->active.p, s->part.p,
+                       tapeY ? tapeY + (size_t)(it + 1) * nc : nullptr, N, C);
+    hipLaunchKernelGGL(k_err_final, dim3(cdiv(s->R, 64)), dim3(64), 0, s->stream, s->part.p, nchunk, s->R, N,
+                       s->active.p, s->iters_dev.p, s->err.p);
+  }
+  if (tapeY && host_exit) {
+    // iterations skipped by the host break keep the last iterate
+    // (the reverse sweep never reads them: it is bounded by tIters)
+  }
+  hipLaunchKernelGGL(k_step_finish, dim3(cdiv(nc, kThreads)), dim3(kThreads), 0, s->stream, s->Xit.p, s->Xn.p, s->X.p,
+                     s->V.p, (long long)nc, S.h, S.damping);
+  if (tIters) HIPX(hipMemcpyAsync(tIters, s->iters_dev.p, s->R * sizeof(int), hipMemcpyDeviceToDevice, s->stream));
+  HIPX(hipGetLastError());
+  return 0;
+}
+
+int forward_impl(pdm_scene s, int steps, const double* x0, const double* v0, const double* act, double* xT, double* vT,
+                 int record) {
+  if (steps < 0) {
+    g_err = "steps must be >= 0";
+    return 1;
+  }
+  if (ensure_batch_buffers(s)) return 1;
+  HostScene& S = s->S;
+  size_t nc = s->NC();
+  int N = S.N;
+  HIPX(hipEventRecord(s->ev0, s->stream));
+  hipLaunchKernelGGL(k_caller_to_engine, dim3(cdiv(nc, kThreads)), dim3(kThreads), 0, s->stream, x0, s->X.p, s->perm.p,
+                     N, s->R);
+  hipLaunchKernelGGL(k_caller_to_engine, dim3(cdiv(nc, kThreads)), dim3(kThreads), 0, s->stream, v0, s->V.p, s->perm.p,
+                     N, s->R);
+  s->have_tape = false;
+  int nck = record ? cdiv(std::max(steps, 1), s->ckpt) : 0;
+  if (record) {
+    if (s->ckX.alloc((size_t)nck * nc) || s->ckV.alloc((size_t)nck * nc)) return 1;
+    size_t na = act ? (size_t)s->R * steps * S.n_groups : 0;
+    if (na) {
+      if (s->act_dev.alloc(na)) return 1;
+      HIPX(hipMemcpyAsync(s->act_dev.p, act, na * 8, hipMemcpyDeviceToDevice, s->stream));
+    }
+    s->act_null = act == nullptr;
+  }
+  if (s->iters_host.size() < (size_t)s->R * steps) s->iters_host.resize((size_t)s->R * steps);
+  DevBuf<int> it_all;
+  if (it_all.alloc((size_t)s->R * std::max(steps, 1))) return 1;
+  for (int n = 0; n < steps; ++n) {
+    if (record && n % s->ckpt == 0) {
+      size_t k = n / s->ckpt;
+      HIPX(hipMemcpyAsync(s->ckX.p + k * nc, s->X.p, nc * 8, hipMemcpyDeviceToDevice, s->stream));
+      HIPX(hipMemcpyAsync(s->ckV.p + k * nc, s->V.p, nc * 8, hipMemcpyDeviceToDevice, s->stream));
+    }
+    if (run_step(s, act, steps, n, nullptr, nullptr, nullptr, it_all.p + (size_t)n * s->R)) return 1;
+  }
+  hipLaunchKernelGGL(k_engine_to_caller, dim3(cdiv(nc, kThreads)), dim3(kThreads), 0, s->stream, s->X.p, xT, s->perm.p,
+                     N, s->R);
+  hipLaunchKernelGGL(k_engine_to_caller, dim3(cdiv(nc, kThreads)), dim3(kThreads), 0, s->stream, s->V.p, vT, s->perm.p,
+                     N, s->R);
+  HIPX(hipEventRecord(s->ev1, s->stream));
+  std::vector<int> ih((size_t)s->R * steps);
+  if (steps) HIPX(hipMemcpyAsync(ih.data(), it_all.p, ih.size() * sizeof(int), hipMemcpyDeviceToHost, s->stream));
+  HIPX(hipStreamSynchronize(s->stream));
+  HIPX(hipGetLastError());
+  float ms = 0;
+  HIPX(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+  s->t_fwd = ms;
+  // iters_host layout: [r][step]
+  for (int n = 0; n < steps; ++n)
+    for (int r = 0; r < s->R; ++r) s->iters_host[(size_t)r * steps + n] = ih[(size_t)n * s->R + r];
+  s->last_steps = steps;
+  if (record) {
+    s->have_tape = true;
+    s->tape_steps = steps;
+  }
+  return 0;
+}
+
+template <typename T>
+int vjp_iteration(pdm_scene s, const double* Yk, const double* Yk1, const double* act, int act_steps, int step) {
+  HostScene& S = s->S;
+  T* corner = (T*)s->corner.p;
+  long long nt = (long long)S.T * s->R;
+  hipLaunchKernelGGL(k_vjp_tet<T>, dim3(cdiv(nt, kThreads)), dim3(kThreads), 0, s->stream, Yk, Yk1, s->Zb.p, s->tv.p,
+                     s->tG.p, s->wr.p, s->wv.p, s->dEr.p, s->dEv.p, s->tslot.p, s->ractive.p, corner, s->gEpart.p, S.T,
+                     s->R);
+  if (S.M > 0) {
+    long long nm = (long long)S.M * s->R;
+    hipLaunchKernelGGL(k_vjp_muscle<T>, dim3(cdiv(nm, kThreads)), dim3(kThreads), 0, s->stream, Yk, s->Zb.p, s->mv.p,
+                       s->mg.p, s->mw.p, s->mgroup.p, s->mslot.p, act, act_steps * S.n_groups, step * S.n_groups,
+                       s->ractive.p, corner, s->gMpart.p, S.M, s->R);
+  }
+  hipLaunchKernelGGL(k_gather_masked<T>, dim3(cdiv(s->NC(), kThreads)), dim3(kThreads), 0, s->stream, corner,
+                     s->vptr.p, s->ractive.p, s->Yb.p, S.N, s->C);
+  return 0;
+}
+
+int backward_impl(pdm_scene s, const double* gxT, const double* gvT, double* gx0, double* gv0, double* gE, double* gnu,
+                  double* gact) {
+  if (!s->have_tape) {
+    g_err = "backward needs a recorded forward (record=1)";
+    return 1;
+  }
+  HostScene& S = s->S;
+  int N = S.N, C = s->C, R = s->R, steps = s->tape_steps;
+  size_t nc = s->NC();
+  int K1 = S.max_iter + 1;
+  int seg = std::min(s->ckpt, std::max(steps, 1));
+  if (s->tY.alloc((size_t)seg * K1 * nc) || s->tXn.alloc((size_t)seg * nc) || s->tVs.alloc((size_t)seg * nc) ||
+      s->tIters.alloc((size_t)seg * R))
+    return 1;
+  if (s->gX.alloc(nc) || s->gV.alloc(nc) || s->Yb.alloc(nc) || s->Xb.alloc(nc) || s->Zs.alloc(nc) ||
+      s->Zb.alloc(nc) || s->LZ.alloc(nc) || s->gEpart.alloc((size_t)S.T * R) || s->ractive.alloc(R))
+    return 1;
+  if (S.M > 0 && s->gMpart.alloc((size_t)S.M * R)) return 1;
+  size_t nact = (size_t)R * steps * std::max(S.n_groups, 1);
+  if (s->gact.alloc(nact)) return 1;
+  HIPX(hipEventRecord(s->ev0, s->stream));
+  HIPX(hipMemsetAsync(s->gEpart.p, 0, (size_t)S.T * R * 8, s->stream));
+  HIPX(hipMemsetAsync(s->gact.p, 0, nact * 8, s->stream));
+  hipLaunchKernelGGL(k_caller_to_engine, dim3(cdiv(nc, kThreads)), dim3(kThreads), 0, s->stream, gxT, s->gX.p,
+                     s->perm.p, N, R);
+  hipLaunchKernelGGL(k_caller_to_engine, dim3(cdiv(nc, kThreads)), dim3(kThreads), 0, s->stream, gvT, s->gV.p,
+                     s->perm.p, N, R);
+  const double* act = s->act_null ? nullptr : s->act_dev.p;
+  int nseg = cdiv(std::max(steps, 1), s->ckpt);
+  std::vector<int> it_h((size_t)seg * R);
+  double gh = S.damping / S.h;
+  for (int sg = nseg - 1; sg >= 0 && steps > 0; --sg) {
+    int s0 = sg * s->ckpt, s1 = std::min(steps, s0 + s->ckpt);
+    // recompute the segment from its checkpoint, recording every iterate
+    HIPX(hipMemcpyAsync(s->X.p, s->ckX.p + (size_t)sg * nc, nc * 8, hipMemcpyDeviceToDevice, s->stream));
+    HIPX(hipMemcpyAsync(s->V.p, s->ckV.p + (size_t)sg * nc, nc * 8, hipMemcpyDeviceToDevice, s->stream));
+    for (int n = s0; n < s1; ++n) {
+      int q = n - s0;
+      if (run_step(s, act, steps, n, s->tY.p + (size_t)q * K1 * nc, s->tXn.p + (size_t)q * nc,
+                   s->tVs.p + (size_t)q * nc, s->tIters.p + (size_t)q * R))
+        return 1;
+    }
+    HIPX(hipMemcpyAsync(it_h.data(), s->tIters.p, (size_t)(s1 - s0) * R * sizeof(int), hipMemcpyDeviceToHost,
+                        s->stream));
+    HIPX(hipStreamSynchronize(s->stream));
+    for (int n = s1 - 1; n >= s0; --n) {
+      int q = n - s0;
+      int kmax = 0;
+      for (int r = 0; r < R; ++r) kmax = std::max(kmax, it_h[(size_t)q * R + r]);
+      hipLaunchKernelGGL(k_adj_seed, dim3(cdiv(nc, kThreads)), dim3(kThreads), 0, s->stream, s->gX.p, s->gV.p, s->Yb.p,
+                         s->Xb.p, s->Zs.p, (long long)nc, gh);
+      const double* Y = s->tY.p + (size_t)q * K1 * nc;
+      for (int k = kmax - 1; k >= 0; --k) {
+        hipLaunchKernelGGL(k_active_rev, dim3(cdiv(R, 64)), dim3(64), 0, s->stream, s->tIters.p + (size_t)q * R, k, R,
+                           s->ractive.p);
+        hipLaunchKernelGGL(k_copy_masked, dim3(cdiv(nc, kThreads)), dim3(kThreads), 0, s->stream, s->Yb.p, s->Zb.p,
+                           s->ractive.p, (long long)nc, C);
+        if (launch_solve(s, s->Zb.p)) return 1;
+        hipLaunchKernelGGL(k_adj_post_solve, dim3(cdiv(nc, kThreads)), dim3(kThreads), 0, s->stream, s->Zb.p,
+                           s->Yb.p, s->Xb.p, s->Zs.p, s->ractive.p, (long long)nc, C);
+        int rc = s->precision == PDM_MIXED
+                     ? vjp_iteration<float>(s, Y + (size_t)k * nc, Y + (size_t)(k + 1) * nc, act, steps, n)
+                     : vjp_iteration<double>(s, Y + (size_t)k * nc, Y + (size_t)(k + 1) * nc, act, steps, n);
+        if (rc) return 1;
+        if (S.M > 0)
+          hipLaunchKernelGGL(k_reduce_groups, dim3(cdiv(S.n_groups * R, 64)), dim3(64), 0, s->stream, s->gMpart.p,
+                             s->gptr.p, s->gmus.p, s->gact.p, S.n_groups, R, steps, n);
+      }
+      hipLaunchKernelGGL(k_spmv, dim3(cdiv(nc, kThreads)), dim3(kThreads), 0, s->stream, s->Lptr.p, s->Lcol.p,
+                         s->Lval.p, s->Zs.p, s->LZ.p, N, C);
+      hipLaunchKernelGGL(k_adj_step_end, dim3(cdiv(nc, kThreads)), dim3(kThreads), 0, s->stream, s->Xb.p, s->Yb.p,
+                         s->LZ.p, s->Zs.p, s->mass.p, s->gX.p, s->gV.p, N, C, S.h);
+    }
+  }
+  if (gx0)
+    hipLaunchKernelGGL(k_engine_to_caller, dim3(cdiv(nc, kThreads)), dim3(kThreads), 0, s->stream, s->gX.p, gx0,
+                       s->perm.p, N, R);
+  if (gv0)
+    hipLaunchKernelGGL(k_engine_to_caller, dim3(cdiv(nc, kThreads)), dim3(kThreads), 0, s->stream, s->gV.p, gv0,
+                       s->perm.p, N, R);
+  DevBuf<double> gEd;
+  if (gEd.alloc(S.T)) return 1;
+  hipLaunchKernelGGL(k_reduce_rows, dim3(cdiv(S.T, kThreads)), dim3(kThreads), 0, s->stream, s->gEpart.p,
+                     s->tet_order.p, gEd.p, S.T, R);
+  if (gE) HIPX(hipMemcpyAsync(gE, gEd.p, (size_t)S.T * 8, hipMemcpyDeviceToDevice, s->stream));
+  if (gact && S.n_groups > 0)
+    HIPX(hipMemcpyAsync(gact, s->gact.p, (size_t)R * steps * S.n_groups * 8, hipMemcpyDeviceToDevice, s->stream));
+  HIPX(hipEventRecord(s->ev1, s->stream));
+  std::vector<double> gEh(S.T);
+  HIPX(hipMemcpyAsync(gEh.data(), gEd.p, (size_t)S.T * 8, hipMemcpyDeviceToHost, s->stream));
+  HIPX(hipStreamSynchronize(s->stream));
+  HIPX(hipGetLastError());
+  if (gnu) {
+    // mMu = E/(1+nu): dL/dnu = -sum_t E_t dL/dE_t / (1+nu)
+    double acc = 0;
+    for (int t = 0; t < S.T; ++t) acc += S.E[t] * gEh[S.tet_order[t]];
+    double val = -acc / (1.0 + S.nu_fwd);
+    HIPX(hipMemcpy(gnu, &val, 8, hipMemcpyHostToDevice));
+  }
+  float ms = 0;
+  HIPX(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+  s->t_bwd = ms;
+  return 0;
+}
+
+}  // namespace
+
+// ===================================================================== C ABI
+extern "C" {
+
+const char* pdm_last_error(void) { return g_err.c_str(); }
+
+int pdm_device_count(int* n) {
+  int c = 0;
+  hipError_t e = hipGetDeviceCount(&c);
+  if (e != hipSuccess) c = 0;
+  *n = c;
+  return 0;
+}
+
+int pdm_scene_create(const pdm_scene_desc* d, pdm_scene* out) {
+  if (!d || !out) {
+    g_err = "null argument";
+    return 1;
+  }
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    g_err = "no HIP device available";
+    return 1;
+  }
+  std::unique_ptr<pdm_scene_s> s(new pdm_scene_s());
+  if (host_setup(d, s.get())) return 1;
+  HIPX(hipStreamCreate(&s->stream));
+  s->own_stream = true;
+  HIPX(hipEventCreate(&s->ev0));
+  HIPX(hipEventCreate(&s->ev1));
+  if (upload_static(s.get())) return 1;
+  if (refactor(s.get())) return 1;
+  *out = s.release();
+  return 0;
+}
+
+int pdm_host_analyse(const pdm_scene_desc* d, pdm_info* info) {
+  if (!d || !info) {
+    g_err = "null argument";
+    return 1;
+  }
+  std::unique_ptr<pdm_scene_s> s(new pdm_scene_s());
+  if (host_setup(d, s.get())) return 1;
+  if (host_factor(s.get())) return 1;
+  s->R = 1;
+  return pdm_scene_info(s.get(), info);
+}
+
+int pdm_host_solve(const pdm_scene_desc* d, int R, const double* b, double* x) {
+  if (!d || !b || !x || R < 1) {
+    g_err = "invalid argument";
+    return 1;
+  }
+  std::unique_ptr<pdm_scene_s> s(new pdm_scene_s());
+  if (host_setup(d, s.get())) return 1;
+  if (host_factor(s.get())) return 1;
+  const Factor& F = s->F;
+  int N = s->S.N;
+  // supernodal forward / backward substitution on the host, same schedule
+  // as the device kernels (update blocks per supernode, children in order)
+  for (int r = 0; r < R; ++r) {
+    std::vector<double> y(3 * (size_t)N);
+    for (int v = 0; v < N; ++v)
+      for (int c = 0; c < 3; ++c) y[3 * v + c] = b[(size_t)r * 3 * N + 3 * s->S.perm[v] + c];
+    std::vector<std::vector<double>> U(F.ns);
+    for (int sn = 0; sn < F.ns; ++sn) {
+      int c0 = F.first[sn], w = F.first[sn + 1] - c0;
+      int m = (int)(F.rowptr[sn + 1] - F.rowptr[sn]);
+      const double* Ls = &F.L[F.lptr[sn]];
+      std::vector<double> Tm((size_t)m * 3, 0.0);
+      for (int i = 0; i < w; ++i)
+        for (int c = 0; c < 3; ++c) Tm[3 * i + c] = y[3 * (c0 + i) + c];
+      for (int q = F.child_ptr[sn]; q < F.child_ptr[sn + 1]; ++q) {
+        int ch = F.child[q];
+        int mbc = (int)(F.rowptr[ch + 1] - F.rowptr[ch]) - (F.first[ch + 1] - F.first[ch]);
+        const int* rel = &F.rel[F.relptr[ch]];
+        for (int a = 0; a < mbc; ++a)
+          for (int c = 0; c < 3; ++c) Tm[3 * rel[a] + c] += U[ch][3 * a + c];
+        std::vector<double>().swap(U[ch]);
+      }
+      for (int k = 0; k < w; ++k)
+        for (int c = 0; c < 3; ++c) {
+          double yk = Tm[3 * k + c] / Ls[(size_t)k * m + k];
+          Tm[3 * k + c] = yk;
+          for (int i = k + 1; i < m; ++i) Tm[3 * i + c] -= Ls[(size_t)k * m + i] * yk;
+        }
+      for (int i = 0; i < w; ++i)
+        for (int c = 0; c < 3; ++c) y[3 * (c0 + i) + c] = Tm[3 * i + c];
+      U[sn].assign(Tm.begin() + 3 * (size_t)w, Tm.end());
+    }
+    for (int sn = F.ns - 1; sn >= 0; --sn) {
+      int c0 = F.first[sn], w = F.first[sn + 1] - c0;
+      int m = (int)(F.rowptr[sn + 1] - F.rowptr[sn]);
+      const int* rows = &F.rows[F.rowptr[sn]];
+      const double* Ls = &F.L[F.lptr[sn]];
+      for (int k = w - 1; k >= 0; --k)
+        for (int c = 0; c < 3; ++c) {
+          double acc = y[3 * (c0 + k) + c];
+          for (int i = k + 1; i < m; ++i) acc -= Ls[(size_t)k * m + i] * y[3 * rows[i] + c];
+          y[3 * (c0 + k) + c] = acc / Ls[(size_t)k * m + k];
+        }
+    }
+    for (int v = 0; v < N; ++v)
+      for (int c = 0; c < 3; ++c) x[(size_t)r * 3 * N + 3 * s->S.perm[v] + c] = y[3 * v + c];
+  }
+  return 0;
+}
+
+void pdm_scene_destroy(pdm_scene s) {
+  if (!s) return;
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  if (s->ev0) (void)hipEventDestroy(s->ev0);
+  if (s->ev1) (void)hipEventDestroy(s->ev1);
+  if (s->own_stream && s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+}
+
+int pdm_scene_info(pdm_scene s, pdm_info* info) {
+  if (!s || !info) {
+    g_err = "null argument";
+    return 1;
+  }
+  info->n_vertices = s->S.N;
+  info->n_tets = s->S.T;
+  info->n_muscles = s->S.M;
+  info->n_groups = s->S.n_groups;
+  info->n_supernodes = s->F.ns;
+  info->n_levels = (int)s->F.level_ptr.size() - 1;
+  info->max_front = s->F.max_m;
+  info->max_width = s->F.max_w;
+  info->nnz_factor = (long long)s->F.nnz;
+  info->factor_flops = s->F.flops_factor;
+  info->reg_used = s->F.reg_used;
+  info->n_contact_vertices = (int)s->S.contact_vtx.size();
+  info->batch = s->R;
+  info->precision = s->precision;
+  return 0;
+}
+
+int pdm_set_stream(pdm_scene s, void* stream) {
+  if (!s) {
+    g_err = "null scene";
+    return 1;
+  }
+  HIPX(hipStreamSynchronize(s->stream));
+  if (stream) {
+    if (s->own_stream) (void)hipStreamDestroy(s->stream);
+    s->stream = (hipStream_t)stream;
+    s->own_stream = false;
+  } else if (!s->own_stream) {
+    HIPX(hipStreamCreate(&s->stream));
+    s->own_stream = true;
+  }
+  return 0;
+}
+
+int pdm_set_material(pdm_scene s, const double* E, double nu_fwd) {
+  if (!s) {
+    g_err = "null scene";
+    return 1;
+  }
+  HIPX(hipStreamSynchronize(s->stream));
+  HostScene& S = s->S;
+  if (E)
+    for (int t = 0; t < S.T; ++t) S.E[t] = E[S.tet_order[t]];
+  S.nu_fwd = nu_fwd;
+  s->have_tape = false;
+  return refactor(s);
+}
+
+int pdm_set_batch(pdm_scene s, int R, int precision, int flags, int checkpoint_every) {
+  if (!s || R < 1 || (precision != PDM_FP64 && precision != PDM_MIXED) || checkpoint_every < 1) {
+    g_err = "invalid batch arguments";
+    return 1;
+  }
+  if (flags & PDM_USE_GRAPH) {
+    g_err = "hipGraph capture is not available in this build of the engine";
+    return 1;
+  }
+  HIPX(hipStreamSynchronize(s->stream));
+  s->R = R;
+  s->C = 3 * R;
+  s->precision = precision;
+  s->flags = flags;
+  s->ckpt = checkpoint_every;
+  s->have_tape = false;
+  return 0;
+}
+
+int pdm_forward(pdm_scene s, int steps, const double* x0, const double* v0, const double* act, double* xT, double* vT,
+                int record) {
+  if (!s || !x0 || !v0 || !xT || !vT) {
+    g_err = "null argument";
+    return 1;
+  }
+  return forward_impl(s, steps, x0, v0, act, xT, vT, record);
+}
+
+int pdm_forward_host(pdm_scene s, int steps, const double* x0, const double* v0, const double* act, double* xT,
+                     double* vT, int record) {
+  if (!s || !x0 || !v0 || !xT || !vT) {
+    g_err = "null argument";
+    return 1;
+  }
+  size_t n = (size_t)s->R * 3 * s->S.N;
+  size_t na = act ? (size_t)s->R * steps * s->S.n_groups : 0;
+  DevBuf<double> dx, dv, da, dxt, dvt;
+  if (dx.upload(x0, n) || dv.upload(v0, n) || dxt.alloc(n) || dvt.alloc(n)) return 1;
+  if (na && da.upload(act, na)) return 1;
+  if (forward_impl(s, steps, dx.p, dv.p, na ? da.p : nullptr, dxt.p, dvt.p, record)) return 1;
+  HIPX(hipMemcpy(xT, dxt.p, n * 8, hipMemcpyDeviceToHost));
+  HIPX(hipMemcpy(vT, dvt.p, n * 8, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int pdm_backward(pdm_scene s, const double* gxT, const double* gvT, double* gx0, double* gv0, double* gE, double* gnu,
+                 double* gact) {
+  if (!s || !gxT || !gvT) {
+    g_err = "null argument";
+    return 1;
+  }
+  return backward_impl(s, gxT, gvT, gx0, gv0, gE, gnu, gact);
+}
+
+int pdm_backward_host(pdm_scene s, const double* gxT, const double* gvT, double* gx0, double* gv0, double* gE,
+                      double* gnu, double* gact) {
+  if (!s || !gxT || !gvT) {
+    g_err = "null argument";
+    return 1;
+  }
+  size_t n = (size_t)s->R * 3 * s->S.N;
+  size_t na = (size_t)s->R * s->tape_steps * s->S.n_groups;
+  DevBuf<double> dgx, dgv, ox, ov, oE, onu, oa;
+  if (dgx.upload(gxT, n) || dgv.upload(gvT, n) || ox.alloc(n) || ov.alloc(n) || oE.alloc(s->S.T) || onu.alloc(1) ||
+      oa.alloc(std::max<size_t>(1, na)))
+    return 1;
+  if (backward_impl(s, dgx.p, dgv.p, ox.p, ov.p, oE.p, onu.p, na ? oa.p : nullptr)) return 1;
+  if (gx0) HIPX(hipMemcpy(gx0, ox.p, n * 8, hipMemcpyDeviceToHost));
+  if (gv0) HIPX(hipMemcpy(gv0, ov.p, n * 8, hipMemcpyDeviceToHost));
+  if (gE) HIPX(hipMemcpy(gE, oE.p, (size_t)s->S.T * 8, hipMemcpyDeviceToHost));
+  if (gnu) HIPX(hipMemcpy(gnu, onu.p, 8, hipMemcpyDeviceToHost));
+  if (gact && na) HIPX(hipMemcpy(gact, oa.p, na * 8, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int pdm_get_iterations(pdm_scene s, int* iters) {
+  if (!s || !iters) {
+    g_err = "null argument";
+    return 1;
+  }
+  std::copy(s->iters_host.begin(), s->iters_host.begin() + (size_t)s->R * s->last_steps, iters);
+  return 0;
+}
+
+int pdm_solve_host(pdm_scene s, const double* b, double* x) {
+  if (!s || !b || !x) {
+    g_err = "null argument";
+    return 1;
+  }
+  if (ensure_batch_buffers(s)) return 1;
+  size_t n = s->NC();
+  DevBuf<double> db;
+  if (db.upload(b, n)) return 1;
+  hipLaunchKernelGGL(k_caller_to_engine, dim3(cdiv(n, kThreads)), dim3(kThreads), 0, s->stream, db.p, s->B.p,
+                     s->perm.p, s->S.N, s->R);
+  if (launch_solve(s, s->B.p)) return 1;
+  hipLaunchKernelGGL(k_engine_to_caller, dim3(cdiv(n, kThreads)), dim3(kThreads), 0, s->stream, s->B.p, db.p,
+                     s->perm.p, s->S.N, s->R);
+  HIPX(hipStreamSynchronize(s->stream));
+  HIPX(hipMemcpy(x, db.p, n * 8, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int pdm_force_host(pdm_scene s, const double* x, const double* act_step, double* Jd) {
+  if (!s || !x || !Jd) {
+    g_err = "null argument";
+    return 1;
+  }
+  if (ensure_batch_buffers(s)) return 1;
+  size_t n = s->NC();
+  DevBuf<double> dx, da;
+  if (dx.upload(x, n)) return 1;
+  size_t na = act_step ? (size_t)s->R * s->S.n_groups : 0;
+  if (na && da.upload(act_step, na)) return 1;
+  hipLaunchKernelGGL(k_caller_to_engine, dim3(cdiv(n, kThreads)), dim3(kThreads), 0, s->stream, dx.p, s->Xit.p,
+                     s->perm.p, s->S.N, s->R);
+  std::vector<int> ones(s->R, 1);
+  HIPX(hipMemcpy(s->active.p, ones.data(), s->R * sizeof(int), hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_attach_rhs, dim3(cdiv(n, kThreads)), dim3(kThreads), 0, s->stream, s->attw.p, s->xrest.p,
+                     s->Cc.p, s->S.N, s->C);
+  if (s->precision == PDM_MIXED)
+    launch_local<float>(s, s->Xit.p, na ? da.p : nullptr, 1, 0);
+  else
+    launch_local<double>(s, s->Xit.p, na ? da.p : nullptr, 1, 0);
+  hipLaunchKernelGGL(k_engine_to_caller, dim3(cdiv(n, kThreads)), dim3(kThreads), 0, s->stream, s->B.p, dx.p,
+                     s->perm.p, s->S.N, s->R);
+  HIPX(hipStreamSynchronize(s->stream));
+  HIPX(hipMemcpy(Jd, dx.p, n * 8, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int pdm_get_timing(pdm_scene s, double* fwd_ms, double* bwd_ms, double* factor_ms) {
+  if (!s) {
+    g_err = "null scene";
+    return 1;
+  }
+  if (fwd_ms) *fwd_ms = s->t_fwd;
+  if (bwd_ms) *bwd_ms = s->t_bwd;
+  if (factor_ms) *factor_ms = s->t_factor;
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,102 @@
+// Internal host-side structures of libpdm (not part of the C ABI).
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace pdm {
+
+// Mesh + material + stepping parameters after setup, in the engine's own
+// vertex order (nested-dissection permutation of the caller's order).
+struct HostScene {
+  int N = 0;                       // vertices
+  int T = 0;                       // tets
+  std::vector<int> tet_order;      // engine tet i = caller tet tet_order[i]
+  int M = 0;                       // muscle constraints
+  std::vector<int> perm;           // perm[new] = caller index
+  std::vector<int> iperm;          // iperm[caller] = new index
+  std::vector<double> xrest;       // 3N, engine order
+  std::vector<double> mass;        // N
+  // tets (engine vertex ids, oriented)
+  std::vector<int> tv;             // 4T
+  std::vector<double> tG;          // 12T: G[k][j] at 12t + 3k + j
+  std::vector<double> tvol;        // T
+  std::vector<double> E;           // T
+  std::vector<double> nu_ctor;     // T
+  double nu_fwd = 0.4;
+  std::vector<double> wr, wv;      // T
+  // muscles
+  std::vector<int> mv;             // 4M
+  std::vector<double> mg;          // 4M
+  std::vector<double> mw;          // M (k * weight * vol)
+  std::vector<int> mgroup;         // M
+  int n_groups = 0;
+  // attachments (engine ids)
+  std::vector<int> att_idx;
+  std::vector<double> att_w;
+  // contact candidates (engine ids) and obstacles
+  std::vector<int> contact_vtx;
+  std::vector<double> planes;      // 6 per plane
+  std::vector<double> spheres;     // 4 per sphere
+  // stepping
+  double h = 0.005, damping = 0.9999;
+  double g[3] = {0, 0, 0};
+  int gravity_on = 1;
+  int max_iter = 30, max_iter1 = 30;
+  int contact = 0;
+  double alpha = 1.0, mu = 1.0;
+  int outer_cap = 16;
+};
+
+// Scalar sparse matrix, CSR, engine order.
+struct Csr {
+  int n = 0;
+  std::vector<int> ptr, col;
+  std::vector<double> val;
+};
+
+// Supernodal Cholesky K_s = L L^T (engine order is already fill-reducing and
+// postordered, so supernodes are contiguous column ranges).
+struct Factor {
+  int n = 0, ns = 0;
+  std::vector<int> first;          // ns+1: first column of each supernode
+  std::vector<int64_t> rowptr;     // ns+1: row structure offsets
+  std::vector<int> rows;           // rows of supernode s (first w are its own columns)
+  std::vector<int64_t> lptr;       // ns+1: offset of the m x w column-major block
+  std::vector<double> L;           // values
+  std::vector<int> parent;         // ns
+  std::vector<int> child_ptr, child;   // children lists (ns+1, ...), ascending order
+  std::vector<int64_t> relptr;     // per supernode: offset into rel for its below rows
+  std::vector<int> rel;            // position of each below row in the parent's row list
+  std::vector<int64_t> uptr;       // ns+1: offset (in rows) of each update block
+  std::vector<int> level_ptr, level_sn;  // forward levels by height (leaves first)
+  int max_m = 0, max_w = 0;
+  int64_t nnz = 0;                 // stored entries of L (incl. supernode padding)
+  double flops_factor = 0;         // numeric factorisation flops
+  double reg_used = 0;
+};
+
+// Incident-corner lists for the deterministic tet->vertex gather.
+struct Corners {
+  int nslot = 0;
+  std::vector<int> vptr;           // N+1
+  std::vector<int> tet_slot;       // 4T: slot of (tet, corner)
+  std::vector<int> mus_slot;       // 4M
+};
+
+// Build the engine scene from caller arrays (see pdm.h for the meaning).
+bool build_scene(HostScene& S, int N, const double* xrest, int T, const int* tets, int mass_same, double total_mass,
+                 std::string& err);
+void nd_order(int N, const double* xyz, const std::vector<std::vector<int>>& adj, std::vector<int>& order);
+void update_weights(HostScene& S);
+void assemble_L(const HostScene& S, Csr& L);  // elastic + muscle + attachment (no mass)
+bool symbolic(const Csr& A, Factor& F, std::string& err);
+bool numeric(const Csr& A, const std::vector<double>& diag_add, Factor& F, std::string& err);
+void build_corners(const HostScene& S, Corners& C);
+int select_muscle_tets(const double* x_caller, const int* tets_caller, int T, const double* start, const double* end,
+                       std::vector<int>& out_tet, std::vector<double>& out_w);
+bool add_muscles(HostScene& S, const double* x_caller, const int* tets_caller, int n_seg, const double* seg_start,
+                 const double* seg_end, const int* seg_group, int n_groups, double k_muscle, std::string& err);
+std::vector<int> surface_vertices(int N, const std::vector<int>& tv, int T);
+
+}  // namespace pdm

@@ -1,0 +1,342 @@
+// Device math shared by the libpdm kernels: 3x3 SVD (Jacobi eigen-analysis of
+// F^T F followed by Gram-Schmidt QR of F V), the corotated projection of
+// CorotateFEMConstraint::EvaluateDVector and its vector-Jacobian product.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace pdm {
+
+template <typename T>
+struct Num;
+template <>
+struct Num<double> {
+  static __device__ __forceinline__ double rsqrt(double x) { return 1.0 / sqrt(x); }
+  static __device__ __forceinline__ double sqrt_(double x) { return sqrt(x); }
+  static constexpr int kSweeps = 6;
+  static constexpr double kTiny = 1e-300;
+  static constexpr double kRel = 1e-15;
+};
+template <>
+struct Num<float> {
+  static __device__ __forceinline__ float rsqrt(float x) { return rsqrtf(x); }
+  static __device__ __forceinline__ float sqrt_(float x) { return sqrtf(x); }
+  static constexpr int kSweeps = 4;
+  static constexpr float kTiny = 1e-30f;
+  static constexpr float kRel = 1e-6f;
+};
+
+// F = U diag(S) V^T, S >= 0 descending, U, V orthogonal (row-major 3x3).
+template <typename T>
+__device__ __forceinline__ void svd3(const T* F, T* U, T* S, T* V) {
+  T a00 = F[0] * F[0] + F[3] * F[3] + F[6] * F[6];
+  T a01 = F[0] * F[1] + F[3] * F[4] + F[6] * F[7];
+  T a02 = F[0] * F[2] + F[3] * F[5] + F[6] * F[8];
+  T a11 = F[1] * F[1] + F[4] * F[4] + F[7] * F[7];
+  T a12 = F[1] * F[2] + F[4] * F[5] + F[7] * F[8];
+  T a22 = F[2] * F[2] + F[5] * F[5] + F[8] * F[8];
+  T v[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+#pragma unroll
+  for (int sw = 0; sw < Num<T>::kSweeps; ++sw) {
+    // (p,q) = (0,1)
+    {
+      T apq = a01;
+      if (fabs(apq) > Num<T>::kTiny) {
+        T tau = (a11 - a00) / (T(2) * apq);
+        T t = (tau >= 0 ? T(1) : T(-1)) / (fabs(tau) + Num<T>::sqrt_(T(1) + tau * tau));
+        T c = Num<T>::rsqrt(T(1) + t * t), s = t * c;
+        a00 -= t * apq;
+        a11 += t * apq;
+        a01 = 0;
+        T r0 = a02, r1 = a12;
+        a02 = c * r0 - s * r1;
+        a12 = s * r0 + c * r1;
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+          T vp = v[r * 3 + 0], vq = v[r * 3 + 1];
+          v[r * 3 + 0] = c * vp - s * vq;
+          v[r * 3 + 1] = s * vp + c * vq;
+        }
+      }
+    }
+    // (0,2)
+    {
+      T apq = a02;
+      if (fabs(apq) > Num<T>::kTiny) {
+        T tau = (a22 - a00) / (T(2) * apq);
+        T t = (tau >= 0 ? T(1) : T(-1)) / (fabs(tau) + Num<T>::sqrt_(T(1) + tau * tau));
+        T c = Num<T>::rsqrt(T(1) + t * t), s = t * c;
+        a00 -= t * apq;
+        a22 += t * apq;
+        a02 = 0;
+        T r0 = a01, r1 = a12;
+        a01 = c * r0 - s * r1;
+        a12 = s * r0 + c * r1;
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+          T vp = v[r * 3 + 0], vq = v[r * 3 + 2];
+          v[r * 3 + 0] = c * vp - s * vq;
+          v[r * 3 + 2] = s * vp + c * vq;
+        }
+      }
+    }
+    // (1,2)
+    {
+      T apq = a12;
+      if (fabs(apq) > Num<T>::kTiny) {
+        T tau = (a22 - a11) / (T(2) * apq);
+        T t = (tau >= 0 ? T(1) : T(-1)) / (fabs(tau) + Num<T>::sqrt_(T(1) + tau * tau));
+        T c = Num<T>::rsqrt(T(1) + t * t), s = t * c;
+        a11 -= t * apq;
+        a22 += t * apq;
+        a12 = 0;
+        T r0 = a01, r1 = a02;
+        a01 = c * r0 - s * r1;
+        a02 = s * r0 + c * r1;
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+          T vp = v[r * 3 + 1], vq = v[r * 3 + 2];
+          v[r * 3 + 1] = c * vp - s * vq;
+          v[r * 3 + 2] = s * vp + c * vq;
+        }
+      }
+    }
+  }
+  // B = F V, columns sorted by norm (descending)
+  T b[9];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) b[r * 3 + c] = F[r * 3 + 0] * v[0 * 3 + c] + F[r * 3 + 1] * v[1 * 3 + c] + F[r * 3 + 2] * v[2 * 3 + c];
+  T n0 = b[0] * b[0] + b[3] * b[3] + b[6] * b[6];
+  T n1 = b[1] * b[1] + b[4] * b[4] + b[7] * b[7];
+  T n2 = b[2] * b[2] + b[5] * b[5] + b[8] * b[8];
+  auto swapc = [&](int i, int j) {
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      T t = b[r * 3 + i];
+      b[r * 3 + i] = b[r * 3 + j];
+      b[r * 3 + j] = t;
+      t = v[r * 3 + i];
+      v[r * 3 + i] = v[r * 3 + j];
+      v[r * 3 + j] = t;
+    }
+  };
+  if (n0 < n1) {
+    swapc(0, 1);
+    T t = n0; n0 = n1; n1 = t;
+  }
+  if (n0 < n2) {
+    swapc(0, 2);
+    T t = n0; n0 = n2; n2 = t;
+  }
+  if (n1 < n2) {
+    swapc(1, 2);
+    T t = n1; n1 = n2; n2 = t;
+  }
+  // Gram-Schmidt QR of b
+  T s0 = Num<T>::sqrt_(n0);
+  T u0[3], u1[3], u2[3];
+  if (s0 > Num<T>::kTiny) {
+    T i0 = T(1) / s0;
+    u0[0] = b[0] * i0; u0[1] = b[3] * i0; u0[2] = b[6] * i0;
+  } else {
+    u0[0] = 1; u0[1] = 0; u0[2] = 0;
+  }
+  T d01 = u0[0] * b[1] + u0[1] * b[4] + u0[2] * b[7];
+  T w1[3] = {b[1] - d01 * u0[0], b[4] - d01 * u0[1], b[7] - d01 * u0[2]};
+  T s1 = Num<T>::sqrt_(w1[0] * w1[0] + w1[1] * w1[1] + w1[2] * w1[2]);
+  if (s1 > Num<T>::kRel * s0 + Num<T>::kTiny) {
+    T i1 = T(1) / s1;
+    u1[0] = w1[0] * i1; u1[1] = w1[1] * i1; u1[2] = w1[2] * i1;
+  } else {
+    // any unit vector orthogonal to u0
+    T ax = fabs(u0[0]), ay = fabs(u0[1]), az = fabs(u0[2]);
+    T e[3] = {0, 0, 0};
+    if (ax <= ay && ax <= az) e[0] = 1; else if (ay <= az) e[1] = 1; else e[2] = 1;
+    T d = u0[0] * e[0] + u0[1] * e[1] + u0[2] * e[2];
+    T w[3] = {e[0] - d * u0[0], e[1] - d * u0[1], e[2] - d * u0[2]};
+    T iw = Num<T>::rsqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+    u1[0] = w[0] * iw; u1[1] = w[1] * iw; u1[2] = w[2] * iw;
+    s1 = 0;
+  }
+  u2[0] = u0[1] * u1[2] - u0[2] * u1[1];
+  u2[1] = u0[2] * u1[0] - u0[0] * u1[2];
+  u2[2] = u0[0] * u1[1] - u0[1] * u1[0];
+  T s2 = u2[0] * b[2] + u2[1] * b[5] + u2[2] * b[8];
+  if (s2 < 0) {
+    s2 = -s2;
+    u2[0] = -u2[0]; u2[1] = -u2[1]; u2[2] = -u2[2];
+  }
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    U[r * 3 + 0] = u0[r];
+    U[r * 3 + 1] = u1[r];
+    U[r * 3 + 2] = u2[r];
+  }
+  S[0] = s0; S[1] = s1; S[2] = s2;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) V[i] = v[i];
+}
+
+template <typename T>
+__device__ __forceinline__ T det3(const T* a) {
+  return a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) + a[2] * (a[3] * a[7] - a[4] * a[6]);
+}
+
+// 5-step linearised volume projection (CorotateFEMConstraint.h:362-374).
+template <typename T>
+__device__ __forceinline__ void volproj(const T* S, T* D) {
+  D[0] = D[1] = D[2] = 0;
+#pragma unroll
+  for (int it = 0; it < 5; ++it) {
+    T a0 = S[0] + D[0], a1 = S[1] + D[1], a2 = S[2] + D[2];
+    T cd = a0 * a1 * a2 - T(1);
+    T g0 = a1 * a2, g1 = a0 * a2, g2 = a0 * a1;
+    T coef = (g0 * D[0] + g1 * D[1] + g2 * D[2] - cd) / (g0 * g0 + g1 * g1 + g2 * g2);
+    D[0] = coef * g0;
+    D[1] = coef * g1;
+    D[2] = coef * g2;
+  }
+}
+
+// Same projection with its Jacobian J[i][j] = d(S_i + D_i)/d S_j (forward mode).
+template <typename T>
+__device__ __forceinline__ void volproj_jac(const T* S, T* D, T* J) {
+  T d[3] = {0, 0, 0};
+  T dd[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+  for (int it = 0; it < 5; ++it) {
+    T a[3] = {S[0] + d[0], S[1] + d[1], S[2] + d[2]};
+    T da[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) da[i * 3 + j] = (i == j ? T(1) : T(0)) + dd[i * 3 + j];
+    T cd = a[0] * a[1] * a[2] - T(1);
+    T g[3] = {a[1] * a[2], a[0] * a[2], a[0] * a[1]};
+    T dcd[3], dg[9];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      dcd[j] = da[0 * 3 + j] * a[1] * a[2] + a[0] * da[1 * 3 + j] * a[2] + a[0] * a[1] * da[2 * 3 + j];
+      dg[0 * 3 + j] = da[1 * 3 + j] * a[2] + a[1] * da[2 * 3 + j];
+      dg[1 * 3 + j] = da[0 * 3 + j] * a[2] + a[0] * da[2 * 3 + j];
+      dg[2 * 3 + j] = da[0 * 3 + j] * a[1] + a[0] * da[1 * 3 + j];
+    }
+    T gg = g[0] * g[0] + g[1] * g[1] + g[2] * g[2];
+    T num = g[0] * d[0] + g[1] * d[1] + g[2] * d[2] - cd;
+    T coef = num / gg;
+    T nd[3], ndd[9];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      T dnum = dg[0 * 3 + j] * d[0] + g[0] * dd[0 * 3 + j] + dg[1 * 3 + j] * d[1] + g[1] * dd[1 * 3 + j] +
+               dg[2 * 3 + j] * d[2] + g[2] * dd[2 * 3 + j] - dcd[j];
+      T dgg = T(2) * (g[0] * dg[0 * 3 + j] + g[1] * dg[1 * 3 + j] + g[2] * dg[2 * 3 + j]);
+      T dcoef = (dnum * gg - num * dgg) / (gg * gg);
+#pragma unroll
+      for (int i = 0; i < 3; ++i) ndd[i * 3 + j] = dcoef * g[i] + coef * dg[i * 3 + j];
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) nd[i] = coef * g[i];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) d[i] = nd[i];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) dd[i] = ndd[i];
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i) D[i] = d[i];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) J[i * 3 + j] = (i == j ? T(1) : T(0)) + dd[i * 3 + j];
+}
+
+// P = wr * md + wv * mdv, where md = U V^T (third column negated when
+// det F < 0) and mdv = U diag(S + D) V^T.
+template <typename T>
+__device__ __forceinline__ void corotate_P(const T* F, T wr, T wv, T* P) {
+  T U[9], S[3], V[9], D[3];
+  svd3<T>(F, U, S, V);
+  volproj<T>(S, D);
+  bool flip = det3<T>(F) < 0;
+  T e0 = S[0] + D[0], e1 = S[1] + D[1], e2 = S[2] + D[2];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      T rot = U[r * 3 + 0] * V[c * 3 + 0] + U[r * 3 + 1] * V[c * 3 + 1] + U[r * 3 + 2] * V[c * 3 + 2];
+      T vol = U[r * 3 + 0] * e0 * V[c * 3 + 0] + U[r * 3 + 1] * e1 * V[c * 3 + 1] + U[r * 3 + 2] * e2 * V[c * 3 + 2];
+      if (flip && c == 2) rot = -rot;
+      P[r * 3 + c] = wr * rot + wv * vol;
+    }
+}
+
+// Adjoint of the weighted projection: Fbar = (dP/dF)^T [Z]; also returns the
+// projections md, mdv at F (needed for the parameter gradient).
+template <typename T>
+__device__ __forceinline__ void corotate_vjp(const T* F, T wr, T wv, const T* Z, T* Fbar, T* md, T* mdv) {
+  T U[9], S[3], V[9], D[3], J[9];
+  svd3<T>(F, U, S, V);
+  volproj_jac<T>(S, D, J);
+  bool flip = det3<T>(F) < 0;
+  T phi[3] = {S[0] + D[0], S[1] + D[1], S[2] + D[2]};
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      T rot = U[r * 3 + 0] * V[c * 3 + 0] + U[r * 3 + 1] * V[c * 3 + 1] + U[r * 3 + 2] * V[c * 3 + 2];
+      md[r * 3 + c] = (flip && c == 2) ? -rot : rot;
+      mdv[r * 3 + c] = U[r * 3 + 0] * phi[0] * V[c * 3 + 0] + U[r * 3 + 1] * phi[1] * V[c * 3 + 1] +
+                       U[r * 3 + 2] * phi[2] * V[c * 3 + 2];
+    }
+  // H = U^T Z V and Hr = U^T (Z Df) V
+  T ZU[9], H[9], Hr[9];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) ZU[i * 3 + c] = U[0 * 3 + i] * Z[0 * 3 + c] + U[1 * 3 + i] * Z[1 * 3 + c] + U[2 * 3 + i] * Z[2 * 3 + c];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      T h = 0, hr = 0;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        T zu = ZU[i * 3 + c];
+        h += zu * V[c * 3 + j];
+        hr += ((flip && c == 2) ? -zu : zu) * V[c * 3 + j];
+      }
+      H[i * 3 + j] = h;
+      Hr[i * 3 + j] = hr;
+    }
+  T O[9];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    O[i * 3 + i] = wv * (J[0 * 3 + i] * H[0] + J[1 * 3 + i] * H[4] + J[2 * 3 + i] * H[8]);
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      if (i == j) continue;
+      T ssum = S[i] + S[j];
+      T br = ssum > 0 ? T(2) / ssum : T(0);
+      T sd = S[i] - S[j];
+      T av = (fabs(sd) > T(1e-6) * (T(1) + fabs(S[i]))) ? (phi[i] - phi[j]) / sd : (J[i * 3 + i] - J[i * 3 + j]);
+      T bv = ssum > 0 ? (phi[i] + phi[j]) / ssum : T(0);
+      T rot = T(0.5) * br * (Hr[i * 3 + j] - Hr[j * 3 + i]);
+      T vol = T(0.5) * (av + bv) * H[i * 3 + j] + T(0.5) * (av - bv) * H[j * 3 + i];
+      O[i * 3 + j] = wr * rot + wv * vol;
+    }
+  // Fbar = U O V^T
+  T UO[9];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) UO[r * 3 + j] = U[r * 3 + 0] * O[0 * 3 + j] + U[r * 3 + 1] * O[1 * 3 + j] + U[r * 3 + 2] * O[2 * 3 + j];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) Fbar[r * 3 + c] = UO[r * 3 + 0] * V[c * 3 + 0] + UO[r * 3 + 1] * V[c * 3 + 1] + UO[r * 3 + 2] * V[c * 3 + 2];
+}
+
+}  // namespace pdm

@@ -1,0 +1,239 @@
+"""Python handle over one libpdm scene (C ABI in include/pdm.h).
+
+``Scene`` owns a ``pdm_scene``: build it from a ``SceneDesc``, choose the
+rollout batch and precision, then run forward rollouts and reverse sweeps on
+host numpy arrays or on device tensors (torch, FP64, caller AoS layout).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import PDM_FP64, PDM_HOST_EARLY_EXIT, PDM_MIXED, PDM_USE_GRAPH, check  # noqa: F401
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _iptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def _vp(a):
+    if a is None:
+        return None
+    if hasattr(a, "data_ptr"):
+        return C.c_void_p(a.data_ptr())
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def make_desc(desc):
+    """SceneDesc -> (pdm_scene_desc, arrays kept alive for its pointers)."""
+    keep = []
+
+    def arr(a, dt):
+        a = np.ascontiguousarray(a, dtype=dt)
+        keep.append(a)
+        return a
+
+    x = arr(desc.x_rest.ravel(), np.float64)
+    t = arr(desc.tets.ravel(), np.int32)
+    E = arr(desc.E, np.float64)
+    nuc = arr(desc.nu_ctor, np.float64)
+    att = arr(desc.attach_idx, np.int32)
+    s0 = arr(desc.seg_start.ravel(), np.float64)
+    s1 = arr(desc.seg_end.ravel(), np.float64)
+    sg = arr(desc.seg_group, np.int32)
+    pl = arr(desc.planes.ravel(), np.float64)
+    sp = arr(desc.spheres.ravel(), np.float64)
+    d = _lib.SceneDescC()
+    d.n_vertices = desc.N
+    d.x_rest = _dptr(x)
+    d.n_tets = desc.T
+    d.tets = _iptr(t)
+    d.mass_same = int(desc.mass_same)
+    d.total_mass = float(desc.total_mass)
+    d.E = _dptr(E)
+    d.nu_fwd = float(desc.nu_fwd)
+    d.nu_ctor = _dptr(nuc)
+    d.n_attach = int(att.size)
+    d.attach_idx = _iptr(att)
+    d.attach_w = float(desc.attach_w)
+    d.n_segments = int(sg.size)
+    d.seg_start = _dptr(s0)
+    d.seg_end = _dptr(s1)
+    d.seg_group = _iptr(sg)
+    d.n_groups = int(desc.n_groups)
+    d.k_muscle = float(desc.k_muscle)
+    d.h = float(desc.h)
+    d.max_iter = int(desc.max_iter)
+    d.damping = float(desc.damping)
+    for i in range(3):
+        d.gravity[i] = float(desc.gravity[i])
+    d.gravity_on = int(desc.gravity_on)
+    d.contact = int(desc.contact)
+    d.max_iter1 = int(desc.max_iter1)
+    d.alpha = float(desc.alpha)
+    d.mu = float(desc.mu)
+    d.n_planes = int(desc.planes.shape[0])
+    d.planes = _dptr(pl)
+    d.n_spheres = int(desc.spheres.shape[0])
+    d.spheres = _dptr(sp)
+    return d, keep
+
+
+class Scene:
+    def __init__(self, desc, R: int = 1, precision: int = PDM_FP64, flags: int = PDM_HOST_EARLY_EXIT,
+                 checkpoint_every: int = 20):
+        L = _lib.load()
+        self.desc = desc
+        self.N = int(desc.N)
+        self.T = int(desc.T)
+        self.n_groups = int(desc.n_groups)
+        d, self._keep = make_desc(desc)
+        h = C.c_void_p()
+        check(L.pdm_scene_create(C.byref(d), C.byref(h)))
+        self.h = h
+        self._steps = 0
+        self.set_batch(R, precision, flags, checkpoint_every)
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            _lib.load().pdm_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------- config
+    def set_batch(self, R: int, precision: int = PDM_FP64, flags: int = PDM_HOST_EARLY_EXIT,
+                  checkpoint_every: int = 20):
+        check(_lib.load().pdm_set_batch(self.h, int(R), int(precision), int(flags), int(checkpoint_every)))
+        self.R = int(R)
+        self.precision = int(precision)
+        self.flags = int(flags)
+
+    def set_material(self, E=None, nu_fwd=None):
+        L = _lib.load()
+        Ea = None
+        if E is not None:
+            Ea = np.ascontiguousarray(np.broadcast_to(np.asarray(E, np.float64), (self.T,)))
+        nu = float(self.desc.nu_fwd if nu_fwd is None else nu_fwd)
+        check(L.pdm_set_material(self.h, _vp(Ea), nu))
+
+    def set_stream(self, stream_ptr):
+        check(_lib.load().pdm_set_stream(self.h, C.c_void_p(stream_ptr) if stream_ptr else None))
+
+    def info(self) -> dict:
+        i = _lib.InfoC()
+        check(_lib.load().pdm_scene_info(self.h, C.byref(i)))
+        return {k: getattr(i, k) for k, _ in _lib.InfoC._fields_}
+
+    # ------------------------------------------------------------- host API
+    def forward_host(self, x0, v0, steps: int, act=None, record: bool = False):
+        R, n3 = self.R, 3 * self.N
+        x0 = np.ascontiguousarray(np.broadcast_to(np.asarray(x0, np.float64).reshape(-1, n3), (R, n3)))
+        v0 = np.ascontiguousarray(np.broadcast_to(np.asarray(v0, np.float64).reshape(-1, n3), (R, n3)))
+        a = None
+        if act is not None and self.n_groups > 0:
+            a = np.ascontiguousarray(np.broadcast_to(np.asarray(act, np.float64).reshape(-1, steps, self.n_groups),
+                                                     (R, steps, self.n_groups)))
+        xT = np.zeros((R, n3))
+        vT = np.zeros((R, n3))
+        check(_lib.load().pdm_forward_host(self.h, int(steps), _vp(x0), _vp(v0), _vp(a), _vp(xT), _vp(vT),
+                                           int(record)))
+        self._steps = steps
+        return xT, vT
+
+    def backward_host(self, gx, gv):
+        R, n3 = self.R, 3 * self.N
+        gx = np.ascontiguousarray(np.asarray(gx, np.float64).reshape(R, n3))
+        gv = np.ascontiguousarray(np.asarray(gv, np.float64).reshape(R, n3))
+        gx0 = np.zeros((R, n3))
+        gv0 = np.zeros((R, n3))
+        gE = np.zeros(self.T)
+        gnu = np.zeros(1)
+        gact = np.zeros((R, max(self._steps, 1), max(self.n_groups, 1)))
+        check(_lib.load().pdm_backward_host(self.h, _vp(gx), _vp(gv), _vp(gx0), _vp(gv0), _vp(gE), _vp(gnu),
+                                            _vp(gact) if self.n_groups else None))
+        return dict(x0=gx0, v0=gv0, E=gE, nu=float(gnu[0]), act=gact[:, :self._steps, :self.n_groups])
+
+    def iterations(self):
+        it = np.zeros(self.R * self._steps, np.int32)
+        if it.size:
+            check(_lib.load().pdm_get_iterations(self.h, _iptr(it)))
+        return it.reshape(self.R, self._steps)
+
+    def solve_host(self, b):
+        b = np.ascontiguousarray(np.asarray(b, np.float64).reshape(self.R, 3 * self.N))
+        x = np.zeros_like(b)
+        check(_lib.load().pdm_solve_host(self.h, _vp(b), _vp(x)))
+        return x
+
+    def force_host(self, x, act_step=None):
+        x = np.ascontiguousarray(np.asarray(x, np.float64).reshape(self.R, 3 * self.N))
+        a = None
+        if act_step is not None and self.n_groups:
+            a = np.ascontiguousarray(np.asarray(act_step, np.float64).reshape(self.R, self.n_groups))
+        out = np.zeros_like(x)
+        check(_lib.load().pdm_force_host(self.h, _vp(x), _vp(a), _vp(out)))
+        return out
+
+    def timing(self):
+        f, b, fa = C.c_double(), C.c_double(), C.c_double()
+        check(_lib.load().pdm_get_timing(self.h, C.byref(f), C.byref(b), C.byref(fa)))
+        return dict(forward_ms=f.value, backward_ms=b.value, factor_ms=fa.value)
+
+    # ------------------------------------------------------------- device API
+    def forward_device(self, x0, v0, steps: int, act=None, xT=None, vT=None, record: bool = False):
+        """x0, v0: FP64 device tensors (R, 3N); act: (R, steps, G) or None.
+        Inputs stay in HBM; outputs are written into xT/vT (allocated if None)."""
+        import torch
+        R, n3 = self.R, 3 * self.N
+        for t in (x0, v0):
+            assert t.dtype == torch.float64 and t.is_cuda and t.is_contiguous() and t.numel() == R * n3
+        if act is not None:
+            assert act.dtype == torch.float64 and act.is_contiguous() and act.numel() == R * steps * self.n_groups
+        if xT is None:
+            xT = torch.empty_like(x0)
+        if vT is None:
+            vT = torch.empty_like(v0)
+        check(_lib.load().pdm_forward(self.h, int(steps), _vp(x0), _vp(v0), _vp(act), _vp(xT), _vp(vT),
+                                      int(record)))
+        self._steps = steps
+        return xT, vT
+
+    def backward_device(self, gx, gv):
+        import torch
+        R, n3 = self.R, 3 * self.N
+        gx0 = torch.empty((R, n3), dtype=torch.float64, device=gx.device)
+        gv0 = torch.empty_like(gx0)
+        gE = torch.empty(self.T, dtype=torch.float64, device=gx.device)
+        gnu = torch.empty(1, dtype=torch.float64, device=gx.device)
+        gact = torch.zeros((R, max(self._steps, 1), max(self.n_groups, 1)), dtype=torch.float64, device=gx.device)
+        check(_lib.load().pdm_backward(self.h, _vp(gx.contiguous()), _vp(gv.contiguous()), _vp(gx0), _vp(gv0),
+                                       _vp(gE), _vp(gnu), _vp(gact) if self.n_groups else None))
+        return dict(x0=gx0, v0=gv0, E=gE, nu=gnu, act=gact[:, :self._steps, :self.n_groups])
+
+
+def host_analyse(desc) -> dict:
+    """Setup + ordering + symbolic + numeric factorisation on the CPU only."""
+    d, keep = make_desc(desc)
+    i = _lib.InfoC()
+    check(_lib.load().pdm_host_analyse(C.byref(d), C.byref(i)))
+    return {k: getattr(i, k) for k, _ in _lib.InfoC._fields_}
+
+
+def host_solve(desc, b):
+    """Solve (M/h^2 + L) x = b on the CPU with the engine's supernodal factor."""
+    d, keep = make_desc(desc)
+    b = np.ascontiguousarray(np.asarray(b, np.float64).reshape(-1, 3 * desc.N))
+    x = np.zeros_like(b)
+    check(_lib.load().pdm_host_solve(C.byref(d), b.shape[0], _vp(b), _vp(x)))
+    return x

@@ -1,0 +1,150 @@
+/*
+ * pdm.h — C ABI of libpdm, the MI355X-native projective-dynamics engine for
+ * diff_fem's corotated-tetrahedral FEM (forward step and its gradient).
+ *
+ * Plain C types only; every function returns 0 on success and a non-zero
+ * status otherwise, with a message in pdm_last_error(). No exceptions cross
+ * this boundary. All state vectors are in the caller's layout: per rollout the
+ * reference's AoS x[3*i + c] (python/pydifem.cc passes std::vector<double>),
+ * rollouts stacked: x[r * 3N + 3*i + c].
+ *
+ * Entry points and the reference interface each replaces:
+ *   pdm_scene_create   <- Environment(total_step, config) + Dfobj::SetDfobj /
+ *                         Dfobj::Initialize + World::Initialize
+ *                         (sim/Environment.h:85-225, sim/Dfobj.h:142-314,
+ *                         442-611, sim/fem/World.h:248-384)
+ *   pdm_set_material   <- forward_all's mMu update + World::PreComputation
+ *                         (python/pydifem.cc:207-221, sim/fem/World.h:1089-1139)
+ *   pdm_forward(_host) <- forward_all / forward_act: Environment::Step x steps
+ *                         (python/pydifem.cc:38-86, 182-259;
+ *                         sim/Environment.h:228-293; sim/fem/World.h:673-1029)
+ *   pdm_backward(_host)<- backward_all / backward_act: CppAD Jacobian of
+ *                         <dldq,q_T>+<dldqd,qd_T> (python/pydifem.cc:90-180, 261-301)
+ *   pdm_solve_host     <- SimplicialLDLT::solve on H2ML (sim/fem/World.h:1660)
+ *   pdm_force_host     <- World::EvaluateDVector + J*d (sim/fem/World.h:1194-1210, 901)
+ *   pdm_host_analyse / pdm_host_solve <- World::PreComputation + FactorizeLDLT on
+ *                         the CPU (sim/fem/World.h:1089-1139, 1212-1232)
+ */
+#ifndef PDM_H_
+#define PDM_H_
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pdm_scene_s* pdm_scene;
+
+/* Scene description (sim/Environment.h:109-210 keys; sim/Dfobj.h:142-314). */
+typedef struct pdm_scene_desc {
+  int n_vertices;            /* N */
+  const double* x_rest;      /* 3N, rest positions (already scaled) */
+  int n_tets;                /* T */
+  const int* tets;           /* 4T, 0-based, any orientation */
+  int mass_same;             /* "Is Mass Same": volume-lumped (1) or uniform (0) */
+  double total_mass;         /* "Mass" */
+  const double* E;           /* T Young's moduli (NULL: 2e5) */
+  double nu_fwd;             /* nu entering mMu = E/(1+nu) */
+  const double* nu_ctor;     /* T nu of the volume-term weight (NULL: 0.4) */
+  int n_attach;              /* "Attach Verts" */
+  const int* attach_idx;
+  double attach_w;           /* "Attach Stiffness" */
+  int n_segments;            /* muscle segments (50 per fibre in the reference) */
+  const double* seg_start;   /* 3 per segment */
+  const double* seg_end;     /* 3 per segment */
+  const int* seg_group;      /* activation group of each segment */
+  int n_groups;              /* activation groups (columns of the act arrays) */
+  double k_muscle;           /* "Muscle Stiffness" */
+  double h;                  /* 1 / "Simulation HZ" */
+  int max_iter;              /* "Max Iteration" (PD loop) */
+  double damping;            /* "Damping coefficient" */
+  double gravity[3];         /* "Gravity" */
+  int gravity_on;            /* "Enable Gravity" */
+  int contact;               /* "Handle collision" */
+  int max_iter1;             /* contact Jacobi cap ("Max Iteration") */
+  double alpha;              /* "World Alpha" */
+  double mu;                 /* friction coefficient of obstacle contacts */
+  int n_planes;              /* static planes: point(3), normal(3) */
+  const double* planes;
+  int n_spheres;             /* static spheres: centre(3), radius */
+  const double* spheres;
+} pdm_scene_desc;
+
+typedef struct pdm_info {
+  int n_vertices, n_tets, n_muscles, n_groups;
+  int n_supernodes, n_levels, max_front, max_width;
+  long long nnz_factor;      /* stored entries of the supernodal factor L */
+  double factor_flops;       /* flops of one numeric factorisation */
+  double reg_used;           /* regularisation applied by the ladder (0 if none) */
+  int n_contact_vertices;
+  int batch;                 /* rollouts R */
+  int precision;             /* PDM_FP64 or PDM_MIXED */
+} pdm_info;
+
+enum { PDM_FP64 = 0, PDM_MIXED = 1 };
+
+/* Flags of pdm_set_batch. */
+enum {
+  PDM_HOST_EARLY_EXIT = 1, /* read the convergence test back each iteration and
+                              stop launching (the reference's per-iteration break);
+                              without it converged rollouts are masked on device */
+  PDM_USE_GRAPH = 2        /* capture one step as a hipGraph and replay it */
+};
+
+const char* pdm_last_error(void);
+int pdm_device_count(int* n);
+
+int pdm_scene_create(const pdm_scene_desc* desc, pdm_scene* out);
+void pdm_scene_destroy(pdm_scene s);
+int pdm_scene_info(pdm_scene s, pdm_info* info);
+
+/* Bind a HIP stream (hipStream_t as void*; NULL = the library's own stream). */
+int pdm_set_stream(pdm_scene s, void* stream);
+
+/* Material update + numeric re-factorisation. E: T values (caller tet order)
+ * or NULL to keep; nu_fwd as forward_all's tau[1]. */
+int pdm_set_material(pdm_scene s, const double* E, double nu_fwd);
+
+/* Rollout batch R, arithmetic precision (PDM_FP64 everywhere, or PDM_MIXED:
+ * FP32 local step, FP64 factor/solves/accumulation), flags, and the
+ * checkpoint interval used by recorded forwards (steps between stored states). */
+int pdm_set_batch(pdm_scene s, int R, int precision, int flags, int checkpoint_every);
+
+/* Forward rollout of `steps` steps for all R rollouts. Device pointers:
+ * x0, v0, xT, vT: R*3N doubles; act: R*steps*n_groups doubles or NULL (zero
+ * activation). record != 0 keeps checkpoints for pdm_backward. */
+int pdm_forward(pdm_scene s, int steps, const double* x0, const double* v0, const double* act, double* xT, double* vT,
+                int record);
+int pdm_forward_host(pdm_scene s, int steps, const double* x0, const double* v0, const double* act, double* xT,
+                     double* vT, int record);
+
+/* Reverse sweep over the last recorded forward: gradient of
+ * sum_r <gxT_r, x_T,r> + <gvT_r, v_T,r> w.r.t. x0, v0 (R*3N), per-tet E (T,
+ * caller order, summed over rollouts), nu_fwd (scalar) and act
+ * (R*steps*n_groups). Device pointers; any output may be NULL. */
+int pdm_backward(pdm_scene s, const double* gxT, const double* gvT, double* gx0, double* gv0, double* gE, double* gnu,
+                 double* gact);
+int pdm_backward_host(pdm_scene s, const double* gxT, const double* gvT, double* gx0, double* gv0, double* gE,
+                      double* gnu, double* gact);
+
+/* Executed PD iterations of the last forward: R*steps ints (host). */
+int pdm_get_iterations(pdm_scene s, int* iters);
+
+/* Diagnostics on host arrays (caller layout, R rollouts): solve
+ * (M/h^2 + L) x = b with the device factor; evaluate J*d at positions x. */
+int pdm_solve_host(pdm_scene s, const double* b, double* x);
+int pdm_force_host(pdm_scene s, const double* x, const double* act_step, double* Jd);
+
+/* Host-only diagnostics (no device needed): run scene setup, ordering,
+ * symbolic analysis and the numeric factorisation on the CPU; report the
+ * factor statistics, or solve (M/h^2 + L) x = b for R rollouts with the same
+ * supernodal schedule the device kernels use. */
+int pdm_host_analyse(const pdm_scene_desc* desc, pdm_info* info);
+int pdm_host_solve(const pdm_scene_desc* desc, int R, const double* b, double* x);
+
+/* Timing of the last forward/backward (ms, HIP events on the library stream). */
+int pdm_get_timing(pdm_scene s, double* fwd_ms, double* bwd_ms, double* factor_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PDM_H_ */

@@ -1,0 +1,144 @@
+"""Parity of the HIP path (through the C ABI) with the CPU FP64 oracle.
+
+Error measure: relative max-norm, ||a - b||_inf / ||b||_inf.
+Bars: 1e-9 for the plain FP64 path (positions and gradients), 1e-5 on
+positions and 1e-4 on gradients for the mixed path (FP32 local step).
+"""
+import numpy as np
+import pytest
+
+from diff_fem_amd import scene
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(np.asarray(b)).max(), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def eng():
+    from diff_fem_amd import engine
+    return engine
+
+
+def small_muscle_scene():
+    x, t = scene.box_grid(6, 3, 3)
+    s0, s1, g, ng = scene.straight_fibres(2, 2, x.min(0), x.max(0), nseg=10)
+    rng = np.random.default_rng(11)
+    return scene.SceneDesc(x_rest=x, tets=t, total_mass=8.0, E=rng.uniform(5e4, 2e5, t.shape[0]), nu_fwd=0.4,
+                           nu_ctor=np.full(t.shape[0], 0.4), attach_idx=np.arange(16, dtype=np.int32),
+                           attach_w=1e20, seg_start=s0, seg_end=s1, seg_group=g, n_groups=ng, k_muscle=5e5,
+                           h=0.01, max_iter=10, gravity=(0, 0, -9.81))
+
+
+def test_solve_matches_oracle(eng, oracle_mod):
+    d = scene.config1()
+    s = eng.Scene(d, R=2)
+    o = oracle_mod.OracleScene(d)
+    rng = np.random.default_rng(0)
+    b = rng.standard_normal((2, 3 * d.N))
+    x = s.solve_host(b)
+    for r in range(2):
+        assert rel(x[r], o.solve3(b[r])) < 1e-11
+
+
+def test_force_matches_oracle(eng, oracle_mod):
+    d = small_muscle_scene()
+    s = eng.Scene(d, R=2)
+    o = oracle_mod.OracleScene(d)
+    assert s.info()["n_muscles"] == o.muscles[0].shape[0]
+    rng = np.random.default_rng(1)
+    x = d.x_rest.ravel()[None, :] + 0.003 * rng.standard_normal((2, 3 * d.N))
+    act = rng.uniform(0, 1, (2, d.n_groups))
+    Jd = s.force_host(x, act)
+    for r in range(2):
+        Jo, _ = o.force_terms(x[r], act[r])
+        assert rel(Jd[r], Jo) < 1e-12
+
+
+def test_forward_config1_fp64_host_exit(eng, oracle_mod):
+    d = scene.config1()
+    s = eng.Scene(d, R=1)
+    o = oracle_mod.OracleScene(d)
+    x0 = d.x_rest.ravel().copy()
+    v0 = np.zeros_like(x0)
+    steps = 100
+    xT, vT = s.forward_host(x0, v0, steps)
+    xo, vo, its = o.rollout(x0, v0, steps)
+    assert np.array_equal(s.iterations()[0], its[:, 0])
+    assert rel(xT[0], xo) < 1e-9
+    assert rel(vT[0], vo) < 1e-9
+
+
+def test_forward_batched_device_masking(eng, oracle_mod):
+    d = scene.config1()
+    R = 3
+    s = eng.Scene(d, R=R, flags=0)
+    o = oracle_mod.OracleScene(d)
+    rng = np.random.default_rng(2)
+    x0 = np.tile(d.x_rest.ravel(), (R, 1))
+    v0 = 0.05 * rng.standard_normal((R, 3 * d.N))
+    v0[:, :] *= np.repeat((np.abs(d.x_rest[:, 0]) > 1e-9)[None, :, None], 3, axis=2).reshape(1, -1)
+    steps = 40
+    xT, vT = s.forward_host(x0, v0, steps)
+    its = s.iterations()
+    for r in range(R):
+        xo, vo, io = o.rollout(x0[r], v0[r], steps)
+        assert np.array_equal(its[r], io[:, 0])
+        assert rel(xT[r], xo) < 1e-9 and rel(vT[r], vo) < 1e-9
+
+
+def test_forward_mixed_precision(eng, oracle_mod):
+    d = small_muscle_scene()
+    R = 2
+    s = eng.Scene(d, R=R, precision=eng.PDM_MIXED, flags=0)
+    o = oracle_mod.OracleScene(d)
+    steps = 30
+    act = np.random.default_rng(3).uniform(0, 1, (R, steps, d.n_groups))
+    x0 = np.tile(d.x_rest.ravel(), (R, 1))
+    v0 = np.zeros_like(x0)
+    xT, vT = s.forward_host(x0, v0, steps, act=act)
+    for r in range(R):
+        xo, vo, _ = o.rollout(x0[r], v0[r], steps, act=act[r])
+        assert rel(xT[r], xo) < 1e-5
+
+
+def _grad_case(eng, oracle_mod, d, precision, steps, R, tol_x, tol_g, ckpt=4):
+    s = eng.Scene(d, R=R, precision=precision, flags=0, checkpoint_every=ckpt)
+    o = oracle_mod.OracleScene(d)
+    rng = np.random.default_rng(4)
+    x0 = np.tile(d.x_rest.ravel(), (R, 1))
+    v0 = 0.01 * rng.standard_normal((R, 3 * d.N))
+    act = rng.uniform(0, 1, (R, steps, d.n_groups)) if d.n_groups else None
+    xT, vT = s.forward_host(x0, v0, steps, act=act, record=True)
+    gx = rng.standard_normal((R, 3 * d.N))
+    gv = 0.01 * rng.standard_normal((R, 3 * d.N))
+    g = s.backward_host(gx, gv)
+    gE_sum = np.zeros(d.T)
+    nu_sum = 0.0
+    for r in range(R):
+        xo, vo, _ = o.rollout(x0[r], v0[r], steps, act=None if act is None else act[r], record=True)
+        assert rel(xT[r], xo) < tol_x
+        go, rc = o.backward(gx[r], gv[r])
+        assert rc == 0
+        assert rel(g["x0"][r], go["x0"]) < tol_g
+        assert rel(g["v0"][r], go["v0"]) < tol_g
+        if act is not None:
+            assert rel(g["act"][r], go["act"]) < tol_g
+        gE_sum += go["E"]
+        nu_sum += go["nu"]
+    assert rel(g["E"], gE_sum) < tol_g
+    assert abs(g["nu"] - nu_sum) <= tol_g * abs(nu_sum)
+
+
+def test_gradient_fp64_config1(eng, oracle_mod):
+    _grad_case(eng, oracle_mod, scene.config1(), eng.PDM_FP64, steps=12, R=2, tol_x=1e-9, tol_g=1e-9)
+
+
+def test_gradient_fp64_muscles(eng, oracle_mod):
+    _grad_case(eng, oracle_mod, small_muscle_scene(), eng.PDM_FP64, steps=9, R=2, tol_x=1e-9, tol_g=1e-9)
+
+
+def test_gradient_mixed_muscles(eng, oracle_mod):
+    _grad_case(eng, oracle_mod, small_muscle_scene(), eng.PDM_MIXED, steps=9, R=2, tol_x=1e-5, tol_g=1e-4)
