@@ -52,7 +52,7 @@ SYMBOLS = [
     "pdm_last_error", "pdm_device_count", "pdm_scene_create", "pdm_scene_destroy", "pdm_scene_info",
     "pdm_set_stream", "pdm_set_material", "pdm_set_batch", "pdm_forward", "pdm_forward_host", "pdm_backward",
     "pdm_backward_host", "pdm_get_iterations", "pdm_solve_host", "pdm_force_host", "pdm_get_timing",
-    "pdm_host_analyse", "pdm_host_solve",
+    "pdm_host_analyse", "pdm_host_solve", "pdm_get_contact_stats",
 ]
 
 _lib = None
@@ -81,6 +81,7 @@ def load():
     for name in ("pdm_backward", "pdm_backward_host"):
         getattr(L, name).argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
     L.pdm_get_iterations.argtypes = [vp, _ip]
+    L.pdm_get_contact_stats.argtypes = [vp, _ip, _ip]
     L.pdm_solve_host.argtypes = [vp, vp, vp]
     L.pdm_force_host.argtypes = [vp, vp, vp, vp]
     L.pdm_get_timing.argtypes = [vp, _dp, _dp, _dp]

@@ -207,6 +207,8 @@ __global__ void k_gather_masked(const T* __restrict__ corner, const int* __restr
 }
 
 struct FactorView {
+  const long long* dptr;
+  const double* Dinv;
   const int* first;
   const long long* rowptr;
   const int* rows;
@@ -221,8 +223,10 @@ struct FactorView {
 
 // Forward substitution (L y = b) of one elimination-tree level. Block =
 // (supernode, column tile). The front T (m x Ct) is staged in LDS: own rows
-// from B, children's update blocks added in child order, then the triangular
-// solve of the diagonal block and the update block U_s = T_below - L_below y.
+// from B, children's update blocks added in child order. Then a blocked
+// right-looking solve: for each 16-row diagonal block, y_b = inv(L_bb) T_b,
+// and every row below (diagonal part and update rows) gets T_i -= L_ib y_b.
+// Rows [0,w) are the solution, rows [w,m) the update block U_s for the parent.
 __global__ __launch_bounds__(256) void k_fwd_level(FactorView f, const int* __restrict__ lev, int ntile, int Ct, int C,
                                                    double* __restrict__ B, double* __restrict__ U) {
   extern __shared__ double sh[];
@@ -233,7 +237,9 @@ __global__ __launch_bounds__(256) void k_fwd_level(FactorView f, const int* __re
   int w = f.first[s + 1] - c0;
   long long rp = f.rowptr[s];
   int m = (int)(f.rowptr[s + 1] - rp);
+  double* yb = sh + (long long)m * Ct;  // kDiagBlock x Ct scratch
   const double* Ls = f.L + f.lptr[s];
+  const double* D = f.Dinv + f.dptr[s];
   int tid = threadIdx.x, nt = blockDim.x;
   for (int i = tid; i < m * ct; i += nt) {
     int row = i / ct, c = i % ct;
@@ -252,15 +258,30 @@ __global__ __launch_bounds__(256) void k_fwd_level(FactorView f, const int* __re
     }
     __syncthreads();
   }
-  // triangular solve, one thread per column
-  if (tid < ct) {
-    for (int k = 0; k < w; ++k) {
-      double yk = sh[k * Ct + tid] / Ls[(long long)k * m + k];
-      sh[k * Ct + tid] = yk;
-      for (int i = k + 1; i < w; ++i) sh[i * Ct + tid] -= Ls[(long long)k * m + i] * yk;
+  for (int k0 = 0; k0 < w; k0 += kDiagBlock) {
+    int bs = min(kDiagBlock, w - k0);
+    for (int i = tid; i < bs * ct; i += nt) {
+      int r = i / ct, c = i % ct;
+      const double* dr = D + r * bs;
+      double acc = 0.0;
+      for (int j = 0; j <= r; ++j) acc += dr[j] * sh[(k0 + j) * Ct + c];
+      yb[r * Ct + c] = acc;
     }
+    __syncthreads();
+    for (int i = tid; i < bs * ct; i += nt) {
+      int r = i / ct, c = i % ct;
+      sh[(k0 + r) * Ct + c] = yb[r * Ct + c];
+    }
+    int rem = m - k0 - bs;
+    for (int i = tid; i < rem * ct; i += nt) {
+      int row = k0 + bs + i / ct, c = i % ct;
+      double acc = 0.0;
+      for (int j = 0; j < bs; ++j) acc += Ls[(long long)(k0 + j) * m + row] * yb[j * Ct + c];
+      sh[row * Ct + c] -= acc;
+    }
+    D += bs * bs;
+    __syncthreads();
   }
-  __syncthreads();
   for (int i = tid; i < w * ct; i += nt) {
     int row = i / ct, c = i % ct;
     B[(long long)(c0 + row) * C + col0 + c] = sh[row * Ct + c];
@@ -270,15 +291,13 @@ __global__ __launch_bounds__(256) void k_fwd_level(FactorView f, const int* __re
     double* Us = U + f.uptr[s] * C;
     for (int i = tid; i < mb * ct; i += nt) {
       int a = i / ct, c = i % ct;
-      double acc = sh[(w + a) * Ct + c];
-      for (int k = 0; k < w; ++k) acc -= Ls[(long long)k * m + w + a] * sh[k * Ct + c];
-      Us[(long long)a * C + col0 + c] = acc;
+      Us[(long long)a * C + col0 + c] = sh[(w + a) * Ct + c];
     }
   }
 }
 
-// Backward substitution (L^T x = y) of one level: t = y_s - L_below^T x_below,
-// then the transposed triangular solve of the diagonal block.
+// Backward substitution (L^T x = y) of one level, blocked from the last
+// diagonal block up: t_b = y_b - sum_{rows below} L_ib^T x_i, x_b = inv(L_bb)^T t_b.
 __global__ __launch_bounds__(256) void k_bwd_level(FactorView f, const int* __restrict__ lev, int ntile, int Ct, int C,
                                                    double* __restrict__ B) {
   extern __shared__ double sh[];
@@ -289,6 +308,7 @@ __global__ __launch_bounds__(256) void k_bwd_level(FactorView f, const int* __re
   int w = f.first[s + 1] - c0;
   long long rp = f.rowptr[s];
   int m = (int)(f.rowptr[s + 1] - rp);
+  double* tb = sh + (long long)m * Ct;
   const int* rows = f.rows + rp;
   const double* Ls = f.L + f.lptr[s];
   int tid = threadIdx.x, nt = blockDim.x;
@@ -297,26 +317,29 @@ __global__ __launch_bounds__(256) void k_bwd_level(FactorView f, const int* __re
     sh[row * Ct + c] = B[(long long)rows[row] * C + col0 + c];
   }
   __syncthreads();
-  int mb = m - w;
-  if (mb > 0) {
-    for (int i = tid; i < w * ct; i += nt) {
-      int k = i / ct, c = i % ct;
-      double acc = sh[k * Ct + c];
-      const double* lk = Ls + (long long)k * m + w;
-      for (int a = 0; a < mb; ++a) acc -= lk[a] * sh[(w + a) * Ct + c];
-      sh[k * Ct + c] = acc;
+  int nblk = (w + kDiagBlock - 1) / kDiagBlock;
+  long long doff = f.dptr[s + 1];
+  for (int bi = nblk - 1; bi >= 0; --bi) {
+    int k0 = bi * kDiagBlock;
+    int bs = min(kDiagBlock, w - k0);
+    doff -= bs * bs;
+    const double* D = f.Dinv + doff;
+    for (int i = tid; i < bs * ct; i += nt) {
+      int j = i / ct, c = i % ct;
+      const double* lj = Ls + (long long)(k0 + j) * m;
+      double acc = sh[(k0 + j) * Ct + c];
+      for (int row = k0 + bs; row < m; ++row) acc -= lj[row] * sh[row * Ct + c];
+      tb[j * Ct + c] = acc;
+    }
+    __syncthreads();
+    for (int i = tid; i < bs * ct; i += nt) {
+      int j = i / ct, c = i % ct;
+      double acc = 0.0;
+      for (int r = j; r < bs; ++r) acc += D[r * bs + j] * tb[r * Ct + c];
+      sh[(k0 + j) * Ct + c] = acc;
     }
     __syncthreads();
   }
-  if (tid < ct) {
-    for (int k = w - 1; k >= 0; --k) {
-      const double* lk = Ls + (long long)k * m;
-      double acc = sh[k * Ct + tid];
-      for (int i = k + 1; i < w; ++i) acc -= lk[i] * sh[i * Ct + tid];
-      sh[k * Ct + tid] = acc / lk[k];
-    }
-  }
-  __syncthreads();
   for (int i = tid; i < w * ct; i += nt) {
     int row = i / ct, c = i % ct;
     B[(long long)(c0 + row) * C + col0 + c] = sh[row * Ct + c];
@@ -604,6 +627,253 @@ __global__ void k_reduce_rows(const double* __restrict__ part, const int* __rest
   out[tet_order[t]] = s;
 }
 
+
+// ------------------------------------------------------------ contact phase
+// Static-obstacle CCD of a vertex moving x0 -> x1 (arcsim collision_test,
+// collision.cpp:211-278, for a static face): an impact when the path crosses
+// the surface in [0,1] or ends within the thickness 1e-4; the contact normal
+// opposes the motion (collision.cpp:269-273).
+constexpr double kThick = 1e-4;
+__device__ bool detect_plane_d(const double* pl, const double* x0, const double* x1, double* n) {
+  double nn = sqrt(pl[3] * pl[3] + pl[4] * pl[4] + pl[5] * pl[5]);
+  double u[3] = {pl[3] / nn, pl[4] / nn, pl[5] / nn};
+  double ds = 0, de = 0, vn = 0;
+  for (int r = 0; r < 3; ++r) {
+    ds += (x0[r] - pl[r]) * u[r];
+    de += (x1[r] - pl[r]) * u[r];
+    vn += (x1[r] - x0[r]) * u[r];
+  }
+  bool hit = false;
+  if (ds != de) {
+    double t = ds / (ds - de);
+    hit = t >= 0 && t <= 1;
+  }
+  if (!hit) hit = fabs(de) < kThick;
+  if (!hit) return false;
+  double sg = vn > 0 ? -1.0 : 1.0;
+  for (int r = 0; r < 3; ++r) n[r] = sg * u[r];
+  return true;
+}
+
+__device__ bool detect_sphere_d(const double* sp, const double* x0, const double* x1, double* n) {
+  double d[3], e[3];
+  for (int r = 0; r < 3; ++r) {
+    d[r] = x1[r] - x0[r];
+    e[r] = x0[r] - sp[r];
+  }
+  double a = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+  double b = 2 * (d[0] * e[0] + d[1] * e[1] + d[2] * e[2]);
+  double c = e[0] * e[0] + e[1] * e[1] + e[2] * e[2] - sp[3] * sp[3];
+  double ts[3];
+  int nt = 0;
+  if (a > 0) {
+    double disc = b * b - 4 * a * c;
+    if (disc >= 0) {
+      double sq = sqrt(disc);
+      double q = -0.5 * (b + (b >= 0 ? sq : -sq));
+      double r1 = q / a, r2 = q != 0 ? c / q : r1;
+      if (r1 > r2) {
+        double t = r1; r1 = r2; r2 = t;
+      }
+      ts[nt++] = r1;
+      ts[nt++] = r2;
+    }
+  }
+  ts[nt++] = 1.0;
+  for (int k = 0; k < nt; ++k) {
+    double t = ts[k];
+    if (t < 0 || t > 1) continue;
+    double p[3];
+    for (int r = 0; r < 3; ++r) p[r] = x0[r] + t * d[r] - sp[r];
+    double len = sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+    if (fabs(len - sp[3]) < kThick && len > 0) {
+      double vn = (p[0] * d[0] + p[1] * d[1] + p[2] * d[2]) / len;
+      double sg = vn > 0 ? -1.0 : 1.0;
+      for (int r = 0; r < 3; ++r) n[r] = sg * p[r] / len;
+      return true;
+    }
+  }
+  return false;
+}
+
+// Detection pass for (contact vertex, rollout): add or refresh impacts keyed
+// by (vertex, obstacle) (collisionBrutal.h:312-327); the insertion stamp
+// orders impacts of one vertex like the reference's impact list.
+__global__ void k_contact_detect(const int* __restrict__ cvtx, int ncv, const double* __restrict__ Xn,
+                                 const double* __restrict__ Ori, int R, const double* __restrict__ planes, int np,
+                                 const double* __restrict__ spheres, int nsph, int* __restrict__ imp_stamp,
+                                 double* __restrict__ imp_n, int pass, const int* __restrict__ cactive,
+                                 int* __restrict__ fresh) {
+  long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx >= (long long)ncv * R) return;
+  int ci = (int)(idx / R), r = (int)(idx % R);
+  if (!cactive[r]) return;
+  int C = 3 * R;
+  int v = cvtx[ci];
+  const double* x0 = Xn + (long long)v * C + 3 * r;
+  const double* x1 = Ori + (long long)v * C + 3 * r;
+  int nobs = np + nsph;
+  bool any = false;
+  for (int o = 0; o < nobs; ++o) {
+    double n[3];
+    bool hit = o < np ? detect_plane_d(planes + 6 * o, x0, x1, n) : detect_sphere_d(spheres + 4 * (o - np), x0, x1, n);
+    if (!hit) continue;
+    any = true;
+    long long k = ((long long)ci * nobs + o) * R + r;
+    if (imp_stamp[k] < 0) imp_stamp[k] = pass * nobs + o;
+    imp_n[3 * k] = n[0];
+    imp_n[3 * k + 1] = n[1];
+    imp_n[3 * k + 2] = n[2];
+  }
+  if (any) atomicOr(&fresh[r], 1);
+}
+
+__global__ void k_contact_reset(int* __restrict__ imp_stamp, long long n, int* __restrict__ cactive,
+                                int* __restrict__ cfirst, int* __restrict__ fresh, int R) {
+  long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx < n) imp_stamp[idx] = -1;
+  if (idx < R) {
+    cactive[idx] = 1;
+    cfirst[idx] = 1;
+    fresh[idx] = 0;
+  }
+}
+
+// Rollouts with no fresh impact leave the contact phase; the ones that stay
+// start an inner Jacobi pass (World.h:938-944).
+__global__ void k_contact_pass_begin(int* __restrict__ cactive, int* __restrict__ fresh, int* __restrict__ iactive,
+                                     int* __restrict__ npass, int R) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= R) return;
+  int a = cactive[r] && fresh[r];
+  cactive[r] = a;
+  iactive[r] = a;
+  npass[r] += a;
+  fresh[r] = 0;
+}
+
+// Start of an inner Jacobi iteration: ori_x = x_it, err = max|ori_x - ori_x_old|
+// (World.h:945-955). Partial maxima per (chunk, column).
+__global__ void k_contact_begin(const double* __restrict__ Xit, double* __restrict__ Ori,
+                                const int* __restrict__ iactive, double* __restrict__ part, int N, int C) {
+  __shared__ double red[4][64];
+  int col = blockIdx.x * 64 + threadIdx.x;
+  int chunk = blockIdx.y;
+  double mx = 0.0;
+  if (col < C && iactive[col / 3]) {
+    int vend = min(N, (chunk + 1) * kChunk);
+    for (int v = chunk * kChunk + threadIdx.y; v < vend; v += 4) {
+      long long i = (long long)v * C + col;
+      double x = Xit[i];
+      mx = fmax(mx, fabs(x - Ori[i]));
+      Ori[i] = x;
+    }
+  }
+  red[threadIdx.y][threadIdx.x] = mx;
+  __syncthreads();
+  if (threadIdx.y == 0 && col < C)
+    part[(long long)chunk * C + col] =
+        fmax(fmax(red[0][threadIdx.x], red[1][threadIdx.x]), fmax(red[2][threadIdx.x], red[3][threadIdx.x]));
+}
+
+__global__ void k_contact_err(const double* __restrict__ part, int nchunk, int R, int* __restrict__ iactive,
+                              int* __restrict__ cfirst, int* __restrict__ citers) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= R || !iactive[r]) return;
+  int C = 3 * R;
+  double m = 0.0;
+  for (int ch = 0; ch < nchunk; ++ch)
+    for (int c = 0; c < 3; ++c) m = fmax(m, part[(long long)ch * C + 3 * r + c]);
+  bool go = cfirst[r] || !(m < kEps);
+  cfirst[r] = 0;
+  iactive[r] = go ? 1 : 0;
+  citers[r] += go ? 1 : 0;
+}
+
+// One Jacobi update of the contact phase per (vertex, rollout), World.h:957-1016:
+//   rhs = h (c + J d(ori) - C (ori - x_n))  with C = offdiag(M/h^2 + L),
+//   contact vertices: rhs replaced by the cone-projected reaction B r
+//   (World.h:1363-1436), prev = alpha rhs + (1-alpha) prev,
+//   x_it = x_n + diag(M/h^2 + L)^-1 prev / h.
+__global__ void k_contact_jacobi(const int* __restrict__ Lptr, const int* __restrict__ Lcol,
+                                 const double* __restrict__ Lval, const double* __restrict__ B,
+                                 const double* __restrict__ Ori, const double* __restrict__ Xn,
+                                 const double* __restrict__ Kdiag, const int* __restrict__ cv_of_v,
+                                 const int* __restrict__ imp_stamp, const double* __restrict__ imp_n, int nobs,
+                                 double mu, double alpha, double h, int first_iter, const int* __restrict__ iactive,
+                                 double* __restrict__ Prev, double* __restrict__ Xit, int N, int R) {
+  long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx >= (long long)N * R) return;
+  int v = (int)(idx / R), r = (int)(idx % R);
+  if (!iactive[r]) return;
+  long long C = 3LL * R;
+  long long base = (long long)v * C + 3 * r;
+  double rc[3] = {B[base], B[base + 1], B[base + 2]};
+  for (int q = Lptr[v]; q < Lptr[v + 1]; ++q) {
+    int u = Lcol[q];
+    if (u == v) continue;
+    long long ub = (long long)u * C + 3 * r;
+    double w = Lval[q];
+    rc[0] -= w * (Ori[ub] - Xn[ub]);
+    rc[1] -= w * (Ori[ub + 1] - Xn[ub + 1]);
+    rc[2] -= w * (Ori[ub + 2] - Xn[ub + 2]);
+  }
+  rc[0] *= h;
+  rc[1] *= h;
+  rc[2] *= h;
+  double pv[3];
+  if (first_iter) {
+    pv[0] = rc[0]; pv[1] = rc[1]; pv[2] = rc[2];
+  } else {
+    pv[0] = Prev[base]; pv[1] = Prev[base + 1]; pv[2] = Prev[base + 2];
+  }
+  int ci = cv_of_v[v];
+  if (ci >= 0) {
+    int best = -1, bst = -1;
+    for (int o = 0; o < nobs; ++o) {
+      int st = imp_stamp[((long long)ci * nobs + o) * R + r];
+      if (st > bst) {
+        bst = st;
+        best = o;
+      }
+    }
+    if (best >= 0) {
+      const double* n = imp_n + 3 * (((long long)ci * nobs + best) * R + r);
+      // computeBasis (collisionBrutal.h:408-426): [n, t, b]
+      double t[3] = {0.0, n[2], -n[1]};
+      double tn = sqrt(t[1] * t[1] + t[2] * t[2]);
+      if (tn < 1e-15) {
+        t[0] = -n[2]; t[1] = 0.0; t[2] = n[0];
+        tn = sqrt(t[0] * t[0] + t[2] * t[2]);
+      }
+      t[0] /= tn; t[1] /= tn; t[2] /= tn;
+      double b[3] = {n[1] * t[2] - n[2] * t[1], n[2] * t[0] - n[0] * t[2], n[0] * t[1] - n[1] * t[0]};
+      double bn = sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+      double nn = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+      double e0[3] = {n[0] / nn, n[1] / nn, n[2] / nn};
+      double e2[3] = {b[0] / bn, b[1] / bn, b[2] / bn};
+      double f0 = e0[0] * rc[0] + e0[1] * rc[1] + e0[2] * rc[2];
+      if (!(f0 > 0.0)) {
+        double f1 = t[0] * rc[0] + t[1] * rc[1] + t[2] * rc[2];
+        double f2 = e2[0] * rc[0] + e2[1] * rc[1] + e2[2] * rc[2];
+        double r0 = -f0, r1 = -f1, r2 = -f2;
+        double rt = sqrt(r1 * r1 + r2 * r2);
+        if (rt > mu * r0) {
+          r1 *= mu * r0 / rt;
+          r2 *= mu * r0 / rt;
+        }
+        for (int c = 0; c < 3; ++c) rc[c] = e0[c] * r0 + t[c] * r1 + e2[c] * r2;
+      }
+    }
+  }
+  double kd = Kdiag[v];
+  for (int c = 0; c < 3; ++c) {
+    double p = rc[c] * alpha + pv[c] * (1.0 - alpha);
+    Prev[base + c] = p;
+    Xit[base + c] = Xn[base + c] + (p / h) / kd;
+  }
+}
+
 // ------------------------------------------------------------------ engine
 template <typename X>
 struct DevBuf {
@@ -667,7 +937,8 @@ struct pdm_scene_s {
   // factor
   DevBuf<int> f_first, f_rows, f_child_ptr, f_child, f_rel, f_lev;
   DevBuf<long long> f_rowptr, f_lptr, f_relptr, f_uptr;
-  DevBuf<double> f_L;
+  DevBuf<double> f_L, f_Dinv;
+  DevBuf<long long> f_dptr;
   std::vector<int> lev_ptr, lev_maxm;
   // dynamic
   DevBuf<double> X, V, Xn, Vs, Cc, Xit, B, Lx, U, part, err;
@@ -683,6 +954,11 @@ struct pdm_scene_s {
   bool have_tape = false;
   int tape_steps = 0;
   bool act_null = true;
+  // contact
+  int ncv = 0, nobs = 0;
+  DevBuf<int> cvtx, cv_of_v, imp_stamp, cactive, iactive, cfirst, fresh, npass, citers;
+  DevBuf<double> Kdiag, d_planes, d_spheres, Ori, Prev, imp_n;
+  std::vector<int> npass_host, citers_host;  // R*steps of the last forward
   // adjoint buffers
   DevBuf<double> gX, gV, Yb, Xb, Zs, Zb, LZ, gEpart, gMpart, gact;
   DevBuf<int> ractive;
@@ -690,7 +966,7 @@ struct pdm_scene_s {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
   FactorView view() const {
-    return FactorView{f_first.p, f_rowptr.p, f_rows.p, f_lptr.p, f_L.p, f_child_ptr.p, f_child.p, f_relptr.p, f_rel.p,
+    return FactorView{f_dptr.p, f_Dinv.p, f_first.p, f_rowptr.p, f_rows.p, f_lptr.p, f_L.p, f_child_ptr.p, f_child.p, f_relptr.p, f_rel.p,
                       f_uptr.p};
   }
   size_t NC() const { return (size_t)S.N * C; }
@@ -711,6 +987,17 @@ int upload_static(pdm_scene s) {
   for (size_t k = 0; k < S.att_idx.size(); ++k) aw[S.att_idx[k]] += S.att_w[k];
   if (s->attw.upload(aw.data(), S.N)) return 1;
   if (s->xrest.upload(S.xrest.data(), 3 * (size_t)S.N)) return 1;
+  s->nobs = (int)(S.planes.size() / 6 + S.spheres.size() / 4);
+  s->ncv = (S.contact && s->nobs > 0) ? (int)S.contact_vtx.size() : 0;
+  if (s->ncv > 0) {
+    std::vector<int> cvo(S.N, -1);
+    for (int k = 0; k < s->ncv; ++k) cvo[S.contact_vtx[k]] = k;
+    if (s->cvtx.upload(S.contact_vtx.data(), s->ncv) || s->cv_of_v.upload(cvo.data(), S.N)) return 1;
+    std::vector<double> pl(S.planes), sp(S.spheres);
+    pl.push_back(0);
+    sp.push_back(0);
+    if (s->d_planes.upload(pl.data(), pl.size()) || s->d_spheres.upload(sp.data(), sp.size())) return 1;
+  }
   if (S.M > 0) {
     if (s->mv.upload(soa(S.mv, S.M, 4).data(), 4 * (size_t)S.M)) return 1;
     if (s->mg.upload(soa(S.mg, S.M, 4).data(), 4 * (size_t)S.M)) return 1;
@@ -739,6 +1026,13 @@ int upload_material(pdm_scene s) {
   }
   if (s->dEr.upload(der.data(), S.T)) return 1;
   if (s->dEv.upload(dev.data(), S.T)) return 1;
+  std::vector<double> kd(S.N);
+  for (int i = 0; i < S.N; ++i) {
+    kd[i] = S.mass[i] / (S.h * S.h);
+    for (int q = s->Ls.ptr[i]; q < s->Ls.ptr[i + 1]; ++q)
+      if (s->Ls.col[q] == i) kd[i] += s->Ls.val[q];
+  }
+  if (s->Kdiag.upload(kd.data(), S.N)) return 1;
   if (s->Lptr.upload(s->Ls.ptr.data(), s->Ls.ptr.size())) return 1;
   if (s->Lcol.upload(s->Ls.col.data(), s->Ls.col.size())) return 1;
   if (s->Lval.upload(s->Ls.val.data(), s->Ls.val.size())) return 1;
@@ -762,6 +1056,9 @@ int upload_factor(pdm_scene s) {
   tmp.assign(F.uptr.begin(), F.uptr.end());
   if (s->f_uptr.upload(tmp.data(), tmp.size())) return 1;
   if (s->f_L.upload(F.L.data(), F.L.size())) return 1;
+  if (s->f_Dinv.upload(F.Dinv.data(), F.Dinv.size())) return 1;
+  tmp.assign(F.dptr.begin(), F.dptr.end());
+  if (s->f_dptr.upload(tmp.data(), tmp.size())) return 1;
   int nlev = (int)F.level_ptr.size() - 1;
   s->lev_ptr = F.level_ptr;
   s->lev_maxm.assign(nlev, 0);
@@ -826,10 +1123,6 @@ int host_setup(const pdm_scene_desc* d, pdm_scene s) {
   S.mu = d->mu;
   S.planes.assign(d->planes, d->planes + 6 * (size_t)d->n_planes);
   S.spheres.assign(d->spheres, d->spheres + 4 * (size_t)d->n_spheres);
-  if (S.contact && (d->n_planes + d->n_spheres) > 0) {
-    g_err = "contact with obstacles is not available in this build of the engine";
-    return 1;
-  }
   if (!(S.h > 0) || S.max_iter < 0) {
     g_err = "invalid stepping parameters";
     return 1;
@@ -882,6 +1175,13 @@ int ensure_batch_buffers(pdm_scene s) {
   if (s->corner.alloc((size_t)s->Cn.nslot * s->C * esz)) return 1;
   if (s->part.alloc((size_t)cdiv(N, kChunk) * s->C)) return 1;
   if (s->err.alloc(s->R) || s->active.alloc(s->R) || s->iters_dev.alloc(s->R)) return 1;
+  if (s->ncv > 0) {
+    size_t ni = (size_t)s->ncv * s->nobs * s->R;
+    if (s->Ori.alloc(nc) || s->Prev.alloc(nc) || s->imp_stamp.alloc(ni) || s->imp_n.alloc(3 * ni) ||
+        s->cactive.alloc(s->R) || s->iactive.alloc(s->R) || s->cfirst.alloc(s->R) || s->fresh.alloc(s->R) ||
+        s->npass.alloc(s->R) || s->citers.alloc(s->R))
+      return 1;
+  }
   return 0;
 }
 
@@ -889,51 +1189,98 @@ int launch_solve(pdm_scene s, double* Bp) {
   FactorView fv = s->view();
   int nlev = (int)s->lev_ptr.size() - 1;
   const size_t kLds = 64 * 1024;
-  for (int l = 0; l < nlev; ++l) {
-    int cnt = s->lev_ptr[l + 1] - s->lev_ptr[l];
-    int m = std::max(1, s->lev_maxm[l]);
-    int Ct = (int)std::min<size_t>((size_t)s->C, std::max<size_t>(1, kLds / (8 * (size_t)m)));
-    if ((size_t)m * Ct * 8 > 160 * 1024) {
-      g_err = "supernode front exceeds LDS";
-      return 1;
+  for (int pass = 0; pass < 2; ++pass) {
+    for (int q = 0; q < nlev; ++q) {
+      int l = pass == 0 ? q : nlev - 1 - q;
+      int cnt = s->lev_ptr[l + 1] - s->lev_ptr[l];
+      size_t rowsl = (size_t)std::max(1, s->lev_maxm[l]) + kDiagBlock;
+      int Ct = (int)std::min<size_t>((size_t)s->C, std::max<size_t>(1, kLds / (8 * rowsl)));
+      if (rowsl * Ct * 8 > kLds) {
+        g_err = "supernode front exceeds the LDS budget";
+        return 1;
+      }
+      int ntile = cdiv(s->C, Ct);
+      if (pass == 0)
+        hipLaunchKernelGGL(k_fwd_level, dim3(cnt * ntile), dim3(kThreads), rowsl * Ct * 8, s->stream, fv,
+                           s->f_lev.p + s->lev_ptr[l], ntile, Ct, s->C, Bp, s->U.p);
+      else
+        hipLaunchKernelGGL(k_bwd_level, dim3(cnt * ntile), dim3(kThreads), rowsl * Ct * 8, s->stream, fv,
+                           s->f_lev.p + s->lev_ptr[l], ntile, Ct, s->C, Bp);
     }
-    int ntile = cdiv(s->C, Ct);
-    hipLaunchKernelGGL(k_fwd_level, dim3(cnt * ntile), dim3(kThreads), (size_t)m * Ct * 8, s->stream, fv,
-                       s->f_lev.p + s->lev_ptr[l], ntile, Ct, s->C, Bp, s->U.p);
-  }
-  for (int l = nlev - 1; l >= 0; --l) {
-    int cnt = s->lev_ptr[l + 1] - s->lev_ptr[l];
-    int m = std::max(1, s->lev_maxm[l]);
-    int Ct = (int)std::min<size_t>((size_t)s->C, std::max<size_t>(1, kLds / (8 * (size_t)m)));
-    int ntile = cdiv(s->C, Ct);
-    hipLaunchKernelGGL(k_bwd_level, dim3(cnt * ntile), dim3(kThreads), (size_t)m * Ct * 8, s->stream, fv,
-                       s->f_lev.p + s->lev_ptr[l], ntile, Ct, s->C, Bp);
   }
   HIPX(hipGetLastError());
   return 0;
 }
 
 template <typename T>
-void launch_local(pdm_scene s, const double* Xsrc, const double* act, int act_steps, int step) {
+void launch_local(pdm_scene s, const double* Xsrc, const double* act, int act_steps, int step, const int* mask) {
   HostScene& S = s->S;
   T* corner = (T*)s->corner.p;
   long long nt = (long long)S.T * s->R;
   hipLaunchKernelGGL(k_local_tet<T>, dim3(cdiv(nt, kThreads)), dim3(kThreads), 0, s->stream, Xsrc, s->tv.p, s->tG.p,
-                     s->wr.p, s->wv.p, s->tslot.p, s->active.p, corner, S.T, s->R);
+                     s->wr.p, s->wv.p, s->tslot.p, mask, corner, S.T, s->R);
   if (S.M > 0) {
     long long nm = (long long)S.M * s->R;
     hipLaunchKernelGGL(k_local_muscle<T>, dim3(cdiv(nm, kThreads)), dim3(kThreads), 0, s->stream, Xsrc, s->mv.p,
                        s->mg.p, s->mw.p, s->mgroup.p, s->mslot.p, act, act_steps * S.n_groups, step * S.n_groups,
-                       s->active.p, corner, S.M, s->R);
+                       mask, corner, S.M, s->R);
   }
   hipLaunchKernelGGL(k_gather<T>, dim3(cdiv(s->NC(), kThreads)), dim3(kThreads), 0, s->stream, s->Cc.p, corner,
                      s->vptr.p, s->B.p, S.N, s->C);
 }
 
+
+// Contact phase of one step (World.h:921-1023) for every rollout that has
+// fresh impacts; the host reads back after each detection pass whether any
+// rollout continues. Rollouts finish with x = ori_x.
+int contact_phase(pdm_scene s, const double* act, int act_steps, int step) {
+  HostScene& S = s->S;
+  int N = S.N, C = s->C, R = s->R;
+  size_t nc = s->NC();
+  int nchunk = cdiv(N, kChunk);
+  dim3 ugrid(cdiv(C, 64), nchunk), ublock(64, 4);
+  int np = (int)(S.planes.size() / 6), nsph = (int)(S.spheres.size() / 4);
+  long long ni = (long long)s->ncv * s->nobs * R;
+  HIPX(hipMemcpyAsync(s->Ori.p, s->Xit.p, nc * 8, hipMemcpyDeviceToDevice, s->stream));
+  hipLaunchKernelGGL(k_contact_reset, dim3(cdiv(std::max<long long>(ni, R), kThreads)), dim3(kThreads), 0, s->stream,
+                     s->imp_stamp.p, ni, s->cactive.p, s->cfirst.p, s->fresh.p, R);
+  std::vector<int> ca(R);
+  for (int pass = 0; pass <= S.outer_cap; ++pass) {
+    hipLaunchKernelGGL(k_contact_detect, dim3(cdiv((long long)s->ncv * R, kThreads)), dim3(kThreads), 0, s->stream,
+                       s->cvtx.p, s->ncv, s->Xn.p, s->Ori.p, R, s->d_planes.p, np, s->d_spheres.p, nsph,
+                       s->imp_stamp.p, s->imp_n.p, pass, s->cactive.p, s->fresh.p);
+    if (pass == S.outer_cap) break;  // cap reached: keep ori_x
+    hipLaunchKernelGGL(k_contact_pass_begin, dim3(cdiv(R, 64)), dim3(64), 0, s->stream, s->cactive.p, s->fresh.p,
+                       s->iactive.p, s->npass.p, R);
+    HIPX(hipMemcpyAsync(ca.data(), s->cactive.p, R * sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    HIPX(hipStreamSynchronize(s->stream));
+    bool any = false;
+    for (int r = 0; r < R; ++r) any |= ca[r] != 0;
+    if (!any) break;
+    for (int i = 0; i < S.max_iter1; ++i) {
+      hipLaunchKernelGGL(k_contact_begin, ugrid, ublock, 0, s->stream, s->Xit.p, s->Ori.p, s->iactive.p, s->part.p, N,
+                         C);
+      hipLaunchKernelGGL(k_contact_err, dim3(cdiv(R, 64)), dim3(64), 0, s->stream, s->part.p, nchunk, R,
+                         s->iactive.p, s->cfirst.p, s->citers.p);
+      if (s->precision == PDM_MIXED)
+        launch_local<float>(s, s->Ori.p, act, act_steps, step, s->iactive.p);
+      else
+        launch_local<double>(s, s->Ori.p, act, act_steps, step, s->iactive.p);
+      hipLaunchKernelGGL(k_contact_jacobi, dim3(cdiv((long long)N * R, kThreads)), dim3(kThreads), 0, s->stream,
+                         s->Lptr.p, s->Lcol.p, s->Lval.p, s->B.p, s->Ori.p, s->Xn.p, s->Kdiag.p, s->cv_of_v.p,
+                         s->imp_stamp.p, s->imp_n.p, s->nobs, S.mu, S.alpha, S.h, i == 0 ? 1 : 0, s->iactive.p,
+                         s->Prev.p, s->Xit.p, N, R);
+    }
+  }
+  HIPX(hipMemcpyAsync(s->Xit.p, s->Ori.p, nc * 8, hipMemcpyDeviceToDevice, s->stream));
+  HIPX(hipGetLastError());
+  return 0;
+}
+
 // One time step on device. tapeY: optional (max_iter+1) x NC buffer for the
 // iterates; tIters: optional R ints receiving the executed iterations.
 int run_step(pdm_scene s, const double* act, int act_steps, int step, double* tapeY, double* tapeXn, double* tapeVs,
-             int* tIters) {
+             int* tIters, int* tNpass = nullptr, int* tCiters = nullptr) {
   HostScene& S = s->S;
   int N = S.N, C = s->C;
   size_t nc = s->NC();
@@ -961,9 +1308,9 @@ int run_step(pdm_scene s, const double* act, int act_steps, int step, double* ta
       if (!any) break;
     }
     if (s->precision == PDM_MIXED)
-      launch_local<float>(s, s->Xit.p, act, act_steps, step);
+      launch_local<float>(s, s->Xit.p, act, act_steps, step, s->active.p);
     else
-      launch_local<double>(s, s->Xit.p, act, act_steps, step);
+      launch_local<double>(s, s->Xit.p, act, act_steps, step, s->active.p);
     if (launch_solve(s, s->B.p)) return 1;
     hipLaunchKernelGGL(k_update, ugrid, ublock, 0, s->stream, s->Xn.p, s->B.p, s->Xit.p, s->active.p, s->part.p,
                        tapeY ? tapeY + (size_t)(it + 1) * nc : nullptr, N, C);
@@ -973,6 +1320,13 @@ int run_step(pdm_scene s, const double* act, int act_steps, int step, double* ta
   if (tapeY && host_exit) {
     // iterations skipped by the host break keep the last iterate
     // (the reverse sweep never reads them: it is bounded by tIters)
+  }
+  if (s->ncv > 0) {
+    HIPX(hipMemsetAsync(s->npass.p, 0, s->R * sizeof(int), s->stream));
+    HIPX(hipMemsetAsync(s->citers.p, 0, s->R * sizeof(int), s->stream));
+    if (contact_phase(s, act, act_steps, step)) return 1;
+    if (tNpass) HIPX(hipMemcpyAsync(tNpass, s->npass.p, s->R * sizeof(int), hipMemcpyDeviceToDevice, s->stream));
+    if (tCiters) HIPX(hipMemcpyAsync(tCiters, s->citers.p, s->R * sizeof(int), hipMemcpyDeviceToDevice, s->stream));
   }
   hipLaunchKernelGGL(k_step_finish, dim3(cdiv(nc, kThreads)), dim3(kThreads), 0, s->stream, s->Xit.p, s->Xn.p, s->X.p,
                      s->V.p, (long long)nc, S.h, S.damping);
@@ -1008,15 +1362,19 @@ int forward_impl(pdm_scene s, int steps, const double* x0, const double* v0, con
     s->act_null = act == nullptr;
   }
   if (s->iters_host.size() < (size_t)s->R * steps) s->iters_host.resize((size_t)s->R * steps);
-  DevBuf<int> it_all;
+  DevBuf<int> it_all, np_all, ci_all;
   if (it_all.alloc((size_t)s->R * std::max(steps, 1))) return 1;
+  if (s->ncv > 0 && (np_all.alloc((size_t)s->R * std::max(steps, 1)) || ci_all.alloc((size_t)s->R * std::max(steps, 1))))
+    return 1;
   for (int n = 0; n < steps; ++n) {
     if (record && n % s->ckpt == 0) {
       size_t k = n / s->ckpt;
       HIPX(hipMemcpyAsync(s->ckX.p + k * nc, s->X.p, nc * 8, hipMemcpyDeviceToDevice, s->stream));
       HIPX(hipMemcpyAsync(s->ckV.p + k * nc, s->V.p, nc * 8, hipMemcpyDeviceToDevice, s->stream));
     }
-    if (run_step(s, act, steps, n, nullptr, nullptr, nullptr, it_all.p + (size_t)n * s->R)) return 1;
+    if (run_step(s, act, steps, n, nullptr, nullptr, nullptr, it_all.p + (size_t)n * s->R,
+                 s->ncv ? np_all.p + (size_t)n * s->R : nullptr, s->ncv ? ci_all.p + (size_t)n * s->R : nullptr))
+      return 1;
   }
   hipLaunchKernelGGL(k_engine_to_caller, dim3(cdiv(nc, kThreads)), dim3(kThreads), 0, s->stream, s->X.p, xT, s->perm.p,
                      N, s->R);
@@ -1025,14 +1383,27 @@ int forward_impl(pdm_scene s, int steps, const double* x0, const double* v0, con
   HIPX(hipEventRecord(s->ev1, s->stream));
   std::vector<int> ih((size_t)s->R * steps);
   if (steps) HIPX(hipMemcpyAsync(ih.data(), it_all.p, ih.size() * sizeof(int), hipMemcpyDeviceToHost, s->stream));
+  std::vector<int> nph(s->ncv && steps ? ih.size() : 0), cih(nph.size());
+  if (!nph.empty()) {
+    HIPX(hipMemcpyAsync(nph.data(), np_all.p, nph.size() * sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    HIPX(hipMemcpyAsync(cih.data(), ci_all.p, cih.size() * sizeof(int), hipMemcpyDeviceToHost, s->stream));
+  }
   HIPX(hipStreamSynchronize(s->stream));
   HIPX(hipGetLastError());
   float ms = 0;
   HIPX(hipEventElapsedTime(&ms, s->ev0, s->ev1));
   s->t_fwd = ms;
   // iters_host layout: [r][step]
+  s->npass_host.assign((size_t)s->R * steps, 0);
+  s->citers_host.assign((size_t)s->R * steps, 0);
   for (int n = 0; n < steps; ++n)
-    for (int r = 0; r < s->R; ++r) s->iters_host[(size_t)r * steps + n] = ih[(size_t)n * s->R + r];
+    for (int r = 0; r < s->R; ++r) {
+      s->iters_host[(size_t)r * steps + n] = ih[(size_t)n * s->R + r];
+      if (!nph.empty()) {
+        s->npass_host[(size_t)r * steps + n] = nph[(size_t)n * s->R + r];
+        s->citers_host[(size_t)r * steps + n] = cih[(size_t)n * s->R + r];
+      }
+    }
   s->last_steps = steps;
   if (record) {
     s->have_tape = true;
@@ -1066,6 +1437,11 @@ int backward_impl(pdm_scene s, const double* gxT, const double* gvT, double* gx0
     g_err = "backward needs a recorded forward (record=1)";
     return 1;
   }
+  for (int v : s->npass_host)
+    if (v > 0) {
+      g_err = "the recorded forward had active contacts: the contact-phase adjoint is not available in this build";
+      return 1;
+    }
   HostScene& S = s->S;
   int N = S.N, C = s->C, R = s->R, steps = s->tape_steps;
   size_t nc = s->NC();
@@ -1421,6 +1797,19 @@ int pdm_get_iterations(pdm_scene s, int* iters) {
   return 0;
 }
 
+int pdm_get_contact_stats(pdm_scene s, int* passes, int* jacobi_iters) {
+  if (!s || !passes || !jacobi_iters) {
+    g_err = "null argument";
+    return 1;
+  }
+  size_t n = (size_t)s->R * s->last_steps;
+  for (size_t k = 0; k < n; ++k) {
+    passes[k] = k < s->npass_host.size() ? s->npass_host[k] : 0;
+    jacobi_iters[k] = k < s->citers_host.size() ? s->citers_host[k] : 0;
+  }
+  return 0;
+}
+
 int pdm_solve_host(pdm_scene s, const double* b, double* x) {
   if (!s || !b || !x) {
     g_err = "null argument";
@@ -1458,9 +1847,9 @@ int pdm_force_host(pdm_scene s, const double* x, const double* act_step, double*
   hipLaunchKernelGGL(k_attach_rhs, dim3(cdiv(n, kThreads)), dim3(kThreads), 0, s->stream, s->attw.p, s->xrest.p,
                      s->Cc.p, s->S.N, s->C);
   if (s->precision == PDM_MIXED)
-    launch_local<float>(s, s->Xit.p, na ? da.p : nullptr, 1, 0);
+    launch_local<float>(s, s->Xit.p, na ? da.p : nullptr, 1, 0, s->active.p);
   else
-    launch_local<double>(s, s->Xit.p, na ? da.p : nullptr, 1, 0);
+    launch_local<double>(s, s->Xit.p, na ? da.p : nullptr, 1, 0, s->active.p);
   hipLaunchKernelGGL(k_engine_to_caller, dim3(cdiv(n, kThreads)), dim3(kThreads), 0, s->stream, s->B.p, dx.p,
                      s->perm.p, s->S.N, s->R);
   HIPX(hipStreamSynchronize(s->stream));

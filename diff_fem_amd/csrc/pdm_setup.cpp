@@ -456,6 +456,16 @@ bool symbolic(const Csr& A, Factor& F, std::string& err) {
   }
   F.first[ns2] = n;
   F.nnz = F.lptr[ns2];
+  F.dptr.assign(ns2 + 1, 0);
+  for (int s = 0; s < ns2; ++s) {
+    int w = F.first[s + 1] - F.first[s];
+    int64_t acc = 0;
+    for (int k0 = 0; k0 < w; k0 += kDiagBlock) {
+      int bs = std::min(kDiagBlock, w - k0);
+      acc += (int64_t)bs * bs;
+    }
+    F.dptr[s + 1] = F.dptr[s] + acc;
+  }
   // children lists
   F.child_ptr.assign(ns2 + 1, 0);
   for (int s = 0; s < ns2; ++s)
@@ -524,6 +534,7 @@ bool symbolic(const Csr& A, Factor& F, std::string& err) {
 bool numeric(const Csr& A, const std::vector<double>& diag_add, Factor& F, std::string& err) {
   int ns = F.ns;
   F.L.assign(F.nnz, 0.0);
+  F.Dinv.assign(F.dptr[ns], 0.0);
   std::vector<std::vector<double>> U(ns);  // update matrices (m-w)^2, row-major lower
   std::vector<int> pos(F.n, -1);
   for (int s = 0; s < ns; ++s) {
@@ -584,6 +595,22 @@ bool numeric(const Csr& A, const std::vector<double>& diag_add, Factor& F, std::
       cj[j] = d;
       double inv = 1.0 / d;
       for (int i = j + 1; i < m; ++i) cj[i] *= inv;
+    }
+    // inverses of the diagonal blocks (used by the blocked device solves)
+    {
+      double* D = &F.Dinv[F.dptr[s]];
+      for (int k0 = 0; k0 < w; k0 += kDiagBlock) {
+        int bs = std::min(kDiagBlock, w - k0);
+        for (int j = 0; j < bs; ++j) {
+          // column j of inv(Lb): forward substitution on e_j
+          for (int r = 0; r < bs; ++r) {
+            double acc = (r == j) ? 1.0 : 0.0;
+            for (int q = j; q < r; ++q) acc -= Ls[(size_t)(k0 + q) * m + (k0 + r)] * D[(size_t)q * bs + j];
+            D[(size_t)r * bs + j] = r < j ? 0.0 : acc / Ls[(size_t)(k0 + r) * m + (k0 + r)];
+          }
+        }
+        D += (size_t)bs * bs;
+      }
     }
     // update matrix U_s = Us - L21 L21^T (lower)
     if (mb > 0) {

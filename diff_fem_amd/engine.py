@@ -170,6 +170,15 @@ class Scene:
             check(_lib.load().pdm_get_iterations(self.h, _iptr(it)))
         return it.reshape(self.R, self._steps)
 
+    def contact_stats(self):
+        """(passes, jacobi_iters), each (R, steps), of the last forward."""
+        n = self.R * self._steps
+        p = np.zeros(n, np.int32)
+        j = np.zeros(n, np.int32)
+        if n:
+            check(_lib.load().pdm_get_contact_stats(self.h, _iptr(p), _iptr(j)))
+        return p.reshape(self.R, self._steps), j.reshape(self.R, self._steps)
+
     def solve_host(self, b):
         b = np.ascontiguousarray(np.asarray(b, np.float64).reshape(self.R, 3 * self.N))
         x = np.zeros_like(b)

@@ -129,6 +129,10 @@ int pdm_backward_host(pdm_scene s, const double* gxT, const double* gvT, double*
 /* Executed PD iterations of the last forward: R*steps ints (host). */
 int pdm_get_iterations(pdm_scene s, int* iters);
 
+/* Contact phase of the last forward: detection passes that found impacts and
+ * executed Jacobi iterations, R*steps ints each (host). */
+int pdm_get_contact_stats(pdm_scene s, int* passes, int* jacobi_iters);
+
 /* Diagnostics on host arrays (caller layout, R rollouts): solve
  * (M/h^2 + L) x = b with the device factor; evaluate J*d at positions x. */
 int pdm_solve_host(pdm_scene s, const double* b, double* x);

@@ -418,7 +418,7 @@ struct Scene {
   int max_iter = 30, max_iter1 = 30;
   int contact = 0;
   double alpha = 1.0, mu = 1.0;
-  int outer_cap = 64;
+  int outer_cap = 16;  // detection passes with a Jacobi loop (the reference has no cap)
   std::vector<Plane> planes;
   std::vector<Sphere> spheres;
   std::vector<int> contact_vtx;  // surface vertices
@@ -808,7 +808,8 @@ void step(Scene& s, double* x, double* v, const double* act, StepTape* tape, int
         if (!found) impacts.push_back(f);
       }
       if (fresh.empty()) break;
-      if (++outer_n > s.outer_cap) break;
+      if (outer_n == s.outer_cap) break;
+      ++outer_n;
       if (tape) tape->contact_active = true;
       std::vector<double> Bs(impacts.size() * 9);
       for (size_t k = 0; k < impacts.size(); ++k) contact_basis(impacts[k].n, &Bs[9 * k]);

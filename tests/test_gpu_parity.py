@@ -142,3 +142,31 @@ def test_gradient_fp64_muscles(eng, oracle_mod):
 
 def test_gradient_mixed_muscles(eng, oracle_mod):
     _grad_case(eng, oracle_mod, small_muscle_scene(), eng.PDM_MIXED, steps=9, R=2, tol_x=1e-5, tol_g=1e-4)
+
+
+def contact_scene(mu=0.5, sphere=False):
+    x, t = scene.box_grid(5, 2, 2, origin=(0, 0, 0.015))
+    sp = np.array([[0.12, 0.05, -0.19, 0.2]]) if sphere else None
+    return scene.SceneDesc(x_rest=x, tets=t, total_mass=2.0, E=np.full(t.shape[0], 8e4), h=0.01, max_iter=10,
+                           gravity=(0, 0, -9.81), contact=True, max_iter1=10, mu=mu,
+                           planes=np.array([[0, 0, 0, 0, 0, 1.0]]), spheres=sp)
+
+
+@pytest.mark.parametrize("sphere", [False, True])
+def test_contact_forward_matches_oracle(eng, oracle_mod, sphere):
+    d = contact_scene(sphere=sphere)
+    R = 2
+    s = eng.Scene(d, R=R, flags=0)
+    o = oracle_mod.OracleScene(d)
+    x0 = np.tile(d.x_rest.ravel(), (R, 1))
+    v0 = np.zeros_like(x0)
+    v0[1, 0::3] = 0.3  # second rollout slides along x
+    steps = 40
+    xT, vT = s.forward_host(x0, v0, steps)
+    passes, jit = s.contact_stats()
+    assert passes.sum() > 0
+    for r in range(R):
+        xo, vo, io = o.rollout(x0[r], v0[r], steps)
+        assert np.array_equal(s.iterations()[r], io[:, 0])
+        assert np.array_equal(passes[r], io[:, 2]) and np.array_equal(jit[r], io[:, 1])
+        assert rel(xT[r], xo) < 1e-9 and rel(vT[r], vo) < 1e-9
