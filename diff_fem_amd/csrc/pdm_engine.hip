@@ -207,6 +207,7 @@ __global__ void k_gather_masked(const T* __restrict__ corner, const int* __restr
 }
 
 struct FactorView {
+  const double* LR;
   const long long* dptr;
   const double* Dinv;
   const int* first;
@@ -221,29 +222,41 @@ struct FactorView {
   const long long* uptr;
 };
 
+// FP64 MFMA 16x16x4 (gfx950): lane l supplies A[l%16][l/16] and B[l/16][l%16];
+// accumulator element i of lane l is D[4i + l/16][l%16].
+typedef double v4d __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ v4d mfma16(double a, double b, v4d c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
 // Forward substitution (L y = b) of one elimination-tree level. Block =
-// (supernode, column tile). The front T (m x Ct) is staged in LDS: own rows
-// from B, children's update blocks added in child order. Then a blocked
-// right-looking solve: for each 16-row diagonal block, y_b = inv(L_bb) T_b,
-// and every row below (diagonal part and update rows) gets T_i -= L_ib y_b.
-// Rows [0,w) are the solution, rows [w,m) the update block U_s for the parent.
+// (supernode, column tile of Ct = 16*nq right-hand sides); 4 waves. The front
+// T (mpad x Ct, mpad = m rounded up to 16) is staged in LDS: own rows from B,
+// children's update blocks added in child order. Then, per 16-row diagonal
+// block: y_b = inv(L_bb) T_b and T_i -= L_ib y_b for every row below, as
+// 16x16 tiles on the matrix cores. Rows [0,w) are the solution, rows [w,m)
+// the update block U_s for the parent.
 __global__ __launch_bounds__(256) void k_fwd_level(FactorView f, const int* __restrict__ lev, int ntile, int Ct, int C,
                                                    double* __restrict__ B, double* __restrict__ U) {
   extern __shared__ double sh[];
-  int s = lev[blockIdx.x / ntile];
-  int col0 = (blockIdx.x % ntile) * Ct;
-  int ct = min(Ct, C - col0);
-  int c0 = f.first[s];
-  int w = f.first[s + 1] - c0;
-  long long rp = f.rowptr[s];
-  int m = (int)(f.rowptr[s + 1] - rp);
-  double* yb = sh + (long long)m * Ct;  // kDiagBlock x Ct scratch
+  const int s = lev[blockIdx.x / ntile];
+  const int col0 = (blockIdx.x % ntile) * Ct;
+  const int ct = min(Ct, C - col0);
+  const int c0 = f.first[s];
+  const int w = f.first[s + 1] - c0;
+  const long long rp = f.rowptr[s];
+  const int m = (int)(f.rowptr[s + 1] - rp);
+  const int mpad = (m + 15) & ~15;
+  double* Y = sh + (long long)mpad * Ct;  // 16 x Ct
   const double* Ls = f.L + f.lptr[s];
   const double* D = f.Dinv + f.dptr[s];
-  int tid = threadIdx.x, nt = blockDim.x;
-  for (int i = tid; i < m * ct; i += nt) {
-    int row = i / ct, c = i % ct;
-    sh[row * Ct + c] = row < w ? B[(long long)(c0 + row) * C + col0 + c] : 0.0;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int nq = Ct >> 4;
+  for (int i = tid; i < mpad * Ct; i += nt) {
+    int row = i / Ct, c = i % Ct;
+    sh[i] = (row < w && c < ct) ? B[(long long)(c0 + row) * C + col0 + c] : 0.0;
   }
   __syncthreads();
   for (int q = f.child_ptr[s]; q < f.child_ptr[s + 1]; ++q) {
@@ -258,35 +271,55 @@ __global__ __launch_bounds__(256) void k_fwd_level(FactorView f, const int* __re
     }
     __syncthreads();
   }
-  for (int k0 = 0; k0 < w; k0 += kDiagBlock) {
-    int bs = min(kDiagBlock, w - k0);
-    for (int i = tid; i < bs * ct; i += nt) {
-      int r = i / ct, c = i % ct;
-      const double* dr = D + r * bs;
-      double acc = 0.0;
-      for (int j = 0; j <= r; ++j) acc += dr[j] * sh[(k0 + j) * Ct + c];
-      yb[r * Ct + c] = acc;
+  for (int k0 = 0; k0 < w; k0 += 16) {
+    const int bs = min(16, w - k0);
+    // y = inv(L_bb) T[k0:k0+16]
+    for (int q = wave; q < nq; q += 4) {
+      v4d acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk) {
+        int k = kk * 4 + lk;
+        double a = (lr < bs && k < bs) ? D[lr * bs + k] : 0.0;
+        double bv = sh[(k0 + k) * Ct + q * 16 + lr];
+        acc = mfma16(a, bv, acc);
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) Y[(4 * i + lk) * Ct + q * 16 + lr] = acc[i];
     }
     __syncthreads();
-    for (int i = tid; i < bs * ct; i += nt) {
-      int r = i / ct, c = i % ct;
-      sh[(k0 + r) * Ct + c] = yb[r * Ct + c];
+    for (int i = tid; i < bs * Ct; i += nt) {
+      int r = i / Ct, c = i % Ct;
+      sh[(k0 + r) * Ct + c] = Y[r * Ct + c];
     }
-    int rem = m - k0 - bs;
-    for (int i = tid; i < rem * ct; i += nt) {
-      int row = k0 + bs + i / ct, c = i % ct;
-      double acc = 0.0;
-      for (int j = 0; j < bs; ++j) acc += Ls[(long long)(k0 + j) * m + row] * yb[j * Ct + c];
-      sh[row * Ct + c] -= acc;
+    // rows below: tiles aligned at k0; rows of the block itself are skipped
+    const int ntr = (mpad - k0) >> 4;
+    const int first_tile = bs == 16 ? 1 : 0;
+    for (int tq = wave + first_tile * nq; tq < ntr * nq; tq += 4) {
+      int rt = k0 + (tq / nq) * 16, q = tq % nq;
+      v4d acc;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) acc[i] = sh[(rt + 4 * i + lk) * Ct + q * 16 + lr];
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk) {
+        int k = kk * 4 + lk;
+        int row = rt + lr;
+        double a = (k < bs && row < m) ? -Ls[(long long)(k0 + k) * m + row] : 0.0;
+        double bv = Y[k * Ct + q * 16 + lr];
+        acc = mfma16(a, bv, acc);
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        int row = rt + 4 * i + lk;
+        if (row >= k0 + bs) sh[row * Ct + q * 16 + lr] = acc[i];
+      }
     }
-    D += bs * bs;
     __syncthreads();
   }
   for (int i = tid; i < w * ct; i += nt) {
     int row = i / ct, c = i % ct;
     B[(long long)(c0 + row) * C + col0 + c] = sh[row * Ct + c];
   }
-  int mb = m - w;
+  const int mb = m - w;
   if (mb > 0) {
     double* Us = U + f.uptr[s] * C;
     for (int i = tid; i < mb * ct; i += nt) {
@@ -296,47 +329,71 @@ __global__ __launch_bounds__(256) void k_fwd_level(FactorView f, const int* __re
   }
 }
 
-// Backward substitution (L^T x = y) of one level, blocked from the last
-// diagonal block up: t_b = y_b - sum_{rows below} L_ib^T x_i, x_b = inv(L_bb)^T t_b.
+// Backward substitution (L^T x = y) of one level, from the last diagonal block
+// up: t_b = y_b - L_below,b^T x_below (rows below already final),
+// x_b = inv(L_bb)^T t_b, both as 16x16 MFMA tiles. The row-major copy LR of
+// each supernode block makes the transposed operand loads contiguous.
 __global__ __launch_bounds__(256) void k_bwd_level(FactorView f, const int* __restrict__ lev, int ntile, int Ct, int C,
                                                    double* __restrict__ B) {
   extern __shared__ double sh[];
-  int s = lev[blockIdx.x / ntile];
-  int col0 = (blockIdx.x % ntile) * Ct;
-  int ct = min(Ct, C - col0);
-  int c0 = f.first[s];
-  int w = f.first[s + 1] - c0;
-  long long rp = f.rowptr[s];
-  int m = (int)(f.rowptr[s + 1] - rp);
-  double* tb = sh + (long long)m * Ct;
+  const int s = lev[blockIdx.x / ntile];
+  const int col0 = (blockIdx.x % ntile) * Ct;
+  const int ct = min(Ct, C - col0);
+  const int c0 = f.first[s];
+  const int w = f.first[s + 1] - c0;
+  const long long rp = f.rowptr[s];
+  const int m = (int)(f.rowptr[s + 1] - rp);
+  const int mpad = (m + 15) & ~15;
+  double* Tb = sh + (long long)mpad * Ct;  // 16 x Ct
   const int* rows = f.rows + rp;
-  const double* Ls = f.L + f.lptr[s];
-  int tid = threadIdx.x, nt = blockDim.x;
-  for (int i = tid; i < m * ct; i += nt) {
-    int row = i / ct, c = i % ct;
-    sh[row * Ct + c] = B[(long long)rows[row] * C + col0 + c];
+  const double* LR = f.LR + f.lptr[s];  // row-major m x w
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int nq = Ct >> 4;
+  for (int i = tid; i < mpad * Ct; i += nt) {
+    int row = i / Ct, c = i % Ct;
+    sh[i] = (row < m && c < ct) ? B[(long long)rows[row] * C + col0 + c] : 0.0;
   }
   __syncthreads();
-  int nblk = (w + kDiagBlock - 1) / kDiagBlock;
+  const int nblk = (w + 15) >> 4;
   long long doff = f.dptr[s + 1];
   for (int bi = nblk - 1; bi >= 0; --bi) {
-    int k0 = bi * kDiagBlock;
-    int bs = min(kDiagBlock, w - k0);
+    const int k0 = bi * 16;
+    const int bs = min(16, w - k0);
     doff -= bs * bs;
     const double* D = f.Dinv + doff;
-    for (int i = tid; i < bs * ct; i += nt) {
-      int j = i / ct, c = i % ct;
-      const double* lj = Ls + (long long)(k0 + j) * m;
-      double acc = sh[(k0 + j) * Ct + c];
-      for (int row = k0 + bs; row < m; ++row) acc -= lj[row] * sh[row * Ct + c];
-      tb[j * Ct + c] = acc;
+    for (int q = wave; q < nq; q += 4) {
+      v4d acc;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        int r = 4 * i + lk;
+        acc[i] = r < bs ? sh[(k0 + r) * Ct + q * 16 + lr] : 0.0;
+      }
+      for (int kb = k0 + bs; kb < m; kb += 4) {
+        int row = kb + lk;
+        double a = (row < m && lr < bs) ? -LR[(long long)row * w + k0 + lr] : 0.0;
+        double bv = row < m ? sh[row * Ct + q * 16 + lr] : 0.0;
+        acc = mfma16(a, bv, acc);
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) Tb[(4 * i + lk) * Ct + q * 16 + lr] = acc[i];
     }
     __syncthreads();
-    for (int i = tid; i < bs * ct; i += nt) {
-      int j = i / ct, c = i % ct;
-      double acc = 0.0;
-      for (int r = j; r < bs; ++r) acc += D[r * bs + j] * tb[r * Ct + c];
-      sh[(k0 + j) * Ct + c] = acc;
+    for (int q = wave; q < nq; q += 4) {
+      v4d acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk) {
+        int k = kk * 4 + lk;
+        double a = (lr < bs && k < bs) ? D[k * bs + lr] : 0.0;  // inv(L_bb)^T
+        double bv = Tb[k * Ct + q * 16 + lr];
+        acc = mfma16(a, bv, acc);
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        int r = 4 * i + lk;
+        if (r < bs) sh[(k0 + r) * Ct + q * 16 + lr] = acc[i];
+      }
     }
     __syncthreads();
   }
@@ -937,7 +994,7 @@ struct pdm_scene_s {
   // factor
   DevBuf<int> f_first, f_rows, f_child_ptr, f_child, f_rel, f_lev;
   DevBuf<long long> f_rowptr, f_lptr, f_relptr, f_uptr;
-  DevBuf<double> f_L, f_Dinv;
+  DevBuf<double> f_L, f_Dinv, f_LR;
   DevBuf<long long> f_dptr;
   std::vector<int> lev_ptr, lev_maxm;
   // dynamic
@@ -966,7 +1023,7 @@ struct pdm_scene_s {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
   FactorView view() const {
-    return FactorView{f_dptr.p, f_Dinv.p, f_first.p, f_rowptr.p, f_rows.p, f_lptr.p, f_L.p, f_child_ptr.p, f_child.p, f_relptr.p, f_rel.p,
+    return FactorView{f_LR.p, f_dptr.p, f_Dinv.p, f_first.p, f_rowptr.p, f_rows.p, f_lptr.p, f_L.p, f_child_ptr.p, f_child.p, f_relptr.p, f_rel.p,
                       f_uptr.p};
   }
   size_t NC() const { return (size_t)S.N * C; }
@@ -1057,6 +1114,18 @@ int upload_factor(pdm_scene s) {
   if (s->f_uptr.upload(tmp.data(), tmp.size())) return 1;
   if (s->f_L.upload(F.L.data(), F.L.size())) return 1;
   if (s->f_Dinv.upload(F.Dinv.data(), F.Dinv.size())) return 1;
+  {
+    std::vector<double> lr(F.L.size());
+    for (int sn = 0; sn < F.ns; ++sn) {
+      int w = F.first[sn + 1] - F.first[sn];
+      int m = (int)(F.rowptr[sn + 1] - F.rowptr[sn]);
+      const double* src = &F.L[F.lptr[sn]];
+      double* dst = &lr[F.lptr[sn]];
+      for (int c = 0; c < w; ++c)
+        for (int r = 0; r < m; ++r) dst[(size_t)r * w + c] = src[(size_t)c * m + r];
+    }
+    if (s->f_LR.upload(lr.data(), lr.size())) return 1;
+  }
   tmp.assign(F.dptr.begin(), F.dptr.end());
   if (s->f_dptr.upload(tmp.data(), tmp.size())) return 1;
   int nlev = (int)F.level_ptr.size() - 1;
@@ -1185,26 +1254,42 @@ int ensure_batch_buffers(pdm_scene s) {
   return 0;
 }
 
+// Column tile per level: Ct = 16*nq right-hand sides, nq <= 4, limited by the
+// LDS budget ((mpad + 16) x Ct doubles) and shrunk while the level has too
+// few (supernode, tile) blocks to fill the GPU.
+int level_tile(pdm_scene s, int l, int* Ct_out) {
+  const size_t kLds = 64 * 1024;
+  int cnt = s->lev_ptr[l + 1] - s->lev_ptr[l];
+  size_t mpad = ((size_t)std::max(1, s->lev_maxm[l]) + 15) & ~(size_t)15;
+  int ncol16 = cdiv(s->C, 16);
+  int nq = 4;
+  while (nq > 1 && (mpad + 16) * 16 * nq * 8 > kLds) nq /= 2;
+  while (nq > 1 && (long long)cnt * cdiv(ncol16, nq) < 512) nq /= 2;
+  if ((mpad + 16) * 16 * nq * 8 > kLds) {
+    g_err = "supernode front exceeds the LDS budget";
+    return 1;
+  }
+  *Ct_out = 16 * nq;
+  return 0;
+}
+
 int launch_solve(pdm_scene s, double* Bp) {
   FactorView fv = s->view();
   int nlev = (int)s->lev_ptr.size() - 1;
-  const size_t kLds = 64 * 1024;
   for (int pass = 0; pass < 2; ++pass) {
     for (int q = 0; q < nlev; ++q) {
       int l = pass == 0 ? q : nlev - 1 - q;
       int cnt = s->lev_ptr[l + 1] - s->lev_ptr[l];
-      size_t rowsl = (size_t)std::max(1, s->lev_maxm[l]) + kDiagBlock;
-      int Ct = (int)std::min<size_t>((size_t)s->C, std::max<size_t>(1, kLds / (8 * rowsl)));
-      if (rowsl * Ct * 8 > kLds) {
-        g_err = "supernode front exceeds the LDS budget";
-        return 1;
-      }
+      int Ct;
+      if (level_tile(s, l, &Ct)) return 1;
+      size_t mpad = ((size_t)std::max(1, s->lev_maxm[l]) + 15) & ~(size_t)15;
+      size_t lds = (mpad + 16) * Ct * 8;
       int ntile = cdiv(s->C, Ct);
       if (pass == 0)
-        hipLaunchKernelGGL(k_fwd_level, dim3(cnt * ntile), dim3(kThreads), rowsl * Ct * 8, s->stream, fv,
+        hipLaunchKernelGGL(k_fwd_level, dim3(cnt * ntile), dim3(kThreads), lds, s->stream, fv,
                            s->f_lev.p + s->lev_ptr[l], ntile, Ct, s->C, Bp, s->U.p);
       else
-        hipLaunchKernelGGL(k_bwd_level, dim3(cnt * ntile), dim3(kThreads), rowsl * Ct * 8, s->stream, fv,
+        hipLaunchKernelGGL(k_bwd_level, dim3(cnt * ntile), dim3(kThreads), lds, s->stream, fv,
                            s->f_lev.p + s->lev_ptr[l], ntile, Ct, s->C, Bp);
     }
   }
