@@ -313,6 +313,7 @@ __global__ __launch_bounds__(256) void k_fwd_level(FactorView f, const int* __re
         if (row >= k0 + bs) sh[row * Ct + q * 16 + lr] = acc[i];
       }
     }
+    D += bs * bs;
     __syncthreads();
   }
   for (int i = tid; i < w * ct; i += nt) {
@@ -1708,12 +1709,27 @@ int pdm_host_solve(const pdm_scene_desc* d, int R, const double* b, double* x) {
           for (int c = 0; c < 3; ++c) Tm[3 * rel[a] + c] += U[ch][3 * a + c];
         std::vector<double>().swap(U[ch]);
       }
-      for (int k = 0; k < w; ++k)
-        for (int c = 0; c < 3; ++c) {
-          double yk = Tm[3 * k + c] / Ls[(size_t)k * m + k];
-          Tm[3 * k + c] = yk;
-          for (int i = k + 1; i < m; ++i) Tm[3 * i + c] -= Ls[(size_t)k * m + i] * yk;
-        }
+      // blocked: y_b = inv(L_bb) T_b, then rows below -= L_ib y_b
+      const double* Db = &F.Dinv[F.dptr[sn]];
+      for (int k0 = 0; k0 < w; k0 += kDiagBlock) {
+        int bs = std::min(kDiagBlock, w - k0);
+        double yb[kDiagBlock * 3];
+        for (int r = 0; r < bs; ++r)
+          for (int c = 0; c < 3; ++c) {
+            double acc = 0.0;
+            for (int k = 0; k < bs; ++k) acc += Db[r * bs + k] * Tm[3 * (k0 + k) + c];
+            yb[3 * r + c] = acc;
+          }
+        for (int r = 0; r < bs; ++r)
+          for (int c = 0; c < 3; ++c) Tm[3 * (k0 + r) + c] = yb[3 * r + c];
+        for (int i = k0 + bs; i < m; ++i)
+          for (int c = 0; c < 3; ++c) {
+            double acc = 0.0;
+            for (int k = 0; k < bs; ++k) acc += Ls[(size_t)(k0 + k) * m + i] * yb[3 * k + c];
+            Tm[3 * i + c] -= acc;
+          }
+        Db += bs * bs;
+      }
       for (int i = 0; i < w; ++i)
         for (int c = 0; c < 3; ++c) y[3 * (c0 + i) + c] = Tm[3 * i + c];
       U[sn].assign(Tm.begin() + 3 * (size_t)w, Tm.end());
@@ -1723,12 +1739,26 @@ int pdm_host_solve(const pdm_scene_desc* d, int R, const double* b, double* x) {
       int m = (int)(F.rowptr[sn + 1] - F.rowptr[sn]);
       const int* rows = &F.rows[F.rowptr[sn]];
       const double* Ls = &F.L[F.lptr[sn]];
-      for (int k = w - 1; k >= 0; --k)
-        for (int c = 0; c < 3; ++c) {
-          double acc = y[3 * (c0 + k) + c];
-          for (int i = k + 1; i < m; ++i) acc -= Ls[(size_t)k * m + i] * y[3 * rows[i] + c];
-          y[3 * (c0 + k) + c] = acc / Ls[(size_t)k * m + k];
-        }
+      int nblk = (w + kDiagBlock - 1) / kDiagBlock;
+      int64_t doff = F.dptr[sn + 1];
+      for (int bi = nblk - 1; bi >= 0; --bi) {
+        int k0 = bi * kDiagBlock, bs = std::min(kDiagBlock, w - k0);
+        doff -= bs * bs;
+        const double* Db = &F.Dinv[doff];
+        double tb[kDiagBlock * 3];
+        for (int j = 0; j < bs; ++j)
+          for (int c = 0; c < 3; ++c) {
+            double acc = y[3 * (c0 + k0 + j) + c];
+            for (int i = k0 + bs; i < m; ++i) acc -= Ls[(size_t)(k0 + j) * m + i] * y[3 * rows[i] + c];
+            tb[3 * j + c] = acc;
+          }
+        for (int j = 0; j < bs; ++j)
+          for (int c = 0; c < 3; ++c) {
+            double acc = 0.0;
+            for (int k = 0; k < bs; ++k) acc += Db[k * bs + j] * tb[3 * k + c];
+            y[3 * (c0 + k0 + j) + c] = acc;
+          }
+      }
     }
     for (int v = 0; v < N; ++v)
       for (int c = 0; c < 3; ++c) x[(size_t)r * 3 * N + 3 * s->S.perm[v] + c] = y[3 * v + c];
