@@ -207,7 +207,6 @@ __global__ void k_gather_masked(const T* __restrict__ corner, const int* __restr
 }
 
 struct FactorView {
-  const double* LR;
   const long long* dptr;
   const double* Dinv;
   const int* first;
@@ -229,13 +228,28 @@ __device__ __forceinline__ v4d mfma16(double a, double b, v4d c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
+// LDS layout of the level kernels (doubles):
+//   T  [mpad][Ct]   the front (mpad = m rounded up to 16)
+//   P  [mpad][16]   the current 16-column panel of L, rows k0..m (row-major)
+//   S  [4][16][Ct]  per-wave partial results / the solved block
+__device__ __forceinline__ void stage_panel(const double* __restrict__ Ls, int m, int k0, int bs, int mpad,
+                                            double* __restrict__ P, int tid, int nt) {
+  // P[(row - k0) * 16 + j] = L[row][k0 + j] for rows k0..mpad (zero outside)
+  int nrow = mpad - k0;
+  for (int i = tid; i < nrow * 16; i += nt) {
+    int j = i / nrow, r = i % nrow;  // consecutive threads: consecutive rows of one column
+    int row = k0 + r;
+    P[r * 16 + j] = (j < bs && row < m) ? Ls[(long long)(k0 + j) * m + row] : 0.0;
+  }
+}
+
 // Forward substitution (L y = b) of one elimination-tree level. Block =
 // (supernode, column tile of Ct = 16*nq right-hand sides); 4 waves. The front
-// T (mpad x Ct, mpad = m rounded up to 16) is staged in LDS: own rows from B,
-// children's update blocks added in child order. Then, per 16-row diagonal
-// block: y_b = inv(L_bb) T_b and T_i -= L_ib y_b for every row below, as
-// 16x16 tiles on the matrix cores. Rows [0,w) are the solution, rows [w,m)
-// the update block U_s for the parent.
+// is staged in LDS: own rows from B, children's update blocks added in child
+// order. Then, per 16-row diagonal block, the panel L[k0:m, k0:k0+16] is
+// staged in LDS, y_b = inv(L_bb) T_b, and every row below gets
+// T_i -= L_ib y_b, as 16x16 tiles on the matrix cores. Rows [0,w) are the
+// solution, rows [w,m) the update block U_s for the parent.
 __global__ __launch_bounds__(256) void k_fwd_level(FactorView f, const int* __restrict__ lev, int ntile, int Ct, int C,
                                                    double* __restrict__ B, double* __restrict__ U) {
   extern __shared__ double sh[];
@@ -247,7 +261,8 @@ __global__ __launch_bounds__(256) void k_fwd_level(FactorView f, const int* __re
   const long long rp = f.rowptr[s];
   const int m = (int)(f.rowptr[s + 1] - rp);
   const int mpad = (m + 15) & ~15;
-  double* Y = sh + (long long)mpad * Ct;  // 16 x Ct
+  double* P = sh + (long long)mpad * Ct;
+  double* Y = P + (long long)mpad * 16;  // 16 x Ct
   const double* Ls = f.L + f.lptr[s];
   const double* D = f.Dinv + f.dptr[s];
   const int tid = threadIdx.x, nt = blockDim.x;
@@ -273,6 +288,7 @@ __global__ __launch_bounds__(256) void k_fwd_level(FactorView f, const int* __re
   }
   for (int k0 = 0; k0 < w; k0 += 16) {
     const int bs = min(16, w - k0);
+    stage_panel(Ls, m, k0, bs, mpad, P, tid, nt);
     // y = inv(L_bb) T[k0:k0+16]
     for (int q = wave; q < nq; q += 4) {
       v4d acc = {0.0, 0.0, 0.0, 0.0};
@@ -295,15 +311,15 @@ __global__ __launch_bounds__(256) void k_fwd_level(FactorView f, const int* __re
     const int ntr = (mpad - k0) >> 4;
     const int first_tile = bs == 16 ? 1 : 0;
     for (int tq = wave + first_tile * nq; tq < ntr * nq; tq += 4) {
-      int rt = k0 + (tq / nq) * 16, q = tq % nq;
+      int tr = tq / nq, q = tq % nq;
+      int rt = k0 + tr * 16;
       v4d acc;
 #pragma unroll
       for (int i = 0; i < 4; ++i) acc[i] = sh[(rt + 4 * i + lk) * Ct + q * 16 + lr];
 #pragma unroll
       for (int kk = 0; kk < 4; ++kk) {
         int k = kk * 4 + lk;
-        int row = rt + lr;
-        double a = (k < bs && row < m) ? -Ls[(long long)(k0 + k) * m + row] : 0.0;
+        double a = -P[(tr * 16 + lr) * 16 + k];
         double bv = Y[k * Ct + q * 16 + lr];
         acc = mfma16(a, bv, acc);
       }
@@ -332,8 +348,9 @@ __global__ __launch_bounds__(256) void k_fwd_level(FactorView f, const int* __re
 
 // Backward substitution (L^T x = y) of one level, from the last diagonal block
 // up: t_b = y_b - L_below,b^T x_below (rows below already final),
-// x_b = inv(L_bb)^T t_b, both as 16x16 MFMA tiles. The row-major copy LR of
-// each supernode block makes the transposed operand loads contiguous.
+// x_b = inv(L_bb)^T t_b, on the matrix cores. The panel is staged in LDS and
+// the contraction over the rows below is split across the waves when the
+// column tile has fewer than 4 subtiles.
 __global__ __launch_bounds__(256) void k_bwd_level(FactorView f, const int* __restrict__ lev, int ntile, int Ct, int C,
                                                    double* __restrict__ B) {
   extern __shared__ double sh[];
@@ -345,13 +362,15 @@ __global__ __launch_bounds__(256) void k_bwd_level(FactorView f, const int* __re
   const long long rp = f.rowptr[s];
   const int m = (int)(f.rowptr[s + 1] - rp);
   const int mpad = (m + 15) & ~15;
-  double* Tb = sh + (long long)mpad * Ct;  // 16 x Ct
+  double* P = sh + (long long)mpad * Ct;
+  double* S = P + (long long)mpad * 16;  // 4 x 16 x Ct
   const int* rows = f.rows + rp;
-  const double* LR = f.LR + f.lptr[s];  // row-major m x w
+  const double* Ls = f.L + f.lptr[s];
   const int tid = threadIdx.x, nt = blockDim.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int lr = lane & 15, lk = lane >> 4;
   const int nq = Ct >> 4;
+  const int ksplit = nq >= 4 ? 1 : 4 / nq;
   for (int i = tid; i < mpad * Ct; i += nt) {
     int row = i / Ct, c = i % Ct;
     sh[i] = (row < m && c < ct) ? B[(long long)rows[row] * C + col0 + c] : 0.0;
@@ -364,21 +383,29 @@ __global__ __launch_bounds__(256) void k_bwd_level(FactorView f, const int* __re
     const int bs = min(16, w - k0);
     doff -= bs * bs;
     const double* D = f.Dinv + doff;
-    for (int q = wave; q < nq; q += 4) {
+    stage_panel(Ls, m, k0, bs, mpad, P, tid, nt);
+    __syncthreads();
+    // partial t over rows [k0+bs, m), split in ksplit chunks of 4-row steps
+    const int kbeg = k0 + bs;
+    const int nsteps = (m - kbeg + 3) >> 2;
+    const int per = (nsteps + ksplit - 1) / ksplit;
+    for (int wq = wave; wq < nq * ksplit; wq += 4) {
+      int q = wq % nq, part = wq / nq;
       v4d acc;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         int r = 4 * i + lk;
-        acc[i] = r < bs ? sh[(k0 + r) * Ct + q * 16 + lr] : 0.0;
+        acc[i] = (part == 0 && r < bs) ? sh[(k0 + r) * Ct + q * 16 + lr] : 0.0;
       }
-      for (int kb = k0 + bs; kb < m; kb += 4) {
-        int row = kb + lk;
-        double a = (row < m && lr < bs) ? -LR[(long long)row * w + k0 + lr] : 0.0;
+      int st0 = part * per, st1 = min(nsteps, st0 + per);
+      for (int st = st0; st < st1; ++st) {
+        int row = kbeg + 4 * st + lk;
+        double a = row < m ? -P[(row - k0) * 16 + lr] : 0.0;
         double bv = row < m ? sh[row * Ct + q * 16 + lr] : 0.0;
         acc = mfma16(a, bv, acc);
       }
 #pragma unroll
-      for (int i = 0; i < 4; ++i) Tb[(4 * i + lk) * Ct + q * 16 + lr] = acc[i];
+      for (int i = 0; i < 4; ++i) S[((long long)part * 16 + 4 * i + lk) * Ct + q * 16 + lr] = acc[i];
     }
     __syncthreads();
     for (int q = wave; q < nq; q += 4) {
@@ -387,7 +414,8 @@ __global__ __launch_bounds__(256) void k_bwd_level(FactorView f, const int* __re
       for (int kk = 0; kk < 4; ++kk) {
         int k = kk * 4 + lk;
         double a = (lr < bs && k < bs) ? D[k * bs + lr] : 0.0;  // inv(L_bb)^T
-        double bv = Tb[k * Ct + q * 16 + lr];
+        double bv = 0.0;
+        for (int p = 0; p < ksplit; ++p) bv += S[((long long)p * 16 + k) * Ct + q * 16 + lr];
         acc = mfma16(a, bv, acc);
       }
 #pragma unroll
@@ -995,7 +1023,7 @@ struct pdm_scene_s {
   // factor
   DevBuf<int> f_first, f_rows, f_child_ptr, f_child, f_rel, f_lev;
   DevBuf<long long> f_rowptr, f_lptr, f_relptr, f_uptr;
-  DevBuf<double> f_L, f_Dinv, f_LR;
+  DevBuf<double> f_L, f_Dinv;
   DevBuf<long long> f_dptr;
   std::vector<int> lev_ptr, lev_maxm;
   // dynamic
@@ -1024,7 +1052,7 @@ struct pdm_scene_s {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
   FactorView view() const {
-    return FactorView{f_LR.p, f_dptr.p, f_Dinv.p, f_first.p, f_rowptr.p, f_rows.p, f_lptr.p, f_L.p, f_child_ptr.p, f_child.p, f_relptr.p, f_rel.p,
+    return FactorView{f_dptr.p, f_Dinv.p, f_first.p, f_rowptr.p, f_rows.p, f_lptr.p, f_L.p, f_child_ptr.p, f_child.p, f_relptr.p, f_rel.p,
                       f_uptr.p};
   }
   size_t NC() const { return (size_t)S.N * C; }
@@ -1115,18 +1143,7 @@ int upload_factor(pdm_scene s) {
   if (s->f_uptr.upload(tmp.data(), tmp.size())) return 1;
   if (s->f_L.upload(F.L.data(), F.L.size())) return 1;
   if (s->f_Dinv.upload(F.Dinv.data(), F.Dinv.size())) return 1;
-  {
-    std::vector<double> lr(F.L.size());
-    for (int sn = 0; sn < F.ns; ++sn) {
-      int w = F.first[sn + 1] - F.first[sn];
-      int m = (int)(F.rowptr[sn + 1] - F.rowptr[sn]);
-      const double* src = &F.L[F.lptr[sn]];
-      double* dst = &lr[F.lptr[sn]];
-      for (int c = 0; c < w; ++c)
-        for (int r = 0; r < m; ++r) dst[(size_t)r * w + c] = src[(size_t)c * m + r];
-    }
-    if (s->f_LR.upload(lr.data(), lr.size())) return 1;
-  }
+
   tmp.assign(F.dptr.begin(), F.dptr.end());
   if (s->f_dptr.upload(tmp.data(), tmp.size())) return 1;
   int nlev = (int)F.level_ptr.size() - 1;
@@ -1255,26 +1272,37 @@ int ensure_batch_buffers(pdm_scene s) {
   return 0;
 }
 
-// Column tile per level: Ct = 16*nq right-hand sides, nq <= 4, limited by the
-// LDS budget ((mpad + 16) x Ct doubles) and shrunk while the level has too
-// few (supernode, tile) blocks to fill the GPU.
-int level_tile(pdm_scene s, int l, int* Ct_out) {
-  const size_t kLds = 64 * 1024;
+// Column tile per level: Ct = 16*nq right-hand sides, nq in {1,2,4}. LDS per
+// block = T (mpad x Ct) + panel (mpad x 16) + scratch (4 x 16 x Ct) doubles;
+// keep it within 64 KB (two blocks per CU) where possible, else up to 160 KB,
+// and shrink nq while the level has too few blocks to fill the GPU.
+constexpr size_t kLdsSoft = 64 * 1024, kLdsMax = 160 * 1024;
+size_t level_lds(size_t mpad, int Ct) { return (mpad * Ct + mpad * 16 + 64 * (size_t)Ct) * 8; }
+
+int level_tile(pdm_scene s, int l, int* Ct_out, size_t* lds_out) {
   int cnt = s->lev_ptr[l + 1] - s->lev_ptr[l];
   size_t mpad = ((size_t)std::max(1, s->lev_maxm[l]) + 15) & ~(size_t)15;
   int ncol16 = cdiv(s->C, 16);
   int nq = 4;
-  while (nq > 1 && (mpad + 16) * 16 * nq * 8 > kLds) nq /= 2;
+  while (nq > 1 && level_lds(mpad, 16 * nq) > kLdsSoft) nq /= 2;
   while (nq > 1 && (long long)cnt * cdiv(ncol16, nq) < 512) nq /= 2;
-  if ((mpad + 16) * 16 * nq * 8 > kLds) {
+  size_t lds = level_lds(mpad, 16 * nq);
+  if (lds > kLdsMax) {
     g_err = "supernode front exceeds the LDS budget";
     return 1;
   }
   *Ct_out = 16 * nq;
+  *lds_out = lds;
   return 0;
 }
 
 int launch_solve(pdm_scene s, double* Bp) {
+  static bool attr_set = false;
+  if (!attr_set) {
+    HIPX(hipFuncSetAttribute((const void*)k_fwd_level, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsMax));
+    HIPX(hipFuncSetAttribute((const void*)k_bwd_level, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsMax));
+    attr_set = true;
+  }
   FactorView fv = s->view();
   int nlev = (int)s->lev_ptr.size() - 1;
   for (int pass = 0; pass < 2; ++pass) {
@@ -1282,9 +1310,8 @@ int launch_solve(pdm_scene s, double* Bp) {
       int l = pass == 0 ? q : nlev - 1 - q;
       int cnt = s->lev_ptr[l + 1] - s->lev_ptr[l];
       int Ct;
-      if (level_tile(s, l, &Ct)) return 1;
-      size_t mpad = ((size_t)std::max(1, s->lev_maxm[l]) + 15) & ~(size_t)15;
-      size_t lds = (mpad + 16) * Ct * 8;
+      size_t lds;
+      if (level_tile(s, l, &Ct, &lds)) return 1;
       int ntile = cdiv(s->C, Ct);
       if (pass == 0)
         hipLaunchKernelGGL(k_fwd_level, dim3(cnt * ntile), dim3(kThreads), lds, s->stream, fv,
