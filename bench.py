@@ -83,7 +83,8 @@ def main():
     R, S = args.rollouts, args.sim_steps
     log("setup: T=%d N=%d R=%d steps=%d" % (d.T, d.N, R, S))
     t0 = time.perf_counter()
-    sc = engine.Scene(d, R=R, precision=engine.PDM_MIXED, flags=0, checkpoint_every=args.ckpt)
+    sc = engine.Scene(d, R=R, precision=engine.PDM_MIXED, flags=engine.PDM_HOST_EARLY_EXIT,
+                      checkpoint_every=args.ckpt)
     info = sc.info()
     log("setup done in %.1fs: %s" % (time.perf_counter() - t0, info))
     dev = torch.device("cuda:0")

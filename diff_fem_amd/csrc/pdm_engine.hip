@@ -234,12 +234,51 @@ __device__ __forceinline__ v4d mfma16(double a, double b, v4d c) {
 //   S  [4][16][Ct]  per-wave partial results / the solved block
 __device__ __forceinline__ void stage_panel(const double* __restrict__ Ls, int m, int k0, int bs, int mpad,
                                             double* __restrict__ P, int tid, int nt) {
-  // P[(row - k0) * 16 + j] = L[row][k0 + j] for rows k0..mpad (zero outside)
-  int nrow = mpad - k0;
-  for (int i = tid; i < nrow * 16; i += nt) {
-    int j = i / nrow, r = i % nrow;  // consecutive threads: consecutive rows of one column
-    int row = k0 + r;
-    P[r * 16 + j] = (j < bs && row < m) ? Ls[(long long)(k0 + j) * m + row] : 0.0;
+  // P[(row - k0) * 16 + j] = L[row][k0 + j] for rows k0..mpad (zero outside).
+  // Loads are issued in batches of 8 per thread so that their latencies overlap.
+  const int nrow = mpad - k0;
+  const int total = nrow * 16;
+  for (int base = tid; base < total; base += nt * 8) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      int i = base + u * nt;
+      int j = i / nrow, r = i % nrow;  // consecutive threads: consecutive rows of one column
+      int row = k0 + r;
+      v[u] = (i < total && j < bs && row < m) ? Ls[(long long)(k0 + j) * m + row] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      int i = base + u * nt;
+      if (i < total) {
+        int j = i / nrow, r = i % nrow;
+        P[r * 16 + j] = v[u];
+      }
+    }
+  }
+}
+
+// Front load: T[row][c] = src(row, c) for row < nvalid, c < ct; zero elsewhere.
+// rows == nullptr: source row c0 + row; else rows[row].
+__device__ __forceinline__ void load_front(const double* __restrict__ B, const int* __restrict__ rows, int c0,
+                                           int nvalid, int mpad, int Ct, int ct, int C, int col0,
+                                           double* __restrict__ T, int tid, int nt) {
+  const int total = mpad * Ct;
+  for (int base = tid; base < total; base += nt * 8) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      int i = base + u * nt;
+      int row = i / Ct, c = i % Ct;
+      bool ok = i < total && row < nvalid && c < ct;
+      long long src = rows ? (long long)(ok ? rows[row] : 0) : (long long)(c0 + row);
+      v[u] = ok ? B[src * C + col0 + c] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      int i = base + u * nt;
+      if (i < total) T[i] = v[u];
+    }
   }
 }
 
@@ -269,10 +308,7 @@ __global__ __launch_bounds__(256) void k_fwd_level(FactorView f, const int* __re
   const int lane = tid & 63, wave = tid >> 6;
   const int lr = lane & 15, lk = lane >> 4;
   const int nq = Ct >> 4;
-  for (int i = tid; i < mpad * Ct; i += nt) {
-    int row = i / Ct, c = i % Ct;
-    sh[i] = (row < w && c < ct) ? B[(long long)(c0 + row) * C + col0 + c] : 0.0;
-  }
+  load_front(B, nullptr, c0, w, mpad, Ct, ct, C, col0, sh, tid, nt);
   __syncthreads();
   for (int q = f.child_ptr[s]; q < f.child_ptr[s + 1]; ++q) {
     int ch = f.child[q];
@@ -280,9 +316,23 @@ __global__ __launch_bounds__(256) void k_fwd_level(FactorView f, const int* __re
     int mbc = (int)(f.rowptr[ch + 1] - f.rowptr[ch]) - wc;
     const int* rel = f.rel + f.relptr[ch];
     const double* Uc = U + f.uptr[ch] * C;
-    for (int i = tid; i < mbc * ct; i += nt) {
-      int a = i / ct, c = i % ct;
-      sh[rel[a] * Ct + c] += Uc[(long long)a * C + col0 + c];
+    const int total = mbc * ct;
+    for (int base = tid; base < total; base += nt * 8) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        int i = base + u * nt;
+        int a = i / ct, c = i % ct;
+        v[u] = i < total ? Uc[(long long)a * C + col0 + c] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        int i = base + u * nt;
+        if (i < total) {
+          int a = i / ct, c = i % ct;
+          sh[rel[a] * Ct + c] += v[u];
+        }
+      }
     }
     __syncthreads();
   }
@@ -371,10 +421,7 @@ __global__ __launch_bounds__(256) void k_bwd_level(FactorView f, const int* __re
   const int lr = lane & 15, lk = lane >> 4;
   const int nq = Ct >> 4;
   const int ksplit = nq >= 4 ? 1 : 4 / nq;
-  for (int i = tid; i < mpad * Ct; i += nt) {
-    int row = i / Ct, c = i % Ct;
-    sh[i] = (row < m && c < ct) ? B[(long long)rows[row] * C + col0 + c] : 0.0;
-  }
+  load_front(B, rows, 0, m, mpad, Ct, ct, C, col0, sh, tid, nt);
   __syncthreads();
   const int nblk = (w + 15) >> 4;
   long long doff = f.dptr[s + 1];
