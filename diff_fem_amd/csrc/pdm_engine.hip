@@ -228,42 +228,13 @@ __device__ __forceinline__ v4d mfma16(double a, double b, v4d c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
-// LDS layout of the level kernels (doubles):
-//   T  [mpad][Ct]   the front (mpad = m rounded up to 16)
-//   P  [mpad][16]   the current 16-column panel of L, rows k0..m (row-major)
-//   S  [4][16][Ct]  per-wave partial results / the solved block
-__device__ __forceinline__ void stage_panel(const double* __restrict__ Ls, int m, int k0, int bs, int mpad,
-                                            double* __restrict__ P, int tid, int nt) {
-  // P[(row - k0) * 16 + j] = L[row][k0 + j] for rows k0..mpad (zero outside).
-  // Loads are issued in batches of 8 per thread so that their latencies overlap.
-  const int nrow = mpad - k0;
-  const int total = nrow * 16;
-  for (int base = tid; base < total; base += nt * 8) {
-    double v[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      int i = base + u * nt;
-      int j = i / nrow, r = i % nrow;  // consecutive threads: consecutive rows of one column
-      int row = k0 + r;
-      v[u] = (i < total && j < bs && row < m) ? Ls[(long long)(k0 + j) * m + row] : 0.0;
-    }
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      int i = base + u * nt;
-      if (i < total) {
-        int j = i / nrow, r = i % nrow;
-        P[r * 16 + j] = v[u];
-      }
-    }
-  }
-}
-
 // Front load: T[row][c] = src(row, c) for row < nvalid, c < ct; zero elsewhere.
-// rows == nullptr: source row c0 + row; else rows[row].
+// rows == nullptr: source row c0 + row; else rows[row]. Loads are issued in
+// batches of 8 per thread so that their latencies overlap.
 __device__ __forceinline__ void load_front(const double* __restrict__ B, const int* __restrict__ rows, int c0,
-                                           int nvalid, int mpad, int Ct, int ct, int C, int col0,
+                                           int nvalid, int nrows, int Ct, int ct, int C, int col0,
                                            double* __restrict__ T, int tid, int nt) {
-  const int total = mpad * Ct;
+  const int total = nrows * Ct;
   for (int base = tid; base < total; base += nt * 8) {
     double v[8];
 #pragma unroll
@@ -282,13 +253,37 @@ __device__ __forceinline__ void load_front(const double* __restrict__ B, const i
   }
 }
 
+// acc += A(16 x K) * Bm(K x 16) over k in [kbeg, kend) in steps of 4, with
+// operands fetched by the callables (lane-level element getters); the loop is
+// unrolled by 4 so that four operand loads are in flight per wave.
+template <typename FA, typename FB>
+__device__ __forceinline__ v4d mfma_chain(v4d acc, int kbeg, int kend, int lk, FA fa, FB fb) {
+  int k = kbeg;
+  for (; k + 16 <= kend; k += 16) {
+    double a0 = fa(k + lk), a1 = fa(k + 4 + lk), a2 = fa(k + 8 + lk), a3 = fa(k + 12 + lk);
+    double b0 = fb(k + lk), b1 = fb(k + 4 + lk), b2 = fb(k + 8 + lk), b3 = fb(k + 12 + lk);
+    acc = mfma16(a0, b0, acc);
+    acc = mfma16(a1, b1, acc);
+    acc = mfma16(a2, b2, acc);
+    acc = mfma16(a3, b3, acc);
+  }
+  for (; k < kend; k += 4) {
+    int kk = k + lk;
+    bool ok = kk < kend;
+    double a = ok ? fa(kk) : 0.0;
+    double b = ok ? fb(kk) : 0.0;
+    acc = mfma16(a, b, acc);
+  }
+  return acc;
+}
+
 // Forward substitution (L y = b) of one elimination-tree level. Block =
-// (supernode, column tile of Ct = 16*nq right-hand sides); 4 waves. The front
-// is staged in LDS: own rows from B, children's update blocks added in child
-// order. Then, per 16-row diagonal block, the panel L[k0:m, k0:k0+16] is
-// staged in LDS, y_b = inv(L_bb) T_b, and every row below gets
-// T_i -= L_ib y_b, as 16x16 tiles on the matrix cores. Rows [0,w) are the
-// solution, rows [w,m) the update block U_s for the parent.
+// (supernode, column tile of Ct = 16*nq right-hand sides), 4 waves. The front
+// T (m x Ct) is staged in LDS: own rows from B, children's update blocks added
+// in child order. Then two dense products on the matrix cores:
+//   y = inv(L_ss) T[0:w]            (inverse precomputed at factorisation)
+//   T[w:m] -= L[w:m, 0:w] y         (update block U_s for the parent)
+// Both are split into 16x16 output tiles distributed over the waves.
 __global__ __launch_bounds__(256) void k_fwd_level(FactorView f, const int* __restrict__ lev, int ntile, int Ct, int C,
                                                    double* __restrict__ B, double* __restrict__ U) {
   extern __shared__ double sh[];
@@ -300,10 +295,10 @@ __global__ __launch_bounds__(256) void k_fwd_level(FactorView f, const int* __re
   const long long rp = f.rowptr[s];
   const int m = (int)(f.rowptr[s + 1] - rp);
   const int mpad = (m + 15) & ~15;
-  double* P = sh + (long long)mpad * Ct;
-  double* Y = P + (long long)mpad * 16;  // 16 x Ct
+  const int wpad = (w + 15) & ~15;
+  double* Y = sh + (long long)mpad * Ct;  // wpad x Ct
   const double* Ls = f.L + f.lptr[s];
-  const double* D = f.Dinv + f.dptr[s];
+  const double* Di = f.Dinv + f.dptr[s];
   const int tid = threadIdx.x, nt = blockDim.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int lr = lane & 15, lk = lane >> 4;
@@ -336,57 +331,46 @@ __global__ __launch_bounds__(256) void k_fwd_level(FactorView f, const int* __re
     }
     __syncthreads();
   }
-  for (int k0 = 0; k0 < w; k0 += 16) {
-    const int bs = min(16, w - k0);
-    stage_panel(Ls, m, k0, bs, mpad, P, tid, nt);
-    // y = inv(L_bb) T[k0:k0+16]
-    for (int q = wave; q < nq; q += 4) {
-      v4d acc = {0.0, 0.0, 0.0, 0.0};
+  // y = inv(L_ss) T[0:w]  (lower triangular: k <= row)
+  const int nrt = wpad >> 4;
+  for (int tq = wave; tq < nrt * nq; tq += 4) {
+    const int rt = (tq / nq) * 16, q = tq % nq;
+    const int row = rt + lr;
+    const int kend = min(w, rt + 16);
+    v4d acc = {0.0, 0.0, 0.0, 0.0};
+    acc = mfma_chain(
+        acc, 0, kend, lk, [&](int k) { return row < w ? Di[(long long)row * w + k] : 0.0; },
+        [&](int k) { return sh[k * Ct + q * 16 + lr]; });
 #pragma unroll
-      for (int kk = 0; kk < 4; ++kk) {
-        int k = kk * 4 + lk;
-        double a = (lr < bs && k < bs) ? D[lr * bs + k] : 0.0;
-        double bv = sh[(k0 + k) * Ct + q * 16 + lr];
-        acc = mfma16(a, bv, acc);
-      }
-#pragma unroll
-      for (int i = 0; i < 4; ++i) Y[(4 * i + lk) * Ct + q * 16 + lr] = acc[i];
-    }
-    __syncthreads();
-    for (int i = tid; i < bs * Ct; i += nt) {
-      int r = i / Ct, c = i % Ct;
-      sh[(k0 + r) * Ct + c] = Y[r * Ct + c];
-    }
-    // rows below: tiles aligned at k0; rows of the block itself are skipped
-    const int ntr = (mpad - k0) >> 4;
-    const int first_tile = bs == 16 ? 1 : 0;
-    for (int tq = wave + first_tile * nq; tq < ntr * nq; tq += 4) {
-      int tr = tq / nq, q = tq % nq;
-      int rt = k0 + tr * 16;
-      v4d acc;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) acc[i] = sh[(rt + 4 * i + lk) * Ct + q * 16 + lr];
-#pragma unroll
-      for (int kk = 0; kk < 4; ++kk) {
-        int k = kk * 4 + lk;
-        double a = -P[(tr * 16 + lr) * 16 + k];
-        double bv = Y[k * Ct + q * 16 + lr];
-        acc = mfma16(a, bv, acc);
-      }
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        int row = rt + 4 * i + lk;
-        if (row >= k0 + bs) sh[row * Ct + q * 16 + lr] = acc[i];
-      }
-    }
-    D += bs * bs;
-    __syncthreads();
+    for (int i = 0; i < 4; ++i) Y[(rt + 4 * i + lk) * Ct + q * 16 + lr] = acc[i];
   }
-  for (int i = tid; i < w * ct; i += nt) {
-    int row = i / ct, c = i % ct;
-    B[(long long)(c0 + row) * C + col0 + c] = sh[row * Ct + c];
-  }
+  __syncthreads();
+  // T[w:m] -= L_below y   (row tiles start at w)
   const int mb = m - w;
+  const int nbt = (mb + 15) >> 4;
+  for (int tq = wave; tq < nbt * nq; tq += 4) {
+    const int rt = w + (tq / nq) * 16, q = tq % nq;
+    const int row = rt + lr;
+    v4d acc;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      int r = rt + 4 * i + lk;
+      acc[i] = r < m ? sh[r * Ct + q * 16 + lr] : 0.0;
+    }
+    acc = mfma_chain(
+        acc, 0, w, lk, [&](int k) { return row < m ? -Ls[(long long)k * m + row] : 0.0; },
+        [&](int k) { return Y[k * Ct + q * 16 + lr]; });
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      int r = rt + 4 * i + lk;
+      if (r < m) sh[r * Ct + q * 16 + lr] = acc[i];
+    }
+  }
+  __syncthreads();
+  for (int i = tid; i < w * ct; i += nt) {
+    int r = i / ct, c = i % ct;
+    B[(long long)(c0 + r) * C + col0 + c] = Y[r * Ct + c];
+  }
   if (mb > 0) {
     double* Us = U + f.uptr[s] * C;
     for (int i = tid; i < mb * ct; i += nt) {
@@ -396,11 +380,9 @@ __global__ __launch_bounds__(256) void k_fwd_level(FactorView f, const int* __re
   }
 }
 
-// Backward substitution (L^T x = y) of one level, from the last diagonal block
-// up: t_b = y_b - L_below,b^T x_below (rows below already final),
-// x_b = inv(L_bb)^T t_b, on the matrix cores. The panel is staged in LDS and
-// the contraction over the rows below is split across the waves when the
-// column tile has fewer than 4 subtiles.
+// Backward substitution (L^T x = y) of one level as two dense products:
+//   t = y_s - L[w:m, 0:w]^T x_below   (rows below are final)
+//   x_s = inv(L_ss)^T t
 __global__ __launch_bounds__(256) void k_bwd_level(FactorView f, const int* __restrict__ lev, int ntile, int Ct, int C,
                                                    double* __restrict__ B) {
   extern __shared__ double sh[];
@@ -412,70 +394,52 @@ __global__ __launch_bounds__(256) void k_bwd_level(FactorView f, const int* __re
   const long long rp = f.rowptr[s];
   const int m = (int)(f.rowptr[s + 1] - rp);
   const int mpad = (m + 15) & ~15;
-  double* P = sh + (long long)mpad * Ct;
-  double* S = P + (long long)mpad * 16;  // 4 x 16 x Ct
+  const int wpad = (w + 15) & ~15;
+  double* Tt = sh + (long long)mpad * Ct;  // wpad x Ct
   const int* rows = f.rows + rp;
   const double* Ls = f.L + f.lptr[s];
+  const double* Di = f.Dinv + f.dptr[s];
   const int tid = threadIdx.x, nt = blockDim.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int lr = lane & 15, lk = lane >> 4;
   const int nq = Ct >> 4;
-  const int ksplit = nq >= 4 ? 1 : 4 / nq;
   load_front(B, rows, 0, m, mpad, Ct, ct, C, col0, sh, tid, nt);
   __syncthreads();
-  const int nblk = (w + 15) >> 4;
-  long long doff = f.dptr[s + 1];
-  for (int bi = nblk - 1; bi >= 0; --bi) {
-    const int k0 = bi * 16;
-    const int bs = min(16, w - k0);
-    doff -= bs * bs;
-    const double* D = f.Dinv + doff;
-    stage_panel(Ls, m, k0, bs, mpad, P, tid, nt);
-    __syncthreads();
-    // partial t over rows [k0+bs, m), split in ksplit chunks of 4-row steps
-    const int kbeg = k0 + bs;
-    const int nsteps = (m - kbeg + 3) >> 2;
-    const int per = (nsteps + ksplit - 1) / ksplit;
-    for (int wq = wave; wq < nq * ksplit; wq += 4) {
-      int q = wq % nq, part = wq / nq;
-      v4d acc;
+  const int nrt = wpad >> 4;
+  for (int tq = wave; tq < nrt * nq; tq += 4) {
+    const int rt = (tq / nq) * 16, q = tq % nq;
+    const int col = rt + lr;  // column of L (output row of t)
+    v4d acc;
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        int r = 4 * i + lk;
-        acc[i] = (part == 0 && r < bs) ? sh[(k0 + r) * Ct + q * 16 + lr] : 0.0;
-      }
-      int st0 = part * per, st1 = min(nsteps, st0 + per);
-      for (int st = st0; st < st1; ++st) {
-        int row = kbeg + 4 * st + lk;
-        double a = row < m ? -P[(row - k0) * 16 + lr] : 0.0;
-        double bv = row < m ? sh[row * Ct + q * 16 + lr] : 0.0;
-        acc = mfma16(a, bv, acc);
-      }
-#pragma unroll
-      for (int i = 0; i < 4; ++i) S[((long long)part * 16 + 4 * i + lk) * Ct + q * 16 + lr] = acc[i];
+    for (int i = 0; i < 4; ++i) {
+      int r = rt + 4 * i + lk;
+      acc[i] = r < w ? sh[r * Ct + q * 16 + lr] : 0.0;
     }
-    __syncthreads();
-    for (int q = wave; q < nq; q += 4) {
-      v4d acc = {0.0, 0.0, 0.0, 0.0};
+    acc = mfma_chain(
+        acc, w, m, lk, [&](int k) { return col < w ? -Ls[(long long)col * m + k] : 0.0; },
+        [&](int k) { return sh[k * Ct + q * 16 + lr]; });
 #pragma unroll
-      for (int kk = 0; kk < 4; ++kk) {
-        int k = kk * 4 + lk;
-        double a = (lr < bs && k < bs) ? D[k * bs + lr] : 0.0;  // inv(L_bb)^T
-        double bv = 0.0;
-        for (int p = 0; p < ksplit; ++p) bv += S[((long long)p * 16 + k) * Ct + q * 16 + lr];
-        acc = mfma16(a, bv, acc);
-      }
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        int r = 4 * i + lk;
-        if (r < bs) sh[(k0 + r) * Ct + q * 16 + lr] = acc[i];
-      }
-    }
-    __syncthreads();
+    for (int i = 0; i < 4; ++i) Tt[(rt + 4 * i + lk) * Ct + q * 16 + lr] = acc[i];
   }
+  __syncthreads();
+  // x = inv(L_ss)^T t  (upper triangular in (i,k): k >= i)
+  for (int tq = wave; tq < nrt * nq; tq += 4) {
+    const int rt = (tq / nq) * 16, q = tq % nq;
+    const int i_row = rt + lr;
+    v4d acc = {0.0, 0.0, 0.0, 0.0};
+    acc = mfma_chain(
+        acc, rt, w, lk, [&](int k) { return i_row < w ? Di[(long long)k * w + i_row] : 0.0; },
+        [&](int k) { return Tt[k * Ct + q * 16 + lr]; });
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      int r = rt + 4 * i + lk;
+      if (r < w) sh[r * Ct + q * 16 + lr] = acc[i];
+    }
+  }
+  __syncthreads();
   for (int i = tid; i < w * ct; i += nt) {
-    int row = i / ct, c = i % ct;
-    B[(long long)(c0 + row) * C + col0 + c] = sh[row * Ct + c];
+    int r = i / ct, c = i % ct;
+    B[(long long)(c0 + r) * C + col0 + c] = sh[r * Ct + c];
   }
 }
 
@@ -1320,11 +1284,11 @@ int ensure_batch_buffers(pdm_scene s) {
 }
 
 // Column tile per level: Ct = 16*nq right-hand sides, nq in {1,2,4}. LDS per
-// block = T (mpad x Ct) + panel (mpad x 16) + scratch (4 x 16 x Ct) doubles;
+// block = T (mpad x Ct) + scratch (wpad x Ct) doubles;
 // keep it within 64 KB (two blocks per CU) where possible, else up to 160 KB,
 // and shrink nq while the level has too few blocks to fill the GPU.
 constexpr size_t kLdsSoft = 64 * 1024, kLdsMax = 160 * 1024;
-size_t level_lds(size_t mpad, int Ct) { return (mpad * Ct + mpad * 16 + 64 * (size_t)Ct) * 8; }
+size_t level_lds(size_t mpad, int Ct) { return (2 * mpad * Ct) * 8; }
 
 int level_tile(pdm_scene s, int l, int* Ct_out, size_t* lds_out) {
   int cnt = s->lev_ptr[l + 1] - s->lev_ptr[l];
@@ -1783,27 +1747,23 @@ int pdm_host_solve(const pdm_scene_desc* d, int R, const double* b, double* x) {
           for (int c = 0; c < 3; ++c) Tm[3 * rel[a] + c] += U[ch][3 * a + c];
         std::vector<double>().swap(U[ch]);
       }
-      // blocked: y_b = inv(L_bb) T_b, then rows below -= L_ib y_b
-      const double* Db = &F.Dinv[F.dptr[sn]];
-      for (int k0 = 0; k0 < w; k0 += kDiagBlock) {
-        int bs = std::min(kDiagBlock, w - k0);
-        double yb[kDiagBlock * 3];
-        for (int r = 0; r < bs; ++r)
-          for (int c = 0; c < 3; ++c) {
-            double acc = 0.0;
-            for (int k = 0; k < bs; ++k) acc += Db[r * bs + k] * Tm[3 * (k0 + k) + c];
-            yb[3 * r + c] = acc;
-          }
-        for (int r = 0; r < bs; ++r)
-          for (int c = 0; c < 3; ++c) Tm[3 * (k0 + r) + c] = yb[3 * r + c];
-        for (int i = k0 + bs; i < m; ++i)
-          for (int c = 0; c < 3; ++c) {
-            double acc = 0.0;
-            for (int k = 0; k < bs; ++k) acc += Ls[(size_t)(k0 + k) * m + i] * yb[3 * k + c];
-            Tm[3 * i + c] -= acc;
-          }
-        Db += bs * bs;
-      }
+      // y = inv(L_ss) T[0:w], then T_below -= L_below y
+      const double* Di = &F.Dinv[F.dptr[sn]];
+      std::vector<double> yb((size_t)w * 3);
+      for (int r = 0; r < w; ++r)
+        for (int c = 0; c < 3; ++c) {
+          double acc = 0.0;
+          for (int k = 0; k <= r; ++k) acc += Di[(size_t)r * w + k] * Tm[3 * k + c];
+          yb[3 * r + c] = acc;
+        }
+      for (int i = w; i < m; ++i)
+        for (int c = 0; c < 3; ++c) {
+          double acc = 0.0;
+          for (int k = 0; k < w; ++k) acc += Ls[(size_t)k * m + i] * yb[3 * k + c];
+          Tm[3 * i + c] -= acc;
+        }
+      for (int r = 0; r < w; ++r)
+        for (int c = 0; c < 3; ++c) Tm[3 * r + c] = yb[3 * r + c];
       for (int i = 0; i < w; ++i)
         for (int c = 0; c < 3; ++c) y[3 * (c0 + i) + c] = Tm[3 * i + c];
       U[sn].assign(Tm.begin() + 3 * (size_t)w, Tm.end());
@@ -1813,26 +1773,21 @@ int pdm_host_solve(const pdm_scene_desc* d, int R, const double* b, double* x) {
       int m = (int)(F.rowptr[sn + 1] - F.rowptr[sn]);
       const int* rows = &F.rows[F.rowptr[sn]];
       const double* Ls = &F.L[F.lptr[sn]];
-      int nblk = (w + kDiagBlock - 1) / kDiagBlock;
-      int64_t doff = F.dptr[sn + 1];
-      for (int bi = nblk - 1; bi >= 0; --bi) {
-        int k0 = bi * kDiagBlock, bs = std::min(kDiagBlock, w - k0);
-        doff -= bs * bs;
-        const double* Db = &F.Dinv[doff];
-        double tb[kDiagBlock * 3];
-        for (int j = 0; j < bs; ++j)
-          for (int c = 0; c < 3; ++c) {
-            double acc = y[3 * (c0 + k0 + j) + c];
-            for (int i = k0 + bs; i < m; ++i) acc -= Ls[(size_t)(k0 + j) * m + i] * y[3 * rows[i] + c];
-            tb[3 * j + c] = acc;
-          }
-        for (int j = 0; j < bs; ++j)
-          for (int c = 0; c < 3; ++c) {
-            double acc = 0.0;
-            for (int k = 0; k < bs; ++k) acc += Db[k * bs + j] * tb[3 * k + c];
-            y[3 * (c0 + k0 + j) + c] = acc;
-          }
-      }
+      // t = y_s - L_below^T x_below, x_s = inv(L_ss)^T t
+      const double* Di = &F.Dinv[F.dptr[sn]];
+      std::vector<double> tb((size_t)w * 3);
+      for (int j = 0; j < w; ++j)
+        for (int c = 0; c < 3; ++c) {
+          double acc = y[3 * (c0 + j) + c];
+          for (int i = w; i < m; ++i) acc -= Ls[(size_t)j * m + i] * y[3 * rows[i] + c];
+          tb[3 * j + c] = acc;
+        }
+      for (int j = 0; j < w; ++j)
+        for (int c = 0; c < 3; ++c) {
+          double acc = 0.0;
+          for (int k = j; k < w; ++k) acc += Di[(size_t)k * w + j] * tb[3 * k + c];
+          y[3 * (c0 + j) + c] = acc;
+        }
     }
     for (int v = 0; v < N; ++v)
       for (int c = 0; c < 3; ++c) x[(size_t)r * 3 * N + 3 * s->S.perm[v] + c] = y[3 * v + c];

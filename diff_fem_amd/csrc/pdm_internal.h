@@ -70,15 +70,13 @@ struct Factor {
   std::vector<int> rel;            // position of each below row in the parent's row list
   std::vector<int64_t> uptr;       // ns+1: offset (in rows) of each update block
   std::vector<int> level_ptr, level_sn;  // forward levels by height (leaves first)
-  std::vector<int64_t> dptr;       // ns+1: offset of the supernode's diagonal-block inverses
-  std::vector<double> Dinv;        // inverses of the 16x16 diagonal blocks of L (row-major, lower)
+  std::vector<int64_t> dptr;       // ns+1: offset of the supernode's w x w diagonal inverse
+  std::vector<double> Dinv;        // inv(L_ss) per supernode (row-major, zero above the diagonal)
   int max_m = 0, max_w = 0;
   int64_t nnz = 0;                 // stored entries of L (incl. supernode padding)
   double flops_factor = 0;         // numeric factorisation flops
   double reg_used = 0;
 };
-
-constexpr int kDiagBlock = 16;  // diagonal-block size of the blocked triangular solves
 
 // Incident-corner lists for the deterministic tet->vertex gather.
 struct Corners {

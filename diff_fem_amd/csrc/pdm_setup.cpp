@@ -458,13 +458,8 @@ bool symbolic(const Csr& A, Factor& F, std::string& err) {
   F.nnz = F.lptr[ns2];
   F.dptr.assign(ns2 + 1, 0);
   for (int s = 0; s < ns2; ++s) {
-    int w = F.first[s + 1] - F.first[s];
-    int64_t acc = 0;
-    for (int k0 = 0; k0 < w; k0 += kDiagBlock) {
-      int bs = std::min(kDiagBlock, w - k0);
-      acc += (int64_t)bs * bs;
-    }
-    F.dptr[s + 1] = F.dptr[s] + acc;
+    int64_t w = F.first[s + 1] - F.first[s];
+    F.dptr[s + 1] = F.dptr[s] + w * w;
   }
   // children lists
   F.child_ptr.assign(ns2 + 1, 0);
@@ -596,21 +591,20 @@ bool numeric(const Csr& A, const std::vector<double>& diag_add, Factor& F, std::
       double inv = 1.0 / d;
       for (int i = j + 1; i < m; ++i) cj[i] *= inv;
     }
-    // inverses of the diagonal blocks (used by the blocked device solves)
+    // inverse of the diagonal triangle L_ss (row-major w x w, zero above the
+    // diagonal): the device solves apply it as one dense product per supernode
     {
       double* D = &F.Dinv[F.dptr[s]];
-      for (int k0 = 0; k0 < w; k0 += kDiagBlock) {
-        int bs = std::min(kDiagBlock, w - k0);
-        for (int j = 0; j < bs; ++j) {
-          // column j of inv(Lb): forward substitution on e_j
-          for (int r = 0; r < bs; ++r) {
-            double acc = (r == j) ? 1.0 : 0.0;
-            for (int q = j; q < r; ++q) acc -= Ls[(size_t)(k0 + q) * m + (k0 + r)] * D[(size_t)q * bs + j];
-            D[(size_t)r * bs + j] = r < j ? 0.0 : acc / Ls[(size_t)(k0 + r) * m + (k0 + r)];
+      for (int j = 0; j < w; ++j)
+        for (int r = 0; r < w; ++r) {
+          if (r < j) {
+            D[(size_t)r * w + j] = 0.0;
+            continue;
           }
+          double acc = (r == j) ? 1.0 : 0.0;
+          for (int q = j; q < r; ++q) acc -= Ls[(size_t)q * m + r] * D[(size_t)q * w + j];
+          D[(size_t)r * w + j] = acc / Ls[(size_t)r * m + r];
         }
-        D += (size_t)bs * bs;
-      }
     }
     // update matrix U_s = Us - L21 L21^T (lower)
     if (mb > 0) {
