@@ -132,6 +132,18 @@ def main():
                    "checkpoint_every": args.ckpt, "contact": bool(d.contact), "nnz_factor": info["nnz_factor"],
                    "forward_ms": float(np.mean(fwd_ms)), "adjoint_ms": float(np.mean(bwd_ms))},
     }
+    # Roofline of the dominant unit: one triangular solve (forward + backward
+    # sweep of k_fwd_level/k_bwd_level over all 3R right-hand sides). At 3R = 768
+    # it is matrix-core bound (FP64 MFMA): algorithmic flops = 2 sweeps x 2 x
+    # nnz(L, lower trapezoid) x 3R.
+    ms = sc.time_solve(20)
+    flops = 2.0 * 2.0 * info["nnz_lower"] * 3 * R
+    achieved = flops / (ms * 1e-3) / 1e12
+    out["roofline"] = {"bound": "mfma", "achieved": achieved, "peak": 78.6, "unit": "TFLOP/s",
+                       "frac": achieved / 78.6, "traffic": None,
+                       "kernel": "supernodal solve k_fwd_level+k_bwd_level, %d RHS" % (3 * R),
+                       "ms_per_solve": ms, "flops_per_solve": flops,
+                       "bytes_alg_per_solve": 2.0 * info["nnz_lower"] * 8 + 2.0 * 2 * 3 * R * d.N * 8}
     sc.close()
     del sc
     if not args.no_plain:

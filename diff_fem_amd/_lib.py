@@ -42,7 +42,7 @@ class InfoC(C.Structure):
     _fields_ = [
         ("n_vertices", C.c_int), ("n_tets", C.c_int), ("n_muscles", C.c_int), ("n_groups", C.c_int),
         ("n_supernodes", C.c_int), ("n_levels", C.c_int), ("max_front", C.c_int), ("max_width", C.c_int),
-        ("nnz_factor", C.c_longlong), ("factor_flops", C.c_double), ("reg_used", C.c_double),
+        ("nnz_factor", C.c_longlong), ("nnz_lower", C.c_longlong), ("factor_flops", C.c_double), ("reg_used", C.c_double),
         ("n_contact_vertices", C.c_int), ("batch", C.c_int), ("precision", C.c_int),
     ]
 
@@ -52,7 +52,7 @@ SYMBOLS = [
     "pdm_last_error", "pdm_device_count", "pdm_scene_create", "pdm_scene_destroy", "pdm_scene_info",
     "pdm_set_stream", "pdm_set_material", "pdm_set_batch", "pdm_forward", "pdm_forward_host", "pdm_backward",
     "pdm_backward_host", "pdm_get_iterations", "pdm_solve_host", "pdm_force_host", "pdm_get_timing",
-    "pdm_host_analyse", "pdm_host_solve", "pdm_get_contact_stats",
+    "pdm_host_analyse", "pdm_host_solve", "pdm_get_contact_stats", "pdm_time_solve",
 ]
 
 _lib = None
@@ -85,6 +85,7 @@ def load():
     L.pdm_solve_host.argtypes = [vp, vp, vp]
     L.pdm_force_host.argtypes = [vp, vp, vp, vp]
     L.pdm_get_timing.argtypes = [vp, _dp, _dp, _dp]
+    L.pdm_time_solve.argtypes = [vp, C.c_int, _dp]
     L.pdm_host_analyse.argtypes = [C.POINTER(SceneDescC), C.POINTER(InfoC)]
     L.pdm_host_solve.argtypes = [C.POINTER(SceneDescC), C.c_int, vp, vp]
     for s in SYMBOLS:

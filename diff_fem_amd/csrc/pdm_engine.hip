@@ -178,7 +178,8 @@ __global__ void k_local_muscle(const double* __restrict__ X, const int* __restri
   }
 }
 
-// B = Cc + sum of the vertex's corner slots (fixed order).
+// B = Cc + sum of the vertex's corner slots (fixed order; four partial sums
+// so that the slot loads overlap).
 template <typename T>
 __global__ void k_gather(const double* __restrict__ Cc, const T* __restrict__ corner, const int* __restrict__ vptr,
                          double* __restrict__ B, int N, int C) {
@@ -186,9 +187,16 @@ __global__ void k_gather(const double* __restrict__ Cc, const T* __restrict__ co
   if (idx >= (long long)N * C) return;
   int v = (int)(idx / C);
   int j = (int)(idx % C);
-  double s = Cc ? Cc[idx] : 0.0;
-  for (int q = vptr[v]; q < vptr[v + 1]; ++q) s += (double)corner[(long long)q * C + j];
-  B[idx] = s;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int q = vptr[v], qe = vptr[v + 1];
+  for (; q + 4 <= qe; q += 4) {
+    s0 += (double)corner[(long long)q * C + j];
+    s1 += (double)corner[(long long)(q + 1) * C + j];
+    s2 += (double)corner[(long long)(q + 2) * C + j];
+    s3 += (double)corner[(long long)(q + 3) * C + j];
+  }
+  for (; q < qe; ++q) s0 += (double)corner[(long long)q * C + j];
+  B[idx] = (Cc ? Cc[idx] : 0.0) + ((s0 + s1) + (s2 + s3));
 }
 
 // Adjoint gather: Ybar = sum of corners for rollouts active at this
@@ -201,9 +209,16 @@ __global__ void k_gather_masked(const T* __restrict__ corner, const int* __restr
   int v = (int)(idx / C);
   int j = (int)(idx % C);
   if (!active[j / 3]) return;
-  double s = 0.0;
-  for (int q = vptr[v]; q < vptr[v + 1]; ++q) s += (double)corner[(long long)q * C + j];
-  B[idx] = s;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int q = vptr[v], qe = vptr[v + 1];
+  for (; q + 4 <= qe; q += 4) {
+    s0 += (double)corner[(long long)q * C + j];
+    s1 += (double)corner[(long long)(q + 1) * C + j];
+    s2 += (double)corner[(long long)(q + 2) * C + j];
+    s3 += (double)corner[(long long)(q + 3) * C + j];
+  }
+  for (; q < qe; ++q) s0 += (double)corner[(long long)q * C + j];
+  B[idx] = (s0 + s1) + (s2 + s3);
 }
 
 struct FactorView {
@@ -284,7 +299,7 @@ __device__ __forceinline__ v4d mfma_chain(v4d acc, int kbeg, int kend, int lk, F
 //   y = inv(L_ss) T[0:w]            (inverse precomputed at factorisation)
 //   T[w:m] -= L[w:m, 0:w] y         (update block U_s for the parent)
 // Both are split into 16x16 output tiles distributed over the waves.
-__global__ __launch_bounds__(256) void k_fwd_level(FactorView f, const int* __restrict__ lev, int ntile, int Ct, int C,
+__global__ __launch_bounds__(1024) void k_fwd_level(FactorView f, const int* __restrict__ lev, int ntile, int Ct, int C,
                                                    double* __restrict__ B, double* __restrict__ U) {
   extern __shared__ double sh[];
   const int s = lev[blockIdx.x / ntile];
@@ -300,7 +315,7 @@ __global__ __launch_bounds__(256) void k_fwd_level(FactorView f, const int* __re
   const double* Ls = f.L + f.lptr[s];
   const double* Di = f.Dinv + f.dptr[s];
   const int tid = threadIdx.x, nt = blockDim.x;
-  const int lane = tid & 63, wave = tid >> 6;
+  const int lane = tid & 63, wave = tid >> 6, nwave = nt >> 6;
   const int lr = lane & 15, lk = lane >> 4;
   const int nq = Ct >> 4;
   load_front(B, nullptr, c0, w, mpad, Ct, ct, C, col0, sh, tid, nt);
@@ -333,7 +348,7 @@ __global__ __launch_bounds__(256) void k_fwd_level(FactorView f, const int* __re
   }
   // y = inv(L_ss) T[0:w]  (lower triangular: k <= row)
   const int nrt = wpad >> 4;
-  for (int tq = wave; tq < nrt * nq; tq += 4) {
+  for (int tq = wave; tq < nrt * nq; tq += nwave) {
     const int rt = (tq / nq) * 16, q = tq % nq;
     const int row = rt + lr;
     const int kend = min(w, rt + 16);
@@ -348,7 +363,7 @@ __global__ __launch_bounds__(256) void k_fwd_level(FactorView f, const int* __re
   // T[w:m] -= L_below y   (row tiles start at w)
   const int mb = m - w;
   const int nbt = (mb + 15) >> 4;
-  for (int tq = wave; tq < nbt * nq; tq += 4) {
+  for (int tq = wave; tq < nbt * nq; tq += nwave) {
     const int rt = w + (tq / nq) * 16, q = tq % nq;
     const int row = rt + lr;
     v4d acc;
@@ -383,7 +398,7 @@ __global__ __launch_bounds__(256) void k_fwd_level(FactorView f, const int* __re
 // Backward substitution (L^T x = y) of one level as two dense products:
 //   t = y_s - L[w:m, 0:w]^T x_below   (rows below are final)
 //   x_s = inv(L_ss)^T t
-__global__ __launch_bounds__(256) void k_bwd_level(FactorView f, const int* __restrict__ lev, int ntile, int Ct, int C,
+__global__ __launch_bounds__(1024) void k_bwd_level(FactorView f, const int* __restrict__ lev, int ntile, int Ct, int C,
                                                    double* __restrict__ B) {
   extern __shared__ double sh[];
   const int s = lev[blockIdx.x / ntile];
@@ -400,13 +415,13 @@ __global__ __launch_bounds__(256) void k_bwd_level(FactorView f, const int* __re
   const double* Ls = f.L + f.lptr[s];
   const double* Di = f.Dinv + f.dptr[s];
   const int tid = threadIdx.x, nt = blockDim.x;
-  const int lane = tid & 63, wave = tid >> 6;
+  const int lane = tid & 63, wave = tid >> 6, nwave = nt >> 6;
   const int lr = lane & 15, lk = lane >> 4;
   const int nq = Ct >> 4;
   load_front(B, rows, 0, m, mpad, Ct, ct, C, col0, sh, tid, nt);
   __syncthreads();
   const int nrt = wpad >> 4;
-  for (int tq = wave; tq < nrt * nq; tq += 4) {
+  for (int tq = wave; tq < nrt * nq; tq += nwave) {
     const int rt = (tq / nq) * 16, q = tq % nq;
     const int col = rt + lr;  // column of L (output row of t)
     v4d acc;
@@ -423,7 +438,7 @@ __global__ __launch_bounds__(256) void k_bwd_level(FactorView f, const int* __re
   }
   __syncthreads();
   // x = inv(L_ss)^T t  (upper triangular in (i,k): k >= i)
-  for (int tq = wave; tq < nrt * nq; tq += 4) {
+  for (int tq = wave; tq < nrt * nq; tq += nwave) {
     const int rt = (tq / nq) * 16, q = tq % nq;
     const int i_row = rt + lr;
     v4d acc = {0.0, 0.0, 0.0, 0.0};
@@ -653,10 +668,7 @@ __global__ void k_vjp_muscle(const double* __restrict__ Yk, const double* __rest
   if (idx >= (long long)M * R) return;
   int m = (int)(idx / R);
   int r = (int)(idx % R);
-  if (!active[r]) {
-    gMpart[idx] = 0.0;
-    return;
-  }
+  if (!active[r]) return;
   int C = 3 * R;
   double a = act ? act[(long long)r * act_stride_r + act_off + mgroup[m]] : 0.0;
   double fz[3] = {0, 0, 0}, ff[3] = {0, 0, 0};
@@ -675,7 +687,7 @@ __global__ void k_vjp_muscle(const double* __restrict__ Yk, const double* __rest
 #pragma unroll
     for (int k = 1; k < 4; ++k) ff[c] += (Yk[(long long)v[k] * C + 3 * r + c] - b) * g[k];
   }
-  gMpart[idx] = -mw[m] * (fz[0] * ff[0] + fz[1] * ff[1] + fz[2] * ff[2]);
+  gMpart[idx] += -mw[m] * (fz[0] * ff[0] + fz[1] * ff[1] + fz[2] * ff[2]);
   double sc = mw[m] * (1.0 - a);
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
@@ -685,16 +697,21 @@ __global__ void k_vjp_muscle(const double* __restrict__ Yk, const double* __rest
   }
 }
 
-// gact[r, step, g] += sum over the group's muscles (fixed order).
+// gact[r, step, g] = sum over the group's muscles (fixed order, 8 partial sums).
 __global__ void k_reduce_groups(const double* __restrict__ gMpart, const int* __restrict__ gptr,
                                 const int* __restrict__ gmus, double* __restrict__ gact, int G, int R, int steps,
                                 int step) {
   int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= G * R) return;
   int g = idx / R, r = idx % R;
-  double s = 0.0;
-  for (int q = gptr[g]; q < gptr[g + 1]; ++q) s += gMpart[(long long)gmus[q] * R + r];
-  gact[((long long)r * steps + step) * G + g] += s;
+  double p[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int q = gptr[g], qe = gptr[g + 1];
+  for (; q + 8 <= qe; q += 8) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) p[u] += gMpart[(long long)gmus[q + u] * R + r];
+  }
+  for (; q < qe; ++q) p[0] += gMpart[(long long)gmus[q] * R + r];
+  gact[((long long)r * steps + step) * G + g] = ((p[0] + p[1]) + (p[2] + p[3])) + ((p[4] + p[5]) + (p[6] + p[7]));
 }
 
 // End of a step's reverse sweep:
@@ -1324,11 +1341,13 @@ int launch_solve(pdm_scene s, double* Bp) {
       size_t lds;
       if (level_tile(s, l, &Ct, &lds)) return 1;
       int ntile = cdiv(s->C, Ct);
+      // few blocks (top of the tree): 16 waves per block to hide load latency
+      int nthr = (long long)cnt * ntile < 256 ? 1024 : kThreads;
       if (pass == 0)
-        hipLaunchKernelGGL(k_fwd_level, dim3(cnt * ntile), dim3(kThreads), lds, s->stream, fv,
+        hipLaunchKernelGGL(k_fwd_level, dim3(cnt * ntile), dim3(nthr), lds, s->stream, fv,
                            s->f_lev.p + s->lev_ptr[l], ntile, Ct, s->C, Bp, s->U.p);
       else
-        hipLaunchKernelGGL(k_bwd_level, dim3(cnt * ntile), dim3(kThreads), lds, s->stream, fv,
+        hipLaunchKernelGGL(k_bwd_level, dim3(cnt * ntile), dim3(nthr), lds, s->stream, fv,
                            s->f_lev.p + s->lev_ptr[l], ntile, Ct, s->C, Bp);
     }
   }
@@ -1612,6 +1631,7 @@ int backward_impl(pdm_scene s, const double* gxT, const double* gvT, double* gx0
       hipLaunchKernelGGL(k_adj_seed, dim3(cdiv(nc, kThreads)), dim3(kThreads), 0, s->stream, s->gX.p, s->gV.p, s->Yb.p,
                          s->Xb.p, s->Zs.p, (long long)nc, gh);
       const double* Y = s->tY.p + (size_t)q * K1 * nc;
+      if (S.M > 0) HIPX(hipMemsetAsync(s->gMpart.p, 0, (size_t)S.M * R * 8, s->stream));
       for (int k = kmax - 1; k >= 0; --k) {
         hipLaunchKernelGGL(k_active_rev, dim3(cdiv(R, 64)), dim3(64), 0, s->stream, s->tIters.p + (size_t)q * R, k, R,
                            s->ractive.p);
@@ -1624,10 +1644,10 @@ int backward_impl(pdm_scene s, const double* gxT, const double* gvT, double* gx0
                      ? vjp_iteration<float>(s, Y + (size_t)k * nc, Y + (size_t)(k + 1) * nc, act, steps, n)
                      : vjp_iteration<double>(s, Y + (size_t)k * nc, Y + (size_t)(k + 1) * nc, act, steps, n);
         if (rc) return 1;
-        if (S.M > 0)
-          hipLaunchKernelGGL(k_reduce_groups, dim3(cdiv(S.n_groups * R, 64)), dim3(64), 0, s->stream, s->gMpart.p,
-                             s->gptr.p, s->gmus.p, s->gact.p, S.n_groups, R, steps, n);
       }
+      if (S.M > 0)
+        hipLaunchKernelGGL(k_reduce_groups, dim3(cdiv(S.n_groups * R, 64)), dim3(64), 0, s->stream, s->gMpart.p,
+                           s->gptr.p, s->gmus.p, s->gact.p, S.n_groups, R, steps, n);
       hipLaunchKernelGGL(k_spmv, dim3(cdiv(nc, kThreads)), dim3(kThreads), 0, s->stream, s->Lptr.p, s->Lcol.p,
                          s->Lval.p, s->Zs.p, s->LZ.p, N, C);
       hipLaunchKernelGGL(k_adj_step_end, dim3(cdiv(nc, kThreads)), dim3(kThreads), 0, s->stream, s->Xb.p, s->Yb.p,
@@ -1818,6 +1838,15 @@ int pdm_scene_info(pdm_scene s, pdm_info* info) {
   info->max_front = s->F.max_m;
   info->max_width = s->F.max_w;
   info->nnz_factor = (long long)s->F.nnz;
+  {
+    long long nl = 0;
+    for (int sn = 0; sn < s->F.ns; ++sn) {
+      long long w = s->F.first[sn + 1] - s->F.first[sn];
+      long long m = s->F.rowptr[sn + 1] - s->F.rowptr[sn];
+      nl += w * (w + 1) / 2 + (m - w) * w;
+    }
+    info->nnz_lower = nl;
+  }
   info->factor_flops = s->F.flops_factor;
   info->reg_used = s->F.reg_used;
   info->n_contact_vertices = (int)s->S.contact_vtx.size();
@@ -1998,6 +2027,24 @@ int pdm_force_host(pdm_scene s, const double* x, const double* act_step, double*
                      s->perm.p, s->S.N, s->R);
   HIPX(hipStreamSynchronize(s->stream));
   HIPX(hipMemcpy(Jd, dx.p, n * 8, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int pdm_time_solve(pdm_scene s, int reps, double* ms_per_solve) {
+  if (!s || reps < 1 || !ms_per_solve) {
+    g_err = "invalid argument";
+    return 1;
+  }
+  if (ensure_batch_buffers(s)) return 1;
+  if (launch_solve(s, s->B.p)) return 1;  // warm-up
+  HIPX(hipEventRecord(s->ev0, s->stream));
+  for (int k = 0; k < reps; ++k)
+    if (launch_solve(s, s->B.p)) return 1;
+  HIPX(hipEventRecord(s->ev1, s->stream));
+  HIPX(hipEventSynchronize(s->ev1));
+  float ms = 0;
+  HIPX(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+  *ms_per_solve = ms / reps;
   return 0;
 }
 

@@ -194,6 +194,13 @@ class Scene:
         check(_lib.load().pdm_force_host(self.h, _vp(x), _vp(a), _vp(out)))
         return out
 
+    def time_solve(self, reps: int = 20) -> float:
+        """Mean ms of one forward+backward triangular solve over all 3R
+        right-hand sides (HIP events on the library stream)."""
+        ms = C.c_double()
+        check(_lib.load().pdm_time_solve(self.h, int(reps), C.byref(ms)))
+        return ms.value
+
     def timing(self):
         f, b, fa = C.c_double(), C.c_double(), C.c_double()
         check(_lib.load().pdm_get_timing(self.h, C.byref(f), C.byref(b), C.byref(fa)))

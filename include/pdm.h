@@ -73,6 +73,7 @@ typedef struct pdm_info {
   int n_vertices, n_tets, n_muscles, n_groups;
   int n_supernodes, n_levels, max_front, max_width;
   long long nnz_factor;      /* stored entries of the supernodal factor L */
+  long long nnz_lower;       /* entries of the lower-trapezoidal supernode blocks (algorithmic nnz(L)) */
   double factor_flops;       /* flops of one numeric factorisation */
   double reg_used;           /* regularisation applied by the ladder (0 if none) */
   int n_contact_vertices;
@@ -144,6 +145,11 @@ int pdm_force_host(pdm_scene s, const double* x, const double* act_step, double*
  * supernodal schedule the device kernels use. */
 int pdm_host_analyse(const pdm_scene_desc* desc, pdm_info* info);
 int pdm_host_solve(const pdm_scene_desc* desc, int R, const double* b, double* x);
+
+/* Time `reps` back-to-back triangular solves (forward + backward sweep over
+ * all R*3 right-hand sides currently in the engine's rhs buffer) with HIP
+ * events on the library stream; returns the mean ms per solve. */
+int pdm_time_solve(pdm_scene s, int reps, double* ms_per_solve);
 
 /* Timing of the last forward/backward (ms, HIP events on the library stream). */
 int pdm_get_timing(pdm_scene s, double* fwd_ms, double* bwd_ms, double* factor_ms);
