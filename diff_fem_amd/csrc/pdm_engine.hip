@@ -595,7 +595,8 @@ __global__ __launch_bounds__(256) void k_vjp_tet(const double* __restrict__ Yk, 
                                                  const double* __restrict__ wv, const double* __restrict__ dEr,
                                                  const double* __restrict__ dEv, const int* __restrict__ tslot,
                                                  const int* __restrict__ active, T* __restrict__ corner,
-                                                 double* __restrict__ gEpart, int Tn, int R) {
+                                                 double* __restrict__ gEpart, int Tn, int R,
+                                                 const double* __restrict__ Xn, const double* __restrict__ CD) {
   long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (idx >= (long long)Tn * R) return;
   int t = (int)(idx / R);
@@ -646,6 +647,19 @@ __global__ __launch_bounds__(256) void k_vjp_tet(const double* __restrict__ Yk, 
   for (int i = 0; i < 9; ++i) {
     gr += Fz[i] * ((double)md[i] - F1[i]);
     gv += Fz[i] * ((double)mdv[i] - F1[i]);
+  }
+  if (CD) {
+    // contact iteration: C and D of the Jacobi update depend on the weights
+    double extra = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      double g2 = Gd[3 * k] * Gd[3 * k] + Gd[3 * k + 1] * Gd[3 * k + 1] + Gd[3 * k + 2] * Gd[3 * k + 2];
+      long long b = (long long)v[k] * C + 3 * r;
+      double zd = Z[b] * (Yk[b] - Xn[b]) + Z[b + 1] * (Yk[b + 1] - Xn[b + 1]) + Z[b + 2] * (Yk[b + 2] - Xn[b + 2]);
+      extra += g2 * (zd + CD[(long long)v[k] * R + r]);
+    }
+    gr += extra;
+    gv += extra;
   }
   gEpart[idx] += gr * dEr[t] + gv * dEv[t];
 #pragma unroll
@@ -843,13 +857,14 @@ __global__ void k_contact_detect(const int* __restrict__ cvtx, int ncv, const do
 }
 
 __global__ void k_contact_reset(int* __restrict__ imp_stamp, long long n, int* __restrict__ cactive,
-                                int* __restrict__ cfirst, int* __restrict__ fresh, int R) {
+                                int* __restrict__ cfirst, int* __restrict__ fresh, int* __restrict__ clast, int R) {
   long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (idx < n) imp_stamp[idx] = -1;
   if (idx < R) {
     cactive[idx] = 1;
     cfirst[idx] = 1;
     fresh[idx] = 0;
+    clast[idx] = 1;
   }
 }
 
@@ -891,7 +906,7 @@ __global__ void k_contact_begin(const double* __restrict__ Xit, double* __restri
 }
 
 __global__ void k_contact_err(const double* __restrict__ part, int nchunk, int R, int* __restrict__ iactive,
-                              int* __restrict__ cfirst, int* __restrict__ citers) {
+                              int* __restrict__ cfirst, int* __restrict__ citers, int* __restrict__ clast) {
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= R || !iactive[r]) return;
   int C = 3 * R;
@@ -902,6 +917,7 @@ __global__ void k_contact_err(const double* __restrict__ part, int nchunk, int R
   cfirst[r] = 0;
   iactive[r] = go ? 1 : 0;
   citers[r] += go ? 1 : 0;
+  clast[r] = go ? 0 : 1;  // 1: the result is this begin's ori_x = x_it
 }
 
 // One Jacobi update of the contact phase per (vertex, rollout), World.h:957-1016:
@@ -915,7 +931,8 @@ __global__ void k_contact_jacobi(const int* __restrict__ Lptr, const int* __rest
                                  const double* __restrict__ Kdiag, const int* __restrict__ cv_of_v,
                                  const int* __restrict__ imp_stamp, const double* __restrict__ imp_n, int nobs,
                                  double mu, double alpha, double h, int first_iter, const int* __restrict__ iactive,
-                                 double* __restrict__ Prev, double* __restrict__ Xit, int N, int R) {
+                                 double* __restrict__ Prev, double* __restrict__ Xit, int N, int R,
+                                 double* __restrict__ tRC, double* __restrict__ tN) {
   long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (idx >= (long long)N * R) return;
   int v = (int)(idx / R), r = (int)(idx % R);
@@ -942,6 +959,15 @@ __global__ void k_contact_jacobi(const int* __restrict__ Lptr, const int* __rest
     pv[0] = Prev[base]; pv[1] = Prev[base + 1]; pv[2] = Prev[base + 2];
   }
   int ci = cv_of_v[v];
+  if (ci >= 0 && tRC) {
+    long long tb = (long long)ci * C + 3 * r;
+    tRC[tb] = rc[0];
+    tRC[tb + 1] = rc[1];
+    tRC[tb + 2] = rc[2];
+    tN[tb] = 0.0;
+    tN[tb + 1] = 0.0;
+    tN[tb + 2] = 0.0;
+  }
   if (ci >= 0) {
     int best = -1, bst = -1;
     for (int o = 0; o < nobs; ++o) {
@@ -953,6 +979,12 @@ __global__ void k_contact_jacobi(const int* __restrict__ Lptr, const int* __rest
     }
     if (best >= 0) {
       const double* n = imp_n + 3 * (((long long)ci * nobs + best) * R + r);
+      if (tN) {
+        long long tb = (long long)ci * C + 3 * r;
+        tN[tb] = n[0];
+        tN[tb + 1] = n[1];
+        tN[tb + 2] = n[2];
+      }
       // computeBasis (collisionBrutal.h:408-426): [n, t, b]
       double t[3] = {0.0, n[2], -n[1]};
       double tn = sqrt(t[1] * t[1] + t[2] * t[2]);
@@ -986,6 +1018,155 @@ __global__ void k_contact_jacobi(const int* __restrict__ Lptr, const int* __rest
     Prev[base + c] = p;
     Xit[base + c] = Xn[base + c] + (p / h) / kd;
   }
+}
+
+
+// ------------------------------------------------------- contact-phase adjoint
+// Adjoint of the cone projection at a contact vertex with frozen branch and
+// normal n (zero normal: no impact, identity).
+__device__ void cone_vjp(const double* n, double mu, const double* f_in, const double* qbar, double* out) {
+  double nn = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+  if (!(nn > 0.0)) {
+    out[0] = qbar[0]; out[1] = qbar[1]; out[2] = qbar[2];
+    return;
+  }
+  double t[3] = {0.0, n[2], -n[1]};
+  double tn = sqrt(t[1] * t[1] + t[2] * t[2]);
+  if (tn < 1e-15) {
+    t[0] = -n[2]; t[1] = 0.0; t[2] = n[0];
+    tn = sqrt(t[0] * t[0] + t[2] * t[2]);
+  }
+  t[0] /= tn; t[1] /= tn; t[2] /= tn;
+  double b[3] = {n[1] * t[2] - n[2] * t[1], n[2] * t[0] - n[0] * t[2], n[0] * t[1] - n[1] * t[0]};
+  double bn = sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+  double e0[3] = {n[0] / nn, n[1] / nn, n[2] / nn}, e2[3] = {b[0] / bn, b[1] / bn, b[2] / bn};
+  double f0 = e0[0] * f_in[0] + e0[1] * f_in[1] + e0[2] * f_in[2];
+  if (f0 > 0.0) {
+    out[0] = qbar[0]; out[1] = qbar[1]; out[2] = qbar[2];
+    return;
+  }
+  double f1 = t[0] * f_in[0] + t[1] * f_in[1] + t[2] * f_in[2];
+  double f2 = e2[0] * f_in[0] + e2[1] * f_in[1] + e2[2] * f_in[2];
+  double r0 = e0[0] * qbar[0] + e0[1] * qbar[1] + e0[2] * qbar[2];
+  double r1 = t[0] * qbar[0] + t[1] * qbar[1] + t[2] * qbar[2];
+  double r2 = e2[0] * qbar[0] + e2[1] * qbar[1] + e2[2] * qbar[2];
+  double ft = sqrt(f1 * f1 + f2 * f2);
+  double g0, g1, g2;
+  if (!(ft > mu * (-f0))) {  // stick
+    g0 = -r0; g1 = -r1; g2 = -r2;
+  } else {  // slide
+    double u1 = f1 / ft, u2 = f2 / ft;
+    double ur = u1 * r1 + u2 * r2;
+    g0 = -r0 + mu * ur;
+    double sc = mu * f0 / ft;
+    g1 = sc * (r1 - ur * u1);
+    g2 = sc * (r2 - ur * u2);
+  }
+  for (int c = 0; c < 3; ++c) out[c] = e0[c] * g0 + t[c] * g1 + e2[c] * g2;
+}
+
+__global__ void k_cadj_init(double* __restrict__ XB, const double* __restrict__ Yb, const int* __restrict__ flast,
+                            long long n, int C) {
+  long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx >= n) return;
+  XB[idx] = flast[(int)(idx % C) / 3] ? Yb[idx] : 0.0;
+}
+
+// Reverse of x_it = x_n + D^-1 prev / h, prev = alpha q + (1-alpha) P and
+// q = cone(rc) for one executed iteration, per (vertex, rollout). Produces
+// z = h rc_bar, the D-dependence coefficient CD for the E-gradient, and the
+// running adjoints of x_n (Xb), prev (PB) and c (Zs).
+__global__ void k_cadj_pre(const double* __restrict__ XB, double* __restrict__ PB, double* __restrict__ Xb,
+                           const double* __restrict__ Xit1, const double* __restrict__ Xn,
+                           const double* __restrict__ Kdiag, const int* __restrict__ cv_of_v,
+                           const double* __restrict__ tRC, const double* __restrict__ tN, double mu, double alpha,
+                           double h, int first, const int* __restrict__ active, double* __restrict__ Zb,
+                           double* __restrict__ Zs, double* __restrict__ CD, int N, int R) {
+  long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx >= (long long)N * R) return;
+  int v = (int)(idx / R), r = (int)(idx % R);
+  long long C = 3LL * R;
+  long long base = (long long)v * C + 3 * r;
+  if (!active[r]) {
+    Zb[base] = 0.0; Zb[base + 1] = 0.0; Zb[base + 2] = 0.0;
+    CD[idx] = 0.0;
+    return;
+  }
+  double kd = Kdiag[v];
+  double pb[3], cd = 0.0;
+  for (int c = 0; c < 3; ++c) {
+    double xb = XB[base + c];
+    Xb[base + c] += xb;
+    pb[c] = PB[base + c] + xb / (kd * h);
+    cd -= xb * (Xit1[base + c] - Xn[base + c]) / kd;
+  }
+  double q[3] = {alpha * pb[0], alpha * pb[1], alpha * pb[2]}, rcb[3] = {q[0], q[1], q[2]};
+  int ci = cv_of_v[v];
+  if (ci >= 0) {
+    long long tb = (long long)ci * C + 3 * r;
+    cone_vjp(tN + tb, mu, tRC + tb, q, rcb);
+  }
+  for (int c = 0; c < 3; ++c) {
+    if (first) {
+      rcb[c] += (1.0 - alpha) * pb[c];
+      PB[base + c] = 0.0;
+    } else {
+      PB[base + c] = (1.0 - alpha) * pb[c];
+    }
+    double z = h * rcb[c];
+    Zb[base + c] = z;
+    Zs[base + c] += z;
+  }
+  CD[idx] = cd;
+}
+
+// rc = h (c + J d(ori) - C (ori - x_n)): Xb += C z, OB = -C z (C = offdiag of
+// the scalar matrix without masses).
+__global__ void k_cadj_cz(const int* __restrict__ Lptr, const int* __restrict__ Lcol, const double* __restrict__ Lval,
+                          const double* __restrict__ Zb, double* __restrict__ Xb, double* __restrict__ OB,
+                          const int* __restrict__ active, int N, int C) {
+  long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx >= (long long)N * C) return;
+  int v = (int)(idx / C), j = (int)(idx % C);
+  if (!active[j / 3]) return;
+  double sum = 0.0;
+  for (int q = Lptr[v]; q < Lptr[v + 1]; ++q) {
+    int u = Lcol[q];
+    if (u != v) sum += Lval[q] * Zb[(long long)u * C + j];
+  }
+  Xb[idx] += sum;
+  OB[idx] = -sum;
+}
+
+template <typename T>
+__global__ void k_gather_masked_add(const T* __restrict__ corner, const int* __restrict__ vptr,
+                                    const int* __restrict__ active, double* __restrict__ B, int N, int C) {
+  long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx >= (long long)N * C) return;
+  int v = (int)(idx / C);
+  int j = (int)(idx % C);
+  if (!active[j / 3]) return;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int q = vptr[v], qe = vptr[v + 1];
+  for (; q + 4 <= qe; q += 4) {
+    s0 += (double)corner[(long long)q * C + j];
+    s1 += (double)corner[(long long)(q + 1) * C + j];
+    s2 += (double)corner[(long long)(q + 2) * C + j];
+    s3 += (double)corner[(long long)(q + 3) * C + j];
+  }
+  for (; q < qe; ++q) s0 += (double)corner[(long long)q * C + j];
+  B[idx] += (s0 + s1) + (s2 + s3);
+}
+
+// x_it_{g-1} adjoint = ori adjoint (+ the final-position adjoint for rollouts
+// whose result is this iteration's input).
+__global__ void k_cadj_post(const double* __restrict__ OB, const double* __restrict__ Gf, const int* __restrict__ inj,
+                            const int* __restrict__ active, double* __restrict__ XB, long long n, int C) {
+  long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx >= n) return;
+  int r = (int)(idx % C) / 3;
+  if (!active[r]) return;
+  XB[idx] = OB[idx] + (inj[r] ? Gf[idx] : 0.0);
 }
 
 // ------------------------------------------------------------------ engine
@@ -1023,6 +1204,31 @@ struct DevBuf {
     }
     return 0;
   }
+};
+
+// Grow a device buffer to `count` elements keeping its first `keep` elements.
+template <typename X>
+int grow_keep(DevBuf<X>& b, size_t count, size_t keep, hipStream_t st) {
+  if (count <= b.n && b.p) return 0;
+  DevBuf<X> nb;
+  if (nb.alloc(count)) return 1;
+  if (keep && b.p) HIPX(hipMemcpyAsync(nb.p, b.p, keep * sizeof(X), hipMemcpyDeviceToDevice, st));
+  HIPX(hipStreamSynchronize(st));
+  std::swap(b.p, nb.p);
+  std::swap(b.n, nb.n);
+  return 0;
+}
+
+// Tape of the contact phase of one step: every launched Jacobi iteration g
+// (all passes) keeps x_it after it, the pre-projection rhs and winning normal
+// of each contact vertex, and which rollouts executed it.
+struct ContactTape {
+  int G = 0;
+  std::vector<char> first;     // iteration opened a detection pass
+  DevBuf<double> xit;          // (G+1) x NC, xit[0] = PD result
+  DevBuf<double> rc, nrm;      // G x ncv x C
+  DevBuf<int> exec;            // G x R
+  DevBuf<int> flast;           // R: result is x_it of the last executed iteration
 };
 
 template <typename X>
@@ -1070,7 +1276,9 @@ struct pdm_scene_s {
   bool act_null = true;
   // contact
   int ncv = 0, nobs = 0;
-  DevBuf<int> cvtx, cv_of_v, imp_stamp, cactive, iactive, cfirst, fresh, npass, citers;
+  DevBuf<int> cvtx, cv_of_v, imp_stamp, cactive, iactive, cfirst, fresh, npass, citers, clast, cinj;
+  std::vector<std::unique_ptr<ContactTape>> ctape;  // per step of the segment being reversed
+  DevBuf<double> cXB, cPB, cOB, cGf, cCD;
   DevBuf<double> Kdiag, d_planes, d_spheres, Ori, Prev, imp_n;
   std::vector<int> npass_host, citers_host;  // R*steps of the last forward
   // adjoint buffers
@@ -1294,7 +1502,7 @@ int ensure_batch_buffers(pdm_scene s) {
     size_t ni = (size_t)s->ncv * s->nobs * s->R;
     if (s->Ori.alloc(nc) || s->Prev.alloc(nc) || s->imp_stamp.alloc(ni) || s->imp_n.alloc(3 * ni) ||
         s->cactive.alloc(s->R) || s->iactive.alloc(s->R) || s->cfirst.alloc(s->R) || s->fresh.alloc(s->R) ||
-        s->npass.alloc(s->R) || s->citers.alloc(s->R))
+        s->npass.alloc(s->R) || s->citers.alloc(s->R) || s->clast.alloc(s->R) || s->cinj.alloc(s->R))
       return 1;
   }
   return 0;
@@ -1376,7 +1584,7 @@ void launch_local(pdm_scene s, const double* Xsrc, const double* act, int act_st
 // Contact phase of one step (World.h:921-1023) for every rollout that has
 // fresh impacts; the host reads back after each detection pass whether any
 // rollout continues. Rollouts finish with x = ori_x.
-int contact_phase(pdm_scene s, const double* act, int act_steps, int step) {
+int contact_phase(pdm_scene s, const double* act, int act_steps, int step, ContactTape* tp) {
   HostScene& S = s->S;
   int N = S.N, C = s->C, R = s->R;
   size_t nc = s->NC();
@@ -1386,8 +1594,15 @@ int contact_phase(pdm_scene s, const double* act, int act_steps, int step) {
   long long ni = (long long)s->ncv * s->nobs * R;
   HIPX(hipMemcpyAsync(s->Ori.p, s->Xit.p, nc * 8, hipMemcpyDeviceToDevice, s->stream));
   hipLaunchKernelGGL(k_contact_reset, dim3(cdiv(std::max<long long>(ni, R), kThreads)), dim3(kThreads), 0, s->stream,
-                     s->imp_stamp.p, ni, s->cactive.p, s->cfirst.p, s->fresh.p, R);
+                     s->imp_stamp.p, ni, s->cactive.p, s->cfirst.p, s->fresh.p, s->clast.p, R);
   std::vector<int> ca(R);
+  size_t cvc = (size_t)s->ncv * C;
+  if (tp) {
+    tp->G = 0;
+    tp->first.clear();
+    if (grow_keep(tp->xit, nc, 0, s->stream)) return 1;
+    HIPX(hipMemcpyAsync(tp->xit.p, s->Xit.p, nc * 8, hipMemcpyDeviceToDevice, s->stream));
+  }
   for (int pass = 0; pass <= S.outer_cap; ++pass) {
     hipLaunchKernelGGL(k_contact_detect, dim3(cdiv((long long)s->ncv * R, kThreads)), dim3(kThreads), 0, s->stream,
                        s->cvtx.p, s->ncv, s->Xn.p, s->Ori.p, R, s->d_planes.p, np, s->d_spheres.p, nsph,
@@ -1400,11 +1615,21 @@ int contact_phase(pdm_scene s, const double* act, int act_steps, int step) {
     bool any = false;
     for (int r = 0; r < R; ++r) any |= ca[r] != 0;
     if (!any) break;
+    if (tp) {
+      size_t g1 = (size_t)tp->G + S.max_iter1;
+      if (grow_keep(tp->xit, (g1 + 1) * nc, (tp->G + 1) * nc, s->stream) ||
+          grow_keep(tp->rc, g1 * cvc, tp->G * cvc, s->stream) || grow_keep(tp->nrm, g1 * cvc, tp->G * cvc, s->stream) ||
+          grow_keep(tp->exec, g1 * R, (size_t)tp->G * R, s->stream))
+        return 1;
+    }
     for (int i = 0; i < S.max_iter1; ++i) {
       hipLaunchKernelGGL(k_contact_begin, ugrid, ublock, 0, s->stream, s->Xit.p, s->Ori.p, s->iactive.p, s->part.p, N,
                          C);
       hipLaunchKernelGGL(k_contact_err, dim3(cdiv(R, 64)), dim3(64), 0, s->stream, s->part.p, nchunk, R,
-                         s->iactive.p, s->cfirst.p, s->citers.p);
+                         s->iactive.p, s->cfirst.p, s->citers.p, s->clast.p);
+      if (tp)
+        HIPX(hipMemcpyAsync(tp->exec.p + (size_t)tp->G * R, s->iactive.p, R * sizeof(int), hipMemcpyDeviceToDevice,
+                            s->stream));
       if (s->precision == PDM_MIXED)
         launch_local<float>(s, s->Ori.p, act, act_steps, step, s->iactive.p);
       else
@@ -1412,8 +1637,19 @@ int contact_phase(pdm_scene s, const double* act, int act_steps, int step) {
       hipLaunchKernelGGL(k_contact_jacobi, dim3(cdiv((long long)N * R, kThreads)), dim3(kThreads), 0, s->stream,
                          s->Lptr.p, s->Lcol.p, s->Lval.p, s->B.p, s->Ori.p, s->Xn.p, s->Kdiag.p, s->cv_of_v.p,
                          s->imp_stamp.p, s->imp_n.p, s->nobs, S.mu, S.alpha, S.h, i == 0 ? 1 : 0, s->iactive.p,
-                         s->Prev.p, s->Xit.p, N, R);
+                         s->Prev.p, s->Xit.p, N, R, tp ? tp->rc.p + (size_t)tp->G * cvc : nullptr,
+                         tp ? tp->nrm.p + (size_t)tp->G * cvc : nullptr);
+      if (tp) {
+        HIPX(hipMemcpyAsync(tp->xit.p + (size_t)(tp->G + 1) * nc, s->Xit.p, nc * 8, hipMemcpyDeviceToDevice,
+                            s->stream));
+        tp->first.push_back(i == 0 ? 1 : 0);
+        tp->G += 1;
+      }
     }
+  }
+  if (tp) {
+    if (grow_keep(tp->flast, R, 0, s->stream)) return 1;
+    HIPX(hipMemcpyAsync(tp->flast.p, s->clast.p, R * sizeof(int), hipMemcpyDeviceToDevice, s->stream));
   }
   HIPX(hipMemcpyAsync(s->Xit.p, s->Ori.p, nc * 8, hipMemcpyDeviceToDevice, s->stream));
   HIPX(hipGetLastError());
@@ -1423,7 +1659,7 @@ int contact_phase(pdm_scene s, const double* act, int act_steps, int step) {
 // One time step on device. tapeY: optional (max_iter+1) x NC buffer for the
 // iterates; tIters: optional R ints receiving the executed iterations.
 int run_step(pdm_scene s, const double* act, int act_steps, int step, double* tapeY, double* tapeXn, double* tapeVs,
-             int* tIters, int* tNpass = nullptr, int* tCiters = nullptr) {
+             int* tIters, int* tNpass = nullptr, int* tCiters = nullptr, ContactTape* ctp = nullptr) {
   HostScene& S = s->S;
   int N = S.N, C = s->C;
   size_t nc = s->NC();
@@ -1467,7 +1703,7 @@ int run_step(pdm_scene s, const double* act, int act_steps, int step, double* ta
   if (s->ncv > 0) {
     HIPX(hipMemsetAsync(s->npass.p, 0, s->R * sizeof(int), s->stream));
     HIPX(hipMemsetAsync(s->citers.p, 0, s->R * sizeof(int), s->stream));
-    if (contact_phase(s, act, act_steps, step)) return 1;
+    if (contact_phase(s, act, act_steps, step, ctp)) return 1;
     if (tNpass) HIPX(hipMemcpyAsync(tNpass, s->npass.p, s->R * sizeof(int), hipMemcpyDeviceToDevice, s->stream));
     if (tCiters) HIPX(hipMemcpyAsync(tCiters, s->citers.p, s->R * sizeof(int), hipMemcpyDeviceToDevice, s->stream));
   }
@@ -1556,21 +1792,76 @@ int forward_impl(pdm_scene s, int steps, const double* x0, const double* v0, con
 }
 
 template <typename T>
-int vjp_iteration(pdm_scene s, const double* Yk, const double* Yk1, const double* act, int act_steps, int step) {
+int vjp_iteration(pdm_scene s, const double* Yk, const double* Yk1, const double* act, int act_steps, int step,
+                  const double* CDp = nullptr, double* out_add = nullptr) {
   HostScene& S = s->S;
   T* corner = (T*)s->corner.p;
   long long nt = (long long)S.T * s->R;
   hipLaunchKernelGGL(k_vjp_tet<T>, dim3(cdiv(nt, kThreads)), dim3(kThreads), 0, s->stream, Yk, Yk1, s->Zb.p, s->tv.p,
                      s->tG.p, s->wr.p, s->wv.p, s->dEr.p, s->dEv.p, s->tslot.p, s->ractive.p, corner, s->gEpart.p, S.T,
-                     s->R);
+                     s->R, CDp ? s->Xn.p : nullptr, CDp);
   if (S.M > 0) {
     long long nm = (long long)S.M * s->R;
     hipLaunchKernelGGL(k_vjp_muscle<T>, dim3(cdiv(nm, kThreads)), dim3(kThreads), 0, s->stream, Yk, s->Zb.p, s->mv.p,
                        s->mg.p, s->mw.p, s->mgroup.p, s->mslot.p, act, act_steps * S.n_groups, step * S.n_groups,
                        s->ractive.p, corner, s->gMpart.p, S.M, s->R);
   }
-  hipLaunchKernelGGL(k_gather_masked<T>, dim3(cdiv(s->NC(), kThreads)), dim3(kThreads), 0, s->stream, corner,
-                     s->vptr.p, s->ractive.p, s->Yb.p, S.N, s->C);
+  if (out_add)
+    hipLaunchKernelGGL(k_gather_masked_add<T>, dim3(cdiv(s->NC(), kThreads)), dim3(kThreads), 0, s->stream, corner,
+                       s->vptr.p, s->ractive.p, out_add, S.N, s->C);
+  else
+    hipLaunchKernelGGL(k_gather_masked<T>, dim3(cdiv(s->NC(), kThreads)), dim3(kThreads), 0, s->stream, corner,
+                       s->vptr.p, s->ractive.p, s->Yb.p, S.N, s->C);
+  return 0;
+}
+
+
+// Reverse of the taped contact phase of one step (branches and contact
+// normals frozen). In: Yb = adjoint of the step's final positions. Out: Yb =
+// adjoint of the PD result; Xb, Zs, gEpart, gMpart accumulate.
+int contact_reverse(pdm_scene s, ContactTape& tp, const double* act, int act_steps, int step) {
+  HostScene& S = s->S;
+  int N = S.N, C = s->C, R = s->R, G = tp.G;
+  size_t nc = s->NC(), cvc = (size_t)s->ncv * C;
+  std::vector<int> ex((size_t)G * R), fl(R);
+  HIPX(hipMemcpyAsync(ex.data(), tp.exec.p, ex.size() * sizeof(int), hipMemcpyDeviceToHost, s->stream));
+  HIPX(hipMemcpyAsync(fl.data(), tp.flast.p, R * sizeof(int), hipMemcpyDeviceToHost, s->stream));
+  HIPX(hipStreamSynchronize(s->stream));
+  std::vector<int> glast(R, -1);
+  for (int g = 0; g < G; ++g)
+    for (int r = 0; r < R; ++r)
+      if (ex[(size_t)g * R + r]) glast[r] = g;
+  HIPX(hipMemcpyAsync(s->cGf.p, s->Yb.p, nc * 8, hipMemcpyDeviceToDevice, s->stream));
+  hipLaunchKernelGGL(k_cadj_init, dim3(cdiv(nc, kThreads)), dim3(kThreads), 0, s->stream, s->cXB.p, s->Yb.p,
+                     tp.flast.p, (long long)nc, C);
+  HIPX(hipMemsetAsync(s->cPB.p, 0, nc * 8, s->stream));
+  std::vector<int> inj((size_t)G * R);
+  for (int g = 0; g < G; ++g)
+    for (int r = 0; r < R; ++r) inj[(size_t)g * R + r] = (g == glast[r] && !fl[r]) ? 1 : 0;
+  DevBuf<int> dinj;
+  if (dinj.upload(inj.data(), inj.size())) return 1;
+  for (int g = G - 1; g >= 0; --g) {
+    bool any = false;
+    for (int r = 0; r < R; ++r) any |= ex[(size_t)g * R + r] != 0;
+    if (!any) continue;
+    HIPX(hipMemcpyAsync(s->ractive.p, tp.exec.p + (size_t)g * R, R * sizeof(int), hipMemcpyDeviceToDevice, s->stream));
+    const double* ori = tp.xit.p + (size_t)g * nc;
+    hipLaunchKernelGGL(k_cadj_pre, dim3(cdiv((long long)N * R, kThreads)), dim3(kThreads), 0, s->stream, s->cXB.p,
+                       s->cPB.p, s->Xb.p, tp.xit.p + (size_t)(g + 1) * nc, s->Xn.p, s->Kdiag.p, s->cv_of_v.p,
+                       tp.rc.p + (size_t)g * cvc, tp.nrm.p + (size_t)g * cvc, S.mu, S.alpha, S.h, (int)tp.first[g],
+                       s->ractive.p, s->Zb.p, s->Zs.p, s->cCD.p, N, R);
+    hipLaunchKernelGGL(k_cadj_cz, dim3(cdiv(nc, kThreads)), dim3(kThreads), 0, s->stream, s->Lptr.p, s->Lcol.p,
+                       s->Lval.p, s->Zb.p, s->Xb.p, s->cOB.p, s->ractive.p, N, C);
+    int rc = s->precision == PDM_MIXED
+                 ? vjp_iteration<float>(s, ori, ori, act, act_steps, step, s->cCD.p, s->cOB.p)
+                 : vjp_iteration<double>(s, ori, ori, act, act_steps, step, s->cCD.p, s->cOB.p);
+    if (rc) return 1;
+    hipLaunchKernelGGL(k_cadj_post, dim3(cdiv(nc, kThreads)), dim3(kThreads), 0, s->stream, s->cOB.p, s->cGf.p,
+                       dinj.p + (size_t)g * R, s->ractive.p, s->cXB.p, (long long)nc, C);
+  }
+  HIPX(hipMemcpyAsync(s->Yb.p, s->cXB.p, nc * 8, hipMemcpyDeviceToDevice, s->stream));
+  HIPX(hipStreamSynchronize(s->stream));  // dinj is released on return
+  HIPX(hipGetLastError());
   return 0;
 }
 
@@ -1580,11 +1871,6 @@ int backward_impl(pdm_scene s, const double* gxT, const double* gvT, double* gx0
     g_err = "backward needs a recorded forward (record=1)";
     return 1;
   }
-  for (int v : s->npass_host)
-    if (v > 0) {
-      g_err = "the recorded forward had active contacts: the contact-phase adjoint is not available in this build";
-      return 1;
-    }
   HostScene& S = s->S;
   int N = S.N, C = s->C, R = s->R, steps = s->tape_steps;
   size_t nc = s->NC();
@@ -1597,6 +1883,11 @@ int backward_impl(pdm_scene s, const double* gxT, const double* gvT, double* gx0
       s->Zb.alloc(nc) || s->LZ.alloc(nc) || s->gEpart.alloc((size_t)S.T * R) || s->ractive.alloc(R))
     return 1;
   if (S.M > 0 && s->gMpart.alloc((size_t)S.M * R)) return 1;
+  if (s->ncv > 0) {
+    if (s->cXB.alloc(nc) || s->cPB.alloc(nc) || s->cOB.alloc(nc) || s->cGf.alloc(nc) || s->cCD.alloc((size_t)N * R))
+      return 1;
+    while ((int)s->ctape.size() < seg) s->ctape.emplace_back(new ContactTape());
+  }
   size_t nact = (size_t)R * steps * std::max(S.n_groups, 1);
   if (s->gact.alloc(nact)) return 1;
   HIPX(hipEventRecord(s->ev0, s->stream));
@@ -1618,7 +1909,8 @@ int backward_impl(pdm_scene s, const double* gxT, const double* gvT, double* gx0
     for (int n = s0; n < s1; ++n) {
       int q = n - s0;
       if (run_step(s, act, steps, n, s->tY.p + (size_t)q * K1 * nc, s->tXn.p + (size_t)q * nc,
-                   s->tVs.p + (size_t)q * nc, s->tIters.p + (size_t)q * R))
+                   s->tVs.p + (size_t)q * nc, s->tIters.p + (size_t)q * R, nullptr, nullptr,
+                   s->ncv > 0 ? s->ctape[q].get() : nullptr))
         return 1;
     }
     HIPX(hipMemcpyAsync(it_h.data(), s->tIters.p, (size_t)(s1 - s0) * R * sizeof(int), hipMemcpyDeviceToHost,
@@ -1632,6 +1924,7 @@ int backward_impl(pdm_scene s, const double* gxT, const double* gvT, double* gx0
                          s->Xb.p, s->Zs.p, (long long)nc, gh);
       const double* Y = s->tY.p + (size_t)q * K1 * nc;
       if (S.M > 0) HIPX(hipMemsetAsync(s->gMpart.p, 0, (size_t)S.M * R * 8, s->stream));
+      if (s->ncv > 0 && s->ctape[q]->G > 0 && contact_reverse(s, *s->ctape[q], act, steps, n)) return 1;
       for (int k = kmax - 1; k >= 0; --k) {
         hipLaunchKernelGGL(k_active_rev, dim3(cdiv(R, 64)), dim3(64), 0, s->stream, s->tIters.p + (size_t)q * R, k, R,
                            s->ractive.p);

@@ -391,11 +391,23 @@ struct Skyline {
   }
 };
 
+// One executed Jacobi iteration of the contact phase, as taped for the
+// reverse sweep: input ori, output x_it, whether it opened a pass, and the
+// winning contact basis + pre-projection rhs of every contact vertex.
+struct ContactIter {
+  std::vector<double> ori, xit;
+  bool first = false;
+  std::vector<int> cv;        // vertices with an impact
+  std::vector<double> B, f;   // 9 and 3 per cv
+};
+
 struct StepTape {
   std::vector<double> xn, vstar;           // start-of-step state (3N)
   std::vector<std::vector<double>> ys;     // PD iterates y_0..y_K
   std::vector<double> act;                 // activations used this step
   bool contact_active = false;
+  std::vector<ContactIter> citers;         // executed contact iterations
+  bool final_latest = true;                // result = x_it of the last iteration (else its input)
 };
 
 struct Scene {
@@ -757,6 +769,8 @@ void step(Scene& s, double* x, double* v, const double* act, StepTape* tape, int
     tape->xn = xn;
     tape->vstar = vstar;
     tape->ys.clear();
+    tape->citers.clear();
+    tape->final_latest = true;
     tape->ys.push_back(qn);
     if (act) tape->act.assign(act, act + s.n_groups);
     else tape->act.assign(s.n_groups, 0.0);
@@ -782,6 +796,7 @@ void step(Scene& s, double* x, double* v, const double* act, StepTape* tape, int
     // Contact phase, World.h:921-1023
     std::vector<Impact> impacts;
     std::vector<double> xit = y, oold(n3, 0.0), prev(n3, 0.0), rc(n3);
+    bool last_begin = true;
     while (true) {
       std::vector<Impact> fresh;
       for (int ci = 0; ci < (int)s.contact_vtx.size(); ++ci) {
@@ -815,6 +830,7 @@ void step(Scene& s, double* x, double* v, const double* act, StepTape* tape, int
       for (size_t k = 0; k < impacts.size(); ++k) contact_basis(impacts[k].n, &Bs[9 * k]);
       for (int i = 0; i < s.max_iter1; ++i) {
         ori_x = xit;
+        last_begin = true;
         double err = 0;
         for (int k = 0; k < n3; ++k) err = std::max(err, std::fabs(ori_x[k] - oold[k]));
         if (err < kEps) break;
@@ -836,8 +852,24 @@ void step(Scene& s, double* x, double* v, const double* act, StepTape* tape, int
           for (int r = 0; r < 3; ++r) xit[3 * vtx + r] = xn[3 * vtx + r] + (prev[3 * vtx + r] / s.h) / s.Kdiag[vtx];
         oold = ori_x;
         ++cit;
+        last_begin = false;
+        if (tape) {
+          ContactIter ci;
+          ci.ori = ori_x;
+          ci.xit = xit;
+          ci.first = i == 0;
+          std::map<int, size_t> win;  // last impact of each vertex wins
+          for (size_t k = 0; k < impacts.size(); ++k) win[impacts[k].vtx] = k;
+          for (auto& kv : win) {
+            ci.cv.push_back(kv.first);
+            ci.B.insert(ci.B.end(), &Bs[9 * kv.second], &Bs[9 * kv.second] + 9);
+            ci.f.insert(ci.f.end(), &orig[3 * kv.first], &orig[3 * kv.first] + 3);
+          }
+          tape->citers.push_back(std::move(ci));
+        }
       }
     }
+    if (tape) tape->final_latest = last_begin;
   }
   if (it_c) *it_c = cit;
   if (outer) *outer = outer_n;
@@ -849,7 +881,130 @@ void step(Scene& s, double* x, double* v, const double* act, StepTape* tape, int
   }
 }
 
-// Reverse sweep of one step (no contact). Inputs: gx, gv = dL/dx_{n+1}, dL/dv_{n+1}.
+
+// VJP of the cone projection at a contact vertex (frozen basis and branch):
+// given qbar (adjoint of the projected rhs) return the adjoint of the rhs.
+void project_contact_vjp(const double* B, double mu, const double* f_in, const double* qbar, double* out) {
+  double fl[3], rb[3];
+  for (int c = 0; c < 3; ++c) {
+    fl[c] = B[0 * 3 + c] * f_in[0] + B[1 * 3 + c] * f_in[1] + B[2 * 3 + c] * f_in[2];
+    rb[c] = B[0 * 3 + c] * qbar[0] + B[1 * 3 + c] * qbar[1] + B[2 * 3 + c] * qbar[2];
+  }
+  if (fl[0] > 0.0) {
+    for (int r = 0; r < 3; ++r) out[r] = qbar[r];
+    return;
+  }
+  double ft = std::sqrt(fl[1] * fl[1] + fl[2] * fl[2]);
+  double fb[3];
+  if (!(ft > mu * (-fl[0]))) {  // stick: r = -f
+    fb[0] = -rb[0];
+    fb[1] = -rb[1];
+    fb[2] = -rb[2];
+  } else {  // slide: r_n = -f_n, r_t = mu f_n f_t / |f_t|
+    double u1 = fl[1] / ft, u2 = fl[2] / ft;
+    double ur = u1 * rb[1] + u2 * rb[2];
+    fb[0] = -rb[0] + mu * ur;
+    double sc = mu * fl[0] / ft;
+    fb[1] = sc * (rb[1] - ur * u1);
+    fb[2] = sc * (rb[2] - ur * u2);
+  }
+  for (int r = 0; r < 3; ++r) out[r] = B[r * 3 + 0] * fb[0] + B[r * 3 + 1] * fb[1] + B[r * 3 + 2] * fb[2];
+}
+
+// Reverse sweep of the taped contact iterations. In: ybar = adjoint of the
+// step's final positions. Out: ybar = adjoint of the PD result y; xbar, zsum
+// and the parameter gradients accumulate.
+void contact_backward(Scene& s, const StepTape& tp, std::vector<double>& ybar, std::vector<double>& xbar,
+                      std::vector<double>& zsum, double* gE, double* gnu, double* gact) {
+  int N = s.N, n3 = 3 * N;
+  const double h = s.h, al = s.alpha;
+  const double* xn = tp.xn.data();
+  const double* act = tp.act.empty() ? nullptr : tp.act.data();
+  int J = (int)tp.citers.size();
+  std::vector<double> gfin = ybar, XB(n3, 0.0), PB(n3, 0.0), rcb(n3), z(n3), coefD(N), ob(n3);
+  if (tp.final_latest) XB = gfin;
+  for (int j = J - 1; j >= 0; --j) {
+    const ContactIter& it = tp.citers[j];
+    for (int k = 0; k < n3; ++k) xbar[k] += XB[k];
+    for (int v = 0; v < N; ++v) {
+      double cd = 0;
+      for (int c = 0; c < 3; ++c) {
+        PB[3 * v + c] += XB[3 * v + c] / (s.Kdiag[v] * h);
+        cd -= XB[3 * v + c] * (it.xit[3 * v + c] - xn[3 * v + c]) / s.Kdiag[v];
+      }
+      coefD[v] = cd;
+    }
+    for (int k = 0; k < n3; ++k) rcb[k] = al * PB[k];
+    for (size_t q = 0; q < it.cv.size(); ++q) {
+      int v = it.cv[q];
+      double qb[3] = {al * PB[3 * v], al * PB[3 * v + 1], al * PB[3 * v + 2]}, o[3];
+      project_contact_vjp(&it.B[9 * q], s.mu, &it.f[3 * q], qb, o);
+      for (int c = 0; c < 3; ++c) rcb[3 * v + c] = o[c];
+    }
+    if (it.first) {
+      for (int k = 0; k < n3; ++k) {
+        rcb[k] += (1.0 - al) * PB[k];
+        PB[k] = 0.0;
+      }
+    } else {
+      for (int k = 0; k < n3; ++k) PB[k] *= (1.0 - al);
+    }
+    for (int k = 0; k < n3; ++k) {
+      z[k] = h * rcb[k];
+      zsum[k] += z[k];
+    }
+    add_Cy(s, z.data(), 1.0, xbar.data());
+    std::fill(ob.begin(), ob.end(), 0.0);
+    add_Cy(s, z.data(), -1.0, ob.data());
+    const double* ori = it.ori.data();
+    for (size_t t = 0; t < s.tets.size(); ++t) {
+      const Tet& te = s.tets[t];
+      double F[9], Fz[9], md[9], mdv[9], Fbar[9];
+      tet_F(te, ori, F);
+      tet_F(te, z.data(), Fz);
+      corotate_project(F, md, mdv);
+      double extra = 0;
+      for (int a = 0; a < 4; ++a) {
+        int v = te.v[a];
+        double g2 = te.G[a][0] * te.G[a][0] + te.G[a][1] * te.G[a][1] + te.G[a][2] * te.G[a][2];
+        double zd = 0;
+        for (int c = 0; c < 3; ++c) zd += z[3 * v + c] * (ori[3 * v + c] - xn[3 * v + c]);
+        extra += g2 * (zd + coefD[v]);
+      }
+      double pr = extra, pv = extra;
+      for (int i = 0; i < 9; ++i) {
+        pr += Fz[i] * (md[i] - F[i]);
+        pv += Fz[i] * (mdv[i] - F[i]);
+      }
+      double opn = 1.0 + s.nu_fwd, vol = te.vol;
+      if (gE) gE[t] += pr * vol / opn + pv * vol * s.nu_ctor[t] / opn;
+      if (gnu) *gnu += -(pr * s.wr[t] + pv * s.wv[t]) / opn;
+      corotate_vjp(F, s.wr[t], s.wv[t], Fz, Fbar);
+      for (int k = 0; k < 4; ++k)
+        for (int r = 0; r < 3; ++r)
+          ob[3 * te.v[k] + r] += Fbar[r * 3 + 0] * te.G[k][0] + Fbar[r * 3 + 1] * te.G[k][1] + Fbar[r * 3 + 2] * te.G[k][2];
+    }
+    for (const Muscle& m : s.muscles) {
+      double a = act ? act[m.group] : 0.0;
+      double Ff[3] = {0, 0, 0}, Fz[3] = {0, 0, 0};
+      for (int k = 0; k < 4; ++k)
+        for (int r = 0; r < 3; ++r) {
+          Ff[r] += ori[3 * m.v[k] + r] * m.g[k];
+          Fz[r] += z[3 * m.v[k] + r] * m.g[k];
+        }
+      if (gact) gact[m.group] += -m.w * (Fz[0] * Ff[0] + Fz[1] * Ff[1] + Fz[2] * Ff[2]);
+      double sc = m.w * (1.0 - a);
+      for (int k = 0; k < 4; ++k)
+        for (int r = 0; r < 3; ++r) ob[3 * m.v[k] + r] += sc * Fz[r] * m.g[k];
+    }
+    XB = ob;
+    if (j == J - 1 && !tp.final_latest)
+      for (int k = 0; k < n3; ++k) XB[k] += gfin[k];
+  }
+  ybar = XB;
+}
+
+// Reverse sweep of one step (contact iterations first, then the PD loop). Inputs: gx, gv = dL/dx_{n+1}, dL/dv_{n+1}.
 // Outputs (accumulated): gx_n, gv_n, gE (per tet), gnu, gact (per group).
 void step_backward(Scene& s, const StepTape& tp, const double* gx, const double* gv, double* gxn, double* gvn,
                    double* gE, double* gnu, double* gact) {
@@ -861,6 +1016,7 @@ void step_backward(Scene& s, const StepTape& tp, const double* gx, const double*
     ybar[k] = gx[k] + gam / h * gv[k];
     xbar[k] = -gam / h * gv[k];
   }
+  if (!tp.citers.empty()) contact_backward(s, tp, ybar, xbar, zsum, gE, gnu, gact);
   const double* act = tp.act.empty() ? nullptr : tp.act.data();
   for (int it = K - 1; it >= 0; --it) {
     const double* yk = tp.ys[it].data();
@@ -1314,8 +1470,8 @@ int orc_max_threads() {
 
 // Gradient of <gx, x_T> + <gv, v_T> w.r.t. x0, v0, per-tet E, nu (the
 // forward value entering mMu) and activations (steps x n_groups), over the tape
-// of the last recorded rollout. Returns 2 if a step had active contacts (the
-// contact-phase reverse sweep is not restated here).
+// of the last recorded rollout; contact iterations are reversed with their
+// branches (take-off / stick / slide) and contact normals frozen.
 int orc_backward(void* p, const double* gxT, const double* gvT, double* gx0, double* gv0, double* gE, double* gnu,
                  double* gact) {
   Scene* s = (Scene*)p;
@@ -1328,7 +1484,6 @@ int orc_backward(void* p, const double* gxT, const double* gvT, double* gx0, dou
   if (gact) std::fill(gact, gact + (size_t)steps * s->n_groups, 0.0);
   int rc = 0;
   for (int n = steps - 1; n >= 0; --n) {
-    if (s->tape[n].contact_active) rc = 2;
     std::vector<double> gxn(n3, 0.0), gvn(n3, 0.0);
     step_backward(*s, s->tape[n], gx.data(), gv.data(), gxn.data(), gvn.data(), gE, gnu,
                   gact ? gact + (size_t)n * s->n_groups : nullptr);

@@ -170,3 +170,31 @@ def test_contact_forward_matches_oracle(eng, oracle_mod, sphere):
         assert np.array_equal(s.iterations()[r], io[:, 0])
         assert np.array_equal(passes[r], io[:, 2]) and np.array_equal(jit[r], io[:, 1])
         assert rel(xT[r], xo) < 1e-9 and rel(vT[r], vo) < 1e-9
+
+
+@pytest.mark.parametrize("sphere", [False, True])
+def test_contact_gradient_matches_oracle(eng, oracle_mod, sphere):
+    """Adjoint through the contact phase (branches and normals frozen, as in
+    the oracle's reverse sweep) at the FP64 bar."""
+    d = contact_scene(sphere=sphere)
+    R = 2
+    s = eng.Scene(d, R=R, flags=0, checkpoint_every=5)
+    o = oracle_mod.OracleScene(d)
+    rng = np.random.default_rng(13)
+    x0 = np.tile(d.x_rest.ravel(), (R, 1))
+    v0 = np.zeros_like(x0)
+    v0[1, 0::3] = 0.3
+    steps = 14
+    xT, vT = s.forward_host(x0, v0, steps, record=True)
+    gx = rng.standard_normal((R, 3 * d.N))
+    gv = 0.01 * rng.standard_normal((R, 3 * d.N))
+    g = s.backward_host(gx, gv)
+    gE = np.zeros(d.T)
+    for r in range(R):
+        xo, vo, io = o.rollout(x0[r], v0[r], steps, record=True)
+        assert io[:, 1].sum() > 0
+        go, _ = o.backward(gx[r], gv[r])
+        assert rel(g["x0"][r], go["x0"]) < 1e-9
+        assert rel(g["v0"][r], go["v0"]) < 1e-9
+        gE += go["E"]
+    assert rel(g["E"], gE) < 1e-9

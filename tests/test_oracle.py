@@ -325,3 +325,40 @@ def test_bridgesus_fixture_runs(oracle_mod):
     xT, vT, its = s.rollout(x0, np.zeros_like(x0), 3)
     assert np.all(np.isfinite(xT))
     assert np.abs(xT.reshape(-1, 3)[d.attach_idx] - d.x_rest[d.attach_idx]).max() < 1e-12
+
+
+def test_contact_gradient_vs_central_fd_plane(oracle_mod):
+    """Contact-phase reverse sweep (take-off / stick / slide branches frozen):
+    exact for static planes (constant contact normal)."""
+    x, t = scene.box_grid(5, 2, 2, origin=(0, 0, 0.015))
+    for mu in (0.5, 0.2):
+        d = scene.SceneDesc(x_rest=x, tets=t, total_mass=2.0, E=np.full(t.shape[0], 8e4), h=0.01, max_iter=10,
+                            gravity=(0, 0, -9.81), contact=True, max_iter1=10, mu=mu,
+                            planes=np.array([[0, 0, 0, 0, 0, 1.0]]))
+        s = oracle_mod.OracleScene(d)
+        x0 = x.ravel().copy()
+        v0 = np.zeros_like(x0)
+        v0[0::3] = 0.3
+        steps = 14
+        rng = np.random.default_rng(12)
+        gx = rng.standard_normal(x0.shape)
+        gv = 0.01 * rng.standard_normal(x0.shape)
+        _, _, its = s.rollout(x0, v0, steps, record=True)
+        assert its[:, 1].sum() > 0
+        g, rc = s.backward(gx, gv)
+
+        def loss(E=8e4, dx=0.0, dv=0.0):
+            s.set_material(np.full(s.T, E), 0.4)
+            xT, vT, _ = s.rollout(x0 + dx, v0 + dv, steps)
+            return gx @ xT + gv @ vT
+
+        hE = 8e4 * 1e-6
+        fd = (loss(E=8e4 + hE) - loss(E=8e4 - hE)) / (2 * hE)
+        assert abs(fd - g["E"].sum()) < 1e-5 * abs(fd)
+        dx = 1e-7 * rng.standard_normal(x0.shape)
+        fd = (loss(dx=dx) - loss(dx=-dx)) / 2
+        assert abs(fd - g["x0"] @ dx) < 1e-5 * abs(fd)
+        dv = 1e-6 * rng.standard_normal(x0.shape)
+        fd = (loss(dv=dv) - loss(dv=-dv)) / 2
+        assert abs(fd - g["v0"] @ dv) < 1e-5 * abs(fd)
+        s.set_material(np.full(s.T, 8e4), 0.4)
