@@ -66,7 +66,7 @@ def main():
     ap.add_argument("--sim-steps", type=int, default=100)
     ap.add_argument("--ckpt", type=int, default=20)
     ap.add_argument("--plain-steps", type=int, default=10, help="sim steps of the plain FP64 comparator")
-    ap.add_argument("--contact", type=int, default=0)
+    ap.add_argument("--contact", type=int, default=1)
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
@@ -118,8 +118,9 @@ def main():
         tm = sc.timing()
         fwd_ms.append(tm["forward_ms"])
         bwd_ms.append(tm["backward_ms"])
-        log("pass %d: %.3fs fwd %.1fms bwd %.1fms mean iters %.2f" % (k, dt, tm["forward_ms"], tm["backward_ms"],
-                                                                      its.mean()))
+        cps, cjt = sc.contact_stats()
+        log("pass %d: %.3fs fwd %.1fms bwd %.1fms mean iters %.2f contact passes/step %.2f jacobi/step %.2f" % (
+            k, dt, tm["forward_ms"], tm["backward_ms"], its.mean(), cps.mean(), cjt.mean()))
     total_t = sum(times)
     value = sum(tet_its) / total_t
     out = {
