@@ -1793,13 +1793,13 @@ int forward_impl(pdm_scene s, int steps, const double* x0, const double* v0, con
 
 template <typename T>
 int vjp_iteration(pdm_scene s, const double* Yk, const double* Yk1, const double* act, int act_steps, int step,
-                  const double* CDp = nullptr, double* out_add = nullptr) {
+                  const double* CDp = nullptr, double* out_add = nullptr, const double* Xn = nullptr) {
   HostScene& S = s->S;
   T* corner = (T*)s->corner.p;
   long long nt = (long long)S.T * s->R;
   hipLaunchKernelGGL(k_vjp_tet<T>, dim3(cdiv(nt, kThreads)), dim3(kThreads), 0, s->stream, Yk, Yk1, s->Zb.p, s->tv.p,
                      s->tG.p, s->wr.p, s->wv.p, s->dEr.p, s->dEv.p, s->tslot.p, s->ractive.p, corner, s->gEpart.p, S.T,
-                     s->R, CDp ? s->Xn.p : nullptr, CDp);
+                     s->R, CDp ? Xn : nullptr, CDp);
   if (S.M > 0) {
     long long nm = (long long)S.M * s->R;
     hipLaunchKernelGGL(k_vjp_muscle<T>, dim3(cdiv(nm, kThreads)), dim3(kThreads), 0, s->stream, Yk, s->Zb.p, s->mv.p,
@@ -1819,7 +1819,7 @@ int vjp_iteration(pdm_scene s, const double* Yk, const double* Yk1, const double
 // Reverse of the taped contact phase of one step (branches and contact
 // normals frozen). In: Yb = adjoint of the step's final positions. Out: Yb =
 // adjoint of the PD result; Xb, Zs, gEpart, gMpart accumulate.
-int contact_reverse(pdm_scene s, ContactTape& tp, const double* act, int act_steps, int step) {
+int contact_reverse(pdm_scene s, ContactTape& tp, const double* Xn, const double* act, int act_steps, int step) {
   HostScene& S = s->S;
   int N = S.N, C = s->C, R = s->R, G = tp.G;
   size_t nc = s->NC(), cvc = (size_t)s->ncv * C;
@@ -1847,14 +1847,14 @@ int contact_reverse(pdm_scene s, ContactTape& tp, const double* act, int act_ste
     HIPX(hipMemcpyAsync(s->ractive.p, tp.exec.p + (size_t)g * R, R * sizeof(int), hipMemcpyDeviceToDevice, s->stream));
     const double* ori = tp.xit.p + (size_t)g * nc;
     hipLaunchKernelGGL(k_cadj_pre, dim3(cdiv((long long)N * R, kThreads)), dim3(kThreads), 0, s->stream, s->cXB.p,
-                       s->cPB.p, s->Xb.p, tp.xit.p + (size_t)(g + 1) * nc, s->Xn.p, s->Kdiag.p, s->cv_of_v.p,
+                       s->cPB.p, s->Xb.p, tp.xit.p + (size_t)(g + 1) * nc, Xn, s->Kdiag.p, s->cv_of_v.p,
                        tp.rc.p + (size_t)g * cvc, tp.nrm.p + (size_t)g * cvc, S.mu, S.alpha, S.h, (int)tp.first[g],
                        s->ractive.p, s->Zb.p, s->Zs.p, s->cCD.p, N, R);
     hipLaunchKernelGGL(k_cadj_cz, dim3(cdiv(nc, kThreads)), dim3(kThreads), 0, s->stream, s->Lptr.p, s->Lcol.p,
                        s->Lval.p, s->Zb.p, s->Xb.p, s->cOB.p, s->ractive.p, N, C);
     int rc = s->precision == PDM_MIXED
-                 ? vjp_iteration<float>(s, ori, ori, act, act_steps, step, s->cCD.p, s->cOB.p)
-                 : vjp_iteration<double>(s, ori, ori, act, act_steps, step, s->cCD.p, s->cOB.p);
+                 ? vjp_iteration<float>(s, ori, ori, act, act_steps, step, s->cCD.p, s->cOB.p, Xn)
+                 : vjp_iteration<double>(s, ori, ori, act, act_steps, step, s->cCD.p, s->cOB.p, Xn);
     if (rc) return 1;
     hipLaunchKernelGGL(k_cadj_post, dim3(cdiv(nc, kThreads)), dim3(kThreads), 0, s->stream, s->cOB.p, s->cGf.p,
                        dinj.p + (size_t)g * R, s->ractive.p, s->cXB.p, (long long)nc, C);
@@ -1924,7 +1924,9 @@ int backward_impl(pdm_scene s, const double* gxT, const double* gvT, double* gx0
                          s->Xb.p, s->Zs.p, (long long)nc, gh);
       const double* Y = s->tY.p + (size_t)q * K1 * nc;
       if (S.M > 0) HIPX(hipMemsetAsync(s->gMpart.p, 0, (size_t)S.M * R * 8, s->stream));
-      if (s->ncv > 0 && s->ctape[q]->G > 0 && contact_reverse(s, *s->ctape[q], act, steps, n)) return 1;
+      if (s->ncv > 0 && s->ctape[q]->G > 0 &&
+          contact_reverse(s, *s->ctape[q], s->tXn.p + (size_t)q * nc, act, steps, n))
+        return 1;
       for (int k = kmax - 1; k >= 0; --k) {
         hipLaunchKernelGGL(k_active_rev, dim3(cdiv(R, 64)), dim3(64), 0, s->stream, s->tIters.p + (size_t)q * R, k, R,
                            s->ractive.p);
