@@ -65,7 +65,8 @@ def main():
     ap.add_argument("--rollouts", type=int, default=256)
     ap.add_argument("--sim-steps", type=int, default=100)
     ap.add_argument("--ckpt", type=int, default=20)
-    ap.add_argument("--plain-steps", type=int, default=10, help="sim steps of the plain FP64 comparator")
+    ap.add_argument("--plain-steps", type=int, default=0,
+                    help="sim steps of the plain FP64 comparator (0: same as --sim-steps)")
     ap.add_argument("--contact", type=int, default=1)
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
@@ -148,14 +149,12 @@ def main():
     sc.close()
     del sc
     if not args.no_plain:
-        ps = args.plain_steps
+        ps = args.plain_steps or S
         pl = engine.Scene(d, R=1, precision=engine.PDM_FP64, flags=engine.PDM_HOST_EARLY_EXIT,
                           checkpoint_every=args.ckpt)
         px0 = x0[:1].contiguous()
         pv0 = v0[:1].contiguous()
         pa = acts[:1, :ps].contiguous()
-        pl.forward_device(px0, pv0, ps, act=pa, record=True)
-        pl.backward_device(gx[:1].contiguous(), gv[:1].contiguous())
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         pl.forward_device(px0, pv0, ps, act=pa, record=True)
