@@ -44,6 +44,16 @@ constexpr int kThreads = 256;
 
 inline int cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
 
+// MI355X dispatches workgroups round-robin over its 8 XCDs (each with its
+// own L2). Remap the hardware block index so that every XCD processes one
+// contiguous range of logical blocks: neighbouring tets (sharing vertices) and
+// the column tiles of one supernode then hit the same L2.
+__device__ __forceinline__ int xcd_block() {
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int q = nb >> 3, rem = nb & 7, k = b & 7;
+  return k * q + min(k, rem) + (b >> 3);
+}
+
 // ------------------------------------------------------------------ kernels
 __global__ void k_caller_to_engine(const double* __restrict__ src, double* __restrict__ dst,
                                    const int* __restrict__ perm, int N, int R) {
@@ -70,7 +80,7 @@ __global__ void k_engine_to_caller(const double* __restrict__ src, double* __res
 // Y = A X for the scalar CSR A applied to every column.
 __global__ void k_spmv(const int* __restrict__ ptr, const int* __restrict__ col, const double* __restrict__ val,
                        const double* __restrict__ X, double* __restrict__ Y, int N, int C) {
-  long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  long long idx = (long long)xcd_block() * blockDim.x + threadIdx.x;
   if (idx >= (long long)N * C) return;
   int v = (int)(idx / C);
   int j = (int)(idx % C);
@@ -105,7 +115,7 @@ __global__ __launch_bounds__(256) void k_local_tet(const double* __restrict__ X,
                                                    const double* __restrict__ wv, const int* __restrict__ tslot,
                                                    const int* __restrict__ active, T* __restrict__ corner, int Tn,
                                                    int R) {
-  long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  long long idx = (long long)xcd_block() * blockDim.x + threadIdx.x;
   if (idx >= (long long)Tn * R) return;
   int t = (int)(idx / R);
   int r = (int)(idx % R);
@@ -147,7 +157,7 @@ __global__ void k_local_muscle(const double* __restrict__ X, const int* __restri
                                const double* __restrict__ mw, const int* __restrict__ mgroup,
                                const int* __restrict__ mslot, const double* __restrict__ act, int act_stride_r,
                                int act_off, const int* __restrict__ active, T* __restrict__ corner, int M, int R) {
-  long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  long long idx = (long long)xcd_block() * blockDim.x + threadIdx.x;
   if (idx >= (long long)M * R) return;
   int m = (int)(idx / R);
   int r = (int)(idx % R);
@@ -183,7 +193,7 @@ __global__ void k_local_muscle(const double* __restrict__ X, const int* __restri
 template <typename T>
 __global__ void k_gather(const double* __restrict__ Cc, const T* __restrict__ corner, const int* __restrict__ vptr,
                          double* __restrict__ B, int N, int C) {
-  long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  long long idx = (long long)xcd_block() * blockDim.x + threadIdx.x;
   if (idx >= (long long)N * C) return;
   int v = (int)(idx / C);
   int j = (int)(idx % C);
@@ -204,7 +214,7 @@ __global__ void k_gather(const double* __restrict__ Cc, const T* __restrict__ co
 template <typename T>
 __global__ void k_gather_masked(const T* __restrict__ corner, const int* __restrict__ vptr,
                                 const int* __restrict__ active, double* __restrict__ B, int N, int C) {
-  long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  long long idx = (long long)xcd_block() * blockDim.x + threadIdx.x;
   if (idx >= (long long)N * C) return;
   int v = (int)(idx / C);
   int j = (int)(idx % C);
@@ -302,8 +312,9 @@ __device__ __forceinline__ v4d mfma_chain(v4d acc, int kbeg, int kend, int lk, F
 __global__ __launch_bounds__(1024) void k_fwd_level(FactorView f, const int* __restrict__ lev, int ntile, int Ct, int C,
                                                    double* __restrict__ B, double* __restrict__ U) {
   extern __shared__ double sh[];
-  const int s = lev[blockIdx.x / ntile];
-  const int col0 = (blockIdx.x % ntile) * Ct;
+  const int lb = xcd_block();
+  const int s = lev[lb / ntile];
+  const int col0 = (lb % ntile) * Ct;
   const int ct = min(Ct, C - col0);
   const int c0 = f.first[s];
   const int w = f.first[s + 1] - c0;
@@ -401,8 +412,9 @@ __global__ __launch_bounds__(1024) void k_fwd_level(FactorView f, const int* __r
 __global__ __launch_bounds__(1024) void k_bwd_level(FactorView f, const int* __restrict__ lev, int ntile, int Ct, int C,
                                                    double* __restrict__ B) {
   extern __shared__ double sh[];
-  const int s = lev[blockIdx.x / ntile];
-  const int col0 = (blockIdx.x % ntile) * Ct;
+  const int lb = xcd_block();
+  const int s = lev[lb / ntile];
+  const int col0 = (lb % ntile) * Ct;
   const int ct = min(Ct, C - col0);
   const int c0 = f.first[s];
   const int w = f.first[s + 1] - c0;
@@ -597,7 +609,7 @@ __global__ __launch_bounds__(256) void k_vjp_tet(const double* __restrict__ Yk, 
                                                  const int* __restrict__ active, T* __restrict__ corner,
                                                  double* __restrict__ gEpart, int Tn, int R,
                                                  const double* __restrict__ Xn, const double* __restrict__ CD) {
-  long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  long long idx = (long long)xcd_block() * blockDim.x + threadIdx.x;
   if (idx >= (long long)Tn * R) return;
   int t = (int)(idx / R);
   int r = (int)(idx % R);
@@ -678,7 +690,7 @@ __global__ void k_vjp_muscle(const double* __restrict__ Yk, const double* __rest
                              const double* __restrict__ act, int act_stride_r, int act_off,
                              const int* __restrict__ active, T* __restrict__ corner, double* __restrict__ gMpart,
                              int M, int R) {
-  long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  long long idx = (long long)xcd_block() * blockDim.x + threadIdx.x;
   if (idx >= (long long)M * R) return;
   int m = (int)(idx / R);
   int r = (int)(idx % R);
@@ -933,7 +945,7 @@ __global__ void k_contact_jacobi(const int* __restrict__ Lptr, const int* __rest
                                  double mu, double alpha, double h, int first_iter, const int* __restrict__ iactive,
                                  double* __restrict__ Prev, double* __restrict__ Xit, int N, int R,
                                  double* __restrict__ tRC, double* __restrict__ tN) {
-  long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  long long idx = (long long)xcd_block() * blockDim.x + threadIdx.x;
   if (idx >= (long long)N * R) return;
   int v = (int)(idx / R), r = (int)(idx % R);
   if (!iactive[r]) return;
@@ -1125,7 +1137,7 @@ __global__ void k_cadj_pre(const double* __restrict__ XB, double* __restrict__ P
 __global__ void k_cadj_cz(const int* __restrict__ Lptr, const int* __restrict__ Lcol, const double* __restrict__ Lval,
                           const double* __restrict__ Zb, double* __restrict__ Xb, double* __restrict__ OB,
                           const int* __restrict__ active, int N, int C) {
-  long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  long long idx = (long long)xcd_block() * blockDim.x + threadIdx.x;
   if (idx >= (long long)N * C) return;
   int v = (int)(idx / C), j = (int)(idx % C);
   if (!active[j / 3]) return;
@@ -1141,7 +1153,7 @@ __global__ void k_cadj_cz(const int* __restrict__ Lptr, const int* __restrict__ 
 template <typename T>
 __global__ void k_gather_masked_add(const T* __restrict__ corner, const int* __restrict__ vptr,
                                     const int* __restrict__ active, double* __restrict__ B, int N, int C) {
-  long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  long long idx = (long long)xcd_block() * blockDim.x + threadIdx.x;
   if (idx >= (long long)N * C) return;
   int v = (int)(idx / C);
   int j = (int)(idx % C);
