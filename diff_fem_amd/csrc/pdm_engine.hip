@@ -109,16 +109,22 @@ __global__ void k_step_prep(const double* __restrict__ X, const double* __restri
   Cc[idx] = mass[v] * vs / h - Lx[idx] + attw[v] * xrest[3 * v + c];
 }
 
-template <typename T>
+// UNI: R is a multiple of 64, so the 64 lanes of a wave share one tet; its
+// index is then made wave-uniform and the per-tet data come in by scalar loads.
+template <typename T, bool UNI>
 __global__ __launch_bounds__(256) void k_local_tet(const double* __restrict__ X, const int* __restrict__ tv,
                                                    const double* __restrict__ tG, const double* __restrict__ wr,
                                                    const double* __restrict__ wv, const int* __restrict__ tslot,
                                                    const int* __restrict__ active, T* __restrict__ corner, int Tn,
-                                                   int R) {
+                                                   int R, const int* __restrict__ mptr, const double* __restrict__ mg,
+                                                   const double* __restrict__ mw, const int* __restrict__ mgroup,
+                                                   int Mn, const double* __restrict__ act, int act_stride_r,
+                                                   int act_off) {
   long long idx = (long long)xcd_block() * blockDim.x + threadIdx.x;
   if (idx >= (long long)Tn * R) return;
   int t = (int)(idx / R);
   int r = (int)(idx % R);
+  if (UNI) t = __builtin_amdgcn_readfirstlane(t);
   if (!active[r]) return;
   int C = 3 * R;
   int v0 = tv[t], v1 = tv[Tn + t], v2 = tv[2 * Tn + t], v3 = tv[3 * Tn + t];
@@ -143,48 +149,106 @@ __global__ __launch_bounds__(256) void k_local_tet(const double* __restrict__ X,
       F[rr * 3 + j] = ds[rr * 3 + 0] * G[3 + j] + ds[rr * 3 + 1] * G[6 + j] + ds[rr * 3 + 2] * G[9 + j];
   T P[9];
   corotate_P<T>(F, (T)wr[t], (T)wv[t], P);
+  T f[12];
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+#pragma unroll
+    for (int rr = 0; rr < 3; ++rr)
+      f[3 * k + rr] = P[rr * 3 + 0] * G[3 * k + 0] + P[rr * 3 + 1] * G[3 * k + 1] + P[rr * 3 + 2] * G[3 * k + 2];
+  // Linear muscles on this tet: d = (1 - a) F f (LinearMuscleConstraint.h:265-271),
+  // corner force w (1 - a) g_k (sum_j g_j x_j), g in this tet's corner order.
+  for (int m = mptr[t], me = mptr[t + 1]; m < me; ++m) {
+    double a = act ? act[(long long)r * act_stride_r + act_off + mgroup[m]] : 0.0;
+    T sc = (T)(mw[m] * (1.0 - a));
+    T g[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) g[k] = (T)mg[(long long)k * Mn + m];
+    T ff[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) ff[c] = g[1] * ds[c * 3 + 0] + g[2] * ds[c * 3 + 1] + g[3] * ds[c * 3 + 2];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) f[3 * k + c] += sc * ff[c] * g[k];
+  }
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     T* o = corner + (long long)tslot[k * Tn + t] * C + 3 * r;
 #pragma unroll
-    for (int rr = 0; rr < 3; ++rr) o[rr] = P[rr * 3 + 0] * G[3 * k + 0] + P[rr * 3 + 1] * G[3 * k + 1] + P[rr * 3 + 2] * G[3 * k + 2];
+    for (int rr = 0; rr < 3; ++rr) o[rr] = f[3 * k + rr];
   }
 }
 
-// Linear muscle: d = (1 - a) F f (LinearMuscleConstraint.h:265-271).
-template <typename T>
-__global__ void k_local_muscle(const double* __restrict__ X, const int* __restrict__ mv, const double* __restrict__ mg,
-                               const double* __restrict__ mw, const int* __restrict__ mgroup,
-                               const int* __restrict__ mslot, const double* __restrict__ act, int act_stride_r,
-                               int act_off, const int* __restrict__ active, T* __restrict__ corner, int M, int R) {
+// Four consecutive slot entries widened to FP64 (one 16-byte load for FP32
+// slots, two for FP64).
+__device__ __forceinline__ void load4(const float* p, double* o) {
+  float4 v = *reinterpret_cast<const float4*>(p);
+  o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
+}
+__device__ __forceinline__ void load4(const double* p, double* o) {
+  double2 a = reinterpret_cast<const double2*>(p)[0], b = reinterpret_cast<const double2*>(p)[1];
+  o[0] = a.x; o[1] = a.y; o[2] = b.x; o[3] = b.y;
+}
+
+// Vectorised segmented gather for C % 4 == 0: each thread sums four adjacent
+// columns of one vertex over its slot range, four slot rows per trip
+// (partial sums s[k][.] for rows q = k mod 4, added in a fixed order).
+// MODE 0: B = Cc + sum. MODE 1: B = sum for active rollouts. MODE 2: B += sum
+// for active rollouts.
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void k_gather4(const double* __restrict__ Cc, const T* __restrict__ corner,
+                                                 const int* __restrict__ vptr, const int* __restrict__ active,
+                                                 double* __restrict__ B, int N, int C) {
+  const int C4 = C >> 2;
   long long idx = (long long)xcd_block() * blockDim.x + threadIdx.x;
-  if (idx >= (long long)M * R) return;
-  int m = (int)(idx / R);
-  int r = (int)(idx % R);
-  if (!active[r]) return;
-  int C = 3 * R;
-  double a = act ? act[(long long)r * act_stride_r + act_off + mgroup[m]] : 0.0;
-  int v[4];
-  T g[4];
+  if (idx >= (long long)N * C4) return;
+  int v = (int)(idx / C4);
+  int j = (int)(idx - (long long)v * C4) * 4;
+  bool m[4] = {true, true, true, true};
+  if (MODE != 0) {
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    v[k] = mv[k * M + m];
-    g[k] = (T)mg[k * M + m];
+    for (int k = 0; k < 4; ++k) m[k] = active[(j + k) / 3] != 0;
+    if (!(m[0] || m[1] || m[2] || m[3])) return;
   }
-  const double* xb = X + (long long)v[0] * C + 3 * r;
-  T ff[3];
+  double s[4][4];
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    double base = xb[c];
-    ff[c] = g[1] * (T)(X[(long long)v[1] * C + 3 * r + c] - base) + g[2] * (T)(X[(long long)v[2] * C + 3 * r + c] - base) +
-            g[3] * (T)(X[(long long)v[3] * C + 3 * r + c] - base);
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) s[a][k] = 0.0;
+  int q = vptr[v], qe = vptr[v + 1];
+  const T* p = corner + (long long)q * C + j;
+  for (; q + 4 <= qe; q += 4, p += 4LL * C) {
+    double x[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) load4(p + (long long)a * C, x[a]);
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) s[a][k] += x[a][k];
   }
-  T sc = (T)(mw[m] * (1.0 - a));
+  for (; q < qe; ++q, p += C) {
+    double x[4];
+    load4(p, x);
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    T* o = corner + (long long)mslot[k * M + m] * C + 3 * r;
+    for (int k = 0; k < 4; ++k) s[0][k] += x[k];
+  }
+  double r[4];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) o[c] = sc * ff[c] * g[k];
+  for (int k = 0; k < 4; ++k) r[k] = (s[0][k] + s[1][k]) + (s[2][k] + s[3][k]);
+  double* o = B + (long long)v * C + j;
+  if (MODE == 0) {
+    double2 c0 = make_double2(0.0, 0.0), c1 = c0;
+    if (Cc) {
+      const double2* c2 = reinterpret_cast<const double2*>(Cc + (long long)v * C + j);
+      c0 = c2[0];
+      c1 = c2[1];
+    }
+    reinterpret_cast<double2*>(o)[0] = make_double2(c0.x + r[0], c0.y + r[1]);
+    reinterpret_cast<double2*>(o)[1] = make_double2(c1.x + r[2], c1.y + r[3]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (m[k]) o[k] = MODE == 2 ? o[k] + r[k] : r[k];
   }
 }
 
@@ -600,7 +664,7 @@ __global__ void k_adj_post_solve(const double* __restrict__ Z, const double* __r
 // Local-step VJP of one tet at iterate y_k with z = K^-1 ybar_{k+1}:
 // corners get Fbar G_k; the per-(tet,rollout) E-gradient accumulates
 // (A z).(d_c(y_k) - A y_{k+1}) dw_c/dE over the two weights.
-template <typename T>
+template <typename T, bool UNI>
 __global__ __launch_bounds__(256) void k_vjp_tet(const double* __restrict__ Yk, const double* __restrict__ Yk1,
                                                  const double* __restrict__ Z, const int* __restrict__ tv,
                                                  const double* __restrict__ tG, const double* __restrict__ wr,
@@ -608,11 +672,16 @@ __global__ __launch_bounds__(256) void k_vjp_tet(const double* __restrict__ Yk, 
                                                  const double* __restrict__ dEv, const int* __restrict__ tslot,
                                                  const int* __restrict__ active, T* __restrict__ corner,
                                                  double* __restrict__ gEpart, int Tn, int R,
-                                                 const double* __restrict__ Xn, const double* __restrict__ CD) {
+                                                 const double* __restrict__ Xn, const double* __restrict__ CD,
+                                                 const int* __restrict__ mptr, const double* __restrict__ mg,
+                                                 const double* __restrict__ mw, const int* __restrict__ mgroup, int Mn,
+                                                 const double* __restrict__ act, int act_stride_r, int act_off,
+                                                 double* __restrict__ gMpart) {
   long long idx = (long long)xcd_block() * blockDim.x + threadIdx.x;
   if (idx >= (long long)Tn * R) return;
   int t = (int)(idx / R);
   int r = (int)(idx % R);
+  if (UNI) t = __builtin_amdgcn_readfirstlane(t);
   if (!active[r]) return;
   int C = 3 * R;
   int v[4] = {tv[t], tv[Tn + t], tv[2 * Tn + t], tv[3 * Tn + t]};
@@ -674,52 +743,40 @@ __global__ __launch_bounds__(256) void k_vjp_tet(const double* __restrict__ Yk, 
     gv += extra;
   }
   gEpart[idx] += gr * dEr[t] + gv * dEv[t];
+  T f[12];
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      f[3 * k + c] = Fb[c * 3 + 0] * G[3 * k + 0] + Fb[c * 3 + 1] * G[3 * k + 1] + Fb[c * 3 + 2] * G[3 * k + 2];
+  // Muscles on this tet: ybar += w (1-a) A^T (A z); dL/da += -w (A z).(F(y_k) f).
+  for (int m = mptr[t], me = mptr[t + 1]; m < me; ++m) {
+    double a = act ? act[(long long)r * act_stride_r + act_off + mgroup[m]] : 0.0;
+    double g[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) g[k] = mg[(long long)k * Mn + m];
+    double fz[3] = {0, 0, 0}, ff[3] = {0, 0, 0};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      double b = Yk[(long long)v[0] * C + 3 * r + c];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) fz[c] += Z[(long long)v[k] * C + 3 * r + c] * g[k];
+#pragma unroll
+      for (int k = 1; k < 4; ++k) ff[c] += (Yk[(long long)v[k] * C + 3 * r + c] - b) * g[k];
+    }
+    double wm = mw[m];
+    gMpart[(long long)m * R + r] += -wm * (fz[0] * ff[0] + fz[1] * ff[1] + fz[2] * ff[2]);
+    double sc = wm * (1.0 - a);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) f[3 * k + c] += (T)(sc * fz[c] * g[k]);
+  }
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     T* o = corner + (long long)tslot[k * Tn + t] * C + 3 * r;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) o[c] = Fb[c * 3 + 0] * G[3 * k + 0] + Fb[c * 3 + 1] * G[3 * k + 1] + Fb[c * 3 + 2] * G[3 * k + 2];
-  }
-}
-
-// Muscle VJP: ybar += w (1-a) A^T (A z); dL/da += -w (A z).(F(y_k) f).
-template <typename T>
-__global__ void k_vjp_muscle(const double* __restrict__ Yk, const double* __restrict__ Z, const int* __restrict__ mv,
-                             const double* __restrict__ mg, const double* __restrict__ mw,
-                             const int* __restrict__ mgroup, const int* __restrict__ mslot,
-                             const double* __restrict__ act, int act_stride_r, int act_off,
-                             const int* __restrict__ active, T* __restrict__ corner, double* __restrict__ gMpart,
-                             int M, int R) {
-  long long idx = (long long)xcd_block() * blockDim.x + threadIdx.x;
-  if (idx >= (long long)M * R) return;
-  int m = (int)(idx / R);
-  int r = (int)(idx % R);
-  if (!active[r]) return;
-  int C = 3 * R;
-  double a = act ? act[(long long)r * act_stride_r + act_off + mgroup[m]] : 0.0;
-  double fz[3] = {0, 0, 0}, ff[3] = {0, 0, 0};
-  int v[4];
-  double g[4];
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    v[k] = mv[k * M + m];
-    g[k] = mg[k * M + m];
-  }
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    double b = Yk[(long long)v[0] * C + 3 * r + c];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) fz[c] += Z[(long long)v[k] * C + 3 * r + c] * g[k];
-#pragma unroll
-    for (int k = 1; k < 4; ++k) ff[c] += (Yk[(long long)v[k] * C + 3 * r + c] - b) * g[k];
-  }
-  gMpart[idx] += -mw[m] * (fz[0] * ff[0] + fz[1] * ff[1] + fz[2] * ff[2]);
-  double sc = mw[m] * (1.0 - a);
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    T* o = corner + (long long)mslot[k * M + m] * C + 3 * r;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) o[c] = (T)(sc * fz[c] * g[k]);
+    for (int c = 0; c < 3; ++c) o[c] = f[3 * k + c];
   }
 }
 
@@ -1262,7 +1319,7 @@ struct pdm_scene_s {
   bool own_stream = false;
   int R = 1, C = 3, precision = PDM_FP64, flags = PDM_HOST_EARLY_EXIT, ckpt = 20;
   // static device data
-  DevBuf<int> perm, tv, tslot, vptr, mv, mslot, mgroup, gptr, gmus, tet_order;
+  DevBuf<int> perm, tv, tslot, vptr, mv, mptr, mgroup, gptr, gmus, tet_order;
   DevBuf<double> mass, attw, xrest, tG, wr, wv, dEr, dEv, mg, mw;
   DevBuf<int> Lptr, Lcol;
   DevBuf<double> Lval;
@@ -1332,12 +1389,16 @@ int upload_static(pdm_scene s) {
     sp.push_back(0);
     if (s->d_planes.upload(pl.data(), pl.size()) || s->d_spheres.upload(sp.data(), sp.size())) return 1;
   }
+  {
+    std::vector<int> mp(S.mptr);
+    if (mp.empty()) mp.assign(S.T + 1, 0);
+    if (s->mptr.upload(mp.data(), mp.size())) return 1;
+  }
   if (S.M > 0) {
     if (s->mv.upload(soa(S.mv, S.M, 4).data(), 4 * (size_t)S.M)) return 1;
     if (s->mg.upload(soa(S.mg, S.M, 4).data(), 4 * (size_t)S.M)) return 1;
     if (s->mw.upload(S.mw.data(), S.M)) return 1;
     if (s->mgroup.upload(S.mgroup.data(), S.M)) return 1;
-    if (s->mslot.upload(soa(s->Cn.mus_slot, S.M, 4).data(), 4 * (size_t)S.M)) return 1;
     std::vector<int> gp(S.n_groups + 1, 0), gm(S.M);
     for (int m = 0; m < S.M; ++m) gp[S.mgroup[m] + 1]++;
     for (int g = 0; g < S.n_groups; ++g) gp[g + 1] += gp[g];
@@ -1580,16 +1641,20 @@ void launch_local(pdm_scene s, const double* Xsrc, const double* act, int act_st
   HostScene& S = s->S;
   T* corner = (T*)s->corner.p;
   long long nt = (long long)S.T * s->R;
-  hipLaunchKernelGGL(k_local_tet<T>, dim3(cdiv(nt, kThreads)), dim3(kThreads), 0, s->stream, Xsrc, s->tv.p, s->tG.p,
-                     s->wr.p, s->wv.p, s->tslot.p, mask, corner, S.T, s->R);
-  if (S.M > 0) {
-    long long nm = (long long)S.M * s->R;
-    hipLaunchKernelGGL(k_local_muscle<T>, dim3(cdiv(nm, kThreads)), dim3(kThreads), 0, s->stream, Xsrc, s->mv.p,
-                       s->mg.p, s->mw.p, s->mgroup.p, s->mslot.p, act, act_steps * S.n_groups, step * S.n_groups,
-                       mask, corner, S.M, s->R);
-  }
-  hipLaunchKernelGGL(k_gather<T>, dim3(cdiv(s->NC(), kThreads)), dim3(kThreads), 0, s->stream, s->Cc.p, corner,
-                     s->vptr.p, s->B.p, S.N, s->C);
+  if (s->R % 64 == 0)
+    hipLaunchKernelGGL((k_local_tet<T, true>), dim3(cdiv(nt, kThreads)), dim3(kThreads), 0, s->stream, Xsrc, s->tv.p,
+                       s->tG.p, s->wr.p, s->wv.p, s->tslot.p, mask, corner, S.T, s->R, s->mptr.p, s->mg.p, s->mw.p,
+                       s->mgroup.p, S.M, act, act_steps * S.n_groups, step * S.n_groups);
+  else
+    hipLaunchKernelGGL((k_local_tet<T, false>), dim3(cdiv(nt, kThreads)), dim3(kThreads), 0, s->stream, Xsrc, s->tv.p,
+                       s->tG.p, s->wr.p, s->wv.p, s->tslot.p, mask, corner, S.T, s->R, s->mptr.p, s->mg.p, s->mw.p,
+                       s->mgroup.p, S.M, act, act_steps * S.n_groups, step * S.n_groups);
+  if (s->C % 4 == 0)
+    hipLaunchKernelGGL((k_gather4<T, 0>), dim3(cdiv(s->NC() / 4, kThreads)), dim3(kThreads), 0, s->stream, s->Cc.p,
+                       corner, s->vptr.p, nullptr, s->B.p, S.N, s->C);
+  else
+    hipLaunchKernelGGL(k_gather<T>, dim3(cdiv(s->NC(), kThreads)), dim3(kThreads), 0, s->stream, s->Cc.p, corner,
+                       s->vptr.p, s->B.p, S.N, s->C);
 }
 
 
@@ -1809,16 +1874,22 @@ int vjp_iteration(pdm_scene s, const double* Yk, const double* Yk1, const double
   HostScene& S = s->S;
   T* corner = (T*)s->corner.p;
   long long nt = (long long)S.T * s->R;
-  hipLaunchKernelGGL(k_vjp_tet<T>, dim3(cdiv(nt, kThreads)), dim3(kThreads), 0, s->stream, Yk, Yk1, s->Zb.p, s->tv.p,
-                     s->tG.p, s->wr.p, s->wv.p, s->dEr.p, s->dEv.p, s->tslot.p, s->ractive.p, corner, s->gEpart.p, S.T,
-                     s->R, CDp ? Xn : nullptr, CDp);
-  if (S.M > 0) {
-    long long nm = (long long)S.M * s->R;
-    hipLaunchKernelGGL(k_vjp_muscle<T>, dim3(cdiv(nm, kThreads)), dim3(kThreads), 0, s->stream, Yk, s->Zb.p, s->mv.p,
-                       s->mg.p, s->mw.p, s->mgroup.p, s->mslot.p, act, act_steps * S.n_groups, step * S.n_groups,
-                       s->ractive.p, corner, s->gMpart.p, S.M, s->R);
+  for (int uni = 0; uni < 2; ++uni) {
+    if ((s->R % 64 == 0) != (uni == 1)) continue;
+    auto kern = uni ? k_vjp_tet<T, true> : k_vjp_tet<T, false>;
+    hipLaunchKernelGGL(kern, dim3(cdiv(nt, kThreads)), dim3(kThreads), 0, s->stream, Yk, Yk1, s->Zb.p, s->tv.p,
+                       s->tG.p, s->wr.p, s->wv.p, s->dEr.p, s->dEv.p, s->tslot.p, s->ractive.p, corner, s->gEpart.p, S.T,
+                       s->R, CDp ? Xn : nullptr, CDp, s->mptr.p, s->mg.p, s->mw.p, s->mgroup.p, S.M, act,
+                       act_steps * S.n_groups, step * S.n_groups, S.M > 0 ? s->gMpart.p : nullptr);
   }
-  if (out_add)
+  if (s->C % 4 == 0) {
+    if (out_add)
+      hipLaunchKernelGGL((k_gather4<T, 2>), dim3(cdiv(s->NC() / 4, kThreads)), dim3(kThreads), 0, s->stream, nullptr,
+                         corner, s->vptr.p, s->ractive.p, out_add, S.N, s->C);
+    else
+      hipLaunchKernelGGL((k_gather4<T, 1>), dim3(cdiv(s->NC() / 4, kThreads)), dim3(kThreads), 0, s->stream, nullptr,
+                         corner, s->vptr.p, s->ractive.p, s->Yb.p, S.N, s->C);
+  } else if (out_add)
     hipLaunchKernelGGL(k_gather_masked_add<T>, dim3(cdiv(s->NC(), kThreads)), dim3(kThreads), 0, s->stream, corner,
                        s->vptr.p, s->ractive.p, out_add, S.N, s->C);
   else

@@ -30,6 +30,7 @@ struct HostScene {
   std::vector<double> mg;          // 4M
   std::vector<double> mw;          // M (k * weight * vol)
   std::vector<int> mgroup;         // M
+  std::vector<int> mptr;           // T+1: muscles of engine tet e are [mptr[e], mptr[e+1])
   int n_groups = 0;
   // attachments (engine ids)
   std::vector<int> att_idx;
@@ -83,7 +84,6 @@ struct Corners {
   int nslot = 0;
   std::vector<int> vptr;           // N+1
   std::vector<int> tet_slot;       // 4T: slot of (tet, corner)
-  std::vector<int> mus_slot;       // 4M
 };
 
 // Build the engine scene from caller arrays (see pdm.h for the meaning).

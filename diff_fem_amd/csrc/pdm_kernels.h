@@ -8,22 +8,60 @@ namespace pdm {
 
 template <typename T>
 struct Num;
+// FP64: IEEE division and square root (the plain path is held to 1e-9).
 template <>
 struct Num<double> {
   static __device__ __forceinline__ double rsqrt(double x) { return 1.0 / sqrt(x); }
   static __device__ __forceinline__ double sqrt_(double x) { return sqrt(x); }
+  static __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
+  static __device__ __forceinline__ double div(double a, double b) { return a / b; }
   static constexpr int kSweeps = 6;
   static constexpr double kTiny = 1e-300;
   static constexpr double kRel = 1e-15;
 };
+// FP32 (mixed path): the hardware v_rcp/v_sqrt/v_rsq approximations (about
+// 1 ulp) instead of the multi-instruction IEEE sequences; the FP32 local step
+// is held to 1e-5, far above the extra rounding.
 template <>
 struct Num<float> {
-  static __device__ __forceinline__ float rsqrt(float x) { return rsqrtf(x); }
-  static __device__ __forceinline__ float sqrt_(float x) { return sqrtf(x); }
+  static __device__ __forceinline__ float rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
+  static __device__ __forceinline__ float sqrt_(float x) { return __builtin_amdgcn_sqrtf(x); }
+  static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+  static __device__ __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
   static constexpr int kSweeps = 4;
   static constexpr float kTiny = 1e-30f;
   static constexpr float kRel = 1e-6f;
 };
+
+// Jacobi rotation (c, s) that zeroes a_pq of the symmetric 2x2 block
+// [[a_pp, a_pq], [a_pq, a_qq]]: t = tan(theta) = sign(tau) / (|tau| + sqrt(1 + tau^2)),
+// tau = (a_qq - a_pp) / (2 a_pq). The FP32 form multiplies through by |2 a_pq|
+// (t = sign(tau) |e| / (|d| + sqrt(d^2 + e^2)), d = a_qq - a_pp, e = 2 a_pq),
+// which needs one reciprocal and no branch; a_pq = 0 gives t = 0 (the tiny
+// addend keeps d = e = 0 finite). Returns false when FP64 skips the rotation.
+template <typename T>
+__device__ __forceinline__ bool jacobi_cs(T app, T aqq, T apq, T* t_out, T* c_out, T* s_out) {
+  if constexpr (sizeof(T) == 8) {
+    if (!(fabs(apq) > Num<T>::kTiny)) return false;
+    T tau = (aqq - app) / (T(2) * apq);
+    T t = (tau >= 0 ? T(1) : T(-1)) / (fabs(tau) + sqrt(T(1) + tau * tau));
+    T c = T(1) / sqrt(T(1) + t * t);
+    *t_out = t;
+    *c_out = c;
+    *s_out = t * c;
+    return true;
+  } else {
+    T d = aqq - app, e = T(2) * apq;
+    T ae = fabsf(e);
+    T num = d * e >= T(0) ? ae : -ae;
+    T t = num * Num<T>::rcp(fabsf(d) + Num<T>::sqrt_(d * d + e * e) + T(1e-30f));
+    T c = Num<T>::rsqrt(T(1) + t * t);
+    *t_out = t;
+    *c_out = c;
+    *s_out = t * c;
+    return true;
+  }
+}
 
 // F = U diag(S) V^T, S >= 0 descending, U, V orthogonal (row-major 3x3).
 template <typename T>
@@ -40,10 +78,8 @@ __device__ __forceinline__ void svd3(const T* F, T* U, T* S, T* V) {
     // (p,q) = (0,1)
     {
       T apq = a01;
-      if (fabs(apq) > Num<T>::kTiny) {
-        T tau = (a11 - a00) / (T(2) * apq);
-        T t = (tau >= 0 ? T(1) : T(-1)) / (fabs(tau) + Num<T>::sqrt_(T(1) + tau * tau));
-        T c = Num<T>::rsqrt(T(1) + t * t), s = t * c;
+      T t, c, s;
+      if (jacobi_cs<T>(a00, a11, apq, &t, &c, &s)) {
         a00 -= t * apq;
         a11 += t * apq;
         a01 = 0;
@@ -61,10 +97,8 @@ __device__ __forceinline__ void svd3(const T* F, T* U, T* S, T* V) {
     // (0,2)
     {
       T apq = a02;
-      if (fabs(apq) > Num<T>::kTiny) {
-        T tau = (a22 - a00) / (T(2) * apq);
-        T t = (tau >= 0 ? T(1) : T(-1)) / (fabs(tau) + Num<T>::sqrt_(T(1) + tau * tau));
-        T c = Num<T>::rsqrt(T(1) + t * t), s = t * c;
+      T t, c, s;
+      if (jacobi_cs<T>(a00, a22, apq, &t, &c, &s)) {
         a00 -= t * apq;
         a22 += t * apq;
         a02 = 0;
@@ -82,10 +116,8 @@ __device__ __forceinline__ void svd3(const T* F, T* U, T* S, T* V) {
     // (1,2)
     {
       T apq = a12;
-      if (fabs(apq) > Num<T>::kTiny) {
-        T tau = (a22 - a11) / (T(2) * apq);
-        T t = (tau >= 0 ? T(1) : T(-1)) / (fabs(tau) + Num<T>::sqrt_(T(1) + tau * tau));
-        T c = Num<T>::rsqrt(T(1) + t * t), s = t * c;
+      T t, c, s;
+      if (jacobi_cs<T>(a11, a22, apq, &t, &c, &s)) {
         a11 -= t * apq;
         a22 += t * apq;
         a12 = 0;
@@ -137,7 +169,7 @@ __device__ __forceinline__ void svd3(const T* F, T* U, T* S, T* V) {
   T s0 = Num<T>::sqrt_(n0);
   T u0[3], u1[3], u2[3];
   if (s0 > Num<T>::kTiny) {
-    T i0 = T(1) / s0;
+    T i0 = Num<T>::rcp(s0);
     u0[0] = b[0] * i0; u0[1] = b[3] * i0; u0[2] = b[6] * i0;
   } else {
     u0[0] = 1; u0[1] = 0; u0[2] = 0;
@@ -146,7 +178,7 @@ __device__ __forceinline__ void svd3(const T* F, T* U, T* S, T* V) {
   T w1[3] = {b[1] - d01 * u0[0], b[4] - d01 * u0[1], b[7] - d01 * u0[2]};
   T s1 = Num<T>::sqrt_(w1[0] * w1[0] + w1[1] * w1[1] + w1[2] * w1[2]);
   if (s1 > Num<T>::kRel * s0 + Num<T>::kTiny) {
-    T i1 = T(1) / s1;
+    T i1 = Num<T>::rcp(s1);
     u1[0] = w1[0] * i1; u1[1] = w1[1] * i1; u1[2] = w1[2] * i1;
   } else {
     // any unit vector orthogonal to u0
@@ -192,7 +224,7 @@ __device__ __forceinline__ void volproj(const T* S, T* D) {
     T a0 = S[0] + D[0], a1 = S[1] + D[1], a2 = S[2] + D[2];
     T cd = a0 * a1 * a2 - T(1);
     T g0 = a1 * a2, g1 = a0 * a2, g2 = a0 * a1;
-    T coef = (g0 * D[0] + g1 * D[1] + g2 * D[2] - cd) / (g0 * g0 + g1 * g1 + g2 * g2);
+    T coef = Num<T>::div(g0 * D[0] + g1 * D[1] + g2 * D[2] - cd, g0 * g0 + g1 * g1 + g2 * g2);
     D[0] = coef * g0;
     D[1] = coef * g1;
     D[2] = coef * g2;
@@ -224,14 +256,15 @@ __device__ __forceinline__ void volproj_jac(const T* S, T* D, T* J) {
     }
     T gg = g[0] * g[0] + g[1] * g[1] + g[2] * g[2];
     T num = g[0] * d[0] + g[1] * d[1] + g[2] * d[2] - cd;
-    T coef = num / gg;
+    T igg = Num<T>::rcp(gg);
+    T coef = num * igg;
     T nd[3], ndd[9];
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
       T dnum = dg[0 * 3 + j] * d[0] + g[0] * dd[0 * 3 + j] + dg[1 * 3 + j] * d[1] + g[1] * dd[1 * 3 + j] +
                dg[2 * 3 + j] * d[2] + g[2] * dd[2 * 3 + j] - dcd[j];
       T dgg = T(2) * (g[0] * dg[0 * 3 + j] + g[1] * dg[1 * 3 + j] + g[2] * dg[2 * 3 + j]);
-      T dcoef = (dnum * gg - num * dgg) / (gg * gg);
+      T dcoef = (dnum * gg - num * dgg) * (igg * igg);
 #pragma unroll
       for (int i = 0; i < 3; ++i) ndd[i * 3 + j] = dcoef * g[i] + coef * dg[i * 3 + j];
     }
@@ -319,10 +352,11 @@ __device__ __forceinline__ void corotate_vjp(const T* F, T wr, T wv, const T* Z,
     for (int j = 0; j < 3; ++j) {
       if (i == j) continue;
       T ssum = S[i] + S[j];
-      T br = ssum > 0 ? T(2) / ssum : T(0);
+      T isum = ssum > 0 ? Num<T>::rcp(ssum) : T(0);
+      T br = T(2) * isum;
       T sd = S[i] - S[j];
-      T av = (fabs(sd) > T(1e-6) * (T(1) + fabs(S[i]))) ? (phi[i] - phi[j]) / sd : (J[i * 3 + i] - J[i * 3 + j]);
-      T bv = ssum > 0 ? (phi[i] + phi[j]) / ssum : T(0);
+      T av = (fabs(sd) > T(1e-6) * (T(1) + fabs(S[i]))) ? Num<T>::div(phi[i] - phi[j], sd) : (J[i * 3 + i] - J[i * 3 + j]);
+      T bv = (phi[i] + phi[j]) * isum;
       T rot = T(0.5) * br * (Hr[i * 3 + j] - Hr[j * 3 + i]);
       T vol = T(0.5) * (av + bv) * H[i * 3 + j] + T(0.5) * (av - bv) * H[j * 3 + i];
       O[i * 3 + j] = wr * rot + wv * vol;

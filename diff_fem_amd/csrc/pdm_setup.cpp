@@ -629,17 +629,12 @@ void build_corners(const HostScene& S, Corners& C) {
   C.vptr.assign(N + 1, 0);
   for (int t = 0; t < S.T; ++t)
     for (int k = 0; k < 4; ++k) C.vptr[S.tv[4 * t + k] + 1]++;
-  for (int m = 0; m < S.M; ++m)
-    for (int k = 0; k < 4; ++k) C.vptr[S.mv[4 * m + k] + 1]++;
   for (int i = 0; i < N; ++i) C.vptr[i + 1] += C.vptr[i];
   C.nslot = C.vptr[N];
   std::vector<int> fill(C.vptr.begin(), C.vptr.end() - 1);
   C.tet_slot.assign(4 * (size_t)S.T, 0);
-  C.mus_slot.assign(4 * (size_t)S.M, 0);
   for (int t = 0; t < S.T; ++t)
     for (int k = 0; k < 4; ++k) C.tet_slot[4 * t + k] = fill[S.tv[4 * t + k]]++;
-  for (int m = 0; m < S.M; ++m)
-    for (int k = 0; k < 4; ++k) C.mus_slot[4 * m + k] = fill[S.mv[4 * m + k]]++;
 }
 
 // Per-segment tet selection of Muscle::Initialize (Muscle.h:109-215):
@@ -729,8 +724,10 @@ bool add_muscles(HostScene& S, const double* X, const int* tets, int n_seg, cons
   S.mw.clear();
   S.mgroup.clear();
   S.n_groups = n_groups;
-  std::vector<int> sel;
+  std::vector<int> sel, mtet;
   std::vector<double> w;
+  std::vector<int> etet(S.T);
+  for (int e = 0; e < S.T; ++e) etet[S.tet_order[e]] = e;
   for (int s = 0; s < n_seg; ++s) {
     if (seg_group[s] < 0 || seg_group[s] >= n_groups) {
       err = "muscle segment group out of range";
@@ -759,9 +756,40 @@ bool add_muscles(HostScene& S, const double* X, const int* tets, int n_seg, cons
       S.mg.push_back(a[2]);
       S.mw.push_back(vol * k_muscle * w[k]);
       S.mgroup.push_back(seg_group[s]);
+      mtet.push_back(etet[sel[k]]);
     }
   }
   S.M = (int)S.mw.size();
+  // Store the constraints grouped by engine tet (segment order kept within a
+  // tet), each in its tet's corner order, so that the element kernels add a
+  // tet's muscle forces into the tet's own corner slots.
+  std::vector<int> ord(S.M);
+  for (int m = 0; m < S.M; ++m) ord[m] = m;
+  std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return mtet[a] < mtet[b]; });
+  std::vector<int> mv(4 * (size_t)S.M), mgroup(S.M);
+  std::vector<double> mg(4 * (size_t)S.M), mw(S.M);
+  S.mptr.assign(S.T + 1, 0);
+  for (int i = 0; i < S.M; ++i) {
+    int m = ord[i], e = mtet[m];
+    S.mptr[e + 1]++;
+    for (int q = 0; q < 4; ++q) {
+      int k = 0;
+      while (k < 4 && S.tv[4 * (size_t)e + k] != S.mv[4 * (size_t)m + q]) ++k;
+      if (k == 4) {
+        err = "muscle constraint does not match its tet";
+        return false;
+      }
+      mv[4 * (size_t)i + k] = S.mv[4 * (size_t)m + q];
+      mg[4 * (size_t)i + k] = S.mg[4 * (size_t)m + q];
+    }
+    mw[i] = S.mw[m];
+    mgroup[i] = S.mgroup[m];
+  }
+  for (int e = 0; e < S.T; ++e) S.mptr[e + 1] += S.mptr[e];
+  S.mv.swap(mv);
+  S.mg.swap(mg);
+  S.mw.swap(mw);
+  S.mgroup.swap(mgroup);
   return true;
 }
 
