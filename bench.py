@@ -57,6 +57,26 @@ def cpu_baseline(d, steps, nthreads):
                 sample="oracle FP64 C++, 1 rollout x %d steps fwd+adjoint of the 40x12x12 muscle box (%.1f s)" % (steps, dt))
 
 
+def solve_traffic(rhs, nnz_lower):
+    """HBM bytes per solve from the committed counter summary (FETCH_SIZE and
+    WRITE_SIZE runs, scaled by a copy kernel of known size), when it was
+    measured on this same factor and right-hand-side count; else None."""
+    path = os.path.join(ROOT, "profiles", "solve_c4_R256_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    for line in open(path):
+        if line.startswith("- ") and ":" in line:
+            k, v = line[2:].split(":", 1)
+            vals[k.strip()] = v.strip().split()[0]
+    try:
+        if int(vals["rhs"]) == rhs and int(vals["nnz_lower"]) == nnz_lower:
+            return float(vals["hbm_bytes_per_solve"])
+    except (KeyError, ValueError):
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -71,6 +91,10 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--solve-only", action="store_true",
+                    help="only time back-to-back solves (short program for counter runs)")
+    ap.add_argument("--copy-calib", type=int, default=0,
+                    help="also run a device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the rollout batch is not sharded across GPUs in this build "
@@ -84,11 +108,24 @@ def main():
     R, S = args.rollouts, args.sim_steps
     log("setup: T=%d N=%d R=%d steps=%d" % (d.T, d.N, R, S))
     t0 = time.perf_counter()
-    sc = engine.Scene(d, R=R, precision=engine.PDM_MIXED, flags=engine.PDM_HOST_EARLY_EXIT,
-                      checkpoint_every=args.ckpt)
+    sc = engine.Scene(d, R=R, precision=engine.PDM_MIXED,
+                      flags=engine.PDM_HOST_EARLY_EXIT | engine.PDM_TIME_SOLVES, checkpoint_every=args.ckpt)
     info = sc.info()
     log("setup done in %.1fs: %s" % (time.perf_counter() - t0, info))
+    log("solve launch shapes per level ((Ct, threads) fwd, bwd): %s" % (sc.solve_shapes(),))
     dev = torch.device("cuda:0")
+    flops = 2.0 * 2.0 * info["nnz_lower"] * 3 * R
+    if args.solve_only:
+        ms = sc.time_solve(20)
+        if args.copy_calib:
+            a = torch.ones(args.copy_calib << 17, dtype=torch.float64, device=dev)
+            b = torch.empty_like(a)
+            for _ in range(3):
+                b.copy_(a)
+            torch.cuda.synchronize()
+        print(json.dumps({"solve_ms": ms, "flops_per_solve": flops, "rhs": 3 * R, "nnz_lower": info["nnz_lower"],
+                          "copy_calib_bytes": args.copy_calib << 20}), flush=True)
+        return
     acts = torch.from_numpy(scene.activations_sinusoid(4, S, d.n_groups, R, d.h)).to(dev).contiguous()
     x0 = torch.from_numpy(np.tile(d.x_rest.ravel(), (R, 1))).to(dev).contiguous()
     v0 = torch.zeros_like(x0)
@@ -106,6 +143,8 @@ def main():
         one_pass()
         torch.cuda.synchronize()
         log("warmup %d done" % w)
+    sc.solve_timing()  # restart the per-solve event counters
+    n_solve, solve_ms = 0, 0.0
     times, tet_its, fwd_ms, bwd_ms = [], [], [], []
     for k in range(args.steps):
         torch.cuda.synchronize()
@@ -117,6 +156,9 @@ def main():
         tet_its.append(float(d.T) * float(its.sum()))
         times.append(dt)
         tm = sc.timing()
+        ns, sm = sc.solve_timing()
+        n_solve += ns
+        solve_ms += sm
         fwd_ms.append(tm["forward_ms"])
         bwd_ms.append(tm["backward_ms"])
         cps, cjt = sc.contact_stats()
@@ -134,18 +176,22 @@ def main():
                    "checkpoint_every": args.ckpt, "contact": bool(d.contact), "nnz_factor": info["nnz_factor"],
                    "forward_ms": float(np.mean(fwd_ms)), "adjoint_ms": float(np.mean(bwd_ms))},
     }
-    # Roofline of the dominant unit: one triangular solve (forward + backward
-    # sweep of k_fwd_level/k_bwd_level over all 3R right-hand sides). At 3R = 768
-    # it is matrix-core bound (FP64 MFMA): algorithmic flops = 2 sweeps x 2 x
-    # nnz(L, lower trapezoid) x 3R.
-    ms = sc.time_solve(20)
-    flops = 2.0 * 2.0 * info["nnz_lower"] * 3 * R
-    achieved = flops / (ms * 1e-3) / 1e12
+    # Roofline of the dominant kernel: the supernodal triangular solve
+    # (k_fwd_level + k_bwd_level over all levels, all 3R right-hand sides), the
+    # largest share of kernel time. Its mean duration comes from HIP events
+    # around every solve of the timed passes, on the library stream. At 3R = 768
+    # it is FP64-MFMA bound by arithmetic intensity: algorithmic flops = 2 sweeps
+    # x 2 x nnz(L, lower trapezoid) x 3R; algorithmic bytes = factor read twice
+    # + right-hand sides read and written once per sweep.
+    ms_live = solve_ms / max(n_solve, 1)
+    achieved = flops / (ms_live * 1e-3) / 1e12
+    bytes_alg = 2.0 * info["nnz_lower"] * 8 + 2.0 * 2 * 3 * R * d.N * 8
     out["roofline"] = {"bound": "mfma", "achieved": achieved, "peak": 78.6, "unit": "TFLOP/s",
-                       "frac": achieved / 78.6, "traffic": None,
+                       "frac": achieved / 78.6, "traffic": solve_traffic(3 * R, info["nnz_lower"]),
                        "kernel": "supernodal solve k_fwd_level+k_bwd_level, %d RHS" % (3 * R),
-                       "ms_per_solve": ms, "flops_per_solve": flops,
-                       "bytes_alg_per_solve": 2.0 * info["nnz_lower"] * 8 + 2.0 * 2 * 3 * R * d.N * 8}
+                       "ms_per_solve": ms_live, "solves_timed": n_solve,
+                       "ms_per_solve_back_to_back": sc.time_solve(20), "flops_per_solve": flops,
+                       "bytes_alg_per_solve": bytes_alg}
     sc.close()
     del sc
     if not args.no_plain:

@@ -16,6 +16,8 @@ PDM_FP64 = 0
 PDM_MIXED = 1
 PDM_HOST_EARLY_EXIT = 1
 PDM_USE_GRAPH = 2
+PDM_TIME_SOLVES = 4
+PDM_NO_AUTOTUNE = 8
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -52,7 +54,8 @@ SYMBOLS = [
     "pdm_last_error", "pdm_device_count", "pdm_scene_create", "pdm_scene_destroy", "pdm_scene_info",
     "pdm_set_stream", "pdm_set_material", "pdm_set_batch", "pdm_forward", "pdm_forward_host", "pdm_backward",
     "pdm_backward_host", "pdm_get_iterations", "pdm_solve_host", "pdm_force_host", "pdm_get_timing",
-    "pdm_host_analyse", "pdm_host_solve", "pdm_get_contact_stats", "pdm_time_solve",
+    "pdm_host_analyse", "pdm_host_solve", "pdm_get_contact_stats", "pdm_time_solve", "pdm_get_solve_timing",
+    "pdm_get_solve_shapes",
 ]
 
 _lib = None

@@ -297,7 +297,9 @@ __global__ void k_gather_masked(const T* __restrict__ corner, const int* __restr
 
 struct FactorView {
   const long long* dptr;
-  const double* Dinv;
+  const double* Dinv;   // inv(L_ss), row-major w x w
+  const double* DinvC;  // the same, column-major (forward sweep reads it down columns)
+  const double* LR;     // L, row-major m x w (backward sweep reads it along rows)
   const int* first;
   const long long* rowptr;
   const int* rows;
@@ -343,25 +345,35 @@ __device__ __forceinline__ void load_front(const double* __restrict__ B, const i
 }
 
 // acc += A(16 x K) * Bm(K x 16) over k in [kbeg, kend) in steps of 4, with
-// operands fetched by the callables (lane-level element getters); the loop is
-// unrolled by 4 so that four operand loads are in flight per wave.
+// operands fetched by the callables (lane-level element getters). Operands of
+// eight MFMAs are fetched before they are consumed, so that eight loads per
+// wave are in flight; the tail (fewer than 32 k) is masked.
 template <typename FA, typename FB>
 __device__ __forceinline__ v4d mfma_chain(v4d acc, int kbeg, int kend, int lk, FA fa, FB fb) {
   int k = kbeg;
-  for (; k + 16 <= kend; k += 16) {
-    double a0 = fa(k + lk), a1 = fa(k + 4 + lk), a2 = fa(k + 8 + lk), a3 = fa(k + 12 + lk);
-    double b0 = fb(k + lk), b1 = fb(k + 4 + lk), b2 = fb(k + 8 + lk), b3 = fb(k + 12 + lk);
-    acc = mfma16(a0, b0, acc);
-    acc = mfma16(a1, b1, acc);
-    acc = mfma16(a2, b2, acc);
-    acc = mfma16(a3, b3, acc);
+  for (; k + 32 <= kend; k += 32) {
+    double a[8], b[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      a[u] = fa(k + 4 * u + lk);
+      b[u] = fb(k + 4 * u + lk);
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc = mfma16(a[u], b[u], acc);
   }
-  for (; k < kend; k += 4) {
-    int kk = k + lk;
-    bool ok = kk < kend;
-    double a = ok ? fa(kk) : 0.0;
-    double b = ok ? fb(kk) : 0.0;
-    acc = mfma16(a, b, acc);
+  if (k < kend) {
+    const int nrem = (kend - k + 3) >> 2;
+    double a[8], b[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      int kk = k + 4 * u + lk;
+      bool ok = u < nrem && kk < kend;
+      a[u] = ok ? fa(kk) : 0.0;
+      b[u] = ok ? fb(kk) : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (u < nrem) acc = mfma16(a[u], b[u], acc);
   }
   return acc;
 }
@@ -388,7 +400,7 @@ __global__ __launch_bounds__(1024) void k_fwd_level(FactorView f, const int* __r
   const int wpad = (w + 15) & ~15;
   double* Y = sh + (long long)mpad * Ct;  // wpad x Ct
   const double* Ls = f.L + f.lptr[s];
-  const double* Di = f.Dinv + f.dptr[s];
+  const double* DiC = f.DinvC + f.dptr[s];
   const int tid = threadIdx.x, nt = blockDim.x;
   const int lane = tid & 63, wave = tid >> 6, nwave = nt >> 6;
   const int lr = lane & 15, lk = lane >> 4;
@@ -429,7 +441,7 @@ __global__ __launch_bounds__(1024) void k_fwd_level(FactorView f, const int* __r
     const int kend = min(w, rt + 16);
     v4d acc = {0.0, 0.0, 0.0, 0.0};
     acc = mfma_chain(
-        acc, 0, kend, lk, [&](int k) { return row < w ? Di[(long long)row * w + k] : 0.0; },
+        acc, 0, kend, lk, [&](int k) { return row < w ? DiC[(long long)k * w + row] : 0.0; },
         [&](int k) { return sh[k * Ct + q * 16 + lr]; });
 #pragma unroll
     for (int i = 0; i < 4; ++i) Y[(rt + 4 * i + lk) * Ct + q * 16 + lr] = acc[i];
@@ -488,7 +500,7 @@ __global__ __launch_bounds__(1024) void k_bwd_level(FactorView f, const int* __r
   const int wpad = (w + 15) & ~15;
   double* Tt = sh + (long long)mpad * Ct;  // wpad x Ct
   const int* rows = f.rows + rp;
-  const double* Ls = f.L + f.lptr[s];
+  const double* LR = f.LR + f.lptr[s];
   const double* Di = f.Dinv + f.dptr[s];
   const int tid = threadIdx.x, nt = blockDim.x;
   const int lane = tid & 63, wave = tid >> 6, nwave = nt >> 6;
@@ -507,7 +519,7 @@ __global__ __launch_bounds__(1024) void k_bwd_level(FactorView f, const int* __r
       acc[i] = r < w ? sh[r * Ct + q * 16 + lr] : 0.0;
     }
     acc = mfma_chain(
-        acc, w, m, lk, [&](int k) { return col < w ? -Ls[(long long)col * m + k] : 0.0; },
+        acc, w, m, lk, [&](int k) { return col < w ? -LR[(long long)k * w + col] : 0.0; },
         [&](int k) { return sh[k * Ct + q * 16 + lr]; });
 #pragma unroll
     for (int i = 0; i < 4; ++i) Tt[(rt + 4 * i + lk) * Ct + q * 16 + lr] = acc[i];
@@ -1326,9 +1338,16 @@ struct pdm_scene_s {
   // factor
   DevBuf<int> f_first, f_rows, f_child_ptr, f_child, f_rel, f_lev;
   DevBuf<long long> f_rowptr, f_lptr, f_relptr, f_uptr;
-  DevBuf<double> f_L, f_Dinv;
+  DevBuf<double> f_L, f_Dinv, f_DinvC, f_LR;
   DevBuf<long long> f_dptr;
-  std::vector<int> lev_ptr, lev_maxm;
+  std::vector<int> lev_ptr, lev_rows;  // lev_rows: max (mpad + wpad) over the level's supernodes
+  // solve launch shape per level and sweep (autotuned per right-hand-side count)
+  struct LevelCfg {
+    int Ct = 16, nthr = 256;
+  };
+  std::vector<LevelCfg> cfg_fwd, cfg_bwd;
+  int tuned_C = -1;
+  bool autotune = true;
   // dynamic
   DevBuf<double> X, V, Xn, Vs, Cc, Xit, B, Lx, U, part, err;
   DevBuf<char> corner;
@@ -1355,9 +1374,14 @@ struct pdm_scene_s {
   DevBuf<int> ractive;
   double t_fwd = 0, t_bwd = 0, t_factor = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // PDM_TIME_SOLVES: event pairs around each solve, folded into solve_ms
+  std::vector<hipEvent_t> sev;
+  size_t sev_used = 0;
+  long long n_solves = 0;
+  double solve_ms = 0.0;
 
   FactorView view() const {
-    return FactorView{f_dptr.p, f_Dinv.p, f_first.p, f_rowptr.p, f_rows.p, f_lptr.p, f_L.p, f_child_ptr.p, f_child.p, f_relptr.p, f_rel.p,
+    return FactorView{f_dptr.p, f_Dinv.p, f_DinvC.p, f_LR.p, f_first.p, f_rowptr.p, f_rows.p, f_lptr.p, f_L.p, f_child_ptr.p, f_child.p, f_relptr.p, f_rel.p,
                       f_uptr.p};
   }
   size_t NC() const { return (size_t)S.N * C; }
@@ -1452,17 +1476,38 @@ int upload_factor(pdm_scene s) {
   if (s->f_uptr.upload(tmp.data(), tmp.size())) return 1;
   if (s->f_L.upload(F.L.data(), F.L.size())) return 1;
   if (s->f_Dinv.upload(F.Dinv.data(), F.Dinv.size())) return 1;
+  {
+    // transposed copies: coalesced operand loads in both sweeps
+    int ns = (int)F.first.size() - 1;
+    std::vector<double> dc(F.Dinv.size()), lr(F.L.size());
+    for (int sn = 0; sn < ns; ++sn) {
+      int w = F.first[sn + 1] - F.first[sn];
+      int m = (int)(F.rowptr[sn + 1] - F.rowptr[sn]);
+      const double* D = &F.Dinv[F.dptr[sn]];
+      double* Dc = &dc[F.dptr[sn]];
+      for (int r = 0; r < w; ++r)
+        for (int k = 0; k < w; ++k) Dc[(size_t)k * w + r] = D[(size_t)r * w + k];
+      const double* Lc = &F.L[F.lptr[sn]];
+      double* Lr = &lr[F.lptr[sn]];
+      for (int c = 0; c < w; ++c)
+        for (int r = 0; r < m; ++r) Lr[(size_t)r * w + c] = Lc[(size_t)c * m + r];
+    }
+    if (s->f_DinvC.upload(dc.data(), dc.size()) || s->f_LR.upload(lr.data(), lr.size())) return 1;
+  }
 
   tmp.assign(F.dptr.begin(), F.dptr.end());
   if (s->f_dptr.upload(tmp.data(), tmp.size())) return 1;
   int nlev = (int)F.level_ptr.size() - 1;
+  std::vector<int> old_ptr = s->lev_ptr, old_rows = s->lev_rows;
   s->lev_ptr = F.level_ptr;
-  s->lev_maxm.assign(nlev, 0);
+  s->lev_rows.assign(nlev, 16);
   for (int l = 0; l < nlev; ++l)
     for (int q = F.level_ptr[l]; q < F.level_ptr[l + 1]; ++q) {
       int sn = F.level_sn[q];
-      s->lev_maxm[l] = std::max(s->lev_maxm[l], (int)(F.rowptr[sn + 1] - F.rowptr[sn]));
+      int m = (int)(F.rowptr[sn + 1] - F.rowptr[sn]), w = F.first[sn + 1] - F.first[sn];
+      s->lev_rows[l] = std::max(s->lev_rows[l], ((m + 15) & ~15) + ((w + 15) & ~15));
     }
+  if (old_ptr != s->lev_ptr || old_rows != s->lev_rows) s->tuned_C = -1;  // level structure changed
   return 0;
 }
 
@@ -1560,6 +1605,8 @@ int host_factor(pdm_scene s) {
   return 0;
 }
 
+int autotune_levels(pdm_scene s);
+
 int ensure_batch_buffers(pdm_scene s) {
   size_t nc = s->NC();
   int N = s->S.N;
@@ -1578,60 +1625,163 @@ int ensure_batch_buffers(pdm_scene s) {
         s->npass.alloc(s->R) || s->citers.alloc(s->R) || s->clast.alloc(s->R) || s->cinj.alloc(s->R))
       return 1;
   }
+  if (s->tuned_C != s->C && autotune_levels(s)) return 1;
   return 0;
 }
 
-// Column tile per level: Ct = 16*nq right-hand sides, nq in {1,2,4}. LDS per
-// block = T (mpad x Ct) + scratch (wpad x Ct) doubles;
-// keep it within 64 KB (two blocks per CU) where possible, else up to 160 KB,
-// and shrink nq while the level has too few blocks to fill the GPU.
+// Launch shape of the level kernels: column tile Ct = 16*nq right-hand sides
+// (nq in {1,2,4}) and the block size. LDS per block = front (mpad x Ct) +
+// scratch (wpad x Ct) doubles, at most 160 KB. The default shape keeps LDS
+// within 64 KB where possible, shrinks nq while the level has too few blocks to
+// fill the GPU, and sizes the block so that about 16 waves share a CU; the
+// autotuner replaces it by the fastest measured shape (shapes only partition
+// the columns and the work of a block, so results do not depend on them).
 constexpr size_t kLdsSoft = 64 * 1024, kLdsMax = 160 * 1024;
-size_t level_lds(size_t mpad, int Ct) { return (2 * mpad * Ct) * 8; }
+size_t level_lds(const pdm_scene_s* s, int l, int Ct) { return (size_t)s->lev_rows[l] * Ct * 8; }
 
-int level_tile(pdm_scene s, int l, int* Ct_out, size_t* lds_out) {
+pdm_scene_s::LevelCfg level_default(const pdm_scene_s* s, int l) {
   int cnt = s->lev_ptr[l + 1] - s->lev_ptr[l];
-  size_t mpad = ((size_t)std::max(1, s->lev_maxm[l]) + 15) & ~(size_t)15;
   int ncol16 = cdiv(s->C, 16);
   int nq = 4;
-  while (nq > 1 && level_lds(mpad, 16 * nq) > kLdsSoft) nq /= 2;
+  while (nq > 1 && level_lds(s, l, 16 * nq) > kLdsSoft) nq /= 2;
   while (nq > 1 && (long long)cnt * cdiv(ncol16, nq) < 512) nq /= 2;
-  size_t lds = level_lds(mpad, 16 * nq);
+  pdm_scene_s::LevelCfg c;
+  c.Ct = 16 * nq;
+  size_t lds = level_lds(s, l, c.Ct);
+  int per_cu = (int)std::max<size_t>(1, std::min<size_t>(8, kLdsMax / std::max<size_t>(lds, 1)));
+  int waves = std::min(16, std::max(4, 16 / per_cu));
+  if ((long long)cnt * cdiv(s->C, c.Ct) < 256) waves = 16;
+  c.nthr = 64 * waves;
+  return c;
+}
+
+int launch_level(pdm_scene s, int pass, int l, const pdm_scene_s::LevelCfg& c, double* Bp) {
+  const FactorView fv = s->view();
+  int cnt = s->lev_ptr[l + 1] - s->lev_ptr[l];
+  size_t lds = level_lds(s, l, c.Ct);
   if (lds > kLdsMax) {
     g_err = "supernode front exceeds the LDS budget";
     return 1;
   }
-  *Ct_out = 16 * nq;
-  *lds_out = lds;
+  int ntile = cdiv(s->C, c.Ct);
+  if (pass == 0)
+    hipLaunchKernelGGL(k_fwd_level, dim3(cnt * ntile), dim3(c.nthr), lds, s->stream, fv, s->f_lev.p + s->lev_ptr[l],
+                       ntile, c.Ct, s->C, Bp, s->U.p);
+  else
+    hipLaunchKernelGGL(k_bwd_level, dim3(cnt * ntile), dim3(c.nthr), lds, s->stream, fv, s->f_lev.p + s->lev_ptr[l],
+                       ntile, c.Ct, s->C, Bp);
   return 0;
 }
 
-int launch_solve(pdm_scene s, double* Bp) {
+int set_level_attrs() {
   static bool attr_set = false;
   if (!attr_set) {
     HIPX(hipFuncSetAttribute((const void*)k_fwd_level, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsMax));
     HIPX(hipFuncSetAttribute((const void*)k_bwd_level, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsMax));
     attr_set = true;
   }
-  FactorView fv = s->view();
+  return 0;
+}
+
+// Autotuner: for every level and sweep, time each admissible launch shape
+// (nq in {1,2,4} x {256,512,1024} threads) with HIP events on the library
+// stream, on the engine's own rhs/update buffers (cleared first, before any
+// right-hand side is placed there), and keep the fastest. Runs once per
+// right-hand-side count.
+int autotune_levels(pdm_scene s) {
   int nlev = (int)s->lev_ptr.size() - 1;
-  for (int pass = 0; pass < 2; ++pass) {
-    for (int q = 0; q < nlev; ++q) {
-      int l = pass == 0 ? q : nlev - 1 - q;
-      int cnt = s->lev_ptr[l + 1] - s->lev_ptr[l];
-      int Ct;
-      size_t lds;
-      if (level_tile(s, l, &Ct, &lds)) return 1;
-      int ntile = cdiv(s->C, Ct);
-      // few blocks (top of the tree): 16 waves per block to hide load latency
-      int nthr = (long long)cnt * ntile < 256 ? 1024 : kThreads;
-      if (pass == 0)
-        hipLaunchKernelGGL(k_fwd_level, dim3(cnt * ntile), dim3(nthr), lds, s->stream, fv,
-                           s->f_lev.p + s->lev_ptr[l], ntile, Ct, s->C, Bp, s->U.p);
-      else
-        hipLaunchKernelGGL(k_bwd_level, dim3(cnt * ntile), dim3(nthr), lds, s->stream, fv,
-                           s->f_lev.p + s->lev_ptr[l], ntile, Ct, s->C, Bp);
+  s->cfg_fwd.resize(nlev);
+  s->cfg_bwd.resize(nlev);
+  for (int l = 0; l < nlev; ++l) s->cfg_fwd[l] = s->cfg_bwd[l] = level_default(s, l);
+  if (!s->autotune) {
+    s->tuned_C = s->C;
+    return 0;
+  }
+  if (set_level_attrs()) return 1;
+  HIPX(hipMemsetAsync(s->B.p, 0, s->NC() * 8, s->stream));
+  HIPX(hipMemsetAsync(s->U.p, 0, std::max<long long>(1, s->F.uptr.back()) * (size_t)s->C * 8, s->stream));
+  hipEvent_t e0, e1;
+  HIPX(hipEventCreate(&e0));
+  HIPX(hipEventCreate(&e1));
+  const int reps = 3;
+  for (int pass = 0; pass < 2; ++pass)
+    for (int l = 0; l < nlev; ++l) {
+      float best = 1e30f;
+      pdm_scene_s::LevelCfg bc = pass == 0 ? s->cfg_fwd[l] : s->cfg_bwd[l];
+      for (int nq = 1; nq <= 4; nq *= 2)
+        for (int nthr = 256; nthr <= 1024; nthr *= 2) {
+          pdm_scene_s::LevelCfg c;
+          c.Ct = 16 * nq;
+          c.nthr = nthr;
+          if (level_lds(s, l, c.Ct) > kLdsMax) continue;
+          if (launch_level(s, pass, l, c, s->B.p)) return 1;  // warm-up
+          HIPX(hipEventRecord(e0, s->stream));
+          for (int r = 0; r < reps; ++r)
+            if (launch_level(s, pass, l, c, s->B.p)) return 1;
+          HIPX(hipEventRecord(e1, s->stream));
+          HIPX(hipEventSynchronize(e1));
+          float ms = 0;
+          HIPX(hipEventElapsedTime(&ms, e0, e1));
+          if (ms < best) {
+            best = ms;
+            bc = c;
+          }
+        }
+      (pass == 0 ? s->cfg_fwd : s->cfg_bwd)[l] = bc;
+    }
+  HIPX(hipEventDestroy(e0));
+  HIPX(hipEventDestroy(e1));
+  HIPX(hipGetLastError());
+  s->tuned_C = s->C;
+  return 0;
+}
+
+// Sum the recorded solve event pairs into solve_ms (waits for the stream).
+int fold_solve_events(pdm_scene s) {
+  if (s->sev_used == 0) return 0;
+  HIPX(hipStreamSynchronize(s->stream));
+  for (size_t k = 0; k + 1 < s->sev_used; k += 2) {
+    float ms = 0;
+    HIPX(hipEventElapsedTime(&ms, s->sev[k], s->sev[k + 1]));
+    s->solve_ms += ms;
+  }
+  s->sev_used = 0;
+  return 0;
+}
+
+int launch_solve_levels(pdm_scene s, double* Bp);
+
+int launch_solve(pdm_scene s, double* Bp) {
+  if (!(s->flags & PDM_TIME_SOLVES)) return launch_solve_levels(s, Bp);
+  if (s->sev_used + 2 > s->sev.size()) {
+    if (s->sev.size() >= 4096) {
+      if (fold_solve_events(s)) return 1;
+    } else {
+      size_t n0 = s->sev.size();
+      s->sev.resize(n0 + 256);
+      for (size_t k = n0; k < s->sev.size(); ++k) HIPX(hipEventCreate(&s->sev[k]));
     }
   }
+  HIPX(hipEventRecord(s->sev[s->sev_used], s->stream));
+  if (launch_solve_levels(s, Bp)) return 1;
+  HIPX(hipEventRecord(s->sev[s->sev_used + 1], s->stream));
+  s->sev_used += 2;
+  s->n_solves += 1;
+  return 0;
+}
+
+int launch_solve_levels(pdm_scene s, double* Bp) {
+  if (set_level_attrs()) return 1;
+  if (s->tuned_C != s->C) {
+    g_err = "solve launch shapes not set up for this batch";
+    return 1;
+  }
+  int nlev = (int)s->lev_ptr.size() - 1;
+  for (int pass = 0; pass < 2; ++pass)
+    for (int q = 0; q < nlev; ++q) {
+      int l = pass == 0 ? q : nlev - 1 - q;
+      if (launch_level(s, pass, l, pass == 0 ? s->cfg_fwd[l] : s->cfg_bwd[l], Bp)) return 1;
+    }
   HIPX(hipGetLastError());
   return 0;
 }
@@ -2196,6 +2346,7 @@ int pdm_host_solve(const pdm_scene_desc* d, int R, const double* b, double* x) {
 void pdm_scene_destroy(pdm_scene s) {
   if (!s) return;
   if (s->stream) (void)hipStreamSynchronize(s->stream);
+  for (hipEvent_t e : s->sev) (void)hipEventDestroy(e);
   if (s->ev0) (void)hipEventDestroy(s->ev0);
   if (s->ev1) (void)hipEventDestroy(s->ev1);
   if (s->own_stream && s->stream) (void)hipStreamDestroy(s->stream);
@@ -2278,6 +2429,9 @@ int pdm_set_batch(pdm_scene s, int R, int precision, int flags, int checkpoint_e
   s->C = 3 * R;
   s->precision = precision;
   s->flags = flags;
+  bool at = (flags & PDM_NO_AUTOTUNE) == 0;
+  if (at != s->autotune) s->tuned_C = -1;
+  s->autotune = at;
   s->ckpt = checkpoint_every;
   s->have_tape = false;
   return 0;
@@ -2423,6 +2577,36 @@ int pdm_time_solve(pdm_scene s, int reps, double* ms_per_solve) {
   float ms = 0;
   HIPX(hipEventElapsedTime(&ms, s->ev0, s->ev1));
   *ms_per_solve = ms / reps;
+  return 0;
+}
+
+int pdm_get_solve_shapes(pdm_scene s, int cap, int* n_levels, int* ct_fwd, int* thr_fwd, int* ct_bwd, int* thr_bwd) {
+  if (!s || !n_levels || cap < 0) {
+    g_err = "invalid argument";
+    return 1;
+  }
+  if (ensure_batch_buffers(s)) return 1;
+  int nlev = (int)s->cfg_fwd.size();
+  *n_levels = nlev;
+  for (int l = 0; l < std::min(cap, nlev); ++l) {
+    if (ct_fwd) ct_fwd[l] = s->cfg_fwd[l].Ct;
+    if (thr_fwd) thr_fwd[l] = s->cfg_fwd[l].nthr;
+    if (ct_bwd) ct_bwd[l] = s->cfg_bwd[l].Ct;
+    if (thr_bwd) thr_bwd[l] = s->cfg_bwd[l].nthr;
+  }
+  return 0;
+}
+
+int pdm_get_solve_timing(pdm_scene s, long long* n_solves, double* total_ms) {
+  if (!s || !n_solves || !total_ms) {
+    g_err = "invalid argument";
+    return 1;
+  }
+  if (fold_solve_events(s)) return 1;
+  *n_solves = s->n_solves;
+  *total_ms = s->solve_ms;
+  s->n_solves = 0;
+  s->solve_ms = 0.0;
   return 0;
 }
 

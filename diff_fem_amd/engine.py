@@ -11,7 +11,8 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import PDM_FP64, PDM_HOST_EARLY_EXIT, PDM_MIXED, PDM_USE_GRAPH, check  # noqa: F401
+from ._lib import (PDM_FP64, PDM_HOST_EARLY_EXIT, PDM_MIXED, PDM_NO_AUTOTUNE, PDM_TIME_SOLVES,  # noqa: F401
+                   PDM_USE_GRAPH, check)
 
 
 def _dptr(a):
@@ -200,6 +201,22 @@ class Scene:
         ms = C.c_double()
         check(_lib.load().pdm_time_solve(self.h, int(reps), C.byref(ms)))
         return ms.value
+
+    def solve_shapes(self):
+        """Per-level launch shapes of the triangular solve: a list of
+        ((Ct, threads) forward, (Ct, threads) backward), level 0 = leaves."""
+        cap = 64
+        a = [(C.c_int * cap)() for _ in range(4)]
+        n = C.c_int()
+        check(_lib.load().pdm_get_solve_shapes(self.h, cap, C.byref(n), *a))
+        return [((a[0][l], a[1][l]), (a[2][l], a[3][l])) for l in range(min(n.value, cap))]
+
+    def solve_timing(self):
+        """(number of solves, summed ms) since the last call; needs the
+        PDM_TIME_SOLVES flag (HIP events around each solve)."""
+        n, ms = C.c_longlong(), C.c_double()
+        check(_lib.load().pdm_get_solve_timing(self.h, C.byref(n), C.byref(ms)))
+        return n.value, ms.value
 
     def timing(self):
         f, b, fa = C.c_double(), C.c_double(), C.c_double()

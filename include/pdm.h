@@ -88,7 +88,11 @@ enum {
   PDM_HOST_EARLY_EXIT = 1, /* read the convergence test back each iteration and
                               stop launching (the reference's per-iteration break);
                               without it converged rollouts are masked on device */
-  PDM_USE_GRAPH = 2        /* capture one step as a hipGraph and replay it */
+  PDM_USE_GRAPH = 2,       /* capture one step as a hipGraph and replay it */
+  PDM_TIME_SOLVES = 4,     /* bracket every triangular solve with HIP events on the
+                              library stream (read with pdm_get_solve_timing) */
+  PDM_NO_AUTOTUNE = 8      /* keep the default solve launch shapes instead of timing
+                              the candidates once per right-hand-side count */
 };
 
 const char* pdm_last_error(void);
@@ -150,6 +154,19 @@ int pdm_host_solve(const pdm_scene_desc* desc, int R, const double* b, double* x
  * all R*3 right-hand sides currently in the engine's rhs buffer) with HIP
  * events on the library stream; returns the mean ms per solve. */
 int pdm_time_solve(pdm_scene s, int reps, double* ms_per_solve);
+
+/* With PDM_TIME_SOLVES: number of triangular solves (forward + backward sweep
+ * over all 3R right-hand sides) run since the last call and their summed
+ * duration in ms (HIP events around each solve on the library stream); the
+ * counters restart at zero. No counterpart in the reference (measurement). */
+int pdm_get_solve_timing(pdm_scene s, long long* n_solves, double* total_ms);
+
+/* Launch shapes of the level kernels for the current batch (chosen by the
+ * autotuner, or the defaults under PDM_NO_AUTOTUNE): for level l < cap,
+ * column tile ct_*[l] and threads per block thr_*[l] of the forward (fwd) and
+ * backward (bwd) sweep. *n_levels receives the level count. No counterpart in
+ * the reference (launch tuning). */
+int pdm_get_solve_shapes(pdm_scene s, int cap, int* n_levels, int* ct_fwd, int* thr_fwd, int* ct_bwd, int* thr_bwd);
 
 /* Timing of the last forward/backward (ms, HIP events on the library stream). */
 int pdm_get_timing(pdm_scene s, double* fwd_ms, double* bwd_ms, double* factor_ms);

@@ -43,6 +43,20 @@ def test_solve_matches_oracle(eng, oracle_mod):
         assert rel(x[r], o.solve3(b[r])) < 1e-11
 
 
+def test_solve_shapes_do_not_change_results(eng):
+    """The autotuner only re-partitions columns and work between blocks:
+    the tuned and the default launch shapes give bitwise equal solves."""
+    d = scene.config2()
+    rng = np.random.default_rng(5)
+    b = rng.standard_normal((24, 3 * d.N))
+    s1 = eng.Scene(d, R=24)
+    s2 = eng.Scene(d, R=24, flags=eng.PDM_HOST_EARLY_EXIT | eng.PDM_NO_AUTOTUNE)
+    x1 = s1.solve_host(b)
+    x2 = s2.solve_host(b)
+    assert np.array_equal(x1, x2)
+    assert len(s1.solve_shapes()) == s1.info()["n_levels"]
+
+
 def test_force_matches_oracle(eng, oracle_mod):
     d = small_muscle_scene()
     s = eng.Scene(d, R=2)
