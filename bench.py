@@ -91,6 +91,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--graph", type=int, default=1,
+                    help="1: replay each step's PD block as a captured hipGraph (device-side early exit)")
     ap.add_argument("--solve-only", action="store_true",
                     help="only time back-to-back solves (short program for counter runs)")
     ap.add_argument("--copy-calib", type=int, default=0,
@@ -108,8 +110,10 @@ def main():
     R, S = args.rollouts, args.sim_steps
     log("setup: T=%d N=%d R=%d steps=%d" % (d.T, d.N, R, S))
     t0 = time.perf_counter()
-    sc = engine.Scene(d, R=R, precision=engine.PDM_MIXED,
-                      flags=engine.PDM_HOST_EARLY_EXIT | engine.PDM_TIME_SOLVES, checkpoint_every=args.ckpt)
+    flags = engine.PDM_HOST_EARLY_EXIT | engine.PDM_TIME_SOLVES
+    if args.graph:
+        flags |= engine.PDM_USE_GRAPH
+    sc = engine.Scene(d, R=R, precision=engine.PDM_MIXED, flags=flags, checkpoint_every=args.ckpt)
     info = sc.info()
     log("setup done in %.1fs: %s" % (time.perf_counter() - t0, info))
     log("solve launch shapes per level ((Ct, threads) fwd, bwd): %s" % (sc.solve_shapes(),))
@@ -174,6 +178,7 @@ def main():
         "config": {"workload": "c4_muscle_40x12x12_R%d_%dsteps_fwd+adjoint%s" % (R, S, "" if d.contact else "_nocontact"),
                    "tets": d.T, "vertices": d.N, "rollouts": R, "sim_steps": S, "max_iter": d.max_iter,
                    "checkpoint_every": args.ckpt, "contact": bool(d.contact), "nnz_factor": info["nnz_factor"],
+                   "hipgraph": bool(args.graph),
                    "forward_ms": float(np.mean(fwd_ms)), "adjoint_ms": float(np.mean(bwd_ms))},
     }
     # Roofline of the dominant kernel: the supernodal triangular solve

@@ -119,13 +119,14 @@ __global__ __launch_bounds__(256) void k_local_tet(const double* __restrict__ X,
                                                    int R, const int* __restrict__ mptr, const double* __restrict__ mg,
                                                    const double* __restrict__ mw, const int* __restrict__ mgroup,
                                                    int Mn, const double* __restrict__ act, int act_stride_r,
-                                                   int act_off) {
+                                                   int act_off, const int* __restrict__ step_dev, int ngroups) {
   long long idx = (long long)xcd_block() * blockDim.x + threadIdx.x;
   if (idx >= (long long)Tn * R) return;
   int t = (int)(idx / R);
   int r = (int)(idx % R);
   if (UNI) t = __builtin_amdgcn_readfirstlane(t);
   if (!active[r]) return;
+  if (step_dev) act_off = *step_dev * ngroups;
   int C = 3 * R;
   int v0 = tv[t], v1 = tv[Tn + t], v2 = tv[2 * Tn + t], v3 = tv[3 * Tn + t];
   const double* x0 = X + (long long)v0 * C + 3 * r;
@@ -198,7 +199,9 @@ __device__ __forceinline__ void load4(const double* p, double* o) {
 template <typename T, int MODE>
 __global__ __launch_bounds__(256) void k_gather4(const double* __restrict__ Cc, const T* __restrict__ corner,
                                                  const int* __restrict__ vptr, const int* __restrict__ active,
-                                                 double* __restrict__ B, int N, int C) {
+                                                 double* __restrict__ B, int N, int C,
+                                                 const int* __restrict__ any = nullptr) {
+  if (any && *any == 0) return;
   const int C4 = C >> 2;
   long long idx = (long long)xcd_block() * blockDim.x + threadIdx.x;
   if (idx >= (long long)N * C4) return;
@@ -386,7 +389,9 @@ __device__ __forceinline__ v4d mfma_chain(v4d acc, int kbeg, int kend, int lk, F
 //   T[w:m] -= L[w:m, 0:w] y         (update block U_s for the parent)
 // Both are split into 16x16 output tiles distributed over the waves.
 __global__ __launch_bounds__(1024) void k_fwd_level(FactorView f, const int* __restrict__ lev, int ntile, int Ct, int C,
-                                                   double* __restrict__ B, double* __restrict__ U) {
+                                                   double* __restrict__ B, double* __restrict__ U,
+                                                   const int* __restrict__ any) {
+  if (any && *any == 0) return;
   extern __shared__ double sh[];
   const int lb = xcd_block();
   const int s = lev[lb / ntile];
@@ -486,7 +491,8 @@ __global__ __launch_bounds__(1024) void k_fwd_level(FactorView f, const int* __r
 //   t = y_s - L[w:m, 0:w]^T x_below   (rows below are final)
 //   x_s = inv(L_ss)^T t
 __global__ __launch_bounds__(1024) void k_bwd_level(FactorView f, const int* __restrict__ lev, int ntile, int Ct, int C,
-                                                   double* __restrict__ B) {
+                                                   double* __restrict__ B, const int* __restrict__ any) {
+  if (any && *any == 0) return;
   extern __shared__ double sh[];
   const int lb = xcd_block();
   const int s = lev[lb / ntile];
@@ -546,13 +552,19 @@ __global__ __launch_bounds__(1024) void k_bwd_level(FactorView f, const int* __r
   }
 }
 
+__global__ void k_set_int(int* __restrict__ p, int v) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *p = v;
+}
+
 // x_new = x_n + B for active rollouts; per-(chunk, column) partial sums of
 // (x_new - x_old)^2 for the early-exit test. Block (64 columns, 4 rows).
 constexpr int kChunk = 64;
 __global__ void k_update(const double* __restrict__ Xn, const double* __restrict__ B, double* __restrict__ Xit,
                          const int* __restrict__ active, double* __restrict__ part, double* __restrict__ tape, int N,
-                         int C) {
+                         int C, int* __restrict__ any) {
   __shared__ double red[4][64];
+  // any: "some rollout still iterates", rebuilt by k_err_final right after
+  if (any && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 && threadIdx.y == 0) *any = 0;
   int col = blockIdx.x * 64 + threadIdx.x;
   int chunk = blockIdx.y;
   double acc = 0.0;
@@ -578,7 +590,7 @@ __global__ void k_update(const double* __restrict__ Xn, const double* __restrict
 // Per rollout: err = ||x_new - x_old||_2 / (3N) (World.h:890-895); a rollout
 // whose err falls under 5e-7 stops iterating. iters counts executed iterations.
 __global__ void k_err_final(const double* __restrict__ part, int nchunk, int R, int N, int* __restrict__ active,
-                            int* __restrict__ iters, double* __restrict__ err) {
+                            int* __restrict__ iters, double* __restrict__ err, int* __restrict__ any) {
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= R) return;
   int C = 3 * R;
@@ -589,24 +601,31 @@ __global__ void k_err_final(const double* __restrict__ part, int nchunk, int R, 
   double e = sqrt(s) / (3.0 * N);
   err[r] = e;
   iters[r] += 1;
-  if (e < kEps) active[r] = 0;
+  if (e < kEps)
+    active[r] = 0;
+  else if (any)
+    atomicOr(any, 1);
 }
 
 // First check of the loop (iteration 0 compares qn with zero).
 __global__ void k_err_init(const double* __restrict__ part, int nchunk, int R, int N, int* __restrict__ active,
-                           int* __restrict__ iters) {
+                           int* __restrict__ iters, int* __restrict__ any) {
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= R) return;
   int C = 3 * R;
   double s = 0.0;
   for (int ch = 0; ch < nchunk; ++ch)
     for (int c = 0; c < 3; ++c) s += part[(long long)ch * C + 3 * r + c];
-  active[r] = (sqrt(s) / (3.0 * N) < kEps) ? 0 : 1;
+  int a = (sqrt(s) / (3.0 * N) < kEps) ? 0 : 1;
+  active[r] = a;
   iters[r] = 0;
+  if (any && a) atomicOr(any, 1);
 }
 
-__global__ void k_sqsum_chunks(const double* __restrict__ X, double* __restrict__ part, int N, int C) {
+__global__ void k_sqsum_chunks(const double* __restrict__ X, double* __restrict__ part, int N, int C,
+                               int* __restrict__ any) {
   __shared__ double red[4][64];
+  if (any && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 && threadIdx.y == 0) *any = 0;
   int col = blockIdx.x * 64 + threadIdx.x;
   int chunk = blockIdx.y;
   double acc = 0.0;
@@ -1374,6 +1393,12 @@ struct pdm_scene_s {
   DevBuf<int> ractive;
   double t_fwd = 0, t_bwd = 0, t_factor = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // PDM_USE_GRAPH: captured step block and its device-side controls
+  hipGraphExec_t gexec = nullptr;
+  const double* g_act = nullptr;
+  int g_act_steps = -1;
+  bool capturing = false;
+  DevBuf<int> any_dev, step_dev;
   // PDM_TIME_SOLVES: event pairs around each solve, folded into solve_ms
   std::vector<hipEvent_t> sev;
   size_t sev_used = 0;
@@ -1655,7 +1680,8 @@ pdm_scene_s::LevelCfg level_default(const pdm_scene_s* s, int l) {
   return c;
 }
 
-int launch_level(pdm_scene s, int pass, int l, const pdm_scene_s::LevelCfg& c, double* Bp) {
+int launch_level(pdm_scene s, int pass, int l, const pdm_scene_s::LevelCfg& c, double* Bp,
+                 const int* any = nullptr) {
   const FactorView fv = s->view();
   int cnt = s->lev_ptr[l + 1] - s->lev_ptr[l];
   size_t lds = level_lds(s, l, c.Ct);
@@ -1666,10 +1692,10 @@ int launch_level(pdm_scene s, int pass, int l, const pdm_scene_s::LevelCfg& c, d
   int ntile = cdiv(s->C, c.Ct);
   if (pass == 0)
     hipLaunchKernelGGL(k_fwd_level, dim3(cnt * ntile), dim3(c.nthr), lds, s->stream, fv, s->f_lev.p + s->lev_ptr[l],
-                       ntile, c.Ct, s->C, Bp, s->U.p);
+                       ntile, c.Ct, s->C, Bp, s->U.p, any);
   else
     hipLaunchKernelGGL(k_bwd_level, dim3(cnt * ntile), dim3(c.nthr), lds, s->stream, fv, s->f_lev.p + s->lev_ptr[l],
-                       ntile, c.Ct, s->C, Bp);
+                       ntile, c.Ct, s->C, Bp, any);
   return 0;
 }
 
@@ -1693,6 +1719,7 @@ int autotune_levels(pdm_scene s) {
   s->cfg_fwd.resize(nlev);
   s->cfg_bwd.resize(nlev);
   for (int l = 0; l < nlev; ++l) s->cfg_fwd[l] = s->cfg_bwd[l] = level_default(s, l);
+  if (set_level_attrs()) return 1;
   if (!s->autotune) {
     s->tuned_C = s->C;
     return 0;
@@ -1749,10 +1776,12 @@ int fold_solve_events(pdm_scene s) {
   return 0;
 }
 
-int launch_solve_levels(pdm_scene s, double* Bp);
+int launch_solve_levels(pdm_scene s, double* Bp, const int* any = nullptr);
 
-int launch_solve(pdm_scene s, double* Bp) {
-  if (!(s->flags & PDM_TIME_SOLVES)) return launch_solve_levels(s, Bp);
+// any: optional device flag; the level kernels return at once when it is 0
+// (no rollout iterates any more), which graph replays rely on.
+int launch_solve(pdm_scene s, double* Bp, const int* any = nullptr) {
+  if (!(s->flags & PDM_TIME_SOLVES) || s->capturing) return launch_solve_levels(s, Bp, any);
   if (s->sev_used + 2 > s->sev.size()) {
     if (s->sev.size() >= 4096) {
       if (fold_solve_events(s)) return 1;
@@ -1770,7 +1799,7 @@ int launch_solve(pdm_scene s, double* Bp) {
   return 0;
 }
 
-int launch_solve_levels(pdm_scene s, double* Bp) {
+int launch_solve_levels(pdm_scene s, double* Bp, const int* any) {
   if (set_level_attrs()) return 1;
   if (s->tuned_C != s->C) {
     g_err = "solve launch shapes not set up for this batch";
@@ -1780,28 +1809,26 @@ int launch_solve_levels(pdm_scene s, double* Bp) {
   for (int pass = 0; pass < 2; ++pass)
     for (int q = 0; q < nlev; ++q) {
       int l = pass == 0 ? q : nlev - 1 - q;
-      if (launch_level(s, pass, l, pass == 0 ? s->cfg_fwd[l] : s->cfg_bwd[l], Bp)) return 1;
+      if (launch_level(s, pass, l, pass == 0 ? s->cfg_fwd[l] : s->cfg_bwd[l], Bp, any)) return 1;
     }
   HIPX(hipGetLastError());
   return 0;
 }
 
 template <typename T>
-void launch_local(pdm_scene s, const double* Xsrc, const double* act, int act_steps, int step, const int* mask) {
+void launch_local(pdm_scene s, const double* Xsrc, const double* act, int act_steps, int step, const int* mask,
+                  const int* step_dev = nullptr, const int* any = nullptr) {
   HostScene& S = s->S;
   T* corner = (T*)s->corner.p;
   long long nt = (long long)S.T * s->R;
-  if (s->R % 64 == 0)
-    hipLaunchKernelGGL((k_local_tet<T, true>), dim3(cdiv(nt, kThreads)), dim3(kThreads), 0, s->stream, Xsrc, s->tv.p,
-                       s->tG.p, s->wr.p, s->wv.p, s->tslot.p, mask, corner, S.T, s->R, s->mptr.p, s->mg.p, s->mw.p,
-                       s->mgroup.p, S.M, act, act_steps * S.n_groups, step * S.n_groups);
-  else
-    hipLaunchKernelGGL((k_local_tet<T, false>), dim3(cdiv(nt, kThreads)), dim3(kThreads), 0, s->stream, Xsrc, s->tv.p,
-                       s->tG.p, s->wr.p, s->wv.p, s->tslot.p, mask, corner, S.T, s->R, s->mptr.p, s->mg.p, s->mw.p,
-                       s->mgroup.p, S.M, act, act_steps * S.n_groups, step * S.n_groups);
+  const int G = S.n_groups;
+  auto kern = s->R % 64 == 0 ? k_local_tet<T, true> : k_local_tet<T, false>;
+  hipLaunchKernelGGL(kern, dim3(cdiv(nt, kThreads)), dim3(kThreads), 0, s->stream, Xsrc, s->tv.p, s->tG.p, s->wr.p,
+                     s->wv.p, s->tslot.p, mask, corner, S.T, s->R, s->mptr.p, s->mg.p, s->mw.p, s->mgroup.p, S.M, act,
+                     act_steps * G, step * G, step_dev, G);
   if (s->C % 4 == 0)
     hipLaunchKernelGGL((k_gather4<T, 0>), dim3(cdiv(s->NC() / 4, kThreads)), dim3(kThreads), 0, s->stream, s->Cc.p,
-                       corner, s->vptr.p, nullptr, s->B.p, S.N, s->C);
+                       corner, s->vptr.p, nullptr, s->B.p, S.N, s->C, any);
   else
     hipLaunchKernelGGL(k_gather<T>, dim3(cdiv(s->NC(), kThreads)), dim3(kThreads), 0, s->stream, s->Cc.p, corner,
                        s->vptr.p, s->B.p, S.N, s->C);
@@ -1883,10 +1910,12 @@ int contact_phase(pdm_scene s, const double* act, int act_steps, int step, Conta
   return 0;
 }
 
-// One time step on device. tapeY: optional (max_iter+1) x NC buffer for the
-// iterates; tIters: optional R ints receiving the executed iterations.
-int run_step(pdm_scene s, const double* act, int act_steps, int step, double* tapeY, double* tapeXn, double* tapeVs,
-             int* tIters, int* tNpass = nullptr, int* tCiters = nullptr, ContactTape* ctp = nullptr) {
+// Start of a step and its PD iterations (World.h:673-699, 835-919). any and
+// step_dev non-null: device-side control for graph capture (no host reads;
+// activations indexed by the device step counter; the gather and the solve
+// return at once when no rollout iterates any more).
+int launch_pd_block(pdm_scene s, const double* act, int act_steps, int step, double* tapeY, double* tapeXn,
+                    double* tapeVs, bool host_exit, int* any, const int* step_dev) {
   HostScene& S = s->S;
   int N = S.N, C = s->C;
   size_t nc = s->NC();
@@ -1900,32 +1929,81 @@ int run_step(pdm_scene s, const double* act, int act_steps, int step, double* ta
   if (tapeXn) HIPX(hipMemcpyAsync(tapeXn, s->Xn.p, nc * 8, hipMemcpyDeviceToDevice, s->stream));
   if (tapeVs) HIPX(hipMemcpyAsync(tapeVs, s->Vs.p, nc * 8, hipMemcpyDeviceToDevice, s->stream));
   if (tapeY) HIPX(hipMemcpyAsync(tapeY, s->Xit.p, nc * 8, hipMemcpyDeviceToDevice, s->stream));
-  hipLaunchKernelGGL(k_sqsum_chunks, ugrid, ublock, 0, s->stream, s->Xit.p, s->part.p, N, C);
+  hipLaunchKernelGGL(k_sqsum_chunks, ugrid, ublock, 0, s->stream, s->Xit.p, s->part.p, N, C, any);
   hipLaunchKernelGGL(k_err_init, dim3(cdiv(s->R, 64)), dim3(64), 0, s->stream, s->part.p, nchunk, s->R, N,
-                     s->active.p, s->iters_dev.p);
-  bool host_exit = (s->flags & PDM_HOST_EARLY_EXIT) != 0;
+                     s->active.p, s->iters_dev.p, any);
   std::vector<int> act_h(s->R);
   for (int it = 0; it < S.max_iter; ++it) {
     if (host_exit) {
       HIPX(hipMemcpyAsync(act_h.data(), s->active.p, s->R * sizeof(int), hipMemcpyDeviceToHost, s->stream));
       HIPX(hipStreamSynchronize(s->stream));
-      bool any = false;
-      for (int r = 0; r < s->R; ++r) any |= act_h[r] != 0;
-      if (!any) break;
+      bool some = false;
+      for (int r = 0; r < s->R; ++r) some |= act_h[r] != 0;
+      if (!some) break;
     }
     if (s->precision == PDM_MIXED)
-      launch_local<float>(s, s->Xit.p, act, act_steps, step, s->active.p);
+      launch_local<float>(s, s->Xit.p, act, act_steps, step, s->active.p, step_dev, any);
     else
-      launch_local<double>(s, s->Xit.p, act, act_steps, step, s->active.p);
-    if (launch_solve(s, s->B.p)) return 1;
+      launch_local<double>(s, s->Xit.p, act, act_steps, step, s->active.p, step_dev, any);
+    if (launch_solve(s, s->B.p, any)) return 1;
     hipLaunchKernelGGL(k_update, ugrid, ublock, 0, s->stream, s->Xn.p, s->B.p, s->Xit.p, s->active.p, s->part.p,
-                       tapeY ? tapeY + (size_t)(it + 1) * nc : nullptr, N, C);
+                       tapeY ? tapeY + (size_t)(it + 1) * nc : nullptr, N, C, any);
     hipLaunchKernelGGL(k_err_final, dim3(cdiv(s->R, 64)), dim3(64), 0, s->stream, s->part.p, nchunk, s->R, N,
-                       s->active.p, s->iters_dev.p, s->err.p);
+                       s->active.p, s->iters_dev.p, s->err.p, any);
   }
-  if (tapeY && host_exit) {
-    // iterations skipped by the host break keep the last iterate
-    // (the reverse sweep never reads them: it is bounded by tIters)
+  HIPX(hipGetLastError());
+  return 0;
+}
+
+void drop_step_graph(pdm_scene s) {
+  if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
+  s->gexec = nullptr;
+}
+
+// PDM_USE_GRAPH: the prep + PD block of a step, captured once per forward call
+// (buffers and the activation pointer are fixed within it) and replayed every
+// step after the device step counter is set.
+int ensure_step_graph(pdm_scene s, const double* act, int act_steps) {
+  if (s->gexec && s->g_act == act && s->g_act_steps == act_steps) return 0;
+  drop_step_graph(s);
+  if (s->any_dev.alloc(1) || s->step_dev.alloc(1)) return 1;
+  HIPX(hipStreamBeginCapture(s->stream, hipStreamCaptureModeRelaxed));
+  s->capturing = true;
+  int rc = launch_pd_block(s, act, act_steps, 0, nullptr, nullptr, nullptr, false, s->any_dev.p, s->step_dev.p);
+  s->capturing = false;
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(s->stream, &g);
+  if (rc || e != hipSuccess || !g) {
+    if (g) (void)hipGraphDestroy(g);
+    if (!rc) g_err = std::string("graph capture failed: ") + hipGetErrorString(e);
+    return 1;
+  }
+  e = hipGraphInstantiate(&s->gexec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    s->gexec = nullptr;
+    g_err = std::string("graph instantiation failed: ") + hipGetErrorString(e);
+    return 1;
+  }
+  s->g_act = act;
+  s->g_act_steps = act_steps;
+  return 0;
+}
+
+// One time step on device. tapeY: optional (max_iter+1) x NC buffer for the
+// iterates; tIters: optional R ints receiving the executed iterations.
+int run_step(pdm_scene s, const double* act, int act_steps, int step, double* tapeY, double* tapeXn, double* tapeVs,
+             int* tIters, int* tNpass = nullptr, int* tCiters = nullptr, ContactTape* ctp = nullptr) {
+  HostScene& S = s->S;
+  size_t nc = s->NC();
+  bool host_exit = (s->flags & PDM_HOST_EARLY_EXIT) != 0;
+  bool graph = (s->flags & PDM_USE_GRAPH) && !tapeY && !tapeXn && !tapeVs;
+  if (graph) {
+    if (ensure_step_graph(s, act, act_steps)) return 1;
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, s->stream, s->step_dev.p, step);
+    HIPX(hipGraphLaunch(s->gexec, s->stream));
+  } else if (launch_pd_block(s, act, act_steps, step, tapeY, tapeXn, tapeVs, host_exit, nullptr, nullptr)) {
+    return 1;
   }
   if (s->ncv > 0) {
     HIPX(hipMemsetAsync(s->npass.p, 0, s->R * sizeof(int), s->stream));
@@ -1948,6 +2026,7 @@ int forward_impl(pdm_scene s, int steps, const double* x0, const double* v0, con
     return 1;
   }
   if (ensure_batch_buffers(s)) return 1;
+  drop_step_graph(s);  // recaptured on the first step of this call
   HostScene& S = s->S;
   size_t nc = s->NC();
   int N = S.N;
@@ -2347,6 +2426,7 @@ void pdm_scene_destroy(pdm_scene s) {
   if (!s) return;
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   for (hipEvent_t e : s->sev) (void)hipEventDestroy(e);
+  drop_step_graph(s);
   if (s->ev0) (void)hipEventDestroy(s->ev0);
   if (s->ev1) (void)hipEventDestroy(s->ev1);
   if (s->own_stream && s->stream) (void)hipStreamDestroy(s->stream);
@@ -2418,10 +2498,6 @@ int pdm_set_material(pdm_scene s, const double* E, double nu_fwd) {
 int pdm_set_batch(pdm_scene s, int R, int precision, int flags, int checkpoint_every) {
   if (!s || R < 1 || (precision != PDM_FP64 && precision != PDM_MIXED) || checkpoint_every < 1) {
     g_err = "invalid batch arguments";
-    return 1;
-  }
-  if (flags & PDM_USE_GRAPH) {
-    g_err = "hipGraph capture is not available in this build of the engine";
     return 1;
   }
   HIPX(hipStreamSynchronize(s->stream));

@@ -88,7 +88,11 @@ enum {
   PDM_HOST_EARLY_EXIT = 1, /* read the convergence test back each iteration and
                               stop launching (the reference's per-iteration break);
                               without it converged rollouts are masked on device */
-  PDM_USE_GRAPH = 2,       /* capture one step as a hipGraph and replay it */
+  PDM_USE_GRAPH = 2,       /* forward: capture the step start + PD iterations once per
+                              call as a hipGraph and replay it every step, with the
+                              early exit decided on the device (converged rollouts
+                              masked, gather and solves skipped once none iterates);
+                              the contact phase and the adjoint stay eager */
   PDM_TIME_SOLVES = 4,     /* bracket every triangular solve with HIP events on the
                               library stream (read with pdm_get_solve_timing) */
   PDM_NO_AUTOTUNE = 8      /* keep the default solve launch shapes instead of timing
