@@ -37,6 +37,7 @@ class SceneDescC(C.Structure):
         ("contact", C.c_int), ("max_iter1", C.c_int), ("alpha", C.c_double), ("mu", C.c_double),
         ("n_planes", C.c_int), ("planes", _dp),
         ("n_spheres", C.c_int), ("spheres", _dp),
+        ("nu_fwd_tet", _dp), ("n_bodies", C.c_int), ("body_of_tet", _ip), ("body_mass", _dp),
     ]
 
 

@@ -1465,8 +1465,8 @@ int upload_material(pdm_scene s) {
   if (s->wv.upload(S.wv.data(), S.T)) return 1;
   std::vector<double> der(S.T), dev(S.T);
   for (int t = 0; t < S.T; ++t) {
-    der[t] = S.tvol[t] / (1.0 + S.nu_fwd);
-    dev[t] = S.tvol[t] * S.nu_ctor[t] / (1.0 + S.nu_fwd);
+    der[t] = S.tvol[t] / (1.0 + S.nu_fwd_t[t]);
+    dev[t] = S.tvol[t] * S.nu_ctor[t] / (1.0 + S.nu_fwd_t[t]);
   }
   if (s->dEr.upload(der.data(), S.T)) return 1;
   if (s->dEv.upload(dev.data(), S.T)) return 1;
@@ -1556,12 +1556,19 @@ int host_setup(const pdm_scene_desc* d, pdm_scene s) {
     g_err = err;
     return 1;
   }
+  S.nu_fwd = d->nu_fwd;
+  S.nu_fwd_t.assign(S.T, d->nu_fwd);
   for (int t = 0; t < S.T; ++t) {
     int ct = S.tet_order[t];
     S.E[t] = d->E ? d->E[ct] : 2e5;
     S.nu_ctor[t] = d->nu_ctor ? d->nu_ctor[ct] : 0.4;
+    if (d->nu_fwd_tet) S.nu_fwd_t[t] = d->nu_fwd_tet[ct];
   }
-  S.nu_fwd = d->nu_fwd;
+  if (d->n_bodies > 0 &&
+      !set_body_masses(S, d->tets, d->n_bodies, d->body_of_tet, d->body_mass, d->mass_same, err)) {
+    g_err = err;
+    return 1;
+  }
   for (int k = 0; k < d->n_attach; ++k) {
     int a = d->attach_idx[k];
     if (a < 0 || a >= S.N) {
@@ -2280,10 +2287,9 @@ int backward_impl(pdm_scene s, const double* gxT, const double* gvT, double* gx0
   HIPX(hipStreamSynchronize(s->stream));
   HIPX(hipGetLastError());
   if (gnu) {
-    // mMu = E/(1+nu): dL/dnu = -sum_t E_t dL/dE_t / (1+nu)
-    double acc = 0;
-    for (int t = 0; t < S.T; ++t) acc += S.E[t] * gEh[S.tet_order[t]];
-    double val = -acc / (1.0 + S.nu_fwd);
+    // mMu = E/(1+nu_t): dL/dnu (one nu shifted for every tet) = -sum_t E_t dL/dE_t / (1+nu_t)
+    double val = 0;
+    for (int t = 0; t < S.T; ++t) val -= S.E[t] * gEh[S.tet_order[t]] / (1.0 + S.nu_fwd_t[t]);
     HIPX(hipMemcpy(gnu, &val, 8, hipMemcpyHostToDevice));
   }
   float ms = 0;
@@ -2490,7 +2496,10 @@ int pdm_set_material(pdm_scene s, const double* E, double nu_fwd) {
   HostScene& S = s->S;
   if (E)
     for (int t = 0; t < S.T; ++t) S.E[t] = E[S.tet_order[t]];
-  S.nu_fwd = nu_fwd;
+  if (!std::isnan(nu_fwd)) {
+    S.nu_fwd = nu_fwd;
+    S.nu_fwd_t.assign(S.T, nu_fwd);
+  }
   s->have_tape = false;
   return refactor(s);
 }

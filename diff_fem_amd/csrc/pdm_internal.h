@@ -24,6 +24,7 @@ struct HostScene {
   std::vector<double> E;           // T
   std::vector<double> nu_ctor;     // T
   double nu_fwd = 0.4;
+  std::vector<double> nu_fwd_t;    // T: nu entering mMu per engine tet
   std::vector<double> wr, wv;      // T
   // muscles
   std::vector<int> mv;             // 4M
@@ -91,12 +92,15 @@ bool build_scene(HostScene& S, int N, const double* xrest, int T, const int* tet
                  std::string& err);
 void nd_order(int N, const double* xyz, const std::vector<std::vector<int>>& adj, std::vector<int>& order);
 void update_weights(HostScene& S);
+bool set_body_masses(HostScene& S, const int* tets, int nb, const int* body_of_tet, const double* body_mass,
+                     int mass_same, std::string& err);
 void assemble_L(const HostScene& S, Csr& L);  // elastic + muscle + attachment (no mass)
 bool symbolic(const Csr& A, Factor& F, std::string& err);
 bool numeric(const Csr& A, const std::vector<double>& diag_add, Factor& F, std::string& err);
 void build_corners(const HostScene& S, Corners& C);
+// cand: optional increasing list of candidate tets (all tets when null)
 int select_muscle_tets(const double* x_caller, const int* tets_caller, int T, const double* start, const double* end,
-                       std::vector<int>& out_tet, std::vector<double>& out_w);
+                       std::vector<int>& out_tet, std::vector<double>& out_w, const std::vector<int>* cand = nullptr);
 bool add_muscles(HostScene& S, const double* x_caller, const int* tets_caller, int n_seg, const double* seg_start,
                  const double* seg_end, const int* seg_group, int n_groups, double k_muscle, std::string& err);
 std::vector<int> surface_vertices(int N, const std::vector<int>& tv, int T);

@@ -291,11 +291,50 @@ bool build_scene(HostScene& S, int N, const double* xrest, int T, const int* tet
   return true;
 }
 
+// Several bodies in one world: body b's mass is lumped on its own tets
+// (Dfobj.h:506-525), m_v = sum over the body's tets at v of Mass_b vol_t / (4 V_b),
+// or Mass_b / N_b per vertex without "Is Mass Same".
+bool set_body_masses(HostScene& S, const int* tets, int nb, const int* body_of_tet, const double* body_mass,
+                     int mass_same, std::string& err) {
+  if (!body_of_tet || !body_mass) {
+    err = "body arrays missing";
+    return false;
+  }
+  std::vector<double> vol_b(nb, 0.0), mc(S.N, 0.0);
+  std::vector<int> vbody(S.N, -1), nv_b(nb, 0);
+  for (int e = 0; e < S.T; ++e) {
+    int t = S.tet_order[e], b = body_of_tet[t];
+    if (b < 0 || b >= nb) {
+      err = "body index out of range";
+      return false;
+    }
+    vol_b[b] += S.tvol[e];
+    for (int k = 0; k < 4; ++k) vbody[tets[4 * t + k]] = b;
+  }
+  for (int b = 0; b < nb; ++b)
+    if (!(vol_b[b] > 0) || !(body_mass[b] > 0)) {
+      err = "body without volume or mass";
+      return false;
+    }
+  for (int i = 0; i < S.N; ++i)
+    if (vbody[i] >= 0) nv_b[vbody[i]]++;
+  for (int e = 0; e < S.T; ++e) {
+    int t = S.tet_order[e], b = body_of_tet[t];
+    for (int k = 0; k < 4; ++k) mc[tets[4 * t + k]] += 0.25 * S.tvol[e] * body_mass[b] / vol_b[b];
+  }
+  for (int i = 0; i < S.N; ++i) {
+    int c = S.perm[i];
+    S.mass[i] = mass_same ? mc[c] : (vbody[c] >= 0 ? body_mass[vbody[c]] / nv_b[vbody[c]] : 0.0);
+  }
+  return true;
+}
+
 void update_weights(HostScene& S) {
+  if (S.nu_fwd_t.size() != (size_t)S.T) S.nu_fwd_t.assign(S.T, S.nu_fwd);
   S.wr.resize(S.T);
   S.wv.resize(S.T);
   for (int t = 0; t < S.T; ++t) {
-    double mMu = S.E[t] / (1.0 + S.nu_fwd);  // CorotateFEMConstraint.h:69; pydifem.cc:217
+    double mMu = S.E[t] / (1.0 + S.nu_fwd_t[t]);  // CorotateFEMConstraint.h:69; pydifem.cc:217
     S.wr[t] = mMu * S.tvol[t];
     S.wv[t] = mMu * S.tvol[t] * S.nu_ctor[t];
   }
@@ -673,12 +712,14 @@ inline double dist3(const double* a, const double* b) {
 }  // namespace
 
 int select_muscle_tets(const double* X, const int* tets, int T, const double* start, const double* end,
-                       std::vector<int>& out_tet, std::vector<double>& out_w) {
+                       std::vector<int>& out_tet, std::vector<double>& out_w, const std::vector<int>* cand) {
   out_tet.clear();
   out_w.clear();
   double seg = dist3(start, end);
   double dir[3] = {(end[0] - start[0]) / seg, (end[1] - start[1]) / seg, (end[2] - start[2]) / seg};
-  for (int j = 0; j < T; ++j) {
+  const int nj = cand ? (int)cand->size() : T;
+  for (int q = 0; q < nj; ++q) {
+    const int j = cand ? (*cand)[q] : q;
     int v[4] = {tets[4 * j], tets[4 * j + 1], tets[4 * j + 2], tets[4 * j + 3]};
     double Dm[9];
     dm_of(X, v[0], v[1], v[2], v[3], Dm);
@@ -717,6 +758,98 @@ int select_muscle_tets(const double* X, const int* tets, int T, const double* st
 // LinearMuscleConstraint data per selected tet (LinearMuscleConstraint.h:89-110,
 // 150-168): stencil g = [-(a+b+c), a, b, c] with (a,b,c) = invDm f, weight
 // vol * k_muscle * exp-weight.
+// Uniform grid over tet bounding boxes: candidate tets of a segment in
+// increasing index order. A tet can be selected only if its box, grown by
+// kReach, meets the segment's box: crossing needs the boxes to meet, a weight
+// above 0.2 needs the centroid within 0.005*ln 5 of the segment, and the
+// end-point containment test allows 1e-4 of relative volume slack (a sliver
+// of the tet's size). The candidates then go through the same test in the
+// same order as the full scan, so the selection is unchanged.
+struct TetGrid {
+  double lo[3], cell;
+  int n[3];
+  std::vector<int> start, items;
+  std::vector<double> box;  // 6 per tet
+  std::vector<int> stamp;
+  int cur = 0;
+  static constexpr double kReach = 0.005 * 1.6094379124341003 + 1e-6;
+  void build(const double* X, const int* tets, int T) {
+    box.resize(6 * (size_t)T);
+    double hi[3] = {-1e300, -1e300, -1e300}, ext = 0;
+    for (int r = 0; r < 3; ++r) lo[r] = 1e300;
+    for (int j = 0; j < T; ++j) {
+      double* b = &box[6 * (size_t)j];
+      for (int r = 0; r < 3; ++r) {
+        b[r] = 1e300;
+        b[3 + r] = -1e300;
+      }
+      for (int k = 0; k < 4; ++k)
+        for (int r = 0; r < 3; ++r) {
+          double x = X[3 * tets[4 * j + k] + r];
+          b[r] = std::min(b[r], x);
+          b[3 + r] = std::max(b[3 + r], x);
+        }
+      double diag = std::sqrt((b[3] - b[0]) * (b[3] - b[0]) + (b[4] - b[1]) * (b[4] - b[1]) + (b[5] - b[2]) * (b[5] - b[2]));
+      for (int r = 0; r < 3; ++r) {
+        b[r] -= kReach + 1e-3 * diag;
+        b[3 + r] += kReach + 1e-3 * diag;
+        lo[r] = std::min(lo[r], b[r]);
+        hi[r] = std::max(hi[r], b[3 + r]);
+        ext = std::max(ext, b[3 + r] - b[r]);
+      }
+    }
+    double dom = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
+    cell = std::max(ext, dom / 96.0);
+    for (int r = 0; r < 3; ++r) n[r] = std::max(1, (int)std::ceil((hi[r] - lo[r]) / cell) + 1);
+    size_t nc = (size_t)n[0] * n[1] * n[2];
+    std::vector<int> cnt(nc + 1, 0);
+    auto each = [&](int j, auto&& f) {
+      const double* b = &box[6 * (size_t)j];
+      int a0[3], a1[3];
+      for (int r = 0; r < 3; ++r) {
+        a0[r] = std::min(n[r] - 1, std::max(0, (int)((b[r] - lo[r]) / cell)));
+        a1[r] = std::min(n[r] - 1, std::max(0, (int)((b[3 + r] - lo[r]) / cell)));
+      }
+      for (int x = a0[0]; x <= a1[0]; ++x)
+        for (int y = a0[1]; y <= a1[1]; ++y)
+          for (int z = a0[2]; z <= a1[2]; ++z) f(((size_t)x * n[1] + y) * n[2] + z);
+    };
+    for (int j = 0; j < T; ++j) each(j, [&](size_t c) { cnt[c + 1]++; });
+    for (size_t c = 0; c < nc; ++c) cnt[c + 1] += cnt[c];
+    start = cnt;
+    items.resize(cnt[nc]);
+    for (int j = 0; j < T; ++j) each(j, [&](size_t c) { items[cnt[c]++] = j; });
+    stamp.assign(T, 0);
+  }
+  void query(const double* a, const double* b, std::vector<int>& out) {
+    out.clear();
+    ++cur;
+    double q0[3], q1[3];
+    int c0[3], c1[3];
+    for (int r = 0; r < 3; ++r) {
+      q0[r] = std::min(a[r], b[r]);
+      q1[r] = std::max(a[r], b[r]);
+      c0[r] = std::min(n[r] - 1, std::max(0, (int)((q0[r] - lo[r]) / cell)));
+      c1[r] = std::min(n[r] - 1, std::max(0, (int)((q1[r] - lo[r]) / cell)));
+    }
+    for (int x = c0[0]; x <= c1[0]; ++x)
+      for (int y = c0[1]; y <= c1[1]; ++y)
+        for (int z = c0[2]; z <= c1[2]; ++z) {
+          size_t c = ((size_t)x * n[1] + y) * n[2] + z;
+          for (int i = start[c]; i < start[c + 1]; ++i) {
+            int j = items[i];
+            if (stamp[j] == cur) continue;
+            stamp[j] = cur;
+            const double* bb = &box[6 * (size_t)j];
+            bool meet = true;
+            for (int r = 0; r < 3; ++r) meet &= bb[r] <= q1[r] && bb[3 + r] >= q0[r];
+            if (meet) out.push_back(j);
+          }
+        }
+    std::sort(out.begin(), out.end());
+  }
+};
+
 bool add_muscles(HostScene& S, const double* X, const int* tets, int n_seg, const double* seg_start,
                  const double* seg_end, const int* seg_group, int n_groups, double k_muscle, std::string& err) {
   S.mv.clear();
@@ -728,6 +861,9 @@ bool add_muscles(HostScene& S, const double* X, const int* tets, int n_seg, cons
   std::vector<double> w;
   std::vector<int> etet(S.T);
   for (int e = 0; e < S.T; ++e) etet[S.tet_order[e]] = e;
+  TetGrid grid;
+  std::vector<int> cand;
+  if (n_seg > 0) grid.build(X, tets, S.T);
   for (int s = 0; s < n_seg; ++s) {
     if (seg_group[s] < 0 || seg_group[s] >= n_groups) {
       err = "muscle segment group out of range";
@@ -742,7 +878,8 @@ bool add_muscles(HostScene& S, const double* X, const int* tets, int n_seg, cons
       return false;
     }
     for (int r = 0; r < 3; ++r) f[r] /= fl;
-    select_muscle_tets(X, tets, S.T, st, en, sel, w);
+    grid.query(st, en, cand);
+    select_muscle_tets(X, tets, S.T, st, en, sel, w, &cand);
     for (size_t k = 0; k < sel.size(); ++k) {
       int v[4] = {tets[4 * sel[k]], tets[4 * sel[k] + 1], tets[4 * sel[k] + 2], tets[4 * sel[k] + 3]};
       double vol, G[12];

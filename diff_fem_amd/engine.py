@@ -51,6 +51,12 @@ def make_desc(desc):
     pl = arr(desc.planes.ravel(), np.float64)
     sp = arr(desc.spheres.ravel(), np.float64)
     d = _lib.SceneDescC()
+    if desc.nu_fwd_tet is not None:
+        d.nu_fwd_tet = _dptr(arr(desc.nu_fwd_tet, np.float64))
+    if desc.body_of_tet is not None:
+        d.n_bodies = int(desc.body_mass.size)
+        d.body_of_tet = _iptr(arr(desc.body_of_tet, np.int32))
+        d.body_mass = _dptr(arr(desc.body_mass, np.float64))
     d.n_vertices = desc.N
     d.x_rest = _dptr(x)
     d.n_tets = desc.T
@@ -125,7 +131,8 @@ class Scene:
         Ea = None
         if E is not None:
             Ea = np.ascontiguousarray(np.broadcast_to(np.asarray(E, np.float64), (self.T,)))
-        nu = float(self.desc.nu_fwd if nu_fwd is None else nu_fwd)
+        # nu_fwd None keeps the per-tet nu of the scene (NaN in the C call)
+        nu = float("nan") if nu_fwd is None else float(nu_fwd)
         check(L.pdm_set_material(self.h, _vp(Ea), nu))
 
     def set_stream(self, stream_ptr):

@@ -55,6 +55,12 @@ class SceneDesc:
     mu: float = 1.0
     planes: np.ndarray = None          # (P,6): point, normal
     spheres: np.ndarray = None         # (Q,4): centre, radius
+    # several bodies in one world (one Dfobj per mesh): per-tet body id, each
+    # body's "Mass", and the nu entering mMu per tet (bodies built with their
+    # own Poisson ratio); None = one body / nu_fwd everywhere
+    body_of_tet: np.ndarray = None     # (T,) int32
+    body_mass: np.ndarray = None       # (B,)
+    nu_fwd_tet: np.ndarray = None      # (T,)
     name: str = "scene"
 
     def __post_init__(self):
@@ -67,6 +73,11 @@ class SceneDesc:
         if self.nu_ctor is None:
             self.nu_ctor = np.full(T, 0.4)
         self.nu_ctor = np.ascontiguousarray(np.broadcast_to(np.asarray(self.nu_ctor, dtype=np.float64), (T,)))
+        if self.body_of_tet is not None:
+            self.body_of_tet = np.ascontiguousarray(self.body_of_tet, dtype=np.int32).reshape(T)
+            self.body_mass = np.ascontiguousarray(self.body_mass, dtype=np.float64).ravel()
+        if self.nu_fwd_tet is not None:
+            self.nu_fwd_tet = np.ascontiguousarray(np.broadcast_to(np.asarray(self.nu_fwd_tet, np.float64), (T,)))
         if self.attach_idx is None:
             self.attach_idx = np.zeros(0, dtype=np.int32)
         self.attach_idx = np.ascontiguousarray(self.attach_idx, dtype=np.int32)
@@ -284,3 +295,65 @@ def config5_meshes(seed: int = 5, n_meshes: int = 64, tmin: int = 2000, tmax: in
                              planes=np.array([[0.0, 0.0, 0.0, 0.0, 0.0, 1.0]]), spheres=sph,
                              name="c5_mesh%02d_%dx%dx%d" % (i, nx, ny, nz), **kw))
     return out
+
+
+def pack_scenes(descs: List[SceneDesc], gap: float = 0.4, name: str = "packed") -> SceneDesc:
+    """Several meshes as the bodies of one world (one Dfobj per mesh, as the
+    reference's "Meshes" list loads them, Environment.h:109-210): meshes are
+    translated onto a square grid of cells in the x-y plane, ``gap`` apart;
+    each keeps its own mass, E, nu and muscle fibres (activation groups are
+    renumbered consecutively); obstacles are the union of the meshes'
+    (translated with their mesh). Step parameters come from the first mesh and
+    must agree across meshes."""
+    if not descs:
+        raise ValueError("no meshes to pack")
+    d0 = descs[0]
+    for d in descs[1:]:
+        for k in ("h", "max_iter", "damping", "gravity_on", "contact", "max_iter1", "alpha", "mu", "k_muscle",
+                  "mass_same"):
+            if getattr(d, k) != getattr(d0, k):
+                raise ValueError("packed meshes disagree on %s" % k)
+        if tuple(d.gravity) != tuple(d0.gravity):
+            raise ValueError("packed meshes disagree on gravity")
+    ext = np.array([d.x_rest.max(0) - d.x_rest.min(0) for d in descs])
+    cell = ext[:, :2].max(0) + gap
+    ncol = int(math.ceil(math.sqrt(len(descs))))
+    xs, ts, bodies, E, nuc, nuf, mass = [], [], [], [], [], [], []
+    s0, s1, sg, att, spheres = [], [], [], [], []
+    nv = 0
+    ng = 0
+    for b, d in enumerate(descs):
+        off = np.array([(b % ncol) * cell[0], (b // ncol) * cell[1], 0.0]) - np.array([*d.x_rest.min(0)[:2], 0.0])
+        xs.append(d.x_rest + off)
+        ts.append(d.tets + nv)
+        bodies.append(np.full(d.T, b, dtype=np.int32))
+        E.append(d.E)
+        nuc.append(d.nu_ctor)
+        nuf.append(d.nu_fwd_tet if d.nu_fwd_tet is not None else np.full(d.T, d.nu_fwd))
+        mass.append(d.total_mass)
+        att.append(d.attach_idx + nv)
+        if d.seg_group.size:
+            s0.append(d.seg_start + off)
+            s1.append(d.seg_end + off)
+            sg.append(d.seg_group + ng)
+            ng += d.n_groups
+        for sp in d.spheres:
+            spheres.append(np.concatenate([sp[:3] + off, sp[3:]]))
+        nv += d.N
+    planes = np.unique(np.concatenate([d.planes for d in descs]), axis=0) if any(d.planes.size for d in descs) else None
+    kw = {}
+    if sg:
+        kw = dict(seg_start=np.concatenate(s0), seg_end=np.concatenate(s1), seg_group=np.concatenate(sg), n_groups=ng)
+    return SceneDesc(x_rest=np.concatenate(xs), tets=np.concatenate(ts), total_mass=float(sum(mass)),
+                     mass_same=d0.mass_same, E=np.concatenate(E), nu_fwd=d0.nu_fwd, nu_ctor=np.concatenate(nuc),
+                     attach_idx=np.concatenate(att), attach_w=d0.attach_w, k_muscle=d0.k_muscle, h=d0.h,
+                     max_iter=d0.max_iter, damping=d0.damping, gravity=d0.gravity, gravity_on=d0.gravity_on,
+                     contact=d0.contact, max_iter1=d0.max_iter1, alpha=d0.alpha, mu=d0.mu, planes=planes,
+                     spheres=np.array(spheres) if spheres else None, body_of_tet=np.concatenate(bodies),
+                     body_mass=np.array(mass), nu_fwd_tet=np.concatenate(nuf), name=name, **kw)
+
+
+def config5(seed: int = 5, n_meshes: int = 64, tmin: int = 2000, tmax: int = 60000) -> SceneDesc:
+    """The packed uneven-mesh workload: ``config5_meshes`` as the bodies of
+    one world (block-diagonal factor, one supernodal forest)."""
+    return pack_scenes(config5_meshes(seed, n_meshes, tmin, tmax), name="c5_packed_%d" % n_meshes)

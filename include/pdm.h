@@ -67,6 +67,15 @@ typedef struct pdm_scene_desc {
   const double* planes;
   int n_spheres;             /* static spheres: centre(3), radius */
   const double* spheres;
+  /* Several bodies in one world: one Dfobj per mesh (Environment.h:109-210,
+   * the "Meshes" list), each with its own "Mass" lumped on its own volume
+   * (Dfobj.h:506-525) and its own Poisson ratio in mMu = E/(1+nu)
+   * (CorotateFEMConstraint.h:62-77). The PD loop, its early exit and the
+   * contact phase stay those of the one World (World.h:835-1023). */
+  const double* nu_fwd_tet;  /* T values of nu entering mMu (NULL: nu_fwd for every tet) */
+  int n_bodies;              /* 0: one body of total_mass */
+  const int* body_of_tet;    /* T body ids in [0, n_bodies) */
+  const double* body_mass;   /* n_bodies masses */
 } pdm_scene_desc;
 
 typedef struct pdm_info {
@@ -111,6 +120,7 @@ int pdm_set_stream(pdm_scene s, void* stream);
 
 /* Material update + numeric re-factorisation. E: T values (caller tet order)
  * or NULL to keep; nu_fwd as forward_all's tau[1]. */
+/* nu_fwd = NaN keeps the current per-tet nu (from the scene description). */
 int pdm_set_material(pdm_scene s, const double* E, double nu_fwd);
 
 /* Rollout batch R, arithmetic precision (PDM_FP64 everywhere, or PDM_MIXED:

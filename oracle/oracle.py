@@ -45,6 +45,8 @@ def lib():
         L.orc_get_contact_vertices.argtypes = [C.c_void_p, _ip]
         L.orc_set_material.argtypes = [C.c_void_p, _dp, C.c_double, _dp]
         L.orc_set_attachments.argtypes = [C.c_void_p, C.c_int, _ip, C.c_double]
+        L.orc_set_bodies.argtypes = [C.c_void_p, C.c_int, _ip, _dp, C.c_int]
+        L.orc_set_nu_fwd_tet.argtypes = [C.c_void_p, _dp]
         L.orc_set_muscles.argtypes = [C.c_void_p, C.c_int, _ip, _dp, _dp, C.c_double, _ip, C.c_int]
         L.orc_muscle_segment_select.argtypes = [C.c_void_p, _ip, C.c_int, _dp, _dp, C.c_int, _ip, _dp]
         L.orc_set_sim.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_double, _dp, C.c_int, C.c_int, C.c_int,
@@ -114,6 +116,13 @@ class OracleScene:
         self.h = L.orc_scene_new(self.N, x, self.T, t, int(desc.mass_same), float(desc.total_mass))
         L.orc_set_material(self.h, np.ascontiguousarray(desc.E, dtype=np.float64), float(desc.nu_fwd),
                            np.ascontiguousarray(desc.nu_ctor, dtype=np.float64))
+        if getattr(desc, "nu_fwd_tet", None) is not None:
+            L.orc_set_nu_fwd_tet(self.h, np.ascontiguousarray(desc.nu_fwd_tet, dtype=np.float64))
+        if getattr(desc, "body_of_tet", None) is not None:
+            rc = L.orc_set_bodies(self.h, int(desc.body_mass.size), np.ascontiguousarray(desc.body_of_tet, np.int32),
+                                  np.ascontiguousarray(desc.body_mass, np.float64), int(desc.mass_same))
+            if rc:
+                raise RuntimeError(L.orc_last_error().decode())
         if len(desc.attach_idx):
             L.orc_set_attachments(self.h, len(desc.attach_idx), np.ascontiguousarray(desc.attach_idx, np.int32),
                                   float(desc.attach_w))

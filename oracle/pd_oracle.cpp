@@ -417,7 +417,7 @@ struct Scene {
   std::vector<Tet> tets;
   std::vector<double> E;      // per tet
   std::vector<double> nu_ctor;  // per tet (volume-term weight factor)
-  double nu_fwd = 0.4;          // enters mMu = E/(1+nu)
+  std::vector<double> nu_fwd_t;  // per tet: nu entering mMu = E/(1+nu)
   std::vector<double> wr, wv;   // per tet weights
   std::vector<int> attach_idx;
   std::vector<double> attach_w;
@@ -484,7 +484,7 @@ void update_weights(Scene& s) {
   s.wr.resize(T);
   s.wv.resize(T);
   for (size_t t = 0; t < T; ++t) {
-    double mMu = s.E[t] / (1.0 + s.nu_fwd);  // CorotateFEMConstraint.h:69, pydifem.cc:217
+    double mMu = s.E[t] / (1.0 + s.nu_fwd_t[t]);  // CorotateFEMConstraint.h:69, pydifem.cc:217
     s.wr[t] = mMu * s.tets[t].vol;           // :247
     s.wv[t] = mMu * s.tets[t].vol * s.nu_ctor[t];  // :267
   }
@@ -976,7 +976,7 @@ void contact_backward(Scene& s, const StepTape& tp, std::vector<double>& ybar, s
         pr += Fz[i] * (md[i] - F[i]);
         pv += Fz[i] * (mdv[i] - F[i]);
       }
-      double opn = 1.0 + s.nu_fwd, vol = te.vol;
+      double opn = 1.0 + s.nu_fwd_t[t], vol = te.vol;
       if (gE) gE[t] += pr * vol / opn + pv * vol * s.nu_ctor[t] / opn;
       if (gnu) *gnu += -(pr * s.wr[t] + pv * s.wv[t]) / opn;
       corotate_vjp(F, s.wr[t], s.wv[t], Fz, Fbar);
@@ -1041,7 +1041,7 @@ void step_backward(Scene& s, const StepTape& tp, const double* gx, const double*
         gr += Fz[i] * (md[i] - F1[i]);
         gvv += Fz[i] * (mdv[i] - F1[i]);
       }
-      double opn = 1.0 + s.nu_fwd;
+      double opn = 1.0 + s.nu_fwd_t[t];
       double vol = te.vol;
       // wr = E vol/(1+nu), wv = E vol nu_c/(1+nu)
       if (gE) gE[t] += gr * vol / opn + gvv * vol * s.nu_ctor[t] / opn;
@@ -1154,8 +1154,46 @@ void* orc_scene_new(int N, const double* xrest, int T, const int* tets, int mass
   s->mass = m;
   s->E.assign(T, 2e5);
   s->nu_ctor.assign(T, 0.4);
+  s->nu_fwd_t.assign(T, 0.4);
   s->contact_vtx = surface_vertices(N, s->tets);
   return s;
+}
+
+// Several bodies in one world (one Dfobj per mesh, Environment.h:109-210):
+// each body's "Mass" is lumped on its own tets (Dfobj.h:506-525).
+int orc_set_bodies(void* p, int nb, const int* body_of_tet, const double* body_mass, int mass_same) {
+  Scene* s = (Scene*)p;
+  int N = s->N;
+  size_t T = s->tets.size();
+  std::vector<double> vol_b(nb, 0.0), m(N, 0.0);
+  std::vector<int> vbody(N, -1), nv_b(nb, 0);
+  for (size_t t = 0; t < T; ++t) {
+    int b = body_of_tet[t];
+    if (b < 0 || b >= nb) {
+      g_err = "body index out of range";
+      return 1;
+    }
+    vol_b[b] += s->tets[t].vol;
+    for (int k = 0; k < 4; ++k) vbody[s->tets[t].v[k]] = b;
+  }
+  for (int i = 0; i < N; ++i)
+    if (vbody[i] >= 0) nv_b[vbody[i]]++;
+  for (size_t t = 0; t < T; ++t) {
+    int b = body_of_tet[t];
+    for (int k = 0; k < 4; ++k) m[s->tets[t].v[k]] += 0.25 * s->tets[t].vol * body_mass[b] / vol_b[b];
+  }
+  if (!mass_same)
+    for (int i = 0; i < N; ++i) m[i] = vbody[i] >= 0 ? body_mass[vbody[i]] / nv_b[vbody[i]] : 0.0;
+  s->mass = m;
+  s->factored = false;
+  return 0;
+}
+
+// Per-tet nu entering mMu (bodies built with their own Poisson ratio).
+void orc_set_nu_fwd_tet(void* p, const double* nu_t) {
+  Scene* s = (Scene*)p;
+  s->nu_fwd_t.assign(nu_t, nu_t + s->tets.size());
+  s->factored = false;
 }
 
 void orc_scene_free(void* p) { delete (Scene*)p; }
@@ -1187,7 +1225,7 @@ void orc_set_material(void* p, const double* E_tet, double nu_fwd, const double*
   Scene* s = (Scene*)p;
   size_t T = s->tets.size();
   s->E.assign(E_tet, E_tet + T);
-  s->nu_fwd = nu_fwd;
+  s->nu_fwd_t.assign(T, nu_fwd);
   s->nu_ctor.assign(nu_ctor_tet, nu_ctor_tet + T);
   s->factored = false;
 }

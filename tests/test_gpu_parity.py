@@ -118,11 +118,13 @@ def test_forward_mixed_precision(eng, oracle_mod):
         assert rel(xT[r], xo) < 1e-5
 
 
-def _grad_case(eng, oracle_mod, d, precision, steps, R, tol_x, tol_g, ckpt=4):
+def _grad_case(eng, oracle_mod, d, precision, steps, R, tol_x, tol_g, ckpt=4, x_jitter=0.0):
     s = eng.Scene(d, R=R, precision=precision, flags=0, checkpoint_every=ckpt)
     o = oracle_mod.OracleScene(d)
     rng = np.random.default_rng(4)
     x0 = np.tile(d.x_rest.ravel(), (R, 1))
+    if x_jitter:
+        x0 = x0 + x_jitter * rng.standard_normal(x0.shape)
     v0 = 0.01 * rng.standard_normal((R, 3 * d.N))
     act = rng.uniform(0, 1, (R, steps, d.n_groups)) if d.n_groups else None
     xT, vT = s.forward_host(x0, v0, steps, act=act, record=True)
@@ -240,3 +242,37 @@ def test_graph_replay_matches_eager(eng, which):
     gb = graph.backward_host(gx, gv)
     for k in ("x0", "v0", "E"):
         assert np.array_equal(ga[k], gb[k]), k
+
+
+def packed_scene():
+    return scene.config5(seed=9, n_meshes=4, tmin=300, tmax=1500)
+
+
+def test_packed_forward_matches_oracle(eng, oracle_mod):
+    """Four uneven meshes as the bodies of one world (per-body mass and nu,
+    block-diagonal factor), with muscles, ground and spheres: FP64 bar."""
+    d = packed_scene()
+    R, steps = 2, 30
+    s = eng.Scene(d, R=R, flags=0)
+    o = oracle_mod.OracleScene(d)
+    rng = np.random.default_rng(21)
+    x0 = np.tile(d.x_rest.ravel(), (R, 1))
+    v0 = 0.05 * rng.standard_normal((R, 3 * d.N))
+    act = rng.uniform(0, 1, (R, steps, d.n_groups))
+    xT, vT = s.forward_host(x0, v0, steps, act=act)
+    passes, jit = s.contact_stats()
+    for r in range(R):
+        xo, vo, io = o.rollout(x0[r], v0[r], steps, act=act[r])
+        assert np.array_equal(s.iterations()[r], io[:, 0])
+        assert np.array_equal(passes[r], io[:, 2]) and np.array_equal(jit[r], io[:, 1])
+        assert rel(xT[r], xo) < 1e-9 and rel(vT[r], vo) < 1e-9
+
+
+def test_packed_gradient_matches_oracle(eng, oracle_mod):
+    """FP64 bar on the packed world. The start state is jittered by 2 mm so
+    that no tet sits near rest: there F has nearly equal singular values, the
+    singular vectors of two different SVD algorithms agree only to
+    eps / (singular-value gap), and the SVD derivative carries that into the
+    gradient (about 2e-9 relative after 6 steps from exact rest)."""
+    d = packed_scene()
+    _grad_case(eng, oracle_mod, d, eng.PDM_FP64, steps=6, R=2, tol_x=1e-9, tol_g=1e-9, ckpt=4, x_jitter=2e-3)

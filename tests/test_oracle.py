@@ -362,3 +362,45 @@ def test_contact_gradient_vs_central_fd_plane(oracle_mod):
         fd = (loss(dv=dv) - loss(dv=-dv)) / 2
         assert abs(fd - g["v0"] @ dv) < 1e-5 * abs(fd)
         s.set_material(np.full(s.T, 8e4), 0.4)
+
+
+def _packed_pair(seed=7, contact=False, max_iter=3):
+    ms = scene.config5_meshes(seed=seed, n_meshes=3, tmin=200, tmax=800)
+    for m in ms:
+        m.contact = contact
+        m.max_iter = max_iter
+    return ms, scene.pack_scenes(ms)
+
+
+def test_packed_body_masses_equal_single_body_masses(oracle_mod):
+    """Each body of a packed scene keeps the lumped masses it has alone
+    (its own "Mass" over its own volume, Dfobj.h:506-525)."""
+    ms, p = _packed_pair()
+    mp = oracle_mod.OracleScene(p, factor=False).masses()
+    off = 0
+    for m, mass in zip(ms, p.body_mass):
+        mm = oracle_mod.OracleScene(m, factor=False).masses()
+        assert np.allclose(mp[off:off + m.N], mm, rtol=1e-14, atol=0)
+        assert abs(mm.sum() - mass) < 1e-12 * mass
+        off += m.N
+
+
+def test_packed_rollout_equals_separate_rollouts(oracle_mod):
+    """Without contact, and with every body running all iterations, the bodies
+    of one world evolve exactly as they do alone (block-diagonal system,
+    per-body nu in mMu): translation of the packed layout aside, positions
+    agree to rounding."""
+    ms, p = _packed_pair()
+    rng = np.random.default_rng(0)
+    v0 = 0.5 * rng.standard_normal(3 * p.N)
+    xT, vT, its = oracle_mod.OracleScene(p).rollout(p.x_rest.ravel().copy(), v0, 5)
+    assert np.all(its[:, 0] == 3)
+    off = 0
+    for m in ms:
+        n3 = 3 * m.N
+        x, v, i2 = oracle_mod.OracleScene(m).rollout(m.x_rest.ravel().copy(), v0[off:off + n3], 5)
+        assert np.all(i2[:, 0] == 3)
+        shift = p.x_rest.ravel()[off:off + n3] - m.x_rest.ravel()
+        assert np.abs(x - (xT[off:off + n3] - shift)).max() < 1e-12
+        assert np.abs(v - vT[off:off + n3]).max() < 1e-10
+        off += n3
