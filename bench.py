@@ -2,10 +2,11 @@
 vs the plain FP64 GPU path, on one MI355X.
 
 Default workload (the batched-rollout configuration): 40x12x12 box
-(34,560 tets, 6,929 vertices), 64 straight fibres, R = 256 rollouts sharing one
-Cholesky factor, seeded sinusoidal activations, 100 steps x 10 iterations,
-forward + adjoint with checkpoints every 20 steps, FP32 local step with FP64
-factor/solves/adjoint accumulation.
+(34,560 tets, 6,929 vertices), 64 straight fibres, ground plane with friction,
+R = 256 rollouts sharing one Cholesky factor, seeded sinusoidal activations,
+100 steps x 10 iterations, forward + adjoint with checkpoints every 50 steps
+(the forward pass records the last segment's tape, the adjoint recomputes the
+first), FP32 local step with FP64 factor/solves/adjoint accumulation.
 
 One bench "step" = one full forward + adjoint pass over the batch (inputs
 already resident in HBM). value = sum over rollouts of T * executed PD
@@ -88,7 +89,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--rollouts", type=int, default=256)
     ap.add_argument("--sim-steps", type=int, default=100)
-    ap.add_argument("--ckpt", type=int, default=20)
+    ap.add_argument("--ckpt", type=int, default=50,
+                    help="checkpoint interval (the forward pass tapes the last segment; about 2 GB of tape per step)")
     ap.add_argument("--plain-steps", type=int, default=0,
                     help="sim steps of the plain FP64 comparator (0: same as --sim-steps)")
     ap.add_argument("--contact", type=int, default=1)

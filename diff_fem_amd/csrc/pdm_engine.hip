@@ -1393,6 +1393,7 @@ struct pdm_scene_s {
   DevBuf<int> ractive;
   double t_fwd = 0, t_bwd = 0, t_factor = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int fwd_taped_seg = -1;  // checkpoint segment whose tapes the forward pass wrote
   // PDM_USE_GRAPH: captured step block and its device-side controls
   hipGraphExec_t gexec = nullptr;
   const double* g_act = nullptr;
@@ -2026,6 +2027,8 @@ int run_step(pdm_scene s, const double* act, int act_steps, int step, double* ta
   return 0;
 }
 
+int ensure_tapes(pdm_scene s, int steps);
+
 int forward_impl(pdm_scene s, int steps, const double* x0, const double* v0, const double* act, double* xT, double* vT,
                  int record) {
   if (steps < 0) {
@@ -2058,16 +2061,33 @@ int forward_impl(pdm_scene s, int steps, const double* x0, const double* v0, con
   if (it_all.alloc((size_t)s->R * std::max(steps, 1))) return 1;
   if (s->ncv > 0 && (np_all.alloc((size_t)s->R * std::max(steps, 1)) || ci_all.alloc((size_t)s->R * std::max(steps, 1))))
     return 1;
+  // The last checkpoint segment is recorded as it runs, so that the reverse
+  // sweep starts without recomputing it.
+  s->fwd_taped_seg = -1;
+  const int last_s0 = record && steps > 0 ? ((steps - 1) / s->ckpt) * s->ckpt : steps;
+  if (record && steps > 0 && ensure_tapes(s, steps)) return 1;
+  const int K1 = S.max_iter + 1;
   for (int n = 0; n < steps; ++n) {
     if (record && n % s->ckpt == 0) {
       size_t k = n / s->ckpt;
       HIPX(hipMemcpyAsync(s->ckX.p + k * nc, s->X.p, nc * 8, hipMemcpyDeviceToDevice, s->stream));
       HIPX(hipMemcpyAsync(s->ckV.p + k * nc, s->V.p, nc * 8, hipMemcpyDeviceToDevice, s->stream));
     }
-    if (run_step(s, act, steps, n, nullptr, nullptr, nullptr, it_all.p + (size_t)n * s->R,
-                 s->ncv ? np_all.p + (size_t)n * s->R : nullptr, s->ncv ? ci_all.p + (size_t)n * s->R : nullptr))
+    int* np = s->ncv ? np_all.p + (size_t)n * s->R : nullptr;
+    int* ci = s->ncv ? ci_all.p + (size_t)n * s->R : nullptr;
+    if (n >= last_s0) {
+      int q = n - last_s0;
+      if (run_step(s, act, steps, n, s->tY.p + (size_t)q * K1 * nc, s->tXn.p + (size_t)q * nc,
+                   s->tVs.p + (size_t)q * nc, s->tIters.p + (size_t)q * s->R, np, ci,
+                   s->ncv > 0 ? s->ctape[q].get() : nullptr))
+        return 1;
+      HIPX(hipMemcpyAsync(it_all.p + (size_t)n * s->R, s->tIters.p + (size_t)q * s->R, s->R * sizeof(int),
+                          hipMemcpyDeviceToDevice, s->stream));
+    } else if (run_step(s, act, steps, n, nullptr, nullptr, nullptr, it_all.p + (size_t)n * s->R, np, ci)) {
       return 1;
+    }
   }
+  if (record && steps > 0) s->fwd_taped_seg = (steps - 1) / s->ckpt;
   hipLaunchKernelGGL(k_engine_to_caller, dim3(cdiv(nc, kThreads)), dim3(kThreads), 0, s->stream, s->X.p, xT, s->perm.p,
                      N, s->R);
   hipLaunchKernelGGL(k_engine_to_caller, dim3(cdiv(nc, kThreads)), dim3(kThreads), 0, s->stream, s->V.p, vT, s->perm.p,
@@ -2184,6 +2204,21 @@ int contact_reverse(pdm_scene s, ContactTape& tp, const double* Xn, const double
   return 0;
 }
 
+// Tape buffers for one checkpoint segment: every iterate of every step, the
+// step-start states, iteration counts and the contact-phase tapes.
+int ensure_tapes(pdm_scene s, int steps) {
+  HostScene& S = s->S;
+  size_t nc = s->NC();
+  int K1 = S.max_iter + 1;
+  int seg = std::min(s->ckpt, std::max(steps, 1));
+  if (s->tY.alloc((size_t)seg * K1 * nc) || s->tXn.alloc((size_t)seg * nc) || s->tVs.alloc((size_t)seg * nc) ||
+      s->tIters.alloc((size_t)seg * s->R))
+    return 1;
+  if (s->ncv > 0)
+    while ((int)s->ctape.size() < seg) s->ctape.emplace_back(new ContactTape());
+  return 0;
+}
+
 int backward_impl(pdm_scene s, const double* gxT, const double* gvT, double* gx0, double* gv0, double* gE, double* gnu,
                   double* gact) {
   if (!s->have_tape) {
@@ -2195,9 +2230,7 @@ int backward_impl(pdm_scene s, const double* gxT, const double* gvT, double* gx0
   size_t nc = s->NC();
   int K1 = S.max_iter + 1;
   int seg = std::min(s->ckpt, std::max(steps, 1));
-  if (s->tY.alloc((size_t)seg * K1 * nc) || s->tXn.alloc((size_t)seg * nc) || s->tVs.alloc((size_t)seg * nc) ||
-      s->tIters.alloc((size_t)seg * R))
-    return 1;
+  if (ensure_tapes(s, steps)) return 1;
   if (s->gX.alloc(nc) || s->gV.alloc(nc) || s->Yb.alloc(nc) || s->Xb.alloc(nc) || s->Zs.alloc(nc) ||
       s->Zb.alloc(nc) || s->LZ.alloc(nc) || s->gEpart.alloc((size_t)S.T * R) || s->ractive.alloc(R))
     return 1;
@@ -2205,7 +2238,6 @@ int backward_impl(pdm_scene s, const double* gxT, const double* gvT, double* gx0
   if (s->ncv > 0) {
     if (s->cXB.alloc(nc) || s->cPB.alloc(nc) || s->cOB.alloc(nc) || s->cGf.alloc(nc) || s->cCD.alloc((size_t)N * R))
       return 1;
-    while ((int)s->ctape.size() < seg) s->ctape.emplace_back(new ContactTape());
   }
   size_t nact = (size_t)R * steps * std::max(S.n_groups, 1);
   if (s->gact.alloc(nact)) return 1;
@@ -2223,9 +2255,14 @@ int backward_impl(pdm_scene s, const double* gxT, const double* gvT, double* gx0
   for (int sg = nseg - 1; sg >= 0 && steps > 0; --sg) {
     int s0 = sg * s->ckpt, s1 = std::min(steps, s0 + s->ckpt);
     // recompute the segment from its checkpoint, recording every iterate
-    HIPX(hipMemcpyAsync(s->X.p, s->ckX.p + (size_t)sg * nc, nc * 8, hipMemcpyDeviceToDevice, s->stream));
-    HIPX(hipMemcpyAsync(s->V.p, s->ckV.p + (size_t)sg * nc, nc * 8, hipMemcpyDeviceToDevice, s->stream));
-    for (int n = s0; n < s1; ++n) {
+    // (the last segment was recorded by the forward pass itself)
+    bool taped = sg == s->fwd_taped_seg;
+    s->fwd_taped_seg = -1;
+    if (!taped) {
+      HIPX(hipMemcpyAsync(s->X.p, s->ckX.p + (size_t)sg * nc, nc * 8, hipMemcpyDeviceToDevice, s->stream));
+      HIPX(hipMemcpyAsync(s->V.p, s->ckV.p + (size_t)sg * nc, nc * 8, hipMemcpyDeviceToDevice, s->stream));
+    }
+    for (int n = s0; n < s1 && !taped; ++n) {
       int q = n - s0;
       if (run_step(s, act, steps, n, s->tY.p + (size_t)q * K1 * nc, s->tXn.p + (size_t)q * nc,
                    s->tVs.p + (size_t)q * nc, s->tIters.p + (size_t)q * R, nullptr, nullptr,
