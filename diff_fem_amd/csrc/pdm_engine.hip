@@ -109,13 +109,41 @@ __global__ void k_step_prep(const double* __restrict__ X, const double* __restri
   Cc[idx] = mass[v] * vs / h - Lx[idx] + attw[v] * xrest[3 * v + c];
 }
 
+// Active-rollout list: alist[0..*acount) = the rollouts r with active[r] != 0,
+// ascending (one block; a block-wide scan per chunk of 1024 flags).
+__global__ __launch_bounds__(1024) void k_compact(const int* __restrict__ active, int R, int* __restrict__ alist,
+                                                  int* __restrict__ acount) {
+  __shared__ int sc[1024];
+  __shared__ int base;
+  if (threadIdx.x == 0) base = 0;
+  __syncthreads();
+  for (int c0 = 0; c0 < R; c0 += 1024) {
+    int i = c0 + threadIdx.x;
+    int f = (i < R && active[i] != 0) ? 1 : 0;
+    sc[threadIdx.x] = f;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+      int v = threadIdx.x >= off ? sc[threadIdx.x - off] : 0;
+      __syncthreads();
+      sc[threadIdx.x] += v;
+      __syncthreads();
+    }
+    if (f) alist[base + sc[threadIdx.x] - 1] = i;
+    __syncthreads();
+    if (threadIdx.x == 1023) base += sc[1023];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *acount = base;
+}
+
 // UNI: R is a multiple of 64, so the 64 lanes of a wave share one tet; its
 // index is then made wave-uniform and the per-tet data come in by scalar loads.
 template <typename T, bool UNI>
 __global__ __launch_bounds__(256) void k_local_tet(const double* __restrict__ X, const int* __restrict__ tv,
                                                    const double* __restrict__ tG, const double* __restrict__ wr,
                                                    const double* __restrict__ wv, const int* __restrict__ tslot,
-                                                   const int* __restrict__ active, T* __restrict__ corner, int Tn,
+                                                   const int* __restrict__ alist, const int* __restrict__ acount,
+                                                   T* __restrict__ corner, int Tn,
                                                    int R, const int* __restrict__ mptr, const double* __restrict__ mg,
                                                    const double* __restrict__ mw, const int* __restrict__ mgroup,
                                                    int Mn, const double* __restrict__ act, int act_stride_r,
@@ -123,9 +151,10 @@ __global__ __launch_bounds__(256) void k_local_tet(const double* __restrict__ X,
   long long idx = (long long)xcd_block() * blockDim.x + threadIdx.x;
   if (idx >= (long long)Tn * R) return;
   int t = (int)(idx / R);
-  int r = (int)(idx % R);
+  const int j = (int)(idx % R);
   if (UNI) t = __builtin_amdgcn_readfirstlane(t);
-  if (!active[r]) return;
+  if (j >= *acount) return;
+  const int r = alist[j];
   if (step_dev) act_off = *step_dev * ngroups;
   int C = 3 * R;
   int v0 = tv[t], v1 = tv[Tn + t], v2 = tv[2 * Tn + t], v3 = tv[3 * Tn + t];
@@ -252,6 +281,46 @@ __global__ __launch_bounds__(256) void k_gather4(const double* __restrict__ Cc, 
 #pragma unroll
     for (int k = 0; k < 4; ++k)
       if (m[k]) o[k] = MODE == 2 ? o[k] + r[k] : r[k];
+  }
+}
+
+// Segmented gather over the listed rollouts only: thread (vertex, j) sums the
+// three columns of rollout alist[j] over the vertex's slot range (two partial
+// sums per column, fixed order). MODE 0: B = Cc + sum. MODE 1: B = sum.
+// MODE 2: B += sum. Columns of unlisted rollouts are left as they are.
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void k_gather_list(const double* __restrict__ Cc, const T* __restrict__ corner,
+                                                     const int* __restrict__ vptr, const int* __restrict__ alist,
+                                                     const int* __restrict__ acount, double* __restrict__ B, int N,
+                                                     int R, const int* __restrict__ any = nullptr) {
+  if (any && *any == 0) return;
+  long long idx = (long long)xcd_block() * blockDim.x + threadIdx.x;
+  if (idx >= (long long)N * R) return;
+  const int v = (int)(idx / R);
+  const int j = (int)(idx - (long long)v * R);
+  if (j >= *acount) return;
+  const int r = alist[j];
+  const long long C = 3LL * R;
+  double a0 = 0, a1 = 0, a2 = 0, b0 = 0, b1 = 0, b2 = 0;
+  int q = vptr[v], qe = vptr[v + 1];
+  const T* p = corner + (long long)q * C + 3 * r;
+  for (; q + 2 <= qe; q += 2, p += 2 * C) {
+    T x0 = p[0], x1 = p[1], x2 = p[2], y0 = p[C], y1 = p[C + 1], y2 = p[C + 2];
+    a0 += (double)x0; a1 += (double)x1; a2 += (double)x2;
+    b0 += (double)y0; b1 += (double)y1; b2 += (double)y2;
+  }
+  if (q < qe) {
+    a0 += (double)p[0]; a1 += (double)p[1]; a2 += (double)p[2];
+  }
+  double* o = B + (long long)v * C + 3 * r;
+  double s0 = a0 + b0, s1 = a1 + b1, s2 = a2 + b2;
+  if (MODE == 0) {
+    const double* c = Cc + (long long)v * C + 3 * r;
+    o[0] = c[0] + s0; o[1] = c[1] + s1; o[2] = c[2] + s2;
+  } else if (MODE == 1) {
+    o[0] = s0; o[1] = s1; o[2] = s2;
+  } else {
+    o[0] += s0; o[1] += s1; o[2] += s2;
   }
 }
 
@@ -701,8 +770,8 @@ __global__ __launch_bounds__(256) void k_vjp_tet(const double* __restrict__ Yk, 
                                                  const double* __restrict__ tG, const double* __restrict__ wr,
                                                  const double* __restrict__ wv, const double* __restrict__ dEr,
                                                  const double* __restrict__ dEv, const int* __restrict__ tslot,
-                                                 const int* __restrict__ active, T* __restrict__ corner,
-                                                 double* __restrict__ gEpart, int Tn, int R,
+                                                 const int* __restrict__ alist, const int* __restrict__ acount,
+                                                 T* __restrict__ corner, double* __restrict__ gEpart, int Tn, int R,
                                                  const double* __restrict__ Xn, const double* __restrict__ CD,
                                                  const int* __restrict__ mptr, const double* __restrict__ mg,
                                                  const double* __restrict__ mw, const int* __restrict__ mgroup, int Mn,
@@ -711,9 +780,10 @@ __global__ __launch_bounds__(256) void k_vjp_tet(const double* __restrict__ Yk, 
   long long idx = (long long)xcd_block() * blockDim.x + threadIdx.x;
   if (idx >= (long long)Tn * R) return;
   int t = (int)(idx / R);
-  int r = (int)(idx % R);
+  const int j = (int)(idx % R);
   if (UNI) t = __builtin_amdgcn_readfirstlane(t);
-  if (!active[r]) return;
+  if (j >= *acount) return;
+  const int r = alist[j];
   int C = 3 * R;
   int v[4] = {tv[t], tv[Tn + t], tv[2 * Tn + t], tv[3 * Tn + t]};
   T G[12];
@@ -773,7 +843,7 @@ __global__ __launch_bounds__(256) void k_vjp_tet(const double* __restrict__ Yk, 
     gr += extra;
     gv += extra;
   }
-  gEpart[idx] += gr * dEr[t] + gv * dEv[t];
+  gEpart[(long long)t * R + r] += gr * dEr[t] + gv * dEv[t];
   T f[12];
 #pragma unroll
   for (int k = 0; k < 4; ++k)
@@ -1394,6 +1464,7 @@ struct pdm_scene_s {
   double t_fwd = 0, t_bwd = 0, t_factor = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int fwd_taped_seg = -1;  // checkpoint segment whose tapes the forward pass wrote
+  DevBuf<int> alist, acount;  // rollouts that still iterate (k_compact)
   // PDM_USE_GRAPH: captured step block and its device-side controls
   hipGraphExec_t gexec = nullptr;
   const double* g_act = nullptr;
@@ -1651,6 +1722,7 @@ int ensure_batch_buffers(pdm_scene s) {
   if (s->corner.alloc((size_t)s->Cn.nslot * s->C * esz)) return 1;
   if (s->part.alloc((size_t)cdiv(N, kChunk) * s->C)) return 1;
   if (s->err.alloc(s->R) || s->active.alloc(s->R) || s->iters_dev.alloc(s->R)) return 1;
+  if (s->alist.alloc(s->R) || s->acount.alloc(1)) return 1;
   if (s->ncv > 0) {
     size_t ni = (size_t)s->ncv * s->nobs * s->R;
     if (s->Ori.alloc(nc) || s->Prev.alloc(nc) || s->imp_stamp.alloc(ni) || s->imp_n.alloc(3 * ni) ||
@@ -1830,16 +1902,14 @@ void launch_local(pdm_scene s, const double* Xsrc, const double* act, int act_st
   T* corner = (T*)s->corner.p;
   long long nt = (long long)S.T * s->R;
   const int G = S.n_groups;
+  // element work only for the rollouts that still iterate
+  hipLaunchKernelGGL(k_compact, dim3(1), dim3(1024), 0, s->stream, mask, s->R, s->alist.p, s->acount.p);
   auto kern = s->R % 64 == 0 ? k_local_tet<T, true> : k_local_tet<T, false>;
   hipLaunchKernelGGL(kern, dim3(cdiv(nt, kThreads)), dim3(kThreads), 0, s->stream, Xsrc, s->tv.p, s->tG.p, s->wr.p,
-                     s->wv.p, s->tslot.p, mask, corner, S.T, s->R, s->mptr.p, s->mg.p, s->mw.p, s->mgroup.p, S.M, act,
-                     act_steps * G, step * G, step_dev, G);
-  if (s->C % 4 == 0)
-    hipLaunchKernelGGL((k_gather4<T, 0>), dim3(cdiv(s->NC() / 4, kThreads)), dim3(kThreads), 0, s->stream, s->Cc.p,
-                       corner, s->vptr.p, nullptr, s->B.p, S.N, s->C, any);
-  else
-    hipLaunchKernelGGL(k_gather<T>, dim3(cdiv(s->NC(), kThreads)), dim3(kThreads), 0, s->stream, s->Cc.p, corner,
-                       s->vptr.p, s->B.p, S.N, s->C);
+                     s->wv.p, s->tslot.p, s->alist.p, s->acount.p, corner, S.T, s->R, s->mptr.p, s->mg.p, s->mw.p,
+                     s->mgroup.p, S.M, act, act_steps * G, step * G, step_dev, G);
+  hipLaunchKernelGGL((k_gather_list<T, 0>), dim3(cdiv((long long)S.N * s->R, kThreads)), dim3(kThreads), 0, s->stream,
+                     s->Cc.p, corner, s->vptr.p, s->alist.p, s->acount.p, s->B.p, S.N, s->R, any);
 }
 
 
@@ -2130,27 +2200,19 @@ int vjp_iteration(pdm_scene s, const double* Yk, const double* Yk1, const double
   HostScene& S = s->S;
   T* corner = (T*)s->corner.p;
   long long nt = (long long)S.T * s->R;
-  for (int uni = 0; uni < 2; ++uni) {
-    if ((s->R % 64 == 0) != (uni == 1)) continue;
-    auto kern = uni ? k_vjp_tet<T, true> : k_vjp_tet<T, false>;
-    hipLaunchKernelGGL(kern, dim3(cdiv(nt, kThreads)), dim3(kThreads), 0, s->stream, Yk, Yk1, s->Zb.p, s->tv.p,
-                       s->tG.p, s->wr.p, s->wv.p, s->dEr.p, s->dEv.p, s->tslot.p, s->ractive.p, corner, s->gEpart.p, S.T,
-                       s->R, CDp ? Xn : nullptr, CDp, s->mptr.p, s->mg.p, s->mw.p, s->mgroup.p, S.M, act,
-                       act_steps * S.n_groups, step * S.n_groups, S.M > 0 ? s->gMpart.p : nullptr);
-  }
-  if (s->C % 4 == 0) {
-    if (out_add)
-      hipLaunchKernelGGL((k_gather4<T, 2>), dim3(cdiv(s->NC() / 4, kThreads)), dim3(kThreads), 0, s->stream, nullptr,
-                         corner, s->vptr.p, s->ractive.p, out_add, S.N, s->C);
-    else
-      hipLaunchKernelGGL((k_gather4<T, 1>), dim3(cdiv(s->NC() / 4, kThreads)), dim3(kThreads), 0, s->stream, nullptr,
-                         corner, s->vptr.p, s->ractive.p, s->Yb.p, S.N, s->C);
-  } else if (out_add)
-    hipLaunchKernelGGL(k_gather_masked_add<T>, dim3(cdiv(s->NC(), kThreads)), dim3(kThreads), 0, s->stream, corner,
-                       s->vptr.p, s->ractive.p, out_add, S.N, s->C);
+  hipLaunchKernelGGL(k_compact, dim3(1), dim3(1024), 0, s->stream, s->ractive.p, s->R, s->alist.p, s->acount.p);
+  auto kern = s->R % 64 == 0 ? k_vjp_tet<T, true> : k_vjp_tet<T, false>;
+  hipLaunchKernelGGL(kern, dim3(cdiv(nt, kThreads)), dim3(kThreads), 0, s->stream, Yk, Yk1, s->Zb.p, s->tv.p, s->tG.p,
+                     s->wr.p, s->wv.p, s->dEr.p, s->dEv.p, s->tslot.p, s->alist.p, s->acount.p, corner, s->gEpart.p,
+                     S.T, s->R, CDp ? Xn : nullptr, CDp, s->mptr.p, s->mg.p, s->mw.p, s->mgroup.p, S.M, act,
+                     act_steps * S.n_groups, step * S.n_groups, S.M > 0 ? s->gMpart.p : nullptr);
+  dim3 gg(cdiv((long long)S.N * s->R, kThreads));
+  if (out_add)
+    hipLaunchKernelGGL((k_gather_list<T, 2>), gg, dim3(kThreads), 0, s->stream, nullptr, corner, s->vptr.p,
+                       s->alist.p, s->acount.p, out_add, S.N, s->R, nullptr);
   else
-    hipLaunchKernelGGL(k_gather_masked<T>, dim3(cdiv(s->NC(), kThreads)), dim3(kThreads), 0, s->stream, corner,
-                       s->vptr.p, s->ractive.p, s->Yb.p, S.N, s->C);
+    hipLaunchKernelGGL((k_gather_list<T, 1>), gg, dim3(kThreads), 0, s->stream, nullptr, corner, s->vptr.p,
+                       s->alist.p, s->acount.p, s->Yb.p, S.N, s->R, nullptr);
   return 0;
 }
 
