@@ -789,59 +789,73 @@ __global__ __launch_bounds__(256) void k_vjp_tet(const double* __restrict__ Yk, 
   T G[12];
 #pragma unroll
   for (int q = 0; q < 12; ++q) G[q] = (T)tG[(long long)q * Tn + t];
-  double Gd[12];
-#pragma unroll
-  for (int q = 0; q < 12; ++q) Gd[q] = tG[(long long)q * Tn + t];
-  T ds[9];
+  // FP64 quantities of the parameter gradient first (short live ranges):
+  // Fz = A z, F1 = A y_{k+1}; the FP32 path keeps only Zt = (T)Fz and the
+  // scalars sum Fz.F1 and the contact term across the SVD.
+  T F[9], Zt[9];
+  double fzf1 = 0.0, extra = 0.0;
   double F1[9], Fz[9];
   {
+    T ds[9];
     const double* b = Yk + (long long)v[0] * C + 3 * r;
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
       for (int k = 1; k < 4; ++k) ds[c * 3 + (k - 1)] = (T)(Yk[(long long)v[k] * C + 3 * r + c] - b[c]);
-  }
 #pragma unroll
-  for (int c = 0; c < 3; ++c)
+    for (int c = 0; c < 3; ++c)
 #pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      double a = 0, z = 0;
+      for (int jj = 0; jj < 3; ++jj)
+        F[c * 3 + jj] = ds[c * 3 + 0] * G[3 + jj] + ds[c * 3 + 1] * G[6 + jj] + ds[c * 3 + 2] * G[9 + jj];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int jj = 0; jj < 3; ++jj) {
+        double a = 0, z = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          double g = tG[(long long)(3 * k + jj) * Tn + t];
+          a += Yk1[(long long)v[k] * C + 3 * r + c] * g;
+          z += Z[(long long)v[k] * C + 3 * r + c] * g;
+        }
+        F1[c * 3 + jj] = a;
+        Fz[c * 3 + jj] = z;
+        Zt[c * 3 + jj] = (T)z;
+      }
+    if (sizeof(T) == 4) {
+#pragma unroll
+      for (int i = 0; i < 9; ++i) fzf1 += Fz[i] * F1[i];
+    }
+    if (CD) {
+      // contact iteration: C and D of the Jacobi update depend on the weights
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        a += Yk1[(long long)v[k] * C + 3 * r + c] * Gd[3 * k + j];
-        z += Z[(long long)v[k] * C + 3 * r + c] * Gd[3 * k + j];
+        double g0 = tG[(long long)(3 * k) * Tn + t], g1 = tG[(long long)(3 * k + 1) * Tn + t],
+               g2 = tG[(long long)(3 * k + 2) * Tn + t];
+        long long bb = (long long)v[k] * C + 3 * r;
+        double zd = Z[bb] * (Yk[bb] - Xn[bb]) + Z[bb + 1] * (Yk[bb + 1] - Xn[bb + 1]) + Z[bb + 2] * (Yk[bb + 2] - Xn[bb + 2]);
+        extra += (g0 * g0 + g1 * g1 + g2 * g2) * (zd + CD[(long long)v[k] * R + r]);
       }
-      F1[c * 3 + j] = a;
-      Fz[c * 3 + j] = z;
     }
-  T F[9], Zt[9];
-#pragma unroll
-  for (int c = 0; c < 3; ++c)
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      F[c * 3 + j] = ds[c * 3 + 0] * G[3 + j] + ds[c * 3 + 1] * G[6 + j] + ds[c * 3 + 2] * G[9 + j];
-      Zt[c * 3 + j] = (T)Fz[c * 3 + j];
-    }
-  T Fb[9], md[9], mdv[9];
-  corotate_vjp<T>(F, (T)wr[t], (T)wv[t], Zt, Fb, md, mdv);
-  double gr = 0, gv = 0;
-#pragma unroll
-  for (int i = 0; i < 9; ++i) {
-    gr += Fz[i] * ((double)md[i] - F1[i]);
-    gv += Fz[i] * ((double)mdv[i] - F1[i]);
   }
-  if (CD) {
-    // contact iteration: C and D of the Jacobi update depend on the weights
-    double extra = 0.0;
+  T Fb[9];
+  double gr, gv;
+  if (sizeof(T) == 8) {
+    // FP64 path: the exact formula sum Fz (md - F1)
+    T md[9], mdv[9];
+    corotate_vjp<T>(F, (T)wr[t], (T)wv[t], Zt, Fb, md, mdv);
+    gr = extra;
+    gv = extra;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      double g2 = Gd[3 * k] * Gd[3 * k] + Gd[3 * k + 1] * Gd[3 * k + 1] + Gd[3 * k + 2] * Gd[3 * k + 2];
-      long long b = (long long)v[k] * C + 3 * r;
-      double zd = Z[b] * (Yk[b] - Xn[b]) + Z[b + 1] * (Yk[b + 1] - Xn[b + 1]) + Z[b + 2] * (Yk[b + 2] - Xn[b + 2]);
-      extra += g2 * (zd + CD[(long long)v[k] * R + r]);
+    for (int i = 0; i < 9; ++i) {
+      gr += Fz[i] * ((double)md[i] - F1[i]);
+      gv += Fz[i] * ((double)mdv[i] - F1[i]);
     }
-    gr += extra;
-    gv += extra;
+  } else {
+    T zmd, zmdv;
+    corotate_vjp<T>(F, (T)wr[t], (T)wv[t], Zt, Fb, nullptr, nullptr, &zmd, &zmdv);
+    gr = (double)zmd - fzf1 + extra;
+    gv = (double)zmdv - fzf1 + extra;
   }
   gEpart[(long long)t * R + r] += gr * dEr[t] + gv * dEv[t];
   T f[12];

@@ -304,23 +304,37 @@ __device__ __forceinline__ void corotate_P(const T* F, T wr, T wv, T* P) {
 }
 
 // Adjoint of the weighted projection: Fbar = (dP/dF)^T [Z]; also returns the
-// projections md, mdv at F (needed for the parameter gradient).
+// projections md, mdv at F (needed for the parameter gradient), or, when md is
+// null, only their inner products with Z in zmd, zmdv.
 template <typename T>
-__device__ __forceinline__ void corotate_vjp(const T* F, T wr, T wv, const T* Z, T* Fbar, T* md, T* mdv) {
+__device__ __forceinline__ void corotate_vjp(const T* F, T wr, T wv, const T* Z, T* Fbar, T* md, T* mdv,
+                                             T* zmd = nullptr, T* zmdv = nullptr) {
   T U[9], S[3], V[9], D[3], J[9];
   svd3<T>(F, U, S, V);
   volproj_jac<T>(S, D, J);
   bool flip = det3<T>(F) < 0;
   T phi[3] = {S[0] + D[0], S[1] + D[1], S[2] + D[2]};
+  T dr = 0, dv = 0;
 #pragma unroll
   for (int r = 0; r < 3; ++r)
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       T rot = U[r * 3 + 0] * V[c * 3 + 0] + U[r * 3 + 1] * V[c * 3 + 1] + U[r * 3 + 2] * V[c * 3 + 2];
-      md[r * 3 + c] = (flip && c == 2) ? -rot : rot;
-      mdv[r * 3 + c] = U[r * 3 + 0] * phi[0] * V[c * 3 + 0] + U[r * 3 + 1] * phi[1] * V[c * 3 + 1] +
-                       U[r * 3 + 2] * phi[2] * V[c * 3 + 2];
+      T a = (flip && c == 2) ? -rot : rot;
+      T b = U[r * 3 + 0] * phi[0] * V[c * 3 + 0] + U[r * 3 + 1] * phi[1] * V[c * 3 + 1] +
+            U[r * 3 + 2] * phi[2] * V[c * 3 + 2];
+      if (md) {
+        md[r * 3 + c] = a;
+        mdv[r * 3 + c] = b;
+      } else {
+        dr += Z[r * 3 + c] * a;
+        dv += Z[r * 3 + c] * b;
+      }
     }
+  if (!md) {
+    *zmd = dr;
+    *zmdv = dv;
+  }
   // H = U^T Z V and Hr = U^T (Z Df) V
   T ZU[9], H[9], Hr[9];
 #pragma unroll
