@@ -125,6 +125,7 @@ def main():
     info = sc.info()
     log("setup done in %.1fs: %s" % (time.perf_counter() - t0, info))
     log("solve launch shapes per level ((Ct, threads) fwd, bwd): %s" % (sc.solve_shapes(),))
+    log("solve column chains (streams): %d" % sc.solve_chains)
     dev = torch.device("cuda:0")
     flops = 2.0 * 2.0 * info["nnz_lower"] * 3 * R
     if args.solve_only:
@@ -207,6 +208,7 @@ def main():
         rl = {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s", "frac": achieved / 8000.0}
     rl.update({"traffic": solve_traffic(3 * R, info["nnz_lower"]),
                "kernel": "supernodal solve k_fwd_level+k_bwd_level, %d RHS" % (3 * R),
+               "column_chains": sc.solve_chains,
                "ms_per_solve": ms_live, "solves_timed": n_solve,
                "ms_per_solve_back_to_back": sc.time_solve(20), "flops_per_solve": flops,
                "bytes_alg_per_solve": bytes_alg})

@@ -41,6 +41,7 @@ thread_local std::string g_err;
 
 constexpr double kEps = 5e-7;  // World.h:214
 constexpr int kThreads = 256;
+constexpr int kMaxSplit = 4;  // column chains of the split solve
 
 inline int cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
 
@@ -459,13 +460,13 @@ __device__ __forceinline__ v4d mfma_chain(v4d acc, int kbeg, int kend, int lk, F
 // Both are split into 16x16 output tiles distributed over the waves.
 __global__ __launch_bounds__(1024) void k_fwd_level(FactorView f, const int* __restrict__ lev, int ntile, int Ct, int C,
                                                    double* __restrict__ B, double* __restrict__ U,
-                                                   const int* __restrict__ any) {
+                                                   const int* __restrict__ any, int cbase, int cw) {
   if (any && *any == 0) return;
   extern __shared__ double sh[];
   const int lb = xcd_block();
   const int s = lev[lb / ntile];
-  const int col0 = (lb % ntile) * Ct;
-  const int ct = min(Ct, C - col0);
+  const int col0 = cbase + (lb % ntile) * Ct;
+  const int ct = min(Ct, cbase + cw - col0);
   const int c0 = f.first[s];
   const int w = f.first[s + 1] - c0;
   const long long rp = f.rowptr[s];
@@ -560,13 +561,14 @@ __global__ __launch_bounds__(1024) void k_fwd_level(FactorView f, const int* __r
 //   t = y_s - L[w:m, 0:w]^T x_below   (rows below are final)
 //   x_s = inv(L_ss)^T t
 __global__ __launch_bounds__(1024) void k_bwd_level(FactorView f, const int* __restrict__ lev, int ntile, int Ct, int C,
-                                                   double* __restrict__ B, const int* __restrict__ any) {
+                                                   double* __restrict__ B, const int* __restrict__ any, int cbase,
+                                                   int cw) {
   if (any && *any == 0) return;
   extern __shared__ double sh[];
   const int lb = xcd_block();
   const int s = lev[lb / ntile];
-  const int col0 = (lb % ntile) * Ct;
-  const int ct = min(Ct, C - col0);
+  const int col0 = cbase + (lb % ntile) * Ct;
+  const int ct = min(Ct, cbase + cw - col0);
   const int c0 = f.first[s];
   const int w = f.first[s + 1] - c0;
   const long long rp = f.rowptr[s];
@@ -1479,6 +1481,25 @@ struct pdm_scene_s {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int fwd_taped_seg = -1;  // checkpoint segment whose tapes the forward pass wrote
   DevBuf<int> alist, acount;  // rollouts that still iterate (k_compact)
+  // split solve: column chains on auxiliary streams (created on first use)
+  int nsplit = 1;
+  hipStream_t aux[kMaxSplit - 1] = {};
+  hipEvent_t ev_fork = nullptr, ev_join[kMaxSplit - 1] = {};
+  int ensure_aux_streams() {
+    if (ev_fork) return 0;
+    for (int k = 0; k < kMaxSplit - 1; ++k) {
+      if (hipStreamCreateWithFlags(&aux[k], hipStreamNonBlocking) != hipSuccess ||
+          hipEventCreateWithFlags(&ev_join[k], hipEventDisableTiming) != hipSuccess) {
+        g_err = "cannot create the solve streams";
+        return 1;
+      }
+    }
+    if (hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming) != hipSuccess) {
+      g_err = "cannot create the solve streams";
+      return 1;
+    }
+    return 0;
+  }
   // PDM_USE_GRAPH: captured step block and its device-side controls
   hipGraphExec_t gexec = nullptr;
   const double* g_act = nullptr;
@@ -1774,8 +1795,9 @@ pdm_scene_s::LevelCfg level_default(const pdm_scene_s* s, int l) {
   return c;
 }
 
-int launch_level(pdm_scene s, int pass, int l, const pdm_scene_s::LevelCfg& c, double* Bp,
-                 const int* any = nullptr) {
+// One level of one sweep over the columns [cbase, cbase + cw) on stream st.
+int launch_level(pdm_scene s, int pass, int l, const pdm_scene_s::LevelCfg& c, double* Bp, const int* any, int cbase,
+                 int cw, hipStream_t st) {
   const FactorView fv = s->view();
   int cnt = s->lev_ptr[l + 1] - s->lev_ptr[l];
   size_t lds = level_lds(s, l, c.Ct);
@@ -1783,14 +1805,29 @@ int launch_level(pdm_scene s, int pass, int l, const pdm_scene_s::LevelCfg& c, d
     g_err = "supernode front exceeds the LDS budget";
     return 1;
   }
-  int ntile = cdiv(s->C, c.Ct);
+  int ntile = cdiv(cw, c.Ct);
   if (pass == 0)
-    hipLaunchKernelGGL(k_fwd_level, dim3(cnt * ntile), dim3(c.nthr), lds, s->stream, fv, s->f_lev.p + s->lev_ptr[l],
-                       ntile, c.Ct, s->C, Bp, s->U.p, any);
+    hipLaunchKernelGGL(k_fwd_level, dim3(cnt * ntile), dim3(c.nthr), lds, st, fv, s->f_lev.p + s->lev_ptr[l], ntile,
+                       c.Ct, s->C, Bp, s->U.p, any, cbase, cw);
   else
-    hipLaunchKernelGGL(k_bwd_level, dim3(cnt * ntile), dim3(c.nthr), lds, s->stream, fv, s->f_lev.p + s->lev_ptr[l],
-                       ntile, c.Ct, s->C, Bp, any);
+    hipLaunchKernelGGL(k_bwd_level, dim3(cnt * ntile), dim3(c.nthr), lds, st, fv, s->f_lev.p + s->lev_ptr[l], ntile,
+                       c.Ct, s->C, Bp, any, cbase, cw);
   return 0;
+}
+
+// Column chunks of the split solve: nsplit contiguous ranges, multiples of 16.
+void split_columns(int C, int nsplit, std::vector<int>& base, std::vector<int>& width) {
+  int n16 = cdiv(C, 16);
+  nsplit = std::max(1, std::min(nsplit, n16));
+  base.assign(nsplit, 0);
+  width.assign(nsplit, 0);
+  int b = 0;
+  for (int i = 0; i < nsplit; ++i) {
+    int w16 = n16 / nsplit + (i < n16 % nsplit ? 1 : 0);
+    base[i] = b;
+    width[i] = std::min(C - b, 16 * w16);
+    b += 16 * w16;
+  }
 }
 
 int set_level_attrs() {
@@ -1802,6 +1839,8 @@ int set_level_attrs() {
   }
   return 0;
 }
+
+int launch_solve_levels(pdm_scene s, double* Bp, const int* any = nullptr);
 
 // Autotuner: for every level and sweep, time each admissible launch shape
 // (nq in {1,2,4} x {256,512,1024} threads) with HIP events on the library
@@ -1835,10 +1874,10 @@ int autotune_levels(pdm_scene s) {
           c.Ct = 16 * nq;
           c.nthr = nthr;
           if (level_lds(s, l, c.Ct) > kLdsMax) continue;
-          if (launch_level(s, pass, l, c, s->B.p)) return 1;  // warm-up
+          if (launch_level(s, pass, l, c, s->B.p, nullptr, 0, s->C, s->stream)) return 1;  // warm-up
           HIPX(hipEventRecord(e0, s->stream));
           for (int r = 0; r < reps; ++r)
-            if (launch_level(s, pass, l, c, s->B.p)) return 1;
+            if (launch_level(s, pass, l, c, s->B.p, nullptr, 0, s->C, s->stream)) return 1;
           HIPX(hipEventRecord(e1, s->stream));
           HIPX(hipEventSynchronize(e1));
           float ms = 0;
@@ -1850,10 +1889,31 @@ int autotune_levels(pdm_scene s) {
         }
       (pass == 0 ? s->cfg_fwd : s->cfg_bwd)[l] = bc;
     }
+  // independent column chains on several streams: time whole solves
+  s->tuned_C = s->C;
+  s->nsplit = 1;
+  float best = 1e30f;
+  int best_split = 1;
+  for (int ns = 1; ns <= kMaxSplit; ++ns) {
+    if (ns > 1 && cdiv(s->C, 16) < ns) break;
+    s->nsplit = ns;
+    if (launch_solve_levels(s, s->B.p)) return 1;
+    HIPX(hipEventRecord(e0, s->stream));
+    for (int r = 0; r < reps; ++r)
+      if (launch_solve_levels(s, s->B.p)) return 1;
+    HIPX(hipEventRecord(e1, s->stream));
+    HIPX(hipEventSynchronize(e1));
+    float ms = 0;
+    HIPX(hipEventElapsedTime(&ms, e0, e1));
+    if (ms < best) {
+      best = ms;
+      best_split = ns;
+    }
+  }
+  s->nsplit = best_split;
   HIPX(hipEventDestroy(e0));
   HIPX(hipEventDestroy(e1));
   HIPX(hipGetLastError());
-  s->tuned_C = s->C;
   return 0;
 }
 
@@ -1870,7 +1930,6 @@ int fold_solve_events(pdm_scene s) {
   return 0;
 }
 
-int launch_solve_levels(pdm_scene s, double* Bp, const int* any = nullptr);
 
 // any: optional device flag; the level kernels return at once when it is 0
 // (no rollout iterates any more), which graph replays rely on.
@@ -1900,10 +1959,29 @@ int launch_solve_levels(pdm_scene s, double* Bp, const int* any) {
     return 1;
   }
   int nlev = (int)s->lev_ptr.size() - 1;
-  for (int pass = 0; pass < 2; ++pass)
-    for (int q = 0; q < nlev; ++q) {
-      int l = pass == 0 ? q : nlev - 1 - q;
-      if (launch_level(s, pass, l, pass == 0 ? s->cfg_fwd[l] : s->cfg_bwd[l], Bp, any)) return 1;
+  std::vector<int> cb, cwid;
+  split_columns(s->C, s->nsplit, cb, cwid);
+  const int ns = (int)cb.size();
+  // chunk 0 runs on the library stream, the others on auxiliary streams that
+  // fork from it and join it again (default priority)
+  if (ns > 1) {
+    if (s->ensure_aux_streams()) return 1;
+    HIPX(hipEventRecord(s->ev_fork, s->stream));
+    for (int k = 1; k < ns; ++k) HIPX(hipStreamWaitEvent(s->aux[k - 1], s->ev_fork, 0));
+  }
+  for (int k = 0; k < ns; ++k) {
+    hipStream_t st = k == 0 ? s->stream : s->aux[k - 1];
+    for (int pass = 0; pass < 2; ++pass)
+      for (int q = 0; q < nlev; ++q) {
+        int l = pass == 0 ? q : nlev - 1 - q;
+        if (launch_level(s, pass, l, pass == 0 ? s->cfg_fwd[l] : s->cfg_bwd[l], Bp, any, cb[k], cwid[k], st))
+          return 1;
+      }
+  }
+  if (ns > 1)
+    for (int k = 1; k < ns; ++k) {
+      HIPX(hipEventRecord(s->ev_join[k - 1], s->aux[k - 1]));
+      HIPX(hipStreamWaitEvent(s->stream, s->ev_join[k - 1], 0));
     }
   HIPX(hipGetLastError());
   return 0;
@@ -2545,6 +2623,11 @@ void pdm_scene_destroy(pdm_scene s) {
   if (!s) return;
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   for (hipEvent_t e : s->sev) (void)hipEventDestroy(e);
+  for (int k = 0; k < kMaxSplit - 1; ++k) {
+    if (s->aux[k]) (void)hipStreamDestroy(s->aux[k]);
+    if (s->ev_join[k]) (void)hipEventDestroy(s->ev_join[k]);
+  }
+  if (s->ev_fork) (void)hipEventDestroy(s->ev_fork);
   drop_step_graph(s);
   if (s->ev0) (void)hipEventDestroy(s->ev0);
   if (s->ev1) (void)hipEventDestroy(s->ev1);
@@ -2778,7 +2861,8 @@ int pdm_time_solve(pdm_scene s, int reps, double* ms_per_solve) {
   return 0;
 }
 
-int pdm_get_solve_shapes(pdm_scene s, int cap, int* n_levels, int* ct_fwd, int* thr_fwd, int* ct_bwd, int* thr_bwd) {
+int pdm_get_solve_shapes(pdm_scene s, int cap, int* n_levels, int* ct_fwd, int* thr_fwd, int* ct_bwd, int* thr_bwd,
+                         int* n_chains) {
   if (!s || !n_levels || cap < 0) {
     g_err = "invalid argument";
     return 1;
@@ -2786,6 +2870,11 @@ int pdm_get_solve_shapes(pdm_scene s, int cap, int* n_levels, int* ct_fwd, int* 
   if (ensure_batch_buffers(s)) return 1;
   int nlev = (int)s->cfg_fwd.size();
   *n_levels = nlev;
+  if (n_chains) {
+    std::vector<int> cb, cw;
+    split_columns(s->C, s->nsplit, cb, cw);
+    *n_chains = (int)cb.size();
+  }
   for (int l = 0; l < std::min(cap, nlev); ++l) {
     if (ct_fwd) ct_fwd[l] = s->cfg_fwd[l].Ct;
     if (thr_fwd) thr_fwd[l] = s->cfg_fwd[l].nthr;

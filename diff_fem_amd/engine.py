@@ -214,8 +214,9 @@ class Scene:
         ((Ct, threads) forward, (Ct, threads) backward), level 0 = leaves."""
         cap = 64
         a = [(C.c_int * cap)() for _ in range(4)]
-        n = C.c_int()
-        check(_lib.load().pdm_get_solve_shapes(self.h, cap, C.byref(n), *a))
+        n, chains = C.c_int(), C.c_int()
+        check(_lib.load().pdm_get_solve_shapes(self.h, cap, C.byref(n), *a, C.byref(chains)))
+        self.solve_chains = chains.value
         return [((a[0][l], a[1][l]), (a[2][l], a[3][l])) for l in range(min(n.value, cap))]
 
     def solve_timing(self):

@@ -178,9 +178,11 @@ int pdm_get_solve_timing(pdm_scene s, long long* n_solves, double* total_ms);
 /* Launch shapes of the level kernels for the current batch (chosen by the
  * autotuner, or the defaults under PDM_NO_AUTOTUNE): for level l < cap,
  * column tile ct_*[l] and threads per block thr_*[l] of the forward (fwd) and
- * backward (bwd) sweep. *n_levels receives the level count. No counterpart in
- * the reference (launch tuning). */
-int pdm_get_solve_shapes(pdm_scene s, int cap, int* n_levels, int* ct_fwd, int* thr_fwd, int* ct_bwd, int* thr_bwd);
+ * backward (bwd) sweep. *n_levels receives the level count, *n_chains the
+ * number of column chains the solve is split into (each on its own stream).
+ * No counterpart in the reference (launch tuning). */
+int pdm_get_solve_shapes(pdm_scene s, int cap, int* n_levels, int* ct_fwd, int* thr_fwd, int* ct_bwd, int* thr_bwd,
+                         int* n_chains);
 
 /* Timing of the last forward/backward (ms, HIP events on the library stream). */
 int pdm_get_timing(pdm_scene s, double* fwd_ms, double* bwd_ms, double* factor_ms);
