@@ -392,27 +392,38 @@ __device__ __forceinline__ v4d mfma16(double a, double b, v4d c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
-// Front load: T[row][c] = src(row, c) for row < nvalid, c < ct; zero elsewhere.
-// rows == nullptr: source row c0 + row; else rows[row]. Loads are issued in
-// batches of 8 per thread so that their latencies overlap.
+// Element loops of the level kernels. Ct is a power of two and the block size
+// a multiple of it, so thread tid always handles column c = tid & (Ct-1) and
+// rows tid >> lg, tid >> lg + rs, ... (rs = blockDim / Ct): no divisions.
+struct TileIdx {
+  int c, r0, rs;
+};
+__device__ __forceinline__ TileIdx tile_idx(int tid, int nt, int lg) {
+  return TileIdx{tid & ((1 << lg) - 1), tid >> lg, nt >> lg};
+}
+
+// Front load: T[row][c] = src(row, c) for row < nvalid, c < ct; zero for the
+// other rows < nrows. rows == nullptr: source row c0 + row; else rows[row].
+// Eight loads per thread are issued before they are stored.
 __device__ __forceinline__ void load_front(const double* __restrict__ B, const int* __restrict__ rows, int c0,
-                                           int nvalid, int nrows, int Ct, int ct, int C, int col0,
-                                           double* __restrict__ T, int tid, int nt) {
-  const int total = nrows * Ct;
-  for (int base = tid; base < total; base += nt * 8) {
+                                           int nvalid, int nrows, int lg, int ct, int C, int col0,
+                                           double* __restrict__ T, TileIdx ti) {
+  const int Ct = 1 << lg;
+  const bool cok = ti.c < ct;
+  const double* Bc = B + col0 + ti.c;
+  for (int rb = ti.r0; rb < nrows; rb += 8 * ti.rs) {
     double v[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
-      int i = base + u * nt;
-      int row = i / Ct, c = i % Ct;
-      bool ok = i < total && row < nvalid && c < ct;
-      long long src = rows ? (long long)(ok ? rows[row] : 0) : (long long)(c0 + row);
-      v[u] = ok ? B[src * C + col0 + c] : 0.0;
+      int row = rb + u * ti.rs;
+      bool ok = cok && row < nvalid;
+      long long src = rows ? (ok ? rows[row] : 0) : (long long)(c0 + row);
+      v[u] = ok ? Bc[src * C] : 0.0;
     }
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
-      int i = base + u * nt;
-      if (i < total) T[i] = v[u];
+      int row = rb + u * ti.rs;
+      if (row < nrows) T[row * Ct + ti.c] = v[u];
     }
   }
 }
@@ -458,11 +469,12 @@ __device__ __forceinline__ v4d mfma_chain(v4d acc, int kbeg, int kend, int lk, F
 //   y = inv(L_ss) T[0:w]            (inverse precomputed at factorisation)
 //   T[w:m] -= L[w:m, 0:w] y         (update block U_s for the parent)
 // Both are split into 16x16 output tiles distributed over the waves.
-__global__ __launch_bounds__(1024) void k_fwd_level(FactorView f, const int* __restrict__ lev, int ntile, int Ct, int C,
+__global__ __launch_bounds__(1024) void k_fwd_level(FactorView f, const int* __restrict__ lev, int ntile, int lg, int C,
                                                    double* __restrict__ B, double* __restrict__ U,
                                                    const int* __restrict__ any, int cbase, int cw) {
   if (any && *any == 0) return;
   extern __shared__ double sh[];
+  const int Ct = 1 << lg;
   const int lb = xcd_block();
   const int s = lev[lb / ntile];
   const int col0 = cbase + (lb % ntile) * Ct;
@@ -472,52 +484,51 @@ __global__ __launch_bounds__(1024) void k_fwd_level(FactorView f, const int* __r
   const long long rp = f.rowptr[s];
   const int m = (int)(f.rowptr[s + 1] - rp);
   const int mpad = (m + 15) & ~15;
-  const int wpad = (w + 15) & ~15;
-  double* Y = sh + (long long)mpad * Ct;  // wpad x Ct
+  double* Y = sh + mpad * Ct;  // wpad x Ct
   const double* Ls = f.L + f.lptr[s];
   const double* DiC = f.DinvC + f.dptr[s];
   const int tid = threadIdx.x, nt = blockDim.x;
   const int lane = tid & 63, wave = tid >> 6, nwave = nt >> 6;
   const int lr = lane & 15, lk = lane >> 4;
   const int nq = Ct >> 4;
-  load_front(B, nullptr, c0, w, mpad, Ct, ct, C, col0, sh, tid, nt);
+  const TileIdx ti = tile_idx(tid, nt, lg);
+  load_front(B, nullptr, c0, w, mpad, lg, ct, C, col0, sh, ti);
   __syncthreads();
   for (int q = f.child_ptr[s]; q < f.child_ptr[s + 1]; ++q) {
     int ch = f.child[q];
     int wc = f.first[ch + 1] - f.first[ch];
     int mbc = (int)(f.rowptr[ch + 1] - f.rowptr[ch]) - wc;
     const int* rel = f.rel + f.relptr[ch];
-    const double* Uc = U + f.uptr[ch] * C;
-    const int total = mbc * ct;
-    for (int base = tid; base < total; base += nt * 8) {
-      double v[8];
+    const double* Uc = U + f.uptr[ch] * C + col0 + ti.c;
+    if (ti.c < ct) {
+      for (int ab = ti.r0; ab < mbc; ab += 8 * ti.rs) {
+        double v[8];
+        int dst[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        int i = base + u * nt;
-        int a = i / ct, c = i % ct;
-        v[u] = i < total ? Uc[(long long)a * C + col0 + c] : 0.0;
-      }
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        int i = base + u * nt;
-        if (i < total) {
-          int a = i / ct, c = i % ct;
-          sh[rel[a] * Ct + c] += v[u];
+        for (int u = 0; u < 8; ++u) {
+          int a = ab + u * ti.rs;
+          v[u] = a < mbc ? Uc[(long long)a * C] : 0.0;
+          dst[u] = a < mbc ? rel[a] : -1;
         }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          if (dst[u] >= 0) sh[dst[u] * Ct + ti.c] += v[u];
       }
     }
     __syncthreads();
   }
   // y = inv(L_ss) T[0:w]  (lower triangular: k <= row)
+  const int wpad = (w + 15) & ~15;
   const int nrt = wpad >> 4;
   for (int tq = wave; tq < nrt * nq; tq += nwave) {
     const int rt = (tq / nq) * 16, q = tq % nq;
     const int row = rt + lr;
     const int kend = min(w, rt + 16);
+    const double* pa = DiC + row;
+    const double* pb = sh + q * 16 + lr;
     v4d acc = {0.0, 0.0, 0.0, 0.0};
     acc = mfma_chain(
-        acc, 0, kend, lk, [&](int k) { return row < w ? DiC[(long long)k * w + row] : 0.0; },
-        [&](int k) { return sh[k * Ct + q * 16 + lr]; });
+        acc, 0, kend, lk, [&](int k) { return row < w ? pa[k * w] : 0.0; }, [&](int k) { return pb[k * Ct]; });
 #pragma unroll
     for (int i = 0; i < 4; ++i) Y[(rt + 4 * i + lk) * Ct + q * 16 + lr] = acc[i];
   }
@@ -528,6 +539,8 @@ __global__ __launch_bounds__(1024) void k_fwd_level(FactorView f, const int* __r
   for (int tq = wave; tq < nbt * nq; tq += nwave) {
     const int rt = w + (tq / nq) * 16, q = tq % nq;
     const int row = rt + lr;
+    const double* pa = Ls + row;
+    const double* pb = Y + q * 16 + lr;
     v4d acc;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -535,8 +548,7 @@ __global__ __launch_bounds__(1024) void k_fwd_level(FactorView f, const int* __r
       acc[i] = r < m ? sh[r * Ct + q * 16 + lr] : 0.0;
     }
     acc = mfma_chain(
-        acc, 0, w, lk, [&](int k) { return row < m ? -Ls[(long long)k * m + row] : 0.0; },
-        [&](int k) { return Y[k * Ct + q * 16 + lr]; });
+        acc, 0, w, lk, [&](int k) { return row < m ? -pa[k * m] : 0.0; }, [&](int k) { return pb[k * Ct]; });
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       int r = rt + 4 * i + lk;
@@ -544,15 +556,12 @@ __global__ __launch_bounds__(1024) void k_fwd_level(FactorView f, const int* __r
     }
   }
   __syncthreads();
-  for (int i = tid; i < w * ct; i += nt) {
-    int r = i / ct, c = i % ct;
-    B[(long long)(c0 + r) * C + col0 + c] = Y[r * Ct + c];
-  }
-  if (mb > 0) {
-    double* Us = U + f.uptr[s] * C;
-    for (int i = tid; i < mb * ct; i += nt) {
-      int a = i / ct, c = i % ct;
-      Us[(long long)a * C + col0 + c] = sh[(w + a) * Ct + c];
+  if (ti.c < ct) {
+    double* Bo = B + (long long)c0 * C + col0 + ti.c;
+    for (int r = ti.r0; r < w; r += ti.rs) Bo[(long long)r * C] = Y[r * Ct + ti.c];
+    if (mb > 0) {
+      double* Us = U + f.uptr[s] * C + col0 + ti.c;
+      for (int a = ti.r0; a < mb; a += ti.rs) Us[(long long)a * C] = sh[(w + a) * Ct + ti.c];
     }
   }
 }
@@ -560,11 +569,12 @@ __global__ __launch_bounds__(1024) void k_fwd_level(FactorView f, const int* __r
 // Backward substitution (L^T x = y) of one level as two dense products:
 //   t = y_s - L[w:m, 0:w]^T x_below   (rows below are final)
 //   x_s = inv(L_ss)^T t
-__global__ __launch_bounds__(1024) void k_bwd_level(FactorView f, const int* __restrict__ lev, int ntile, int Ct, int C,
+__global__ __launch_bounds__(1024) void k_bwd_level(FactorView f, const int* __restrict__ lev, int ntile, int lg, int C,
                                                    double* __restrict__ B, const int* __restrict__ any, int cbase,
                                                    int cw) {
   if (any && *any == 0) return;
   extern __shared__ double sh[];
+  const int Ct = 1 << lg;
   const int lb = xcd_block();
   const int s = lev[lb / ntile];
   const int col0 = cbase + (lb % ntile) * Ct;
@@ -575,7 +585,7 @@ __global__ __launch_bounds__(1024) void k_bwd_level(FactorView f, const int* __r
   const int m = (int)(f.rowptr[s + 1] - rp);
   const int mpad = (m + 15) & ~15;
   const int wpad = (w + 15) & ~15;
-  double* Tt = sh + (long long)mpad * Ct;  // wpad x Ct
+  double* Tt = sh + mpad * Ct;  // wpad x Ct
   const int* rows = f.rows + rp;
   const double* LR = f.LR + f.lptr[s];
   const double* Di = f.Dinv + f.dptr[s];
@@ -583,12 +593,15 @@ __global__ __launch_bounds__(1024) void k_bwd_level(FactorView f, const int* __r
   const int lane = tid & 63, wave = tid >> 6, nwave = nt >> 6;
   const int lr = lane & 15, lk = lane >> 4;
   const int nq = Ct >> 4;
-  load_front(B, rows, 0, m, mpad, Ct, ct, C, col0, sh, tid, nt);
+  const TileIdx ti = tile_idx(tid, nt, lg);
+  load_front(B, rows, 0, m, mpad, lg, ct, C, col0, sh, ti);
   __syncthreads();
   const int nrt = wpad >> 4;
   for (int tq = wave; tq < nrt * nq; tq += nwave) {
     const int rt = (tq / nq) * 16, q = tq % nq;
     const int col = rt + lr;  // column of L (output row of t)
+    const double* pa = LR + col;
+    const double* pb = sh + q * 16 + lr;
     v4d acc;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -596,8 +609,7 @@ __global__ __launch_bounds__(1024) void k_bwd_level(FactorView f, const int* __r
       acc[i] = r < w ? sh[r * Ct + q * 16 + lr] : 0.0;
     }
     acc = mfma_chain(
-        acc, w, m, lk, [&](int k) { return col < w ? -LR[(long long)k * w + col] : 0.0; },
-        [&](int k) { return sh[k * Ct + q * 16 + lr]; });
+        acc, w, m, lk, [&](int k) { return col < w ? -pa[k * w] : 0.0; }, [&](int k) { return pb[k * Ct]; });
 #pragma unroll
     for (int i = 0; i < 4; ++i) Tt[(rt + 4 * i + lk) * Ct + q * 16 + lr] = acc[i];
   }
@@ -606,10 +618,11 @@ __global__ __launch_bounds__(1024) void k_bwd_level(FactorView f, const int* __r
   for (int tq = wave; tq < nrt * nq; tq += nwave) {
     const int rt = (tq / nq) * 16, q = tq % nq;
     const int i_row = rt + lr;
+    const double* pa = Di + i_row;
+    const double* pb = Tt + q * 16 + lr;
     v4d acc = {0.0, 0.0, 0.0, 0.0};
     acc = mfma_chain(
-        acc, rt, w, lk, [&](int k) { return i_row < w ? Di[(long long)k * w + i_row] : 0.0; },
-        [&](int k) { return Tt[k * Ct + q * 16 + lr]; });
+        acc, rt, w, lk, [&](int k) { return i_row < w ? pa[k * w] : 0.0; }, [&](int k) { return pb[k * Ct]; });
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       int r = rt + 4 * i + lk;
@@ -617,9 +630,9 @@ __global__ __launch_bounds__(1024) void k_bwd_level(FactorView f, const int* __r
     }
   }
   __syncthreads();
-  for (int i = tid; i < w * ct; i += nt) {
-    int r = i / ct, c = i % ct;
-    B[(long long)(c0 + r) * C + col0 + c] = sh[r * Ct + c];
+  if (ti.c < ct) {
+    double* Bo = B + (long long)c0 * C + col0 + ti.c;
+    for (int r = ti.r0; r < w; r += ti.rs) Bo[(long long)r * C] = sh[r * Ct + ti.c];
   }
 }
 
@@ -1806,12 +1819,18 @@ int launch_level(pdm_scene s, int pass, int l, const pdm_scene_s::LevelCfg& c, d
     return 1;
   }
   int ntile = cdiv(cw, c.Ct);
+  int lg = 0;
+  while ((1 << lg) < c.Ct) ++lg;
+  if ((1 << lg) != c.Ct || c.nthr % c.Ct != 0) {
+    g_err = "invalid solve launch shape";
+    return 1;
+  }
   if (pass == 0)
     hipLaunchKernelGGL(k_fwd_level, dim3(cnt * ntile), dim3(c.nthr), lds, st, fv, s->f_lev.p + s->lev_ptr[l], ntile,
-                       c.Ct, s->C, Bp, s->U.p, any, cbase, cw);
+                       lg, s->C, Bp, s->U.p, any, cbase, cw);
   else
     hipLaunchKernelGGL(k_bwd_level, dim3(cnt * ntile), dim3(c.nthr), lds, st, fv, s->f_lev.p + s->lev_ptr[l], ntile,
-                       c.Ct, s->C, Bp, any, cbase, cw);
+                       lg, s->C, Bp, any, cbase, cw);
   return 0;
 }
 
