@@ -1129,13 +1129,14 @@ __global__ void k_contact_jacobi(const int* __restrict__ Lptr, const int* __rest
                                  const double* __restrict__ Ori, const double* __restrict__ Xn,
                                  const double* __restrict__ Kdiag, const int* __restrict__ cv_of_v,
                                  const int* __restrict__ imp_stamp, const double* __restrict__ imp_n, int nobs,
-                                 double mu, double alpha, double h, int first_iter, const int* __restrict__ iactive,
-                                 double* __restrict__ Prev, double* __restrict__ Xit, int N, int R,
-                                 double* __restrict__ tRC, double* __restrict__ tN) {
+                                 double mu, double alpha, double h, int first_iter, const int* __restrict__ alist,
+                                 const int* __restrict__ acount, double* __restrict__ Prev, double* __restrict__ Xit,
+                                 int N, int R, double* __restrict__ tRC, double* __restrict__ tN) {
   long long idx = (long long)xcd_block() * blockDim.x + threadIdx.x;
   if (idx >= (long long)N * R) return;
-  int v = (int)(idx / R), r = (int)(idx % R);
-  if (!iactive[r]) return;
+  const int v = (int)(idx / R), j = (int)(idx % R);
+  if (j >= *acount) return;
+  const int r = alist[j];
   long long C = 3LL * R;
   long long base = (long long)v * C + 3 * r;
   double rc[3] = {B[base], B[base + 1], B[base + 2]};
@@ -1279,18 +1280,15 @@ __global__ void k_cadj_pre(const double* __restrict__ XB, double* __restrict__ P
                            const double* __restrict__ Xit1, const double* __restrict__ Xn,
                            const double* __restrict__ Kdiag, const int* __restrict__ cv_of_v,
                            const double* __restrict__ tRC, const double* __restrict__ tN, double mu, double alpha,
-                           double h, int first, const int* __restrict__ active, double* __restrict__ Zb,
-                           double* __restrict__ Zs, double* __restrict__ CD, int N, int R) {
-  long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+                           double h, int first, const int* __restrict__ alist, const int* __restrict__ acount,
+                           double* __restrict__ Zb, double* __restrict__ Zs, double* __restrict__ CD, int N, int R) {
+  long long idx = (long long)xcd_block() * blockDim.x + threadIdx.x;
   if (idx >= (long long)N * R) return;
-  int v = (int)(idx / R), r = (int)(idx % R);
+  const int v = (int)(idx / R), j = (int)(idx % R);
+  if (j >= *acount) return;  // entries of other rollouts are never read
+  const int r = alist[j];
   long long C = 3LL * R;
   long long base = (long long)v * C + 3 * r;
-  if (!active[r]) {
-    Zb[base] = 0.0; Zb[base + 1] = 0.0; Zb[base + 2] = 0.0;
-    CD[idx] = 0.0;
-    return;
-  }
   double kd = Kdiag[v];
   double pb[3], cd = 0.0;
   for (int c = 0; c < 3; ++c) {
@@ -1316,25 +1314,27 @@ __global__ void k_cadj_pre(const double* __restrict__ XB, double* __restrict__ P
     Zb[base + c] = z;
     Zs[base + c] += z;
   }
-  CD[idx] = cd;
+  CD[(long long)v * R + r] = cd;
 }
 
 // rc = h (c + J d(ori) - C (ori - x_n)): Xb += C z, OB = -C z (C = offdiag of
 // the scalar matrix without masses).
 __global__ void k_cadj_cz(const int* __restrict__ Lptr, const int* __restrict__ Lcol, const double* __restrict__ Lval,
                           const double* __restrict__ Zb, double* __restrict__ Xb, double* __restrict__ OB,
-                          const int* __restrict__ active, int N, int C) {
+                          const int* __restrict__ alist, const int* __restrict__ acount, int N, int C) {
   long long idx = (long long)xcd_block() * blockDim.x + threadIdx.x;
   if (idx >= (long long)N * C) return;
-  int v = (int)(idx / C), j = (int)(idx % C);
-  if (!active[j / 3]) return;
+  const int v = (int)(idx / C), jj = (int)(idx % C);
+  if (jj >= 3 * *acount) return;
+  const int j = 3 * alist[jj / 3] + jj % 3;
   double sum = 0.0;
   for (int q = Lptr[v]; q < Lptr[v + 1]; ++q) {
     int u = Lcol[q];
     if (u != v) sum += Lval[q] * Zb[(long long)u * C + j];
   }
-  Xb[idx] += sum;
-  OB[idx] = -sum;
+  const long long o = (long long)v * C + j;
+  Xb[o] += sum;
+  OB[o] = -sum;
 }
 
 template <typename T>
@@ -1360,12 +1360,16 @@ __global__ void k_gather_masked_add(const T* __restrict__ corner, const int* __r
 // x_it_{g-1} adjoint = ori adjoint (+ the final-position adjoint for rollouts
 // whose result is this iteration's input).
 __global__ void k_cadj_post(const double* __restrict__ OB, const double* __restrict__ Gf, const int* __restrict__ inj,
-                            const int* __restrict__ active, double* __restrict__ XB, long long n, int C) {
+                            const int* __restrict__ alist, const int* __restrict__ acount, double* __restrict__ XB,
+                            long long n, int C) {
   long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (idx >= n) return;
-  int r = (int)(idx % C) / 3;
-  if (!active[r]) return;
-  XB[idx] = OB[idx] + (inj[r] ? Gf[idx] : 0.0);
+  const long long v = idx / C;
+  const int jj = (int)(idx - v * C);
+  if (jj >= 3 * *acount) return;
+  const int r = alist[jj / 3];
+  const long long o = v * C + 3 * r + jj % 3;
+  XB[o] = OB[o] + (inj[r] ? Gf[o] : 0.0);
 }
 
 // ------------------------------------------------------------------ engine
@@ -2079,8 +2083,8 @@ int contact_phase(pdm_scene s, const double* act, int act_steps, int step, Conta
         launch_local<double>(s, s->Ori.p, act, act_steps, step, s->iactive.p);
       hipLaunchKernelGGL(k_contact_jacobi, dim3(cdiv((long long)N * R, kThreads)), dim3(kThreads), 0, s->stream,
                          s->Lptr.p, s->Lcol.p, s->Lval.p, s->B.p, s->Ori.p, s->Xn.p, s->Kdiag.p, s->cv_of_v.p,
-                         s->imp_stamp.p, s->imp_n.p, s->nobs, S.mu, S.alpha, S.h, i == 0 ? 1 : 0, s->iactive.p,
-                         s->Prev.p, s->Xit.p, N, R, tp ? tp->rc.p + (size_t)tp->G * cvc : nullptr,
+                         s->imp_stamp.p, s->imp_n.p, s->nobs, S.mu, S.alpha, S.h, i == 0 ? 1 : 0, s->alist.p,
+                         s->acount.p, s->Prev.p, s->Xit.p, N, R, tp ? tp->rc.p + (size_t)tp->G * cvc : nullptr,
                          tp ? tp->nrm.p + (size_t)tp->G * cvc : nullptr);
       if (tp) {
         HIPX(hipMemcpyAsync(tp->xit.p + (size_t)(tp->G + 1) * nc, s->Xit.p, nc * 8, hipMemcpyDeviceToDevice,
@@ -2358,18 +2362,19 @@ int contact_reverse(pdm_scene s, ContactTape& tp, const double* Xn, const double
     if (!any) continue;
     HIPX(hipMemcpyAsync(s->ractive.p, tp.exec.p + (size_t)g * R, R * sizeof(int), hipMemcpyDeviceToDevice, s->stream));
     const double* ori = tp.xit.p + (size_t)g * nc;
+    hipLaunchKernelGGL(k_compact, dim3(1), dim3(1024), 0, s->stream, s->ractive.p, R, s->alist.p, s->acount.p);
     hipLaunchKernelGGL(k_cadj_pre, dim3(cdiv((long long)N * R, kThreads)), dim3(kThreads), 0, s->stream, s->cXB.p,
                        s->cPB.p, s->Xb.p, tp.xit.p + (size_t)(g + 1) * nc, Xn, s->Kdiag.p, s->cv_of_v.p,
                        tp.rc.p + (size_t)g * cvc, tp.nrm.p + (size_t)g * cvc, S.mu, S.alpha, S.h, (int)tp.first[g],
-                       s->ractive.p, s->Zb.p, s->Zs.p, s->cCD.p, N, R);
+                       s->alist.p, s->acount.p, s->Zb.p, s->Zs.p, s->cCD.p, N, R);
     hipLaunchKernelGGL(k_cadj_cz, dim3(cdiv(nc, kThreads)), dim3(kThreads), 0, s->stream, s->Lptr.p, s->Lcol.p,
-                       s->Lval.p, s->Zb.p, s->Xb.p, s->cOB.p, s->ractive.p, N, C);
+                       s->Lval.p, s->Zb.p, s->Xb.p, s->cOB.p, s->alist.p, s->acount.p, N, C);
     int rc = s->precision == PDM_MIXED
                  ? vjp_iteration<float>(s, ori, ori, act, act_steps, step, s->cCD.p, s->cOB.p, Xn)
                  : vjp_iteration<double>(s, ori, ori, act, act_steps, step, s->cCD.p, s->cOB.p, Xn);
     if (rc) return 1;
     hipLaunchKernelGGL(k_cadj_post, dim3(cdiv(nc, kThreads)), dim3(kThreads), 0, s->stream, s->cOB.p, s->cGf.p,
-                       dinj.p + (size_t)g * R, s->ractive.p, s->cXB.p, (long long)nc, C);
+                       dinj.p + (size_t)g * R, s->alist.p, s->acount.p, s->cXB.p, (long long)nc, C);
   }
   HIPX(hipMemcpyAsync(s->Yb.p, s->cXB.p, nc * 8, hipMemcpyDeviceToDevice, s->stream));
   HIPX(hipStreamSynchronize(s->stream));  // dinj is released on return
