@@ -125,7 +125,7 @@ def main():
     info = sc.info()
     log("setup done in %.1fs: %s" % (time.perf_counter() - t0, info))
     log("solve launch shapes per level ((Ct, threads) fwd, bwd): %s" % (sc.solve_shapes(),))
-    log("solve column chains (streams): %d" % sc.solve_chains)
+    log("solve column chains (streams): %d, subtree cut level: %d" % (sc.solve_chains, sc.solve_cut))
     dev = torch.device("cuda:0")
     flops = 2.0 * 2.0 * info["nnz_lower"] * 3 * R
     if args.solve_only:

@@ -56,7 +56,7 @@ SYMBOLS = [
     "pdm_set_stream", "pdm_set_material", "pdm_set_batch", "pdm_forward", "pdm_forward_host", "pdm_backward",
     "pdm_backward_host", "pdm_get_iterations", "pdm_solve_host", "pdm_force_host", "pdm_get_timing",
     "pdm_host_analyse", "pdm_host_solve", "pdm_get_contact_stats", "pdm_time_solve", "pdm_get_solve_timing",
-    "pdm_get_solve_shapes",
+    "pdm_get_solve_shapes", "pdm_set_solve_plan",
 ]
 
 _lib = None

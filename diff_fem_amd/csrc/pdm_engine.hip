@@ -469,16 +469,9 @@ __device__ __forceinline__ v4d mfma_chain(v4d acc, int kbeg, int kend, int lk, F
 //   y = inv(L_ss) T[0:w]            (inverse precomputed at factorisation)
 //   T[w:m] -= L[w:m, 0:w] y         (update block U_s for the parent)
 // Both are split into 16x16 output tiles distributed over the waves.
-__global__ __launch_bounds__(1024) void k_fwd_level(FactorView f, const int* __restrict__ lev, int ntile, int lg, int C,
-                                                   double* __restrict__ B, double* __restrict__ U,
-                                                   const int* __restrict__ any, int cbase, int cw) {
-  if (any && *any == 0) return;
-  extern __shared__ double sh[];
+__device__ __forceinline__ void fwd_supernode(const FactorView& f, int s, int col0, int ct, int lg, int C,
+                                              double* __restrict__ B, double* __restrict__ U, double* __restrict__ sh) {
   const int Ct = 1 << lg;
-  const int lb = xcd_block();
-  const int s = lev[lb / ntile];
-  const int col0 = cbase + (lb % ntile) * Ct;
-  const int ct = min(Ct, cbase + cw - col0);
   const int c0 = f.first[s];
   const int w = f.first[s + 1] - c0;
   const long long rp = f.rowptr[s];
@@ -566,19 +559,44 @@ __global__ __launch_bounds__(1024) void k_fwd_level(FactorView f, const int* __r
   }
 }
 
-// Backward substitution (L^T x = y) of one level as two dense products:
-//   t = y_s - L[w:m, 0:w]^T x_below   (rows below are final)
-//   x_s = inv(L_ss)^T t
-__global__ __launch_bounds__(1024) void k_bwd_level(FactorView f, const int* __restrict__ lev, int ntile, int lg, int C,
-                                                   double* __restrict__ B, const int* __restrict__ any, int cbase,
-                                                   int cw) {
+__global__ __launch_bounds__(1024) void k_fwd_level(FactorView f, const int* __restrict__ lev, int ntile, int lg, int C,
+                                                   double* __restrict__ B, double* __restrict__ U,
+                                                   const int* __restrict__ any, int cbase, int cw) {
   if (any && *any == 0) return;
   extern __shared__ double sh[];
   const int Ct = 1 << lg;
   const int lb = xcd_block();
-  const int s = lev[lb / ntile];
+  const int col0 = cbase + (lb % ntile) * Ct;
+  fwd_supernode(f, lev[lb / ntile], col0, min(Ct, cbase + cw - col0), lg, C, B, U, sh);
+}
+
+// Forward substitution of whole subtrees (the lower levels of the tree): block
+// = (subtree, column tile); the block walks the subtree's supernodes in
+// postorder (a contiguous index range, children first), so no kernel boundary
+// separates a child from its parent. Update blocks go through global memory
+// and are read back by the same block.
+__global__ __launch_bounds__(1024) void k_fwd_subtree(FactorView f, const int2* __restrict__ sub, int ntile, int lg,
+                                                     int C, double* __restrict__ B, double* __restrict__ U,
+                                                     const int* __restrict__ any, int cbase, int cw) {
+  if (any && *any == 0) return;
+  extern __shared__ double sh[];
+  const int Ct = 1 << lg;
+  const int lb = xcd_block();
+  const int2 rg = sub[lb / ntile];
   const int col0 = cbase + (lb % ntile) * Ct;
   const int ct = min(Ct, cbase + cw - col0);
+  for (int s = rg.x; s <= rg.y; ++s) {
+    fwd_supernode(f, s, col0, ct, lg, C, B, U, sh);
+    __syncthreads();
+  }
+}
+
+// Backward substitution (L^T x = y) of one level as two dense products:
+//   t = y_s - L[w:m, 0:w]^T x_below   (rows below are final)
+//   x_s = inv(L_ss)^T t
+__device__ __forceinline__ void bwd_supernode(const FactorView& f, int s, int col0, int ct, int lg, int C,
+                                              double* __restrict__ B, double* __restrict__ sh) {
+  const int Ct = 1 << lg;
   const int c0 = f.first[s];
   const int w = f.first[s + 1] - c0;
   const long long rp = f.rowptr[s];
@@ -633,6 +651,35 @@ __global__ __launch_bounds__(1024) void k_bwd_level(FactorView f, const int* __r
   if (ti.c < ct) {
     double* Bo = B + (long long)c0 * C + col0 + ti.c;
     for (int r = ti.r0; r < w; r += ti.rs) Bo[(long long)r * C] = sh[r * Ct + ti.c];
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_bwd_level(FactorView f, const int* __restrict__ lev, int ntile, int lg, int C,
+                                                   double* __restrict__ B, const int* __restrict__ any, int cbase,
+                                                   int cw) {
+  if (any && *any == 0) return;
+  extern __shared__ double sh[];
+  const int Ct = 1 << lg;
+  const int lb = xcd_block();
+  const int col0 = cbase + (lb % ntile) * Ct;
+  bwd_supernode(f, lev[lb / ntile], col0, min(Ct, cbase + cw - col0), lg, C, B, sh);
+}
+
+// Backward substitution of whole subtrees: parents before children (reverse
+// postorder), after the levels above them.
+__global__ __launch_bounds__(1024) void k_bwd_subtree(FactorView f, const int2* __restrict__ sub, int ntile, int lg,
+                                                     int C, double* __restrict__ B, const int* __restrict__ any,
+                                                     int cbase, int cw) {
+  if (any && *any == 0) return;
+  extern __shared__ double sh[];
+  const int Ct = 1 << lg;
+  const int lb = xcd_block();
+  const int2 rg = sub[lb / ntile];
+  const int col0 = cbase + (lb % ntile) * Ct;
+  const int ct = min(Ct, cbase + cw - col0);
+  for (int s = rg.y; s >= rg.x; --s) {
+    bwd_supernode(f, s, col0, ct, lg, C, B, sh);
+    __syncthreads();
   }
 }
 
@@ -1498,6 +1545,12 @@ struct pdm_scene_s {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int fwd_taped_seg = -1;  // checkpoint segment whose tapes the forward pass wrote
   DevBuf<int> alist, acount;  // rollouts that still iterate (k_compact)
+  // subtree cut: levels below `cut` run as whole subtrees (k_*_subtree)
+  int cut = 0;
+  LevelCfg cfg_sub;
+  std::vector<std::vector<int2>> sub_ranges;  // per cut level: (first, last) supernode of each subtree
+  DevBuf<int2> d_sub;
+  int d_sub_cut = -1;
   // split solve: column chains on auxiliary streams (created on first use)
   int nsplit = 1;
   hipStream_t aux[kMaxSplit - 1] = {};
@@ -1657,6 +1710,25 @@ int upload_factor(pdm_scene s) {
       s->lev_rows[l] = std::max(s->lev_rows[l], ((m + 15) & ~15) + ((w + 15) & ~15));
     }
   if (old_ptr != s->lev_ptr || old_rows != s->lev_rows) s->tuned_C = -1;  // level structure changed
+  {
+    // subtrees for every cut level c: roots of height < c whose parent has
+    // height >= c (or none); a subtree is the postorder range [first, root]
+    int ns = (int)F.first.size() - 1;
+    std::vector<int> height(ns, 0), fd(ns);
+    for (int l = 0; l < nlev; ++l)
+      for (int q = F.level_ptr[l]; q < F.level_ptr[l + 1]; ++q) height[F.level_sn[q]] = l;
+    for (int sn = 0; sn < ns; ++sn) {
+      fd[sn] = sn;
+      for (int q = F.child_ptr[sn]; q < F.child_ptr[sn + 1]; ++q) fd[sn] = std::min(fd[sn], fd[F.child[q]]);
+    }
+    s->sub_ranges.assign(nlev + 1, {});
+    for (int c = 1; c <= nlev; ++c)
+      for (int sn = 0; sn < ns; ++sn) {
+        int par = F.parent[sn];
+        if (height[sn] < c && (par < 0 || height[par] >= c)) s->sub_ranges[c].push_back(make_int2(fd[sn], sn));
+      }
+    s->d_sub_cut = -1;
+  }
   return 0;
 }
 
@@ -1838,6 +1910,42 @@ int launch_level(pdm_scene s, int pass, int l, const pdm_scene_s::LevelCfg& c, d
   return 0;
 }
 
+// Subtree kernels of cut level `cut` over the columns [cbase, cbase + cw).
+int launch_subtrees(pdm_scene s, int pass, int cut, const pdm_scene_s::LevelCfg& c, double* Bp, const int* any,
+                    int cbase, int cw, hipStream_t st) {
+  const FactorView fv = s->view();
+  int nsub = (int)s->sub_ranges[cut].size();
+  size_t rows = 16;
+  for (int l = 0; l < cut; ++l) rows = std::max<size_t>(rows, s->lev_rows[l]);
+  size_t lds = rows * c.Ct * 8;
+  if (lds > kLdsMax) {
+    g_err = "supernode front exceeds the LDS budget";
+    return 1;
+  }
+  int lg = 0;
+  while ((1 << lg) < c.Ct) ++lg;
+  int ntile = cdiv(cw, c.Ct);
+  if (nsub == 0) return 0;
+  if (pass == 0)
+    hipLaunchKernelGGL(k_fwd_subtree, dim3(nsub * ntile), dim3(c.nthr), lds, st, fv, s->d_sub.p, ntile, lg, s->C, Bp,
+                       s->U.p, any, cbase, cw);
+  else
+    hipLaunchKernelGGL(k_bwd_subtree, dim3(nsub * ntile), dim3(c.nthr), lds, st, fv, s->d_sub.p, ntile, lg, s->C, Bp,
+                       any, cbase, cw);
+  return 0;
+}
+
+// Device copy of the subtree ranges of cut level c (outside any capture).
+int set_cut(pdm_scene s, int c) {
+  s->cut = c;
+  if (c > 0 && s->d_sub_cut != c) {
+    const auto& r = s->sub_ranges[c];
+    if (s->d_sub.upload(r.data(), std::max<size_t>(1, r.size()))) return 1;
+    s->d_sub_cut = c;
+  }
+  return 0;
+}
+
 // Column chunks of the split solve: nsplit contiguous ranges, multiples of 16.
 void split_columns(int C, int nsplit, std::vector<int>& base, std::vector<int>& width) {
   int n16 = cdiv(C, 16);
@@ -1858,6 +1966,8 @@ int set_level_attrs() {
   if (!attr_set) {
     HIPX(hipFuncSetAttribute((const void*)k_fwd_level, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsMax));
     HIPX(hipFuncSetAttribute((const void*)k_bwd_level, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsMax));
+    HIPX(hipFuncSetAttribute((const void*)k_fwd_subtree, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsMax));
+    HIPX(hipFuncSetAttribute((const void*)k_bwd_subtree, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsMax));
     attr_set = true;
   }
   return 0;
@@ -1876,6 +1986,8 @@ int autotune_levels(pdm_scene s) {
   s->cfg_bwd.resize(nlev);
   for (int l = 0; l < nlev; ++l) s->cfg_fwd[l] = s->cfg_bwd[l] = level_default(s, l);
   if (set_level_attrs()) return 1;
+  s->cut = 0;
+  s->nsplit = 1;
   if (!s->autotune) {
     s->tuned_C = s->C;
     return 0;
@@ -1912,9 +2024,44 @@ int autotune_levels(pdm_scene s) {
         }
       (pass == 0 ? s->cfg_fwd : s->cfg_bwd)[l] = bc;
     }
-  // independent column chains on several streams: time whole solves
+  // whole-solve choices, timed with the tuned level shapes: the subtree cut
+  // (levels below it run as whole subtrees) with its launch shape, then the
+  // number of independent column chains on separate streams
   s->tuned_C = s->C;
   s->nsplit = 1;
+  auto time_solve_now = [&](float* out) -> int {
+    if (launch_solve_levels(s, s->B.p)) return 1;
+    HIPX(hipEventRecord(e0, s->stream));
+    for (int r = 0; r < reps; ++r)
+      if (launch_solve_levels(s, s->B.p)) return 1;
+    HIPX(hipEventRecord(e1, s->stream));
+    HIPX(hipEventSynchronize(e1));
+    HIPX(hipEventElapsedTime(out, e0, e1));
+    return 0;
+  };
+  {
+    float bestc = 1e30f;
+    int best_cut = 0;
+    pdm_scene_s::LevelCfg best_sc;
+    const pdm_scene_s::LevelCfg cands[3] = {{16, 256}, {16, 512}, {32, 512}};
+    for (int c = 0; c <= std::min(nlev - 1, 6); ++c)
+      for (int k = 0; k < (c == 0 ? 1 : 3); ++k) {
+        size_t rows = 16;
+        for (int l = 0; l < c; ++l) rows = std::max<size_t>(rows, s->lev_rows[l]);
+        if (rows * cands[k].Ct * 8 > kLdsMax) continue;
+        if (set_cut(s, c)) return 1;
+        s->cfg_sub = cands[k];
+        float ms;
+        if (time_solve_now(&ms)) return 1;
+        if (ms < bestc) {
+          bestc = ms;
+          best_cut = c;
+          best_sc = cands[k];
+        }
+      }
+    if (set_cut(s, best_cut)) return 1;
+    s->cfg_sub = best_sc;
+  }
   float best = 1e30f;
   int best_split = 1;
   for (int ns = 1; ns <= kMaxSplit; ++ns) {
@@ -1992,14 +2139,15 @@ int launch_solve_levels(pdm_scene s, double* Bp, const int* any) {
     HIPX(hipEventRecord(s->ev_fork, s->stream));
     for (int k = 1; k < ns; ++k) HIPX(hipStreamWaitEvent(s->aux[k - 1], s->ev_fork, 0));
   }
+  const int cut = s->cut;
   for (int k = 0; k < ns; ++k) {
     hipStream_t st = k == 0 ? s->stream : s->aux[k - 1];
-    for (int pass = 0; pass < 2; ++pass)
-      for (int q = 0; q < nlev; ++q) {
-        int l = pass == 0 ? q : nlev - 1 - q;
-        if (launch_level(s, pass, l, pass == 0 ? s->cfg_fwd[l] : s->cfg_bwd[l], Bp, any, cb[k], cwid[k], st))
-          return 1;
-      }
+    if (cut > 0 && launch_subtrees(s, 0, cut, s->cfg_sub, Bp, any, cb[k], cwid[k], st)) return 1;
+    for (int l = cut; l < nlev; ++l)
+      if (launch_level(s, 0, l, s->cfg_fwd[l], Bp, any, cb[k], cwid[k], st)) return 1;
+    for (int l = nlev - 1; l >= cut; --l)
+      if (launch_level(s, 1, l, s->cfg_bwd[l], Bp, any, cb[k], cwid[k], st)) return 1;
+    if (cut > 0 && launch_subtrees(s, 1, cut, s->cfg_sub, Bp, any, cb[k], cwid[k], st)) return 1;
   }
   if (ns > 1)
     for (int k = 1; k < ns; ++k) {
@@ -2886,7 +3034,7 @@ int pdm_time_solve(pdm_scene s, int reps, double* ms_per_solve) {
 }
 
 int pdm_get_solve_shapes(pdm_scene s, int cap, int* n_levels, int* ct_fwd, int* thr_fwd, int* ct_bwd, int* thr_bwd,
-                         int* n_chains) {
+                         int* n_chains, int* subtree_cut) {
   if (!s || !n_levels || cap < 0) {
     g_err = "invalid argument";
     return 1;
@@ -2899,12 +3047,36 @@ int pdm_get_solve_shapes(pdm_scene s, int cap, int* n_levels, int* ct_fwd, int* 
     split_columns(s->C, s->nsplit, cb, cw);
     *n_chains = (int)cb.size();
   }
+  if (subtree_cut) *subtree_cut = s->cut;
   for (int l = 0; l < std::min(cap, nlev); ++l) {
     if (ct_fwd) ct_fwd[l] = s->cfg_fwd[l].Ct;
     if (thr_fwd) thr_fwd[l] = s->cfg_fwd[l].nthr;
     if (ct_bwd) ct_bwd[l] = s->cfg_bwd[l].Ct;
     if (thr_bwd) thr_bwd[l] = s->cfg_bwd[l].nthr;
   }
+  return 0;
+}
+
+int pdm_set_solve_plan(pdm_scene s, int subtree_cut, int n_chains) {
+  if (!s) {
+    g_err = "null scene";
+    return 1;
+  }
+  if (ensure_batch_buffers(s)) return 1;
+  int nlev = (int)s->lev_ptr.size() - 1;
+  if (subtree_cut < 0 || subtree_cut >= std::max(1, nlev) || n_chains < 1 || n_chains > kMaxSplit) {
+    g_err = "invalid solve plan";
+    return 1;
+  }
+  size_t rows = 16;
+  for (int l = 0; l < subtree_cut; ++l) rows = std::max<size_t>(rows, s->lev_rows[l]);
+  if (rows * s->cfg_sub.Ct * 8 > kLdsMax) {
+    g_err = "subtree fronts exceed the LDS budget";
+    return 1;
+  }
+  HIPX(hipStreamSynchronize(s->stream));
+  if (set_cut(s, subtree_cut)) return 1;
+  s->nsplit = n_chains;
   return 0;
 }
 

@@ -214,10 +214,16 @@ class Scene:
         ((Ct, threads) forward, (Ct, threads) backward), level 0 = leaves."""
         cap = 64
         a = [(C.c_int * cap)() for _ in range(4)]
-        n, chains = C.c_int(), C.c_int()
-        check(_lib.load().pdm_get_solve_shapes(self.h, cap, C.byref(n), *a, C.byref(chains)))
+        n, chains, cut = C.c_int(), C.c_int(), C.c_int()
+        check(_lib.load().pdm_get_solve_shapes(self.h, cap, C.byref(n), *a, C.byref(chains), C.byref(cut)))
         self.solve_chains = chains.value
+        self.solve_cut = cut.value
         return [((a[0][l], a[1][l]), (a[2][l], a[3][l])) for l in range(min(n.value, cap))]
+
+    def set_solve_plan(self, subtree_cut: int, n_chains: int):
+        """Override the autotuned whole-solve plan (subtree cut level, column
+        chains); results do not depend on it."""
+        check(_lib.load().pdm_set_solve_plan(self.h, int(subtree_cut), int(n_chains)))
 
     def solve_timing(self):
         """(number of solves, summed ms) since the last call; needs the

@@ -179,10 +179,17 @@ int pdm_get_solve_timing(pdm_scene s, long long* n_solves, double* total_ms);
  * autotuner, or the defaults under PDM_NO_AUTOTUNE): for level l < cap,
  * column tile ct_*[l] and threads per block thr_*[l] of the forward (fwd) and
  * backward (bwd) sweep. *n_levels receives the level count, *n_chains the
- * number of column chains the solve is split into (each on its own stream).
+ * number of column chains the solve is split into (each on its own stream),
+ * *subtree_cut the level below which whole subtrees run in one launch (0: none).
  * No counterpart in the reference (launch tuning). */
 int pdm_get_solve_shapes(pdm_scene s, int cap, int* n_levels, int* ct_fwd, int* thr_fwd, int* ct_bwd, int* thr_bwd,
-                         int* n_chains);
+                         int* n_chains, int* subtree_cut);
+
+/* Override the autotuner's whole-solve plan for the current batch: levels below
+ * subtree_cut run as whole subtrees (0: none), and the columns are split into
+ * n_chains chains on separate streams (1..4). Results do not depend on the
+ * plan. No counterpart in the reference (launch tuning). */
+int pdm_set_solve_plan(pdm_scene s, int subtree_cut, int n_chains);
 
 /* Timing of the last forward/backward (ms, HIP events on the library stream). */
 int pdm_get_timing(pdm_scene s, double* fwd_ms, double* bwd_ms, double* factor_ms);

@@ -55,6 +55,11 @@ def test_solve_shapes_do_not_change_results(eng):
     x2 = s2.solve_host(b)
     assert np.array_equal(x1, x2)
     assert len(s1.solve_shapes()) == s1.info()["n_levels"]
+    # every plan: subtree cut levels and column chains
+    for cut in range(0, min(4, s1.info()["n_levels"])):
+        for chains in (1, 2, 3):
+            s1.set_solve_plan(cut, chains)
+            assert np.array_equal(s1.solve_host(b), x2), (cut, chains)
 
 
 def test_force_matches_oracle(eng, oracle_mod):
