@@ -720,16 +720,31 @@ __global__ void k_update(const double* __restrict__ Xn, const double* __restrict
 
 // Per rollout: err = ||x_new - x_old||_2 / (3N) (World.h:890-895); a rollout
 // whose err falls under 5e-7 stops iterating. iters counts executed iterations.
-__global__ void k_err_final(const double* __restrict__ part, int nchunk, int R, int N, int* __restrict__ active,
-                            int* __restrict__ iters, double* __restrict__ err, int* __restrict__ any) {
-  int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= R) return;
-  int C = 3 * R;
-  if (!active[r]) return;
-  double s = 0.0;
-  for (int ch = 0; ch < nchunk; ++ch)
-    for (int c = 0; c < 3; ++c) s += part[(long long)ch * C + 3 * r + c];
-  double e = sqrt(s) / (3.0 * N);
+// Per-rollout reductions of the (chunk, column) partials: one wave per
+// rollout, lanes stride over the chunks, then a fixed-order wave reduction.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
+  return v;
+}
+
+__global__ __launch_bounds__(256) void k_err_final(const double* __restrict__ part, int nchunk, int R, int N,
+                                                   int* __restrict__ active, int* __restrict__ iters,
+                                                   double* __restrict__ err, int* __restrict__ any) {
+  const int r = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (r >= R || !active[r]) return;
+  const long long C = 3LL * R;
+  double sacc = 0.0;
+  for (int ch = lane; ch < nchunk; ch += 64)
+    sacc += part[ch * C + 3 * r] + part[ch * C + 3 * r + 1] + part[ch * C + 3 * r + 2];
+  sacc = wave_sum(sacc);
+  if (lane != 0) return;
+  double e = sqrt(sacc) / (3.0 * N);
   err[r] = e;
   iters[r] += 1;
   if (e < kEps)
@@ -737,17 +752,18 @@ __global__ void k_err_final(const double* __restrict__ part, int nchunk, int R, 
   else if (any)
     atomicOr(any, 1);
 }
-
-// First check of the loop (iteration 0 compares qn with zero).
-__global__ void k_err_init(const double* __restrict__ part, int nchunk, int R, int N, int* __restrict__ active,
-                           int* __restrict__ iters, int* __restrict__ any) {
-  int r = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_err_init(const double* __restrict__ part, int nchunk, int R, int N,
+                                                  int* __restrict__ active, int* __restrict__ iters,
+                                                  int* __restrict__ any) {
+  const int r = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (r >= R) return;
-  int C = 3 * R;
-  double s = 0.0;
-  for (int ch = 0; ch < nchunk; ++ch)
-    for (int c = 0; c < 3; ++c) s += part[(long long)ch * C + 3 * r + c];
-  int a = (sqrt(s) / (3.0 * N) < kEps) ? 0 : 1;
+  const long long C = 3LL * R;
+  double sacc = 0.0;
+  for (int ch = lane; ch < nchunk; ch += 64)
+    sacc += part[ch * C + 3 * r] + part[ch * C + 3 * r + 1] + part[ch * C + 3 * r + 2];
+  sacc = wave_sum(sacc);
+  if (lane != 0) return;
+  int a = (sqrt(sacc) / (3.0 * N) < kEps) ? 0 : 1;
   active[r] = a;
   iters[r] = 0;
   if (any && a) atomicOr(any, 1);
@@ -1130,7 +1146,8 @@ __global__ void k_contact_pass_begin(int* __restrict__ cactive, int* __restrict_
 // Start of an inner Jacobi iteration: ori_x = x_it, err = max|ori_x - ori_x_old|
 // (World.h:945-955). Partial maxima per (chunk, column).
 __global__ void k_contact_begin(const double* __restrict__ Xit, double* __restrict__ Ori,
-                                const int* __restrict__ iactive, double* __restrict__ part, int N, int C) {
+                                const int* __restrict__ iactive, double* __restrict__ part, int N, int C,
+                                const double* __restrict__ Xn, double* __restrict__ Dn) {
   __shared__ double red[4][64];
   int col = blockIdx.x * 64 + threadIdx.x;
   int chunk = blockIdx.y;
@@ -1142,6 +1159,7 @@ __global__ void k_contact_begin(const double* __restrict__ Xit, double* __restri
       double x = Xit[i];
       mx = fmax(mx, fabs(x - Ori[i]));
       Ori[i] = x;
+      Dn[i] = x - Xn[i];  // ori - x_n, read by the neighbours' Jacobi update
     }
   }
   red[threadIdx.y][threadIdx.x] = mx;
@@ -1151,14 +1169,17 @@ __global__ void k_contact_begin(const double* __restrict__ Xit, double* __restri
         fmax(fmax(red[0][threadIdx.x], red[1][threadIdx.x]), fmax(red[2][threadIdx.x], red[3][threadIdx.x]));
 }
 
-__global__ void k_contact_err(const double* __restrict__ part, int nchunk, int R, int* __restrict__ iactive,
-                              int* __restrict__ cfirst, int* __restrict__ citers, int* __restrict__ clast) {
-  int r = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_contact_err(const double* __restrict__ part, int nchunk, int R,
+                                                     int* __restrict__ iactive, int* __restrict__ cfirst,
+                                                     int* __restrict__ citers, int* __restrict__ clast) {
+  const int r = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (r >= R || !iactive[r]) return;
-  int C = 3 * R;
+  const long long C = 3LL * R;
   double m = 0.0;
-  for (int ch = 0; ch < nchunk; ++ch)
-    for (int c = 0; c < 3; ++c) m = fmax(m, part[(long long)ch * C + 3 * r + c]);
+  for (int ch = lane; ch < nchunk; ch += 64)
+    m = fmax(m, fmax(part[ch * C + 3 * r], fmax(part[ch * C + 3 * r + 1], part[ch * C + 3 * r + 2])));
+  m = wave_max(m);
+  if (lane != 0) return;
   bool go = cfirst[r] || !(m < kEps);
   cfirst[r] = 0;
   iactive[r] = go ? 1 : 0;
@@ -1173,7 +1194,7 @@ __global__ void k_contact_err(const double* __restrict__ part, int nchunk, int R
 //   x_it = x_n + diag(M/h^2 + L)^-1 prev / h.
 __global__ void k_contact_jacobi(const int* __restrict__ Lptr, const int* __restrict__ Lcol,
                                  const double* __restrict__ Lval, const double* __restrict__ B,
-                                 const double* __restrict__ Ori, const double* __restrict__ Xn,
+                                 const double* __restrict__ Dn, const double* __restrict__ Xn,
                                  const double* __restrict__ Kdiag, const int* __restrict__ cv_of_v,
                                  const int* __restrict__ imp_stamp, const double* __restrict__ imp_n, int nobs,
                                  double mu, double alpha, double h, int first_iter, const int* __restrict__ alist,
@@ -1192,9 +1213,9 @@ __global__ void k_contact_jacobi(const int* __restrict__ Lptr, const int* __rest
     if (u == v) continue;
     long long ub = (long long)u * C + 3 * r;
     double w = Lval[q];
-    rc[0] -= w * (Ori[ub] - Xn[ub]);
-    rc[1] -= w * (Ori[ub + 1] - Xn[ub + 1]);
-    rc[2] -= w * (Ori[ub + 2] - Xn[ub + 2]);
+    rc[0] -= w * Dn[ub];
+    rc[1] -= w * Dn[ub + 1];
+    rc[2] -= w * Dn[ub + 2];
   }
   rc[0] *= h;
   rc[1] *= h;
@@ -1536,7 +1557,7 @@ struct pdm_scene_s {
   DevBuf<int> cvtx, cv_of_v, imp_stamp, cactive, iactive, cfirst, fresh, npass, citers, clast, cinj;
   std::vector<std::unique_ptr<ContactTape>> ctape;  // per step of the segment being reversed
   DevBuf<double> cXB, cPB, cOB, cGf, cCD;
-  DevBuf<double> Kdiag, d_planes, d_spheres, Ori, Prev, imp_n;
+  DevBuf<double> Kdiag, d_planes, d_spheres, Ori, Dn, Prev, imp_n;
   std::vector<int> npass_host, citers_host;  // R*steps of the last forward
   // adjoint buffers
   DevBuf<double> gX, gV, Yb, Xb, Zs, Zb, LZ, gEpart, gMpart, gact;
@@ -1849,7 +1870,7 @@ int ensure_batch_buffers(pdm_scene s) {
   if (s->alist.alloc(s->R) || s->acount.alloc(1)) return 1;
   if (s->ncv > 0) {
     size_t ni = (size_t)s->ncv * s->nobs * s->R;
-    if (s->Ori.alloc(nc) || s->Prev.alloc(nc) || s->imp_stamp.alloc(ni) || s->imp_n.alloc(3 * ni) ||
+    if (s->Ori.alloc(nc) || s->Dn.alloc(nc) || s->Prev.alloc(nc) || s->imp_stamp.alloc(ni) || s->imp_n.alloc(3 * ni) ||
         s->cactive.alloc(s->R) || s->iactive.alloc(s->R) || s->cfirst.alloc(s->R) || s->fresh.alloc(s->R) ||
         s->npass.alloc(s->R) || s->citers.alloc(s->R) || s->clast.alloc(s->R) || s->cinj.alloc(s->R))
       return 1;
@@ -2219,8 +2240,8 @@ int contact_phase(pdm_scene s, const double* act, int act_steps, int step, Conta
     }
     for (int i = 0; i < S.max_iter1; ++i) {
       hipLaunchKernelGGL(k_contact_begin, ugrid, ublock, 0, s->stream, s->Xit.p, s->Ori.p, s->iactive.p, s->part.p, N,
-                         C);
-      hipLaunchKernelGGL(k_contact_err, dim3(cdiv(R, 64)), dim3(64), 0, s->stream, s->part.p, nchunk, R,
+                         C, s->Xn.p, s->Dn.p);
+      hipLaunchKernelGGL(k_contact_err, dim3(cdiv(R, 4)), dim3(256), 0, s->stream, s->part.p, nchunk, R,
                          s->iactive.p, s->cfirst.p, s->citers.p, s->clast.p);
       if (tp)
         HIPX(hipMemcpyAsync(tp->exec.p + (size_t)tp->G * R, s->iactive.p, R * sizeof(int), hipMemcpyDeviceToDevice,
@@ -2230,7 +2251,7 @@ int contact_phase(pdm_scene s, const double* act, int act_steps, int step, Conta
       else
         launch_local<double>(s, s->Ori.p, act, act_steps, step, s->iactive.p);
       hipLaunchKernelGGL(k_contact_jacobi, dim3(cdiv((long long)N * R, kThreads)), dim3(kThreads), 0, s->stream,
-                         s->Lptr.p, s->Lcol.p, s->Lval.p, s->B.p, s->Ori.p, s->Xn.p, s->Kdiag.p, s->cv_of_v.p,
+                         s->Lptr.p, s->Lcol.p, s->Lval.p, s->B.p, s->Dn.p, s->Xn.p, s->Kdiag.p, s->cv_of_v.p,
                          s->imp_stamp.p, s->imp_n.p, s->nobs, S.mu, S.alpha, S.h, i == 0 ? 1 : 0, s->alist.p,
                          s->acount.p, s->Prev.p, s->Xit.p, N, R, tp ? tp->rc.p + (size_t)tp->G * cvc : nullptr,
                          tp ? tp->nrm.p + (size_t)tp->G * cvc : nullptr);
@@ -2271,7 +2292,7 @@ int launch_pd_block(pdm_scene s, const double* act, int act_steps, int step, dou
   if (tapeVs) HIPX(hipMemcpyAsync(tapeVs, s->Vs.p, nc * 8, hipMemcpyDeviceToDevice, s->stream));
   if (tapeY) HIPX(hipMemcpyAsync(tapeY, s->Xit.p, nc * 8, hipMemcpyDeviceToDevice, s->stream));
   hipLaunchKernelGGL(k_sqsum_chunks, ugrid, ublock, 0, s->stream, s->Xit.p, s->part.p, N, C, any);
-  hipLaunchKernelGGL(k_err_init, dim3(cdiv(s->R, 64)), dim3(64), 0, s->stream, s->part.p, nchunk, s->R, N,
+  hipLaunchKernelGGL(k_err_init, dim3(cdiv(s->R, 4)), dim3(256), 0, s->stream, s->part.p, nchunk, s->R, N,
                      s->active.p, s->iters_dev.p, any);
   std::vector<int> act_h(s->R);
   for (int it = 0; it < S.max_iter; ++it) {
@@ -2289,7 +2310,7 @@ int launch_pd_block(pdm_scene s, const double* act, int act_steps, int step, dou
     if (launch_solve(s, s->B.p, any)) return 1;
     hipLaunchKernelGGL(k_update, ugrid, ublock, 0, s->stream, s->Xn.p, s->B.p, s->Xit.p, s->active.p, s->part.p,
                        tapeY ? tapeY + (size_t)(it + 1) * nc : nullptr, N, C, any);
-    hipLaunchKernelGGL(k_err_final, dim3(cdiv(s->R, 64)), dim3(64), 0, s->stream, s->part.p, nchunk, s->R, N,
+    hipLaunchKernelGGL(k_err_final, dim3(cdiv(s->R, 4)), dim3(256), 0, s->stream, s->part.p, nchunk, s->R, N,
                        s->active.p, s->iters_dev.p, s->err.p, any);
   }
   HIPX(hipGetLastError());
