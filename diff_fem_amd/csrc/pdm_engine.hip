@@ -285,44 +285,43 @@ __global__ __launch_bounds__(256) void k_gather4(const double* __restrict__ Cc, 
   }
 }
 
-// Segmented gather over the listed rollouts only: thread (vertex, j) sums the
-// three columns of rollout alist[j] over the vertex's slot range (two partial
-// sums per column, fixed order). MODE 0: B = Cc + sum. MODE 1: B = sum.
-// MODE 2: B += sum. Columns of unlisted rollouts are left as they are.
+// Segmented gather over the listed rollouts only: thread (vertex, jj) sums
+// column 3 alist[jj/3] + jj%3 over the vertex's slot range (four partial sums,
+// fixed order), so that a wave reads contiguous slot entries. MODE 0:
+// B = Cc + sum. MODE 1: B = sum. MODE 2: B += sum. Columns of unlisted
+// rollouts are left as they are.
 template <typename T, int MODE>
 __global__ __launch_bounds__(256) void k_gather_list(const double* __restrict__ Cc, const T* __restrict__ corner,
                                                      const int* __restrict__ vptr, const int* __restrict__ alist,
                                                      const int* __restrict__ acount, double* __restrict__ B, int N,
                                                      int R, const int* __restrict__ any = nullptr) {
   if (any && *any == 0) return;
-  long long idx = (long long)xcd_block() * blockDim.x + threadIdx.x;
-  if (idx >= (long long)N * R) return;
-  const int v = (int)(idx / R);
-  const int j = (int)(idx - (long long)v * R);
-  if (j >= *acount) return;
-  const int r = alist[j];
   const long long C = 3LL * R;
-  double a0 = 0, a1 = 0, a2 = 0, b0 = 0, b1 = 0, b2 = 0;
+  long long idx = (long long)xcd_block() * blockDim.x + threadIdx.x;
+  if (idx >= (long long)N * C) return;
+  const int v = (int)(idx / C);
+  const int jj = (int)(idx - (long long)v * C);
+  if (jj >= 3 * *acount) return;
+  const int r3 = jj / 3;
+  const int col = 3 * alist[r3] + (jj - 3 * r3);
+  double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
   int q = vptr[v], qe = vptr[v + 1];
-  const T* p = corner + (long long)q * C + 3 * r;
-  for (; q + 2 <= qe; q += 2, p += 2 * C) {
-    T x0 = p[0], x1 = p[1], x2 = p[2], y0 = p[C], y1 = p[C + 1], y2 = p[C + 2];
-    a0 += (double)x0; a1 += (double)x1; a2 += (double)x2;
-    b0 += (double)y0; b1 += (double)y1; b2 += (double)y2;
+  const T* p = corner + (long long)q * C + col;
+  for (; q + 4 <= qe; q += 4, p += 4 * C) {
+    s0 += (double)p[0];
+    s1 += (double)p[C];
+    s2 += (double)p[2 * C];
+    s3 += (double)p[3 * C];
   }
-  if (q < qe) {
-    a0 += (double)p[0]; a1 += (double)p[1]; a2 += (double)p[2];
-  }
-  double* o = B + (long long)v * C + 3 * r;
-  double s0 = a0 + b0, s1 = a1 + b1, s2 = a2 + b2;
-  if (MODE == 0) {
-    const double* c = Cc + (long long)v * C + 3 * r;
-    o[0] = c[0] + s0; o[1] = c[1] + s1; o[2] = c[2] + s2;
-  } else if (MODE == 1) {
-    o[0] = s0; o[1] = s1; o[2] = s2;
-  } else {
-    o[0] += s0; o[1] += s1; o[2] += s2;
-  }
+  for (; q < qe; ++q, p += C) s0 += (double)p[0];
+  const double sum = (s0 + s1) + (s2 + s3);
+  const long long o = (long long)v * C + col;
+  if (MODE == 0)
+    B[o] = Cc[o] + sum;
+  else if (MODE == 1)
+    B[o] = sum;
+  else
+    B[o] += sum;
 }
 
 // B = Cc + sum of the vertex's corner slots (fixed order; four partial sums
@@ -2192,7 +2191,7 @@ void launch_local(pdm_scene s, const double* Xsrc, const double* act, int act_st
   hipLaunchKernelGGL(kern, dim3(cdiv(nt, kThreads)), dim3(kThreads), 0, s->stream, Xsrc, s->tv.p, s->tG.p, s->wr.p,
                      s->wv.p, s->tslot.p, s->alist.p, s->acount.p, corner, S.T, s->R, s->mptr.p, s->mg.p, s->mw.p,
                      s->mgroup.p, S.M, act, act_steps * G, step * G, step_dev, G);
-  hipLaunchKernelGGL((k_gather_list<T, 0>), dim3(cdiv((long long)S.N * s->R, kThreads)), dim3(kThreads), 0, s->stream,
+  hipLaunchKernelGGL((k_gather_list<T, 0>), dim3(cdiv((long long)S.N * 3 * s->R, kThreads)), dim3(kThreads), 0, s->stream,
                      s->Cc.p, corner, s->vptr.p, s->alist.p, s->acount.p, s->B.p, S.N, s->R, any);
 }
 
@@ -2490,7 +2489,7 @@ int vjp_iteration(pdm_scene s, const double* Yk, const double* Yk1, const double
                      s->wr.p, s->wv.p, s->dEr.p, s->dEv.p, s->tslot.p, s->alist.p, s->acount.p, corner, s->gEpart.p,
                      S.T, s->R, CDp ? Xn : nullptr, CDp, s->mptr.p, s->mg.p, s->mw.p, s->mgroup.p, S.M, act,
                      act_steps * S.n_groups, step * S.n_groups, S.M > 0 ? s->gMpart.p : nullptr);
-  dim3 gg(cdiv((long long)S.N * s->R, kThreads));
+  dim3 gg(cdiv((long long)S.N * 3 * s->R, kThreads));
   if (out_add)
     hipLaunchKernelGGL((k_gather_list<T, 2>), gg, dim3(kThreads), 0, s->stream, nullptr, corner, s->vptr.p,
                        s->alist.p, s->acount.p, out_add, S.N, s->R, nullptr);
