@@ -73,8 +73,8 @@ __device__ __forceinline__ void svd3(const T* F, T* U, T* S, T* V) {
   T a12 = F[1] * F[2] + F[4] * F[5] + F[7] * F[8];
   T a22 = F[2] * F[2] + F[5] * F[5] + F[8] * F[8];
   T v[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
-#pragma unroll
-  for (int sw = 0; sw < Num<T>::kSweeps; ++sw) {
+  // one cyclic Jacobi sweep over (0,1), (0,2), (1,2)
+  auto sweep = [&]() {
     // (p,q) = (0,1)
     {
       T apq = a01;
@@ -132,6 +132,17 @@ __device__ __forceinline__ void svd3(const T* F, T* U, T* S, T* V) {
         }
       }
     }
+    };
+  if constexpr (sizeof(T) == 8) {
+#pragma unroll
+    for (int sw = 0; sw < Num<T>::kSweeps; ++sw) sweep();
+  } else {
+    // FP32: three sweeps, a fourth only while the off-diagonal part of F^T F
+    // is above 1e-7 of its trace (rare for the strains met here)
+#pragma unroll
+    for (int sw = 0; sw < 3; ++sw) sweep();
+    T off = fabs(a01) + fabs(a02) + fabs(a12);
+    if (off > T(1e-7) * (a00 + a11 + a22)) sweep();
   }
   // B = F V, columns sorted by norm (descending)
   T b[9];
