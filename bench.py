@@ -101,6 +101,8 @@ def main():
                     help="1: replay each step's PD block as a captured hipGraph (device-side early exit)")
     ap.add_argument("--solve-only", action="store_true",
                     help="only time back-to-back solves (short program for counter runs)")
+    ap.add_argument("--plan", default="",
+                    help="CUT,CHAINS: fix the whole-solve plan instead of the autotuned one (counter runs)")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="also run a device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -128,6 +130,11 @@ def main():
     log("solve column chains (streams): %d, subtree cut level: %d" % (sc.solve_chains, sc.solve_cut))
     dev = torch.device("cuda:0")
     flops = 2.0 * 2.0 * info["nnz_lower"] * 3 * R
+    if args.plan:
+        cut, chains = (int(v) for v in args.plan.split(","))
+        sc.set_solve_plan(cut, chains)
+        sc.solve_shapes()
+        log("solve plan fixed: subtree cut %d, %d column chain(s)" % (sc.solve_cut, sc.solve_chains))
     if args.solve_only:
         ms = sc.time_solve(20)
         if args.copy_calib:
