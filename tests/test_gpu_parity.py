@@ -281,3 +281,75 @@ def test_packed_gradient_matches_oracle(eng, oracle_mod):
     gradient (about 2e-9 relative after 6 steps from exact rest)."""
     d = packed_scene()
     _grad_case(eng, oracle_mod, d, eng.PDM_FP64, steps=6, R=2, tol_x=1e-9, tol_g=1e-9, ckpt=4, x_jitter=2e-3)
+
+
+@pytest.mark.parametrize("contact,steps,eps_a,eps_e,tol_e", [(False, 200, 1e-4, 1e-3, 1e-3), (True, 50, 1e-6, 1e-6, 1e-3)])
+def test_config3_gradient_vs_central_differences(eng, contact, steps, eps_a, eps_e, tol_e):
+    """Config 3: the 30x10x10 muscle box (18,000 tets, 64 fibres), parameters
+    per-tet E and activations a[steps x 64], loss = |centroid(x_T) - target|^2
+    with a seeded target, FP64 adjoint with checkpoints every 20 steps.
+    Directional derivatives along seeded directions agree with central
+    differences, and the perturbed rollouts execute the same iterations as
+    the base rollout (the adjoint differentiates the executed control flow, as
+    the reference's CppAD tape does).
+
+    The difference quotients have a noise floor: each rollout stops its PD
+    iterations at the 5e-7 tolerance, so the loss carries a small
+    parameter-dependent truncation error (about 1e-10 of the loss here). E
+    directions therefore use a relative step of 1e-3 (1e-6 with contact), a 1e-3
+    bar and directions without sign cancellation; activation directions agree
+    to 1e-4.
+
+    With ground contact the loss is only piecewise smooth: stick, slide and
+    take-off switch per vertex and per iteration. A central difference that
+    spans a switch measures the jump, so the contact case uses 50 steps
+    (36 of them in contact) and small steps. Over 200 steps of sliding,
+    differences of any size cross switches; there the GPU gradient is pinned
+    to the oracle's instead (contact gradient tests)."""
+    d = scene.config2()
+    d.contact = contact
+    rng = np.random.default_rng(30)
+    x0 = d.x_rest.ravel().copy()
+    v0 = np.zeros_like(x0)
+    act = scene.activations_uniform(31, steps, d.n_groups, 1)
+    target = d.x_rest.mean(0) + np.array([0.02, -0.01, 0.01])
+
+    def loss(xT):
+        c = xT.reshape(-1, 3).mean(0)
+        return float(((c - target) ** 2).sum())
+
+    s = eng.Scene(d, R=1, flags=eng.PDM_HOST_EARLY_EXIT, checkpoint_every=20)
+    xT, _ = s.forward_host(x0[None], v0[None], steps, act=act, record=True)
+    its0 = s.iterations().copy()
+    c = xT[0].reshape(-1, 3).mean(0)
+    gx = np.tile(2.0 * (c - target) / d.N, d.N)[None]
+    g = s.backward_host(gx, np.zeros_like(gx))
+    # activations: two directions, four rollouts in one batch
+    da = [rng.uniform(-1, 1, act.shape) for _ in range(2)]
+    sa = eng.Scene(d, R=4, flags=eng.PDM_HOST_EARLY_EXIT)
+    acts = np.concatenate([act + eps_a * da[0], act - eps_a * da[0], act + eps_a * da[1], act - eps_a * da[1]])
+    xa, _ = sa.forward_host(np.tile(x0, (4, 1)), np.tile(v0, (4, 1)), steps, act=acts)
+    for r in range(4):
+        assert np.array_equal(sa.iterations()[r], its0[0]), r
+    for k in range(2):
+        fd = (loss(xa[2 * k]) - loss(xa[2 * k + 1])) / (2 * eps_a)
+        an = float((g["act"][0] * da[k][0]).sum())
+        assert abs(fd - an) <= 1e-4 * abs(an), (k, fd, an)
+    # per-tet E: a uniform scaling and (without contact) a random relative
+    # direction whose signs follow the gradient's (no cancellation, so the
+    # directional derivative stands well above the difference quotient's noise
+    # floor). With contact, per-tet perturbations move single vertices across
+    # stick/slide boundaries; the per-tet split of the contact E-gradient is
+    # pinned on the oracle instead (test_contact_per_tet_E_gradient_vs_fd).
+    dirs = [d.E.copy()] if contact else [d.E.copy(), rng.uniform(0.5, 1, d.T) * d.E * np.sign(g["E"])]
+    for k, dE in enumerate(dirs):
+        vals = []
+        for sgn in (1, -1):
+            se = eng.Scene(d, R=1, flags=eng.PDM_HOST_EARLY_EXIT)
+            se.set_material(d.E + sgn * eps_e * dE)
+            xe, _ = se.forward_host(x0[None], v0[None], steps, act=act)
+            assert np.array_equal(se.iterations()[0], its0[0]), (k, sgn)
+            vals.append(loss(xe[0]))
+        fd = (vals[0] - vals[1]) / (2 * eps_e)
+        an = float((g["E"] * dE).sum())
+        assert abs(fd - an) <= tol_e * abs(an), (k, fd, an)

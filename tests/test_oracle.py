@@ -364,6 +364,41 @@ def test_contact_gradient_vs_central_fd_plane(oracle_mod):
         s.set_material(np.full(s.T, 8e4), 0.4)
 
 
+def test_contact_per_tet_E_gradient_vs_fd(oracle_mod):
+    """Per-tet split of the E-gradient through the contact phase (the D and C
+    weight dependence of the Jacobi update lands on the right tets): single-tet
+    and random per-tet directions against central differences, sliding box on
+    a plane."""
+    x, t = scene.box_grid(5, 2, 2, origin=(0, 0, 0.015))
+    d = scene.SceneDesc(x_rest=x, tets=t, total_mass=2.0, E=np.full(t.shape[0], 8e4), h=0.01, max_iter=10,
+                        gravity=(0, 0, -9.81), contact=True, max_iter1=10, mu=0.5,
+                        planes=np.array([[0, 0, 0, 0, 0, 1.0]]))
+    s = oracle_mod.OracleScene(d)
+    x0 = x.ravel().copy()
+    v0 = np.zeros_like(x0)
+    v0[0::3] = 0.3
+    steps = 14
+    rng = np.random.default_rng(12)
+    gx = rng.standard_normal(x0.shape)
+    gv = 0.01 * rng.standard_normal(x0.shape)
+    _, _, its0 = s.rollout(x0, v0, steps, record=True)
+    assert its0[:, 1].sum() > 0
+    g, rc = s.backward(gx, gv)
+    E0 = d.E.copy()
+
+    def loss(E):
+        s.set_material(E, 0.4)
+        xT, vT, its = s.rollout(x0, v0, steps)
+        assert np.array_equal(its, its0)
+        return gx @ xT + gv @ vT
+
+    for dE in (np.eye(s.T)[3] * E0, np.eye(s.T)[40] * E0, rng.uniform(-1, 1, s.T) * E0):
+        eps = 1e-5
+        fd = (loss(E0 + eps * dE) - loss(E0 - eps * dE)) / (2 * eps)
+        assert abs(fd - g["E"] @ dE) < 1e-5 * abs(fd), (fd, g["E"] @ dE)
+    s.set_material(E0, 0.4)
+
+
 def _packed_pair(seed=7, contact=False, max_iter=3):
     ms = scene.config5_meshes(seed=seed, n_meshes=3, tmin=200, tmax=800)
     for m in ms:
