@@ -242,11 +242,18 @@ __device__ __forceinline__ void volproj(const T* S, T* D) {
   }
 }
 
-// Same projection with its Jacobian J[i][j] = d(S_i + D_i)/d S_j (forward mode).
+// Same projection with its Jacobian J[i][j] = d(S_i + D_i)/d S_j (forward
+// mode), and the divided differences rdd = (phi_i - phi_j)/(S_i - S_j) of
+// phi = S + D for the pairs (0,1), (0,2), (1,2), carried through the
+// iterations instead of formed by division: with a the current iterate and k
+// the third index, g_i - g_j = -a_k (a_i - a_j), so the ratio r becomes
+// 1 - coef a_k r (r = 1 at D = 0). No cancellation where S_i ~ S_j, and the
+// value at S_i = S_j is the derivative limit.
 template <typename T>
-__device__ __forceinline__ void volproj_jac(const T* S, T* D, T* J) {
+__device__ __forceinline__ void volproj_jac(const T* S, T* D, T* J, T* rdd) {
   T d[3] = {0, 0, 0};
   T dd[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  T r0 = 1, r1 = 1, r2 = 1;
 #pragma unroll
   for (int it = 0; it < 5; ++it) {
     T a[3] = {S[0] + d[0], S[1] + d[1], S[2] + d[2]};
@@ -269,6 +276,9 @@ __device__ __forceinline__ void volproj_jac(const T* S, T* D, T* J) {
     T num = g[0] * d[0] + g[1] * d[1] + g[2] * d[2] - cd;
     T igg = Num<T>::rcp(gg);
     T coef = num * igg;
+    r0 = T(1) - coef * a[2] * r0;
+    r1 = T(1) - coef * a[1] * r1;
+    r2 = T(1) - coef * a[0] * r2;
     T nd[3], ndd[9];
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
@@ -288,6 +298,9 @@ __device__ __forceinline__ void volproj_jac(const T* S, T* D, T* J) {
   }
 #pragma unroll
   for (int i = 0; i < 3; ++i) D[i] = d[i];
+  rdd[0] = r0;
+  rdd[1] = r1;
+  rdd[2] = r2;
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -320,9 +333,9 @@ __device__ __forceinline__ void corotate_P(const T* F, T wr, T wv, T* P) {
 template <typename T>
 __device__ __forceinline__ void corotate_vjp(const T* F, T wr, T wv, const T* Z, T* Fbar, T* md, T* mdv,
                                              T* zmd = nullptr, T* zmdv = nullptr) {
-  T U[9], S[3], V[9], D[3], J[9];
+  T U[9], S[3], V[9], D[3], J[9], rdd[3];
   svd3<T>(F, U, S, V);
-  volproj_jac<T>(S, D, J);
+  volproj_jac<T>(S, D, J, rdd);
   bool flip = det3<T>(F) < 0;
   T phi[3] = {S[0] + D[0], S[1] + D[1], S[2] + D[2]};
   T dr = 0, dv = 0;
@@ -379,8 +392,7 @@ __device__ __forceinline__ void corotate_vjp(const T* F, T wr, T wv, const T* Z,
       T ssum = S[i] + S[j];
       T isum = ssum > 0 ? Num<T>::rcp(ssum) : T(0);
       T br = T(2) * isum;
-      T sd = S[i] - S[j];
-      T av = (fabs(sd) > T(1e-6) * (T(1) + fabs(S[i]))) ? Num<T>::div(phi[i] - phi[j], sd) : (J[i * 3 + i] - J[i * 3 + j]);
+      T av = rdd[i + j - 1];  // pair (0,1) -> 0, (0,2) -> 1, (1,2) -> 2
       T bv = (phi[i] + phi[j]) * isum;
       T rot = T(0.5) * br * (Hr[i * 3 + j] - Hr[j * 3 + i]);
       T vol = T(0.5) * (av + bv) * H[i * 3 + j] + T(0.5) * (av - bv) * H[j * 3 + i];

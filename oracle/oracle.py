@@ -66,6 +66,7 @@ def lib():
         L.orc_corotate_vjp.argtypes = [_dp, C.c_double, C.c_double, _dp, _dp]
         L.orc_svd3.argtypes = [_dp, _dp, _dp, _dp]
         L.orc_volume_projection.argtypes = [_dp, _dp]
+        L.orc_volume_projection_jac.argtypes = [_dp, _dp, _dp, _dp]
         L.orc_detect_plane.argtypes = [_dp, _dp, _dp, _dp, _dp]
         L.orc_project_contact.argtypes = [_dp, C.c_double, _dp, _dp]
         L.orc_last_error.restype = C.c_char_p
@@ -268,6 +269,14 @@ def volume_projection(S):
     D = np.zeros(3)
     lib().orc_volume_projection(np.ascontiguousarray(S, np.float64), D)
     return D
+
+
+def volume_projection_jac(S):
+    """(D, J = d(S + D)/dS, rdd = divided differences of S + D for the
+    pairs (0,1), (0,2), (1,2))."""
+    D, J, r = np.zeros(3), np.zeros((3, 3)), np.zeros(3)
+    lib().orc_volume_projection_jac(np.ascontiguousarray(S, np.float64), D, J, r)
+    return D, J, r
 
 
 def detect_plane(p, n, x0, x1):

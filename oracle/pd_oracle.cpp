@@ -209,9 +209,16 @@ void corotate_project(const double* F, double* md, double* mdv) {
 
 // Jacobian-transpose of phi(S) = S + D(S) (the 5-step projection), by forward
 // mode over the 3 input directions: returns J[i][j] = d phi_i / d S_j.
-void volume_projection_jac(const double* S, double* Dout, double J[3][3]) {
+// Also the divided differences rdd = ((phi_0-phi_1)/(S_0-S_1),
+// (phi_0-phi_2)/(S_0-S_2), (phi_1-phi_2)/(S_1-S_2)) of phi = S + D, without
+// dividing by S_i - S_j: with a = S + D the current iterate and k the third
+// index, g_i - g_j = -a_k (a_i - a_j), so the new difference is
+// (S_i - S_j)(1 - coef a_k r) where r is the current ratio (1 at D = 0). The
+// ratio is exact also where S_i = S_j (it is then the derivative limit).
+void volume_projection_jac(const double* S, double* Dout, double J[3][3], double* rdd) {
   double D[3] = {0, 0, 0};
   double dD[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};  // dD[i][j] = dD_i/dS_j
+  double r[3] = {1.0, 1.0, 1.0};
   for (int it = 0; it < 5; ++it) {
     double a[3] = {S[0] + D[0], S[1] + D[1], S[2] + D[2]};
     double da[3][3];
@@ -236,6 +243,9 @@ void volume_projection_jac(const double* S, double* Dout, double J[3][3]) {
       double dgg = 2.0 * (g[0] * dg[0][j] + g[1] * dg[1][j] + g[2] * dg[2][j]);
       dcoef[j] = (dnum * gg - num * dgg) / (gg * gg);
     }
+    r[0] = 1.0 - coef * a[2] * r[0];
+    r[1] = 1.0 - coef * a[1] * r[1];
+    r[2] = 1.0 - coef * a[0] * r[2];
     double nD[3], ndD[3][3];
     for (int i = 0; i < 3; ++i) {
       nD[i] = coef * g[i];
@@ -246,6 +256,7 @@ void volume_projection_jac(const double* S, double* Dout, double J[3][3]) {
   }
   for (int i = 0; i < 3; ++i) {
     Dout[i] = D[i];
+    rdd[i] = r[i];
     for (int j = 0; j < 3; ++j) J[i][j] = (i == j ? 1.0 : 0.0) + dD[i][j];
   }
 }
@@ -256,13 +267,14 @@ void volume_projection_jac(const double* S, double* Dout, double J[3][3]) {
 // G(F) = U diag(phi(S)) V^T: with P = U^T dF V,
 //   (U^T dG V)_ii = sum_j dphi_i/dS_j P_jj
 //   (U^T dG V)_ij = s_ij (phi_i-phi_j)/(S_i-S_j) + a_ij (phi_i+phi_j)/(S_i+S_j), i != j,
+// (the divided difference from volume_projection_jac, exact near S_i = S_j),
 // s = sym part, a = skew part of P. The off-diagonal map is self-adjoint.
 void corotate_vjp(const double* F, double wr, double wv, const double* Z, double* Fbar) {
   double U[9], S[3], V[9];
   svd3(F, U, S, V);
   bool flip = det3(F) < 0;
-  double D[3], J[3][3];
-  volume_projection_jac(S, D, J);
+  double D[3], J[3][3], rdd[3];
+  volume_projection_jac(S, D, J, rdd);
   double phi[3] = {S[0] + D[0], S[1] + D[1], S[2] + D[2]};
   // Rotation part: md = (U V^T) * Df, Df = diag(1,1,-1) if flipped.
   // d md = dR * Df  ->  adjoint input Zr = Z * Df.
@@ -291,12 +303,7 @@ void corotate_vjp(const double* F, double wr, double wv, const double* Z, double
       double ssum = S[i] + S[j];
       double beta_r = ssum > 0 ? 2.0 / ssum : 0.0;
       double alpha_v, beta_v;
-      double sd = S[i] - S[j];
-      if (std::fabs(sd) > 1e-10 * std::max(1.0, std::fabs(S[i]))) {
-        alpha_v = (phi[i] - phi[j]) / sd;
-      } else {
-        alpha_v = J[i][i] - J[i][j];
-      }
+      alpha_v = rdd[i + j - 1];  // pair (0,1) -> 0, (0,2) -> 1, (1,2) -> 2
       beta_v = ssum > 0 ? (phi[i] + phi[j]) / ssum : 0.0;
       // Forward map on the (ij, ji) pair is [[(a+b)/2, (a-b)/2], [(a-b)/2, (a+b)/2]]
       // (symmetric), so the adjoint is the same map.
@@ -1540,6 +1547,12 @@ void orc_corotate_vjp(const double* F, double wr, double wv, const double* Z, do
 }
 void orc_svd3(const double* F, double* U, double* S, double* V) { svd3(F, U, S, V); }
 void orc_volume_projection(const double* S, double* D) { volume_projection(S, D); }
+void orc_volume_projection_jac(const double* S, double* D, double* J, double* rdd) {
+  double Jm[3][3];
+  volume_projection_jac(S, D, Jm, rdd);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) J[i * 3 + j] = Jm[i][j];
+}
 int orc_detect_plane(const double* p, const double* n, const double* x0, const double* x1, double* nout) {
   Plane pl;
   double nn = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);

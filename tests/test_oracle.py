@@ -118,6 +118,43 @@ def test_vjp_matches_finite_differences(oracle_mod):
         assert abs(fd - np.sum(Fb * dF)) < 1e-6 * max(1.0, abs(fd)), trial
 
 
+def _volproj_exact(S):
+    """The 5-step volume projection (CorotateFEMConstraint.h:366-374) in exact
+    rational arithmetic: phi = S + D."""
+    from fractions import Fraction as Fr
+    S = [Fr(float(v)) for v in S]
+    D = [Fr(0)] * 3
+    for _ in range(5):
+        a = [S[i] + D[i] for i in range(3)]
+        c = a[0] * a[1] * a[2] - 1
+        g = [a[1] * a[2], a[0] * a[2], a[0] * a[1]]
+        coef = (sum(g[i] * D[i] for i in range(3)) - c) / sum(x * x for x in g)
+        D = [coef * x for x in g]
+    return [S[i] + D[i] for i in range(3)], S
+
+
+def test_volume_projection_divided_differences_are_exact(oracle_mod):
+    """The VJP's (phi_i - phi_j)/(S_i - S_j) is carried through the projection
+    instead of formed by division: it matches the exact rational divided
+    difference also for singular values 1e-12 apart, and equals the
+    derivative limit J_ii - J_ij where two coincide."""
+    rng = np.random.default_rng(12)
+    pairs = [(0, 1), (0, 2), (1, 2)]
+    for gap in (0.3, 1e-3, 1e-6, 1e-9, 1e-12):
+        for _ in range(4):
+            S = np.sort(rng.uniform(0.6, 1.4, 3))[::-1].copy()
+            S[1] = S[0] - gap
+            D, J, r = oracle_mod.volume_projection_jac(S)
+            np.testing.assert_allclose(D, oracle_mod.volume_projection(S), rtol=0, atol=1e-15)
+            phi, Sx = _volproj_exact(S)
+            for k, (i, j) in enumerate(pairs):
+                exact = float((phi[i] - phi[j]) / (Sx[i] - Sx[j]))
+                assert abs(r[k] - exact) < 1e-13 * max(1.0, abs(exact)), (gap, k, r[k], exact)
+    S = np.array([1.2, 1.2, 0.8])
+    D, J, r = oracle_mod.volume_projection_jac(S)
+    assert abs(r[0] - (J[0, 0] - J[0, 1])) < 1e-13
+
+
 # ---------------------------------------------------------------- setup + assembly
 def small_muscle_scene(contact=False):
     x, t = scene.box_grid(4, 2, 2)
