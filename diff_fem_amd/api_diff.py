@@ -1,8 +1,10 @@
 """torch.autograd wrappers (the reference's python/api_diff.py, FP64).
 
-``sim_all`` / ``sim_act`` keep the reference's call signatures
-(api_diff.py:46, 72) but carry float64 tensors end to end (the reference
-casts results to float32, api_diff.py:16-20, 56-57).
+``sim_all`` / ``sim_act`` keep the reference's call signatures and its
+calls into the module (api_diff.py:4-70: SimAct saves the phase and the first
+mesh's activation levels and restores both before backward_act) but carry
+float64 tensors end to end (the reference casts results to float32,
+api_diff.py:16-20, 56-57).
 
 ``rollout`` is the batched device path: R rollouts of one scene as one
 autograd node whose backward is the hand-written adjoint (checkpointed
@@ -44,7 +46,7 @@ class SimAct(torch.autograd.Function):
     def forward(ctx, q, qd, jq, jqd, tau, env, is_act):
         ctx.env = env
         ctx.is_act = is_act
-        ctx.act_level = env._act.copy()
+        ctx.act_level = pd.vector2double(env.mDfobjs[0].GetActivationLevelsAggregate())
         ctx.phase = env.mPhase
         ctx.save_for_backward(q, qd, jq, jqd, tau)
         out = pd.forward_act(q.detach().cpu().numpy(), qd.detach().cpu().numpy(), jq.detach().cpu().numpy(),
@@ -56,7 +58,7 @@ class SimAct(torch.autograd.Function):
         q, qd, jq, jqd, tau = ctx.saved_tensors
         env = ctx.env
         env.mPhase = ctx.phase
-        env._act[:] = ctx.act_level
+        env.mDfobjs[0].SetActivationLevelsAggregate(pd.double2vector(ctx.act_level))
         n = lambda t: t.detach().cpu().numpy()  # noqa: E731
         out = pd.backward_act(n(q), n(qd), n(jq), n(jqd), n(tau), n(dldq), n(dldqd), n(dldjq), n(dldjqd), env,
                               ctx.is_act)

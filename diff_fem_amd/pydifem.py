@@ -22,7 +22,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from . import engine, scene
+from . import actuation, engine, scene
 
 __version__ = "dev"
 
@@ -43,35 +43,71 @@ class World:
 
 
 class Dfobj:
-    """Subset of Dfobj exposed by the reference binding (pydifem.cc:367-373)."""
+    """One mesh of the scene: the subset of Dfobj exposed by the reference
+    binding (pydifem.cc:367-373), plus its phase (Dfobj::mPhase) and actuator.
+    ``mActions`` has 4 entries per muscle for actuator type 2
+    (Dfobj.h:335-383), none otherwise."""
 
-    def __init__(self, env):
+    def __init__(self, env, info: scene.DfobjInfo):
         self._env = env
-        self.mNumUnknown = 3 * env._desc.N
-        self.mActions = np.zeros(env._desc.n_groups)
+        self._info = info
+        self.mNumUnknown = 3 * info.n_vertices
+        self.mPhase = 0
+        n_act = 4 * info.n_muscles if info.actuator_type == 2 else 0
+        self.mActions = np.zeros(n_act)
+        self._key = actuation.key_table(info.n_muscles) if n_act else None
+        g0 = info.group_offset
+        self._groups = slice(g0, g0 + info.n_sampling * info.n_muscles)
 
     def GetActivationLevelsAggregate(self):
-        return self._env._act.copy()
+        """The muscles' levels, 50 per muscle (Dfobj.h:689-707)."""
+        return self._env._act_state[self._groups].copy()
 
     def SetActivationLevelsAggregate(self, a):
+        """Sets the muscles' levels (Dfobj.h:709-720); the levels the
+        constraints use change at the next SetActions or backward_act."""
         a = np.asarray(a, np.float64).ravel()
-        if a.size != self._env._act.size:
-            raise ValueError("expected %d activation levels" % self._env._act.size)
-        self._env._act[:] = a
+        n = self._groups.stop - self._groups.start
+        if a.size != n:
+            raise ValueError("expected %d activation levels" % n)
+        self._env._act_state[self._groups] = a
+
+    def _set_actions(self, actions):
+        """Dfobj::SetActions for a muscle actuator (Dfobj.h:316-331): new
+        levels from the normalised actions at this Dfobj's phase; they become
+        both the muscles' levels and the constraints' activations. Returns the
+        tape of the gradient."""
+        env = self._env
+        lv, tape = actuation.set_actions(env._act_state[self._groups], self._key, self.mPhase, actions)
+        env._act_state[self._groups] = lv
+        env._act_seg[self._groups] = lv
+        return tape
 
 
 class Environment:
+    """Environment (Environment.h:85-225): one World, one Dfobj per mesh, the
+    phase counter ``mPhase`` (readable and writable, as the binding exposes
+    it) and the activation state of every muscle segment. Two activation
+    vectors are kept, as the reference keeps two: the muscles' levels
+    (Muscle::mActivationLevels, read and written by the *ActivationLevels*
+    calls) and the levels the muscle constraints use (set by SetActions,
+    Reset and the non-actuated branch of backward_act)."""
+
     def __init__(self, total_step: int, desc: scene.SceneDesc):
         self.mTotalStep = int(total_step)
         self._desc = desc
         self._scene = engine.Scene(desc, R=1, precision=engine.PDM_FP64, flags=engine.PDM_HOST_EARLY_EXIT,
                                    checkpoint_every=20)
-        self._act = np.zeros(desc.n_groups)
+        self._act_state = np.zeros(desc.n_groups)
+        self._act_seg = np.zeros(desc.n_groups)
         self._E = np.asarray(desc.E, np.float64).copy()
         self._nu = float(desc.nu_fwd)
         self._init_x = desc.x_rest.ravel().copy()
         self.mSoftWorld = World(self)
-        self.mDfobjs = [Dfobj(self)]
+        infos = desc.dfobjs
+        if not infos:
+            infos = [scene.DfobjInfo(vertex_offset=0, n_vertices=desc.N, tet_offset=0, n_tets=desc.T)]
+        self.mDfobjs = [Dfobj(self, info) for info in infos]
         self.mPhase = 0
         self._recorded = None
 
@@ -85,10 +121,46 @@ class Environment:
                 f.write("t %d %d %d %d\n" % tuple(t))
 
     def Reset(self):
+        """Environment::Reset (Environment.h:388-399): muscle levels and the
+        constraints' activations to 0, every Dfobj's phase to 0, the world to
+        its rest state. ``mPhase`` itself is left as it is."""
         self.mSoftWorld.mX = self._init_x.copy()
         self.mSoftWorld.mV = np.zeros_like(self._init_x)
-        self.mPhase = 0
-        self._act[:] = 0.0
+        for d in self.mDfobjs:
+            d.mPhase = 0
+        self._act_state[:] = 0.0
+        self._act_seg[:] = 0.0
+
+    def IncPhase(self):
+        """Environment::IncPhase (Environment.h:372-378)."""
+        self.mPhase += 1
+        for d in self.mDfobjs:
+            d.mPhase = self.mPhase
+
+    def SetPhase(self, phase: int):
+        """Environment::SetPhase (Environment.h:380-386): sets the Dfobjs'
+        phase only."""
+        for d in self.mDfobjs:
+            d.mPhase = int(phase)
+
+    def GetPhase(self):
+        return self.mPhase
+
+    def SetActions(self, actions):
+        """Environment::SetActions (Environment.h:476-486): consecutive
+        slices of ``actions`` go to the Dfobjs, each as long as its
+        ``mActions``. Returns [(dfobj, slice, tape)] for the gradient."""
+        actions = np.asarray(actions, np.float64).ravel()
+        need = sum(d.mActions.size for d in self.mDfobjs)
+        if actions.size < need:
+            raise ValueError("expected at least %d actions, got %d" % (need, actions.size))
+        out, k = [], 0
+        for d in self.mDfobjs:
+            n = d.mActions.size
+            if n:
+                out.append((slice(k, k + n), d._set_actions(actions[k:k + n])))
+            k += n
+        return out
 
     def _set_material(self, tau):
         tau = np.asarray(tau, np.float64).ravel()
@@ -115,15 +187,19 @@ def _as_vec(a, n):
     return a
 
 
+def _seg_act(env, steps=1):
+    return np.tile(env._act_seg, (steps, 1)) if env._desc.n_groups else None
+
+
 def forward_all(q, qd, tau, env: Environment):
-    """x=q, v=qd, mMu from tau[0:2], re-factor, mTotalStep steps
+    """x=q, v=qd, mMu = tau[0]/(1+tau[1]) on every corotated constraint,
+    re-factor, mTotalStep steps at the constraints' current activations
     (pydifem.cc:182-259). Returns [q_T, qd_T]."""
     n3 = 3 * env._desc.N
     x0, v0 = _as_vec(q, n3), _as_vec(qd, n3)
     env._set_material(tau)
     steps = env.mTotalStep
-    act = np.tile(env._act, (steps, 1)) if env._desc.n_groups else None
-    xT, vT = env._scene.forward_host(x0, v0, steps, act=act, record=True)
+    xT, vT = env._scene.forward_host(x0, v0, steps, act=_seg_act(env, steps), record=True)
     env._recorded = ("all", np.asarray(tau, np.float64).ravel().size)
     env.mSoftWorld.mX = xT[0].copy()
     env.mSoftWorld.mV = vT[0].copy()
@@ -133,7 +209,7 @@ def forward_all(q, qd, tau, env: Environment):
 def backward_all(q, qd, tau, dldq, dldqd, env: Environment):
     """Gradient of <dldq, q_T> + <dldqd, qd_T> over the last forward_all
     (pydifem.cc:261-301): [dq0, dqd0, dtau] with dtau[0] = dE, dtau[1] = dnu,
-    then the environment is reset."""
+    then Environment::Reset."""
     if env._recorded is None or env._recorded[0] != "all":
         raise RuntimeError("backward_all needs a preceding forward_all on this environment")
     n3 = 3 * env._desc.N
@@ -148,36 +224,42 @@ def backward_all(q, qd, tau, dldq, dldqd, env: Environment):
 
 
 def forward_act(q, qd, jq, jqd, tau, env: Environment, is_act: bool):
-    """One Environment::Step from (q, qd) (pydifem.cc:38-86). With is_act the
-    activations are set from tau (one level per activation group) and the
-    phase advances."""
+    """One Environment::Step from (q, qd) (pydifem.cc:38-86). With is_act,
+    SetActions(tau) at the Dfobjs' phase, then IncPhase."""
     n3 = 3 * env._desc.N
     x0, v0 = _as_vec(q, n3), _as_vec(qd, n3)
-    if is_act and env._desc.n_groups:
-        env._act[:] = np.clip(np.asarray(tau, np.float64).ravel()[: env._desc.n_groups], 0.0, 1.0)
-        env.mPhase += 1
-    act = env._act[None, :].copy() if env._desc.n_groups else None
-    xT, vT = env._scene.forward_host(x0, v0, 1, act=act, record=False)
+    if is_act:
+        env.SetActions(tau)
+        env.IncPhase()
+    xT, vT = env._scene.forward_host(x0, v0, 1, act=_seg_act(env), record=False)
     env.mSoftWorld.mX = xT[0].copy()
     env.mSoftWorld.mV = vT[0].copy()
     return [xT[0].tolist(), vT[0].tolist(), list(np.asarray(jq, float)), list(np.asarray(jqd, float))]
 
 
 def backward_act(q, qd, jq, jqd, tau, dldq, dldqd, dldjq, dldjqd, env: Environment, is_act: bool):
-    """Re-runs the step from (q, qd) and returns its gradient
-    (pydifem.cc:90-180): [dq, dqd, djq, djqd, dtau]; dtau holds the
-    activation gradient when is_act."""
+    """Gradient of one step from (q, qd) (pydifem.cc:90-180):
+    [dq, dqd, djq, djqd, dtau]. With is_act, SetActions(tau) at the Dfobjs'
+    phase, which then becomes mPhase + 1 (mPhase itself is not advanced);
+    without, the constraints take the muscles' levels. dtau is the gradient
+    through the muscle actuation (zero without is_act)."""
     n3 = 3 * env._desc.N
     x0, v0 = _as_vec(q, n3), _as_vec(qd, n3)
-    act = env._act[None, :].copy() if env._desc.n_groups else None
-    if is_act and env._desc.n_groups:
-        act = np.clip(np.asarray(tau, np.float64).ravel()[: env._desc.n_groups], 0.0, 1.0)[None, :]
-    env._scene.forward_host(x0, v0, 1, act=act, record=True)
+    tau = np.asarray(tau, np.float64).ravel()
+    tapes = []
+    if is_act:
+        cur = env.mPhase
+        tapes = env.SetActions(tau)
+        env.SetPhase(cur + 1)
+    else:
+        env._act_seg[:] = env._act_state
+    env._scene.forward_host(x0, v0, 1, act=_seg_act(env), record=True)
     g = env._scene.backward_host(_as_vec(dldq, n3)[None], _as_vec(dldqd, n3)[None])
-    dtau = np.zeros(np.asarray(tau, np.float64).size)
-    if is_act and env._desc.n_groups:
-        k = min(dtau.size, env._desc.n_groups)
-        dtau[:k] = g["act"][0, 0, :k]
+    dtau = np.zeros(tau.size)
+    if tapes:
+        gseg = g["act"][0, 0]
+        for (sl, tape), d in zip(tapes, [d for d in env.mDfobjs if d.mActions.size]):
+            dtau[sl] = actuation.set_actions_vjp(gseg[d._groups], tape)
     return [g["x0"][0].tolist(), g["v0"][0].tolist(), [0.0] * len(np.asarray(jq).ravel()),
             [0.0] * len(np.asarray(jqd).ravel()), dtau.tolist()]
 

@@ -24,6 +24,7 @@ from typing import List, Optional
 import numpy as np
 
 EDGE = 0.05
+MUSCLE_SEGMENTS = 50   # Dfobj::MakeMuscles' num_sampling (Dfobj.h:767)
 
 
 @dataclasses.dataclass
@@ -61,6 +62,9 @@ class SceneDesc:
     body_of_tet: np.ndarray = None     # (T,) int32
     body_mass: np.ndarray = None       # (B,)
     nu_fwd_tet: np.ndarray = None      # (T,)
+    # one entry per reference Dfobj (mesh) of a loaded scene, in load order:
+    # DfobjInfo records; None = one body without actuator
+    dfobjs: list = None
     name: str = "scene"
 
     def __post_init__(self):
@@ -102,6 +106,22 @@ class SceneDesc:
     @property
     def T(self) -> int:
         return self.tets.shape[0]
+
+
+@dataclasses.dataclass
+class DfobjInfo:
+    """What the host side keeps of one reference Dfobj: its vertex and tet
+    ranges in the world, its actuator type (Dfobj.h:150-160; 0 none, 2 muscle)
+    and its muscles, each with ``n_sampling`` segments whose activation groups
+    are ``group_offset + n_sampling * muscle + segment``."""
+    vertex_offset: int
+    n_vertices: int
+    tet_offset: int
+    n_tets: int
+    actuator_type: int = 0
+    n_muscles: int = 0
+    n_sampling: int = MUSCLE_SEGMENTS
+    group_offset: int = 0
 
 
 def box_grid(nx: int, ny: int, nz: int, edge: float = EDGE, origin=(0.0, 0.0, 0.0)):
@@ -209,7 +229,10 @@ def activations_sinusoid(seed: int, steps: int, n_groups: int, R: int, h: float)
 
 def load_obj_tet(path: str, scale: float = 0.05):
     """``v x y z`` / ``t i0 i1 i2 i3`` (0-based) tet OBJ, vertices scaled by
-    ``scale`` (OBJMesh.h:40-97; Dfobj.h:163-190 default Scale 0.05)."""
+    ``scale`` (OBJMesh.h:40-97; Dfobj.h:163-190 default Scale 0.05). Lines of
+    other kinds are skipped, as the reference's reader does."""
+    if not os.path.exists(path):
+        raise ValueError("Can't read file " + path)
     vs, ts = [], []
     with open(path) as f:
         for line in f:
@@ -220,48 +243,209 @@ def load_obj_tet(path: str, scale: float = 0.05):
                 vs.append([float(p[1]), float(p[2]), float(p[3])])
             elif p[0] == "t":
                 ts.append([int(p[1]), int(p[2]), int(p[3]), int(p[4])])
-    return np.asarray(vs, dtype=np.float64) * scale, np.asarray(ts, dtype=np.int32)
+    x = np.asarray(vs, dtype=np.float64).reshape(-1, 3) * scale
+    t = np.asarray(ts, dtype=np.int32).reshape(-1, 4)
+    if t.size and (t.min() < 0 or t.max() >= x.shape[0]):
+        raise ValueError("%s: tet vertex index out of range" % path)
+    return x, t
+
+
+def load_muscle_file(path: str, scale: float = 0.05) -> List[np.ndarray]:
+    """Fibre polylines of a ``.muscle`` file (Dfobj::MakeMuscles,
+    Dfobj.h:734-797): one point per line, ``fibre segment x y z``; a fibre
+    ends where the fibre index changes; points are scaled like the mesh
+    (``mScalingMatrix``). Blank lines are skipped. The reader starts at fibre
+    index 0, so a file whose first fibre has another index would give an empty
+    first fibre (undefined in the reference): that is an error here."""
+    if not os.path.exists(path):
+        raise ValueError("Muscle file doesn't exist: " + path)
+    fibres: List[list] = []
+    cur, pts = 0, []
+    with open(path) as f:
+        for line in f:
+            p = line.split()
+            if not p:
+                continue
+            if len(p) < 5:
+                raise ValueError("%s: malformed muscle line %r" % (path, line))
+            idx = int(p[0])
+            if idx != cur:
+                if not pts:
+                    raise ValueError("%s: fibre %d has no points" % (path, cur))
+                fibres.append(pts)
+                cur, pts = idx, []
+            pts.append([float(p[2]), float(p[3]), float(p[4])])
+    if pts:
+        fibres.append(pts)
+    return [np.asarray(q, dtype=np.float64) * scale for q in fibres]
+
+
+def resample_fibre(points: np.ndarray, nseg: int = MUSCLE_SEGMENTS):
+    """Segments of one fibre polyline (Muscle::Muscle, Muscle.h:59-107): walk
+    the polyline accumulating edge lengths; when the sum exceeds
+    arc_length / nseg, close a segment from the last start point to the
+    current point and restart the sum there. The last end point is replaced by
+    the polyline's last point.
+
+    The reference loop also adds the edge that starts at the last point, which
+    reads one point past the end (Muscle.h:92-93). Here that step always
+    closes a segment ending at the last point: it is the one reading under
+    which a fibre such as the octopus's (99 evenly spaced points) has the
+    ``nseg`` segments that Muscle::Initialize reads; any other reading leaves
+    it one short, and Initialize then reads past the segment list. A fibre
+    with fewer than ``nseg`` segments is rejected; extra segments are unused,
+    as Initialize uses the first ``nseg``."""
+    pts = np.asarray(points, dtype=np.float64).reshape(-1, 3)
+    n = pts.shape[0]
+    if n < 2:
+        raise ValueError("a muscle fibre needs at least two points")
+    edge = np.linalg.norm(pts[1:] - pts[:-1], axis=1)
+    arc = 0.0
+    for e in edge:
+        arc += e
+    samp = arc / nseg
+    starts, ends = [], []
+    acc, si = 0.0, 0
+    for i in range(n - 1):
+        acc += edge[i]
+        if acc > samp:
+            starts.append(pts[si])
+            ends.append(pts[i])
+            acc, si = 0.0, i
+    starts.append(pts[si])
+    ends.append(pts[n - 1])
+    if len(starts) < nseg:
+        raise ValueError("muscle fibre resamples to %d segments, fewer than %d" % (len(starts), nseg))
+    return np.asarray(starts[:nseg]), np.asarray(ends[:nseg])
+
+
+def _unsupported(what: str):
+    raise NotImplementedError("scene feature not supported by this engine: " + what)
+
+
+def _json_len(obj: dict, key: str) -> int:
+    """Size of an optional JSON array: a missing key reads as empty (the
+    reference is built with NDEBUG, and rapidjson's null value then has size 0,
+    document.h:1163-1177)."""
+    v = obj.get(key)
+    return len(v) if isinstance(v, list) else 0
 
 
 def load_json_scene(config_path: str, data_root: Optional[str] = None) -> SceneDesc:
-    """Reference JSON scene (first mesh), keys and defaults of
-    Environment.h:109-181 and Dfobj.h:142-250. ``config_path`` is resolved
-    against ``data_root`` (the reference prefixes SOFTCON_DIR, Environment.h:90);
-    mesh files are resolved against ``<data_root>/data``."""
+    """Reference JSON scene (Environment.h:85-225, Dfobj.h:142-314,
+    Dfobj.h:442-611): the "Parameters" block and every "Meshes" entry, each
+    mesh a body of the one world with its own mass, material, attachments and
+    muscle file. Missing keys take the reference's defaults (Dfobj(5E5, 2E5,
+    0.4), Mass 10, Scale 0.05, Simulation HZ 200, Max Iteration 30, damping
+    0.9999, gravity 0); a missing "Attach Stiffness" next to "Attach Verts"
+    reads 0, as a null rapidjson value does in the reference's NDEBUG build.
+
+    ``config_path`` is resolved against ``data_root`` (the reference prefixes
+    SOFTCON_DIR, Environment.h:90), mesh and muscle files against
+    ``<data_root>/data`` (Dfobj.h:192-199, 724-731), falling back to the
+    config's own folder. Scene features outside the projective-dynamics path
+    (cloth, obstacle meshes, rigid bones, pneumatic volume muscles, self
+    collision, underwater drag, skeleton or pneumatic actuators) raise
+    NotImplementedError."""
     root = data_root if data_root is not None else os.environ.get("PDM_DATA_ROOT", "")
     path = os.path.join(root, config_path.lstrip("/")) if root else config_path
     if not os.path.exists(path):
         raise ValueError("Couldn't open file " + path)
     with open(path) as f:
-        js = json.load(f)
+        text = f.read()
+    try:
+        js = json.loads(text)
+    except json.JSONDecodeError:
+        js = None
+    if not isinstance(js, dict):
+        raise ValueError("JSON reader - Error : root must be an object")
     para = js["Parameters"][0]
     hz = int(para.get("Simulation HZ", 200))
     max_iter = int(para.get("Max Iteration", 30))
     damping = float(para.get("Damping coefficient", 0.9999))
     contact = bool(para.get("Handle collision", False))
-    gravity = tuple(float(g) for g in para.get("Gravity", [0.0, 0.0, 0.0]))
+    g = para.get("Gravity", [0.0, 0.0, 0.0])
+    if len(g) < 3:
+        raise ValueError("The Gravity vector should have the length of 3 but got length %d" % len(g))
+    gravity = tuple(float(v) for v in g[:3])
     alpha = float(para.get("World Alpha", 1.0))
-    meshes = js.get("Meshes", [])
-    if len(meshes) != 1:
-        raise ValueError("exactly one mesh per scene is supported (got %d)" % len(meshes))
-    m = meshes[0]
-    scale = float(m.get("Scale", 0.05))
-    data_dir = os.path.join(root, "data") if root else os.path.dirname(path)
-    mesh_path = os.path.join(data_dir, m["Obj filename"])
-    if not os.path.exists(mesh_path):
-        mesh_path = os.path.join(os.path.dirname(path), os.path.basename(m["Obj filename"]))
-    x, t = load_obj_tet(mesh_path, scale)
-    T = t.shape[0]
-    E = float(m.get("Youngs Modulus", 2e5))
-    nu = float(m.get("Poisson Ratio", 0.4))
-    attach = np.asarray(sorted(set(int(a) for a in m.get("Attach Verts", []))), dtype=np.int32)
-    return SceneDesc(x_rest=x, tets=t, total_mass=float(m.get("Mass", 10.0)),
-                     mass_same=bool(m.get("Is Mass Same", True)), E=np.full(T, E), nu_fwd=nu,
-                     nu_ctor=np.full(T, nu), attach_idx=attach, attach_w=float(m.get("Attach Stiffness", 0.0)),
-                     k_muscle=float(m.get("Muscle Stiffness", 5e5)), h=1.0 / hz, max_iter=max_iter,
-                     damping=damping, gravity=gravity, gravity_on=bool(m.get("Enable Gravity", 1)),
-                     contact=contact, max_iter1=max_iter, alpha=alpha, mu=1.0,
-                     name=os.path.splitext(os.path.basename(config_path))[0])
+    if para.get("Self-collision", False):
+        _unsupported("Self-collision")
+    if int(para.get("Is underwater", 0)):
+        _unsupported("Is underwater (drag forces)")
+    if _json_len(js, "Clothes"):
+        _unsupported("Clothes")
+    if _json_len(js, "Obstacles"):
+        _unsupported("Obstacles (triangle-mesh obstacles)")
+    meshes = js.get("Meshes") or []
+    if not meshes:
+        raise ValueError("scene has no meshes")
+    cfg_dir = os.path.dirname(path)
+    data_dir = os.path.join(root, "data") if root else cfg_dir
+
+    def resolve(name):
+        p = os.path.join(data_dir, name)
+        return p if os.path.exists(p) else os.path.join(cfg_dir, os.path.basename(name))
+
+    bodies, infos = [], []
+    nv = nt = ng = 0
+    for k, m in enumerate(meshes):
+        if "Obj filename" not in m:
+            raise ValueError("mesh %d has no \"Obj filename\"" % k)
+        act_type = int(m.get("Actuator type", 0))
+        if act_type not in (0, 2):
+            _unsupported("Actuator type %d" % act_type)
+        if _json_len(m, "Bones"):
+            _unsupported("Bones (rigid links)")
+        if _json_len(m, "Volume Verts"):
+            _unsupported("Volume Verts (pneumatic constraints)")
+        scale = float(m.get("Scale", 0.05))
+        if not scale > 0:
+            raise ValueError("mesh %d: Scale must be positive" % k)
+        x, t = load_obj_tet(resolve(m["Obj filename"]), scale)
+        T = t.shape[0]
+        E = float(m.get("Youngs Modulus", 2e5))
+        nu = float(m.get("Poisson Ratio", 0.4))
+        kw = {}
+        n_mus = 0
+        if "Muscle filename" in m:
+            fibres = load_muscle_file(resolve(m["Muscle filename"]), scale)
+            s0, s1, grp = [], [], []
+            for i, fpts in enumerate(fibres):
+                a, b = resample_fibre(fpts)
+                s0.append(a)
+                s1.append(b)
+                grp.append(np.arange(MUSCLE_SEGMENTS, dtype=np.int32) + MUSCLE_SEGMENTS * i)
+            n_mus = len(fibres)
+            if n_mus:
+                kw = dict(seg_start=np.concatenate(s0), seg_end=np.concatenate(s1), seg_group=np.concatenate(grp),
+                          n_groups=MUSCLE_SEGMENTS * n_mus)
+        attach = np.zeros(0, dtype=np.int32)
+        attach_w = 0.0
+        if "Attach Verts" in m:
+            attach = np.asarray(sorted(set(int(a) for a in m["Attach Verts"])), dtype=np.int32)
+            if attach.size and (attach.min() < 0 or attach.max() >= x.shape[0]):
+                raise ValueError("mesh %d: attach vertex out of range" % k)
+            attach_w = float(m.get("Attach Stiffness", 0.0) or 0.0)
+        bodies.append(SceneDesc(x_rest=x, tets=t, total_mass=float(m.get("Mass", 10.0)),
+                                mass_same=bool(m.get("Is Mass Same", True)), E=np.full(T, E), nu_fwd=nu,
+                                nu_ctor=np.full(T, nu), attach_idx=attach, attach_w=attach_w,
+                                k_muscle=float(m.get("Muscle Stiffness", 5e5)), h=1.0 / hz, max_iter=max_iter,
+                                damping=damping, gravity=gravity, gravity_on=bool(int(m.get("Enable Gravity", 1))),
+                                contact=contact, max_iter1=max_iter, alpha=alpha, mu=1.0, **kw))
+        infos.append(DfobjInfo(vertex_offset=nv, n_vertices=x.shape[0], tet_offset=nt, n_tets=T,
+                               actuator_type=act_type, n_muscles=n_mus, group_offset=ng))
+        nv += x.shape[0]
+        nt += T
+        ng += MUSCLE_SEGMENTS * n_mus
+    name = os.path.splitext(os.path.basename(config_path))[0]
+    if len(bodies) == 1:
+        d = bodies[0]
+        d.name = name
+    else:
+        d = pack_scenes(bodies, name=name, translate=False)
+    d.dfobjs = infos
+    return d
 
 
 def config5_meshes(seed: int = 5, n_meshes: int = 64, tmin: int = 2000, tmax: int = 60000) -> List[SceneDesc]:
@@ -297,24 +481,47 @@ def config5_meshes(seed: int = 5, n_meshes: int = 64, tmin: int = 2000, tmax: in
     return out
 
 
-def pack_scenes(descs: List[SceneDesc], gap: float = 0.4, name: str = "packed") -> SceneDesc:
+def pack_scenes(descs: List[SceneDesc], gap: float = 0.4, name: str = "packed", translate: bool = True) -> SceneDesc:
     """Several meshes as the bodies of one world (one Dfobj per mesh, as the
-    reference's "Meshes" list loads them, Environment.h:109-210): meshes are
-    translated onto a square grid of cells in the x-y plane, ``gap`` apart;
-    each keeps its own mass, E, nu and muscle fibres (activation groups are
-    renumbered consecutively); obstacles are the union of the meshes'
-    (translated with their mesh). Step parameters come from the first mesh and
-    must agree across meshes."""
+    reference's "Meshes" list loads them, Environment.h:109-210): with
+    ``translate`` the meshes are moved onto a square grid of cells in the x-y
+    plane, ``gap`` apart (without it they stay where they are, as the
+    reference places every mesh at offset 0); each keeps its own mass, E, nu
+    and muscle fibres (activation groups are renumbered consecutively);
+    obstacles are the union of the meshes' (moved with their mesh). Step
+    parameters, the attachment stiffness of bodies with attachments and the
+    muscle stiffness of bodies with muscles must agree across meshes, as the
+    engine holds one value of each.
+
+    A fibre selects the tets it crosses or passes within about 8 mm of
+    (Muscle.h:109-215), and the reference only searches its own mesh. Here the
+    search covers the world, so untranslated bodies with muscles must stay
+    1 cm apart (bounding boxes), which makes the two searches agree."""
     if not descs:
         raise ValueError("no meshes to pack")
     d0 = descs[0]
     for d in descs[1:]:
-        for k in ("h", "max_iter", "damping", "gravity_on", "contact", "max_iter1", "alpha", "mu", "k_muscle",
-                  "mass_same"):
+        for k in ("h", "max_iter", "damping", "gravity_on", "contact", "max_iter1", "alpha", "mu", "mass_same"):
             if getattr(d, k) != getattr(d0, k):
                 raise ValueError("packed meshes disagree on %s" % k)
         if tuple(d.gravity) != tuple(d0.gravity):
             raise ValueError("packed meshes disagree on gravity")
+    att_w = {d.attach_w for d in descs if d.attach_idx.size}
+    k_mus = {d.k_muscle for d in descs if d.seg_group.size}
+    if len(att_w) > 1:
+        raise ValueError("packed meshes disagree on the attachment stiffness")
+    if len(k_mus) > 1:
+        raise ValueError("packed meshes disagree on the muscle stiffness")
+    if not translate:
+        mus = [d for d in descs if d.seg_group.size]
+        for d in mus:
+            for e in descs:
+                if e is d:
+                    continue
+                lo = np.maximum(d.x_rest.min(0), e.x_rest.min(0))
+                hi = np.minimum(d.x_rest.max(0), e.x_rest.max(0))
+                if np.all(lo - hi < 0.01):
+                    raise NotImplementedError("a body with muscles lies within 1 cm of another body")
     ext = np.array([d.x_rest.max(0) - d.x_rest.min(0) for d in descs])
     cell = ext[:, :2].max(0) + gap
     ncol = int(math.ceil(math.sqrt(len(descs))))
@@ -323,7 +530,9 @@ def pack_scenes(descs: List[SceneDesc], gap: float = 0.4, name: str = "packed") 
     nv = 0
     ng = 0
     for b, d in enumerate(descs):
-        off = np.array([(b % ncol) * cell[0], (b // ncol) * cell[1], 0.0]) - np.array([*d.x_rest.min(0)[:2], 0.0])
+        off = np.zeros(3)
+        if translate:
+            off = np.array([(b % ncol) * cell[0], (b // ncol) * cell[1], 0.0]) - np.array([*d.x_rest.min(0)[:2], 0.0])
         xs.append(d.x_rest + off)
         ts.append(d.tets + nv)
         bodies.append(np.full(d.T, b, dtype=np.int32))
@@ -346,7 +555,8 @@ def pack_scenes(descs: List[SceneDesc], gap: float = 0.4, name: str = "packed") 
         kw = dict(seg_start=np.concatenate(s0), seg_end=np.concatenate(s1), seg_group=np.concatenate(sg), n_groups=ng)
     return SceneDesc(x_rest=np.concatenate(xs), tets=np.concatenate(ts), total_mass=float(sum(mass)),
                      mass_same=d0.mass_same, E=np.concatenate(E), nu_fwd=d0.nu_fwd, nu_ctor=np.concatenate(nuc),
-                     attach_idx=np.concatenate(att), attach_w=d0.attach_w, k_muscle=d0.k_muscle, h=d0.h,
+                     attach_idx=np.concatenate(att), attach_w=att_w.pop() if att_w else d0.attach_w,
+                     k_muscle=k_mus.pop() if k_mus else d0.k_muscle, h=d0.h,
                      max_iter=d0.max_iter, damping=d0.damping, gravity=d0.gravity, gravity_on=d0.gravity_on,
                      contact=d0.contact, max_iter1=d0.max_iter1, alpha=d0.alpha, mu=d0.mu, planes=planes,
                      spheres=np.array(spheres) if spheres else None, body_of_tet=np.concatenate(bodies),

@@ -1,0 +1,284 @@
+"""Scene loading (JSON / OBJ / muscle files) and muscle actuation, on the CPU.
+
+Pins:
+  * loader: the fixture scenes' sizes and keys, the reference's defaults for
+    missing keys (Dfobj.h:124-130, 161-190; Environment.h:109-181), its errors;
+  * fibre resampling (Muscle.h:59-107): closed form for an evenly spaced
+    polyline (the octopus fibres), contiguity, rejection of short fibres;
+  * key table (Dfobj.h:628-665): closed form of the first two columns;
+  * actuation forward (Muscle.h:224-258, Normalizer.h:93-114): special cases
+    with closed forms; its reverse: central differences;
+  * pydifem's phase and activation bookkeeping (pydifem.cc:38-180,
+    Environment.h:372-399, api_diff.py:4-44) on a stub engine that records
+    the activations each step receives.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from diff_fem_amd import actuation, scene
+
+ROOT = os.path.join(os.path.dirname(__file__), "golden", "scenes")
+
+
+def test_bridgesus_scene_loads():
+    d = scene.load_json_scene("/data/bridgesus/bridgesus.json", ROOT)
+    assert (d.N, d.T) == (212, 668)
+    raw = np.loadtxt(os.path.join(ROOT, "data/bridgesus/bridgesus200_tet_cen.obj"), usecols=(1, 2, 3),
+                     comments=None, converters=None, max_rows=212)
+    np.testing.assert_array_equal(d.x_rest, raw * 0.05)
+    with open(os.path.join(ROOT, "data/bridgesus/bridgesus.json")) as f:
+        verts = json.load(f)["Meshes"][0]["Attach Verts"]
+    assert d.attach_idx.size == 65 and list(d.attach_idx) == sorted(set(verts)) and d.attach_w == 1e20
+    assert d.gravity == (0.0, -100.0, 0.0) and d.h == 1.0 / 200 and d.max_iter == 30 and d.max_iter1 == 30
+    assert d.contact and d.planes.shape[0] == 0 and d.spheres.shape[0] == 0
+    assert np.all(d.E == 2e5) and np.all(d.nu_ctor == 0.4) and d.nu_fwd == 0.4
+    assert d.total_mass == 10.0 and d.mass_same and d.gravity_on and d.k_muscle == 5e5
+    assert len(d.dfobjs) == 1 and d.dfobjs[0].actuator_type == 0
+
+
+def _write_scene(tmp_path, para, meshes, extra=None, text=None):
+    data = tmp_path / "data"
+    data.mkdir(exist_ok=True)
+    x, t = scene.box_grid(2, 1, 1, edge=1.0)
+    with open(data / "box.obj", "w") as f:
+        for p in x:
+            f.write("v %.17g %.17g %.17g\n" % tuple(p))
+        f.write("# comment\nvn 0 0 1\n")
+        for q in t:
+            f.write("t %d %d %d %d\n" % tuple(q))
+    js = {"Parameters": [para], "Meshes": meshes}
+    js.update(extra or {})
+    with open(tmp_path / "s.json", "w") as f:
+        f.write(text if text is not None else json.dumps(js))
+    return x, t
+
+
+def test_missing_keys_take_reference_defaults(tmp_path):
+    x, t = _write_scene(tmp_path, {}, [{"Obj filename": "box.obj", "Attach Verts": [3, 1, 3]}])
+    d = scene.load_json_scene("s.json", str(tmp_path))
+    np.testing.assert_array_equal(d.x_rest, x * 0.05)
+    np.testing.assert_array_equal(d.tets, t)
+    assert d.h == 1.0 / 200 and d.max_iter == 30 and d.damping == 0.9999 and d.gravity == (0.0, 0.0, 0.0)
+    assert not d.contact and d.alpha == 1.0 and d.mu == 1.0
+    assert d.total_mass == 10.0 and d.mass_same and d.gravity_on
+    assert np.all(d.E == 2e5) and d.nu_fwd == 0.4 and d.k_muscle == 5e5
+    # "Attach Verts" without "Attach Stiffness": a null value reads 0
+    assert list(d.attach_idx) == [1, 3] and d.attach_w == 0.0
+
+
+def test_two_meshes_are_two_bodies(tmp_path):
+    m = {"Obj filename": "box.obj", "Scale": 1.0}
+    _write_scene(tmp_path, {"Simulation HZ": 100, "Max Iteration": 10, "Gravity": [0, -9.8, 0]},
+                 [dict(m, Mass=2.0, **{"Youngs Modulus": 1e5}), dict(m, Mass=3.0, **{"Poisson Ratio": 0.3})])
+    d = scene.load_json_scene("s.json", str(tmp_path))
+    assert d.N == 2 * 12 and d.T == 2 * 12
+    np.testing.assert_array_equal(d.x_rest[:12], d.x_rest[12:])       # both at offset 0
+    np.testing.assert_array_equal(d.tets[12:], d.tets[:12] + 12)
+    assert list(d.body_mass) == [2.0, 3.0] and list(np.unique(d.body_of_tet)) == [0, 1]
+    assert np.all(d.E[:12] == 1e5) and np.all(d.E[12:] == 2e5)
+    assert np.all(d.nu_ctor[:12] == 0.4) and np.all(d.nu_ctor[12:] == 0.3)
+    assert [(i.vertex_offset, i.tet_offset) for i in d.dfobjs] == [(0, 0), (12, 12)]
+    assert d.h == 0.01 and d.max_iter == 10 and d.gravity == (0.0, -9.8, 0.0)
+
+
+@pytest.mark.parametrize("para,mesh,extra,err", [
+    ({"Self-collision": True}, {}, None, NotImplementedError),
+    ({"Is underwater": 1}, {}, None, NotImplementedError),
+    ({}, {}, {"Obstacles": [{"Obj filename": "box.obj"}]}, NotImplementedError),
+    ({}, {}, {"Clothes": [{"Obj filename": "box.obj"}]}, NotImplementedError),
+    ({}, {"Bones": [{"Parent ID": -1}]}, None, NotImplementedError),
+    ({}, {"Volume Verts": [[0, 1]]}, None, NotImplementedError),
+    ({}, {"Actuator type": 3}, None, NotImplementedError),
+    ({}, {"Actuator type": 1}, None, NotImplementedError),
+    ({"Gravity": [0, 1]}, {}, None, ValueError),
+])
+def test_unsupported_and_invalid_scenes_raise(tmp_path, para, mesh, extra, err):
+    _write_scene(tmp_path, para, [dict({"Obj filename": "box.obj"}, **mesh)], extra)
+    with pytest.raises(err):
+        scene.load_json_scene("s.json", str(tmp_path))
+
+
+def test_unreadable_scenes_raise_value_error(tmp_path):
+    with pytest.raises(ValueError, match="Couldn't open file"):
+        scene.load_json_scene("missing.json", str(tmp_path))
+    _write_scene(tmp_path, {}, [], text='{"Parameters": [{ Simulation HZ: 10 }]}')
+    with pytest.raises(ValueError, match="root must be an object"):
+        scene.load_json_scene("s.json", str(tmp_path))
+    _write_scene(tmp_path, {}, [{"Scale": 1.0}])
+    with pytest.raises(ValueError, match="Obj filename"):
+        scene.load_json_scene("s.json", str(tmp_path))
+
+
+def test_octopus_fibres_resample_to_fifty_segments():
+    """Evenly spaced fibres of 99 points: the running length first exceeds
+    arc/50 after two edges, so segment 0 is (p0, p1), segment k is
+    (p_{2k-1}, p_{2k+1}) for k = 1..48, and segment 49 is (p97, p98)."""
+    fibres = scene.load_muscle_file(os.path.join(ROOT, "data/octopus.muscle"), 0.05)
+    assert len(fibres) == 16 and all(f.shape == (99, 3) for f in fibres)
+    raw = np.loadtxt(os.path.join(ROOT, "data/octopus.muscle"))
+    np.testing.assert_array_equal(fibres[3], raw[raw[:, 0] == 3][:, 2:] * 0.05)
+    for p in fibres:
+        a, b = scene.resample_fibre(p)
+        idx_s = [0] + [2 * k - 1 for k in range(1, 49)] + [97]
+        idx_e = [1] + [2 * k + 1 for k in range(1, 49)] + [98]
+        np.testing.assert_array_equal(a, p[idx_s])
+        np.testing.assert_array_equal(b, p[idx_e])
+        np.testing.assert_array_equal(a[1:], b[:-1])                   # contiguous
+
+
+def test_resample_rejects_short_fibres_and_bad_files(tmp_path):
+    p = np.stack([np.linspace(0, 1, 30), np.zeros(30), np.zeros(30)], 1)
+    with pytest.raises(ValueError, match="fewer than 50"):
+        scene.resample_fibre(p)
+    f = tmp_path / "m.muscle"
+    f.write_text("1 0 0 0 0\n1 1 1 0 0\n")
+    with pytest.raises(ValueError, match="no points"):
+        scene.load_muscle_file(str(f))
+
+
+def test_octopus_scene_groups():
+    d = scene.load_json_scene("/data/octopus/octopus_dry.json", ROOT)
+    assert (d.N, d.T) == (977, 2160)
+    assert d.n_groups == 800 and d.seg_start.shape == (800, 3)
+    np.testing.assert_array_equal(d.seg_group, np.arange(800))
+    info = d.dfobjs[0]
+    assert info.actuator_type == 2 and info.n_muscles == 16
+    assert d.gravity == (0.0, 0.0, 0.0) and not d.contact
+    with pytest.raises(NotImplementedError, match="Is underwater"):
+        scene.load_json_scene("/data/octopus/octopus.json", ROOT)
+
+
+def test_key_table_closed_form():
+    k = actuation.key_table(16)
+    c0 = np.concatenate([0.025 * np.arange(1, 21), [0.375, 0.25, 0.125], np.zeros(27)])
+    c1 = np.concatenate([np.zeros(20), 0.03125 * np.arange(1, 11), [0.1875, 0.0625], np.zeros(18)])
+    np.testing.assert_allclose(k[:, 0], c0, rtol=0, atol=1e-15)
+    np.testing.assert_allclose(k[:, 1], c1, rtol=0, atol=1e-15)
+    np.testing.assert_array_equal(k[:, 2::2], np.repeat(k[:, :1], 7, 1))
+    np.testing.assert_array_equal(k[:, 3::2], np.repeat(k[:, 1:2], 7, 1))
+    with pytest.raises(NotImplementedError):
+        actuation.key_table(8)
+
+
+def test_norm_to_real_maps_and_clamps():
+    real, mask = actuation.norm_to_real(np.array([-5, -5, -5, -5, 5, 5, 5, 5, 0, 0, 0, 0, 7, -6, 2.5, -2.5]))
+    np.testing.assert_allclose(real[0], [-0.25, 0, 0, 0])
+    np.testing.assert_allclose(real[1], [0.25, 1, 1, 10])
+    np.testing.assert_allclose(real[2], [0, 0.5, 0.5, 5])
+    np.testing.assert_allclose(real[3], [0.25, 0, 0.75, 2.5])
+    assert mask[:3].all() and list(mask[3]) == [False, False, True, True]
+
+
+def test_propagate_special_cases():
+    rng = np.random.default_rng(3)
+    lv = rng.uniform(0, 1, 50)
+    # no pass: every level becomes 0
+    out, _ = actuation.propagate(lv, 0.3, np.array([0.1, 0.5, 0.0, 0.0]))
+    np.testing.assert_array_equal(out, np.zeros(50))
+    # alpha = 0, one pass: the head takes key + delta, the rest hold
+    out, _ = actuation.propagate(lv, 0.3, np.array([0.1, 0.0, 0.0, 1.0]))
+    np.testing.assert_allclose(out, np.concatenate([[0.4], lv[1:]]), rtol=0, atol=1e-15)
+    # alpha = 1, k passes (speed 2.5 -> 3): a shift by three
+    out, _ = actuation.propagate(lv, 0.3, np.array([0.1, 1.0, 0.0, 2.5]))
+    np.testing.assert_allclose(out, np.concatenate([[0.4] * 3, lv[:-3]]), rtol=0, atol=1e-15)
+    # clamp to [0, 1]
+    out, _ = actuation.propagate(lv, 1.0, np.array([0.25, 1.0, 0.0, 1.0]))
+    assert out[0] == 1.0
+    out, _ = actuation.propagate(lv, 0.0, np.array([-0.25, 1.0, 0.0, 1.0]))
+    assert out[0] == 0.0
+
+
+def test_set_actions_vjp_matches_central_differences():
+    key = actuation.key_table(16)
+    rng = np.random.default_rng(7)
+    lv = rng.uniform(0.05, 0.6, 800)
+    a = rng.uniform(-3.5, 3.5, 64)
+    a[3::4] = rng.choice([-3.8, -1.9, 0.7, 2.3], 16)   # speeds 1.2, 3.1, 5.7, 7.3: away from integers
+    g = rng.standard_normal(800)
+    for phase in (0, 7, 23):
+        out, tape = actuation.set_actions(lv, key, phase, a)
+        ga = actuation.set_actions_vjp(g, tape)
+        eps = 1e-6
+        fd = np.zeros(64)
+        for i in range(64):
+            e = np.zeros(64)
+            e[i] = eps
+            fp = actuation.set_actions(lv, key, phase, a + e)[0]
+            fm = actuation.set_actions(lv, key, phase, a - e)[0]
+            fd[i] = np.dot(g, fp - fm) / (2 * eps)
+        np.testing.assert_allclose(ga, fd, rtol=1e-6, atol=1e-8)
+        assert np.all(ga[3::4] == 0.0)          # speed: no gradient
+        assert np.all(ga[2::4] == 0.0)          # the unused third action
+
+
+class _StubScene:
+    """Stands in for engine.Scene: records the activations each call gets;
+    positions pass through; the backward returns unit activation gradients."""
+    log = []
+
+    def __init__(self, desc, **kw):
+        self.G = desc.n_groups
+
+    def set_material(self, E=None, nu_fwd=None):
+        pass
+
+    def forward_host(self, x0, v0, steps, act=None, record=False):
+        _StubScene.log.append(None if act is None else np.array(act))
+        return np.asarray(x0)[None], np.asarray(v0)[None]
+
+    def backward_host(self, gx, gv):
+        return dict(x0=gx, v0=gv, E=np.zeros(1), nu=0.0, act=np.ones((1, 1, self.G)))
+
+
+def test_pydifem_phase_and_activation_bookkeeping(monkeypatch):
+    from diff_fem_amd import api_diff, engine, pydifem as pd
+    import torch
+    monkeypatch.setattr(engine, "Scene", _StubScene)
+    _StubScene.log.clear()
+    env = pd.init_env(3, "/data/octopus/octopus_dry.json", ROOT)
+    d = env.mDfobjs[0]
+    assert d.mActions.size == 64 and env.mPhase == 0
+    key = actuation.key_table(16)
+    rng = np.random.default_rng(1)
+    taus = rng.uniform(-4, 4, (3, 64))
+    q = env.mSoftWorld.mX.copy()
+    qd = np.zeros_like(q)
+    z = np.zeros(0)
+    # forward: SetActions at the Dfobj phase, then IncPhase
+    lv1 = actuation.set_actions(np.zeros(800), key, 0, taus[0])[0]
+    pd.forward_act(q, qd, z, z, taus[0], env, True)
+    np.testing.assert_array_equal(_StubScene.log[-1][0], lv1)
+    assert env.mPhase == 1 and d.mPhase == 1
+    lv2 = actuation.set_actions(lv1, key, 1, taus[1])[0]
+    pd.forward_act(q, qd, z, z, taus[1], env, True)
+    np.testing.assert_array_equal(_StubScene.log[-1][0], lv2)
+    # not actuated: the constraints keep their activations, the phase stays
+    d.SetActivationLevelsAggregate(np.full(800, 0.5))
+    pd.forward_act(q, qd, z, z, taus[2], env, False)
+    np.testing.assert_array_equal(_StubScene.log[-1][0], lv2)
+    assert env.mPhase == 2
+    np.testing.assert_array_equal(d.GetActivationLevelsAggregate(), np.full(800, 0.5))
+    # backward, not actuated: the constraints take the muscles' levels
+    pd.backward_act(q, qd, z, z, taus[2], q, qd, z, z, env, False)
+    np.testing.assert_array_equal(_StubScene.log[-1][0], np.full(800, 0.5))
+    # backward as SimAct does it: phase and levels restored first; SetActions
+    # then runs at the Dfobj phase (2 here), which becomes mPhase + 1
+    env.mPhase = 0
+    d.SetActivationLevelsAggregate(np.zeros(800))
+    out = pd.backward_act(q, qd, z, z, taus[0], q, qd, z, z, env, True)
+    lv_b, tape = actuation.set_actions(np.zeros(800), key, 2, taus[0])
+    np.testing.assert_array_equal(_StubScene.log[-1][0], lv_b)
+    assert d.mPhase == 1 and env.mPhase == 0
+    np.testing.assert_allclose(out[4], actuation.set_actions_vjp(np.ones(800), tape), rtol=0, atol=0)
+    # SimAct saves and restores the same state around the module calls
+    env.mPhase = 0
+    env.Reset()
+    assert d.mPhase == 0 and not env._act_seg.any()
+    t = lambda a: torch.tensor(a, dtype=torch.float64, requires_grad=True)  # noqa: E731
+    qq, vv, _, _ = api_diff.sim_act(t(q), t(qd), t(z), t(z), t(taus[0]), env, True)
+    np.testing.assert_array_equal(_StubScene.log[-1][0], lv1)
+    (qq.sum() + vv.sum()).backward()
+    np.testing.assert_array_equal(_StubScene.log[-1][0], actuation.set_actions(np.zeros(800), key, 1, taus[0])[0])
