@@ -92,22 +92,110 @@ __global__ void k_spmv(const int* __restrict__ ptr, const int* __restrict__ col,
 
 // Start of a step: gravity kick (Environment.h:236-245), qn = x + h v*
 // (World.h:692-699) and the iteration-invariant rhs c = M v*/h - L x + W p.
+// Underwater (FF non-null) the kick is v* = v + h M^-1 f with f the vertex's
+// sum of its faces' drag forces in face order (Environment.h:246-250,
+// World.h:692), and there is no gravity.
 __global__ void k_step_prep(const double* __restrict__ X, const double* __restrict__ V, const double* __restrict__ Lx,
                             const double* __restrict__ mass, const double* __restrict__ attw,
-                            const double* __restrict__ xrest, double* __restrict__ Xn, double* __restrict__ Vs,
-                            double* __restrict__ Cc, double* __restrict__ Xit, int N, int C, double h, double g0,
-                            double g1, double g2, int gravity_on) {
+                            const double* __restrict__ xrest, double* __restrict__ Xn, double* __restrict__ Cc,
+                            double* __restrict__ Xit, int N, int C, double h, double g0, double g1, double g2,
+                            int gravity_on, const double* __restrict__ FF, const int* __restrict__ vfptr,
+                            const int* __restrict__ vfq) {
   long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (idx >= (long long)N * C) return;
   int v = (int)(idx / C);
-  int c = (int)(idx % C) % 3;
-  double g = c == 0 ? g0 : (c == 1 ? g1 : g2);
-  double vs = V[idx] + (gravity_on ? g * h : 0.0);
+  int j = (int)(idx - (long long)v * C);
+  int c = j % 3;
+  double vs;
+  if (FF) {
+    double f = 0.0;
+    for (int q = vfptr[v]; q < vfptr[v + 1]; ++q) f += FF[(long long)(vfq[q] / 3) * C + j];
+    vs = V[idx] + h * ((1.0 / mass[v]) * f);
+  } else {
+    double g = c == 0 ? g0 : (c == 1 ? g1 : g2);
+    vs = V[idx] + (gravity_on ? g * h : 0.0);
+  }
   double x = X[idx];
   Xn[idx] = x;
-  Vs[idx] = vs;
   Xit[idx] = x + h * vs;
   Cc[idx] = mass[v] * vs / h - Lx[idx] + attw[v] * xrest[3 * v + c];
+}
+
+// Drag force of contour face f for column triple r at the step-start state:
+// FF[f][3r..3r+2] = drag/3 + lift/3 (Dfobj.h:874-994).
+__global__ void k_drag_face(const double* __restrict__ X, const double* __restrict__ V, const int* __restrict__ cf,
+                            double* __restrict__ FF, int nf, int R, int C) {
+  long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx >= (long long)nf * R) return;
+  int f = (int)(idx / R), r = (int)(idx - (long long)f * R);
+  const long long o0 = (long long)cf[3 * f] * C + 3 * r, o1 = (long long)cf[3 * f + 1] * C + 3 * r,
+                  o2 = (long long)cf[3 * f + 2] * C + 3 * r;
+  double a[3], b[3], n[3], w[3], F[3];
+  for (int c = 0; c < 3; ++c) {
+    a[c] = X[o1 + c] - X[o0 + c];
+    b[c] = X[o2 + c] - X[o0 + c];
+    w[c] = 0.0 - (V[o0 + c] + V[o1 + c] + V[o2 + c]) / 3.0;
+  }
+  n[0] = a[1] * b[2] - a[2] * b[1];
+  n[1] = a[2] * b[0] - a[0] * b[2];
+  n[2] = a[0] * b[1] - a[1] * b[0];
+  drag_force(n, w, F);
+  for (int c = 0; c < 3; ++c) FF[(long long)f * C + 3 * r + c] = F[c];
+}
+
+// Reverse of the drag kick for face f, column triple r: gF = sum over the
+// face's vertices of h m^-1 gVs; then, through n = (p2-p1) x (p3-p1) and
+// w = -(v1+v2+v3)/3, per-corner position gradients GXf[3f+k] and the shared
+// velocity gradient GWf[f] (= -gw/3).
+__global__ void k_drag_vjp(const double* __restrict__ X, const double* __restrict__ V, const double* __restrict__ gVs,
+                           const double* __restrict__ mass, const int* __restrict__ cf, double* __restrict__ GXf,
+                           double* __restrict__ GWf, int nf, int R, int C, double h) {
+  long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx >= (long long)nf * R) return;
+  int f = (int)(idx / R), r = (int)(idx - (long long)f * R);
+  int vi[3] = {cf[3 * f], cf[3 * f + 1], cf[3 * f + 2]};
+  long long o[3];
+  for (int k = 0; k < 3; ++k) o[k] = (long long)vi[k] * C + 3 * r;
+  double a[3], b[3], n[3], w[3], gF[3] = {0, 0, 0};
+  for (int c = 0; c < 3; ++c) {
+    a[c] = X[o[1] + c] - X[o[0] + c];
+    b[c] = X[o[2] + c] - X[o[0] + c];
+    w[c] = 0.0 - (V[o[0] + c] + V[o[1] + c] + V[o[2] + c]) / 3.0;
+    for (int k = 0; k < 3; ++k) gF[c] += h * (1.0 / mass[vi[k]]) * gVs[o[k] + c];
+  }
+  n[0] = a[1] * b[2] - a[2] * b[1];
+  n[1] = a[2] * b[0] - a[0] * b[2];
+  n[2] = a[0] * b[1] - a[1] * b[0];
+  double gn[3], gw[3];
+  drag_force_vjp(n, w, gF, gn, gw);
+  // n = a x b: ga = b x gn, gb = gn x a
+  double ga[3] = {b[1] * gn[2] - b[2] * gn[1], b[2] * gn[0] - b[0] * gn[2], b[0] * gn[1] - b[1] * gn[0]};
+  double gb[3] = {gn[1] * a[2] - gn[2] * a[1], gn[2] * a[0] - gn[0] * a[2], gn[0] * a[1] - gn[1] * a[0]};
+  const long long base = (long long)3 * f * C + 3 * r;
+  for (int c = 0; c < 3; ++c) {
+    GXf[base + c] = -(ga[c] + gb[c]);
+    GXf[base + C + c] = ga[c];
+    GXf[base + 2 * C + c] = gb[c];
+    GWf[(long long)f * C + 3 * r + c] = -gw[c] / 3.0;
+  }
+}
+
+// gX += sum of the vertex's corner position gradients, gV += sum of its faces'
+// velocity gradients (ascending face order).
+__global__ void k_drag_gather(const double* __restrict__ GXf, const double* __restrict__ GWf,
+                              const int* __restrict__ vfptr, const int* __restrict__ vfq, double* __restrict__ gX,
+                              double* __restrict__ gV, int N, int C) {
+  long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx >= (long long)N * C) return;
+  int v = (int)(idx / C);
+  int j = (int)(idx - (long long)v * C);
+  double sx = 0.0, sv = 0.0;
+  for (int q = vfptr[v]; q < vfptr[v + 1]; ++q) {
+    sx += GXf[(long long)vfq[q] * C + j];
+    sv += GWf[(long long)(vfq[q] / 3) * C + j];
+  }
+  gX[idx] += sx;
+  gV[idx] += sv;
 }
 
 // Active-rollout list: alist[0..*acount) = the rollouts r with active[r] != 0,
@@ -1538,19 +1626,24 @@ struct pdm_scene_s {
   int tuned_C = -1;
   bool autotune = true;
   // dynamic
-  DevBuf<double> X, V, Xn, Vs, Cc, Xit, B, Lx, U, part, err;
+  DevBuf<double> X, V, Xn, Cc, Xit, B, Lx, U, part, err;
   DevBuf<char> corner;
   DevBuf<int> active, iters_dev;
   std::vector<int> iters_host;  // R*steps of the last forward
   int last_steps = 0;
   // tape
   DevBuf<double> ckX, ckV;      // checkpoints
-  DevBuf<double> tY, tXn, tVs;  // segment tape
+  DevBuf<double> tY, tXn, tVn;  // segment tape (tVn: step-start velocities, drag only)
   DevBuf<int> tIters;           // per step of segment: R
   DevBuf<double> act_dev;       // activations of the recorded forward
   bool have_tape = false;
   int tape_steps = 0;
   bool act_null = true;
+  // underwater drag: contour faces, incident faces per vertex, face forces and
+  // the reverse's per-corner / per-face gradients
+  int nfaces = 0;
+  DevBuf<int> dcf, dvfptr, dvfq;
+  DevBuf<double> FF, GXf, GWf;
   // contact
   int ncv = 0, nobs = 0;
   DevBuf<int> cvtx, cv_of_v, imp_stamp, cactive, iactive, cfirst, fresh, npass, citers, clast, cinj;
@@ -1624,6 +1717,10 @@ int upload_static(pdm_scene s) {
   for (size_t k = 0; k < S.att_idx.size(); ++k) aw[S.att_idx[k]] += S.att_w[k];
   if (s->attw.upload(aw.data(), S.N)) return 1;
   if (s->xrest.upload(S.xrest.data(), 3 * (size_t)S.N)) return 1;
+  s->nfaces = S.underwater ? (int)(S.cf.size() / 3) : 0;
+  if (s->nfaces > 0 && (s->dcf.upload(S.cf.data(), S.cf.size()) || s->dvfptr.upload(S.vfptr.data(), S.N + 1) ||
+                        s->dvfq.upload(S.vfq.data(), S.vfq.size())))
+    return 1;
   s->nobs = (int)(S.planes.size() / 6 + S.spheres.size() / 4);
   s->ncv = (S.contact && s->nobs > 0) ? (int)S.contact_vtx.size() : 0;
   if (s->ncv > 0) {
@@ -1812,6 +1909,13 @@ int host_setup(const pdm_scene_desc* d, pdm_scene s) {
   S.mu = d->mu;
   S.planes.assign(d->planes, d->planes + 6 * (size_t)d->n_planes);
   S.spheres.assign(d->spheres, d->spheres + 4 * (size_t)d->n_spheres);
+  S.underwater = d->underwater;
+  if (S.underwater) {
+    std::vector<int> cf = contour_faces(d->x_rest, d->tets, d->n_tets);
+    S.cf.resize(cf.size());
+    for (size_t k = 0; k < cf.size(); ++k) S.cf[k] = S.iperm[cf[k]];
+    vertex_faces(S.N, S.cf, S.vfptr, S.vfq);
+  }
   if (!(S.h > 0) || S.max_iter < 0) {
     g_err = "invalid stepping parameters";
     return 1;
@@ -1858,8 +1962,11 @@ int autotune_levels(pdm_scene s);
 int ensure_batch_buffers(pdm_scene s) {
   size_t nc = s->NC();
   int N = s->S.N;
-  if (s->X.alloc(nc) || s->V.alloc(nc) || s->Xn.alloc(nc) || s->Vs.alloc(nc) || s->Cc.alloc(nc) || s->Xit.alloc(nc) ||
-      s->B.alloc(nc) || s->Lx.alloc(nc))
+  if (s->X.alloc(nc) || s->V.alloc(nc) || s->Xn.alloc(nc) || s->Cc.alloc(nc) || s->Xit.alloc(nc) || s->B.alloc(nc) ||
+      s->Lx.alloc(nc))
+    return 1;
+  if (s->nfaces > 0 && (s->FF.alloc((size_t)s->nfaces * s->C) || s->GXf.alloc((size_t)3 * s->nfaces * s->C) ||
+                        s->GWf.alloc((size_t)s->nfaces * s->C)))
     return 1;
   if (s->U.alloc(std::max<long long>(1, s->F.uptr.back()) * (size_t)s->C)) return 1;
   size_t esz = s->precision == PDM_MIXED ? sizeof(float) : sizeof(double);
@@ -2276,7 +2383,7 @@ int contact_phase(pdm_scene s, const double* act, int act_steps, int step, Conta
 // activations indexed by the device step counter; the gather and the solve
 // return at once when no rollout iterates any more).
 int launch_pd_block(pdm_scene s, const double* act, int act_steps, int step, double* tapeY, double* tapeXn,
-                    double* tapeVs, bool host_exit, int* any, const int* step_dev) {
+                    double* tapeVn, bool host_exit, int* any, const int* step_dev) {
   HostScene& S = s->S;
   int N = S.N, C = s->C;
   size_t nc = s->NC();
@@ -2284,11 +2391,14 @@ int launch_pd_block(pdm_scene s, const double* act, int act_steps, int step, dou
   dim3 ugrid(cdiv(C, 64), nchunk), ublock(64, 4);
   hipLaunchKernelGGL(k_spmv, dim3(cdiv(nc, kThreads)), dim3(kThreads), 0, s->stream, s->Lptr.p, s->Lcol.p, s->Lval.p,
                      s->X.p, s->Lx.p, N, C);
+  if (s->nfaces > 0)
+    hipLaunchKernelGGL(k_drag_face, dim3(cdiv((long long)s->nfaces * s->R, kThreads)), dim3(kThreads), 0, s->stream,
+                       s->X.p, s->V.p, s->dcf.p, s->FF.p, s->nfaces, s->R, C);
   hipLaunchKernelGGL(k_step_prep, dim3(cdiv(nc, kThreads)), dim3(kThreads), 0, s->stream, s->X.p, s->V.p, s->Lx.p,
-                     s->mass.p, s->attw.p, s->xrest.p, s->Xn.p, s->Vs.p, s->Cc.p, s->Xit.p, N, C, S.h, S.g[0], S.g[1],
-                     S.g[2], S.gravity_on);
+                     s->mass.p, s->attw.p, s->xrest.p, s->Xn.p, s->Cc.p, s->Xit.p, N, C, S.h, S.g[0], S.g[1], S.g[2],
+                     S.gravity_on, s->nfaces > 0 ? s->FF.p : nullptr, s->dvfptr.p, s->dvfq.p);
   if (tapeXn) HIPX(hipMemcpyAsync(tapeXn, s->Xn.p, nc * 8, hipMemcpyDeviceToDevice, s->stream));
-  if (tapeVs) HIPX(hipMemcpyAsync(tapeVs, s->Vs.p, nc * 8, hipMemcpyDeviceToDevice, s->stream));
+  if (tapeVn) HIPX(hipMemcpyAsync(tapeVn, s->V.p, nc * 8, hipMemcpyDeviceToDevice, s->stream));
   if (tapeY) HIPX(hipMemcpyAsync(tapeY, s->Xit.p, nc * 8, hipMemcpyDeviceToDevice, s->stream));
   hipLaunchKernelGGL(k_sqsum_chunks, ugrid, ublock, 0, s->stream, s->Xit.p, s->part.p, N, C, any);
   hipLaunchKernelGGL(k_err_init, dim3(cdiv(s->R, 4)), dim3(256), 0, s->stream, s->part.p, nchunk, s->R, N,
@@ -2353,17 +2463,17 @@ int ensure_step_graph(pdm_scene s, const double* act, int act_steps) {
 
 // One time step on device. tapeY: optional (max_iter+1) x NC buffer for the
 // iterates; tIters: optional R ints receiving the executed iterations.
-int run_step(pdm_scene s, const double* act, int act_steps, int step, double* tapeY, double* tapeXn, double* tapeVs,
+int run_step(pdm_scene s, const double* act, int act_steps, int step, double* tapeY, double* tapeXn, double* tapeVn,
              int* tIters, int* tNpass = nullptr, int* tCiters = nullptr, ContactTape* ctp = nullptr) {
   HostScene& S = s->S;
   size_t nc = s->NC();
   bool host_exit = (s->flags & PDM_HOST_EARLY_EXIT) != 0;
-  bool graph = (s->flags & PDM_USE_GRAPH) && !tapeY && !tapeXn && !tapeVs;
+  bool graph = (s->flags & PDM_USE_GRAPH) && !tapeY && !tapeXn && !tapeVn;
   if (graph) {
     if (ensure_step_graph(s, act, act_steps)) return 1;
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, s->stream, s->step_dev.p, step);
     HIPX(hipGraphLaunch(s->gexec, s->stream));
-  } else if (launch_pd_block(s, act, act_steps, step, tapeY, tapeXn, tapeVs, host_exit, nullptr, nullptr)) {
+  } else if (launch_pd_block(s, act, act_steps, step, tapeY, tapeXn, tapeVn, host_exit, nullptr, nullptr)) {
     return 1;
   }
   if (s->ncv > 0) {
@@ -2431,7 +2541,7 @@ int forward_impl(pdm_scene s, int steps, const double* x0, const double* v0, con
     if (n >= last_s0) {
       int q = n - last_s0;
       if (run_step(s, act, steps, n, s->tY.p + (size_t)q * K1 * nc, s->tXn.p + (size_t)q * nc,
-                   s->tVs.p + (size_t)q * nc, s->tIters.p + (size_t)q * s->R, np, ci,
+                   s->nfaces > 0 ? s->tVn.p + (size_t)q * nc : nullptr, s->tIters.p + (size_t)q * s->R, np, ci,
                    s->ncv > 0 ? s->ctape[q].get() : nullptr))
         return 1;
       HIPX(hipMemcpyAsync(it_all.p + (size_t)n * s->R, s->tIters.p + (size_t)q * s->R, s->R * sizeof(int),
@@ -2557,9 +2667,9 @@ int ensure_tapes(pdm_scene s, int steps) {
   size_t nc = s->NC();
   int K1 = S.max_iter + 1;
   int seg = std::min(s->ckpt, std::max(steps, 1));
-  if (s->tY.alloc((size_t)seg * K1 * nc) || s->tXn.alloc((size_t)seg * nc) || s->tVs.alloc((size_t)seg * nc) ||
-      s->tIters.alloc((size_t)seg * s->R))
+  if (s->tY.alloc((size_t)seg * K1 * nc) || s->tXn.alloc((size_t)seg * nc) || s->tIters.alloc((size_t)seg * s->R))
     return 1;
+  if (s->nfaces > 0 && s->tVn.alloc((size_t)seg * nc)) return 1;
   if (s->ncv > 0)
     while ((int)s->ctape.size() < seg) s->ctape.emplace_back(new ContactTape());
   return 0;
@@ -2611,7 +2721,7 @@ int backward_impl(pdm_scene s, const double* gxT, const double* gvT, double* gx0
     for (int n = s0; n < s1 && !taped; ++n) {
       int q = n - s0;
       if (run_step(s, act, steps, n, s->tY.p + (size_t)q * K1 * nc, s->tXn.p + (size_t)q * nc,
-                   s->tVs.p + (size_t)q * nc, s->tIters.p + (size_t)q * R, nullptr, nullptr,
+                   s->nfaces > 0 ? s->tVn.p + (size_t)q * nc : nullptr, s->tIters.p + (size_t)q * R, nullptr, nullptr,
                    s->ncv > 0 ? s->ctape[q].get() : nullptr))
         return 1;
     }
@@ -2649,6 +2759,14 @@ int backward_impl(pdm_scene s, const double* gxT, const double* gvT, double* gx0
                          s->Lval.p, s->Zs.p, s->LZ.p, N, C);
       hipLaunchKernelGGL(k_adj_step_end, dim3(cdiv(nc, kThreads)), dim3(kThreads), 0, s->stream, s->Xb.p, s->Yb.p,
                          s->LZ.p, s->Zs.p, s->mass.p, s->gX.p, s->gV.p, N, C, S.h);
+      if (s->nfaces > 0) {
+        // gV holds dL/dv* here; the drag kick v* = v + h M^-1 f(x, v) adds its reverse
+        hipLaunchKernelGGL(k_drag_vjp, dim3(cdiv((long long)s->nfaces * R, kThreads)), dim3(kThreads), 0, s->stream,
+                           s->tXn.p + (size_t)q * nc, s->tVn.p + (size_t)q * nc, s->gV.p, s->mass.p, s->dcf.p,
+                           s->GXf.p, s->GWf.p, s->nfaces, R, C, S.h);
+        hipLaunchKernelGGL(k_drag_gather, dim3(cdiv(nc, kThreads)), dim3(kThreads), 0, s->stream, s->GXf.p, s->GWf.p,
+                           s->dvfptr.p, s->dvfq.p, s->gX.p, s->gV.p, N, C);
+      }
     }
   }
   if (gx0)
@@ -2853,6 +2971,7 @@ int pdm_scene_info(pdm_scene s, pdm_info* info) {
   info->factor_flops = s->F.flops_factor;
   info->reg_used = s->F.reg_used;
   info->n_contact_vertices = (int)s->S.contact_vtx.size();
+  info->n_drag_faces = s->nfaces;
   info->batch = s->R;
   info->precision = s->precision;
   return 0;

@@ -40,6 +40,12 @@ struct HostScene {
   std::vector<int> contact_vtx;
   std::vector<double> planes;      // 6 per plane
   std::vector<double> spheres;     // 4 per sphere
+  // underwater drag: outward-oriented surface faces (engine ids, 3 per face,
+  // in the reference's face order) and, per vertex, its incident faces as
+  // 3 * face + corner in ascending face order (vfptr: N+1)
+  int underwater = 0;
+  std::vector<int> cf;
+  std::vector<int> vfptr, vfq;
   // stepping
   double h = 0.005, damping = 0.9999;
   double g[3] = {0, 0, 0};
@@ -103,6 +109,11 @@ int select_muscle_tets(const double* x_caller, const int* tets_caller, int T, co
                        std::vector<int>& out_tet, std::vector<double>& out_w, const std::vector<int>* cand = nullptr);
 bool add_muscles(HostScene& S, const double* x_caller, const int* tets_caller, int n_seg, const double* seg_start,
                  const double* seg_end, const int* seg_group, int n_groups, double k_muscle, std::string& err);
+// Outward-oriented surface faces of a tet mesh (caller numbering), in the
+// order Dfobj::Initialize lists its contours (Dfobj.h:497-596).
+std::vector<int> contour_faces(const double* xrest, const int* tets, int T);
+// Incident faces of every vertex (3 * face + corner, ascending face order).
+void vertex_faces(int N, const std::vector<int>& cf, std::vector<int>& vfptr, std::vector<int>& vfq);
 std::vector<int> surface_vertices(int N, const std::vector<int>& tv, int T);
 
 }  // namespace pdm

@@ -410,4 +410,132 @@ __device__ __forceinline__ void corotate_vjp(const T* F, T wr, T wv, const T* Z,
     for (int c = 0; c < 3; ++c) Fbar[r * 3 + c] = UO[r * 3 + 0] * V[c * 3 + 0] + UO[r * 3 + 1] * V[c * 3 + 1] + UO[r * 3 + 2] * V[c * 3 + 2];
 }
 
+// ------------------------------------------------------------ surface drag
+// Underwater drag and lift on one surface face (Dfobj::ComputeDragForces,
+// Dfobj.h:846-994). Inputs: n = (p2 - p1) x (p3 - p1) of the outward-oriented
+// face and w = v_water - mean face velocity (v_water = 0). Output: the force
+// added to each of the face's three vertices, drag/3 + lift/3. With a = n/|n|
+// and wh = w/|w|: x = -a.wh, aoa = pi/2 - acos(x); faces with x <= 0 (moving
+// inwards) get no force; drag = 500 area Cd(aoa) |w|^2 wh with Cd = sin^2(aoa);
+// lift = 500 area Cl(aoa) |w|^2 l with l = normalize(wh x (-a)) x wh and Cl
+// the reference's piecewise-linear table in degrees, zero beyond 85 degrees;
+// each of drag and lift is scaled down to 1000 N if longer. Nothing under
+// |w| < 1e-6.
+//
+// S is double (the force) or Dual1 (one forward-mode tangent: a Jacobian
+// column). The branches are taken on values, as the reference's taped
+// comparisons are. Cd(aoa(x)) is identically x^2, and its derivative is taken
+// as 2x (the composite through acos is singular at x = 1, its product is not).
+struct Dual1 {
+  double v, d;
+  __host__ __device__ Dual1(double v_ = 0.0, double d_ = 0.0) : v(v_), d(d_) {}
+};
+__host__ __device__ inline Dual1 operator+(Dual1 a, Dual1 b) { return Dual1(a.v + b.v, a.d + b.d); }
+__host__ __device__ inline Dual1 operator-(Dual1 a, Dual1 b) { return Dual1(a.v - b.v, a.d - b.d); }
+__host__ __device__ inline Dual1 operator-(Dual1 a) { return Dual1(-a.v, -a.d); }
+__host__ __device__ inline Dual1 operator*(Dual1 a, Dual1 b) { return Dual1(a.v * b.v, a.d * b.v + a.v * b.d); }
+__host__ __device__ inline Dual1 operator/(Dual1 a, Dual1 b) {
+  double q = a.v / b.v;
+  return Dual1(q, (a.d - q * b.d) / b.v);
+}
+__host__ __device__ inline double dval(double a) { return a; }
+__host__ __device__ inline double dval(Dual1 a) { return a.v; }
+__host__ __device__ inline double droot(double a) { return sqrt(a); }
+__host__ __device__ inline Dual1 droot(Dual1 a) {
+  double r = sqrt(a.v);
+  return Dual1(r, a.d * 0.5 / r);
+}
+
+constexpr double kDragMaxForce = 1.0e3, kDragCutoff = 85.0, kPi = 3.14159265358979323846;
+
+__host__ __device__ inline double drag_aoa(double x) {
+  return 0.5 * kPi - acos(x > 1.0 ? 1.0 : (x < -1.0 ? -1.0 : x));
+}
+__host__ __device__ inline double drag_cd(double x) {
+  double aoa = drag_aoa(x);
+  return 1.0 * (0.5 * (-cos(2.0 * aoa) + 1.0)) + 0.0;
+}
+__host__ __device__ inline Dual1 drag_cd(Dual1 x) { return Dual1(drag_cd(x.v), 2.0 * x.v * x.d); }
+// Cl at theta degrees and its slope d Cl / d theta.
+__host__ __device__ inline double drag_cl_table(double theta, double* slope) {
+  const double X[5] = {-90.0, -10.0, -5.0, 15.0, 90.0}, Y[5] = {-0.5, -1.26, 0.0, 1.8, 0.5};
+  *slope = 0.0;
+  if (fabs(theta) > kDragCutoff) return 0.0;
+  int k = -1;
+  if (X[0] <= theta && theta < X[1]) k = 0;
+  else if (X[1] <= theta && theta < X[2]) k = 1;
+  else if (X[2] <= theta && theta <= X[3]) k = 2;
+  else if (X[3] <= theta && theta <= X[4]) k = 3;
+  if (k < 0) return 0.0;
+  double t = (theta - X[k]) / (X[k + 1] - X[k]);
+  *slope = (Y[k + 1] - Y[k]) / (X[k + 1] - X[k]);
+  return (1.0 - t) * Y[k] + t * Y[k + 1];
+}
+__host__ __device__ inline double drag_cl(double x) {
+  double sl;
+  return drag_cl_table(drag_aoa(x) * 180.0 / kPi, &sl);
+}
+__host__ __device__ inline Dual1 drag_cl(Dual1 x) {
+  double sl;
+  double c = drag_cl_table(drag_aoa(x.v) * 180.0 / kPi, &sl);
+  if (sl == 0.0) return Dual1(c, 0.0);
+  return Dual1(c, sl * (180.0 / kPi) / sqrt(1.0 - x.v * x.v) * x.d);
+}
+
+template <typename S>
+__host__ __device__ inline void drag_clamp(S* f) {
+  S q = f[0] * f[0] + f[1] * f[1] + f[2] * f[2];
+  if (sqrt(dval(q)) > kDragMaxForce) {
+    S l = droot(q);
+    for (int c = 0; c < 3; ++c) f[c] = S(kDragMaxForce) * (f[c] / l);
+  }
+}
+
+template <typename S>
+__host__ __device__ inline void drag_force(const S* n, const S* w, S* F) {
+  F[0] = F[1] = F[2] = S(0.0);
+  S w2 = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+  if (sqrt(dval(w2)) < 1e-6) return;
+  S wl = droot(w2);
+  S wh[3] = {w[0] / wl, w[1] / wl, w[2] / wl};
+  S nl = droot(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+  S d[3] = {-(n[0] / nl), -(n[1] / nl), -(n[2] / nl)};
+  S area = S(0.5) * nl;
+  S x = d[0] * wh[0] + d[1] * wh[1] + d[2] * wh[2];
+  S drag[3] = {S(0.0), S(0.0), S(0.0)}, lift[3] = {S(0.0), S(0.0), S(0.0)};
+  if (dval(x) > 0.0) {
+    S k = S(500.0) * area * drag_cd(x) * w2;
+    for (int c = 0; c < 3; ++c) drag[c] = k * wh[c];
+    S u[3] = {wh[1] * d[2] - wh[2] * d[1], wh[2] * d[0] - wh[0] * d[2], wh[0] * d[1] - wh[1] * d[0]};
+    S u2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    if (dval(u2) > 0.0) {
+      S ul = droot(u2);
+      for (int c = 0; c < 3; ++c) u[c] = u[c] / ul;
+    }
+    S l[3] = {u[1] * wh[2] - u[2] * wh[1], u[2] * wh[0] - u[0] * wh[2], u[0] * wh[1] - u[1] * wh[0]};
+    S kl = S(500.0) * area * drag_cl(x) * w2;
+    for (int c = 0; c < 3; ++c) lift[c] = kl * l[c];
+  }
+  drag_clamp(drag);
+  drag_clamp(lift);
+  for (int c = 0; c < 3; ++c) F[c] = drag[c] / S(3.0) + lift[c] / S(3.0);
+}
+
+// Transposed Jacobian of drag_force applied to gF: gn = (dF/dn)^T gF and
+// gw = (dF/dw)^T gF, one forward-mode pass per input.
+__host__ __device__ inline void drag_force_vjp(const double* n, const double* w, const double* gF, double* gn,
+                                               double* gw) {
+  for (int s = 0; s < 6; ++s) {
+    Dual1 nd[3], wd[3], F[3];
+    for (int c = 0; c < 3; ++c) {
+      nd[c] = Dual1(n[c], s == c ? 1.0 : 0.0);
+      wd[c] = Dual1(w[c], s == 3 + c ? 1.0 : 0.0);
+    }
+    drag_force(nd, wd, F);
+    double g = gF[0] * F[0].d + gF[1] * F[1].d + gF[2] * F[2].d;
+    if (s < 3) gn[s] = g;
+    else gw[s - 3] = g;
+  }
+}
+
 }  // namespace pdm

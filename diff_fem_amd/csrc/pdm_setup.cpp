@@ -145,6 +145,56 @@ void nd_order(int N, const double* xyz, const std::vector<std::vector<int>>& adj
   }
 }
 
+// Faces (s0,s1,s2), (s0,s1,s3), (s0,s2,s3), (s1,s2,s3) of every tet's sorted
+// vertex ids, in tet order; a face that no other tet shares is a contour, and
+// it is turned outward: with n = (q1 - q0) x (q2 - q1), q0 and q1 swap when n
+// points towards the owning tet's centre. The reference finds unshared faces
+// by an all-pairs scan; a sort of the 4T keys finds the same set.
+std::vector<int> contour_faces(const double* xrest, const int* tets, int T) {
+  const int fc[4][3] = {{0, 1, 2}, {0, 1, 3}, {0, 2, 3}, {1, 2, 3}};
+  std::vector<std::array<int, 3>> tri(4 * (size_t)T);
+  for (int t = 0; t < T; ++t) {
+    int sv[4] = {tets[4 * t], tets[4 * t + 1], tets[4 * t + 2], tets[4 * t + 3]};
+    std::sort(sv, sv + 4);
+    for (int k = 0; k < 4; ++k) tri[4 * (size_t)t + k] = {sv[fc[k][0]], sv[fc[k][1]], sv[fc[k][2]]};
+  }
+  std::vector<int> ord(tri.size());
+  std::iota(ord.begin(), ord.end(), 0);
+  std::sort(ord.begin(), ord.end(), [&](int a, int b) { return tri[a] < tri[b] || (tri[a] == tri[b] && a < b); });
+  std::vector<char> uniq(tri.size(), 0);
+  for (size_t k = 0; k < ord.size(); ++k) {
+    bool same_prev = k > 0 && tri[ord[k - 1]] == tri[ord[k]];
+    bool same_next = k + 1 < ord.size() && tri[ord[k + 1]] == tri[ord[k]];
+    uniq[ord[k]] = !(same_prev || same_next);
+  }
+  std::vector<int> out;
+  for (size_t i = 0; i < tri.size(); ++i) {
+    if (!uniq[i]) continue;
+    std::array<int, 3> f = tri[i];
+    const int* tv = tets + 4 * (i / 4);
+    double c[3], q0[3], e1[3], e2[3];
+    for (int r = 0; r < 3; ++r) {
+      c[r] = 0.25 * (xrest[3 * tv[0] + r] + xrest[3 * tv[1] + r] + xrest[3 * tv[2] + r] + xrest[3 * tv[3] + r]);
+      q0[r] = xrest[3 * f[0] + r];
+      e1[r] = xrest[3 * f[1] + r] - q0[r];
+      e2[r] = xrest[3 * f[2] + r] - xrest[3 * f[1] + r];
+    }
+    double n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+    if ((c[0] - q0[0]) * n[0] + (c[1] - q0[1]) * n[1] + (c[2] - q0[2]) * n[2] > 0.0) std::swap(f[0], f[1]);
+    out.insert(out.end(), f.begin(), f.end());
+  }
+  return out;
+}
+
+void vertex_faces(int N, const std::vector<int>& cf, std::vector<int>& vfptr, std::vector<int>& vfq) {
+  vfptr.assign(N + 1, 0);
+  for (int v : cf) ++vfptr[v + 1];
+  for (int i = 0; i < N; ++i) vfptr[i + 1] += vfptr[i];
+  vfq.assign(cf.size(), 0);
+  std::vector<int> fill(vfptr.begin(), vfptr.end() - 1);
+  for (size_t q = 0; q < cf.size(); ++q) vfq[fill[cf[q]]++] = (int)q;  // q = 3 * face + corner, ascending
+}
+
 std::vector<int> surface_vertices(int N, const std::vector<int>& tv, int T) {
   std::map<std::array<int, 3>, int> cnt;
   const int fc[4][3] = {{0, 1, 2}, {0, 1, 3}, {0, 2, 3}, {1, 2, 3}};

@@ -81,6 +81,7 @@ def make_desc(desc):
     for i in range(3):
         d.gravity[i] = float(desc.gravity[i])
     d.gravity_on = int(desc.gravity_on)
+    d.underwater = int(desc.underwater)
     d.contact = int(desc.contact)
     d.max_iter1 = int(desc.max_iter1)
     d.alpha = float(desc.alpha)

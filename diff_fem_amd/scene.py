@@ -50,6 +50,7 @@ class SceneDesc:
     damping: float = 0.9999
     gravity: tuple = (0.0, 0.0, 0.0)
     gravity_on: bool = True
+    underwater: bool = False           # "Is underwater": surface drag, no gravity
     contact: bool = False
     max_iter1: int = 30
     alpha: float = 1.0
@@ -345,7 +346,7 @@ def load_json_scene(config_path: str, data_root: Optional[str] = None) -> SceneD
     ``<data_root>/data`` (Dfobj.h:192-199, 724-731), falling back to the
     config's own folder. Scene features outside the projective-dynamics path
     (cloth, obstacle meshes, rigid bones, pneumatic volume muscles, self
-    collision, underwater drag, skeleton or pneumatic actuators) raise
+    collision, skeleton or pneumatic actuators, drag on several meshes) raise
     NotImplementedError."""
     root = data_root if data_root is not None else os.environ.get("PDM_DATA_ROOT", "")
     path = os.path.join(root, config_path.lstrip("/")) if root else config_path
@@ -371,8 +372,7 @@ def load_json_scene(config_path: str, data_root: Optional[str] = None) -> SceneD
     alpha = float(para.get("World Alpha", 1.0))
     if para.get("Self-collision", False):
         _unsupported("Self-collision")
-    if int(para.get("Is underwater", 0)):
-        _unsupported("Is underwater (drag forces)")
+    underwater = bool(int(para.get("Is underwater", 0)))
     if _json_len(js, "Clothes"):
         _unsupported("Clothes")
     if _json_len(js, "Obstacles"):
@@ -380,6 +380,11 @@ def load_json_scene(config_path: str, data_root: Optional[str] = None) -> SceneD
     meshes = js.get("Meshes") or []
     if not meshes:
         raise ValueError("scene has no meshes")
+    if underwater and len(meshes) > 1:
+        # Dfobj::ComputeDragForces indexes the world's state with the mesh's own
+        # vertex ids and returns a world-sized force for a mesh-sized segment
+        # (Dfobj.h:874-881, Environment.h:247): defined for one mesh only
+        _unsupported("Is underwater with several meshes")
     cfg_dir = os.path.dirname(path)
     data_dir = os.path.join(root, "data") if root else cfg_dir
 
@@ -432,7 +437,8 @@ def load_json_scene(config_path: str, data_root: Optional[str] = None) -> SceneD
                                 nu_ctor=np.full(T, nu), attach_idx=attach, attach_w=attach_w,
                                 k_muscle=float(m.get("Muscle Stiffness", 5e5)), h=1.0 / hz, max_iter=max_iter,
                                 damping=damping, gravity=gravity, gravity_on=bool(int(m.get("Enable Gravity", 1))),
-                                contact=contact, max_iter1=max_iter, alpha=alpha, mu=1.0, **kw))
+                                underwater=underwater, contact=contact, max_iter1=max_iter, alpha=alpha, mu=1.0,
+                                **kw))
         infos.append(DfobjInfo(vertex_offset=nv, n_vertices=x.shape[0], tet_offset=nt, n_tets=T,
                                actuator_type=act_type, n_muscles=n_mus, group_offset=ng))
         nv += x.shape[0]
@@ -501,7 +507,8 @@ def pack_scenes(descs: List[SceneDesc], gap: float = 0.4, name: str = "packed", 
         raise ValueError("no meshes to pack")
     d0 = descs[0]
     for d in descs[1:]:
-        for k in ("h", "max_iter", "damping", "gravity_on", "contact", "max_iter1", "alpha", "mu", "mass_same"):
+        for k in ("h", "max_iter", "damping", "gravity_on", "underwater", "contact", "max_iter1", "alpha", "mu",
+                  "mass_same"):
             if getattr(d, k) != getattr(d0, k):
                 raise ValueError("packed meshes disagree on %s" % k)
         if tuple(d.gravity) != tuple(d0.gravity):
@@ -558,6 +565,7 @@ def pack_scenes(descs: List[SceneDesc], gap: float = 0.4, name: str = "packed", 
                      attach_idx=np.concatenate(att), attach_w=att_w.pop() if att_w else d0.attach_w,
                      k_muscle=k_mus.pop() if k_mus else d0.k_muscle, h=d0.h,
                      max_iter=d0.max_iter, damping=d0.damping, gravity=d0.gravity, gravity_on=d0.gravity_on,
+                     underwater=d0.underwater,
                      contact=d0.contact, max_iter1=d0.max_iter1, alpha=d0.alpha, mu=d0.mu, planes=planes,
                      spheres=np.array(spheres) if spheres else None, body_of_tet=np.concatenate(bodies),
                      body_mass=np.array(mass), nu_fwd_tet=np.concatenate(nuf), name=name, **kw)

@@ -76,6 +76,10 @@ typedef struct pdm_scene_desc {
   int n_bodies;              /* 0: one body of total_mass */
   const int* body_of_tet;    /* T body ids in [0, n_bodies) */
   const double* body_mass;   /* n_bodies masses */
+  /* "Is underwater" (Environment.h:162-168, 236-250): no gravity; each step
+   * starts with v* = v + h M^-1 f, f the drag and lift of every surface face
+   * at the step-start state (Dfobj::ComputeDragForces, Dfobj.h:846-994). */
+  int underwater;
 } pdm_scene_desc;
 
 typedef struct pdm_info {
@@ -88,6 +92,7 @@ typedef struct pdm_info {
   int n_contact_vertices;
   int batch;                 /* rollouts R */
   int precision;             /* PDM_FP64 or PDM_MIXED */
+  int n_drag_faces;          /* surface faces carrying drag (0 unless underwater) */
 } pdm_info;
 
 enum { PDM_FP64 = 0, PDM_MIXED = 1 };

@@ -67,6 +67,11 @@ def lib():
         L.orc_svd3.argtypes = [_dp, _dp, _dp, _dp]
         L.orc_volume_projection.argtypes = [_dp, _dp]
         L.orc_volume_projection_jac.argtypes = [_dp, _dp, _dp, _dp]
+        L.orc_set_underwater.argtypes = [C.c_void_p, C.c_int]
+        L.orc_num_drag_faces.argtypes = [C.c_void_p]
+        L.orc_get_drag_faces.argtypes = [C.c_void_p, _ip]
+        L.orc_drag_face.argtypes = [_dp, _dp, _dp, C.c_void_p]
+        L.orc_drag_forces.argtypes = [C.c_void_p, _dp, _dp, _dp]
         L.orc_detect_plane.argtypes = [_dp, _dp, _dp, _dp, _dp]
         L.orc_project_contact.argtypes = [_dp, C.c_double, _dp, _dp]
         L.orc_last_error.restype = C.c_char_p
@@ -138,6 +143,8 @@ class OracleScene:
                       int(desc.contact), int(desc.max_iter1), float(desc.alpha), float(desc.mu))
         L.orc_set_obstacles(self.h, desc.planes.shape[0], np.ascontiguousarray(desc.planes).ravel().copy(),
                             desc.spheres.shape[0], np.ascontiguousarray(desc.spheres).ravel().copy())
+        if getattr(desc, "underwater", False):
+            L.orc_set_underwater(self.h, 1)
         if factor:
             self.factor()
 
@@ -157,6 +164,17 @@ class OracleScene:
         lib().orc_set_material(self.h, E, float(nu_fwd),
                                np.ascontiguousarray(np.broadcast_to(np.asarray(nc, np.float64), (self.T,))))
         self.factor()
+
+    def drag_faces(self):
+        out = np.zeros((lib().orc_num_drag_faces(self.h), 3), np.int32)
+        lib().orc_get_drag_faces(self.h, out)
+        return out
+
+    def drag_forces(self, x, v):
+        f = np.zeros(3 * self.N)
+        lib().orc_drag_forces(self.h, np.ascontiguousarray(x, np.float64).ravel(),
+                              np.ascontiguousarray(v, np.float64).ravel(), f)
+        return f
 
     def masses(self):
         m = np.zeros(self.N)
@@ -269,6 +287,15 @@ def volume_projection(S):
     D = np.zeros(3)
     lib().orc_volume_projection(np.ascontiguousarray(S, np.float64), D)
     return D
+
+
+def drag_face(n, w, jac=False):
+    """Drag + lift per vertex of one face (force, and its 3x6 Jacobian in
+    (n, w) by the complex step)."""
+    F = np.zeros(3)
+    J = np.zeros((3, 6)) if jac else None
+    lib().orc_drag_face(np.ascontiguousarray(n, np.float64), np.ascontiguousarray(w, np.float64), F, _ptr(J))
+    return (F, J) if jac else F
 
 
 def volume_projection_jac(S):

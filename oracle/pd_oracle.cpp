@@ -36,6 +36,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <complex>
 #include <cstring>
 #include <map>
 #include <string>
@@ -410,6 +411,7 @@ struct ContactIter {
 
 struct StepTape {
   std::vector<double> xn, vstar;           // start-of-step state (3N)
+  std::vector<double> vn;                  // start-of-step velocity (3N)
   std::vector<std::vector<double>> ys;     // PD iterates y_0..y_K
   std::vector<double> act;                 // activations used this step
   bool contact_active = false;
@@ -441,6 +443,9 @@ struct Scene {
   std::vector<Plane> planes;
   std::vector<Sphere> spheres;
   std::vector<int> contact_vtx;  // surface vertices
+  // underwater drag (Environment.h:162-168, 236-250; Dfobj.h:846-994)
+  int underwater = 0;
+  std::vector<std::array<int, 3>> contours;  // outward surface faces
   // factor
   std::vector<std::map<int, double>> K;  // scalar K_s rows (original order)
   std::vector<double> Kdiag;
@@ -757,14 +762,180 @@ void project_contact(const double* B, double mu, const double* f_in, double* out
   for (int r = 0; r < 3; ++r) out[r] = B[r * 3 + 0] * rr[0] + B[r * 3 + 1] * rr[1] + B[r * 3 + 2] * rr[2];
 }
 
+// ------------------------------------------------------------ underwater drag
+// Contours of Dfobj::Initialize (Dfobj.h:497-596): the sorted-vertex faces
+// (s0,s1,s2), (s0,s1,s3), (s0,s2,s3), (s1,s2,s3) of every tet, in tet order,
+// that no other tet shares, turned so that n = (q1-q0) x (q2-q1) points away
+// from the owning tet's centre.
+std::vector<std::array<int, 3>> build_contours(const Scene& s) {
+  std::vector<std::array<int, 3>> tri;
+  for (const Tet& t : s.tets) {
+    int sv[4] = {t.v[0], t.v[1], t.v[2], t.v[3]};
+    std::sort(sv, sv + 4);
+    tri.push_back({sv[0], sv[1], sv[2]});
+    tri.push_back({sv[0], sv[1], sv[3]});
+    tri.push_back({sv[0], sv[2], sv[3]});
+    tri.push_back({sv[1], sv[2], sv[3]});
+  }
+  std::map<std::array<int, 3>, int> cnt;
+  for (auto& f : tri) cnt[f]++;
+  std::vector<std::array<int, 3>> out;
+  const double* X = s.xrest.data();
+  for (size_t i = 0; i < tri.size(); ++i) {
+    if (cnt[tri[i]] != 1) continue;
+    std::array<int, 3> f = tri[i];
+    const Tet& t = s.tets[i / 4];
+    double c[3], n[3], a[3], b[3];
+    for (int r = 0; r < 3; ++r) {
+      c[r] = 0.25 * (X[3 * t.v[0] + r] + X[3 * t.v[1] + r] + X[3 * t.v[2] + r] + X[3 * t.v[3] + r]);
+      a[r] = X[3 * f[1] + r] - X[3 * f[0] + r];
+      b[r] = X[3 * f[2] + r] - X[3 * f[1] + r];
+    }
+    n[0] = a[1] * b[2] - a[2] * b[1];
+    n[1] = a[2] * b[0] - a[0] * b[2];
+    n[2] = a[0] * b[1] - a[1] * b[0];
+    double dp = 0;
+    for (int r = 0; r < 3; ++r) dp += (c[r] - X[3 * f[0] + r]) * n[r];
+    if (dp > 0.0) std::swap(f[0], f[1]);
+    out.push_back(f);
+  }
+  return out;
+}
+
+// Face force of ComputeDragForces for n = (p2-p1) x (p3-p1) and the relative
+// water velocity w: (drag + lift) / 3, each part scaled down to 1000 N.
+// Written for T = double and T = complex<double>: the complex instance gives
+// the Jacobian by the complex step (df/dx_k = Im f(x + i eps e_k) / eps), with
+// every branch decided on real parts.
+template <typename T> double re(const T& a) { return std::real(a); }
+template <typename T>
+void drag_face_t(const T* n, const T* w, T* F) {
+  const double cl_x[5] = {-90, -10, -5, 15, 90}, cl_y[5] = {-0.5, -1.26, 0.0, 1.8, 0.5};
+  for (int c = 0; c < 3; ++c) F[c] = T(0.0);
+  T vv = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+  if (std::sqrt(re(vv)) < 1e-6) return;
+  T vl = std::sqrt(vv);
+  T vh[3] = {w[0] / vl, w[1] / vl, w[2] / vl};
+  T nn = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+  T d[3] = {-(n[0] / nn), -(n[1] / nn), -(n[2] / nn)};
+  T area = 0.5 * nn;
+  T dd = d[0] * vh[0] + d[1] * vh[1] + d[2] * vh[2];
+  T fd[3] = {T(0.0), T(0.0), T(0.0)}, fl[3] = {T(0.0), T(0.0), T(0.0)};
+  if (re(dd) > 0.0) {
+    T x = re(dd) > 1.0 ? T(1.0) : dd;
+    T aoa = 0.5 * M_PI - std::acos(x);
+    T cd = (1.0 - 0.0) * (0.5 * (-std::cos(2.0 * aoa) + 1.0)) + 0.0;
+    for (int c = 0; c < 3; ++c) fd[c] = 0.5 * 1000.0 * (1.0 * area) * cd * vv * vh[c];
+    T u[3] = {vh[1] * d[2] - vh[2] * d[1], vh[2] * d[0] - vh[0] * d[2], vh[0] * d[1] - vh[1] * d[0]};
+    T uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    if (re(uu) > 0.0) {
+      T ul = std::sqrt(uu);
+      for (int c = 0; c < 3; ++c) u[c] = u[c] / ul;
+    }
+    T l[3] = {u[1] * vh[2] - u[2] * vh[1], u[2] * vh[0] - u[0] * vh[2], u[0] * vh[1] - u[1] * vh[0]};
+    T theta = aoa * 180.0 / M_PI;
+    T cl = T(0.0);
+    double th = re(theta);
+    if (std::fabs(th) <= 85.0) {
+      for (int k = 0; k < 4; ++k) {
+        bool in = k < 2 ? (cl_x[k] <= th && th < cl_x[k + 1]) : (cl_x[k] <= th && th <= cl_x[k + 1]);
+        if (in) {
+          T t = (theta - cl_x[k]) / (cl_x[k + 1] - cl_x[k]);
+          cl = (1.0 - t) * cl_y[k] + t * cl_y[k + 1];
+          break;
+        }
+      }
+    }
+    for (int c = 0; c < 3; ++c) fl[c] = 0.5 * 1000.0 * (1.0 * area) * cl * vv * l[c];
+  }
+  for (T* f : {fd, fl}) {
+    T q = f[0] * f[0] + f[1] * f[1] + f[2] * f[2];
+    if (std::sqrt(re(q)) > 1.0e3) {
+      T ql = std::sqrt(q);
+      for (int c = 0; c < 3; ++c) f[c] = 1.0e3 * (f[c] / ql);
+    }
+  }
+  for (int c = 0; c < 3; ++c) F[c] = fd[c] / 3.0 + fl[c] / 3.0;
+}
+
+// J[c][k] = dF_c / d(n, w)_k, k = 0..5.
+void drag_face_jac(const double* n, const double* w, double J[3][6]) {
+  const double eps = 1e-30;
+  for (int k = 0; k < 6; ++k) {
+    std::complex<double> nc[3], wc[3], F[3];
+    for (int c = 0; c < 3; ++c) {
+      nc[c] = std::complex<double>(n[c], k == c ? eps : 0.0);
+      wc[c] = std::complex<double>(w[c], k == 3 + c ? eps : 0.0);
+    }
+    drag_face_t(nc, wc, F);
+    for (int c = 0; c < 3; ++c) J[c][k] = std::imag(F[c]) / eps;
+  }
+}
+
+// Per-vertex drag forces f (3N) of the state (x, v).
+void drag_forces(const Scene& s, const double* x, const double* v, double* f) {
+  std::fill(f, f + 3 * (size_t)s.N, 0.0);
+  for (const auto& t : s.contours) {
+    double n[3], w[3], a[3], b[3], F[3];
+    for (int r = 0; r < 3; ++r) {
+      a[r] = x[3 * t[1] + r] - x[3 * t[0] + r];
+      b[r] = x[3 * t[2] + r] - x[3 * t[0] + r];
+      w[r] = 0.0 - (v[3 * t[0] + r] + v[3 * t[1] + r] + v[3 * t[2] + r]) / 3.0;
+    }
+    n[0] = a[1] * b[2] - a[2] * b[1];
+    n[1] = a[2] * b[0] - a[0] * b[2];
+    n[2] = a[0] * b[1] - a[1] * b[0];
+    drag_face_t(n, w, F);
+    for (int k = 0; k < 3; ++k)
+      for (int r = 0; r < 3; ++r) f[3 * t[k] + r] += F[r];
+  }
+}
+
+// Reverse of v* += h M^-1 f(x, v): given gvs = dL/dv*, accumulate into gx, gv.
+void drag_backward(const Scene& s, const double* x, const double* v, const double* gvs, double* gx, double* gv) {
+  for (const auto& t : s.contours) {
+    double n[3], w[3], a[3], b[3], gF[3] = {0, 0, 0}, J[3][6];
+    for (int r = 0; r < 3; ++r) {
+      a[r] = x[3 * t[1] + r] - x[3 * t[0] + r];
+      b[r] = x[3 * t[2] + r] - x[3 * t[0] + r];
+      w[r] = 0.0 - (v[3 * t[0] + r] + v[3 * t[1] + r] + v[3 * t[2] + r]) / 3.0;
+      for (int k = 0; k < 3; ++k) gF[r] += s.h / s.mass[t[k]] * gvs[3 * t[k] + r];
+    }
+    n[0] = a[1] * b[2] - a[2] * b[1];
+    n[1] = a[2] * b[0] - a[0] * b[2];
+    n[2] = a[0] * b[1] - a[1] * b[0];
+    drag_face_jac(n, w, J);
+    double gn[3], gw[3];
+    for (int k = 0; k < 3; ++k) {
+      gn[k] = gF[0] * J[0][k] + gF[1] * J[1][k] + gF[2] * J[2][k];
+      gw[k] = gF[0] * J[0][3 + k] + gF[1] * J[1][3 + k] + gF[2] * J[2][3 + k];
+    }
+    // n = a x b with a = p2 - p1, b = p3 - p1
+    double ga[3] = {b[1] * gn[2] - b[2] * gn[1], b[2] * gn[0] - b[0] * gn[2], b[0] * gn[1] - b[1] * gn[0]};
+    double gb[3] = {gn[1] * a[2] - gn[2] * a[1], gn[2] * a[0] - gn[0] * a[2], gn[0] * a[1] - gn[1] * a[0]};
+    for (int r = 0; r < 3; ++r) {
+      gx[3 * t[0] + r] -= ga[r] + gb[r];
+      gx[3 * t[1] + r] += ga[r];
+      gx[3 * t[2] + r] += gb[r];
+      for (int k = 0; k < 3; ++k) gv[3 * t[k] + r] -= gw[r] / 3.0;
+    }
+  }
+}
+
 // One time step. x,v in/out (3N). act: activations per group (or null).
 void step(Scene& s, double* x, double* v, const double* act, StepTape* tape, int* it_pd, int* it_c, int* outer) {
   int N = s.N, n3 = 3 * N;
   // Environment::Step gravity kick (Environment.h:236-245) + TimeStepping (World.h:692-699)
   std::vector<double> vstar(v, v + n3);
-  if (s.gravity_on)
+  if (s.underwater) {  // Environment.h:246-250, World.h:692
+    std::vector<double> f(n3);
+    drag_forces(s, x, v, f.data());
+    for (int i = 0; i < N; ++i)
+      for (int r = 0; r < 3; ++r) vstar[3 * i + r] += s.h * ((1.0 / s.mass[i]) * f[3 * i + r]);
+  } else if (s.gravity_on) {
     for (int i = 0; i < N; ++i)
       for (int r = 0; r < 3; ++r) vstar[3 * i + r] += s.g[r] * s.h;
+  }
   std::vector<double> xn(x, x + n3), qn(n3);
   for (int k = 0; k < n3; ++k) qn[k] = xn[k] + s.h * vstar[k];
   // constant part c = M v*/h - L_full xn ; per iteration rhs = c + J d(y)
@@ -775,6 +946,7 @@ void step(Scene& s, double* x, double* v, const double* act, StepTape* tape, int
   if (tape) {
     tape->xn = xn;
     tape->vstar = vstar;
+    tape->vn.assign(v, v + n3);
     tape->ys.clear();
     tape->citers.clear();
     tape->final_latest = true;
@@ -1087,6 +1259,7 @@ void step_backward(Scene& s, const StepTape& tp, const double* gx, const double*
     gxn[k] += xbar[k];
     gvn[k] += vsb[k];
   }
+  if (s.underwater) drag_backward(s, tp.xn.data(), tp.vn.data(), vsb.data(), gxn, gvn);
 }
 
 std::vector<int> surface_vertices(int N, const std::vector<Tet>& tets) {
@@ -1333,6 +1506,28 @@ int orc_muscle_segment_select(void* p, const int* tets, int T, const double* sta
   }
   return n;
 }
+
+void orc_set_underwater(void* p, int on) {
+  Scene* s = (Scene*)p;
+  s->underwater = on;
+  s->contours = on ? build_contours(*s) : std::vector<std::array<int, 3>>();
+}
+int orc_num_drag_faces(void* p) { return (int)((Scene*)p)->contours.size(); }
+void orc_get_drag_faces(void* p, int* out) {
+  Scene* s = (Scene*)p;
+  for (size_t i = 0; i < s->contours.size(); ++i)
+    for (int k = 0; k < 3; ++k) out[3 * i + k] = s->contours[i][k];
+}
+void orc_drag_face(const double* n, const double* w, double* F, double* J) {
+  drag_face_t(n, w, F);
+  if (J) {
+    double Jm[3][6];
+    drag_face_jac(n, w, Jm);
+    for (int c = 0; c < 3; ++c)
+      for (int k = 0; k < 6; ++k) J[6 * c + k] = Jm[c][k];
+  }
+}
+void orc_drag_forces(void* p, const double* x, const double* v, double* f) { drag_forces(*(Scene*)p, x, v, f); }
 
 void orc_set_sim(void* p, double h, int max_iter, double damping, const double* g, int gravity_on, int contact,
                  int max_iter1, double alpha, double mu) {

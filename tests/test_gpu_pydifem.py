@@ -2,7 +2,8 @@
 own scene files (tests/golden/scenes): forward_all / backward_all on
 bridgesus (attachments, gravity -100 y, collision handling on without
 obstacles) and forward_act / backward_act on the octopus (16 muscles,
-actuator type 2; drag forces switched off).
+actuator type 2), without and with its underwater drag; underwater drag on
+a batch of rollouts of a box through the engine.
 
 The oracle gets the same scene description and the segment activations that
 the actuation produced; the actuation itself (host code) is pinned on the CPU
@@ -88,3 +89,71 @@ def test_octopus_act_steps_and_gradient(oracle_mod):
     ref_tau = actuation.set_actions_vjp(go["act"][0], tape)
     assert np.abs(ref_tau).max() > 0
     assert rel(dtau, ref_tau) < 1e-9
+
+
+def test_octopus_underwater_act_steps_and_gradient(oracle_mod):
+    """octopus.json as the reference ships it: underwater, so each step
+    starts with the drag kick of the surface faces; the body starts moving
+    at about 0.3 m/s so that faces see flow from every side."""
+    from diff_fem_amd import pydifem as pd
+    env = pd.init_env(1, "/data/octopus/octopus.json", ROOT)
+    d = env._desc
+    assert d.underwater
+    o = oracle_mod.OracleScene(d)
+    assert env._scene.info()["n_drag_faces"] == o.drag_faces().shape[0] > 0
+    key = actuation.key_table(16)
+    rng = np.random.default_rng(18)
+    q = np.asarray(pd.vector2double(env.mSoftWorld.mX))
+    qd = np.tile([0.1, -0.25, 0.15], d.N) + 0.05 * rng.standard_normal(q.size)
+    assert np.abs(o.drag_forces(q, qd)).max() > 0
+    z = np.zeros(0)
+    levels = np.zeros(800)
+    phase = 0
+    for k, is_act in enumerate([True, False, True]):
+        tau = rng.uniform(-4.5, 4.5, 64)
+        if is_act:
+            levels = actuation.set_actions(levels, key, phase, tau)[0]
+            phase += 1
+        q1, qd1, _, _ = pd.forward_act(q, qd, z, z, tau, env, is_act)
+        xo, vo, _ = o.rollout(q, qd, 1, act=levels[None])
+        assert rel(q1, xo) < 1e-9 and rel(qd1, vo) < 1e-9, k
+        q, qd = np.asarray(q1), np.asarray(qd1)
+    tau = rng.uniform(-4.5, 4.5, 64)
+    saved = env.mDfobjs[0].GetActivationLevelsAggregate()
+    gx, gv = rng.standard_normal(q.size), rng.standard_normal(q.size)
+    dq, dqd, _, _, dtau = pd.backward_act(q, qd, z, z, tau, gx, gv, z, z, env, True)
+    lv, tape = actuation.set_actions(saved, key, env.mDfobjs[0].mPhase - 1, tau)
+    o.rollout(q, qd, 1, act=lv[None], record=True)
+    go, _ = o.backward(gx, gv)
+    assert rel(dq, go["x0"]) < 1e-9 and rel(dqd, go["v0"]) < 1e-9
+    assert rel(dtau, actuation.set_actions_vjp(go["act"][0], tape)) < 1e-9
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_underwater_box_batch(oracle_mod, precision):
+    """Three rollouts of an underwater box with different start velocities:
+    the drag kernels run per (face, rollout); forward and gradient per
+    rollout against the oracle (FP64 bar, or the mixed bars)."""
+    from diff_fem_amd import engine as eng
+    x, t = scene.box_grid(6, 3, 2)
+    d = scene.SceneDesc(x_rest=x, tets=t, total_mass=2.0, E=np.full(t.shape[0], 8e4), underwater=True, h=0.01,
+                        max_iter=20, gravity=(0.0, 0.0, -9.81))
+    prec = eng.PDM_FP64 if precision == "fp64" else eng.PDM_MIXED
+    tol_x, tol_g = (1e-9, 1e-9) if precision == "fp64" else (1e-5, 1e-4)
+    R, steps = 3, 6
+    s = eng.Scene(d, R=R, precision=prec, flags=0, checkpoint_every=4)
+    assert s.info()["n_drag_faces"] == 2 * 2 * (6 * 3 + 3 * 2 + 6 * 2)
+    o = oracle_mod.OracleScene(d)
+    rng = np.random.default_rng(21)
+    x0 = np.tile(d.x_rest.ravel(), (R, 1)) + 1e-3 * rng.standard_normal((R, 3 * d.N))
+    v0 = np.stack([np.tile(u, d.N) for u in ([0.0, 0.0, 0.6], [0.4, -0.3, 0.0], [-0.2, 0.5, -0.4])])
+    xT, vT = s.forward_host(x0, v0, steps, record=True)
+    gx, gv = rng.standard_normal((R, 3 * d.N)), rng.standard_normal((R, 3 * d.N))
+    g = s.backward_host(gx, gv)
+    for r in range(R):
+        xo, vo, _ = o.rollout(x0[r], v0[r], steps, record=True)
+        assert rel(xT[r], xo) < tol_x and rel(vT[r], vo) < tol_x, r
+        go, _ = o.backward(gx[r], gv[r])
+        assert rel(g["x0"][r], go["x0"]) < tol_g and rel(g["v0"][r], go["v0"]) < tol_g, r
+    # no gravity under water: without the drag the box would fly straight
+    assert np.abs(vT[0].reshape(-1, 3)[:, 2].mean()) < 0.6

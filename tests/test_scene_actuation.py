@@ -86,7 +86,6 @@ def test_two_meshes_are_two_bodies(tmp_path):
 
 @pytest.mark.parametrize("para,mesh,extra,err", [
     ({"Self-collision": True}, {}, None, NotImplementedError),
-    ({"Is underwater": 1}, {}, None, NotImplementedError),
     ({}, {}, {"Obstacles": [{"Obj filename": "box.obj"}]}, NotImplementedError),
     ({}, {}, {"Clothes": [{"Obj filename": "box.obj"}]}, NotImplementedError),
     ({}, {"Bones": [{"Parent ID": -1}]}, None, NotImplementedError),
@@ -98,6 +97,15 @@ def test_two_meshes_are_two_bodies(tmp_path):
 def test_unsupported_and_invalid_scenes_raise(tmp_path, para, mesh, extra, err):
     _write_scene(tmp_path, para, [dict({"Obj filename": "box.obj"}, **mesh)], extra)
     with pytest.raises(err):
+        scene.load_json_scene("s.json", str(tmp_path))
+
+
+def test_underwater_needs_one_mesh(tmp_path):
+    _write_scene(tmp_path, {"Is underwater": 1}, [{"Obj filename": "box.obj"}])
+    d = scene.load_json_scene("s.json", str(tmp_path))
+    assert d.underwater
+    _write_scene(tmp_path, {"Is underwater": 1}, [{"Obj filename": "box.obj"}] * 2)
+    with pytest.raises(NotImplementedError, match="Is underwater"):
         scene.load_json_scene("s.json", str(tmp_path))
 
 
@@ -146,9 +154,8 @@ def test_octopus_scene_groups():
     np.testing.assert_array_equal(d.seg_group, np.arange(800))
     info = d.dfobjs[0]
     assert info.actuator_type == 2 and info.n_muscles == 16
-    assert d.gravity == (0.0, 0.0, 0.0) and not d.contact
-    with pytest.raises(NotImplementedError, match="Is underwater"):
-        scene.load_json_scene("/data/octopus/octopus.json", ROOT)
+    assert d.gravity == (0.0, 0.0, 0.0) and not d.contact and not d.underwater
+    assert scene.load_json_scene("/data/octopus/octopus.json", ROOT).underwater
 
 
 def test_key_table_closed_form():
