@@ -242,69 +242,74 @@ __device__ __forceinline__ void volproj(const T* S, T* D) {
   }
 }
 
-// Same projection with its Jacobian J[i][j] = d(S_i + D_i)/d S_j (forward
-// mode), and the divided differences rdd = (phi_i - phi_j)/(S_i - S_j) of
-// phi = S + D for the pairs (0,1), (0,2), (1,2), carried through the
-// iterations instead of formed by division: with a the current iterate and k
-// the third index, g_i - g_j = -a_k (a_i - a_j), so the ratio r becomes
-// 1 - coef a_k r (r = 1 at D = 0). No cancellation where S_i ~ S_j, and the
-// value at S_i = S_j is the derivative limit.
+// Volume projection with the two things its VJP needs: gS = J^T hd, where
+// J = d(S + D)/dS, by a reverse sweep over the five stored iterates (one
+// adjoint vector instead of nine forward tangents), and the divided
+// differences rdd = (phi_i - phi_j)/(S_i - S_j) of phi = S + D for the pairs
+// (0,1), (0,2), (1,2), carried through the iterations as ratios (with a the
+// current iterate and k the third index, g_i - g_j = -a_k (a_i - a_j), so a
+// ratio r becomes 1 - coef a_k r; r = 1 at D = 0): no cancellation where
+// S_i ~ S_j, and the value at S_i = S_j is the derivative limit.
 template <typename T>
-__device__ __forceinline__ void volproj_jac(const T* S, T* D, T* J, T* rdd) {
-  T d[3] = {0, 0, 0};
-  T dd[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+__device__ __forceinline__ void volproj_vjp(const T* S, const T* hd, T* D, T* rdd, T* gS) {
+  T dk[5][3];
   T r0 = 1, r1 = 1, r2 = 1;
+  T d0 = 0, d1 = 0, d2 = 0;
 #pragma unroll
   for (int it = 0; it < 5; ++it) {
-    T a[3] = {S[0] + d[0], S[1] + d[1], S[2] + d[2]};
-    T da[9];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int j = 0; j < 3; ++j) da[i * 3 + j] = (i == j ? T(1) : T(0)) + dd[i * 3 + j];
-    T cd = a[0] * a[1] * a[2] - T(1);
-    T g[3] = {a[1] * a[2], a[0] * a[2], a[0] * a[1]};
-    T dcd[3], dg[9];
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      dcd[j] = da[0 * 3 + j] * a[1] * a[2] + a[0] * da[1 * 3 + j] * a[2] + a[0] * a[1] * da[2 * 3 + j];
-      dg[0 * 3 + j] = da[1 * 3 + j] * a[2] + a[1] * da[2 * 3 + j];
-      dg[1 * 3 + j] = da[0 * 3 + j] * a[2] + a[0] * da[2 * 3 + j];
-      dg[2 * 3 + j] = da[0 * 3 + j] * a[1] + a[0] * da[1 * 3 + j];
-    }
-    T gg = g[0] * g[0] + g[1] * g[1] + g[2] * g[2];
-    T num = g[0] * d[0] + g[1] * d[1] + g[2] * d[2] - cd;
-    T igg = Num<T>::rcp(gg);
-    T coef = num * igg;
-    r0 = T(1) - coef * a[2] * r0;
-    r1 = T(1) - coef * a[1] * r1;
-    r2 = T(1) - coef * a[0] * r2;
-    T nd[3], ndd[9];
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      T dnum = dg[0 * 3 + j] * d[0] + g[0] * dd[0 * 3 + j] + dg[1 * 3 + j] * d[1] + g[1] * dd[1 * 3 + j] +
-               dg[2 * 3 + j] * d[2] + g[2] * dd[2 * 3 + j] - dcd[j];
-      T dgg = T(2) * (g[0] * dg[0 * 3 + j] + g[1] * dg[1 * 3 + j] + g[2] * dg[2 * 3 + j]);
-      T dcoef = (dnum * gg - num * dgg) * (igg * igg);
-#pragma unroll
-      for (int i = 0; i < 3; ++i) ndd[i * 3 + j] = dcoef * g[i] + coef * dg[i * 3 + j];
-    }
-#pragma unroll
-    for (int i = 0; i < 3; ++i) nd[i] = coef * g[i];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) d[i] = nd[i];
-#pragma unroll
-    for (int i = 0; i < 9; ++i) dd[i] = ndd[i];
+    dk[it][0] = d0;
+    dk[it][1] = d1;
+    dk[it][2] = d2;
+    T a0 = S[0] + d0, a1 = S[1] + d1, a2 = S[2] + d2;
+    T cd = a0 * a1 * a2 - T(1);
+    T g0 = a1 * a2, g1 = a0 * a2, g2 = a0 * a1;
+    T coef = (g0 * d0 + g1 * d1 + g2 * d2 - cd) * Num<T>::rcp(g0 * g0 + g1 * g1 + g2 * g2);
+    r0 = T(1) - coef * a2 * r0;
+    r1 = T(1) - coef * a1 * r1;
+    r2 = T(1) - coef * a0 * r2;
+    d0 = coef * g0;
+    d1 = coef * g1;
+    d2 = coef * g2;
   }
-#pragma unroll
-  for (int i = 0; i < 3; ++i) D[i] = d[i];
+  D[0] = d0;
+  D[1] = d1;
+  D[2] = d2;
   rdd[0] = r0;
   rdd[1] = r1;
   rdd[2] = r2;
+  // reverse: phi = S + d_5; bar d_5 = hd, bar S = hd + sum over iterations
+  T bd0 = hd[0], bd1 = hd[1], bd2 = hd[2];
+  T bs0 = hd[0], bs1 = hd[1], bs2 = hd[2];
 #pragma unroll
-  for (int i = 0; i < 3; ++i)
-#pragma unroll
-    for (int j = 0; j < 3; ++j) J[i * 3 + j] = (i == j ? T(1) : T(0)) + dd[i * 3 + j];
+  for (int it = 4; it >= 0; --it) {
+    T e0 = dk[it][0], e1 = dk[it][1], e2 = dk[it][2];
+    T a0 = S[0] + e0, a1 = S[1] + e1, a2 = S[2] + e2;
+    T cd = a0 * a1 * a2 - T(1);
+    T g0 = a1 * a2, g1 = a0 * a2, g2 = a0 * a1;
+    T igg = Num<T>::rcp(g0 * g0 + g1 * g1 + g2 * g2);
+    T num = g0 * e0 + g1 * e1 + g2 * e2 - cd;
+    T coef = num * igg;
+    // d_new = coef g, coef = num / gg, num = g.e - cd, gg = |g|^2
+    T bcoef = bd0 * g0 + bd1 * g1 + bd2 * g2;
+    T bnum = bcoef * igg;
+    T bgg = -bcoef * coef * igg;
+    T bg0 = bd0 * coef + T(2) * g0 * bgg + bnum * e0;
+    T bg1 = bd1 * coef + T(2) * g1 * bgg + bnum * e1;
+    T bg2 = bd2 * coef + T(2) * g2 * bgg + bnum * e2;
+    // num -> e (old iterate) directly and cd = a0 a1 a2 - 1 (bar cd = -bnum)
+    T ba0 = -bnum * g0 + bg1 * a2 + bg2 * a1;
+    T ba1 = -bnum * g1 + bg0 * a2 + bg2 * a0;
+    T ba2 = -bnum * g2 + bg0 * a1 + bg1 * a0;
+    bs0 += ba0;
+    bs1 += ba1;
+    bs2 += ba2;
+    bd0 = ba0 + bnum * g0;
+    bd1 = ba1 + bnum * g1;
+    bd2 = ba2 + bnum * g2;
+  }
+  gS[0] = bs0;
+  gS[1] = bs1;
+  gS[2] = bs2;
 }
 
 // P = wr * md + wv * mdv, where md = U V^T (third column negated when
@@ -333,10 +338,38 @@ __device__ __forceinline__ void corotate_P(const T* F, T wr, T wv, T* P) {
 template <typename T>
 __device__ __forceinline__ void corotate_vjp(const T* F, T wr, T wv, const T* Z, T* Fbar, T* md, T* mdv,
                                              T* zmd = nullptr, T* zmdv = nullptr) {
-  T U[9], S[3], V[9], D[3], J[9], rdd[3];
+  T U[9], S[3], V[9], D[3], rdd[3];
   svd3<T>(F, U, S, V);
-  volproj_jac<T>(S, D, J, rdd);
   bool flip = det3<T>(F) < 0;
+  // H = U^T Z V and Hr = U^T (Z Df) V
+  T ZU[9], H[9], Hr[9];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) ZU[i * 3 + c] = U[0 * 3 + i] * Z[0 * 3 + c] + U[1 * 3 + i] * Z[1 * 3 + c] + U[2 * 3 + i] * Z[2 * 3 + c];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      T h = 0, hr = 0;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        T zu = ZU[i * 3 + c];
+        h += zu * V[c * 3 + j];
+        hr += ((flip && c == 2) ? -zu : zu) * V[c * 3 + j];
+      }
+      H[i * 3 + j] = h;
+      Hr[i * 3 + j] = hr;
+    }
+  // volume projection, its divided differences and the diagonal of the
+  // volume term's adjoint: O_ii = wv (J^T diag H)_i
+  T O[9];
+  {
+    T hd[3] = {H[0], H[4], H[8]}, gS[3];
+    volproj_vjp<T>(S, hd, D, rdd, gS);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) O[i * 3 + i] = wv * gS[i];
+  }
   T phi[3] = {S[0] + D[0], S[1] + D[1], S[2] + D[2]};
   T dr = 0, dv = 0;
 #pragma unroll
@@ -358,31 +391,6 @@ __device__ __forceinline__ void corotate_vjp(const T* F, T wr, T wv, const T* Z,
   if (!md) {
     *zmd = dr;
     *zmdv = dv;
-  }
-  // H = U^T Z V and Hr = U^T (Z Df) V
-  T ZU[9], H[9], Hr[9];
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-#pragma unroll
-    for (int c = 0; c < 3; ++c) ZU[i * 3 + c] = U[0 * 3 + i] * Z[0 * 3 + c] + U[1 * 3 + i] * Z[1 * 3 + c] + U[2 * 3 + i] * Z[2 * 3 + c];
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      T h = 0, hr = 0;
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        T zu = ZU[i * 3 + c];
-        h += zu * V[c * 3 + j];
-        hr += ((flip && c == 2) ? -zu : zu) * V[c * 3 + j];
-      }
-      H[i * 3 + j] = h;
-      Hr[i * 3 + j] = hr;
-    }
-  T O[9];
-#pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    O[i * 3 + i] = wv * (J[0 * 3 + i] * H[0] + J[1 * 3 + i] * H[4] + J[2 * 3 + i] * H[8]);
   }
 #pragma unroll
   for (int i = 0; i < 3; ++i)
