@@ -1173,14 +1173,137 @@ __device__ bool detect_sphere_d(const double* sp, const double* x0, const double
   return false;
 }
 
-// Detection pass for (contact vertex, rollout): add or refresh impacts keyed
-// by (vertex, obstacle) (collisionBrutal.h:312-327); the insertion stamp
-// orders impacts of one vertex like the reference's impact list.
+// ArcSim's kDOP-18 box test in single precision (bvh.cpp:54-96, bvh.hpp:62-150):
+// the swept box of the vertex (x0, x1) against the triangle's box dilated by
+// d on the three axes and sqrt(2) d on the six diagonal ones.
+__device__ inline void kdop_add(const double* p, float* mn, float* mx) {
+  float a = (float)p[0], b = (float)p[1], c = (float)p[2];
+  float q[9] = {a, b, c, a + b, a + c, b + c, a - b, a - c, b - c};
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    mn[i] = fminf(mn[i], q[i]);
+    mx[i] = fmaxf(mx[i], q[i]);
+  }
+}
+__device__ inline bool kdop_overlap(const double* x0, const double* x1, const double* Y, double thick) {
+  float an[9], ax[9], bn[9], bx[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    an[i] = bn[i] = 3.0e38f;
+    ax[i] = bx[i] = -3.0e38f;
+  }
+  kdop_add(x1, an, ax);
+  kdop_add(x0, an, ax);
+  kdop_add(Y, bn, bx);
+  kdop_add(Y + 3, bn, bx);
+  kdop_add(Y + 6, bn, bx);
+  const double d = (double)(float)thick, d2 = sqrt(2.0) * d;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    float lo = (float)((double)bn[i] - (i < 3 ? d : d2)), hi = (float)((double)bx[i] + (i < 3 ? d : d2));
+    if (an[i] > hi || ax[i] < lo) return false;
+  }
+  return true;
+}
+
+__device__ inline double stp3(const double* a, const double* b, const double* c) {
+  return a[0] * (b[1] * c[2] - b[2] * c[1]) + a[1] * (b[2] * c[0] - b[0] * c[2]) + a[2] * (b[0] * c[1] - b[1] * c[0]);
+}
+
+// Vertex x0 -> x1 against a static obstacle triangle Y (3 corners): ArcSim's
+// vertex-face test (collision.cpp:127-138, 211-278) with the face at rest. The
+// coplanarity cubic then has a single root, t = -a0/a1 (the quadratic and
+// cubic terms vanish); at that t and at t = 1 the vertex hits when it lies
+// within the thickness of the face's plane and inside the face (barycentric
+// weights >= -1e-6), or at t = 1 within the thickness of a corner or of an
+// edge's line. The normal is the face normal turned against the vertex's
+// motion, unnormalised as the reference leaves it.
+__device__ bool detect_tri_d(const double* Y, const double* x0, const double* x1, double* nout) {
+  if (!kdop_overlap(x0, x1, Y, kThick)) return false;
+  const double *y0 = Y, *y1 = Y + 3, *y2 = Y + 6;
+  double v0[3], X1[3], X2[3], X3[3], V[3], e1[3], e2[3];
+  for (int r = 0; r < 3; ++r) {
+    v0[r] = x1[r] - x0[r];
+    X1[r] = y0[r] - x0[r];
+    X2[r] = y1[r] - x0[r];
+    X3[r] = y2[r] - x0[r];
+    V[r] = 0.0 - v0[r];
+    e1[r] = y1[r] - y0[r];
+    e2[r] = y2[r] - y0[r];
+  }
+  double l1 = sqrt(e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2]), l2 = sqrt(e2[0] * e2[0] + e2[1] * e2[1] + e2[2] * e2[2]);
+  for (int r = 0; r < 3; ++r) {
+    e1[r] /= l1;
+    e2[r] /= l2;
+  }
+  double fn[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+  double f2 = fn[0] * fn[0] + fn[1] * fn[1] + fn[2] * fn[2];
+  if (!(f2 >= 1e-6)) return false;  // degenerate face: infinite distance
+  double fl = sqrt(f2), nu[3] = {fn[0] / fl, fn[1] / fl, fn[2] / fl};
+  double a0 = stp3(X1, X2, X3);
+  double a1 = stp3(V, X2, X3) + stp3(X1, V, X3) + stp3(X1, X2, V);
+  double ts[2];
+  int nt = 0;
+  if (a1 != 0.0) ts[nt++] = -a0 / a1;
+  ts[nt++] = 1.0;
+  for (int k = 0; k < nt; ++k) {
+    double t = ts[k];
+    if (t < 0 || t > 1) continue;
+    double p[3], q0[3], q1[3], q2[3];
+    for (int r = 0; r < 3; ++r) {
+      p[r] = x0[r] + t * v0[r];
+      q0[r] = y0[r] - p[r];
+      q1[r] = y1[r] - p[r];
+      q2[r] = y2[r] - p[r];
+    }
+    double h = (p[0] - y0[0]) * nu[0] + (p[1] - y0[1]) * nu[1] + (p[2] - y0[2]) * nu[2];
+    double b0 = stp3(q1, q2, nu), b1 = stp3(q2, q0, nu), b2 = stp3(q0, q1, nu);
+    double sb = b0 + b1 + b2;
+    double w1 = -b0 / sb, w2 = -b1 / sb, w3 = -b2 / sb;
+    bool inside = fmin(fmin(-w1, -w2), -w3) >= -1e-6;
+    if (t == 1.0) {
+      const double* Yk[3] = {y0, y1, y2};
+      for (int c = 0; c < 3; ++c) {
+        const double *A = Yk[c], *B = Yk[(c + 1) % 3];
+        double pa[3] = {p[0] - A[0], p[1] - A[1], p[2] - A[2]};
+        double ab[3] = {B[0] - A[0], B[1] - A[1], B[2] - A[2]};
+        inside = inside || sqrt(pa[0] * pa[0] + pa[1] * pa[1] + pa[2] * pa[2]) < kThick;
+        double dd = pa[0] * ab[0] + pa[1] * ab[1] + pa[2] * ab[2], n2 = ab[0] * ab[0] + ab[1] * ab[1] + ab[2] * ab[2];
+        double e[3];
+        for (int r = 0; r < 3; ++r) e[r] = pa[r] - dd * ab[r] / n2;
+        double el = sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+        inside = inside || el < kThick;
+      }
+    }
+    double wv[3] = {w1 * V[0] + w2 * V[0] + w3 * V[0], w1 * V[1] + w2 * V[1] + w3 * V[1],
+                    w1 * V[2] + w2 * V[2] + w3 * V[2]};
+    double sg = (nu[0] * wv[0] + nu[1] * wv[1] + nu[2] * wv[2]) > 0 ? -1.0 : 1.0;
+    if (fabs(h) < kThick && inside) {
+      for (int r = 0; r < 3; ++r) nout[r] = sg * fn[r];
+      return true;
+    }
+  }
+  return false;
+}
+
+// Detection pass for (contact vertex, rollout). The reference keeps a list of
+// impacts keyed by (vertex, obstacle face), appending new ones and refreshing
+// known ones in place (collisionBrutal.h:291-327), and a vertex's response
+// uses the last of its impacts in that list (World.h:1417-1425). Only that
+// winner matters, so each (vertex, rollout) keeps: the winner's stamp
+// (pass * nobs + obstacle, the list position order) and normal, and the set of
+// obstacles seen this step (up to kSeen; more sets cactive[R], an error). A
+// known obstacle only refreshes its normal; a new one wins when its stamp is
+// the largest. Obstacles: planes, spheres, then triangles (via their BVH).
+constexpr int kSeen = 8;
 __global__ void k_contact_detect(const int* __restrict__ cvtx, int ncv, const double* __restrict__ Xn,
                                  const double* __restrict__ Ori, int R, const double* __restrict__ planes, int np,
-                                 const double* __restrict__ spheres, int nsph, int* __restrict__ imp_stamp,
-                                 double* __restrict__ imp_n, int pass, const int* __restrict__ cactive,
-                                 int* __restrict__ fresh) {
+                                 const double* __restrict__ spheres, int nsph, const double* __restrict__ tris,
+                                 int ntri, const double* __restrict__ bvh_box, const int* __restrict__ bvh_a,
+                                 const int* __restrict__ bvh_b, const int* __restrict__ bvh_tri,
+                                 int* __restrict__ win_st, int* __restrict__ win_o, double* __restrict__ win_n,
+                                 int* __restrict__ nseen, int* __restrict__ seen, int pass,
+                                 int* __restrict__ cactive, int* __restrict__ fresh) {
   long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (idx >= (long long)ncv * R) return;
   int ci = (int)(idx / R), r = (int)(idx % R);
@@ -1189,32 +1312,93 @@ __global__ void k_contact_detect(const int* __restrict__ cvtx, int ncv, const do
   int v = cvtx[ci];
   const double* x0 = Xn + (long long)v * C + 3 * r;
   const double* x1 = Ori + (long long)v * C + 3 * r;
-  int nobs = np + nsph;
-  bool any = false;
-  for (int o = 0; o < nobs; ++o) {
-    double n[3];
-    bool hit = o < np ? detect_plane_d(planes + 6 * o, x0, x1, n) : detect_sphere_d(spheres + 4 * (o - np), x0, x1, n);
-    if (!hit) continue;
+  const int nobs = np + nsph + ntri;
+  int wst = win_st[idx], wo = win_o[idx], ns = nseen[idx];
+  double wn[3] = {win_n[3 * idx], win_n[3 * idx + 1], win_n[3 * idx + 2]};
+  bool any = false, over = false;
+  auto record = [&](int o, const double* n) {
     any = true;
-    long long k = ((long long)ci * nobs + o) * R + r;
-    if (imp_stamp[k] < 0) imp_stamp[k] = pass * nobs + o;
-    imp_n[3 * k] = n[0];
-    imp_n[3 * k + 1] = n[1];
-    imp_n[3 * k + 2] = n[2];
+    for (int k = 0; k < ns; ++k)
+      if (seen[(long long)k * ncv * R + idx] == o) {
+        if (o == wo)
+          for (int c = 0; c < 3; ++c) wn[c] = n[c];
+        return;
+      }
+    if (ns == kSeen) {
+      over = true;
+      return;
+    }
+    seen[(long long)ns * ncv * R + idx] = o;
+    ++ns;
+    int st = pass * nobs + o;
+    if (st > wst) {
+      wst = st;
+      wo = o;
+      for (int c = 0; c < 3; ++c) wn[c] = n[c];
+    }
+  };
+  double n[3];
+  for (int o = 0; o < np; ++o)
+    if (detect_plane_d(planes + 6 * o, x0, x1, n)) record(o, n);
+  for (int o = 0; o < nsph; ++o)
+    if (detect_sphere_d(spheres + 4 * o, x0, x1, n)) record(np + o, n);
+  if (ntri > 0) {
+    // swept box of the vertex, widened by the thickness and a margin for the
+    // single-precision box test, against the BVH's double boxes
+    double lo[3], hi[3];
+    for (int c = 0; c < 3; ++c) {
+      double m = 2.0 * kThick + 1e-6 * (1.0 + fabs(x0[c]) + fabs(x1[c]));
+      lo[c] = fmin(x0[c], x1[c]) - m;
+      hi[c] = fmax(x0[c], x1[c]) + m;
+    }
+    int stack[64], sp = 0;
+    stack[sp++] = 0;
+    while (sp > 0) {
+      int k = stack[--sp];
+      const double* b = bvh_box + 6 * (long long)k;
+      if (lo[0] > b[3] || lo[1] > b[4] || lo[2] > b[5] || hi[0] < b[0] || hi[1] < b[1] || hi[2] < b[2]) continue;
+      int a = bvh_a[k];
+      if (a >= 0) {
+        if (sp + 2 <= 64) {
+          stack[sp++] = bvh_b[k];
+          stack[sp++] = a;
+        } else {
+          over = true;
+        }
+        continue;
+      }
+      for (int q = -1 - a, qe = -1 - a + bvh_b[k]; q < qe; ++q) {
+        int t = bvh_tri[q];
+        if (detect_tri_d(tris + 9 * (long long)t, x0, x1, n)) record(np + nsph + t, n);
+      }
+    }
   }
+  win_st[idx] = wst;
+  win_o[idx] = wo;
+  nseen[idx] = ns;
+  win_n[3 * idx] = wn[0];
+  win_n[3 * idx + 1] = wn[1];
+  win_n[3 * idx + 2] = wn[2];
   if (any) atomicOr(&fresh[r], 1);
+  if (over) atomicOr(&cactive[R], 1);
 }
 
-__global__ void k_contact_reset(int* __restrict__ imp_stamp, long long n, int* __restrict__ cactive,
-                                int* __restrict__ cfirst, int* __restrict__ fresh, int* __restrict__ clast, int R) {
+__global__ void k_contact_reset(int* __restrict__ win_st, int* __restrict__ win_o, int* __restrict__ nseen, long long n,
+                                int* __restrict__ cactive, int* __restrict__ cfirst, int* __restrict__ fresh,
+                                int* __restrict__ clast, int R) {
   long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (idx < n) imp_stamp[idx] = -1;
+  if (idx < n) {
+    win_st[idx] = -1;
+    win_o[idx] = -1;
+    nseen[idx] = 0;
+  }
   if (idx < R) {
     cactive[idx] = 1;
     cfirst[idx] = 1;
     fresh[idx] = 0;
     clast[idx] = 1;
   }
+  if (idx == 0) cactive[R] = 0;
 }
 
 // Rollouts with no fresh impact leave the contact phase; the ones that stay
@@ -1283,7 +1467,7 @@ __global__ void k_contact_jacobi(const int* __restrict__ Lptr, const int* __rest
                                  const double* __restrict__ Lval, const double* __restrict__ B,
                                  const double* __restrict__ Dn, const double* __restrict__ Xn,
                                  const double* __restrict__ Kdiag, const int* __restrict__ cv_of_v,
-                                 const int* __restrict__ imp_stamp, const double* __restrict__ imp_n, int nobs,
+                                 const int* __restrict__ win_st, const double* __restrict__ win_n,
                                  double mu, double alpha, double h, int first_iter, const int* __restrict__ alist,
                                  const int* __restrict__ acount, double* __restrict__ Prev, double* __restrict__ Xit,
                                  int N, int R, double* __restrict__ tRC, double* __restrict__ tN) {
@@ -1324,16 +1508,9 @@ __global__ void k_contact_jacobi(const int* __restrict__ Lptr, const int* __rest
     tN[tb + 2] = 0.0;
   }
   if (ci >= 0) {
-    int best = -1, bst = -1;
-    for (int o = 0; o < nobs; ++o) {
-      int st = imp_stamp[((long long)ci * nobs + o) * R + r];
-      if (st > bst) {
-        bst = st;
-        best = o;
-      }
-    }
-    if (best >= 0) {
-      const double* n = imp_n + 3 * (((long long)ci * nobs + best) * R + r);
+    const long long wi = (long long)ci * R + r;
+    if (win_st[wi] >= 0) {
+      const double* n = win_n + 3 * wi;
       if (tN) {
         long long tb = (long long)ci * C + 3 * r;
         tN[tb] = n[0];
@@ -1646,10 +1823,13 @@ struct pdm_scene_s {
   DevBuf<double> FF, GXf, GWf;
   // contact
   int ncv = 0, nobs = 0;
-  DevBuf<int> cvtx, cv_of_v, imp_stamp, cactive, iactive, cfirst, fresh, npass, citers, clast, cinj;
+  DevBuf<int> cvtx, cv_of_v, win_st, win_o, nseen, seen, cactive, iactive, cfirst, fresh, npass, citers, clast, cinj;
+  int ntri = 0;
+  DevBuf<double> d_tris, d_bvh_box;
+  DevBuf<int> d_bvh_a, d_bvh_b, d_bvh_tri;
   std::vector<std::unique_ptr<ContactTape>> ctape;  // per step of the segment being reversed
   DevBuf<double> cXB, cPB, cOB, cGf, cCD;
-  DevBuf<double> Kdiag, d_planes, d_spheres, Ori, Dn, Prev, imp_n;
+  DevBuf<double> Kdiag, d_planes, d_spheres, Ori, Dn, Prev, win_n;
   std::vector<int> npass_host, citers_host;  // R*steps of the last forward
   // adjoint buffers
   DevBuf<double> gX, gV, Yb, Xb, Zs, Zb, LZ, gEpart, gMpart, gact;
@@ -1721,7 +1901,8 @@ int upload_static(pdm_scene s) {
   if (s->nfaces > 0 && (s->dcf.upload(S.cf.data(), S.cf.size()) || s->dvfptr.upload(S.vfptr.data(), S.N + 1) ||
                         s->dvfq.upload(S.vfq.data(), S.vfq.size())))
     return 1;
-  s->nobs = (int)(S.planes.size() / 6 + S.spheres.size() / 4);
+  s->ntri = (int)(S.tris.size() / 9);
+  s->nobs = (int)(S.planes.size() / 6 + S.spheres.size() / 4) + s->ntri;
   s->ncv = (S.contact && s->nobs > 0) ? (int)S.contact_vtx.size() : 0;
   if (s->ncv > 0) {
     std::vector<int> cvo(S.N, -1);
@@ -1731,6 +1912,11 @@ int upload_static(pdm_scene s) {
     pl.push_back(0);
     sp.push_back(0);
     if (s->d_planes.upload(pl.data(), pl.size()) || s->d_spheres.upload(sp.data(), sp.size())) return 1;
+    if (s->ntri > 0 &&
+        (s->d_tris.upload(S.tris.data(), S.tris.size()) || s->d_bvh_box.upload(S.bvh_box.data(), S.bvh_box.size()) ||
+         s->d_bvh_a.upload(S.bvh_a.data(), S.bvh_a.size()) || s->d_bvh_b.upload(S.bvh_b.data(), S.bvh_b.size()) ||
+         s->d_bvh_tri.upload(S.bvh_tri.data(), S.bvh_tri.size())))
+      return 1;
   }
   {
     std::vector<int> mp(S.mptr);
@@ -1909,6 +2095,14 @@ int host_setup(const pdm_scene_desc* d, pdm_scene s) {
   S.mu = d->mu;
   S.planes.assign(d->planes, d->planes + 6 * (size_t)d->n_planes);
   S.spheres.assign(d->spheres, d->spheres + 4 * (size_t)d->n_spheres);
+  if (d->n_tris > 0) {
+    if (!d->tris) {
+      g_err = "n_tris > 0 without tris";
+      return 1;
+    }
+    S.tris.assign(d->tris, d->tris + 9 * (size_t)d->n_tris);
+    build_tri_bvh(S);
+  }
   S.underwater = d->underwater;
   if (S.underwater) {
     std::vector<int> cf = contour_faces(d->x_rest, d->tets, d->n_tets);
@@ -1975,10 +2169,11 @@ int ensure_batch_buffers(pdm_scene s) {
   if (s->err.alloc(s->R) || s->active.alloc(s->R) || s->iters_dev.alloc(s->R)) return 1;
   if (s->alist.alloc(s->R) || s->acount.alloc(1)) return 1;
   if (s->ncv > 0) {
-    size_t ni = (size_t)s->ncv * s->nobs * s->R;
-    if (s->Ori.alloc(nc) || s->Dn.alloc(nc) || s->Prev.alloc(nc) || s->imp_stamp.alloc(ni) || s->imp_n.alloc(3 * ni) ||
-        s->cactive.alloc(s->R) || s->iactive.alloc(s->R) || s->cfirst.alloc(s->R) || s->fresh.alloc(s->R) ||
-        s->npass.alloc(s->R) || s->citers.alloc(s->R) || s->clast.alloc(s->R) || s->cinj.alloc(s->R))
+    size_t ni = (size_t)s->ncv * s->R;
+    if (s->Ori.alloc(nc) || s->Dn.alloc(nc) || s->Prev.alloc(nc) || s->win_st.alloc(ni) || s->win_o.alloc(ni) ||
+        s->nseen.alloc(ni) || s->seen.alloc(kSeen * ni) || s->win_n.alloc(3 * ni) || s->cactive.alloc(s->R + 1) ||
+        s->iactive.alloc(s->R) || s->cfirst.alloc(s->R) || s->fresh.alloc(s->R) || s->npass.alloc(s->R) ||
+        s->citers.alloc(s->R) || s->clast.alloc(s->R) || s->cinj.alloc(s->R))
       return 1;
   }
   if (s->tuned_C != s->C && autotune_levels(s)) return 1;
@@ -2313,11 +2508,11 @@ int contact_phase(pdm_scene s, const double* act, int act_steps, int step, Conta
   int nchunk = cdiv(N, kChunk);
   dim3 ugrid(cdiv(C, 64), nchunk), ublock(64, 4);
   int np = (int)(S.planes.size() / 6), nsph = (int)(S.spheres.size() / 4);
-  long long ni = (long long)s->ncv * s->nobs * R;
+  long long ni = (long long)s->ncv * R;
   HIPX(hipMemcpyAsync(s->Ori.p, s->Xit.p, nc * 8, hipMemcpyDeviceToDevice, s->stream));
   hipLaunchKernelGGL(k_contact_reset, dim3(cdiv(std::max<long long>(ni, R), kThreads)), dim3(kThreads), 0, s->stream,
-                     s->imp_stamp.p, ni, s->cactive.p, s->cfirst.p, s->fresh.p, s->clast.p, R);
-  std::vector<int> ca(R);
+                     s->win_st.p, s->win_o.p, s->nseen.p, ni, s->cactive.p, s->cfirst.p, s->fresh.p, s->clast.p, R);
+  std::vector<int> ca(R + 1);
   size_t cvc = (size_t)s->ncv * C;
   if (tp) {
     tp->G = 0;
@@ -2327,13 +2522,18 @@ int contact_phase(pdm_scene s, const double* act, int act_steps, int step, Conta
   }
   for (int pass = 0; pass <= S.outer_cap; ++pass) {
     hipLaunchKernelGGL(k_contact_detect, dim3(cdiv((long long)s->ncv * R, kThreads)), dim3(kThreads), 0, s->stream,
-                       s->cvtx.p, s->ncv, s->Xn.p, s->Ori.p, R, s->d_planes.p, np, s->d_spheres.p, nsph,
-                       s->imp_stamp.p, s->imp_n.p, pass, s->cactive.p, s->fresh.p);
+                       s->cvtx.p, s->ncv, s->Xn.p, s->Ori.p, R, s->d_planes.p, np, s->d_spheres.p, nsph, s->d_tris.p,
+                       s->ntri, s->d_bvh_box.p, s->d_bvh_a.p, s->d_bvh_b.p, s->d_bvh_tri.p, s->win_st.p, s->win_o.p,
+                       s->win_n.p, s->nseen.p, s->seen.p, pass, s->cactive.p, s->fresh.p);
     if (pass == S.outer_cap) break;  // cap reached: keep ori_x
     hipLaunchKernelGGL(k_contact_pass_begin, dim3(cdiv(R, 64)), dim3(64), 0, s->stream, s->cactive.p, s->fresh.p,
                        s->iactive.p, s->npass.p, R);
-    HIPX(hipMemcpyAsync(ca.data(), s->cactive.p, R * sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    HIPX(hipMemcpyAsync(ca.data(), s->cactive.p, (R + 1) * sizeof(int), hipMemcpyDeviceToHost, s->stream));
     HIPX(hipStreamSynchronize(s->stream));
+    if (ca[R]) {
+      g_err = "contact: a vertex met more than 8 distinct obstacle faces in one step";
+      return 1;
+    }
     bool any = false;
     for (int r = 0; r < R; ++r) any |= ca[r] != 0;
     if (!any) break;
@@ -2358,7 +2558,7 @@ int contact_phase(pdm_scene s, const double* act, int act_steps, int step, Conta
         launch_local<double>(s, s->Ori.p, act, act_steps, step, s->iactive.p);
       hipLaunchKernelGGL(k_contact_jacobi, dim3(cdiv((long long)N * R, kThreads)), dim3(kThreads), 0, s->stream,
                          s->Lptr.p, s->Lcol.p, s->Lval.p, s->B.p, s->Dn.p, s->Xn.p, s->Kdiag.p, s->cv_of_v.p,
-                         s->imp_stamp.p, s->imp_n.p, s->nobs, S.mu, S.alpha, S.h, i == 0 ? 1 : 0, s->alist.p,
+                         s->win_st.p, s->win_n.p, S.mu, S.alpha, S.h, i == 0 ? 1 : 0, s->alist.p,
                          s->acount.p, s->Prev.p, s->Xit.p, N, R, tp ? tp->rc.p + (size_t)tp->G * cvc : nullptr,
                          tp ? tp->nrm.p + (size_t)tp->G * cvc : nullptr);
       if (tp) {
@@ -2972,6 +3172,7 @@ int pdm_scene_info(pdm_scene s, pdm_info* info) {
   info->reg_used = s->F.reg_used;
   info->n_contact_vertices = (int)s->S.contact_vtx.size();
   info->n_drag_faces = s->nfaces;
+  info->n_obstacle_tris = s->ntri;
   info->batch = s->R;
   info->precision = s->precision;
   return 0;

@@ -40,6 +40,12 @@ struct HostScene {
   std::vector<int> contact_vtx;
   std::vector<double> planes;      // 6 per plane
   std::vector<double> spheres;     // 4 per sphere
+  std::vector<double> tris;        // 9 per static obstacle triangle
+  // bounding-volume hierarchy over tris: node k has box bvh_box[6k..6k+5]
+  // (min xyz, max xyz); inner node: bvh_a = left child, bvh_b = right child;
+  // leaf: bvh_a = -1 - first, bvh_b = count, its triangles bvh_tri[first..]
+  std::vector<double> bvh_box;
+  std::vector<int> bvh_a, bvh_b, bvh_tri;
   // underwater drag: outward-oriented surface faces (engine ids, 3 per face,
   // in the reference's face order) and, per vertex, its incident faces as
   // 3 * face + corner in ascending face order (vfptr: N+1)
@@ -112,6 +118,8 @@ bool add_muscles(HostScene& S, const double* x_caller, const int* tets_caller, i
 // Outward-oriented surface faces of a tet mesh (caller numbering), in the
 // order Dfobj::Initialize lists its contours (Dfobj.h:497-596).
 std::vector<int> contour_faces(const double* xrest, const int* tets, int T);
+// Median-split bounding-volume hierarchy of the obstacle triangles (S.tris).
+void build_tri_bvh(HostScene& S);
 // Incident faces of every vertex (3 * face + corner, ascending face order).
 void vertex_faces(int N, const std::vector<int>& cf, std::vector<int>& vfptr, std::vector<int>& vfq);
 std::vector<int> surface_vertices(int N, const std::vector<int>& tv, int T);

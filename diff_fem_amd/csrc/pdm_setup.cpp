@@ -186,6 +186,67 @@ std::vector<int> contour_faces(const double* xrest, const int* tets, int T) {
   return out;
 }
 
+// Leaves hold at most 4 triangles; a node splits its triangles at the median
+// centroid along its box's longest axis (ties by triangle id, so the tree
+// depends only on the input).
+void build_tri_bvh(HostScene& S) {
+  int F = (int)(S.tris.size() / 9);
+  S.bvh_box.clear();
+  S.bvh_a.clear();
+  S.bvh_b.clear();
+  S.bvh_tri.resize(F);
+  std::iota(S.bvh_tri.begin(), S.bvh_tri.end(), 0);
+  if (F == 0) return;
+  auto cen = [&](int t, int ax) {
+    const double* q = &S.tris[9 * (size_t)t];
+    return q[ax] + q[3 + ax] + q[6 + ax];
+  };
+  struct Item { int node, lo, hi; };
+  std::vector<Item> st;
+  auto new_node = [&]() {
+    S.bvh_box.insert(S.bvh_box.end(), 6, 0.0);
+    S.bvh_a.push_back(0);
+    S.bvh_b.push_back(0);
+    return (int)S.bvh_a.size() - 1;
+  };
+  st.push_back({new_node(), 0, F});
+  while (!st.empty()) {
+    Item it = st.back();
+    st.pop_back();
+    double* bx = &S.bvh_box[6 * (size_t)it.node];
+    for (int r = 0; r < 3; ++r) {
+      bx[r] = 1e300;
+      bx[3 + r] = -1e300;
+    }
+    for (int k = it.lo; k < it.hi; ++k) {
+      const double* q = &S.tris[9 * (size_t)S.bvh_tri[k]];
+      for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r) {
+          bx[r] = std::min(bx[r], q[3 * c + r]);
+          bx[3 + r] = std::max(bx[3 + r], q[3 * c + r]);
+        }
+    }
+    if (it.hi - it.lo <= 4) {
+      S.bvh_a[it.node] = -1 - it.lo;
+      S.bvh_b[it.node] = it.hi - it.lo;
+      continue;
+    }
+    int ax = 0;
+    for (int r = 1; r < 3; ++r)
+      if (bx[3 + r] - bx[r] > bx[3 + ax] - bx[ax]) ax = r;
+    int mid = (it.lo + it.hi) / 2;
+    std::nth_element(S.bvh_tri.begin() + it.lo, S.bvh_tri.begin() + mid, S.bvh_tri.begin() + it.hi, [&](int a, int b) {
+      double ca = cen(a, ax), cb = cen(b, ax);
+      return ca < cb || (ca == cb && a < b);
+    });
+    int l = new_node(), r = new_node();
+    S.bvh_a[it.node] = l;
+    S.bvh_b[it.node] = r;
+    st.push_back({l, it.lo, mid});
+    st.push_back({r, mid, it.hi});
+  }
+}
+
 void vertex_faces(int N, const std::vector<int>& cf, std::vector<int>& vfptr, std::vector<int>& vfq) {
   vfptr.assign(N + 1, 0);
   for (int v : cf) ++vfptr[v + 1];

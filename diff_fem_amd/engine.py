@@ -90,6 +90,9 @@ def make_desc(desc):
     d.planes = _dptr(pl)
     d.n_spheres = int(desc.spheres.shape[0])
     d.spheres = _dptr(sp)
+    if desc.tris.shape[0]:
+        d.n_tris = int(desc.tris.shape[0])
+        d.tris = _dptr(arr(desc.tris.ravel(), np.float64))
     return d, keep
 
 

@@ -57,6 +57,7 @@ class SceneDesc:
     mu: float = 1.0
     planes: np.ndarray = None          # (P,6): point, normal
     spheres: np.ndarray = None         # (Q,4): centre, radius
+    tris: np.ndarray = None            # (F,9): static obstacle triangles, three corners each
     # several bodies in one world (one Dfobj per mesh): per-tet body id, each
     # body's "Mass", and the nu entering mMu per tet (bodies built with their
     # own Poisson ratio); None = one body / nu_fwd everywhere
@@ -99,6 +100,9 @@ class SceneDesc:
             self.spheres = np.zeros((0, 4))
         self.planes = np.ascontiguousarray(self.planes, dtype=np.float64).reshape(-1, 6)
         self.spheres = np.ascontiguousarray(self.spheres, dtype=np.float64).reshape(-1, 4)
+        if self.tris is None:
+            self.tris = np.zeros((0, 9))
+        self.tris = np.ascontiguousarray(self.tris, dtype=np.float64).reshape(-1, 9)
 
     @property
     def N(self) -> int:
@@ -251,6 +255,38 @@ def load_obj_tet(path: str, scale: float = 0.05):
     return x, t
 
 
+def load_obj_triangles(path: str) -> np.ndarray:
+    """Static obstacle triangles of an OBJ surface mesh, (F, 9) float64: ``v``
+    lines and ``f`` polygons whose vertex tokens are ``i``, ``i/t``, ``i//n``
+    or ``i/t/n`` with 1-based (or negative, relative) vertex indices; a
+    polygon with k corners gives the k-2 triangles of a fan from its first
+    corner. The reference reads obstacles unscaled at offset 0
+    (Dfobstacle.h:110-160) with the OBJ reader of OBJMesh.h:40-97."""
+    if not os.path.exists(path):
+        raise ValueError("Can't read file " + path)
+    vs, faces = [], []
+    with open(path) as f:
+        for line in f:
+            p = line.split()
+            if not p:
+                continue
+            if p[0] == "v":
+                vs.append([float(p[1]), float(p[2]), float(p[3])])
+            elif p[0] == "f":
+                faces.append([int(tok.split("/")[0]) for tok in p[1:]])
+    x = np.asarray(vs, dtype=np.float64).reshape(-1, 3)
+    out = []
+    for face in faces:
+        if len(face) < 3:
+            raise ValueError("%s: face with fewer than 3 corners" % path)
+        ids = [i - 1 if i > 0 else x.shape[0] + i for i in face]
+        if min(ids) < 0 or max(ids) >= x.shape[0] or 0 in face:
+            raise ValueError("%s: face vertex index out of range" % path)
+        for k in range(1, len(ids) - 1):
+            out.append(np.concatenate([x[ids[0]], x[ids[k]], x[ids[k + 1]]]))
+    return np.asarray(out, dtype=np.float64).reshape(-1, 9)
+
+
 def load_muscle_file(path: str, scale: float = 0.05) -> List[np.ndarray]:
     """Fibre polylines of a ``.muscle`` file (Dfobj::MakeMuscles,
     Dfobj.h:734-797): one point per line, ``fibre segment x y z``; a fibre
@@ -344,10 +380,11 @@ def load_json_scene(config_path: str, data_root: Optional[str] = None) -> SceneD
     ``config_path`` is resolved against ``data_root`` (the reference prefixes
     SOFTCON_DIR, Environment.h:90), mesh and muscle files against
     ``<data_root>/data`` (Dfobj.h:192-199, 724-731), falling back to the
-    config's own folder. Scene features outside the projective-dynamics path
-    (cloth, obstacle meshes, rigid bones, pneumatic volume muscles, self
-    collision, skeleton or pneumatic actuators, drag on several meshes) raise
-    NotImplementedError."""
+    config's own folder. "Obstacles" are static triangle meshes (OBJ
+    ``f`` polygons, fan-triangulated). Scene features outside the
+    projective-dynamics path (cloth, rigid bones, pneumatic volume muscles,
+    self collision, skeleton or pneumatic actuators, drag on several meshes)
+    raise NotImplementedError."""
     root = data_root if data_root is not None else os.environ.get("PDM_DATA_ROOT", "")
     path = os.path.join(root, config_path.lstrip("/")) if root else config_path
     if not os.path.exists(path):
@@ -375,8 +412,6 @@ def load_json_scene(config_path: str, data_root: Optional[str] = None) -> SceneD
     underwater = bool(int(para.get("Is underwater", 0)))
     if _json_len(js, "Clothes"):
         _unsupported("Clothes")
-    if _json_len(js, "Obstacles"):
-        _unsupported("Obstacles (triangle-mesh obstacles)")
     meshes = js.get("Meshes") or []
     if not meshes:
         raise ValueError("scene has no meshes")
@@ -391,6 +426,12 @@ def load_json_scene(config_path: str, data_root: Optional[str] = None) -> SceneD
     def resolve(name):
         p = os.path.join(data_dir, name)
         return p if os.path.exists(p) else os.path.join(cfg_dir, os.path.basename(name))
+
+    obstacles = js.get("Obstacles") or []
+    tris = [load_obj_triangles(resolve(o["Obj filename"])) for o in obstacles if "Obj filename" in o]
+    if len(tris) != len(obstacles):
+        raise ValueError("an obstacle has no \"Obj filename\"")
+    tris = np.concatenate(tris) if tris else None
 
     bodies, infos = [], []
     nv = nt = ng = 0
@@ -438,7 +479,7 @@ def load_json_scene(config_path: str, data_root: Optional[str] = None) -> SceneD
                                 k_muscle=float(m.get("Muscle Stiffness", 5e5)), h=1.0 / hz, max_iter=max_iter,
                                 damping=damping, gravity=gravity, gravity_on=bool(int(m.get("Enable Gravity", 1))),
                                 underwater=underwater, contact=contact, max_iter1=max_iter, alpha=alpha, mu=1.0,
-                                **kw))
+                                tris=tris if k == 0 else None, **kw))
         infos.append(DfobjInfo(vertex_offset=nv, n_vertices=x.shape[0], tet_offset=nt, n_tets=T,
                                actuator_type=act_type, n_muscles=n_mus, group_offset=ng))
         nv += x.shape[0]
@@ -533,6 +574,7 @@ def pack_scenes(descs: List[SceneDesc], gap: float = 0.4, name: str = "packed", 
     cell = ext[:, :2].max(0) + gap
     ncol = int(math.ceil(math.sqrt(len(descs))))
     xs, ts, bodies, E, nuc, nuf, mass = [], [], [], [], [], [], []
+    offs = []
     s0, s1, sg, att, spheres = [], [], [], [], []
     nv = 0
     ng = 0
@@ -555,8 +597,10 @@ def pack_scenes(descs: List[SceneDesc], gap: float = 0.4, name: str = "packed", 
             ng += d.n_groups
         for sp in d.spheres:
             spheres.append(np.concatenate([sp[:3] + off, sp[3:]]))
+        offs.append(off)
         nv += d.N
     planes = np.unique(np.concatenate([d.planes for d in descs]), axis=0) if any(d.planes.size for d in descs) else None
+    tris = [d.tris + np.tile(offs[b], 3) for b, d in enumerate(descs) if d.tris.size]
     kw = {}
     if sg:
         kw = dict(seg_start=np.concatenate(s0), seg_end=np.concatenate(s1), seg_group=np.concatenate(sg), n_groups=ng)
@@ -567,7 +611,8 @@ def pack_scenes(descs: List[SceneDesc], gap: float = 0.4, name: str = "packed", 
                      max_iter=d0.max_iter, damping=d0.damping, gravity=d0.gravity, gravity_on=d0.gravity_on,
                      underwater=d0.underwater,
                      contact=d0.contact, max_iter1=d0.max_iter1, alpha=d0.alpha, mu=d0.mu, planes=planes,
-                     spheres=np.array(spheres) if spheres else None, body_of_tet=np.concatenate(bodies),
+                     spheres=np.array(spheres) if spheres else None, tris=np.concatenate(tris) if tris else None,
+                     body_of_tet=np.concatenate(bodies),
                      body_mass=np.array(mass), nu_fwd_tet=np.concatenate(nuf), name=name, **kw)
 
 

@@ -80,6 +80,11 @@ typedef struct pdm_scene_desc {
    * starts with v* = v + h M^-1 f, f the drag and lift of every surface face
    * at the step-start state (Dfobj::ComputeDragForces, Dfobj.h:846-994). */
   int underwater;
+  /* Static obstacle triangles ("Obstacles" meshes, Dfobstacle.h:110-160):
+   * 9 values per triangle (three corners). Surface vertices collide with
+   * them by ArcSim's vertex-face test (collision.cpp:127-278). */
+  int n_tris;
+  const double* tris;
 } pdm_scene_desc;
 
 typedef struct pdm_info {
@@ -93,6 +98,7 @@ typedef struct pdm_info {
   int batch;                 /* rollouts R */
   int precision;             /* PDM_FP64 or PDM_MIXED */
   int n_drag_faces;          /* surface faces carrying drag (0 unless underwater) */
+  int n_obstacle_tris;       /* static obstacle triangles */
 } pdm_info;
 
 enum { PDM_FP64 = 0, PDM_MIXED = 1 };

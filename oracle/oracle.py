@@ -68,6 +68,8 @@ def lib():
         L.orc_volume_projection.argtypes = [_dp, _dp]
         L.orc_volume_projection_jac.argtypes = [_dp, _dp, _dp, _dp]
         L.orc_set_underwater.argtypes = [C.c_void_p, C.c_int]
+        L.orc_set_tris.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.orc_detect_tri.argtypes = [_dp, _dp, _dp, _dp]
         L.orc_num_drag_faces.argtypes = [C.c_void_p]
         L.orc_get_drag_faces.argtypes = [C.c_void_p, _ip]
         L.orc_drag_face.argtypes = [_dp, _dp, _dp, C.c_void_p]
@@ -145,6 +147,9 @@ class OracleScene:
                             desc.spheres.shape[0], np.ascontiguousarray(desc.spheres).ravel().copy())
         if getattr(desc, "underwater", False):
             L.orc_set_underwater(self.h, 1)
+        tris = getattr(desc, "tris", None)
+        if tris is not None and len(tris):
+            L.orc_set_tris(self.h, len(tris), np.ascontiguousarray(tris, np.float64).ravel().copy())
         if factor:
             self.factor()
 
@@ -304,6 +309,15 @@ def volume_projection_jac(S):
     D, J, r = np.zeros(3), np.zeros((3, 3)), np.zeros(3)
     lib().orc_volume_projection_jac(np.ascontiguousarray(S, np.float64), D, J, r)
     return D, J, r
+
+
+def detect_tri(tri, x0, x1):
+    """(hit, normal) of a vertex moving x0 -> x1 against a static triangle
+    (9 values)."""
+    out = np.zeros(3)
+    hit = lib().orc_detect_tri(np.ascontiguousarray(tri, np.float64).ravel(), np.ascontiguousarray(x0, np.float64),
+                               np.ascontiguousarray(x1, np.float64), out)
+    return bool(hit), out
 
 
 def detect_plane(p, n, x0, x1):

@@ -442,6 +442,7 @@ struct Scene {
   int outer_cap = 16;  // detection passes with a Jacobi loop (the reference has no cap)
   std::vector<Plane> planes;
   std::vector<Sphere> spheres;
+  std::vector<std::array<double, 9>> tris;  // static obstacle triangles
   std::vector<int> contact_vtx;  // surface vertices
   // underwater drag (Environment.h:162-168, 236-250; Dfobj.h:846-994)
   int underwater = 0;
@@ -726,6 +727,113 @@ bool detect_sphere(const Sphere& sp, const double* x0, const double* x1, double*
 }
 
 // computeBasis, collisionBrutal.h:408-426: columns [n, t, b].
+// Vertex-vs-static-triangle test of ArcSim (collision.cpp:127-138, 211-278)
+// with the face at rest: the kDOP-18 box filter in single precision
+// (bvh.cpp:54-96), then the coplanarity time (the only root of the degenerate
+// cubic) and t = 1; a hit lies within the thickness of the face's plane and
+// inside the face, or at t = 1 within the thickness of a corner or an edge
+// line. n = +-(unit(y1-y0) x unit(y2-y0)), turned against the motion.
+static bool kdop18_overlap(const double* a0, const double* a1, const double* tri) {
+  auto dists = [](const double* p, float* d) {
+    float x = (float)p[0], y = (float)p[1], z = (float)p[2];
+    d[0] = x; d[1] = y; d[2] = z;
+    d[3] = x + y; d[4] = x + z; d[5] = y + z;
+    d[6] = x - y; d[7] = x - z; d[8] = y - z;
+  };
+  float A[2][9], B[3][9], amin[9], amax[9], bmin[9], bmax[9];
+  dists(a1, A[0]);
+  dists(a0, A[1]);
+  for (int k = 0; k < 3; ++k) dists(tri + 3 * k, B[k]);
+  for (int i = 0; i < 9; ++i) {
+    amin[i] = std::min(A[0][i], A[1][i]);
+    amax[i] = std::max(A[0][i], A[1][i]);
+    bmin[i] = std::min(std::min(B[0][i], B[1][i]), B[2][i]);
+    bmax[i] = std::max(std::max(B[0][i], B[1][i]), B[2][i]);
+  }
+  double d = (double)(float)kThickness;
+  for (int i = 0; i < 9; ++i) {
+    double m = i < 3 ? d : std::sqrt(2.0) * d;
+    float lo = (float)(bmin[i] - m), hi = (float)(bmax[i] + m);
+    if (amin[i] > hi || amax[i] < lo) return false;
+  }
+  return true;
+}
+
+static double triple(const double* a, const double* b, const double* c) {
+  double bc[3] = {b[1] * c[2] - b[2] * c[1], b[2] * c[0] - b[0] * c[2], b[0] * c[1] - b[1] * c[0]};
+  return a[0] * bc[0] + a[1] * bc[1] + a[2] * bc[2];
+}
+
+bool detect_tri(const double* tri, const double* x0, const double* x1, double* n) {
+  if (!kdop18_overlap(x0, x1, tri)) return false;
+  const double* Y[3] = {tri, tri + 3, tri + 6};
+  double mv[3], rel[3][3], V[3];
+  for (int r = 0; r < 3; ++r) {
+    mv[r] = x1[r] - x0[r];
+    V[r] = 0.0 - mv[r];
+    for (int k = 0; k < 3; ++k) rel[k][r] = Y[k][r] - x0[r];
+  }
+  double u1[3], u2[3];
+  for (int r = 0; r < 3; ++r) {
+    u1[r] = Y[1][r] - Y[0][r];
+    u2[r] = Y[2][r] - Y[0][r];
+  }
+  double n1 = std::sqrt(u1[0] * u1[0] + u1[1] * u1[1] + u1[2] * u1[2]);
+  double n2 = std::sqrt(u2[0] * u2[0] + u2[1] * u2[1] + u2[2] * u2[2]);
+  for (int r = 0; r < 3; ++r) {
+    u1[r] /= n1;
+    u2[r] /= n2;
+  }
+  double fc[3] = {u1[1] * u2[2] - u1[2] * u2[1], u1[2] * u2[0] - u1[0] * u2[2], u1[0] * u2[1] - u1[1] * u2[0]};
+  double fcc = fc[0] * fc[0] + fc[1] * fc[1] + fc[2] * fc[2];
+  if (!(fcc >= 1e-6)) return false;
+  double fl = std::sqrt(fcc), un[3] = {fc[0] / fl, fc[1] / fl, fc[2] / fl};
+  double c0 = triple(rel[0], rel[1], rel[2]);
+  double c1 = triple(V, rel[1], rel[2]) + triple(rel[0], V, rel[2]) + triple(rel[0], rel[1], V);
+  std::vector<double> ts;
+  if (c1 != 0.0) ts.push_back(-c0 / c1);
+  ts.push_back(1.0);
+  for (double t : ts) {
+    if (t < 0 || t > 1) continue;
+    double p[3], q[3][3];
+    for (int r = 0; r < 3; ++r) p[r] = x0[r] + t * mv[r];
+    for (int k = 0; k < 3; ++k)
+      for (int r = 0; r < 3; ++r) q[k][r] = Y[k][r] - p[r];
+    double hgt = 0;
+    for (int r = 0; r < 3; ++r) hgt += (p[r] - Y[0][r]) * un[r];
+    double b0 = triple(q[1], q[2], un), b1 = triple(q[2], q[0], un), b2 = triple(q[0], q[1], un);
+    double w[3] = {-b0 / (b0 + b1 + b2), -b1 / (b0 + b1 + b2), -b2 / (b0 + b1 + b2)};
+    bool inside = std::min(std::min(-w[0], -w[1]), -w[2]) >= -1e-6;
+    if (t == 1.0) {
+      for (int k = 0; k < 3; ++k) {
+        double dv[3];
+        for (int r = 0; r < 3; ++r) dv[r] = p[r] - Y[k][r];
+        if (std::sqrt(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]) < kThickness) inside = true;
+      }
+      for (int k = 0; k < 3; ++k) {
+        const double *A = Y[k], *B = Y[(k + 1) % 3];
+        double pa[3], ab[3];
+        for (int r = 0; r < 3; ++r) {
+          pa[r] = p[r] - A[r];
+          ab[r] = B[r] - A[r];
+        }
+        double dp = pa[0] * ab[0] + pa[1] * ab[1] + pa[2] * ab[2], ll = ab[0] * ab[0] + ab[1] * ab[1] + ab[2] * ab[2];
+        double e[3];
+        for (int r = 0; r < 3; ++r) e[r] = pa[r] - dp * ab[r] / ll;
+        if (std::sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]) < kThickness) inside = true;
+      }
+    }
+    double rv[3];
+    for (int r = 0; r < 3; ++r) rv[r] = w[0] * V[r] + w[1] * V[r] + w[2] * V[r];
+    double sg = (un[0] * rv[0] + un[1] * rv[1] + un[2] * rv[2]) > 0 ? -1.0 : 1.0;
+    if (std::fabs(hgt) < kThickness && inside) {
+      for (int r = 0; r < 3; ++r) n[r] = sg * fc[r];
+      return true;
+    }
+  }
+  return false;
+}
+
 void contact_basis(const double* n, double* B) {
   double t[3] = {0, n[2], -n[1]};  // n x (1,0,0)
   double tn = std::sqrt(t[1] * t[1] + t[2] * t[2]);
@@ -971,7 +1079,7 @@ void step(Scene& s, double* x, double* v, const double* act, StepTape* tape, int
   if (it_pd) *it_pd = iters;
   std::vector<double> ori_x = y;
   int cit = 0, outer_n = 0;
-  if (s.contact && !s.contact_vtx.empty() && (!s.planes.empty() || !s.spheres.empty())) {
+  if (s.contact && !s.contact_vtx.empty() && (!s.planes.empty() || !s.spheres.empty() || !s.tris.empty())) {
     // Contact phase, World.h:921-1023
     std::vector<Impact> impacts;
     std::vector<double> xit = y, oold(n3, 0.0), prev(n3, 0.0), rc(n3);
@@ -988,6 +1096,10 @@ void step(Scene& s, double* x, double* v, const double* act, StepTape* tape, int
         }
         for (const Sphere& sp : s.spheres) {
           if (detect_sphere(sp, &xn[3 * vtx], &ori_x[3 * vtx], nrm)) fresh.push_back({vtx, ob, {nrm[0], nrm[1], nrm[2]}});
+          ++ob;
+        }
+        for (const auto& tr : s.tris) {
+          if (detect_tri(tr.data(), &xn[3 * vtx], &ori_x[3 * vtx], nrm)) fresh.push_back({vtx, ob, {nrm[0], nrm[1], nrm[2]}});
           ++ob;
         }
       }
@@ -1507,6 +1619,15 @@ int orc_muscle_segment_select(void* p, const int* tets, int T, const double* sta
   return n;
 }
 
+void orc_set_tris(void* p, int n, const double* tris) {
+  Scene* s = (Scene*)p;
+  s->tris.resize(n);
+  for (int i = 0; i < n; ++i)
+    for (int k = 0; k < 9; ++k) s->tris[i][k] = tris[9 * i + k];
+}
+int orc_detect_tri(const double* tri, const double* x0, const double* x1, double* n) {
+  return detect_tri(tri, x0, x1, n) ? 1 : 0;
+}
 void orc_set_underwater(void* p, int on) {
   Scene* s = (Scene*)p;
   s->underwater = on;

@@ -157,3 +157,73 @@ def test_underwater_box_batch(oracle_mod, precision):
         assert rel(g["x0"][r], go["x0"]) < tol_g and rel(g["v0"][r], go["v0"]) < tol_g, r
     # no gravity under water: without the drag the box would fly straight
     assert np.abs(vT[0].reshape(-1, 3)[:, 2].mean()) < 0.6
+
+
+def _ramp_tris(nx=6, ny=6, tilt=0.15):
+    """A tilted, bumpy height field of 2 nx ny triangles under the box."""
+    xs, ys = np.linspace(-0.2, 0.5, nx + 1), np.linspace(-0.2, 0.4, ny + 1)
+    zf = lambda x, y: tilt * x + 0.01 * np.sin(9 * x) * np.cos(7 * y) - 0.02  # noqa: E731
+    out = []
+    for i in range(nx):
+        for j in range(ny):
+            p = [(xs[a], ys[b], zf(xs[a], ys[b])) for a, b in ((i, j), (i + 1, j), (i + 1, j + 1), (i, j + 1))]
+            out.append(np.concatenate([p[0], p[1], p[2]]))
+            out.append(np.concatenate([p[0], p[2], p[3]]))
+    return np.array(out)
+
+
+@pytest.mark.parametrize("floor", ["plane", "tris", "ramp"])
+def test_triangle_obstacles_batch(oracle_mod, floor):
+    """A box dropped with three start velocities onto an analytic plane, the
+    same floor as two triangles, or a tilted bumpy triangle ramp (72 faces,
+    the box slides across face edges): forward and gradient per rollout
+    against the oracle; the two-triangle floor gives the plane's results."""
+    from diff_fem_amd import engine as eng
+    x, t = scene.box_grid(4, 2, 2)
+    x = x + np.array([0.05, 0.05, 0.06])
+    kw = {"plane": dict(planes=np.array([[0, 0, 0, 0, 0, 1.0]])),
+          "tris": dict(tris=np.array([[-1, -1, 0, 3, -1, 0, 3, 3, 0], [-1, -1, 0, 3, 3, 0, -1, 3, 0.0]])),
+          "ramp": dict(tris=_ramp_tris())}[floor]
+    d = scene.SceneDesc(x_rest=x, tets=t, total_mass=1.0, E=np.full(t.shape[0], 5e4), h=0.01, max_iter=10,
+                        gravity=(0, 0, -9.81), contact=True, mu=0.5, **kw)
+    R, steps = 3, 24
+    s = eng.Scene(d, R=R, precision=eng.PDM_FP64, flags=0, checkpoint_every=8)
+    o = oracle_mod.OracleScene(d)
+    rng = np.random.default_rng(5)
+    x0 = np.tile(d.x_rest.ravel(), (R, 1))
+    v0 = np.stack([np.tile(u, d.N) for u in ([0.3, 0.1, -0.5], [-0.2, 0.3, -0.8], [0.5, -0.1, -0.3])])
+    xT, vT = s.forward_host(x0, v0, steps, record=True)
+    passes, jac = s.contact_stats()
+    assert jac.sum() > 0
+    gx, gv = rng.standard_normal((R, 3 * d.N)), rng.standard_normal((R, 3 * d.N))
+    g = s.backward_host(gx, gv)
+    for r in range(R):
+        xo, vo, its = o.rollout(x0[r], v0[r], steps, record=True)
+        assert np.array_equal(jac[r], its[:, 1]), r
+        assert rel(xT[r], xo) < 1e-9 and rel(vT[r], vo) < 1e-9, r
+        go, _ = o.backward(gx[r], gv[r])
+        assert rel(g["x0"][r], go["x0"]) < 1e-9 and rel(g["v0"][r], go["v0"]) < 1e-9, r
+    if floor == "tris":
+        dp = scene.SceneDesc(x_rest=x, tets=t, total_mass=1.0, E=np.full(t.shape[0], 5e4), h=0.01, max_iter=10,
+                             gravity=(0, 0, -9.81), contact=True, mu=0.5, planes=np.array([[0, 0, 0, 0, 0, 1.0]]))
+        sp = eng.Scene(dp, R=R, precision=eng.PDM_FP64, flags=0)
+        xp, _ = sp.forward_host(x0, v0, steps)
+        assert np.abs(xp - xT).max() < 1e-13
+
+
+def test_baymax_falls_onto_its_obstacle(oracle_mod):
+    """baymax.json as shipped: a soft body (E = 800) dropping onto the
+    obstacle mesh of its scene (the 10x10 quad grid, 162 triangles), 40 steps
+    at 100 Hz; positions against the oracle, and the body stops at the floor."""
+    from diff_fem_amd import pydifem as pd
+    env = pd.init_env(40, "/data/baymax/baymax.json", ROOT)
+    d = env._desc
+    assert env._scene.info()["n_obstacle_tris"] == 162
+    q0 = pd.vector2double(env.mSoftWorld.mX)
+    qT, qdT = pd.forward_all(q0, [0.0] * len(q0), [800.0, 0.4], env)
+    o = oracle_mod.OracleScene(d)
+    o.set_material(np.full(d.T, 800.0), 0.4)
+    xo, vo, its = o.rollout(np.asarray(q0), np.zeros(len(q0)), 40)
+    assert its[:, 1].sum() > 0
+    assert rel(qT, xo) < 1e-9 and rel(qdT, vo) < 1e-9
+    assert np.asarray(qT).reshape(-1, 3)[:, 1].min() > -0.321262 - 0.01

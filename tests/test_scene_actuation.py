@@ -86,7 +86,6 @@ def test_two_meshes_are_two_bodies(tmp_path):
 
 @pytest.mark.parametrize("para,mesh,extra,err", [
     ({"Self-collision": True}, {}, None, NotImplementedError),
-    ({}, {}, {"Obstacles": [{"Obj filename": "box.obj"}]}, NotImplementedError),
     ({}, {}, {"Clothes": [{"Obj filename": "box.obj"}]}, NotImplementedError),
     ({}, {"Bones": [{"Parent ID": -1}]}, None, NotImplementedError),
     ({}, {"Volume Verts": [[0, 1]]}, None, NotImplementedError),
@@ -107,6 +106,14 @@ def test_underwater_needs_one_mesh(tmp_path):
     _write_scene(tmp_path, {"Is underwater": 1}, [{"Obj filename": "box.obj"}] * 2)
     with pytest.raises(NotImplementedError, match="Is underwater"):
         scene.load_json_scene("s.json", str(tmp_path))
+
+
+def test_obstacles_load_as_triangles(tmp_path):
+    _write_scene(tmp_path, {"Handle collision": True}, [{"Obj filename": "box.obj"}],
+                 {"Obstacles": [{"Obj filename": "floor.obj", "Scale": 3}]})
+    (tmp_path / "data" / "floor.obj").write_text("v -1 -1 0\nv 1 -1 0\nv 1 1 0\nv -1 1 0\nf 1 2 3 4\n")
+    d = scene.load_json_scene("s.json", str(tmp_path))
+    np.testing.assert_array_equal(d.tris, [[-1, -1, 0, 1, -1, 0, 1, 1, 0], [-1, -1, 0, 1, 1, 0, -1, 1, 0]])
 
 
 def test_unreadable_scenes_raise_value_error(tmp_path):
