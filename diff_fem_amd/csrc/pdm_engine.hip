@@ -887,9 +887,35 @@ __global__ void k_attach_rhs(const double* __restrict__ attw, const double* __re
 }
 
 // End of step: v = damping (x+ - x)/h ; x = x+ (World.h:757-766, 708).
+// Device copy of FP64 arrays, two values per thread (the runtime's blit
+// kernel moves these state-sized buffers at a fraction of HBM speed).
+__global__ __launch_bounds__(256) void k_copy_f64(double* __restrict__ dst, const double* __restrict__ src,
+                                                  long long n) {
+  long long i = 2 * ((long long)blockIdx.x * blockDim.x + threadIdx.x);
+  if (i + 1 < n) {
+    *reinterpret_cast<double2*>(dst + i) = *reinterpret_cast<const double2*>(src + i);
+  } else if (i < n) {
+    dst[i] = src[i];
+  }
+}
+__global__ __launch_bounds__(256) void k_copy_f64_1(double* __restrict__ dst, const double* __restrict__ src,
+                                                    long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
+// End of a step: v = (x_it - x_n)/h * damping, x = x_it (World.h:757-766,
+// 708). Threads r < R also record the step's counters into the tapes.
 __global__ void k_step_finish(const double* __restrict__ Xit, const double* __restrict__ Xn, double* __restrict__ X,
-                              double* __restrict__ V, long long n, double h, double damping) {
+                              double* __restrict__ V, long long n, double h, double damping, int R,
+                              const int* __restrict__ iters, int* __restrict__ tIters, const int* __restrict__ npass,
+                              int* __restrict__ tNpass, const int* __restrict__ citers, int* __restrict__ tCiters) {
   long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx < R) {
+    if (tIters) tIters[idx] = iters[idx];
+    if (tNpass) tNpass[idx] = npass[idx];
+    if (tCiters) tCiters[idx] = citers[idx];
+  }
   if (idx >= n) return;
   double x1 = Xit[idx];
   V[idx] = (x1 - Xn[idx]) * (1.0 / h) * damping;
@@ -1385,7 +1411,7 @@ __global__ void k_contact_detect(const int* __restrict__ cvtx, int ncv, const do
 
 __global__ void k_contact_reset(int* __restrict__ win_st, int* __restrict__ win_o, int* __restrict__ nseen, long long n,
                                 int* __restrict__ cactive, int* __restrict__ cfirst, int* __restrict__ fresh,
-                                int* __restrict__ clast, int R) {
+                                int* __restrict__ clast, int* __restrict__ npass, int* __restrict__ citers, int R) {
   long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (idx < n) {
     win_st[idx] = -1;
@@ -1397,6 +1423,8 @@ __global__ void k_contact_reset(int* __restrict__ win_st, int* __restrict__ win_
     cfirst[idx] = 1;
     fresh[idx] = 0;
     clast[idx] = 1;
+    npass[idx] = 0;
+    citers[idx] = 0;
   }
   if (idx == 0) cactive[R] = 0;
 }
@@ -1442,9 +1470,14 @@ __global__ void k_contact_begin(const double* __restrict__ Xit, double* __restri
 
 __global__ __launch_bounds__(256) void k_contact_err(const double* __restrict__ part, int nchunk, int R,
                                                      int* __restrict__ iactive, int* __restrict__ cfirst,
-                                                     int* __restrict__ citers, int* __restrict__ clast) {
+                                                     int* __restrict__ citers, int* __restrict__ clast,
+                                                     int* __restrict__ exec) {
   const int r = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-  if (r >= R || !iactive[r]) return;
+  if (r >= R) return;
+  if (!iactive[r]) {
+    if (exec && lane == 0) exec[r] = 0;
+    return;
+  }
   const long long C = 3LL * R;
   double m = 0.0;
   for (int ch = lane; ch < nchunk; ch += 64)
@@ -1456,6 +1489,7 @@ __global__ __launch_bounds__(256) void k_contact_err(const double* __restrict__ 
   iactive[r] = go ? 1 : 0;
   citers[r] += go ? 1 : 0;
   clast[r] = go ? 0 : 1;  // 1: the result is this begin's ori_x = x_it
+  if (exec) exec[r] = go ? 1 : 0;
 }
 
 // One Jacobi update of the contact phase per (vertex, rollout), World.h:957-1016:
@@ -1829,6 +1863,8 @@ struct pdm_scene_s {
   DevBuf<int> d_bvh_a, d_bvh_b, d_bvh_tri;
   std::vector<std::unique_ptr<ContactTape>> ctape;  // per step of the segment being reversed
   DevBuf<double> cXB, cPB, cOB, cGf, cCD;
+  DevBuf<int> cinjG;          // contact reverse: injection flags per (iteration, rollout)
+  std::vector<int> cinj_h;
   DevBuf<double> Kdiag, d_planes, d_spheres, Ori, Dn, Prev, win_n;
   std::vector<int> npass_host, citers_host;  // R*steps of the last forward
   // adjoint buffers
@@ -2501,6 +2537,20 @@ void launch_local(pdm_scene s, const double* Xsrc, const double* act, int act_st
 // Contact phase of one step (World.h:921-1023) for every rollout that has
 // fresh impacts; the host reads back after each detection pass whether any
 // rollout continues. Rollouts finish with x = ori_x.
+// FP64 device-to-device copy on the scene's stream (two values per thread when
+// both pointers are 16-byte aligned).
+int dcopy(pdm_scene s, double* dst, const double* src, size_t n) {
+  if (n == 0) return 0;
+  bool al = (((uintptr_t)dst | (uintptr_t)src) & 15) == 0;
+  if (al)
+    hipLaunchKernelGGL(k_copy_f64, dim3(cdiv((long long)(n + 1) / 2, 256)), dim3(256), 0, s->stream, dst, src,
+                       (long long)n);
+  else
+    hipLaunchKernelGGL(k_copy_f64_1, dim3(cdiv((long long)n, 256)), dim3(256), 0, s->stream, dst, src, (long long)n);
+  HIPX(hipGetLastError());
+  return 0;
+}
+
 int contact_phase(pdm_scene s, const double* act, int act_steps, int step, ContactTape* tp) {
   HostScene& S = s->S;
   int N = S.N, C = s->C, R = s->R;
@@ -2509,16 +2559,17 @@ int contact_phase(pdm_scene s, const double* act, int act_steps, int step, Conta
   dim3 ugrid(cdiv(C, 64), nchunk), ublock(64, 4);
   int np = (int)(S.planes.size() / 6), nsph = (int)(S.spheres.size() / 4);
   long long ni = (long long)s->ncv * R;
-  HIPX(hipMemcpyAsync(s->Ori.p, s->Xit.p, nc * 8, hipMemcpyDeviceToDevice, s->stream));
+  if (dcopy(s, s->Ori.p, s->Xit.p, nc)) return 1;
   hipLaunchKernelGGL(k_contact_reset, dim3(cdiv(std::max<long long>(ni, R), kThreads)), dim3(kThreads), 0, s->stream,
-                     s->win_st.p, s->win_o.p, s->nseen.p, ni, s->cactive.p, s->cfirst.p, s->fresh.p, s->clast.p, R);
+                     s->win_st.p, s->win_o.p, s->nseen.p, ni, s->cactive.p, s->cfirst.p, s->fresh.p, s->clast.p,
+                     s->npass.p, s->citers.p, R);
   std::vector<int> ca(R + 1);
   size_t cvc = (size_t)s->ncv * C;
   if (tp) {
     tp->G = 0;
     tp->first.clear();
     if (grow_keep(tp->xit, nc, 0, s->stream)) return 1;
-    HIPX(hipMemcpyAsync(tp->xit.p, s->Xit.p, nc * 8, hipMemcpyDeviceToDevice, s->stream));
+    if (dcopy(s, tp->xit.p, s->Xit.p, nc)) return 1;
   }
   for (int pass = 0; pass <= S.outer_cap; ++pass) {
     hipLaunchKernelGGL(k_contact_detect, dim3(cdiv((long long)s->ncv * R, kThreads)), dim3(kThreads), 0, s->stream,
@@ -2548,10 +2599,8 @@ int contact_phase(pdm_scene s, const double* act, int act_steps, int step, Conta
       hipLaunchKernelGGL(k_contact_begin, ugrid, ublock, 0, s->stream, s->Xit.p, s->Ori.p, s->iactive.p, s->part.p, N,
                          C, s->Xn.p, s->Dn.p);
       hipLaunchKernelGGL(k_contact_err, dim3(cdiv(R, 4)), dim3(256), 0, s->stream, s->part.p, nchunk, R,
-                         s->iactive.p, s->cfirst.p, s->citers.p, s->clast.p);
-      if (tp)
-        HIPX(hipMemcpyAsync(tp->exec.p + (size_t)tp->G * R, s->iactive.p, R * sizeof(int), hipMemcpyDeviceToDevice,
-                            s->stream));
+                         s->iactive.p, s->cfirst.p, s->citers.p, s->clast.p,
+                         tp ? tp->exec.p + (size_t)tp->G * R : nullptr);
       if (s->precision == PDM_MIXED)
         launch_local<float>(s, s->Ori.p, act, act_steps, step, s->iactive.p);
       else
@@ -2562,8 +2611,7 @@ int contact_phase(pdm_scene s, const double* act, int act_steps, int step, Conta
                          s->acount.p, s->Prev.p, s->Xit.p, N, R, tp ? tp->rc.p + (size_t)tp->G * cvc : nullptr,
                          tp ? tp->nrm.p + (size_t)tp->G * cvc : nullptr);
       if (tp) {
-        HIPX(hipMemcpyAsync(tp->xit.p + (size_t)(tp->G + 1) * nc, s->Xit.p, nc * 8, hipMemcpyDeviceToDevice,
-                            s->stream));
+        if (dcopy(s, tp->xit.p + (size_t)(tp->G + 1) * nc, s->Xit.p, nc)) return 1;
         tp->first.push_back(i == 0 ? 1 : 0);
         tp->G += 1;
       }
@@ -2573,7 +2621,7 @@ int contact_phase(pdm_scene s, const double* act, int act_steps, int step, Conta
     if (grow_keep(tp->flast, R, 0, s->stream)) return 1;
     HIPX(hipMemcpyAsync(tp->flast.p, s->clast.p, R * sizeof(int), hipMemcpyDeviceToDevice, s->stream));
   }
-  HIPX(hipMemcpyAsync(s->Xit.p, s->Ori.p, nc * 8, hipMemcpyDeviceToDevice, s->stream));
+  if (dcopy(s, s->Xit.p, s->Ori.p, nc)) return 1;
   HIPX(hipGetLastError());
   return 0;
 }
@@ -2597,9 +2645,9 @@ int launch_pd_block(pdm_scene s, const double* act, int act_steps, int step, dou
   hipLaunchKernelGGL(k_step_prep, dim3(cdiv(nc, kThreads)), dim3(kThreads), 0, s->stream, s->X.p, s->V.p, s->Lx.p,
                      s->mass.p, s->attw.p, s->xrest.p, s->Xn.p, s->Cc.p, s->Xit.p, N, C, S.h, S.g[0], S.g[1], S.g[2],
                      S.gravity_on, s->nfaces > 0 ? s->FF.p : nullptr, s->dvfptr.p, s->dvfq.p);
-  if (tapeXn) HIPX(hipMemcpyAsync(tapeXn, s->Xn.p, nc * 8, hipMemcpyDeviceToDevice, s->stream));
-  if (tapeVn) HIPX(hipMemcpyAsync(tapeVn, s->V.p, nc * 8, hipMemcpyDeviceToDevice, s->stream));
-  if (tapeY) HIPX(hipMemcpyAsync(tapeY, s->Xit.p, nc * 8, hipMemcpyDeviceToDevice, s->stream));
+  if (tapeXn && dcopy(s, tapeXn, s->Xn.p, nc)) return 1;
+  if (tapeVn && dcopy(s, tapeVn, s->V.p, nc)) return 1;
+  if (tapeY && dcopy(s, tapeY, s->Xit.p, nc)) return 1;
   hipLaunchKernelGGL(k_sqsum_chunks, ugrid, ublock, 0, s->stream, s->Xit.p, s->part.p, N, C, any);
   hipLaunchKernelGGL(k_err_init, dim3(cdiv(s->R, 4)), dim3(256), 0, s->stream, s->part.p, nchunk, s->R, N,
                      s->active.p, s->iters_dev.p, any);
@@ -2676,16 +2724,11 @@ int run_step(pdm_scene s, const double* act, int act_steps, int step, double* ta
   } else if (launch_pd_block(s, act, act_steps, step, tapeY, tapeXn, tapeVn, host_exit, nullptr, nullptr)) {
     return 1;
   }
-  if (s->ncv > 0) {
-    HIPX(hipMemsetAsync(s->npass.p, 0, s->R * sizeof(int), s->stream));
-    HIPX(hipMemsetAsync(s->citers.p, 0, s->R * sizeof(int), s->stream));
-    if (contact_phase(s, act, act_steps, step, ctp)) return 1;
-    if (tNpass) HIPX(hipMemcpyAsync(tNpass, s->npass.p, s->R * sizeof(int), hipMemcpyDeviceToDevice, s->stream));
-    if (tCiters) HIPX(hipMemcpyAsync(tCiters, s->citers.p, s->R * sizeof(int), hipMemcpyDeviceToDevice, s->stream));
-  }
+  if (s->ncv > 0 && contact_phase(s, act, act_steps, step, ctp)) return 1;
+  const bool cs = s->ncv > 0;
   hipLaunchKernelGGL(k_step_finish, dim3(cdiv(nc, kThreads)), dim3(kThreads), 0, s->stream, s->Xit.p, s->Xn.p, s->X.p,
-                     s->V.p, (long long)nc, S.h, S.damping);
-  if (tIters) HIPX(hipMemcpyAsync(tIters, s->iters_dev.p, s->R * sizeof(int), hipMemcpyDeviceToDevice, s->stream));
+                     s->V.p, (long long)nc, S.h, S.damping, s->R, s->iters_dev.p, tIters, s->npass.p,
+                     cs ? tNpass : nullptr, s->citers.p, cs ? tCiters : nullptr);
   HIPX(hipGetLastError());
   return 0;
 }
@@ -2733,8 +2776,7 @@ int forward_impl(pdm_scene s, int steps, const double* x0, const double* v0, con
   for (int n = 0; n < steps; ++n) {
     if (record && n % s->ckpt == 0) {
       size_t k = n / s->ckpt;
-      HIPX(hipMemcpyAsync(s->ckX.p + k * nc, s->X.p, nc * 8, hipMemcpyDeviceToDevice, s->stream));
-      HIPX(hipMemcpyAsync(s->ckV.p + k * nc, s->V.p, nc * 8, hipMemcpyDeviceToDevice, s->stream));
+      if (dcopy(s, s->ckX.p + k * nc, s->X.p, nc) || dcopy(s, s->ckV.p + k * nc, s->V.p, nc)) return 1;
     }
     int* np = s->ncv ? np_all.p + (size_t)n * s->R : nullptr;
     int* ci = s->ncv ? ci_all.p + (size_t)n * s->R : nullptr;
@@ -2788,12 +2830,15 @@ int forward_impl(pdm_scene s, int steps, const double* x0, const double* v0, con
 }
 
 template <typename T>
-int vjp_iteration(pdm_scene s, const double* Yk, const double* Yk1, const double* act, int act_steps, int step,
-                  const double* CDp = nullptr, double* out_add = nullptr, const double* Xn = nullptr) {
+int vjp_iteration(pdm_scene s, const int* active, const double* Yk, const double* Yk1, const double* act,
+                  int act_steps, int step, const double* CDp = nullptr, double* out_add = nullptr,
+                  const double* Xn = nullptr) {
   HostScene& S = s->S;
   T* corner = (T*)s->corner.p;
   long long nt = (long long)S.T * s->R;
-  hipLaunchKernelGGL(k_compact, dim3(1), dim3(1024), 0, s->stream, s->ractive.p, s->R, s->alist.p, s->acount.p);
+  // active null: the rollout list is already compacted
+  if (active)
+    hipLaunchKernelGGL(k_compact, dim3(1), dim3(1024), 0, s->stream, active, s->R, s->alist.p, s->acount.p);
   auto kern = s->R % 64 == 0 ? k_vjp_tet<T, true> : k_vjp_tet<T, false>;
   hipLaunchKernelGGL(kern, dim3(cdiv(nt, kThreads)), dim3(kThreads), 0, s->stream, Yk, Yk1, s->Zb.p, s->tv.p, s->tG.p,
                      s->wr.p, s->wv.p, s->dEr.p, s->dEv.p, s->tslot.p, s->alist.p, s->acount.p, corner, s->gEpart.p,
@@ -2825,22 +2870,26 @@ int contact_reverse(pdm_scene s, ContactTape& tp, const double* Xn, const double
   for (int g = 0; g < G; ++g)
     for (int r = 0; r < R; ++r)
       if (ex[(size_t)g * R + r]) glast[r] = g;
-  HIPX(hipMemcpyAsync(s->cGf.p, s->Yb.p, nc * 8, hipMemcpyDeviceToDevice, s->stream));
+  if (dcopy(s, s->cGf.p, s->Yb.p, nc)) return 1;
   hipLaunchKernelGGL(k_cadj_init, dim3(cdiv(nc, kThreads)), dim3(kThreads), 0, s->stream, s->cXB.p, s->Yb.p,
                      tp.flast.p, (long long)nc, C);
   HIPX(hipMemsetAsync(s->cPB.p, 0, nc * 8, s->stream));
-  std::vector<int> inj((size_t)G * R);
+  // injection flags per (iteration, rollout); the host copy stays alive until
+  // the next call, whose first synchronisation ends this upload
+  std::vector<int>& inj = s->cinj_h;
+  inj.assign((size_t)G * R, 0);
   for (int g = 0; g < G; ++g)
     for (int r = 0; r < R; ++r) inj[(size_t)g * R + r] = (g == glast[r] && !fl[r]) ? 1 : 0;
-  DevBuf<int> dinj;
-  if (dinj.upload(inj.data(), inj.size())) return 1;
+  if (s->cinjG.alloc(inj.size())) return 1;
+  DevBuf<int>& dinj = s->cinjG;
+  HIPX(hipMemcpyAsync(dinj.p, inj.data(), inj.size() * sizeof(int), hipMemcpyHostToDevice, s->stream));
   for (int g = G - 1; g >= 0; --g) {
     bool any = false;
     for (int r = 0; r < R; ++r) any |= ex[(size_t)g * R + r] != 0;
     if (!any) continue;
-    HIPX(hipMemcpyAsync(s->ractive.p, tp.exec.p + (size_t)g * R, R * sizeof(int), hipMemcpyDeviceToDevice, s->stream));
     const double* ori = tp.xit.p + (size_t)g * nc;
-    hipLaunchKernelGGL(k_compact, dim3(1), dim3(1024), 0, s->stream, s->ractive.p, R, s->alist.p, s->acount.p);
+    hipLaunchKernelGGL(k_compact, dim3(1), dim3(1024), 0, s->stream, tp.exec.p + (size_t)g * R, R, s->alist.p,
+                       s->acount.p);
     hipLaunchKernelGGL(k_cadj_pre, dim3(cdiv((long long)N * R, kThreads)), dim3(kThreads), 0, s->stream, s->cXB.p,
                        s->cPB.p, s->Xb.p, tp.xit.p + (size_t)(g + 1) * nc, Xn, s->Kdiag.p, s->cv_of_v.p,
                        tp.rc.p + (size_t)g * cvc, tp.nrm.p + (size_t)g * cvc, S.mu, S.alpha, S.h, (int)tp.first[g],
@@ -2848,14 +2897,13 @@ int contact_reverse(pdm_scene s, ContactTape& tp, const double* Xn, const double
     hipLaunchKernelGGL(k_cadj_cz, dim3(cdiv(nc, kThreads)), dim3(kThreads), 0, s->stream, s->Lptr.p, s->Lcol.p,
                        s->Lval.p, s->Zb.p, s->Xb.p, s->cOB.p, s->alist.p, s->acount.p, N, C);
     int rc = s->precision == PDM_MIXED
-                 ? vjp_iteration<float>(s, ori, ori, act, act_steps, step, s->cCD.p, s->cOB.p, Xn)
-                 : vjp_iteration<double>(s, ori, ori, act, act_steps, step, s->cCD.p, s->cOB.p, Xn);
+                 ? vjp_iteration<float>(s, nullptr, ori, ori, act, act_steps, step, s->cCD.p, s->cOB.p, Xn)
+                 : vjp_iteration<double>(s, nullptr, ori, ori, act, act_steps, step, s->cCD.p, s->cOB.p, Xn);
     if (rc) return 1;
     hipLaunchKernelGGL(k_cadj_post, dim3(cdiv(nc, kThreads)), dim3(kThreads), 0, s->stream, s->cOB.p, s->cGf.p,
                        dinj.p + (size_t)g * R, s->alist.p, s->acount.p, s->cXB.p, (long long)nc, C);
   }
-  HIPX(hipMemcpyAsync(s->Yb.p, s->cXB.p, nc * 8, hipMemcpyDeviceToDevice, s->stream));
-  HIPX(hipStreamSynchronize(s->stream));  // dinj is released on return
+  if (dcopy(s, s->Yb.p, s->cXB.p, nc)) return 1;
   HIPX(hipGetLastError());
   return 0;
 }
@@ -2915,8 +2963,7 @@ int backward_impl(pdm_scene s, const double* gxT, const double* gvT, double* gx0
     bool taped = sg == s->fwd_taped_seg;
     s->fwd_taped_seg = -1;
     if (!taped) {
-      HIPX(hipMemcpyAsync(s->X.p, s->ckX.p + (size_t)sg * nc, nc * 8, hipMemcpyDeviceToDevice, s->stream));
-      HIPX(hipMemcpyAsync(s->V.p, s->ckV.p + (size_t)sg * nc, nc * 8, hipMemcpyDeviceToDevice, s->stream));
+      if (dcopy(s, s->X.p, s->ckX.p + (size_t)sg * nc, nc) || dcopy(s, s->V.p, s->ckV.p + (size_t)sg * nc, nc)) return 1;
     }
     for (int n = s0; n < s1 && !taped; ++n) {
       int q = n - s0;
@@ -2948,8 +2995,10 @@ int backward_impl(pdm_scene s, const double* gxT, const double* gvT, double* gx0
         hipLaunchKernelGGL(k_adj_post_solve, dim3(cdiv(nc, kThreads)), dim3(kThreads), 0, s->stream, s->Zb.p,
                            s->Yb.p, s->Xb.p, s->Zs.p, s->ractive.p, (long long)nc, C);
         int rc = s->precision == PDM_MIXED
-                     ? vjp_iteration<float>(s, Y + (size_t)k * nc, Y + (size_t)(k + 1) * nc, act, steps, n)
-                     : vjp_iteration<double>(s, Y + (size_t)k * nc, Y + (size_t)(k + 1) * nc, act, steps, n);
+                     ? vjp_iteration<float>(s, s->ractive.p, Y + (size_t)k * nc, Y + (size_t)(k + 1) * nc, act,
+                                            steps, n)
+                     : vjp_iteration<double>(s, s->ractive.p, Y + (size_t)k * nc, Y + (size_t)(k + 1) * nc, act,
+                                             steps, n);
         if (rc) return 1;
       }
       if (S.M > 0)
