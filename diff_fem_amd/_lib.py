@@ -39,6 +39,7 @@ class SceneDescC(C.Structure):
         ("n_spheres", C.c_int), ("spheres", _dp),
         ("nu_fwd_tet", _dp), ("n_bodies", C.c_int), ("body_of_tet", _ip), ("body_mass", _dp),
         ("underwater", C.c_int), ("n_tris", C.c_int), ("tris", _dp),
+        ("self_collision", C.c_int), ("self_mu", C.c_double),
     ]
 
 
@@ -48,7 +49,7 @@ class InfoC(C.Structure):
         ("n_supernodes", C.c_int), ("n_levels", C.c_int), ("max_front", C.c_int), ("max_width", C.c_int),
         ("nnz_factor", C.c_longlong), ("nnz_lower", C.c_longlong), ("factor_flops", C.c_double), ("reg_used", C.c_double),
         ("n_contact_vertices", C.c_int), ("batch", C.c_int), ("precision", C.c_int), ("n_drag_faces", C.c_int),
-        ("n_obstacle_tris", C.c_int),
+        ("n_obstacle_tris", C.c_int), ("n_self_faces", C.c_int),
     ]
 
 

@@ -1429,13 +1429,538 @@ __global__ void k_contact_reset(int* __restrict__ win_st, int* __restrict__ win_
   if (idx == 0) cactive[R] = 0;
 }
 
+// ------------------------------------------------------------ self contact
+// Surface vertices against surface faces of the body mesh ("Self-collision",
+// collisionBrutal.h:281-406, 643-700; World.h:1246-1315, 1446-1600). Contact
+// indices a number the surface vertices as the reference does.
+constexpr int kSelfImp = 8;    // impacts kept per (vertex, rollout) in a step
+constexpr int kSelfMem = 16;   // vertices of one self-contact group
+
+__device__ inline double sgn_d(double x) { return x < 0 ? -1.0 : 1.0; }
+// ArcSim's root finders (collision.cpp:151-170, 280-326)
+__device__ int solve_quadratic_d(double a, double b, double c, double x[2]) {
+  double d = b * b - 4 * a * c;
+  if (d < 0) {
+    x[0] = -b / (2 * a);
+    return 0;
+  }
+  double q = -(b + sgn_d(b) * sqrt(d)) / 2;
+  int i = 0;
+  if (fabs(a) > 1e-12 * fabs(q)) x[i++] = q / a;
+  if (fabs(q) > 1e-12 * fabs(c)) x[i++] = c / q;
+  if (i == 2 && x[0] > x[1]) {
+    double t = x[0];
+    x[0] = x[1];
+    x[1] = t;
+  }
+  return i;
+}
+__device__ double newtons_method_d(double a, double b, double c, double d, double x0, int init_dir) {
+  if (init_dir != 0) {
+    double y0 = d + x0 * (c + x0 * (b + x0 * a)), ddy0 = 2 * b + x0 * (6 * a);
+    x0 += init_dir * sqrt(fabs(2 * y0 / ddy0));
+  }
+  for (int iter = 0; iter < 100; iter++) {
+    double y = d + x0 * (c + x0 * (b + x0 * a));
+    double dy = c + x0 * (2 * b + x0 * 3 * a);
+    if (dy == 0) return x0;
+    double x1 = x0 - y / dy;
+    if (fabs(x0 - x1) < 1e-6) return x0;
+    x0 = x1;
+  }
+  return x0;
+}
+__device__ int solve_cubic_d(double a, double b, double c, double d, double x[3]) {
+  double xc[2] = {0.0, 0.0};
+  int ncrit = solve_quadratic_d(3 * a, 2 * b, c, xc);
+  if (ncrit == 0) {
+    x[0] = newtons_method_d(a, b, c, d, xc[0], 0);
+    return 1;
+  } else if (ncrit == 1) {
+    return solve_quadratic_d(b, c, d, x);
+  }
+  double yc[2] = {d + xc[0] * (c + xc[0] * (b + xc[0] * a)), d + xc[1] * (c + xc[1] * (b + xc[1] * a))};
+  int i = 0;
+  if (yc[0] * a >= 0) x[i++] = newtons_method_d(a, b, c, d, xc[0], -1);
+  if (yc[0] * yc[1] <= 0) {
+    int closer = fabs(yc[0]) < fabs(yc[1]) ? 0 : 1;
+    x[i++] = newtons_method_d(a, b, c, d, xc[closer], closer == 0 ? 1 : -1);
+  }
+  if (yc[1] * a <= 0) x[i++] = newtons_method_d(a, b, c, d, xc[1], 1);
+  return i;
+}
+
+// Moving vertex (p0 -> p1) against a moving face (y0[k] -> y1[k]) with
+// thickness 2e-4 (find_face_impacts uses mult 2): kDOP box filter, the
+// coplanarity cubic's roots in [0,1] and t = 1, distance and inside tests, the
+// normal from the step-start corners turned against the relative motion.
+__device__ bool vf_moving_d(const double* p0, const double* p1, const double* const* y0, const double* const* y1,
+                            double* tout, double* dout, double* nout) {
+  const double thk = 2.0 * kThick;
+  {
+    float an[9], ax[9], bn[9], bx[9];
+    for (int i = 0; i < 9; ++i) {
+      an[i] = bn[i] = 3.0e38f;
+      ax[i] = bx[i] = -3.0e38f;
+    }
+    kdop_add(p1, an, ax);
+    kdop_add(p0, an, ax);
+    for (int k = 0; k < 3; ++k) {
+      kdop_add(y1[k], bn, bx);
+      kdop_add(y0[k], bn, bx);
+    }
+    const double d = (double)(float)thk, d2 = sqrt(2.0) * d;
+    for (int i = 0; i < 9; ++i) {
+      float lo = (float)((double)bn[i] - (i < 3 ? d : d2)), hi = (float)((double)bx[i] + (i < 3 ? d : d2));
+      if (an[i] > hi || ax[i] < lo) return false;
+    }
+  }
+  double v0[3], x[3][3], v[3][3];
+  for (int r = 0; r < 3; ++r) {
+    v0[r] = p1[r] - p0[r];
+    for (int k = 0; k < 3; ++k) {
+      x[k][r] = y0[k][r] - p0[r];
+      v[k][r] = (y1[k][r] - y0[k][r]) - v0[r];
+    }
+  }
+  double a0 = stp3(x[0], x[1], x[2]);
+  double a1 = stp3(v[0], x[1], x[2]) + stp3(x[0], v[1], x[2]) + stp3(x[0], x[1], v[2]);
+  double a2 = stp3(x[0], v[1], v[2]) + stp3(v[0], x[1], v[2]) + stp3(v[0], v[1], x[2]);
+  double a3 = stp3(v[0], v[1], v[2]);
+  double ts[4];
+  int nsol = solve_cubic_d(a3, a2, a1, a0, ts);
+  ts[nsol++] = 1;
+  double c[3];
+  {
+    double s1[3], s2[3];
+    for (int r = 0; r < 3; ++r) {
+      s1[r] = y0[1][r] - y0[0][r];
+      s2[r] = y0[2][r] - y0[0][r];
+    }
+    double m1 = sqrt(s1[0] * s1[0] + s1[1] * s1[1] + s1[2] * s1[2]), m2 = sqrt(s2[0] * s2[0] + s2[1] * s2[1] + s2[2] * s2[2]);
+    for (int r = 0; r < 3; ++r) {
+      s1[r] /= m1;
+      s2[r] /= m2;
+    }
+    c[0] = s1[1] * s2[2] - s1[2] * s2[1];
+    c[1] = s1[2] * s2[0] - s1[0] * s2[2];
+    c[2] = s1[0] * s2[1] - s1[1] * s2[0];
+  }
+  for (int i = 0; i < nsol; i++) {
+    double t = ts[i];
+    if (t < 0 || t > 1) continue;
+    double P[3], Y[3][3];
+    for (int r = 0; r < 3; ++r) {
+      P[r] = p0[r] + t * (p1[r] - p0[r]);
+      for (int k = 0; k < 3; ++k) Y[k][r] = y0[k][r] + t * (y1[k][r] - y0[k][r]);
+    }
+    double e1[3], e2[3];
+    for (int r = 0; r < 3; ++r) {
+      e1[r] = Y[1][r] - Y[0][r];
+      e2[r] = Y[2][r] - Y[0][r];
+    }
+    double l1 = sqrt(e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2]), l2 = sqrt(e2[0] * e2[0] + e2[1] * e2[1] + e2[2] * e2[2]);
+    for (int r = 0; r < 3; ++r) {
+      e1[r] /= l1;
+      e2[r] /= l2;
+    }
+    double nn[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+    double n2 = nn[0] * nn[0] + nn[1] * nn[1] + nn[2] * nn[2];
+    if (!(n2 >= 1e-6)) continue;
+    double nl = sqrt(n2);
+    for (int r = 0; r < 3; ++r) nn[r] /= nl;
+    double h = (P[0] - Y[0][0]) * nn[0] + (P[1] - Y[0][1]) * nn[1] + (P[2] - Y[0][2]) * nn[2];
+    double q[3][3];
+    for (int k = 0; k < 3; ++k)
+      for (int r = 0; r < 3; ++r) q[k][r] = Y[k][r] - P[r];
+    double b0 = stp3(q[1], q[2], nn), b1 = stp3(q[2], q[0], nn), b2 = stp3(q[0], q[1], nn);
+    double sb = b0 + b1 + b2;
+    double w[3] = {-b0 / sb, -b1 / sb, -b2 / sb};
+    bool inside = fmin(fmin(-w[0], -w[1]), -w[2]) >= -1e-6;
+    if (t == 1) {
+      for (int k = 0; k < 3; ++k) {
+        double dv[3] = {P[0] - Y[k][0], P[1] - Y[k][1], P[2] - Y[k][2]};
+        inside = inside || sqrt(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]) < thk;
+      }
+      for (int k = 0; k < 3; ++k) {
+        const double *A = Y[k], *B = Y[(k + 1) % 3];
+        double pa[3] = {P[0] - A[0], P[1] - A[1], P[2] - A[2]}, ab[3] = {B[0] - A[0], B[1] - A[1], B[2] - A[2]};
+        double dp = pa[0] * ab[0] + pa[1] * ab[1] + pa[2] * ab[2], ll = ab[0] * ab[0] + ab[1] * ab[1] + ab[2] * ab[2];
+        double e[3];
+        for (int r = 0; r < 3; ++r) e[r] = pa[r] - dp * ab[r] / ll;
+        inside = inside || sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]) < thk;
+      }
+    }
+    double rv[3];
+    for (int r = 0; r < 3; ++r) rv[r] = w[0] * v[0][r] + w[1] * v[1][r] + w[2] * v[2][r];
+    double sg = (nn[0] * rv[0] + nn[1] * rv[1] + nn[2] * rv[2]) > 0 ? -1.0 : 1.0;
+    if (fabs(h) < thk && inside) {
+      *tout = t;
+      *dout = fabs(h);
+      for (int r = 0; r < 3; ++r) nout[r] = sg * c[r];
+      return true;
+    }
+  }
+  return false;
+}
+
+// Per-step reset of the self-contact state: impact lists, the force carried
+// between handlings and the group counts.
+__global__ void k_self_reset(int* __restrict__ nimp, long long nar, double* __restrict__ self_cur, long long nc,
+                             int* __restrict__ ngrp, int* __restrict__ ngrp_prev, int R) {
+  long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx < nar) nimp[idx] = 0;
+  if (idx < nc) self_cur[idx] = 0.0;
+  if (idx < R) {
+    ngrp[idx] = 0;
+    ngrp_prev[idx] = 0;
+  }
+}
+
+// Detection pass for (contact vertex a, rollout): every surface face not
+// containing a; hits are added to a's impact list or replace the entry of the
+// same face (collisionBrutal.h:291-306), new ones in face order.
+__global__ void k_self_detect(const int* __restrict__ sc_vtx, const int* __restrict__ sc_face, int ncs, int nsf,
+                              const double* __restrict__ Xn, const double* __restrict__ Ori, int R,
+                              const int* __restrict__ cactive, int* __restrict__ nimp, int* __restrict__ imp_f,
+                              double* __restrict__ imp_t, double* __restrict__ imp_d, double* __restrict__ imp_n,
+                              int* __restrict__ flag) {
+  long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx >= (long long)ncs * R) return;
+  int a = (int)(idx / R), r = (int)(idx % R);
+  if (!cactive[r]) return;
+  const long long C = 3LL * R, NA = (long long)ncs * R;
+  const double* p0 = Xn + (long long)sc_vtx[a] * C + 3 * r;
+  const double* p1 = Ori + (long long)sc_vtx[a] * C + 3 * r;
+  double lo[3], hi[3];
+  for (int c = 0; c < 3; ++c) {
+    double m = 4.0 * kThick + 1e-6 * (1.0 + fabs(p0[c]) + fabs(p1[c]));
+    lo[c] = fmin(p0[c], p1[c]) - m;
+    hi[c] = fmax(p0[c], p1[c]) + m;
+  }
+  int n = nimp[idx];
+  for (int f = 0; f < nsf; ++f) {
+    int f0 = sc_face[3 * f], f1 = sc_face[3 * f + 1], f2 = sc_face[3 * f + 2];
+    if (f0 == a || f1 == a || f2 == a) continue;
+    const double* y0[3] = {Xn + (long long)sc_vtx[f0] * C + 3 * r, Xn + (long long)sc_vtx[f1] * C + 3 * r,
+                           Xn + (long long)sc_vtx[f2] * C + 3 * r};
+    const double* y1[3] = {Ori + (long long)sc_vtx[f0] * C + 3 * r, Ori + (long long)sc_vtx[f1] * C + 3 * r,
+                           Ori + (long long)sc_vtx[f2] * C + 3 * r};
+    bool out = false;
+    for (int c = 0; c < 3 && !out; ++c) {
+      double fl = fmin(fmin(fmin(y0[0][c], y1[0][c]), fmin(y0[1][c], y1[1][c])), fmin(y0[2][c], y1[2][c]));
+      double fh = fmax(fmax(fmax(y0[0][c], y1[0][c]), fmax(y0[1][c], y1[1][c])), fmax(y0[2][c], y1[2][c]));
+      out = fl > hi[c] || fh < lo[c];
+    }
+    if (out) continue;
+    double t, d, nv[3];
+    if (!vf_moving_d(p0, p1, y0, y1, &t, &d, nv)) continue;
+    int k = 0;
+    while (k < n && imp_f[(long long)k * NA + idx] != f) ++k;
+    if (k == n) {
+      if (n == kSelfImp) {
+        atomicOr(flag, 1);
+        continue;
+      }
+      imp_f[(long long)k * NA + idx] = f;
+      ++n;
+    }
+    imp_t[(long long)k * NA + idx] = t;
+    imp_d[(long long)k * NA + idx] = d;
+    for (int c = 0; c < 3; ++c) imp_n[(long long)(3 * k + c) * NA + idx] = nv[c];
+  }
+  nimp[idx] = n;
+}
+
+__global__ void k_self_group_reset(int* __restrict__ gcount, int* __restrict__ glead, long long nfr,
+                                   int* __restrict__ ngrp, int* __restrict__ ngrp_prev, const int* __restrict__ cactive,
+                                   int R) {
+  long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx < nfr) {
+    gcount[idx] = 0;
+    glead[idx] = 0x7fffffff;
+  }
+  if (idx < R && cactive[idx]) {
+    ngrp_prev[idx] = ngrp[idx];
+    ngrp[idx] = 0;
+  }
+}
+
+// Closest impact of each vertex (collisionBrutal.h:648-668: smaller t; with
+// t = 1 on the kept one, smaller d), and its registration with the group of
+// the face it hit: member slot, group count, lowest member (the leader, whose
+// impact normal the group uses).
+__global__ void k_self_closest(int ncs, int nsf, int R, const int* __restrict__ cactive, const int* __restrict__ nimp,
+                               const int* __restrict__ imp_f, const double* __restrict__ imp_t,
+                               const double* __restrict__ imp_d, int* __restrict__ cface,
+                               int* __restrict__ gcount, int* __restrict__ glead, int* __restrict__ gmem,
+                               int* __restrict__ ngrp, int* __restrict__ flag) {
+  long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx >= (long long)ncs * R) return;
+  int a = (int)(idx / R), r = (int)(idx % R);
+  if (!cactive[r]) return;
+  const long long NA = (long long)ncs * R;
+  int n = nimp[idx], best = -1;
+  double bt = 0, bd = 0;
+  for (int k = 0; k < n; ++k) {
+    double t = imp_t[(long long)k * NA + idx], d = imp_d[(long long)k * NA + idx];
+    if (best < 0 || t < bt || (bt == 1 && bd > d)) {
+      best = k;
+      bt = t;
+      bd = d;
+    }
+  }
+  cface[idx] = best;  // slot of the closest impact, -1: none
+  if (best < 0) return;
+  int f = imp_f[(long long)best * NA + idx];
+  long long fr = (long long)f * R + r;
+  int slot = atomicAdd(&gcount[fr], 1);
+  if (slot < kSelfMem)
+    gmem[(long long)slot * nsf * R + fr] = a;
+  else
+    atomicOr(flag, 1);
+  if (slot == 0) atomicAdd(&ngrp[r], 1);
+  atomicMin(&glead[fr], a);
+}
+
+// One self-contact group (face f, rollout r) of a handling (World.h:1451-1530):
+// momentum-exchange among the face and its approaching vertices with the
+// stick/slide friction of do_solve_friction (World.h:1246-1315), in the basis
+// of the leader's impact normal. Writes the face corners' changes GF and the
+// members' reactions GV (zero for members that recede).
+__global__ void k_self_group(const int* __restrict__ sc_vtx, const int* __restrict__ sc_face, int ncs, int nsf,
+                             int R, const int* __restrict__ iactive, const int* __restrict__ gcount,
+                             const int* __restrict__ glead, const int* __restrict__ gmem,
+                             const int* __restrict__ cface, const double* __restrict__ imp_n,
+                             const double* __restrict__ Fo, const double* __restrict__ CN,
+                             const double* __restrict__ Kdiag, double mu, double* __restrict__ GF,
+                             double* __restrict__ GV) {
+  long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx >= (long long)nsf * R) return;
+  int f = (int)(idx / R), r = (int)(idx % R);
+  if (!iactive[r]) return;
+  int cnt = gcount[idx];
+  if (cnt == 0) return;
+  cnt = min(cnt, kSelfMem);
+  const long long C = 3LL * R, NA = (long long)ncs * R;
+  int mem[kSelfMem];
+  for (int k = 0; k < cnt; ++k) mem[k] = gmem[(long long)k * nsf * R + idx];
+  for (int i = 1; i < cnt; ++i)  // ascending contact index
+    for (int j = i; j > 0 && mem[j - 1] > mem[j]; --j) {
+      int t = mem[j];
+      mem[j] = mem[j - 1];
+      mem[j - 1] = t;
+    }
+  int lead = glead[idx];
+  long long li = (long long)lead * R + r;
+  int ls = cface[li];
+  double n[3];
+  for (int c = 0; c < 3; ++c) n[c] = imp_n[(long long)(3 * ls + c) * NA + li];
+  // computeBasis (collisionBrutal.h:408-426): columns e0, t, e2
+  double t[3] = {0.0, n[2], -n[1]};
+  double tn = sqrt(t[1] * t[1] + t[2] * t[2]);
+  if (tn < 1e-15) {
+    t[0] = -n[2]; t[1] = 0.0; t[2] = n[0];
+    tn = sqrt(t[0] * t[0] + t[2] * t[2]);
+  }
+  t[0] /= tn; t[1] /= tn; t[2] /= tn;
+  double b[3] = {n[1] * t[2] - n[2] * t[1], n[2] * t[0] - n[0] * t[2], n[0] * t[1] - n[1] * t[0]};
+  double bn = sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+  double nn = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+  double E[3][3];  // E[c] = column c of the basis
+  for (int r2 = 0; r2 < 3; ++r2) {
+    E[0][r2] = n[r2] / nn;
+    E[1][r2] = t[r2];
+    E[2][r2] = b[r2] / bn;
+  }
+  auto toloc = [&](const double* w, double* l) {
+    for (int c = 0; c < 3; ++c) l[c] = E[c][0] * w[0] + E[c][1] * w[1] + E[c][2] * w[2];
+  };
+  auto toworld = [&](const double* l, double* w) {
+    for (int r2 = 0; r2 < 3; ++r2) w[r2] = E[0][r2] * l[0] + E[1][r2] * l[1] + E[2][r2] * l[2];
+  };
+  int fv[3] = {sc_vtx[sc_face[3 * f]], sc_vtx[sc_face[3 * f + 1]], sc_vtx[sc_face[3 * f + 2]]};
+  double m0 = Kdiag[fv[0]] + Kdiag[fv[1]] + Kdiag[fv[2]];
+  double f0w[3] = {0, 0, 0}, f0[3], r0[3] = {0, 0, 0};
+  for (int q = 0; q < 3; ++q)
+    for (int c = 0; c < 3; ++c) {
+      long long o = (long long)fv[q] * C + 3 * r + c;
+      f0w[c] += Fo[o] + CN[o];
+    }
+  toloc(f0w, f0);
+  int vl[kSelfMem], nv = 0;
+  double ms[kSelfMem], fs[kSelfMem][3], rs[kSelfMem][3];
+  bool stick[kSelfMem];
+  double sm = 0, smv = 0;
+  for (int k = 0; k < cnt; ++k) {
+    int vk = sc_vtx[mem[k]];
+    double fw[3], fl[3];
+    for (int c = 0; c < 3; ++c) {
+      long long o = (long long)vk * C + 3 * r + c;
+      fw[c] = Fo[o] + CN[o];
+    }
+    toloc(fw, fl);
+    for (int c = 0; c < 3; ++c) GV[(long long)mem[k] * C + 3 * r + c] = 0.0;
+    if (fl[0] / Kdiag[vk] - f0[0] / m0 <= 0) {
+      vl[nv] = k;
+      ms[nv] = Kdiag[vk];
+      for (int c = 0; c < 3; ++c) fs[nv][c] = fl[c];
+      ++nv;
+    }
+  }
+  for (int k = 0; k < nv; ++k) {
+    sm += ms[k];
+    smv += fs[k][0];
+  }
+  for (int k = 0; k < nv; ++k) {
+    double fn = (f0[0] + smv) * ms[k] / (m0 + sm) - fs[k][0];
+    rs[k][0] = fn;
+    rs[k][1] = 0.0;
+    rs[k][2] = 0.0;
+    r0[0] -= fn;
+    stick[k] = true;
+  }
+  while (true) {
+    // do_solve_friction
+    double pst[3] = {f0[0], f0[1], f0[2]}, mst = m0, psl[3] = {0, 0, 0}, msl = 0, fnl = 0;
+    for (int k = 0; k < nv; ++k) {
+      if (stick[k]) {
+        for (int c = 0; c < 3; ++c) pst[c] += fs[k][c];
+        mst += ms[k];
+      } else {
+        for (int c = 0; c < 3; ++c) psl[c] += fs[k][c];
+        msl += ms[k];
+        fnl += rs[k][0];
+      }
+    }
+    double v[3] = {pst[0] / mst, pst[1] / mst, pst[2] / mst};
+    r0[1] = v[1] * m0 - f0[1];
+    r0[2] = v[2] * m0 - f0[2];
+    for (int k = 0; k < nv; ++k)
+      if (stick[k]) {
+        rs[k][1] = v[1] * ms[k] - fs[k][1];
+        rs[k][2] = v[2] * ms[k] - fs[k][2];
+      }
+    if (msl > 0) {
+      double rel[3];
+      for (int c = 0; c < 3; ++c) rel[c] = psl[c] / msl - pst[c] / mst;
+      double tl = sqrt(rel[1] * rel[1] + rel[2] * rel[2]);
+      for (int c = 0; c < 3; ++c) rel[c] /= tl;
+      double mult = mu * fnl / (msl + mst);
+      r0[1] += rel[1] * mult * m0;
+      r0[2] += rel[2] * mult * m0;
+      for (int k = 0; k < nv; ++k)
+        if (stick[k]) {
+          rs[k][1] += rel[1] * mult * ms[k];
+          rs[k][2] += rel[2] * mult * ms[k];
+        }
+      double av[3] = {(f0[0] + r0[0]) / m0, (f0[1] + r0[1]) / m0, (f0[2] + r0[2]) / m0};
+      for (int k = 0; k < nv; ++k)
+        if (!stick[k]) {
+          double rv[3];
+          for (int c = 0; c < 3; ++c) rv[c] = fs[k][c] / ms[k] - av[c];
+          double t2 = sqrt(rv[1] * rv[1] + rv[2] * rv[2]);
+          for (int c = 0; c < 3; ++c) rv[c] /= t2;
+          double m = -mu * rs[k][0] * ms[k] / (ms[k] + mst);
+          rs[k][1] = m * rv[1];
+          rs[k][2] = m * rv[2];
+        }
+    }
+    bool changed = false;
+    for (int k = 0; k < nv; ++k)
+      if (stick[k] && rs[k][1] * rs[k][1] + rs[k][2] * rs[k][2] > mu * rs[k][0] * rs[k][0]) {
+        changed = true;
+        stick[k] = false;
+      }
+    if (!changed) break;
+  }
+  double r0w[3], bf0[3], v0[3];
+  toworld(r0, r0w);
+  toworld(f0, bf0);
+  for (int c = 0; c < 3; ++c) v0[c] = (bf0[c] + r0w[c]) / m0;
+  for (int q = 0; q < 3; ++q)
+    for (int c = 0; c < 3; ++c) {
+      long long o = (long long)fv[q] * C + 3 * r + c;
+      GF[(long long)(3 * f + q) * C + 3 * r + c] = v0[c] * Kdiag[fv[q]] - (Fo[o] + CN[o]);
+    }
+  for (int k = 0; k < nv; ++k) {
+    double w[3];
+    toworld(rs[k], w);
+    for (int c = 0; c < 3; ++c) GV[(long long)mem[vl[k]] * C + 3 * r + c] = w[c];
+  }
+}
+
+// End of a handling with self contact (World.h:1585-1600): rhs = forces +
+// obstacle reactions + self-contact forces, which are also carried to the next
+// handling; then the relaxed Jacobi update of k_contact_jacobi.
+__global__ void k_self_finish(const int* __restrict__ scf_ptr, const int* __restrict__ scf_q,
+                              const int* __restrict__ sc_of_v, int ncs, int R, const int* __restrict__ gcount,
+                              const int* __restrict__ glead, const int* __restrict__ imp_f,
+                              const int* __restrict__ cface, const int* __restrict__ ngrp,
+                              const double* __restrict__ GF, const double* __restrict__ GV,
+                              const double* __restrict__ RC, const double* __restrict__ CN,
+                              const double* __restrict__ Fo, double* __restrict__ self_cur, double* __restrict__ Prev,
+                              double* __restrict__ Xit, const double* __restrict__ Xn,
+                              const double* __restrict__ Kdiag, double alpha, double h, int first_iter,
+                              const int* __restrict__ alist, const int* __restrict__ acount, int N) {
+  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)N * R) return;
+  const int v = (int)(idx / R), j = (int)(idx % R);
+  if (j >= *acount) return;
+  const int r = alist[j];
+  if (ngrp[r] == 0) return;
+  const long long C = 3LL * R, base = (long long)v * C + 3 * r;
+  // the vertex's shares from the groups of its incident faces and from its own
+  // group, added in group order (ascending leader), as the handling adds them
+  double sf[3] = {0, 0, 0};
+  int own_lead = 0x7fffffff;
+  const int a = sc_of_v[v];
+  if (a >= 0) {
+    const long long ai = (long long)a * R + r;
+    const int slot = cface[ai];
+    if (slot >= 0) own_lead = glead[(long long)imp_f[(long long)slot * ncs * R + ai] * R + r];
+  }
+  for (int last = -1;;) {
+    int best = 0x7fffffff, bq = -1;
+    for (int q = scf_ptr[v]; q < scf_ptr[v + 1]; ++q) {
+      long long fr = (long long)(scf_q[q] / 3) * R + r;
+      if (gcount[fr] == 0) continue;
+      int L = glead[fr];
+      if (L > last && L < best) {
+        best = L;
+        bq = q;
+      }
+    }
+    if (own_lead > last && own_lead < best) {
+      best = own_lead;
+      bq = -2;
+    }
+    if (best == 0x7fffffff) break;
+    const double* g = bq == -2 ? GV + (long long)a * C + 3 * r : GF + (long long)scf_q[bq] * C + 3 * r;
+    for (int c = 0; c < 3; ++c) sf[c] += g[c];
+    last = best;
+  }
+  double kd = Kdiag[v];
+  for (int c = 0; c < 3; ++c) {
+    double rc = RC[base + c] + (CN[base + c] + sf[c]);
+    self_cur[base + c] = sf[c];
+    double pv = first_iter ? Fo[base + c] : Prev[base + c];
+    double p = rc * alpha + pv * (1.0 - alpha);
+    Prev[base + c] = p;
+    Xit[base + c] = Xn[base + c] + (p / h) / kd;
+  }
+}
+
 // Rollouts with no fresh impact leave the contact phase; the ones that stay
 // start an inner Jacobi pass (World.h:938-944).
 __global__ void k_contact_pass_begin(int* __restrict__ cactive, int* __restrict__ fresh, int* __restrict__ iactive,
-                                     int* __restrict__ npass, int R) {
+                                     int* __restrict__ npass, int R, const int* __restrict__ ngrp = nullptr,
+                                     const int* __restrict__ ngrp_prev = nullptr) {
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= R) return;
-  int a = cactive[r] && fresh[r];
+  // with self contact a pass also follows when the number of groups changed
+  // (collisionBrutal.h:404)
+  int a = cactive[r] && (fresh[r] || (ngrp && ngrp[r] != ngrp_prev[r]));
   cactive[r] = a;
   iactive[r] = a;
   npass[r] += a;
@@ -1504,7 +2029,15 @@ __global__ void k_contact_jacobi(const int* __restrict__ Lptr, const int* __rest
                                  const int* __restrict__ win_st, const double* __restrict__ win_n,
                                  double mu, double alpha, double h, int first_iter, const int* __restrict__ alist,
                                  const int* __restrict__ acount, double* __restrict__ Prev, double* __restrict__ Xit,
-                                 int N, int R, double* __restrict__ tRC, double* __restrict__ tN) {
+                                 int N, int R, double* __restrict__ tRC, double* __restrict__ tN,
+                                 double* __restrict__ self_cur = nullptr, double* __restrict__ Fo = nullptr,
+                                 double* __restrict__ CN = nullptr, double* __restrict__ RC = nullptr,
+                                 const int* __restrict__ ngrp = nullptr) {
+  // self_cur non-null: self contact is on (World.h:1373-1430 with
+  // m_handle_self_collision): the obstacle reaction is computed from the
+  // forces plus the previous self-contact forces and kept in CN; it replaces
+  // the rhs only while the rollout has no self-contact group; with groups the
+  // update is left to k_self_finish (RC = rhs before the self-contact sums).
   long long idx = (long long)xcd_block() * blockDim.x + threadIdx.x;
   if (idx >= (long long)N * R) return;
   const int v = (int)(idx / R), j = (int)(idx % R);
@@ -1530,6 +2063,14 @@ __global__ void k_contact_jacobi(const int* __restrict__ Lptr, const int* __rest
     pv[0] = rc[0]; pv[1] = rc[1]; pv[2] = rc[2];
   } else {
     pv[0] = Prev[base]; pv[1] = Prev[base + 1]; pv[2] = Prev[base + 2];
+  }
+  const bool self = self_cur != nullptr;
+  const bool grp = self && ngrp[r] > 0;
+  if (self) {
+    for (int c = 0; c < 3; ++c) {
+      Fo[base + c] = rc[c];
+      CN[base + c] = 0.0;
+    }
   }
   int ci = cv_of_v[v];
   if (ci >= 0 && tRC) {
@@ -1564,20 +2105,33 @@ __global__ void k_contact_jacobi(const int* __restrict__ Lptr, const int* __rest
       double nn = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
       double e0[3] = {n[0] / nn, n[1] / nn, n[2] / nn};
       double e2[3] = {b[0] / bn, b[1] / bn, b[2] / bn};
-      double f0 = e0[0] * rc[0] + e0[1] * rc[1] + e0[2] * rc[2];
+      double fin[3] = {rc[0], rc[1], rc[2]};
+      if (self)
+        for (int c = 0; c < 3; ++c) fin[c] += self_cur[base + c];
+      double f0 = e0[0] * fin[0] + e0[1] * fin[1] + e0[2] * fin[2];
       if (!(f0 > 0.0)) {
-        double f1 = t[0] * rc[0] + t[1] * rc[1] + t[2] * rc[2];
-        double f2 = e2[0] * rc[0] + e2[1] * rc[1] + e2[2] * rc[2];
+        double f1 = t[0] * fin[0] + t[1] * fin[1] + t[2] * fin[2];
+        double f2 = e2[0] * fin[0] + e2[1] * fin[1] + e2[2] * fin[2];
         double r0 = -f0, r1 = -f1, r2 = -f2;
         double rt = sqrt(r1 * r1 + r2 * r2);
         if (rt > mu * r0) {
           r1 *= mu * r0 / rt;
           r2 *= mu * r0 / rt;
         }
-        for (int c = 0; c < 3; ++c) rc[c] = e0[c] * r0 + t[c] * r1 + e2[c] * r2;
+        for (int c = 0; c < 3; ++c) {
+          double o = e0[c] * r0 + t[c] * r1 + e2[c] * r2;
+          if (self) CN[base + c] = o;
+          if (!grp) rc[c] = o;
+        }
       }
     }
   }
+  if (grp) {
+    for (int c = 0; c < 3; ++c) RC[base + c] = rc[c];
+    return;
+  }
+  if (self)
+    for (int c = 0; c < 3; ++c) self_cur[base + c] = 0.0;
   double kd = Kdiag[v];
   for (int c = 0; c < 3; ++c) {
     double p = rc[c] * alpha + pv[c] * (1.0 - alpha);
@@ -1850,6 +2404,12 @@ struct pdm_scene_s {
   bool have_tape = false;
   int tape_steps = 0;
   bool act_null = true;
+  // self contact: surface mesh, per (vertex, rollout) impact lists and closest
+  // impact, per (face, rollout) groups, and the handling's force buffers
+  int ncs = 0, nsf = 0;
+  DevBuf<int> d_sc_vtx, d_sc_face, d_sc_of_v, d_scf_ptr, d_scf_q;
+  DevBuf<int> s_nimp, s_impf, s_cface, s_gcount, s_glead, s_gmem, s_ngrp, s_ngrp_prev;
+  DevBuf<double> s_impt, s_impd, s_impn, s_Fo, s_CN, s_RC, s_cur, s_GF, s_GV;
   // underwater drag: contour faces, incident faces per vertex, face forces and
   // the reverse's per-corner / per-face gradients
   int nfaces = 0;
@@ -1939,7 +2499,14 @@ int upload_static(pdm_scene s) {
     return 1;
   s->ntri = (int)(S.tris.size() / 9);
   s->nobs = (int)(S.planes.size() / 6 + S.spheres.size() / 4) + s->ntri;
-  s->ncv = (S.contact && s->nobs > 0) ? (int)S.contact_vtx.size() : 0;
+  s->ncs = (S.contact && S.self_collision) ? (int)S.sc_vtx.size() : 0;
+  s->nsf = s->ncs > 0 ? (int)(S.sc_face.size() / 3) : 0;
+  if (s->ncs > 0 && (s->d_sc_vtx.upload(S.sc_vtx.data(), S.sc_vtx.size()) ||
+                     s->d_sc_face.upload(S.sc_face.data(), S.sc_face.size()) ||
+                     s->d_sc_of_v.upload(S.sc_of_v.data(), S.N) || s->d_scf_ptr.upload(S.scf_ptr.data(), S.N + 1) ||
+                     s->d_scf_q.upload(S.scf_q.data(), S.scf_q.size())))
+    return 1;
+  s->ncv = (S.contact && (s->nobs > 0 || s->ncs > 0)) ? (int)S.contact_vtx.size() : 0;
   if (s->ncv > 0) {
     std::vector<int> cvo(S.N, -1);
     for (int k = 0; k < s->ncv; ++k) cvo[S.contact_vtx[k]] = k;
@@ -2139,6 +2706,9 @@ int host_setup(const pdm_scene_desc* d, pdm_scene s) {
     S.tris.assign(d->tris, d->tris + 9 * (size_t)d->n_tris);
     build_tri_bvh(S);
   }
+  S.self_collision = d->self_collision;
+  S.self_mu = d->self_mu;
+  if (S.self_collision) build_self_mesh(S, d->tets, d->n_tets);
   S.underwater = d->underwater;
   if (S.underwater) {
     std::vector<int> cf = contour_faces(d->x_rest, d->tets, d->n_tets);
@@ -2210,6 +2780,15 @@ int ensure_batch_buffers(pdm_scene s) {
         s->nseen.alloc(ni) || s->seen.alloc(kSeen * ni) || s->win_n.alloc(3 * ni) || s->cactive.alloc(s->R + 1) ||
         s->iactive.alloc(s->R) || s->cfirst.alloc(s->R) || s->fresh.alloc(s->R) || s->npass.alloc(s->R) ||
         s->citers.alloc(s->R) || s->clast.alloc(s->R) || s->cinj.alloc(s->R))
+      return 1;
+  }
+  if (s->ncs > 0) {
+    size_t na = (size_t)s->ncs * s->R, nf = (size_t)s->nsf * s->R;
+    if (s->s_nimp.alloc(na) || s->s_impf.alloc(kSelfImp * na) || s->s_impt.alloc(kSelfImp * na) ||
+        s->s_impd.alloc(kSelfImp * na) || s->s_impn.alloc(3 * kSelfImp * na) || s->s_cface.alloc(na) ||
+        s->s_gcount.alloc(nf) || s->s_glead.alloc(nf) || s->s_gmem.alloc(kSelfMem * nf) || s->s_ngrp.alloc(s->R) ||
+        s->s_ngrp_prev.alloc(s->R) || s->s_Fo.alloc(nc) || s->s_CN.alloc(nc) || s->s_RC.alloc(nc) ||
+        s->s_cur.alloc(nc) || s->s_GF.alloc(3 * nf * 3) || s->s_GV.alloc(na * 3))
       return 1;
   }
   if (s->tuned_C != s->C && autotune_levels(s)) return 1;
@@ -2563,6 +3142,13 @@ int contact_phase(pdm_scene s, const double* act, int act_steps, int step, Conta
   hipLaunchKernelGGL(k_contact_reset, dim3(cdiv(std::max<long long>(ni, R), kThreads)), dim3(kThreads), 0, s->stream,
                      s->win_st.p, s->win_o.p, s->nseen.p, ni, s->cactive.p, s->cfirst.p, s->fresh.p, s->clast.p,
                      s->npass.p, s->citers.p, R);
+  const bool self = s->ncs > 0;
+  if (self) {
+    long long na = (long long)s->ncs * R;
+    hipLaunchKernelGGL(k_self_reset, dim3(cdiv(std::max<long long>(std::max<long long>(na, (long long)nc), R), kThreads)),
+                       dim3(kThreads), 0, s->stream, s->s_nimp.p, na, s->s_cur.p, (long long)nc, s->s_ngrp.p,
+                       s->s_ngrp_prev.p, R);
+  }
   std::vector<int> ca(R + 1);
   size_t cvc = (size_t)s->ncv * C;
   if (tp) {
@@ -2576,13 +3162,24 @@ int contact_phase(pdm_scene s, const double* act, int act_steps, int step, Conta
                        s->cvtx.p, s->ncv, s->Xn.p, s->Ori.p, R, s->d_planes.p, np, s->d_spheres.p, nsph, s->d_tris.p,
                        s->ntri, s->d_bvh_box.p, s->d_bvh_a.p, s->d_bvh_b.p, s->d_bvh_tri.p, s->win_st.p, s->win_o.p,
                        s->win_n.p, s->nseen.p, s->seen.p, pass, s->cactive.p, s->fresh.p);
+    if (self) {
+      long long na = (long long)s->ncs * R, nf = (long long)s->nsf * R;
+      hipLaunchKernelGGL(k_self_detect, dim3(cdiv(na, kThreads)), dim3(kThreads), 0, s->stream, s->d_sc_vtx.p,
+                         s->d_sc_face.p, s->ncs, s->nsf, s->Xn.p, s->Ori.p, R, s->cactive.p, s->s_nimp.p, s->s_impf.p,
+                         s->s_impt.p, s->s_impd.p, s->s_impn.p, s->cactive.p + R);
+      hipLaunchKernelGGL(k_self_group_reset, dim3(cdiv(std::max<long long>(nf, R), kThreads)), dim3(kThreads), 0,
+                         s->stream, s->s_gcount.p, s->s_glead.p, nf, s->s_ngrp.p, s->s_ngrp_prev.p, s->cactive.p, R);
+      hipLaunchKernelGGL(k_self_closest, dim3(cdiv(na, kThreads)), dim3(kThreads), 0, s->stream, s->ncs, s->nsf, R,
+                         s->cactive.p, s->s_nimp.p, s->s_impf.p, s->s_impt.p, s->s_impd.p, s->s_cface.p,
+                         s->s_gcount.p, s->s_glead.p, s->s_gmem.p, s->s_ngrp.p, s->cactive.p + R);
+    }
     if (pass == S.outer_cap) break;  // cap reached: keep ori_x
     hipLaunchKernelGGL(k_contact_pass_begin, dim3(cdiv(R, 64)), dim3(64), 0, s->stream, s->cactive.p, s->fresh.p,
-                       s->iactive.p, s->npass.p, R);
+                       s->iactive.p, s->npass.p, R, self ? s->s_ngrp.p : nullptr, self ? s->s_ngrp_prev.p : nullptr);
     HIPX(hipMemcpyAsync(ca.data(), s->cactive.p, (R + 1) * sizeof(int), hipMemcpyDeviceToHost, s->stream));
     HIPX(hipStreamSynchronize(s->stream));
     if (ca[R]) {
-      g_err = "contact: a vertex met more than 8 distinct obstacle faces in one step";
+      g_err = "contact: a vertex met more than 8 distinct obstacle faces, or a self-contact list overflowed, in one step";
       return 1;
     }
     bool any = false;
@@ -2609,7 +3206,20 @@ int contact_phase(pdm_scene s, const double* act, int act_steps, int step, Conta
                          s->Lptr.p, s->Lcol.p, s->Lval.p, s->B.p, s->Dn.p, s->Xn.p, s->Kdiag.p, s->cv_of_v.p,
                          s->win_st.p, s->win_n.p, S.mu, S.alpha, S.h, i == 0 ? 1 : 0, s->alist.p,
                          s->acount.p, s->Prev.p, s->Xit.p, N, R, tp ? tp->rc.p + (size_t)tp->G * cvc : nullptr,
-                         tp ? tp->nrm.p + (size_t)tp->G * cvc : nullptr);
+                         tp ? tp->nrm.p + (size_t)tp->G * cvc : nullptr, self ? s->s_cur.p : nullptr,
+                         self ? s->s_Fo.p : nullptr, self ? s->s_CN.p : nullptr, self ? s->s_RC.p : nullptr,
+                         self ? s->s_ngrp.p : nullptr);
+      if (self) {
+        hipLaunchKernelGGL(k_self_group, dim3(cdiv((long long)s->nsf * R, kThreads)), dim3(kThreads), 0, s->stream,
+                           s->d_sc_vtx.p, s->d_sc_face.p, s->ncs, s->nsf, R, s->iactive.p, s->s_gcount.p,
+                           s->s_glead.p, s->s_gmem.p, s->s_cface.p, s->s_impn.p, s->s_Fo.p, s->s_CN.p, s->Kdiag.p,
+                           S.self_mu, s->s_GF.p, s->s_GV.p);
+        hipLaunchKernelGGL(k_self_finish, dim3(cdiv((long long)N * R, kThreads)), dim3(kThreads), 0, s->stream,
+                           s->d_scf_ptr.p, s->d_scf_q.p, s->d_sc_of_v.p, s->ncs, R, s->s_gcount.p, s->s_glead.p,
+                           s->s_impf.p, s->s_cface.p, s->s_ngrp.p,
+                           s->s_GF.p, s->s_GV.p, s->s_RC.p, s->s_CN.p, s->s_Fo.p, s->s_cur.p, s->Prev.p, s->Xit.p,
+                           s->Xn.p, s->Kdiag.p, S.alpha, S.h, i == 0 ? 1 : 0, s->alist.p, s->acount.p, N);
+      }
       if (tp) {
         if (dcopy(s, tp->xit.p + (size_t)(tp->G + 1) * nc, s->Xit.p, nc)) return 1;
         tp->first.push_back(i == 0 ? 1 : 0);
@@ -2929,6 +3539,10 @@ int backward_impl(pdm_scene s, const double* gxT, const double* gvT, double* gx0
     g_err = "backward needs a recorded forward (record=1)";
     return 1;
   }
+  if (s->ncs > 0) {
+    g_err = "the reverse sweep does not cover self contact";
+    return 1;
+  }
   HostScene& S = s->S;
   int N = S.N, C = s->C, R = s->R, steps = s->tape_steps;
   size_t nc = s->NC();
@@ -3222,6 +3836,7 @@ int pdm_scene_info(pdm_scene s, pdm_info* info) {
   info->n_contact_vertices = (int)s->S.contact_vtx.size();
   info->n_drag_faces = s->nfaces;
   info->n_obstacle_tris = s->ntri;
+  info->n_self_faces = s->nsf;
   info->batch = s->R;
   info->precision = s->precision;
   return 0;

@@ -51,6 +51,13 @@ struct HostScene {
   // 3 * face + corner in ascending face order (vfptr: N+1)
   int underwater = 0;
   std::vector<int> cf;
+  // self contact: surface vertices in the reference's contact numbering
+  // (first appearance over the sorted surface triangles), surface triangles
+  // in contact indices, and per engine vertex its incident self faces as
+  // 3 * face + corner (ascending)
+  int self_collision = 0;
+  double self_mu = 1.0;
+  std::vector<int> sc_vtx, sc_face, sc_of_v, scf_ptr, scf_q;
   std::vector<int> vfptr, vfq;
   // stepping
   double h = 0.005, damping = 0.9999;
@@ -118,6 +125,9 @@ bool add_muscles(HostScene& S, const double* x_caller, const int* tets_caller, i
 // Outward-oriented surface faces of a tet mesh (caller numbering), in the
 // order Dfobj::Initialize lists its contours (Dfobj.h:497-596).
 std::vector<int> contour_faces(const double* xrest, const int* tets, int T);
+// Surface mesh for self contact (Dfobj.h:497-566, World.h:540-558), from the
+// caller's tets; fills S.sc_* in engine vertex ids.
+void build_self_mesh(HostScene& S, const int* tets, int T);
 // Median-split bounding-volume hierarchy of the obstacle triangles (S.tris).
 void build_tri_bvh(HostScene& S);
 // Incident faces of every vertex (3 * face + corner, ascending face order).

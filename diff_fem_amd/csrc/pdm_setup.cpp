@@ -247,6 +247,43 @@ void build_tri_bvh(HostScene& S) {
   }
 }
 
+void build_self_mesh(HostScene& S, const int* tets, int T) {
+  const int fc[4][3] = {{0, 1, 2}, {0, 1, 3}, {0, 2, 3}, {1, 2, 3}};
+  std::vector<std::array<int, 3>> tri(4 * (size_t)T);
+  for (int t = 0; t < T; ++t) {
+    int sv[4] = {tets[4 * t], tets[4 * t + 1], tets[4 * t + 2], tets[4 * t + 3]};
+    std::sort(sv, sv + 4);
+    for (int k = 0; k < 4; ++k) tri[4 * (size_t)t + k] = {sv[fc[k][0]], sv[fc[k][1]], sv[fc[k][2]]};
+  }
+  std::vector<int> ord(tri.size());
+  std::iota(ord.begin(), ord.end(), 0);
+  std::sort(ord.begin(), ord.end(), [&](int a, int b) { return tri[a] < tri[b] || (tri[a] == tri[b] && a < b); });
+  std::vector<char> uniq(tri.size(), 0);
+  for (size_t k = 0; k < ord.size(); ++k) {
+    bool sp = k > 0 && tri[ord[k - 1]] == tri[ord[k]], sn = k + 1 < ord.size() && tri[ord[k + 1]] == tri[ord[k]];
+    uniq[ord[k]] = !(sp || sn);
+  }
+  S.sc_vtx.clear();
+  S.sc_face.clear();
+  std::vector<int> cidx(S.N, -1);  // caller vertex -> contact index
+  for (size_t i = 0; i < tri.size(); ++i) {
+    if (!uniq[i]) continue;
+    for (int k = 0; k < 3; ++k) {
+      int v = tri[i][k];
+      if (cidx[v] < 0) {
+        cidx[v] = (int)S.sc_vtx.size();
+        S.sc_vtx.push_back(S.iperm[v]);
+      }
+      S.sc_face.push_back(cidx[v]);
+    }
+  }
+  S.sc_of_v.assign(S.N, -1);
+  for (size_t a = 0; a < S.sc_vtx.size(); ++a) S.sc_of_v[S.sc_vtx[a]] = (int)a;
+  std::vector<int> fe(S.sc_face.size());
+  for (size_t q = 0; q < fe.size(); ++q) fe[q] = S.sc_vtx[S.sc_face[q]];
+  vertex_faces(S.N, fe, S.scf_ptr, S.scf_q);
+}
+
 void vertex_faces(int N, const std::vector<int>& cf, std::vector<int>& vfptr, std::vector<int>& vfq) {
   vfptr.assign(N + 1, 0);
   for (int v : cf) ++vfptr[v + 1];

@@ -82,6 +82,8 @@ def make_desc(desc):
         d.gravity[i] = float(desc.gravity[i])
     d.gravity_on = int(desc.gravity_on)
     d.underwater = int(desc.underwater)
+    d.self_collision = int(desc.self_collision)
+    d.self_mu = float(desc.self_mu)
     d.contact = int(desc.contact)
     d.max_iter1 = int(desc.max_iter1)
     d.alpha = float(desc.alpha)

@@ -51,6 +51,8 @@ class SceneDesc:
     gravity: tuple = (0.0, 0.0, 0.0)
     gravity_on: bool = True
     underwater: bool = False           # "Is underwater": surface drag, no gravity
+    self_collision: bool = False       # "Self-collision": surface vertices against surface faces
+    self_mu: float = 1.0               # "World Friction": friction of self contact
     contact: bool = False
     max_iter1: int = 30
     alpha: float = 1.0
@@ -383,8 +385,9 @@ def load_json_scene(config_path: str, data_root: Optional[str] = None) -> SceneD
     config's own folder. "Obstacles" are static triangle meshes (OBJ
     ``f`` polygons, fan-triangulated). Scene features outside the
     projective-dynamics path (cloth, rigid bones, pneumatic volume muscles,
-    self collision, skeleton or pneumatic actuators, drag on several meshes)
-    raise NotImplementedError."""
+    skeleton or pneumatic actuators, drag on several meshes) raise
+    NotImplementedError. "Self-collision" is simulated forward; its reverse
+    sweep is not available."""
     root = data_root if data_root is not None else os.environ.get("PDM_DATA_ROOT", "")
     path = os.path.join(root, config_path.lstrip("/")) if root else config_path
     if not os.path.exists(path):
@@ -407,8 +410,8 @@ def load_json_scene(config_path: str, data_root: Optional[str] = None) -> SceneD
         raise ValueError("The Gravity vector should have the length of 3 but got length %d" % len(g))
     gravity = tuple(float(v) for v in g[:3])
     alpha = float(para.get("World Alpha", 1.0))
-    if para.get("Self-collision", False):
-        _unsupported("Self-collision")
+    self_collision = bool(para.get("Self-collision", False))
+    self_mu = float(para.get("World Friction", 1.0))
     underwater = bool(int(para.get("Is underwater", 0)))
     if _json_len(js, "Clothes"):
         _unsupported("Clothes")
@@ -479,7 +482,7 @@ def load_json_scene(config_path: str, data_root: Optional[str] = None) -> SceneD
                                 k_muscle=float(m.get("Muscle Stiffness", 5e5)), h=1.0 / hz, max_iter=max_iter,
                                 damping=damping, gravity=gravity, gravity_on=bool(int(m.get("Enable Gravity", 1))),
                                 underwater=underwater, contact=contact, max_iter1=max_iter, alpha=alpha, mu=1.0,
-                                tris=tris if k == 0 else None, **kw))
+                                tris=tris if k == 0 else None, self_collision=self_collision, self_mu=self_mu, **kw))
         infos.append(DfobjInfo(vertex_offset=nv, n_vertices=x.shape[0], tet_offset=nt, n_tets=T,
                                actuator_type=act_type, n_muscles=n_mus, group_offset=ng))
         nv += x.shape[0]
@@ -549,7 +552,7 @@ def pack_scenes(descs: List[SceneDesc], gap: float = 0.4, name: str = "packed", 
     d0 = descs[0]
     for d in descs[1:]:
         for k in ("h", "max_iter", "damping", "gravity_on", "underwater", "contact", "max_iter1", "alpha", "mu",
-                  "mass_same"):
+                  "mass_same", "self_collision", "self_mu"):
             if getattr(d, k) != getattr(d0, k):
                 raise ValueError("packed meshes disagree on %s" % k)
         if tuple(d.gravity) != tuple(d0.gravity):
@@ -609,7 +612,7 @@ def pack_scenes(descs: List[SceneDesc], gap: float = 0.4, name: str = "packed", 
                      attach_idx=np.concatenate(att), attach_w=att_w.pop() if att_w else d0.attach_w,
                      k_muscle=k_mus.pop() if k_mus else d0.k_muscle, h=d0.h,
                      max_iter=d0.max_iter, damping=d0.damping, gravity=d0.gravity, gravity_on=d0.gravity_on,
-                     underwater=d0.underwater,
+                     underwater=d0.underwater, self_collision=d0.self_collision, self_mu=d0.self_mu,
                      contact=d0.contact, max_iter1=d0.max_iter1, alpha=d0.alpha, mu=d0.mu, planes=planes,
                      spheres=np.array(spheres) if spheres else None, tris=np.concatenate(tris) if tris else None,
                      body_of_tet=np.concatenate(bodies),

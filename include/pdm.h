@@ -85,6 +85,11 @@ typedef struct pdm_scene_desc {
    * them by ArcSim's vertex-face test (collision.cpp:127-278). */
   int n_tris;
   const double* tris;
+  /* "Self-collision" (World.h:1246-1315, 1446-1600): surface vertices against
+   * the body's own surface faces, with "World Friction" self_mu. Forward
+   * only: pdm_backward refuses scenes that enable it. */
+  int self_collision;
+  double self_mu;
 } pdm_scene_desc;
 
 typedef struct pdm_info {
@@ -99,6 +104,7 @@ typedef struct pdm_info {
   int precision;             /* PDM_FP64 or PDM_MIXED */
   int n_drag_faces;          /* surface faces carrying drag (0 unless underwater) */
   int n_obstacle_tris;       /* static obstacle triangles */
+  int n_self_faces;          /* surface faces tested for self contact */
 } pdm_info;
 
 enum { PDM_FP64 = 0, PDM_MIXED = 1 };

@@ -69,6 +69,10 @@ def lib():
         L.orc_volume_projection_jac.argtypes = [_dp, _dp, _dp, _dp]
         L.orc_set_underwater.argtypes = [C.c_void_p, C.c_int]
         L.orc_set_tris.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.orc_set_self_collision.argtypes = [C.c_void_p, C.c_int, C.c_double]
+        L.orc_solve_cubic.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, _dp]
+        L.orc_vf_moving.argtypes = [_dp, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.orc_self_group.argtypes = [_dp, _dp, _dp, C.c_int, _dp, _dp, C.c_double, _dp, _dp]
         L.orc_detect_tri.argtypes = [_dp, _dp, _dp, _dp]
         L.orc_num_drag_faces.argtypes = [C.c_void_p]
         L.orc_get_drag_faces.argtypes = [C.c_void_p, _ip]
@@ -147,6 +151,8 @@ class OracleScene:
                             desc.spheres.shape[0], np.ascontiguousarray(desc.spheres).ravel().copy())
         if getattr(desc, "underwater", False):
             L.orc_set_underwater(self.h, 1)
+        if getattr(desc, "self_collision", False):
+            L.orc_set_self_collision(self.h, 1, float(desc.self_mu))
         tris = getattr(desc, "tris", None)
         if tris is not None and len(tris):
             L.orc_set_tris(self.h, len(tris), np.ascontiguousarray(tris, np.float64).ravel().copy())
@@ -309,6 +315,29 @@ def volume_projection_jac(S):
     D, J, r = np.zeros(3), np.zeros((3, 3)), np.zeros(3)
     lib().orc_volume_projection_jac(np.ascontiguousarray(S, np.float64), D, J, r)
     return D, J, r
+
+
+def solve_cubic(a, b, c, d):
+    x = np.zeros(3)
+    n = lib().orc_solve_cubic(float(a), float(b), float(c), float(d), x)
+    return x[:n]
+
+
+def vf_moving(p0, p1, y0, y1):
+    """(hit, t, d, n) of a vertex p0 -> p1 against a face y0 -> y1 (3x3 each)."""
+    t, d, n = np.zeros(1), np.zeros(1), np.zeros(3)
+    f = lambda a: np.ascontiguousarray(a, np.float64).ravel()  # noqa: E731
+    hit = lib().orc_vf_moving(f(p0), f(p1), f(y0), f(y1), t, d, n)
+    return bool(hit), float(t[0]), float(d[0]), n
+
+
+def self_group(n, f3, m3, fv, mv, mu):
+    """Self-contact forces of one group: (face corners 3x3, members kx3)."""
+    f = lambda a: np.ascontiguousarray(a, np.float64).ravel()  # noqa: E731
+    fv = np.asarray(fv, np.float64).reshape(-1, 3)
+    o3, ov = np.zeros(9), np.zeros(fv.size)
+    lib().orc_self_group(f(n), f(f3), f(m3), fv.shape[0], f(fv), f(mv), float(mu), o3, ov)
+    return o3.reshape(3, 3), ov.reshape(-1, 3)
 
 
 def detect_tri(tri, x0, x1):

@@ -417,6 +417,7 @@ struct StepTape {
   bool contact_active = false;
   std::vector<ContactIter> citers;         // executed contact iterations
   bool final_latest = true;                // result = x_it of the last iteration (else its input)
+  bool self_active = false;                // a self-contact group acted (no reverse for it)
 };
 
 struct Scene {
@@ -439,6 +440,13 @@ struct Scene {
   int max_iter = 30, max_iter1 = 30;
   int contact = 0;
   double alpha = 1.0, mu = 1.0;
+  // self contact ("Self-collision", World.h:1246-1315, 1446-1586): the surface
+  // mesh in the reference's contact numbering (vertices in order of first
+  // appearance over the sorted surface triangles, Dfobj.h:497-566)
+  int self_collision = 0;
+  double self_mu = 1.0;                       // "World Friction"
+  std::vector<int> sc_vtx;                    // contact index -> vertex
+  std::vector<std::array<int, 3>> sc_face;    // surface triangles, contact indices
   int outer_cap = 16;  // detection passes with a Jacobi loop (the reference has no cap)
   std::vector<Plane> planes;
   std::vector<Sphere> spheres;
@@ -834,6 +842,274 @@ bool detect_tri(const double* tri, const double* x0, const double* x1, double* n
   return false;
 }
 
+// ---------------------------------------------------------------- self contact
+// ArcSim's polynomial root finders (collision.cpp:151-170, 280-326), restated:
+// the roots in [lo, hi] order of a x^3 + b x^2 + c x + d used by the moving
+// vertex-face test.
+static double sgn_d(double x) { return x < 0 ? -1.0 : 1.0; }
+static int solve_quadratic_as(double a, double b, double c, double x[2]) {
+  double d = b * b - 4 * a * c;
+  if (d < 0) {
+    x[0] = -b / (2 * a);
+    return 0;
+  }
+  double q = -(b + sgn_d(b) * std::sqrt(d)) / 2;
+  int i = 0;
+  if (std::fabs(a) > 1e-12 * std::fabs(q)) x[i++] = q / a;
+  if (std::fabs(q) > 1e-12 * std::fabs(c)) x[i++] = c / q;
+  if (i == 2 && x[0] > x[1]) std::swap(x[0], x[1]);
+  return i;
+}
+static double newtons_method_as(double a, double b, double c, double d, double x0, int init_dir) {
+  if (init_dir != 0) {
+    double y0 = d + x0 * (c + x0 * (b + x0 * a)), ddy0 = 2 * b + x0 * (6 * a);
+    x0 += init_dir * std::sqrt(std::fabs(2 * y0 / ddy0));
+  }
+  for (int iter = 0; iter < 100; iter++) {
+    double y = d + x0 * (c + x0 * (b + x0 * a));
+    double dy = c + x0 * (2 * b + x0 * 3 * a);
+    if (dy == 0) return x0;
+    double x1 = x0 - y / dy;
+    if (std::fabs(x0 - x1) < 1e-6) return x0;
+    x0 = x1;
+  }
+  return x0;
+}
+static int solve_cubic_as(double a, double b, double c, double d, double x[3]) {
+  double xc[2] = {0.0, 0.0};
+  int ncrit = solve_quadratic_as(3 * a, 2 * b, c, xc);
+  if (ncrit == 0) {
+    x[0] = newtons_method_as(a, b, c, d, xc[0], 0);
+    return 1;
+  } else if (ncrit == 1) {
+    return solve_quadratic_as(b, c, d, x);
+  }
+  double yc[2] = {d + xc[0] * (c + xc[0] * (b + xc[0] * a)), d + xc[1] * (c + xc[1] * (b + xc[1] * a))};
+  int i = 0;
+  if (yc[0] * a >= 0) x[i++] = newtons_method_as(a, b, c, d, xc[0], -1);
+  if (yc[0] * yc[1] <= 0) {
+    int closer = std::fabs(yc[0]) < std::fabs(yc[1]) ? 0 : 1;
+    x[i++] = newtons_method_as(a, b, c, d, xc[closer], closer == 0 ? 1 : -1);
+  }
+  if (yc[1] * a <= 0) x[i++] = newtons_method_as(a, b, c, d, xc[1], 1);
+  return i;
+}
+
+// ArcSim's kDOP-18 box test (single precision) between the swept vertex and
+// the swept face (both end points of all three corners), margin `thick`.
+static bool kdop18_overlap_pts(const double* const* A, int na, const double* const* Bp, int nb, double thick) {
+  auto dists = [](const double* p, float* d) {
+    float x = (float)p[0], y = (float)p[1], z = (float)p[2];
+    d[0] = x; d[1] = y; d[2] = z;
+    d[3] = x + y; d[4] = x + z; d[5] = y + z;
+    d[6] = x - y; d[7] = x - z; d[8] = y - z;
+  };
+  float amin[9], amax[9], bmin[9], bmax[9], q[9];
+  for (int i = 0; i < 9; ++i) {
+    amin[i] = bmin[i] = 3.0e38f;
+    amax[i] = bmax[i] = -3.0e38f;
+  }
+  for (int k = 0; k < na; ++k) {
+    dists(A[k], q);
+    for (int i = 0; i < 9; ++i) { amin[i] = std::min(amin[i], q[i]); amax[i] = std::max(amax[i], q[i]); }
+  }
+  for (int k = 0; k < nb; ++k) {
+    dists(Bp[k], q);
+    for (int i = 0; i < 9; ++i) { bmin[i] = std::min(bmin[i], q[i]); bmax[i] = std::max(bmax[i], q[i]); }
+  }
+  double d = (double)(float)thick;
+  for (int i = 0; i < 9; ++i) {
+    double m = i < 3 ? d : std::sqrt(2.0) * d;
+    float lo = (float)(bmin[i] - m), hi = (float)(bmax[i] + m);
+    if (amin[i] > hi || amax[i] < lo) return false;
+  }
+  return true;
+}
+
+struct SelfImpact { int a, f[3]; double t, d, n[3]; };
+
+// Moving vertex vs moving face (collision.cpp:127-278), mult = 2 as
+// find_face_impacts uses for self contact. p0/p1: vertex at step start / now;
+// y0[k]/y1[k]: face corner k at step start / now.
+bool vf_moving(const double* p0, const double* p1, const double* const* y0, const double* const* y1, SelfImpact& im) {
+  const double mult = 2.0, thk = kThickness * mult;
+  const double* A[2] = {p1, p0};
+  const double* Bp[6] = {y1[0], y0[0], y1[1], y0[1], y1[2], y0[2]};
+  if (!kdop18_overlap_pts(A, 2, Bp, 6, kThickness * mult)) return false;
+  double v0[3], x[3][3], v[3][3];
+  for (int r = 0; r < 3; ++r) {
+    v0[r] = p1[r] - p0[r];
+    for (int k = 0; k < 3; ++k) {
+      x[k][r] = y0[k][r] - p0[r];
+      v[k][r] = (y1[k][r] - y0[k][r]) - v0[r];
+    }
+  }
+  double a0 = triple(x[0], x[1], x[2]);
+  double a1 = triple(v[0], x[1], x[2]) + triple(x[0], v[1], x[2]) + triple(x[0], x[1], v[2]);
+  double a2 = triple(x[0], v[1], v[2]) + triple(v[0], x[1], v[2]) + triple(v[0], v[1], x[2]);
+  double a3 = triple(v[0], v[1], v[2]);
+  double ts[10];
+  int nsol = solve_cubic_as(a3, a2, a1, a0, ts);
+  ts[nsol++] = 1;
+  for (int i = 0; i < nsol; i++) {
+    double t = ts[i];
+    if (t < 0 || t > 1) continue;
+    double P[3], Y[3][3];
+    for (int r = 0; r < 3; ++r) {
+      P[r] = p0[r] + t * (p1[r] - p0[r]);
+      for (int k = 0; k < 3; ++k) Y[k][r] = y0[k][r] + t * (y1[k][r] - y0[k][r]);
+    }
+    // signed_vf_distance
+    double e1[3], e2[3];
+    for (int r = 0; r < 3; ++r) { e1[r] = Y[1][r] - Y[0][r]; e2[r] = Y[2][r] - Y[0][r]; }
+    double l1 = std::sqrt(e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2]);
+    double l2 = std::sqrt(e2[0] * e2[0] + e2[1] * e2[1] + e2[2] * e2[2]);
+    for (int r = 0; r < 3; ++r) { e1[r] /= l1; e2[r] /= l2; }
+    double nn[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+    double n2 = nn[0] * nn[0] + nn[1] * nn[1] + nn[2] * nn[2];
+    if (!(n2 >= 1e-6)) continue;  // infinite distance: never a hit
+    double nl = std::sqrt(n2);
+    for (int r = 0; r < 3; ++r) nn[r] /= nl;
+    double h = 0;
+    for (int r = 0; r < 3; ++r) h += (P[r] - Y[0][r]) * nn[r];
+    double q[3][3];
+    for (int k = 0; k < 3; ++k)
+      for (int r = 0; r < 3; ++r) q[k][r] = Y[k][r] - P[r];
+    double b0 = triple(q[1], q[2], nn), b1 = triple(q[2], q[0], nn), b2 = triple(q[0], q[1], nn);
+    double w[3] = {-b0 / (b0 + b1 + b2), -b1 / (b0 + b1 + b2), -b2 / (b0 + b1 + b2)};
+    bool inside = std::min(std::min(-w[0], -w[1]), -w[2]) >= -1e-6;
+    if (t == 1) {
+      for (int k = 0; k < 3; ++k) {
+        double dv[3] = {P[0] - Y[k][0], P[1] - Y[k][1], P[2] - Y[k][2]};
+        inside = inside || std::sqrt(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]) < thk;
+      }
+      for (int k = 0; k < 3; ++k) {
+        const double *Aa = Y[k], *Bb = Y[(k + 1) % 3];
+        double pa[3], ab[3];
+        for (int r = 0; r < 3; ++r) { pa[r] = P[r] - Aa[r]; ab[r] = Bb[r] - Aa[r]; }
+        double dp = pa[0] * ab[0] + pa[1] * ab[1] + pa[2] * ab[2], ll = ab[0] * ab[0] + ab[1] * ab[1] + ab[2] * ab[2];
+        double e[3];
+        for (int r = 0; r < 3; ++r) e[r] = pa[r] - dp * ab[r] / ll;
+        inside = inside || std::sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]) < thk;
+      }
+    }
+    double rv[3];
+    for (int r = 0; r < 3; ++r) rv[r] = w[0] * v[0][r] + w[1] * v[1][r] + w[2] * v[2][r];
+    // normal from the step-start corners, turned against the relative motion
+    double s1[3], s2[3];
+    for (int r = 0; r < 3; ++r) { s1[r] = y0[1][r] - y0[0][r]; s2[r] = y0[2][r] - y0[0][r]; }
+    double m1 = std::sqrt(s1[0] * s1[0] + s1[1] * s1[1] + s1[2] * s1[2]);
+    double m2 = std::sqrt(s2[0] * s2[0] + s2[1] * s2[1] + s2[2] * s2[2]);
+    for (int r = 0; r < 3; ++r) { s1[r] /= m1; s2[r] /= m2; }
+    double c[3] = {s1[1] * s2[2] - s1[2] * s2[1], s1[2] * s2[0] - s1[0] * s2[2], s1[0] * s2[1] - s1[1] * s2[0]};
+    double sg = (nn[0] * rv[0] + nn[1] * rv[1] + nn[2] * rv[2]) > 0 ? -1.0 : 1.0;
+    if (std::fabs(h) < thk && inside) {
+      im.t = t;
+      im.d = std::fabs(h);
+      for (int r = 0; r < 3; ++r) im.n[r] = sg * c[r];
+      return true;
+    }
+  }
+  return false;
+}
+
+// Stick/slide friction among a self-contact group (World::do_solve_friction,
+// World.h:1246-1315), in the group's local basis.
+void solve_group_friction(const double* p0, double m0, double* r0, const std::vector<std::array<double, 3>>& ps,
+                          const std::vector<double>& ms, std::vector<std::array<double, 3>>& rs,
+                          const std::vector<char>& stick, double mu) {
+  double pst[3] = {p0[0], p0[1], p0[2]}, mst = m0;
+  int n = (int)stick.size();
+  for (int k = 0; k < n; ++k)
+    if (stick[k]) {
+      for (int r = 0; r < 3; ++r) pst[r] += ps[k][r];
+      mst += ms[k];
+    }
+  double psl[3] = {0, 0, 0}, msl = 0, fn = 0;
+  for (int k = 0; k < n; ++k)
+    if (!stick[k]) {
+      for (int r = 0; r < 3; ++r) psl[r] += ps[k][r];
+      msl += ms[k];
+      fn += rs[k][0];
+    }
+  double v[3] = {pst[0] / mst, pst[1] / mst, pst[2] / mst};
+  r0[1] = v[1] * m0 - p0[1];
+  r0[2] = v[2] * m0 - p0[2];
+  for (int k = 0; k < n; ++k)
+    if (stick[k]) {
+      rs[k][1] = v[1] * ms[k] - ps[k][1];
+      rs[k][2] = v[2] * ms[k] - ps[k][2];
+    }
+  if (msl > 0) {
+    double rel[3];
+    for (int r = 0; r < 3; ++r) rel[r] = psl[r] / msl - pst[r] / mst;
+    double tl = std::sqrt(rel[1] * rel[1] + rel[2] * rel[2]);
+    for (int r = 0; r < 3; ++r) rel[r] /= tl;
+    double mult = mu * fn / (msl + mst);
+    r0[1] += rel[1] * mult * m0;
+    r0[2] += rel[2] * mult * m0;
+    for (int k = 0; k < n; ++k)
+      if (stick[k]) {
+        rs[k][1] += rel[1] * mult * ms[k];
+        rs[k][2] += rel[2] * mult * ms[k];
+      }
+    double av[3] = {(p0[0] + r0[0]) / m0, (p0[1] + r0[1]) / m0, (p0[2] + r0[2]) / m0};
+    for (int k = 0; k < n; ++k)
+      if (!stick[k]) {
+        double rv[3];
+        for (int r = 0; r < 3; ++r) rv[r] = ps[k][r] / ms[k] - av[r];
+        double t2 = std::sqrt(rv[1] * rv[1] + rv[2] * rv[2]);
+        for (int r = 0; r < 3; ++r) rv[r] /= t2;
+        double m = -mu * rs[k][0] * ms[k] / (ms[k] + mst);
+        rs[k][1] = m * rv[1];
+        rs[k][2] = m * rv[2];
+      }
+  }
+}
+
+struct SelfGroup {
+  int f[3];               // face, contact indices
+  double n[3];            // normal of its first vertex's impact
+  std::vector<int> verts; // contact indices, ascending
+};
+
+// getSelfCollisionsInfo (collisionBrutal.h:643-700): per vertex the closest
+// impact of the list (smaller t; among t = 1, smaller d), then vertices
+// grouped by the face they hit, in ascending vertex order.
+std::vector<SelfGroup> self_groups(const std::vector<SelfImpact>& imps) {
+  std::map<int, const SelfImpact*> closest;
+  for (const SelfImpact& im : imps) {
+    auto it = closest.find(im.a);
+    if (it == closest.end()) {
+      closest[im.a] = &im;
+      continue;
+    }
+    const SelfImpact* old = it->second;
+    if (im.t < old->t) {
+      it->second = &im;
+      continue;
+    }
+    if (old->t == 1 && old->d > im.d) it->second = &im;
+  }
+  std::vector<SelfGroup> out;
+  for (auto& kv : closest) {
+    const SelfImpact* im = kv.second;
+    size_t g = 0;
+    for (; g < out.size(); ++g)
+      if (out[g].f[0] == im->f[0] && out[g].f[1] == im->f[1] && out[g].f[2] == im->f[2]) break;
+    if (g < out.size()) {
+      out[g].verts.push_back(kv.first);
+    } else {
+      SelfGroup sg;
+      for (int k = 0; k < 3; ++k) sg.f[k] = im->f[k];
+      for (int r = 0; r < 3; ++r) sg.n[r] = im->n[r];
+      sg.verts.push_back(kv.first);
+      out.push_back(sg);
+    }
+  }
+  return out;
+}
+
 void contact_basis(const double* n, double* B) {
   double t[3] = {0, n[2], -n[1]};  // n x (1,0,0)
   double tn = std::sqrt(t[1] * t[1] + t[2] * t[2]);
@@ -1079,9 +1355,16 @@ void step(Scene& s, double* x, double* v, const double* act, StepTape* tape, int
   if (it_pd) *it_pd = iters;
   std::vector<double> ori_x = y;
   int cit = 0, outer_n = 0;
-  if (s.contact && !s.contact_vtx.empty() && (!s.planes.empty() || !s.spheres.empty() || !s.tris.empty())) {
+  const bool selfon = s.self_collision && !s.sc_face.empty();
+  if (s.contact && !s.contact_vtx.empty() &&
+      (!s.planes.empty() || !s.spheres.empty() || !s.tris.empty() || selfon)) {
     // Contact phase, World.h:921-1023
     std::vector<Impact> impacts;
+    std::vector<SelfImpact> simps;
+    std::vector<SelfGroup> groups;
+    // self contact: the reactions of the previous handling feed the next one
+    // (double buffers of World.h:1446-1600, cleared at the step start)
+    std::vector<double> self_cur(n3, 0.0), self_next(n3, 0.0), con_next(n3, 0.0);
     std::vector<double> xit = y, oold(n3, 0.0), prev(n3, 0.0), rc(n3);
     bool last_begin = true;
     while (true) {
@@ -1113,7 +1396,40 @@ void step(Scene& s, double* x, double* v, const double* act, StepTape* tape, int
           }
         if (!found) impacts.push_back(f);
       }
-      if (fresh.empty()) break;
+      size_t ngroups_before = groups.size();
+      if (selfon) {
+        // vertex vs surface face of the body mesh (collision.cpp:57-64, 127-138)
+        std::vector<SelfImpact> sfresh;
+        int nsc = (int)s.sc_vtx.size();
+        for (int a = 0; a < nsc; ++a) {
+          int va = s.sc_vtx[a];
+          for (const auto& f : s.sc_face) {
+            if (f[0] == a || f[1] == a || f[2] == a) continue;
+            const double* y0[3] = {&xn[3 * s.sc_vtx[f[0]]], &xn[3 * s.sc_vtx[f[1]]], &xn[3 * s.sc_vtx[f[2]]]};
+            const double* y1[3] = {&ori_x[3 * s.sc_vtx[f[0]]], &ori_x[3 * s.sc_vtx[f[1]]], &ori_x[3 * s.sc_vtx[f[2]]]};
+            SelfImpact im;
+            if (vf_moving(&xn[3 * va], &ori_x[3 * va], y0, y1, im)) {
+              im.a = a;
+              for (int k = 0; k < 3; ++k) im.f[k] = f[k];
+              sfresh.push_back(im);
+            }
+          }
+        }
+        for (auto& f : sfresh) {  // add-or-replace by the four node ids (collisionBrutal.h:291-306)
+          bool found = false;
+          for (auto& im : simps)
+            if (im.a == f.a && im.f[0] == f.f[0] && im.f[1] == f.f[1] && im.f[2] == f.f[2]) {
+              im = f;
+              found = true;
+              break;
+            }
+          if (!found) simps.push_back(f);
+        }
+        groups = self_groups(simps);
+      }
+      // BaseUpdateCollisions: done when no obstacle impact is new and the
+      // number of self-contact groups did not change (collisionBrutal.h:404)
+      if (fresh.empty() && ngroups_before == groups.size()) break;
       if (outer_n == s.outer_cap) break;
       ++outer_n;
       if (tape) tape->contact_active = true;
@@ -1134,9 +1450,93 @@ void step(Scene& s, double* x, double* v, const double* act, StepTape* tape, int
         for (int k = 0; k < n3; ++k) rc[k] *= s.h;
         if (i == 0) prev = rc;
         std::vector<double> orig = rc;
-        for (size_t k = 0; k < impacts.size(); ++k) {
-          int vtx = impacts[k].vtx;
-          project_contact(&Bs[9 * k], s.mu, &orig[3 * vtx], &rc[3 * vtx]);
+        const bool have_self = selfon && !groups.empty();
+        if (!impacts.empty() || have_self) {  // CollisionHandling, World.h:1317-1600
+          for (size_t k = 0; k < impacts.size(); ++k) {
+            int vtx = impacts[k].vtx;
+            double fin[3], out[3];
+            for (int r = 0; r < 3; ++r) fin[r] = orig[3 * vtx + r] + (selfon ? self_cur[3 * vtx + r] : 0.0);
+            const double* B = &Bs[9 * k];
+            if (B[0] * fin[0] + B[3] * fin[1] + B[6] * fin[2] > 0.0) continue;  // take-off
+            project_contact(B, s.mu, fin, out);
+            for (int r = 0; r < 3; ++r) {
+              if (!have_self) rc[3 * vtx + r] = out[r];
+              if (selfon) con_next[3 * vtx + r] = out[r];
+            }
+          }
+          if (have_self) {
+            for (const SelfGroup& g : groups) {
+              double B[9];
+              contact_basis(g.n, B);
+              auto toloc = [&](const double* w, double* l) {
+                for (int c = 0; c < 3; ++c) l[c] = B[0 * 3 + c] * w[0] + B[1 * 3 + c] * w[1] + B[2 * 3 + c] * w[2];
+              };
+              auto toworld = [&](const double* l, double* w) {
+                for (int r = 0; r < 3; ++r) w[r] = B[r * 3 + 0] * l[0] + B[r * 3 + 1] * l[1] + B[r * 3 + 2] * l[2];
+              };
+              int fv[3] = {s.sc_vtx[g.f[0]], s.sc_vtx[g.f[1]], s.sc_vtx[g.f[2]]};
+              double m0 = s.Kdiag[fv[0]] + s.Kdiag[fv[1]] + s.Kdiag[fv[2]];
+              double f0w[3] = {0, 0, 0}, f0[3], r0[3] = {0, 0, 0};
+              for (int q = 0; q < 3; ++q)
+                for (int r = 0; r < 3; ++r) f0w[r] += orig[3 * fv[q] + r] + con_next[3 * fv[q] + r];
+              toloc(f0w, f0);
+              std::vector<int> vl;
+              for (int k : g.verts) {
+                int vk = s.sc_vtx[k];
+                double fw[3], fl[3];
+                for (int r = 0; r < 3; ++r) fw[r] = orig[3 * vk + r] + con_next[3 * vk + r];
+                toloc(fw, fl);
+                if (fl[0] / s.Kdiag[vk] - f0[0] / m0 <= 0) vl.push_back(vk);
+              }
+              int nv = (int)vl.size();
+              std::vector<double> ms(nv);
+              std::vector<std::array<double, 3>> fs(nv), rs(nv);
+              double sm = 0, smv = 0;
+              for (int k = 0; k < nv; ++k) {
+                ms[k] = s.Kdiag[vl[k]];
+                sm += ms[k];
+                double fw[3];
+                for (int r = 0; r < 3; ++r) fw[r] = orig[3 * vl[k] + r] + con_next[3 * vl[k] + r];
+                toloc(fw, fs[k].data());
+                smv += fs[k][0];
+              }
+              for (int k = 0; k < nv; ++k) {
+                double fn = (f0[0] + smv) * ms[k] / (m0 + sm) - fs[k][0];
+                rs[k] = {fn, 0.0, 0.0};
+                r0[0] -= fn;
+              }
+              std::vector<char> stick(nv, 1);
+              while (true) {
+                solve_group_friction(f0, m0, r0, fs, ms, rs, stick, s.self_mu);
+                bool changed = false;
+                for (int k = 0; k < nv; ++k)
+                  if (stick[k] && rs[k][1] * rs[k][1] + rs[k][2] * rs[k][2] > s.self_mu * rs[k][0] * rs[k][0]) {
+                    changed = true;
+                    stick[k] = 0;
+                  }
+                if (!changed) break;
+              }
+              double r0w[3], bf0[3], v0[3];
+              toworld(r0, r0w);
+              toworld(f0, bf0);
+              for (int r = 0; r < 3; ++r) v0[r] = (bf0[r] + r0w[r]) / m0;
+              for (int q = 0; q < 3; ++q)
+                for (int r = 0; r < 3; ++r)
+                  self_next[3 * fv[q] + r] += v0[r] * s.Kdiag[fv[q]] - (orig[3 * fv[q] + r] + con_next[3 * fv[q] + r]);
+              for (int k = 0; k < nv; ++k) {
+                double w[3];
+                toworld(rs[k].data(), w);
+                for (int r = 0; r < 3; ++r) self_next[3 * vl[k] + r] += w[r];
+              }
+            }
+            for (int k = 0; k < n3; ++k) rc[k] += con_next[k] + self_next[k];
+            if (tape) tape->self_active = true;
+          }
+          if (selfon) {
+            self_cur.swap(self_next);
+            std::fill(self_next.begin(), self_next.end(), 0.0);
+            std::fill(con_next.begin(), con_next.end(), 0.0);
+          }
         }
         for (int k = 0; k < n3; ++k) prev[k] = rc[k] * s.alpha + prev[k] * (1.0 - s.alpha);
         for (int vtx = 0; vtx < N; ++vtx)
@@ -1628,6 +2028,111 @@ void orc_set_tris(void* p, int n, const double* tris) {
 int orc_detect_tri(const double* tri, const double* x0, const double* x1, double* n) {
   return detect_tri(tri, x0, x1, n) ? 1 : 0;
 }
+// Self contact: builds the surface mesh in the reference's contact numbering
+// (Dfobj.h:497-566, World.h:540-558).
+void orc_set_self_collision(void* p, int on, double mu) {
+  Scene* s = (Scene*)p;
+  s->self_collision = on;
+  s->self_mu = mu;
+  s->sc_vtx.clear();
+  s->sc_face.clear();
+  if (!on) return;
+  std::vector<std::array<int, 3>> tri;
+  for (const Tet& t : s->tets) {
+    int sv[4] = {t.v[0], t.v[1], t.v[2], t.v[3]};
+    std::sort(sv, sv + 4);
+    tri.push_back({sv[0], sv[1], sv[2]});
+    tri.push_back({sv[0], sv[1], sv[3]});
+    tri.push_back({sv[0], sv[2], sv[3]});
+    tri.push_back({sv[1], sv[2], sv[3]});
+  }
+  std::map<std::array<int, 3>, int> cnt;
+  for (auto& f : tri) cnt[f]++;
+  std::map<int, int> cidx;
+  for (auto& f : tri) {
+    if (cnt[f] != 1) continue;
+    std::array<int, 3> cf;
+    for (int k = 0; k < 3; ++k) {
+      auto it = cidx.find(f[k]);
+      if (it == cidx.end()) {
+        it = cidx.emplace(f[k], (int)s->sc_vtx.size()).first;
+        s->sc_vtx.push_back(f[k]);
+      }
+      cf[k] = it->second;
+    }
+    s->sc_face.push_back(cf);
+  }
+}
+int orc_solve_cubic(double a, double b, double c, double d, double* x) { return solve_cubic_as(a, b, c, d, x); }
+int orc_vf_moving(const double* p0, const double* p1, const double* y0, const double* y1, double* t, double* d,
+                  double* n) {
+  const double* a0[3] = {y0, y0 + 3, y0 + 6};
+  const double* a1[3] = {y1, y1 + 3, y1 + 6};
+  SelfImpact im;
+  if (!vf_moving(p0, p1, a0, a1, im)) return 0;
+  *t = im.t;
+  *d = im.d;
+  for (int r = 0; r < 3; ++r) n[r] = im.n[r];
+  return 1;
+}
+// One self-contact group's response (World.h:1451-1530): face forces f3 (3x3)
+// and masses m3, members' forces fv (nv x 3) and masses mv, normal n; outputs
+// the face corners' and the members' self-contact forces (world frame).
+void orc_self_group(const double* n, const double* f3, const double* m3, int nv, const double* fv, const double* mv,
+                    double mu, double* out3, double* outv) {
+  double B[9];
+  contact_basis(n, B);
+  auto toloc = [&](const double* w, double* l) {
+    for (int c = 0; c < 3; ++c) l[c] = B[0 * 3 + c] * w[0] + B[1 * 3 + c] * w[1] + B[2 * 3 + c] * w[2];
+  };
+  auto toworld = [&](const double* l, double* w) {
+    for (int r = 0; r < 3; ++r) w[r] = B[r * 3 + 0] * l[0] + B[r * 3 + 1] * l[1] + B[r * 3 + 2] * l[2];
+  };
+  double m0 = m3[0] + m3[1] + m3[2], f0w[3] = {0, 0, 0}, f0[3], r0[3] = {0, 0, 0};
+  for (int q = 0; q < 3; ++q)
+    for (int r = 0; r < 3; ++r) f0w[r] += f3[3 * q + r];
+  toloc(f0w, f0);
+  std::vector<int> vl;
+  for (int k = 0; k < nv; ++k) {
+    double fl[3];
+    toloc(fv + 3 * k, fl);
+    if (fl[0] / mv[k] - f0[0] / m0 <= 0) vl.push_back(k);
+  }
+  int m = (int)vl.size();
+  std::vector<double> ms(m);
+  std::vector<std::array<double, 3>> fs(m), rs(m);
+  double sm = 0, smv = 0;
+  for (int k = 0; k < m; ++k) {
+    ms[k] = mv[vl[k]];
+    sm += ms[k];
+    toloc(fv + 3 * vl[k], fs[k].data());
+    smv += fs[k][0];
+  }
+  for (int k = 0; k < m; ++k) {
+    double fn = (f0[0] + smv) * ms[k] / (m0 + sm) - fs[k][0];
+    rs[k] = {fn, 0.0, 0.0};
+    r0[0] -= fn;
+  }
+  std::vector<char> stick(m, 1);
+  while (true) {
+    solve_group_friction(f0, m0, r0, fs, ms, rs, stick, mu);
+    bool changed = false;
+    for (int k = 0; k < m; ++k)
+      if (stick[k] && rs[k][1] * rs[k][1] + rs[k][2] * rs[k][2] > mu * rs[k][0] * rs[k][0]) {
+        changed = true;
+        stick[k] = 0;
+      }
+    if (!changed) break;
+  }
+  double r0w[3], bf0[3], v0[3];
+  toworld(r0, r0w);
+  toworld(f0, bf0);
+  for (int r = 0; r < 3; ++r) v0[r] = (bf0[r] + r0w[r]) / m0;
+  for (int q = 0; q < 3; ++q)
+    for (int r = 0; r < 3; ++r) out3[3 * q + r] = v0[r] * m3[q] - f3[3 * q + r];
+  std::fill(outv, outv + 3 * nv, 0.0);
+  for (int k = 0; k < m; ++k) toworld(rs[k].data(), outv + 3 * vl[k]);
+}
 void orc_set_underwater(void* p, int on) {
   Scene* s = (Scene*)p;
   s->underwater = on;
@@ -1838,6 +2343,11 @@ int orc_backward(void* p, const double* gxT, const double* gvT, double* gx0, dou
   Scene* s = (Scene*)p;
   int n3 = 3 * s->N;
   int steps = (int)s->tape.size();
+  for (const StepTape& tp : s->tape)
+    if (tp.self_active) {
+      g_err = "the reverse sweep does not cover self contact";
+      return 1;
+    }
   std::vector<double> gx(gxT, gxT + n3), gv(gvT, gvT + n3);
   size_t T = s->tets.size();
   if (gE) std::fill(gE, gE + T, 0.0);

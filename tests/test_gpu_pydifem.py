@@ -227,3 +227,52 @@ def test_baymax_falls_onto_its_obstacle(oracle_mod):
     assert its[:, 1].sum() > 0
     assert rel(qT, xo) < 1e-9 and rel(qdT, vo) < 1e-9
     assert np.asarray(qT).reshape(-1, 3)[:, 1].min() > -0.321262 - 0.01
+
+
+def _two_boxes(mu=0.5):
+    xa, ta = scene.box_grid(3, 3, 1)
+    xb, tb = scene.box_grid(2, 2, 1)
+    xb = xb + np.array([0.02, 0.03, 0.054])
+    da = scene.SceneDesc(x_rest=xa, tets=ta, total_mass=1.0, E=np.full(ta.shape[0], 1e5),
+                         attach_idx=np.arange(xa.shape[0], dtype=np.int32), attach_w=1e8)
+    db = scene.SceneDesc(x_rest=xb, tets=tb, total_mass=0.5, E=np.full(tb.shape[0], 1e5))
+    d = scene.pack_scenes([da, db], translate=False)
+    d.h, d.max_iter, d.gravity, d.contact, d.max_iter1 = 0.01, 10, (0.0, 0.0, -9.81), True, 10
+    d.self_collision, d.self_mu = True, mu
+    return d, xa.shape[0]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_self_contact_batch(oracle_mod, precision):
+    """Self contact between two bodies of one world: a box dropped onto a
+    pinned box, three rollouts with different velocities. Forward against the
+    oracle (same contact passes and Jacobi iterations in FP64); the reverse
+    sweep is refused."""
+    from diff_fem_amd import engine as eng
+    d, na = _two_boxes()
+    prec = eng.PDM_FP64 if precision == "fp64" else eng.PDM_MIXED
+    tol = 1e-9 if precision == "fp64" else 1e-5
+    R, steps = 3, 20
+    s = eng.Scene(d, R=R, precision=prec, flags=0, checkpoint_every=8)
+    assert s.info()["n_self_faces"] > 0
+    o = oracle_mod.OracleScene(d)
+    x0 = np.tile(d.x_rest.ravel(), (R, 1))
+    v0 = np.zeros((R, 3 * d.N))
+    for r, u in enumerate(([0.0, 0.0, -0.6], [0.3, -0.2, -0.8], [-0.1, 0.4, -0.4])):
+        v0[r, 3 * na:] = np.tile(u, d.N - na)
+    xT, vT = s.forward_host(x0, v0, steps, record=True)
+    passes, jac = s.contact_stats()
+    for r in range(R):
+        xo, vo, its = o.rollout(x0[r], v0[r], steps)
+        if precision == "fp64":
+            assert np.array_equal(jac[r], its[:, 1]), r
+        # velocities v = (x_{n+1} - x_n)/h of a box come to rest: their error is
+        # measured against the rollout's speed scale (max of start and end)
+        vscale = max(np.abs(vo).max(), np.abs(v0[r]).max())
+        assert rel(xT[r], xo) < tol, r
+        if precision == "fp64":            # the mixed bar is stated on positions
+            assert np.abs(vT[r] - vo).max() / vscale < tol, r
+        assert xo.reshape(-1, 3)[na:, 2].min() > 0.03           # the top box rests on the bottom one
+    assert jac.sum() > 0
+    with pytest.raises(RuntimeError, match="self contact"):
+        s.backward_host(np.ones((R, 3 * d.N)), np.zeros((R, 3 * d.N)))

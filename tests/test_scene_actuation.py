@@ -85,7 +85,6 @@ def test_two_meshes_are_two_bodies(tmp_path):
 
 
 @pytest.mark.parametrize("para,mesh,extra,err", [
-    ({"Self-collision": True}, {}, None, NotImplementedError),
     ({}, {}, {"Clothes": [{"Obj filename": "box.obj"}]}, NotImplementedError),
     ({}, {"Bones": [{"Parent ID": -1}]}, None, NotImplementedError),
     ({}, {"Volume Verts": [[0, 1]]}, None, NotImplementedError),

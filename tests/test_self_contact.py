@@ -1,0 +1,170 @@
+"""Self contact in the oracle ("Self-collision": surface vertices against the
+body's own surface faces; collisionBrutal.h:281-406, 643-700; World.h:1246-1315,
+1446-1600), on the CPU.
+
+Pins: ArcSim's cubic solver against numpy's roots; the moving vertex-face test
+is Galilean invariant and agrees with the static-face test; one group's
+response balances its normal reactions (and all of them when everything
+sticks), gives a stuck vertex the face's common velocity and a sliding one
+the reference's mass-weighted Coulomb friction; two stacked boxes do not pass through each
+other with self contact and do without; the reverse sweep refuses self
+contact; the loader reads "Self-collision" and "World Friction".
+"""
+import json
+
+import numpy as np
+import pytest
+
+from diff_fem_amd import scene
+
+
+def test_cubic_solver_matches_numpy(oracle_mod):
+    rng = np.random.default_rng(2)
+    for _ in range(200):
+        r = np.sort(rng.uniform(0, 1, 3))
+        if np.min(np.diff(r)) < 0.05:
+            continue
+        a = rng.uniform(0.5, 3) * rng.choice([-1, 1])
+        co = a * np.poly(r)
+        x = oracle_mod.solve_cubic(*co)
+        assert x.size == 3
+        np.testing.assert_allclose(x, r, atol=1e-6)
+    # leading coefficient zero: the quadratic's roots
+    x = oracle_mod.solve_cubic(0.0, 2.0, -3.0, 1.0)
+    np.testing.assert_allclose(np.sort(x), [0.5, 1.0], atol=1e-12)
+
+
+def _face():
+    return np.array([[0.0, 0.0, 0.0], [0.1, 0.0, 0.0], [0.0, 0.1, 0.0]])
+
+
+def test_vf_moving_is_galilean_invariant(oracle_mod):
+    """A vertex moving by d past a static face, and a static vertex with the
+    face moving by -d: the same coplanarity time and normal."""
+    rng = np.random.default_rng(3)
+    hits = 0
+    for _ in range(50):
+        p0 = np.array([0.03, 0.02, 0.004]) + 0.003 * rng.standard_normal(3)
+        dvec = np.array([0.0, 0.0, -0.01]) + 0.002 * rng.standard_normal(3)
+        y = _face()
+        h1, t1, d1, n1 = oracle_mod.vf_moving(p0, p0 + dvec, y, y)
+        h2, t2, d2, n2 = oracle_mod.vf_moving(p0, p0, y, y - dvec)
+        assert h1 == h2
+        if h1:
+            hits += 1
+            # (y - d) - y differs from -d by rounding: the cubic and quadratic
+            # terms are rounding noise instead of zero, Newton starts elsewhere,
+            # and its 1e-6 stopping rule returns the previous iterate
+            assert abs(t1 - t2) < 1e-6 and abs(d1 - d2) < 1e-8
+            np.testing.assert_allclose(n1, n2, atol=1e-12)
+    assert hits > 10
+
+
+def test_vf_moving_on_a_static_face_agrees_with_the_obstacle_test(oracle_mod):
+    y = _face()
+    for p0, p1 in [([0.03, 0.02, 0.01], [0.03, 0.02, -0.01]), ([0.02, 0.05, -0.01], [0.02, 0.05, 0.01])]:
+        h, t, d, n = oracle_mod.vf_moving(np.array(p0), np.array(p1), y, y)
+        ht, nt = oracle_mod.detect_tri(y.ravel(), p0, p1)
+        assert h and ht and abs(t - 0.5) < 1e-12
+        np.testing.assert_allclose(n, nt, atol=1e-15)
+    assert not oracle_mod.vf_moving(np.array([0.2, 0.2, 0.01]), np.array([0.2, 0.2, -0.01]), y, y)[0]
+
+
+def _group(rng, nv):
+    n = np.array([0.0, 0.0, 1.0]) + 0.1 * rng.standard_normal(3)
+    m3, mv = rng.uniform(1, 3, 3), rng.uniform(1, 3, nv)
+    f3 = rng.standard_normal((3, 3))
+    fv = rng.standard_normal((nv, 3)) - 6.0 * n / np.linalg.norm(n)   # members pushed into the face
+    return n, f3, m3, fv, mv
+
+
+def test_group_response_balances(oracle_mod):
+    """The normal reactions always cancel (fn for each member, -sum fn on the
+    face). The tangential ones cancel when every member sticks; with mixed
+    stick and slide the reference's friction rule (World.h:1280-1309) does
+    not balance them (its check is an assert compiled out of its build)."""
+    rng = np.random.default_rng(4)
+    for nv in (1, 2, 4):
+        for mu in (0.0, 0.3, 1.0, 1e6):
+            n, f3, m3, fv, mv = _group(rng, nv)
+            o3, ov = oracle_mod.self_group(n, f3, m3, fv, mv, mu)
+            tot = o3.sum(0) + ov.sum(0)
+            scale = max(1.0, np.abs(f3).max(), np.abs(fv).max())
+            assert abs(tot @ (n / np.linalg.norm(n))) < 1e-12 * scale
+            if mu >= 1e6:                      # everything sticks
+                assert np.abs(tot).max() < 1e-12 * scale
+
+
+def test_stuck_member_takes_the_common_velocity(oracle_mod):
+    """One approaching member with little tangential force sticks: face
+    corners and member end with the same 'velocity' (force / mass)."""
+    n = np.array([0.0, 0.0, 1.0])
+    f3 = np.array([[0.0, 0.0, 1.0], [0.1, 0.0, 1.0], [0.0, 0.1, 1.0]])
+    m3 = np.array([1.0, 2.0, 1.5])
+    fv = np.array([[0.05, 0.0, -5.0]])
+    mv = np.array([1.2])
+    o3, ov = oracle_mod.self_group(n, f3, m3, fv, mv, 1.0)
+    v = (fv[0] + ov[0]) / mv[0]
+    for q in range(3):
+        np.testing.assert_allclose((f3[q] + o3[q]) / m3[q], v, atol=1e-13)
+    np.testing.assert_allclose(v, (f3.sum(0) + fv[0]) / (m3.sum() + mv[0]), atol=1e-13)
+
+
+def test_sliding_member_feels_coulomb_friction(oracle_mod):
+    """A member with a large tangential force slides: its tangential reaction
+    opposes the slide with magnitude mu fn m / (m + m_stick), the reference's
+    mass-weighted share of the Coulomb bound (World.h:1303-1307); here only
+    the face sticks (m_stick = 3)."""
+    n = np.array([0.0, 0.0, 1.0])
+    f3 = np.zeros((3, 3))
+    m3 = np.array([1.0, 1.0, 1.0])
+    fv = np.array([[30.0, 0.0, -1.0]])
+    mv = np.array([1.0])
+    mu = 0.2
+    o3, ov = oracle_mod.self_group(n, f3, m3, fv, mv, mu)
+    assert abs(np.hypot(ov[0, 0], ov[0, 1]) - mu * ov[0, 2] * mv[0] / (mv[0] + m3.sum())) < 1e-12 and ov[0, 2] > 0
+    fn = fv[0, 2] * mv[0] / (mv[0] + m3.sum()) - fv[0, 2]     # normal reaction (face force 0)
+    assert abs(ov[0, 2] - fn) < 1e-12
+    assert ov[0, 0] < 0                         # opposes the slide
+
+
+def two_boxes(self_on, gap=0.004, mu=0.5):
+    xa, ta = scene.box_grid(3, 3, 1)
+    xb, tb = scene.box_grid(2, 2, 1)
+    xb = xb + np.array([0.02, 0.03, 0.05 + gap])
+    da = scene.SceneDesc(x_rest=xa, tets=ta, total_mass=1.0, E=np.full(ta.shape[0], 1e5),
+                         attach_idx=np.arange(xa.shape[0], dtype=np.int32), attach_w=1e8)
+    db = scene.SceneDesc(x_rest=xb, tets=tb, total_mass=0.5, E=np.full(tb.shape[0], 1e5))
+    d = scene.pack_scenes([da, db], translate=False)
+    d.h, d.max_iter, d.gravity, d.contact, d.max_iter1 = 0.01, 10, (0.0, 0.0, -9.81), True, 10
+    d.self_collision, d.self_mu = self_on, mu
+    return d, xa.shape[0]
+
+
+def test_stacked_boxes_do_not_pass_through(oracle_mod):
+    res = {}
+    for on in (False, True):
+        d, na = two_boxes(on)
+        o = oracle_mod.OracleScene(d)
+        v0 = np.zeros(3 * d.N)
+        v0[3 * na:] = np.tile([0.0, 0.0, -0.6], d.N - na)
+        x, v, its = o.rollout(d.x_rest, v0, 25, record=True)
+        res[on] = x.reshape(-1, 3)[na:, 2].min()
+        g, rc = o.backward(np.ones(3 * d.N), np.zeros(3 * d.N))
+        assert (rc != 0) == on                   # no reverse sweep through self contact
+    assert res[False] < -0.2 and res[True] > 0.03
+
+
+def test_loader_reads_self_collision(tmp_path):
+    (tmp_path / "data").mkdir()
+    x, t = scene.box_grid(1, 1, 1)
+    with open(tmp_path / "data" / "b.obj", "w") as f:
+        for p in x:
+            f.write("v %.17g %.17g %.17g\n" % tuple(p / 0.05))
+        for q in t:
+            f.write("t %d %d %d %d\n" % tuple(q))
+    js = {"Parameters": [{"Self-collision": True, "World Friction": 0.25, "Handle collision": True}],
+          "Meshes": [{"Obj filename": "b.obj"}]}
+    (tmp_path / "s.json").write_text(json.dumps(js))
+    d = scene.load_json_scene("s.json", str(tmp_path))
+    assert d.self_collision and d.self_mu == 0.25 and d.contact
