@@ -956,7 +956,7 @@ __global__ void k_adj_post_solve(const double* __restrict__ Z, const double* __r
 // corners get Fbar G_k; the per-(tet,rollout) E-gradient accumulates
 // (A z).(d_c(y_k) - A y_{k+1}) dw_c/dE over the two weights.
 template <typename T, bool UNI>
-__global__ __launch_bounds__(256) void k_vjp_tet(const double* __restrict__ Yk, const double* __restrict__ Yk1,
+__global__ __launch_bounds__(256, (sizeof(T) == 4 && UNI) ? 4 : 1) void k_vjp_tet(const double* __restrict__ Yk, const double* __restrict__ Yk1,
                                                  const double* __restrict__ Z, const int* __restrict__ tv,
                                                  const double* __restrict__ tG, const double* __restrict__ wr,
                                                  const double* __restrict__ wv, const double* __restrict__ dEr,
