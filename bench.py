@@ -4,9 +4,10 @@ vs the plain FP64 GPU path, on one MI355X.
 Default workload (the batched-rollout configuration): 40x12x12 box
 (34,560 tets, 6,929 vertices), 64 straight fibres, ground plane with friction,
 R = 256 rollouts sharing one Cholesky factor, seeded sinusoidal activations,
-100 steps x 10 iterations, forward + adjoint with checkpoints every 50 steps
-(the forward pass records the last segment's tape, the adjoint recomputes the
-first), FP32 local step with FP64 factor/solves/adjoint accumulation.
+100 steps x 10 iterations, forward + adjoint with the whole rollout taped by
+the forward pass (about 232 GB of the 309 GB of HBM, so the adjoint recomputes
+nothing; --ckpt k keeps checkpoints every k steps and recomputes segments),
+FP32 local step with FP64 factor/solves/adjoint accumulation.
 
 One bench "step" = one full forward + adjoint pass over the batch (inputs
 already resident in HBM). value = sum over rollouts of T * executed PD
@@ -89,8 +90,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--rollouts", type=int, default=256)
     ap.add_argument("--sim-steps", type=int, default=100)
-    ap.add_argument("--ckpt", type=int, default=50,
-                    help="checkpoint interval (the forward pass tapes the last segment; about 2 GB of tape per step)")
+    ap.add_argument("--ckpt", type=int, default=0,
+                    help="checkpoint interval; 0: tape the whole rollout in the forward pass (no recompute in the "
+                         "adjoint; about 2.3 GB of HBM per step for the 40x12x12 box at R = 256)")
     ap.add_argument("--plain-steps", type=int, default=0,
                     help="sim steps of the plain FP64 comparator (0: same as --sim-steps)")
     ap.add_argument("--contact", type=int, default=1)
@@ -116,6 +118,8 @@ def main():
 
     d = build_workload(args)
     R, S = args.rollouts, args.sim_steps
+    if args.ckpt <= 0:
+        args.ckpt = S
     if args.workload == "c5":
         R = 1 if args.rollouts == 256 else args.rollouts
     log("setup: T=%d N=%d R=%d steps=%d" % (d.T, d.N, R, S))
@@ -162,7 +166,8 @@ def main():
     for w in range(args.warmup):
         one_pass()
         torch.cuda.synchronize()
-        log("warmup %d done" % w)
+        free_b, total_b = torch.cuda.mem_get_info(dev)
+        log("warmup %d done; device memory in use %.1f of %.1f GB" % (w, (total_b - free_b) / 1e9, total_b / 1e9))
     sc.solve_timing()  # restart the per-solve event counters
     n_solve, solve_ms = 0, 0.0
     times, tet_its, fwd_ms, bwd_ms = [], [], [], []

@@ -2078,9 +2078,11 @@ __global__ void k_contact_jacobi(const int* __restrict__ Lptr, const int* __rest
     tRC[tb] = rc[0];
     tRC[tb + 1] = rc[1];
     tRC[tb + 2] = rc[2];
-    tN[tb] = 0.0;
-    tN[tb + 1] = 0.0;
-    tN[tb + 2] = 0.0;
+    if (tN) {  // once per pass: the winners do not change within a pass
+      tN[tb] = 0.0;
+      tN[tb + 1] = 0.0;
+      tN[tb + 2] = 0.0;
+    }
   }
   if (ci >= 0) {
     const long long wi = (long long)ci * R + r;
@@ -2343,13 +2345,18 @@ int grow_keep(DevBuf<X>& b, size_t count, size_t keep, hipStream_t st) {
 }
 
 // Tape of the contact phase of one step: every launched Jacobi iteration g
-// (all passes) keeps x_it after it, the pre-projection rhs and winning normal
-// of each contact vertex, and which rollouts executed it.
+// (all passes) keeps x_it after it, the pre-projection rhs of each contact
+// vertex and which rollouts executed it; each detection pass keeps the winning
+// normals (a rollout that executes any iteration of a pass executes its first,
+// which writes them).
 struct ContactTape {
   int G = 0;
   std::vector<char> first;     // iteration opened a detection pass
   DevBuf<double> xit;          // (G+1) x NC, xit[0] = PD result
-  DevBuf<double> rc, nrm;      // G x ncv x C
+  int P = 0;                   // detection passes taped
+  std::vector<int> pass_of;    // pass of each iteration
+  DevBuf<double> rc;           // G x ncv x C
+  DevBuf<double> nrm;          // P x ncv x C: the winning normals of each pass
   DevBuf<int> exec;            // G x R
   DevBuf<int> flast;           // R: result is x_it of the last executed iteration
 };
@@ -3153,7 +3160,9 @@ int contact_phase(pdm_scene s, const double* act, int act_steps, int step, Conta
   size_t cvc = (size_t)s->ncv * C;
   if (tp) {
     tp->G = 0;
+    tp->P = 0;
     tp->first.clear();
+    tp->pass_of.clear();
     if (grow_keep(tp->xit, nc, 0, s->stream)) return 1;
     if (dcopy(s, tp->xit.p, s->Xit.p, nc)) return 1;
   }
@@ -3188,7 +3197,8 @@ int contact_phase(pdm_scene s, const double* act, int act_steps, int step, Conta
     if (tp) {
       size_t g1 = (size_t)tp->G + S.max_iter1;
       if (grow_keep(tp->xit, (g1 + 1) * nc, (tp->G + 1) * nc, s->stream) ||
-          grow_keep(tp->rc, g1 * cvc, tp->G * cvc, s->stream) || grow_keep(tp->nrm, g1 * cvc, tp->G * cvc, s->stream) ||
+          grow_keep(tp->rc, g1 * cvc, tp->G * cvc, s->stream) ||
+          grow_keep(tp->nrm, (size_t)(tp->P + 1) * cvc, (size_t)tp->P * cvc, s->stream) ||
           grow_keep(tp->exec, g1 * R, (size_t)tp->G * R, s->stream))
         return 1;
     }
@@ -3206,7 +3216,7 @@ int contact_phase(pdm_scene s, const double* act, int act_steps, int step, Conta
                          s->Lptr.p, s->Lcol.p, s->Lval.p, s->B.p, s->Dn.p, s->Xn.p, s->Kdiag.p, s->cv_of_v.p,
                          s->win_st.p, s->win_n.p, S.mu, S.alpha, S.h, i == 0 ? 1 : 0, s->alist.p,
                          s->acount.p, s->Prev.p, s->Xit.p, N, R, tp ? tp->rc.p + (size_t)tp->G * cvc : nullptr,
-                         tp ? tp->nrm.p + (size_t)tp->G * cvc : nullptr, self ? s->s_cur.p : nullptr,
+                         tp && i == 0 ? tp->nrm.p + (size_t)tp->P * cvc : nullptr, self ? s->s_cur.p : nullptr,
                          self ? s->s_Fo.p : nullptr, self ? s->s_CN.p : nullptr, self ? s->s_RC.p : nullptr,
                          self ? s->s_ngrp.p : nullptr);
       if (self) {
@@ -3223,9 +3233,11 @@ int contact_phase(pdm_scene s, const double* act, int act_steps, int step, Conta
       if (tp) {
         if (dcopy(s, tp->xit.p + (size_t)(tp->G + 1) * nc, s->Xit.p, nc)) return 1;
         tp->first.push_back(i == 0 ? 1 : 0);
+        tp->pass_of.push_back(tp->P);
         tp->G += 1;
       }
     }
+    if (tp) tp->P += 1;
   }
   if (tp) {
     if (grow_keep(tp->flast, R, 0, s->stream)) return 1;
@@ -3502,7 +3514,7 @@ int contact_reverse(pdm_scene s, ContactTape& tp, const double* Xn, const double
                        s->acount.p);
     hipLaunchKernelGGL(k_cadj_pre, dim3(cdiv((long long)N * R, kThreads)), dim3(kThreads), 0, s->stream, s->cXB.p,
                        s->cPB.p, s->Xb.p, tp.xit.p + (size_t)(g + 1) * nc, Xn, s->Kdiag.p, s->cv_of_v.p,
-                       tp.rc.p + (size_t)g * cvc, tp.nrm.p + (size_t)g * cvc, S.mu, S.alpha, S.h, (int)tp.first[g],
+                       tp.rc.p + (size_t)g * cvc, tp.nrm.p + (size_t)tp.pass_of[g] * cvc, S.mu, S.alpha, S.h, (int)tp.first[g],
                        s->alist.p, s->acount.p, s->Zb.p, s->Zs.p, s->cCD.p, N, R);
     hipLaunchKernelGGL(k_cadj_cz, dim3(cdiv(nc, kThreads)), dim3(kThreads), 0, s->stream, s->Lptr.p, s->Lcol.p,
                        s->Lval.p, s->Zb.p, s->Xb.p, s->cOB.p, s->alist.p, s->acount.p, N, C);
