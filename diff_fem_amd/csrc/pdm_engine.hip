@@ -455,16 +455,18 @@ __global__ void k_gather_masked(const T* __restrict__ corner, const int* __restr
   B[idx] = (s0 + s1) + (s2 + s3);
 }
 
+// Per supernode s (w columns, m rows, the first w its own) the sweeps use
+//   W = [inv(L_ss); -L_below inv(L_ss)]   (m x w, top block lower triangular)
+// so that each sweep is one dense product per supernode:
+//   forward:  [y_s; U_s] = [0; T_below] + W T_top
+//   backward: x_s = W^T [y_s; x_below]
 struct FactorView {
-  const long long* dptr;
-  const double* Dinv;   // inv(L_ss), row-major w x w
-  const double* DinvC;  // the same, column-major (forward sweep reads it down columns)
-  const double* LR;     // L, row-major m x w (backward sweep reads it along rows)
+  const double* W;      // column-major m x w at lptr[s] (forward sweep reads it down columns)
+  const double* WR;     // the same, row-major (backward sweep reads it along rows)
   const int* first;
   const long long* rowptr;
   const int* rows;
   const long long* lptr;
-  const double* L;
   const int* child_ptr;
   const int* child;
   const long long* relptr;
@@ -552,10 +554,11 @@ __device__ __forceinline__ v4d mfma_chain(v4d acc, int kbeg, int kend, int lk, F
 // Forward substitution (L y = b) of one elimination-tree level. Block =
 // (supernode, column tile of Ct = 16*nq right-hand sides), 4 waves. The front
 // T (m x Ct) is staged in LDS: own rows from B, children's update blocks added
-// in child order. Then two dense products on the matrix cores:
-//   y = inv(L_ss) T[0:w]            (inverse precomputed at factorisation)
-//   T[w:m] -= L[w:m, 0:w] y         (update block U_s for the parent)
-// Both are split into 16x16 output tiles distributed over the waves.
+// in child order. Then one dense product on the matrix cores, split into
+// 16x16 output tiles distributed over the waves:
+//   y = inv(L_ss) T[0:w]                       (top tiles, into Y)
+//   T[w:m] += (-L_below inv(L_ss)) T[0:w]      (update block U_s for the parent)
+// Both read only T[0:w] and write disjoint rows, so no barrier separates them.
 __device__ __forceinline__ void fwd_supernode(const FactorView& f, int s, int col0, int ct, int lg, int C,
                                               double* __restrict__ B, double* __restrict__ U, double* __restrict__ sh) {
   const int Ct = 1 << lg;
@@ -565,8 +568,7 @@ __device__ __forceinline__ void fwd_supernode(const FactorView& f, int s, int co
   const int m = (int)(f.rowptr[s + 1] - rp);
   const int mpad = (m + 15) & ~15;
   double* Y = sh + mpad * Ct;  // wpad x Ct
-  const double* Ls = f.L + f.lptr[s];
-  const double* DiC = f.DinvC + f.dptr[s];
+  const double* Ws = f.W + f.lptr[s];
   const int tid = threadIdx.x, nt = blockDim.x;
   const int lane = tid & 63, wave = tid >> 6, nwave = nt >> 6;
   const int lr = lane & 15, lk = lane >> 4;
@@ -597,42 +599,43 @@ __device__ __forceinline__ void fwd_supernode(const FactorView& f, int s, int co
     }
     __syncthreads();
   }
-  // y = inv(L_ss) T[0:w]  (lower triangular: k <= row)
   const int wpad = (w + 15) & ~15;
   const int nrt = wpad >> 4;
-  for (int tq = wave; tq < nrt * nq; tq += nwave) {
-    const int rt = (tq / nq) * 16, q = tq % nq;
-    const int row = rt + lr;
-    const int kend = min(w, rt + 16);
-    const double* pa = DiC + row;
-    const double* pb = sh + q * 16 + lr;
-    v4d acc = {0.0, 0.0, 0.0, 0.0};
-    acc = mfma_chain(
-        acc, 0, kend, lk, [&](int k) { return row < w ? pa[k * w] : 0.0; }, [&](int k) { return pb[k * Ct]; });
-#pragma unroll
-    for (int i = 0; i < 4; ++i) Y[(rt + 4 * i + lk) * Ct + q * 16 + lr] = acc[i];
-  }
-  __syncthreads();
-  // T[w:m] -= L_below y   (row tiles start at w)
   const int mb = m - w;
   const int nbt = (mb + 15) >> 4;
-  for (int tq = wave; tq < nbt * nq; tq += nwave) {
-    const int rt = w + (tq / nq) * 16, q = tq % nq;
-    const int row = rt + lr;
-    const double* pa = Ls + row;
-    const double* pb = Y + q * 16 + lr;
-    v4d acc;
+  for (int tq = wave; tq < (nrt + nbt) * nq; tq += nwave) {
+    const int tr = tq / nq, q = tq % nq;
+    const double* pb = sh + q * 16 + lr;
+    if (tr < nrt) {
+      // y rows (inv(L_ss) is lower triangular: k <= row)
+      const int rt = tr * 16;
+      const int row = rt + lr;
+      const double* pa = Ws + row;
+      v4d acc = {0.0, 0.0, 0.0, 0.0};
+      acc = mfma_chain(
+          acc, 0, min(w, rt + 16), lk, [&](int k) { return row < w ? pa[(long long)k * m] : 0.0; },
+          [&](int k) { return pb[k * Ct]; });
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      int r = rt + 4 * i + lk;
-      acc[i] = r < m ? sh[r * Ct + q * 16 + lr] : 0.0;
-    }
-    acc = mfma_chain(
-        acc, 0, w, lk, [&](int k) { return row < m ? -pa[k * m] : 0.0; }, [&](int k) { return pb[k * Ct]; });
+      for (int i = 0; i < 4; ++i) Y[(rt + 4 * i + lk) * Ct + q * 16 + lr] = acc[i];
+    } else {
+      // update rows (row tiles start at w)
+      const int rt = w + (tr - nrt) * 16;
+      const int row = rt + lr;
+      const double* pa = Ws + row;
+      v4d acc;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      int r = rt + 4 * i + lk;
-      if (r < m) sh[r * Ct + q * 16 + lr] = acc[i];
+      for (int i = 0; i < 4; ++i) {
+        int r = rt + 4 * i + lk;
+        acc[i] = r < m ? sh[r * Ct + q * 16 + lr] : 0.0;
+      }
+      acc = mfma_chain(
+          acc, 0, w, lk, [&](int k) { return row < m ? pa[(long long)k * m] : 0.0; },
+          [&](int k) { return pb[k * Ct]; });
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        int r = rt + 4 * i + lk;
+        if (r < m) sh[r * Ct + q * 16 + lr] = acc[i];
+      }
     }
   }
   __syncthreads();
@@ -678,9 +681,10 @@ __global__ __launch_bounds__(1024) void k_fwd_subtree(FactorView f, const int2* 
   }
 }
 
-// Backward substitution (L^T x = y) of one level as two dense products:
-//   t = y_s - L[w:m, 0:w]^T x_below   (rows below are final)
-//   x_s = inv(L_ss)^T t
+// Backward substitution (L^T x = y) of one level as one dense product:
+//   x_s = inv(L_ss)^T y_s + (-L_below inv(L_ss))^T x_below = W^T T
+// with T = [y_s; x_below] (rows below are final). W's top block is lower
+// triangular, so output row i needs k >= i only.
 __device__ __forceinline__ void bwd_supernode(const FactorView& f, int s, int col0, int ct, int lg, int C,
                                               double* __restrict__ B, double* __restrict__ sh) {
   const int Ct = 1 << lg;
@@ -692,8 +696,7 @@ __device__ __forceinline__ void bwd_supernode(const FactorView& f, int s, int co
   const int wpad = (w + 15) & ~15;
   double* Tt = sh + mpad * Ct;  // wpad x Ct
   const int* rows = f.rows + rp;
-  const double* LR = f.LR + f.lptr[s];
-  const double* Di = f.Dinv + f.dptr[s];
+  const double* WRs = f.WR + f.lptr[s];
   const int tid = threadIdx.x, nt = blockDim.x;
   const int lane = tid & 63, wave = tid >> 6, nwave = nt >> 6;
   const int lr = lane & 15, lk = lane >> 4;
@@ -704,40 +707,20 @@ __device__ __forceinline__ void bwd_supernode(const FactorView& f, int s, int co
   const int nrt = wpad >> 4;
   for (int tq = wave; tq < nrt * nq; tq += nwave) {
     const int rt = (tq / nq) * 16, q = tq % nq;
-    const int col = rt + lr;  // column of L (output row of t)
-    const double* pa = LR + col;
+    const int col = rt + lr;  // column of W (output row of x_s)
+    const double* pa = WRs + col;
     const double* pb = sh + q * 16 + lr;
-    v4d acc;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      int r = rt + 4 * i + lk;
-      acc[i] = r < w ? sh[r * Ct + q * 16 + lr] : 0.0;
-    }
+    v4d acc = {0.0, 0.0, 0.0, 0.0};
     acc = mfma_chain(
-        acc, w, m, lk, [&](int k) { return col < w ? -pa[k * w] : 0.0; }, [&](int k) { return pb[k * Ct]; });
+        acc, rt, m, lk, [&](int k) { return col < w ? pa[(long long)k * w] : 0.0; },
+        [&](int k) { return pb[k * Ct]; });
 #pragma unroll
     for (int i = 0; i < 4; ++i) Tt[(rt + 4 * i + lk) * Ct + q * 16 + lr] = acc[i];
   }
   __syncthreads();
-  // x = inv(L_ss)^T t  (upper triangular in (i,k): k >= i)
-  for (int tq = wave; tq < nrt * nq; tq += nwave) {
-    const int rt = (tq / nq) * 16, q = tq % nq;
-    const int i_row = rt + lr;
-    const double* pa = Di + i_row;
-    const double* pb = Tt + q * 16 + lr;
-    v4d acc = {0.0, 0.0, 0.0, 0.0};
-    acc = mfma_chain(
-        acc, rt, w, lk, [&](int k) { return i_row < w ? pa[k * w] : 0.0; }, [&](int k) { return pb[k * Ct]; });
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      int r = rt + 4 * i + lk;
-      if (r < w) sh[r * Ct + q * 16 + lr] = acc[i];
-    }
-  }
-  __syncthreads();
   if (ti.c < ct) {
     double* Bo = B + (long long)c0 * C + col0 + ti.c;
-    for (int r = ti.r0; r < w; r += ti.rs) Bo[(long long)r * C] = sh[r * Ct + ti.c];
+    for (int r = ti.r0; r < w; r += ti.rs) Bo[(long long)r * C] = Tt[r * Ct + ti.c];
   }
 }
 
@@ -2387,8 +2370,7 @@ struct pdm_scene_s {
   // factor
   DevBuf<int> f_first, f_rows, f_child_ptr, f_child, f_rel, f_lev;
   DevBuf<long long> f_rowptr, f_lptr, f_relptr, f_uptr;
-  DevBuf<double> f_L, f_Dinv, f_DinvC, f_LR;
-  DevBuf<long long> f_dptr;
+  DevBuf<double> f_W, f_WR;
   std::vector<int> lev_ptr, lev_rows;  // lev_rows: max (mpad + wpad) over the level's supernodes
   // solve launch shape per level and sweep (autotuned per right-hand-side count)
   struct LevelCfg {
@@ -2479,8 +2461,8 @@ struct pdm_scene_s {
   double solve_ms = 0.0;
 
   FactorView view() const {
-    return FactorView{f_dptr.p, f_Dinv.p, f_DinvC.p, f_LR.p, f_first.p, f_rowptr.p, f_rows.p, f_lptr.p, f_L.p, f_child_ptr.p, f_child.p, f_relptr.p, f_rel.p,
-                      f_uptr.p};
+    return FactorView{f_W.p, f_WR.p, f_first.p, f_rowptr.p, f_rows.p, f_lptr.p, f_child_ptr.p, f_child.p,
+                      f_relptr.p, f_rel.p, f_uptr.p};
   }
   size_t NC() const { return (size_t)S.N * C; }
 };
@@ -2589,29 +2571,40 @@ int upload_factor(pdm_scene s) {
   if (s->f_relptr.upload(tmp.data(), tmp.size())) return 1;
   tmp.assign(F.uptr.begin(), F.uptr.end());
   if (s->f_uptr.upload(tmp.data(), tmp.size())) return 1;
-  if (s->f_L.upload(F.L.data(), F.L.size())) return 1;
-  if (s->f_Dinv.upload(F.Dinv.data(), F.Dinv.size())) return 1;
   {
-    // transposed copies: coalesced operand loads in both sweeps
+    // W = [inv(L_ss); -L_below inv(L_ss)] per supernode, column-major and
+    // row-major (coalesced operand loads in both sweeps)
     int ns = (int)F.first.size() - 1;
-    std::vector<double> dc(F.Dinv.size()), lr(F.L.size());
+    std::vector<double> wc(F.L.size()), wr(F.L.size());
+    std::vector<double> row;
     for (int sn = 0; sn < ns; ++sn) {
       int w = F.first[sn + 1] - F.first[sn];
       int m = (int)(F.rowptr[sn + 1] - F.rowptr[sn]);
-      const double* D = &F.Dinv[F.dptr[sn]];
-      double* Dc = &dc[F.dptr[sn]];
-      for (int r = 0; r < w; ++r)
-        for (int k = 0; k < w; ++k) Dc[(size_t)k * w + r] = D[(size_t)r * w + k];
-      const double* Lc = &F.L[F.lptr[sn]];
-      double* Lr = &lr[F.lptr[sn]];
-      for (int c = 0; c < w; ++c)
-        for (int r = 0; r < m; ++r) Lr[(size_t)r * w + c] = Lc[(size_t)c * m + r];
+      const double* D = &F.Dinv[F.dptr[sn]];  // row-major, zero above the diagonal
+      const double* Lc = &F.L[F.lptr[sn]];    // column-major m x w
+      double* Wc = &wc[F.lptr[sn]];
+      double* Wr = &wr[F.lptr[sn]];
+      row.assign(w, 0.0);
+      for (int r = 0; r < m; ++r) {
+        if (r < w) {
+          for (int k = 0; k < w; ++k) row[k] = D[(size_t)r * w + k];
+        } else {
+          // -sum_j L[r][j] inv(L_ss)[j][k], j >= k
+          std::fill(row.begin(), row.end(), 0.0);
+          for (int jj = 0; jj < w; ++jj) {
+            const double l = Lc[(size_t)jj * m + r];
+            const double* Dj = D + (size_t)jj * w;
+            for (int k = 0; k <= jj; ++k) row[k] -= l * Dj[k];
+          }
+        }
+        for (int k = 0; k < w; ++k) {
+          Wc[(size_t)k * m + r] = row[k];
+          Wr[(size_t)r * w + k] = row[k];
+        }
+      }
     }
-    if (s->f_DinvC.upload(dc.data(), dc.size()) || s->f_LR.upload(lr.data(), lr.size())) return 1;
+    if (s->f_W.upload(wc.data(), wc.size()) || s->f_WR.upload(wr.data(), wr.size())) return 1;
   }
-
-  tmp.assign(F.dptr.begin(), F.dptr.end());
-  if (s->f_dptr.upload(tmp.data(), tmp.size())) return 1;
   int nlev = (int)F.level_ptr.size() - 1;
   std::vector<int> old_ptr = s->lev_ptr, old_rows = s->lev_rows;
   s->lev_ptr = F.level_ptr;
