@@ -55,6 +55,16 @@ __device__ __forceinline__ int xcd_block() {
   return k * q + min(k, rem) + (b >> 3);
 }
 
+// The same over the first nb blocks only (those that carry work when the
+// work is packed at the front of a larger grid); -1 for the others. The busy
+// blocks then still spread over all eight XCDs.
+__device__ __forceinline__ int xcd_block_n(int nb) {
+  const int b = blockIdx.x;
+  if (b >= nb) return -1;
+  const int q = nb >> 3, rem = nb & 7, k = b & 7;
+  return k * q + min(k, rem) + (b >> 3);
+}
+
 // ------------------------------------------------------------------ kernels
 __global__ void k_caller_to_engine(const double* __restrict__ src, double* __restrict__ dst,
                                    const int* __restrict__ perm, int N, int R) {
@@ -225,8 +235,50 @@ __global__ __launch_bounds__(1024) void k_compact(const int* __restrict__ active
   if (threadIdx.x == 0) *acount = base;
 }
 
-// UNI: R is a multiple of 64, so the 64 lanes of a wave share one tet; its
-// index is then made wave-uniform and the per-tet data come in by scalar loads.
+// Active-rollout indexing of the element kernels. Work items are (tet t,
+// listed rollout j < A = *acount). Two variants share the work:
+//   wave-uniform (R a multiple of 64, A >= kDenseBelow): item = t P + j with
+//     P = A rounded up to 64, so the 64 lanes of a wave share one tet; its
+//     index is made wave-uniform and the per-tet data come in by scalar loads
+//     (lanes j >= A idle);
+//   dense (A < kDenseBelow when R is a multiple of 64, every A otherwise):
+//     item = t A + j, lanes span several tets. Per item it costs about three
+//     times the wave-uniform variant, so it pays only while most lanes of the
+//     wave-uniform one would idle.
+// A launch of the variant that does not apply exits after one load.
+constexpr int kDenseBelow = 24;
+// Blocks of a launch: the wave-uniform variant covers T P <= T R items, the
+// dense one T (kDenseBelow - 1) items, or T R when it runs alone.
+inline long long element_blocks(long long Tn, int amax) { return (Tn * amax + 255) / 256; }
+
+// Item of this thread: (tet t, listed rollout j); false for idle lanes and for
+// the variant that does not apply. 32-bit: the host keeps T R below 2^31.
+template <bool UNI>
+__device__ __forceinline__ bool element_item(const int* __restrict__ acount, int Tn, int R, int* t, int* j) {
+  const int A = *acount;
+  if (UNI) {
+    if (A < kDenseBelow) return false;
+    const unsigned P = (unsigned)(A + 63) & ~63u;
+    const unsigned n = (unsigned)Tn * P;
+    const int b = xcd_block_n((int)((n + blockDim.x - 1) / blockDim.x));
+    if (b < 0) return false;
+    const unsigned idx = (unsigned)b * blockDim.x + threadIdx.x;
+    if (idx >= n) return false;
+    *t = __builtin_amdgcn_readfirstlane((int)(idx / P));
+    *j = (int)(idx - (unsigned)*t * P);
+    return *j < A;
+  }
+  if ((R % 64 == 0 && A >= kDenseBelow) || A == 0) return false;
+  const unsigned n = (unsigned)Tn * (unsigned)A;
+  const int b = xcd_block_n((int)((n + blockDim.x - 1) / blockDim.x));
+  if (b < 0) return false;
+  const unsigned idx = (unsigned)b * blockDim.x + threadIdx.x;
+  if (idx >= n) return false;
+  *t = (int)(idx / (unsigned)A);
+  *j = (int)(idx - (unsigned)*t * (unsigned)A);
+  return true;
+}
+
 template <typename T, bool UNI>
 __global__ __launch_bounds__(256) void k_local_tet(const double* __restrict__ X, const int* __restrict__ tv,
                                                    const double* __restrict__ tG, const double* __restrict__ wr,
@@ -237,12 +289,8 @@ __global__ __launch_bounds__(256) void k_local_tet(const double* __restrict__ X,
                                                    const double* __restrict__ mw, const int* __restrict__ mgroup,
                                                    int Mn, const double* __restrict__ act, int act_stride_r,
                                                    int act_off, const int* __restrict__ step_dev, int ngroups) {
-  long long idx = (long long)xcd_block() * blockDim.x + threadIdx.x;
-  if (idx >= (long long)Tn * R) return;
-  int t = (int)(idx / R);
-  const int j = (int)(idx % R);
-  if (UNI) t = __builtin_amdgcn_readfirstlane(t);
-  if (j >= *acount) return;
+  int t, j;
+  if (!element_item<UNI>(acount, Tn, R, &t, &j)) return;
   const int r = alist[j];
   if (step_dev) act_off = *step_dev * ngroups;
   int C = 3 * R;
@@ -951,12 +999,8 @@ __global__ __launch_bounds__(256, (sizeof(T) == 4 && UNI) ? 4 : 1) void k_vjp_te
                                                  const double* __restrict__ mw, const int* __restrict__ mgroup, int Mn,
                                                  const double* __restrict__ act, int act_stride_r, int act_off,
                                                  double* __restrict__ gMpart) {
-  long long idx = (long long)xcd_block() * blockDim.x + threadIdx.x;
-  if (idx >= (long long)Tn * R) return;
-  int t = (int)(idx / R);
-  const int j = (int)(idx % R);
-  if (UNI) t = __builtin_amdgcn_readfirstlane(t);
-  if (j >= *acount) return;
+  int t, j;
+  if (!element_item<UNI>(acount, Tn, R, &t, &j)) return;
   const int r = alist[j];
   int C = 3 * R;
   int v[4] = {tv[t], tv[Tn + t], tv[2 * Tn + t], tv[3 * Tn + t]};
@@ -2762,6 +2806,10 @@ int autotune_levels(pdm_scene s);
 int ensure_batch_buffers(pdm_scene s) {
   size_t nc = s->NC();
   int N = s->S.N;
+  if ((long long)s->S.T * ((s->R + 63) & ~63) >= (1LL << 31)) {
+    g_err = "tets x rollouts must stay below 2^31 (element kernels index items in 32 bits)";
+    return 1;
+  }
   if (s->X.alloc(nc) || s->V.alloc(nc) || s->Xn.alloc(nc) || s->Cc.alloc(nc) || s->Xit.alloc(nc) || s->B.alloc(nc) ||
       s->Lx.alloc(nc))
     return 1;
@@ -3104,10 +3152,15 @@ void launch_local(pdm_scene s, const double* Xsrc, const double* act, int act_st
   const int G = S.n_groups;
   // element work only for the rollouts that still iterate
   hipLaunchKernelGGL(k_compact, dim3(1), dim3(1024), 0, s->stream, mask, s->R, s->alist.p, s->acount.p);
-  auto kern = s->R % 64 == 0 ? k_local_tet<T, true> : k_local_tet<T, false>;
-  hipLaunchKernelGGL(kern, dim3(cdiv(nt, kThreads)), dim3(kThreads), 0, s->stream, Xsrc, s->tv.p, s->tG.p, s->wr.p,
-                     s->wv.p, s->tslot.p, s->alist.p, s->acount.p, corner, S.T, s->R, s->mptr.p, s->mg.p, s->mw.p,
-                     s->mgroup.p, S.M, act, act_steps * G, step * G, step_dev, G);
+  // wave-uniform variant for kDenseBelow or more listed rollouts, dense one for fewer
+  const bool uni = s->R % 64 == 0;
+  if (uni)
+    hipLaunchKernelGGL((k_local_tet<T, true>), dim3(cdiv(nt, kThreads)), dim3(kThreads), 0, s->stream, Xsrc, s->tv.p,
+                       s->tG.p, s->wr.p, s->wv.p, s->tslot.p, s->alist.p, s->acount.p, corner, S.T, s->R, s->mptr.p,
+                       s->mg.p, s->mw.p, s->mgroup.p, S.M, act, act_steps * G, step * G, step_dev, G);
+  hipLaunchKernelGGL((k_local_tet<T, false>), dim3(element_blocks(S.T, uni ? kDenseBelow - 1 : s->R)), dim3(kThreads), 0, s->stream,
+                     Xsrc, s->tv.p, s->tG.p, s->wr.p, s->wv.p, s->tslot.p, s->alist.p, s->acount.p, corner, S.T, s->R,
+                     s->mptr.p, s->mg.p, s->mw.p, s->mgroup.p, S.M, act, act_steps * G, step * G, step_dev, G);
   hipLaunchKernelGGL((k_gather_list<T, 0>), dim3(cdiv((long long)S.N * 3 * s->R, kThreads)), dim3(kThreads), 0, s->stream,
                      s->Cc.p, corner, s->vptr.p, s->alist.p, s->acount.p, s->B.p, S.N, s->R, any);
 }
@@ -3454,11 +3507,18 @@ int vjp_iteration(pdm_scene s, const int* active, const double* Yk, const double
   // active null: the rollout list is already compacted
   if (active)
     hipLaunchKernelGGL(k_compact, dim3(1), dim3(1024), 0, s->stream, active, s->R, s->alist.p, s->acount.p);
-  auto kern = s->R % 64 == 0 ? k_vjp_tet<T, true> : k_vjp_tet<T, false>;
-  hipLaunchKernelGGL(kern, dim3(cdiv(nt, kThreads)), dim3(kThreads), 0, s->stream, Yk, Yk1, s->Zb.p, s->tv.p, s->tG.p,
-                     s->wr.p, s->wv.p, s->dEr.p, s->dEv.p, s->tslot.p, s->alist.p, s->acount.p, corner, s->gEpart.p,
-                     S.T, s->R, CDp ? Xn : nullptr, CDp, s->mptr.p, s->mg.p, s->mw.p, s->mgroup.p, S.M, act,
-                     act_steps * S.n_groups, step * S.n_groups, S.M > 0 ? s->gMpart.p : nullptr);
+  const bool uni = s->R % 64 == 0;
+  if (uni)
+    hipLaunchKernelGGL((k_vjp_tet<T, true>), dim3(cdiv(nt, kThreads)), dim3(kThreads), 0, s->stream, Yk, Yk1,
+                       s->Zb.p, s->tv.p, s->tG.p, s->wr.p, s->wv.p, s->dEr.p, s->dEv.p, s->tslot.p, s->alist.p,
+                       s->acount.p, corner, s->gEpart.p, S.T, s->R, CDp ? Xn : nullptr, CDp, s->mptr.p, s->mg.p,
+                       s->mw.p, s->mgroup.p, S.M, act, act_steps * S.n_groups, step * S.n_groups,
+                       S.M > 0 ? s->gMpart.p : nullptr);
+  hipLaunchKernelGGL((k_vjp_tet<T, false>), dim3(element_blocks(S.T, uni ? kDenseBelow - 1 : s->R)), dim3(kThreads), 0, s->stream,
+                     Yk, Yk1, s->Zb.p, s->tv.p, s->tG.p, s->wr.p, s->wv.p, s->dEr.p, s->dEv.p, s->tslot.p, s->alist.p,
+                     s->acount.p, corner, s->gEpart.p, S.T, s->R, CDp ? Xn : nullptr, CDp, s->mptr.p, s->mg.p,
+                     s->mw.p, s->mgroup.p, S.M, act, act_steps * S.n_groups, step * S.n_groups,
+                     S.M > 0 ? s->gMpart.p : nullptr);
   dim3 gg(cdiv((long long)S.N * 3 * s->R, kThreads));
   if (out_add)
     hipLaunchKernelGGL((k_gather_list<T, 2>), gg, dim3(kThreads), 0, s->stream, nullptr, corner, s->vptr.p,
