@@ -425,8 +425,9 @@ __global__ __launch_bounds__(256) void k_gather4(const double* __restrict__ Cc, 
 // column 3 alist[jj/3] + jj%3 over the vertex's slot range (four partial sums,
 // fixed order), so that a wave reads contiguous slot entries. MODE 0:
 // B = Cc + sum. MODE 1: B = sum. MODE 2: B += sum. Columns of unlisted
-// rollouts are left as they are.
-template <typename T, int MODE>
+// rollouts are left as they are. UNI: 3R is a multiple of 64 (one vertex per
+// wave, its slot range by scalar loads).
+template <typename T, int MODE, bool UNI>
 __global__ __launch_bounds__(256) void k_gather_list(const double* __restrict__ Cc, const T* __restrict__ corner,
                                                      const int* __restrict__ vptr, const int* __restrict__ alist,
                                                      const int* __restrict__ acount, double* __restrict__ B, int N,
@@ -435,8 +436,9 @@ __global__ __launch_bounds__(256) void k_gather_list(const double* __restrict__ 
   const long long C = 3LL * R;
   long long idx = (long long)xcd_block() * blockDim.x + threadIdx.x;
   if (idx >= (long long)N * C) return;
-  const int v = (int)(idx / C);
+  int v = (int)(idx / C);
   const int jj = (int)(idx - (long long)v * C);
+  if (UNI) v = __builtin_amdgcn_readfirstlane(v);
   if (jj >= 3 * *acount) return;
   const int r3 = jj / 3;
   const int col = 3 * alist[r3] + (jj - 3 * r3);
@@ -2049,6 +2051,9 @@ __global__ __launch_bounds__(256) void k_contact_err(const double* __restrict__ 
 //   contact vertices: rhs replaced by the cone-projected reaction B r
 //   (World.h:1363-1436), prev = alpha rhs + (1-alpha) prev,
 //   x_it = x_n + diag(M/h^2 + L)^-1 prev / h.
+// UNI: R is a multiple of 64, so a wave's 64 threads share one vertex (its
+// matrix row then comes in by scalar loads).
+template <bool UNI>
 __global__ void k_contact_jacobi(const int* __restrict__ Lptr, const int* __restrict__ Lcol,
                                  const double* __restrict__ Lval, const double* __restrict__ B,
                                  const double* __restrict__ Dn, const double* __restrict__ Xn,
@@ -2067,20 +2072,47 @@ __global__ void k_contact_jacobi(const int* __restrict__ Lptr, const int* __rest
   // update is left to k_self_finish (RC = rhs before the self-contact sums).
   long long idx = (long long)xcd_block() * blockDim.x + threadIdx.x;
   if (idx >= (long long)N * R) return;
-  const int v = (int)(idx / R), j = (int)(idx % R);
+  int v = (int)(idx / R);
+  const int j = (int)(idx % R);
+  if (UNI) v = __builtin_amdgcn_readfirstlane(v);
   if (j >= *acount) return;
   const int r = alist[j];
   long long C = 3LL * R;
   long long base = (long long)v * C + 3 * r;
   double rc[3] = {B[base], B[base + 1], B[base + 2]};
-  for (int q = Lptr[v]; q < Lptr[v + 1]; ++q) {
-    int u = Lcol[q];
-    if (u == v) continue;
-    long long ub = (long long)u * C + 3 * r;
-    double w = Lval[q];
-    rc[0] -= w * Dn[ub];
-    rc[1] -= w * Dn[ub + 1];
-    rc[2] -= w * Dn[ub + 2];
+  {
+    // neighbours in row order; the loads of four are issued together
+    int q = Lptr[v];
+    const int qe = Lptr[v + 1];
+    for (; q + 4 <= qe; q += 4) {
+      int u[4];
+      double w[4], d[4][3];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        u[k] = Lcol[q + k];
+        w[k] = Lval[q + k];
+        const double* p = Dn + (long long)u[k] * C + 3 * r;
+        d[k][0] = p[0];
+        d[k][1] = p[1];
+        d[k][2] = p[2];
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (u[k] != v) {
+          rc[0] -= w[k] * d[k][0];
+          rc[1] -= w[k] * d[k][1];
+          rc[2] -= w[k] * d[k][2];
+        }
+    }
+    for (; q < qe; ++q) {
+      int u = Lcol[q];
+      if (u == v) continue;
+      long long ub = (long long)u * C + 3 * r;
+      double w = Lval[q];
+      rc[0] -= w * Dn[ub];
+      rc[1] -= w * Dn[ub + 1];
+      rc[2] -= w * Dn[ub + 2];
+    }
   }
   rc[0] *= h;
   rc[1] *= h;
@@ -2225,6 +2257,7 @@ __global__ void k_cadj_init(double* __restrict__ XB, const double* __restrict__ 
 // q = cone(rc) for one executed iteration, per (vertex, rollout). Produces
 // z = h rc_bar, the D-dependence coefficient CD for the E-gradient, and the
 // running adjoints of x_n (Xb), prev (PB) and c (Zs).
+template <bool UNI>
 __global__ void k_cadj_pre(const double* __restrict__ XB, double* __restrict__ PB, double* __restrict__ Xb,
                            const double* __restrict__ Xit1, const double* __restrict__ Xn,
                            const double* __restrict__ Kdiag, const int* __restrict__ cv_of_v,
@@ -2233,7 +2266,9 @@ __global__ void k_cadj_pre(const double* __restrict__ XB, double* __restrict__ P
                            double* __restrict__ Zb, double* __restrict__ Zs, double* __restrict__ CD, int N, int R) {
   long long idx = (long long)xcd_block() * blockDim.x + threadIdx.x;
   if (idx >= (long long)N * R) return;
-  const int v = (int)(idx / R), j = (int)(idx % R);
+  int v = (int)(idx / R);
+  const int j = (int)(idx % R);
+  if (UNI) v = __builtin_amdgcn_readfirstlane(v);
   if (j >= *acount) return;  // entries of other rollouts are never read
   const int r = alist[j];
   long long C = 3LL * R;
@@ -2267,17 +2302,36 @@ __global__ void k_cadj_pre(const double* __restrict__ XB, double* __restrict__ P
 }
 
 // rc = h (c + J d(ori) - C (ori - x_n)): Xb += C z, OB = -C z (C = offdiag of
-// the scalar matrix without masses).
+// the scalar matrix without masses). UNI: C is a multiple of 64 (one vertex
+// per wave).
+template <bool UNI>
 __global__ void k_cadj_cz(const int* __restrict__ Lptr, const int* __restrict__ Lcol, const double* __restrict__ Lval,
                           const double* __restrict__ Zb, double* __restrict__ Xb, double* __restrict__ OB,
                           const int* __restrict__ alist, const int* __restrict__ acount, int N, int C) {
   long long idx = (long long)xcd_block() * blockDim.x + threadIdx.x;
   if (idx >= (long long)N * C) return;
-  const int v = (int)(idx / C), jj = (int)(idx % C);
+  int v = (int)(idx / C);
+  const int jj = (int)(idx % C);
+  if (UNI) v = __builtin_amdgcn_readfirstlane(v);
   if (jj >= 3 * *acount) return;
   const int j = 3 * alist[jj / 3] + jj % 3;
   double sum = 0.0;
-  for (int q = Lptr[v]; q < Lptr[v + 1]; ++q) {
+  int q = Lptr[v];
+  const int qe = Lptr[v + 1];
+  for (; q + 4 <= qe; q += 4) {
+    int u[4];
+    double w[4], z[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      u[k] = Lcol[q + k];
+      w[k] = Lval[q + k];
+      z[k] = Zb[(long long)u[k] * C + j];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (u[k] != v) sum += w[k] * z[k];
+  }
+  for (; q < qe; ++q) {
     int u = Lcol[q];
     if (u != v) sum += Lval[q] * Zb[(long long)u * C + j];
   }
@@ -3161,7 +3215,7 @@ void launch_local(pdm_scene s, const double* Xsrc, const double* act, int act_st
   hipLaunchKernelGGL((k_local_tet<T, false>), dim3(element_blocks(S.T, uni ? kDenseBelow - 1 : s->R)), dim3(kThreads), 0, s->stream,
                      Xsrc, s->tv.p, s->tG.p, s->wr.p, s->wv.p, s->tslot.p, s->alist.p, s->acount.p, corner, S.T, s->R,
                      s->mptr.p, s->mg.p, s->mw.p, s->mgroup.p, S.M, act, act_steps * G, step * G, step_dev, G);
-  hipLaunchKernelGGL((k_gather_list<T, 0>), dim3(cdiv((long long)S.N * 3 * s->R, kThreads)), dim3(kThreads), 0, s->stream,
+  hipLaunchKernelGGL((uni ? k_gather_list<T, 0, true> : k_gather_list<T, 0, false>), dim3(cdiv((long long)S.N * 3 * s->R, kThreads)), dim3(kThreads), 0, s->stream,
                      s->Cc.p, corner, s->vptr.p, s->alist.p, s->acount.p, s->B.p, S.N, s->R, any);
 }
 
@@ -3258,7 +3312,7 @@ int contact_phase(pdm_scene s, const double* act, int act_steps, int step, Conta
         launch_local<float>(s, s->Ori.p, act, act_steps, step, s->iactive.p);
       else
         launch_local<double>(s, s->Ori.p, act, act_steps, step, s->iactive.p);
-      hipLaunchKernelGGL(k_contact_jacobi, dim3(cdiv((long long)N * R, kThreads)), dim3(kThreads), 0, s->stream,
+      hipLaunchKernelGGL((R % 64 == 0 ? k_contact_jacobi<true> : k_contact_jacobi<false>), dim3(cdiv((long long)N * R, kThreads)), dim3(kThreads), 0, s->stream,
                          s->Lptr.p, s->Lcol.p, s->Lval.p, s->B.p, s->Dn.p, s->Xn.p, s->Kdiag.p, s->cv_of_v.p,
                          s->win_st.p, s->win_n.p, S.mu, S.alpha, S.h, i == 0 ? 1 : 0, s->alist.p,
                          s->acount.p, s->Prev.p, s->Xit.p, N, R, tp ? tp->rc.p + (size_t)tp->G * cvc : nullptr,
@@ -3521,10 +3575,10 @@ int vjp_iteration(pdm_scene s, const int* active, const double* Yk, const double
                      S.M > 0 ? s->gMpart.p : nullptr);
   dim3 gg(cdiv((long long)S.N * 3 * s->R, kThreads));
   if (out_add)
-    hipLaunchKernelGGL((k_gather_list<T, 2>), gg, dim3(kThreads), 0, s->stream, nullptr, corner, s->vptr.p,
+    hipLaunchKernelGGL((uni ? k_gather_list<T, 2, true> : k_gather_list<T, 2, false>), gg, dim3(kThreads), 0, s->stream, nullptr, corner, s->vptr.p,
                        s->alist.p, s->acount.p, out_add, S.N, s->R, nullptr);
   else
-    hipLaunchKernelGGL((k_gather_list<T, 1>), gg, dim3(kThreads), 0, s->stream, nullptr, corner, s->vptr.p,
+    hipLaunchKernelGGL((uni ? k_gather_list<T, 1, true> : k_gather_list<T, 1, false>), gg, dim3(kThreads), 0, s->stream, nullptr, corner, s->vptr.p,
                        s->alist.p, s->acount.p, s->Yb.p, S.N, s->R, nullptr);
   return 0;
 }
@@ -3565,11 +3619,11 @@ int contact_reverse(pdm_scene s, ContactTape& tp, const double* Xn, const double
     const double* ori = tp.xit.p + (size_t)g * nc;
     hipLaunchKernelGGL(k_compact, dim3(1), dim3(1024), 0, s->stream, tp.exec.p + (size_t)g * R, R, s->alist.p,
                        s->acount.p);
-    hipLaunchKernelGGL(k_cadj_pre, dim3(cdiv((long long)N * R, kThreads)), dim3(kThreads), 0, s->stream, s->cXB.p,
+    hipLaunchKernelGGL((R % 64 == 0 ? k_cadj_pre<true> : k_cadj_pre<false>), dim3(cdiv((long long)N * R, kThreads)), dim3(kThreads), 0, s->stream, s->cXB.p,
                        s->cPB.p, s->Xb.p, tp.xit.p + (size_t)(g + 1) * nc, Xn, s->Kdiag.p, s->cv_of_v.p,
                        tp.rc.p + (size_t)g * cvc, tp.nrm.p + (size_t)tp.pass_of[g] * cvc, S.mu, S.alpha, S.h, (int)tp.first[g],
                        s->alist.p, s->acount.p, s->Zb.p, s->Zs.p, s->cCD.p, N, R);
-    hipLaunchKernelGGL(k_cadj_cz, dim3(cdiv(nc, kThreads)), dim3(kThreads), 0, s->stream, s->Lptr.p, s->Lcol.p,
+    hipLaunchKernelGGL((R % 64 == 0 ? k_cadj_cz<true> : k_cadj_cz<false>), dim3(cdiv(nc, kThreads)), dim3(kThreads), 0, s->stream, s->Lptr.p, s->Lcol.p,
                        s->Lval.p, s->Zb.p, s->Xb.p, s->cOB.p, s->alist.p, s->acount.p, N, C);
     int rc = s->precision == PDM_MIXED
                  ? vjp_iteration<float>(s, nullptr, ori, ori, act, act_steps, step, s->cCD.p, s->cOB.p, Xn)
