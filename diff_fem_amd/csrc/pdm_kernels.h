@@ -347,19 +347,15 @@ __device__ __forceinline__ void corotate_vjp(const T* F, T wr, T wv, const T* Z,
   for (int i = 0; i < 3; ++i)
 #pragma unroll
     for (int c = 0; c < 3; ++c) ZU[i * 3 + c] = U[0 * 3 + i] * Z[0 * 3 + c] + U[1 * 3 + i] * Z[1 * 3 + c] + U[2 * 3 + i] * Z[2 * 3 + c];
+  // Hr = H - 2 (U^T Z)[:, 2] V[2, :] when the third column is flipped
+  const T fl = flip ? T(2) : T(0);
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-      T h = 0, hr = 0;
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        T zu = ZU[i * 3 + c];
-        h += zu * V[c * 3 + j];
-        hr += ((flip && c == 2) ? -zu : zu) * V[c * 3 + j];
-      }
+      T h = ZU[i * 3 + 0] * V[0 * 3 + j] + ZU[i * 3 + 1] * V[1 * 3 + j] + ZU[i * 3 + 2] * V[2 * 3 + j];
       H[i * 3 + j] = h;
-      Hr[i * 3 + j] = hr;
+      Hr[i * 3 + j] = h - fl * ZU[i * 3 + 2] * V[2 * 3 + j];
     }
   // volume projection, its divided differences and the diagonal of the
   // volume term's adjoint: O_ii = wv (J^T diag H)_i
@@ -371,26 +367,20 @@ __device__ __forceinline__ void corotate_vjp(const T* F, T wr, T wv, const T* Z,
     for (int i = 0; i < 3; ++i) O[i * 3 + i] = wv * gS[i];
   }
   T phi[3] = {S[0] + D[0], S[1] + D[1], S[2] + D[2]};
-  T dr = 0, dv = 0;
+  if (md) {
 #pragma unroll
-  for (int r = 0; r < 3; ++r)
+    for (int r = 0; r < 3; ++r)
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      T rot = U[r * 3 + 0] * V[c * 3 + 0] + U[r * 3 + 1] * V[c * 3 + 1] + U[r * 3 + 2] * V[c * 3 + 2];
-      T a = (flip && c == 2) ? -rot : rot;
-      T b = U[r * 3 + 0] * phi[0] * V[c * 3 + 0] + U[r * 3 + 1] * phi[1] * V[c * 3 + 1] +
-            U[r * 3 + 2] * phi[2] * V[c * 3 + 2];
-      if (md) {
-        md[r * 3 + c] = a;
-        mdv[r * 3 + c] = b;
-      } else {
-        dr += Z[r * 3 + c] * a;
-        dv += Z[r * 3 + c] * b;
+      for (int c = 0; c < 3; ++c) {
+        T rot = U[r * 3 + 0] * V[c * 3 + 0] + U[r * 3 + 1] * V[c * 3 + 1] + U[r * 3 + 2] * V[c * 3 + 2];
+        md[r * 3 + c] = (flip && c == 2) ? -rot : rot;
+        mdv[r * 3 + c] = U[r * 3 + 0] * phi[0] * V[c * 3 + 0] + U[r * 3 + 1] * phi[1] * V[c * 3 + 1] +
+                         U[r * 3 + 2] * phi[2] * V[c * 3 + 2];
       }
-    }
-  if (!md) {
-    *zmd = dr;
-    *zmdv = dv;
+  } else {
+    // Z : (U Df V^T) = tr(U^T Z Df V) = tr Hr and Z : (U diag(phi) V^T) = sum phi_k H_kk
+    *zmd = Hr[0] + Hr[4] + Hr[8];
+    *zmdv = phi[0] * H[0] + phi[1] * H[4] + phi[2] * H[8];
   }
 #pragma unroll
   for (int i = 0; i < 3; ++i)
