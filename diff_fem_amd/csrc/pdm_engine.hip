@@ -442,20 +442,37 @@ __global__ __launch_bounds__(256) void k_gather_list(const double* __restrict__ 
   if (jj >= 3 * *acount) return;
   const int r3 = jj / 3;
   const int col = 3 * alist[r3] + (jj - 3 * r3);
+  const long long o = (long long)v * C + col;
+  const double c0 = MODE == 0 ? Cc[o] : 0.0;
   double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
   int q = vptr[v], qe = vptr[v + 1];
   const T* p = corner + (long long)q * C + col;
-  for (; q + 4 <= qe; q += 4, p += 4 * C) {
+  // slot k of the range goes to partial sum k mod 4; eight loads in flight
+  for (; q + 8 <= qe; q += 8, p += 8 * C) {
+    double x[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) x[u] = (double)p[u * C];
+    s0 += x[0];
+    s1 += x[1];
+    s2 += x[2];
+    s3 += x[3];
+    s0 += x[4];
+    s1 += x[5];
+    s2 += x[6];
+    s3 += x[7];
+  }
+  if (q + 4 <= qe) {
     s0 += (double)p[0];
     s1 += (double)p[C];
     s2 += (double)p[2 * C];
     s3 += (double)p[3 * C];
+    q += 4;
+    p += 4 * C;
   }
   for (; q < qe; ++q, p += C) s0 += (double)p[0];
   const double sum = (s0 + s1) + (s2 + s3);
-  const long long o = (long long)v * C + col;
   if (MODE == 0)
-    B[o] = Cc[o] + sum;
+    B[o] = c0 + sum;
   else if (MODE == 1)
     B[o] = sum;
   else
@@ -2062,7 +2079,8 @@ __global__ void k_contact_jacobi(const int* __restrict__ Lptr, const int* __rest
                                  double mu, double alpha, double h, int first_iter, const int* __restrict__ alist,
                                  const int* __restrict__ acount, double* __restrict__ Prev, double* __restrict__ Xit,
                                  int N, int R, double* __restrict__ tRC, double* __restrict__ tN,
-                                 double* __restrict__ self_cur = nullptr, double* __restrict__ Fo = nullptr,
+                                 double* __restrict__ tX, double* __restrict__ self_cur = nullptr,
+                                 double* __restrict__ Fo = nullptr,
                                  double* __restrict__ CN = nullptr, double* __restrict__ RC = nullptr,
                                  const int* __restrict__ ngrp = nullptr) {
   // self_cur non-null: self contact is on (World.h:1373-1430 with
@@ -2197,7 +2215,9 @@ __global__ void k_contact_jacobi(const int* __restrict__ Lptr, const int* __rest
   for (int c = 0; c < 3; ++c) {
     double p = rc[c] * alpha + pv[c] * (1.0 - alpha);
     Prev[base + c] = p;
-    Xit[base + c] = Xn[base + c] + (p / h) / kd;
+    const double x = Xn[base + c] + (p / h) / kd;
+    Xit[base + c] = x;
+    if (tX) tX[base + c] = x;  // the iteration's tape slot
   }
 }
 
@@ -3295,6 +3315,10 @@ int contact_phase(pdm_scene s, const double* act, int act_steps, int step, Conta
     for (int r = 0; r < R; ++r) any |= ca[r] != 0;
     if (!any) break;
     if (tp) {
+      // x_it before the pass's first iteration, for every rollout; within the
+      // pass the executing rollouts only drop out, so the Jacobi kernel's own
+      // store of x_it fills the later slots for every rollout that reads them
+      if (tp->G > 0 && !self && dcopy(s, tp->xit.p + (size_t)tp->G * nc, s->Xit.p, nc)) return 1;
       size_t g1 = (size_t)tp->G + S.max_iter1;
       if (grow_keep(tp->xit, (g1 + 1) * nc, (tp->G + 1) * nc, s->stream) ||
           grow_keep(tp->rc, g1 * cvc, tp->G * cvc, s->stream) ||
@@ -3316,7 +3340,8 @@ int contact_phase(pdm_scene s, const double* act, int act_steps, int step, Conta
                          s->Lptr.p, s->Lcol.p, s->Lval.p, s->B.p, s->Dn.p, s->Xn.p, s->Kdiag.p, s->cv_of_v.p,
                          s->win_st.p, s->win_n.p, S.mu, S.alpha, S.h, i == 0 ? 1 : 0, s->alist.p,
                          s->acount.p, s->Prev.p, s->Xit.p, N, R, tp ? tp->rc.p + (size_t)tp->G * cvc : nullptr,
-                         tp && i == 0 ? tp->nrm.p + (size_t)tp->P * cvc : nullptr, self ? s->s_cur.p : nullptr,
+                         tp && i == 0 ? tp->nrm.p + (size_t)tp->P * cvc : nullptr,
+                         tp && !self ? tp->xit.p + (size_t)(tp->G + 1) * nc : nullptr, self ? s->s_cur.p : nullptr,
                          self ? s->s_Fo.p : nullptr, self ? s->s_CN.p : nullptr, self ? s->s_RC.p : nullptr,
                          self ? s->s_ngrp.p : nullptr);
       if (self) {
@@ -3331,7 +3356,8 @@ int contact_phase(pdm_scene s, const double* act, int act_steps, int step, Conta
                            s->Xn.p, s->Kdiag.p, S.alpha, S.h, i == 0 ? 1 : 0, s->alist.p, s->acount.p, N);
       }
       if (tp) {
-        if (dcopy(s, tp->xit.p + (size_t)(tp->G + 1) * nc, s->Xit.p, nc)) return 1;
+        // with self contact k_self_finish also writes x_it: keep the whole copy
+        if (self && dcopy(s, tp->xit.p + (size_t)(tp->G + 1) * nc, s->Xit.p, nc)) return 1;
         tp->first.push_back(i == 0 ? 1 : 0);
         tp->pass_of.push_back(tp->P);
         tp->G += 1;
