@@ -3,7 +3,8 @@ vs the plain FP64 GPU path, on one MI355X.
 
 Default workload (the batched-rollout configuration): 40x12x12 box
 (34,560 tets, 6,929 vertices), 64 straight fibres, ground plane with friction,
-R = 256 rollouts sharing one Cholesky factor, seeded sinusoidal activations,
+R = 256 rollouts sharing one Cholesky factor (run as two concurrent
+sub-batches of 128 on their own streams), seeded sinusoidal activations,
 100 steps x 10 iterations, forward + adjoint with the whole rollout taped by
 the forward pass (about 232 GB of the 309 GB of HBM, so the adjoint recomputes
 nothing; --ckpt k keeps checkpoints every k steps and recomputes segments),
@@ -89,6 +90,9 @@ def main():
     ap.add_argument("--steps", type=int, default=2, help="timed forward+adjoint passes")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--rollouts", type=int, default=256)
+    ap.add_argument("--parts", type=int, default=2,
+                    help="independent sub-batches run concurrently on their own streams (engine.SplitScene); "
+                         "one part's latency-bound solve levels overlap the other's element kernels")
     ap.add_argument("--sim-steps", type=int, default=100)
     ap.add_argument("--ckpt", type=int, default=0,
                     help="checkpoint interval; 0: tape the whole rollout in the forward pass (no recompute in the "
@@ -122,23 +126,37 @@ def main():
         args.ckpt = S
     if args.workload == "c5":
         R = 1 if args.rollouts == 256 else args.rollouts
-    log("setup: T=%d N=%d R=%d steps=%d" % (d.T, d.N, R, S))
+    parts = max(1, min(args.parts, R))
+    Rp = R // parts  # rollouts (and 3 Rp right-hand sides) of the first part's solves
+    log("setup: T=%d N=%d R=%d (%d part(s)) steps=%d" % (d.T, d.N, R, parts, S))
     t0 = time.perf_counter()
     flags = engine.PDM_HOST_EARLY_EXIT | engine.PDM_TIME_SOLVES
     if args.graph:
         flags |= engine.PDM_USE_GRAPH
-    sc = engine.Scene(d, R=R, precision=engine.PDM_MIXED, flags=flags, checkpoint_every=args.ckpt)
+    if args.solve_only:  # one part's solve: the shape the pipeline launches
+        sc = engine.Scene(d, R=Rp, precision=engine.PDM_MIXED, flags=flags, checkpoint_every=args.ckpt)
+        parts_list = [sc]
+    elif parts > 1:
+        sc = engine.SplitScene(d, R=R, parts=parts, precision=engine.PDM_MIXED, flags=flags,
+                               checkpoint_every=args.ckpt)
+        parts_list = sc.parts
+    else:
+        sc = engine.Scene(d, R=R, precision=engine.PDM_MIXED, flags=flags, checkpoint_every=args.ckpt)
+        parts_list = [sc]
+    p0 = parts_list[0]
     info = sc.info()
     log("setup done in %.1fs: %s" % (time.perf_counter() - t0, info))
-    log("solve launch shapes per level ((Ct, threads) fwd, bwd): %s" % (sc.solve_shapes(),))
-    log("solve column chains (streams): %d, subtree cut level: %d" % (sc.solve_chains, sc.solve_cut))
+    for k, p in enumerate(parts_list):
+        log("part %d: solve launch shapes per level ((Ct, threads) fwd, bwd): %s" % (k, p.solve_shapes(),))
+        log("part %d: solve column chains (streams): %d, subtree cut level: %d" % (k, p.solve_chains, p.solve_cut))
     dev = torch.device("cuda:0")
-    flops = 2.0 * 2.0 * info["nnz_lower"] * 3 * R
+    flops = 2.0 * 2.0 * info["nnz_lower"] * 3 * Rp
     if args.plan:
         cut, chains = (int(v) for v in args.plan.split(","))
-        sc.set_solve_plan(cut, chains)
-        sc.solve_shapes()
-        log("solve plan fixed: subtree cut %d, %d column chain(s)" % (sc.solve_cut, sc.solve_chains))
+        for p in parts_list:
+            p.set_solve_plan(cut, chains)
+            p.solve_shapes()
+        log("solve plan fixed: subtree cut %d, %d column chain(s)" % (p0.solve_cut, p0.solve_chains))
     if args.solve_only:
         ms = sc.time_solve(20)
         if args.copy_calib:
@@ -147,7 +165,7 @@ def main():
             for _ in range(3):
                 b.copy_(a)
             torch.cuda.synchronize()
-        print(json.dumps({"solve_ms": ms, "flops_per_solve": flops, "rhs": 3 * R, "nnz_lower": info["nnz_lower"],
+        print(json.dumps({"solve_ms": ms, "flops_per_solve": flops, "rhs": 3 * Rp, "nnz_lower": info["nnz_lower"],
                           "copy_calib_bytes": args.copy_calib << 20}), flush=True)
         return
     acts = torch.from_numpy(scene.activations_sinusoid(4, S, d.n_groups, R, d.h)).to(dev).contiguous()
@@ -200,30 +218,35 @@ def main():
                                 if args.workload == "c4" else "c5_packed_64_meshes_R%d_%dsteps_fwd+adjoint" % (R, S)),
                    "tets": d.T, "vertices": d.N, "rollouts": R, "sim_steps": S, "max_iter": d.max_iter,
                    "checkpoint_every": args.ckpt, "contact": bool(d.contact), "nnz_factor": info["nnz_factor"],
-                   "hipgraph": bool(args.graph),
+                   "hipgraph": bool(args.graph), "sub_batches": parts,
                    "forward_ms": float(np.mean(fwd_ms)), "adjoint_ms": float(np.mean(bwd_ms))},
     }
     # Roofline of the dominant kernel: the supernodal triangular solve
-    # (k_fwd_level + k_bwd_level over all levels, all 3R right-hand sides), the
-    # largest share of kernel time. Its mean duration comes from HIP events
-    # around every solve of the timed passes, on the library stream. At 3R = 768
-    # it is FP64-MFMA bound by arithmetic intensity: algorithmic flops = 2 sweeps
-    # x 2 x nnz(L, lower trapezoid) x 3R; algorithmic bytes = factor read twice
-    # + right-hand sides read and written once per sweep.
+    # (k_fwd_level + k_bwd_level over all levels, one part's 3R/parts
+    # right-hand sides per launch sequence), the largest share of kernel time.
+    # Its mean duration comes from HIP events around every solve of the timed
+    # passes, on each part's stream (so it includes the other part's kernels
+    # sharing the GPU meanwhile). At 3R/parts = 384 it is FP64-MFMA bound by
+    # arithmetic intensity: algorithmic flops = 2 sweeps x 2 x nnz(L, lower
+    # trapezoid) x RHS; algorithmic bytes = factor read twice + right-hand
+    # sides read and written once per sweep.
     ms_live = solve_ms / max(n_solve, 1)
-    bytes_alg = 2.0 * info["nnz_lower"] * 8 + 2.0 * 2 * 3 * R * d.N * 8
+    bytes_alg = 2.0 * info["nnz_lower"] * 8 + 2.0 * 2 * 3 * Rp * d.N * 8
     if flops / bytes_alg >= 78.6e12 / 8.0e12:  # above the FP64 ridge: matrix-core bound
         achieved = flops / (ms_live * 1e-3) / 1e12
         rl = {"bound": "mfma", "achieved": achieved, "peak": 78.6, "unit": "TFLOP/s", "frac": achieved / 78.6}
     else:  # few right-hand sides: the factor stream bounds it
         achieved = bytes_alg / (ms_live * 1e-3) / 1e9
         rl = {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s", "frac": achieved / 8000.0}
-    rl.update({"traffic": solve_traffic(3 * R, info["nnz_lower"]),
-               "kernel": "supernodal solve k_fwd_level+k_bwd_level, %d RHS" % (3 * R),
-               "column_chains": sc.solve_chains,
+    rl.update({"traffic": solve_traffic(3 * Rp, info["nnz_lower"]),
+               "kernel": "supernodal solve k_fwd_level+k_bwd_level, %d RHS" % (3 * Rp),
+               "column_chains": p0.solve_chains,
                "ms_per_solve": ms_live, "solves_timed": n_solve,
-               "ms_per_solve_back_to_back": sc.time_solve(20), "flops_per_solve": flops,
+               "ms_per_solve_back_to_back": p0.time_solve(20), "flops_per_solve": flops,
                "bytes_alg_per_solve": bytes_alg})
+    # the same solve alone on the GPU (no other part's kernels beside it)
+    rl["frac_back_to_back"] = flops / (rl["ms_per_solve_back_to_back"] * 1e-3) / 1e12 / 78.6 \
+        if rl["bound"] == "mfma" else bytes_alg / (rl["ms_per_solve_back_to_back"] * 1e-3) / 1e9 / 8000.0
     out["roofline"] = rl
     sc.close()
     del sc

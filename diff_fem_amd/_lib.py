@@ -72,6 +72,10 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise OSError("libpdm.so not built: run diff_fem_amd.build.build() (hipcc, gfx950)")
+    # torch first: its HIP runtime (SONAME libamdhip64.so.7) is then the one
+    # libpdm binds to. Loaded the other way round, torch brings a second
+    # runtime into the process, which finds no device.
+    import torch  # noqa: F401
     L = C.CDLL(LIB_PATH)
     vp = C.c_void_p
     L.pdm_last_error.restype = C.c_char_p

@@ -275,6 +275,109 @@ class Scene:
         return dict(x0=gx0, v0=gv0, E=gE, nu=gnu, act=gact[:, :self._steps, :self.n_groups])
 
 
+class SplitScene:
+    """R rollouts run as `parts` independent sub-batches, each a Scene with
+    its own HIP stream and its own tape, driven concurrently from one host
+    thread per part. Rollouts never interact, so each sub-batch computes
+    exactly what the whole batch would; what changes is the overlap: the
+    latency-bound upper levels of one part's triangular solve leave compute
+    units idle that the other part's element kernels fill. Same device API as
+    Scene (forward_device / backward_device on (R, 3N) tensors); the shared
+    parameters' gradients (E, nu) are summed over the parts in part order."""
+
+    def __init__(self, desc, R: int, parts: int = 2, precision: int = PDM_FP64, flags: int = PDM_HOST_EARLY_EXIT,
+                 checkpoint_every: int = 20):
+        if parts < 1 or parts > R:
+            raise ValueError("parts must be between 1 and R")
+        sizes = [R // parts + (1 if k < R % parts else 0) for k in range(parts)]
+        self.offsets = np.concatenate([[0], np.cumsum(sizes)]).astype(int)
+        self.parts = [Scene(desc, R=r, precision=precision, flags=flags, checkpoint_every=checkpoint_every)
+                      for r in sizes]
+        self.R, self.N, self.T, self.n_groups = int(R), int(desc.N), int(desc.T), int(desc.n_groups)
+        self._steps = 0
+
+    def _rows(self, k):
+        return slice(int(self.offsets[k]), int(self.offsets[k + 1]))
+
+    def _run(self, fn):
+        """fn(k) for every part, part 0 on the calling thread; the first
+        exception is raised after all parts have returned."""
+        import threading
+        errs = [None] * len(self.parts)
+
+        def call(k):
+            try:
+                fn(k)
+            except BaseException as e:  # re-raised below
+                errs[k] = e
+        th = [threading.Thread(target=call, args=(k,)) for k in range(1, len(self.parts))]
+        for t in th:
+            t.start()
+        call(0)
+        for t in th:
+            t.join()
+        for e in errs:
+            if e is not None:
+                raise e
+
+    def close(self):
+        for p in self.parts:
+            p.close()
+
+    def forward_device(self, x0, v0, steps: int, act=None, xT=None, vT=None, record: bool = False):
+        import torch
+        if xT is None:
+            xT = torch.empty_like(x0)
+        if vT is None:
+            vT = torch.empty_like(v0)
+        rows = [self._rows(k) for k in range(len(self.parts))]
+        self._run(lambda k: self.parts[k].forward_device(
+            x0[rows[k]], v0[rows[k]], steps, act=None if act is None else act[rows[k]], xT=xT[rows[k]],
+            vT=vT[rows[k]], record=record))
+        self._steps = steps
+        return xT, vT
+
+    def backward_device(self, gx, gv):
+        import torch
+        out = [None] * len(self.parts)
+
+        def bwd(k):
+            r = self._rows(k)
+            out[k] = self.parts[k].backward_device(gx[r].contiguous(), gv[r].contiguous())
+        self._run(bwd)
+        g = dict(x0=torch.cat([o["x0"] for o in out]), v0=torch.cat([o["v0"] for o in out]),
+                 act=torch.cat([o["act"] for o in out]))
+        g["E"] = out[0]["E"].clone()
+        g["nu"] = out[0]["nu"].clone()
+        for o in out[1:]:
+            g["E"] += o["E"]
+            g["nu"] += o["nu"]
+        return g
+
+    def iterations(self):
+        return np.concatenate([p.iterations() for p in self.parts])
+
+    def contact_stats(self):
+        st = [p.contact_stats() for p in self.parts]
+        return np.concatenate([a for a, _ in st]), np.concatenate([b for _, b in st])
+
+    def timing(self):
+        t = [p.timing() for p in self.parts]
+        return {k: max(x[k] for x in t) for k in t[0]}
+
+    def solve_timing(self):
+        """(number of solves, summed ms) over all parts since the last call;
+        each solve covers one part's 3R/parts right-hand sides."""
+        st = [p.solve_timing() for p in self.parts]
+        return sum(n for n, _ in st), sum(ms for _, ms in st)
+
+    def info(self) -> dict:
+        i = dict(self.parts[0].info())
+        i["batch"] = self.R
+        i["parts"] = len(self.parts)
+        return i
+
+
 def host_analyse(desc) -> dict:
     """Setup + ordering + symbolic + numeric factorisation on the CPU only."""
     d, keep = make_desc(desc)

@@ -434,3 +434,38 @@ def test_batch64_drop_matches_oracle(eng, oracle_mod, precision):
         assert jump > 0.3 * err, (r, err, jump)
     # the per-tet E gradient sums over rollouts, kinked ones included
     assert rel(g["E"], gE_sum) < (tol_g if not kinked else 1e-2)
+
+
+def test_split_scene_matches_one_scene(eng):
+    """engine.SplitScene (two sub-batches, each on its own stream and host
+    thread) against one Scene on the same batch with contact, graphs and a
+    whole-rollout tape: per-rollout states, iteration counts and gradients
+    bitwise equal; the parameter gradients summed over the parts equal to
+    rounding."""
+    import torch
+    d = _drop_batch_scene()
+    R, steps = 8, 8
+    flags = eng.PDM_HOST_EARLY_EXIT | eng.PDM_USE_GRAPH
+    rng = np.random.default_rng(80)
+    dev = torch.device("cuda:0")
+    x0 = torch.from_numpy(np.tile(d.x_rest.ravel(), (R, 1))).to(dev)
+    v0 = torch.from_numpy(np.tile([0.1, -0.05, -0.8], (R, d.N)) + 0.05 * rng.standard_normal((R, 3 * d.N))).to(dev)
+    act = torch.from_numpy(rng.uniform(0, 1, (R, steps, d.n_groups))).to(dev)
+    gx = torch.from_numpy(rng.standard_normal((R, 3 * d.N))).to(dev)
+    gv = torch.from_numpy(0.01 * rng.standard_normal((R, 3 * d.N))).to(dev)
+    out = []
+    for sc in (eng.Scene(d, R=R, precision=eng.PDM_FP64, flags=flags, checkpoint_every=steps),
+               eng.SplitScene(d, R=R, parts=2, precision=eng.PDM_FP64, flags=flags, checkpoint_every=steps)):
+        xT, vT = sc.forward_device(x0, v0, steps, act=act, record=True)
+        g = sc.backward_device(gx, gv)
+        torch.cuda.synchronize()
+        out.append((xT.cpu().numpy(), vT.cpu().numpy(), {k: v.cpu().numpy() for k, v in g.items()}, sc.iterations(),
+                    sc.contact_stats()[1]))
+        sc.close()
+    (x1, v1, g1, i1, j1), (x2, v2, g2, i2, j2) = out
+    assert j1.sum() > 0
+    assert np.array_equal(i1, i2) and np.array_equal(j1, j2)
+    assert np.array_equal(x1, x2) and np.array_equal(v1, v2)
+    for k in ("x0", "v0", "act"):
+        assert np.array_equal(g1[k], g2[k]), k
+    assert rel(g2["E"], g1["E"]) < 1e-12 and abs(g2["nu"][0] - g1["nu"][0]) <= 1e-12 * abs(g1["nu"][0])
