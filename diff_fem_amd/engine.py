@@ -324,6 +324,10 @@ class SplitScene:
         for p in self.parts:
             p.close()
 
+    def set_material(self, E=None, nu_fwd=None):
+        for p in self.parts:
+            p.set_material(E, nu_fwd)
+
     def forward_device(self, x0, v0, steps: int, act=None, xT=None, vT=None, record: bool = False):
         import torch
         if xT is None:
