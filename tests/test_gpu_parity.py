@@ -469,3 +469,43 @@ def test_split_scene_matches_one_scene(eng):
     for k in ("x0", "v0", "act"):
         assert np.array_equal(g1[k], g2[k]), k
     assert rel(g2["E"], g1["E"]) < 1e-12 and abs(g2["nu"][0] - g1["nu"][0]) <= 1e-12 * abs(g1["nu"][0])
+
+
+@pytest.mark.parametrize("parts", [1, 2])
+def test_rollout_autograd_matches_oracle(eng, oracle_mod, parts):
+    """api_diff.rollout (the torch.autograd.Function over the device API) on a
+    Scene and on a two-part SplitScene: x_T, v_T and the gradients of a scalar
+    loss with respect to x0, v0, the per-tet E and the activations, against the
+    oracle's forward and reverse sweep per rollout (FP64 bar)."""
+    import torch
+    from diff_fem_amd import api_diff
+    d = _drop_batch_scene()
+    R, steps = 4, 8
+    dev = torch.device("cuda:0")
+    kw = dict(precision=eng.PDM_FP64, flags=eng.PDM_HOST_EARLY_EXIT, checkpoint_every=4)
+    sc = eng.Scene(d, R=R, **kw) if parts == 1 else eng.SplitScene(d, R=R, parts=parts, **kw)
+    rng = np.random.default_rng(90)
+    x0n = np.tile(d.x_rest.ravel(), (R, 1))
+    v0n = np.tile([0.05, 0.1, -0.7], (R, d.N)) + 0.05 * rng.standard_normal((R, 3 * d.N))
+    actn = rng.uniform(0, 1, (R, steps, d.n_groups))
+    wx, wv = rng.standard_normal((R, 3 * d.N)), 0.01 * rng.standard_normal((R, 3 * d.N))
+    x0 = torch.tensor(x0n, device=dev, requires_grad=True)
+    v0 = torch.tensor(v0n, device=dev, requires_grad=True)
+    E = torch.tensor(np.asarray(d.E, np.float64), device=dev, requires_grad=True)
+    act = torch.tensor(actn, device=dev, requires_grad=True)
+    xT, vT = api_diff.rollout(x0, v0, E, act, sc, steps)
+    loss = (xT * torch.tensor(wx, device=dev)).sum() + (vT * torch.tensor(wv, device=dev)).sum()
+    loss.backward()
+    o = oracle_mod.OracleScene(d)
+    gE = np.zeros(d.T)
+    for r in range(R):
+        xo, vo, _ = o.rollout(x0n[r], v0n[r], steps, act=actn[r], record=True)
+        go, rc = o.backward(wx[r], wv[r])
+        assert rc == 0
+        assert rel(xT[r].detach().cpu().numpy(), xo) < 1e-9 and rel(vT[r].detach().cpu().numpy(), vo) < 1e-9
+        assert rel(x0.grad[r].cpu().numpy(), go["x0"]) < 1e-9
+        assert rel(v0.grad[r].cpu().numpy(), go["v0"]) < 1e-9
+        assert rel(act.grad[r].cpu().numpy(), go["act"]) < 1e-9
+        gE += go["E"]
+    assert rel(E.grad.cpu().numpy(), gE) < 1e-9
+    sc.close()
