@@ -346,81 +346,6 @@ __global__ __launch_bounds__(256) void k_local_tet(const double* __restrict__ X,
   }
 }
 
-// Four consecutive slot entries widened to FP64 (one 16-byte load for FP32
-// slots, two for FP64).
-__device__ __forceinline__ void load4(const float* p, double* o) {
-  float4 v = *reinterpret_cast<const float4*>(p);
-  o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
-}
-__device__ __forceinline__ void load4(const double* p, double* o) {
-  double2 a = reinterpret_cast<const double2*>(p)[0], b = reinterpret_cast<const double2*>(p)[1];
-  o[0] = a.x; o[1] = a.y; o[2] = b.x; o[3] = b.y;
-}
-
-// Vectorised segmented gather for C % 4 == 0: each thread sums four adjacent
-// columns of one vertex over its slot range, four slot rows per trip
-// (partial sums s[k][.] for rows q = k mod 4, added in a fixed order).
-// MODE 0: B = Cc + sum. MODE 1: B = sum for active rollouts. MODE 2: B += sum
-// for active rollouts.
-template <typename T, int MODE>
-__global__ __launch_bounds__(256) void k_gather4(const double* __restrict__ Cc, const T* __restrict__ corner,
-                                                 const int* __restrict__ vptr, const int* __restrict__ active,
-                                                 double* __restrict__ B, int N, int C,
-                                                 const int* __restrict__ any = nullptr) {
-  if (any && *any == 0) return;
-  const int C4 = C >> 2;
-  long long idx = (long long)xcd_block() * blockDim.x + threadIdx.x;
-  if (idx >= (long long)N * C4) return;
-  int v = (int)(idx / C4);
-  int j = (int)(idx - (long long)v * C4) * 4;
-  bool m[4] = {true, true, true, true};
-  if (MODE != 0) {
-#pragma unroll
-    for (int k = 0; k < 4; ++k) m[k] = active[(j + k) / 3] != 0;
-    if (!(m[0] || m[1] || m[2] || m[3])) return;
-  }
-  double s[4][4];
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int k = 0; k < 4; ++k) s[a][k] = 0.0;
-  int q = vptr[v], qe = vptr[v + 1];
-  const T* p = corner + (long long)q * C + j;
-  for (; q + 4 <= qe; q += 4, p += 4LL * C) {
-    double x[4][4];
-#pragma unroll
-    for (int a = 0; a < 4; ++a) load4(p + (long long)a * C, x[a]);
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-      for (int k = 0; k < 4; ++k) s[a][k] += x[a][k];
-  }
-  for (; q < qe; ++q, p += C) {
-    double x[4];
-    load4(p, x);
-#pragma unroll
-    for (int k = 0; k < 4; ++k) s[0][k] += x[k];
-  }
-  double r[4];
-#pragma unroll
-  for (int k = 0; k < 4; ++k) r[k] = (s[0][k] + s[1][k]) + (s[2][k] + s[3][k]);
-  double* o = B + (long long)v * C + j;
-  if (MODE == 0) {
-    double2 c0 = make_double2(0.0, 0.0), c1 = c0;
-    if (Cc) {
-      const double2* c2 = reinterpret_cast<const double2*>(Cc + (long long)v * C + j);
-      c0 = c2[0];
-      c1 = c2[1];
-    }
-    reinterpret_cast<double2*>(o)[0] = make_double2(c0.x + r[0], c0.y + r[1]);
-    reinterpret_cast<double2*>(o)[1] = make_double2(c1.x + r[2], c1.y + r[3]);
-  } else {
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-      if (m[k]) o[k] = MODE == 2 ? o[k] + r[k] : r[k];
-  }
-}
-
 // Segmented gather over the listed rollouts only: thread (vertex, jj) sums
 // column 3 alist[jj/3] + jj%3 over the vertex's slot range (four partial sums,
 // fixed order), so that a wave reads contiguous slot entries. MODE 0:
@@ -477,49 +402,6 @@ __global__ __launch_bounds__(256) void k_gather_list(const double* __restrict__ 
     B[o] = sum;
   else
     B[o] += sum;
-}
-
-// B = Cc + sum of the vertex's corner slots (fixed order; four partial sums
-// so that the slot loads overlap).
-template <typename T>
-__global__ void k_gather(const double* __restrict__ Cc, const T* __restrict__ corner, const int* __restrict__ vptr,
-                         double* __restrict__ B, int N, int C) {
-  long long idx = (long long)xcd_block() * blockDim.x + threadIdx.x;
-  if (idx >= (long long)N * C) return;
-  int v = (int)(idx / C);
-  int j = (int)(idx % C);
-  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-  int q = vptr[v], qe = vptr[v + 1];
-  for (; q + 4 <= qe; q += 4) {
-    s0 += (double)corner[(long long)q * C + j];
-    s1 += (double)corner[(long long)(q + 1) * C + j];
-    s2 += (double)corner[(long long)(q + 2) * C + j];
-    s3 += (double)corner[(long long)(q + 3) * C + j];
-  }
-  for (; q < qe; ++q) s0 += (double)corner[(long long)q * C + j];
-  B[idx] = (Cc ? Cc[idx] : 0.0) + ((s0 + s1) + (s2 + s3));
-}
-
-// Adjoint gather: Ybar = sum of corners for rollouts active at this
-// iteration; others keep their value.
-template <typename T>
-__global__ void k_gather_masked(const T* __restrict__ corner, const int* __restrict__ vptr,
-                                const int* __restrict__ active, double* __restrict__ B, int N, int C) {
-  long long idx = (long long)xcd_block() * blockDim.x + threadIdx.x;
-  if (idx >= (long long)N * C) return;
-  int v = (int)(idx / C);
-  int j = (int)(idx % C);
-  if (!active[j / 3]) return;
-  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-  int q = vptr[v], qe = vptr[v + 1];
-  for (; q + 4 <= qe; q += 4) {
-    s0 += (double)corner[(long long)q * C + j];
-    s1 += (double)corner[(long long)(q + 1) * C + j];
-    s2 += (double)corner[(long long)(q + 2) * C + j];
-    s3 += (double)corner[(long long)(q + 3) * C + j];
-  }
-  for (; q < qe; ++q) s0 += (double)corner[(long long)q * C + j];
-  B[idx] = (s0 + s1) + (s2 + s3);
 }
 
 // Per supernode s (w columns, m rows, the first w its own) the sweeps use
@@ -2358,26 +2240,6 @@ __global__ void k_cadj_cz(const int* __restrict__ Lptr, const int* __restrict__ 
   const long long o = (long long)v * C + j;
   Xb[o] += sum;
   OB[o] = -sum;
-}
-
-template <typename T>
-__global__ void k_gather_masked_add(const T* __restrict__ corner, const int* __restrict__ vptr,
-                                    const int* __restrict__ active, double* __restrict__ B, int N, int C) {
-  long long idx = (long long)xcd_block() * blockDim.x + threadIdx.x;
-  if (idx >= (long long)N * C) return;
-  int v = (int)(idx / C);
-  int j = (int)(idx % C);
-  if (!active[j / 3]) return;
-  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-  int q = vptr[v], qe = vptr[v + 1];
-  for (; q + 4 <= qe; q += 4) {
-    s0 += (double)corner[(long long)q * C + j];
-    s1 += (double)corner[(long long)(q + 1) * C + j];
-    s2 += (double)corner[(long long)(q + 2) * C + j];
-    s3 += (double)corner[(long long)(q + 3) * C + j];
-  }
-  for (; q < qe; ++q) s0 += (double)corner[(long long)q * C + j];
-  B[idx] += (s0 + s1) + (s2 + s3);
 }
 
 // x_it_{g-1} adjoint = ori adjoint (+ the final-position adjoint for rollouts
