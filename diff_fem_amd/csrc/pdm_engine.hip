@@ -721,7 +721,28 @@ __global__ void k_update(const double* __restrict__ Xn, const double* __restrict
   if (col < C) {
     bool act = active[col / 3] != 0;
     int vend = min(N, (chunk + 1) * kChunk);
-    for (int v = chunk * kChunk + threadIdx.y; v < vend; v += 4) {
+    int v = chunk * kChunk + threadIdx.y;
+    // four rows per trip with their loads issued together; the squares are
+    // added in row order
+    for (; v + 12 < vend; v += 16) {
+      double old[4], nw[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const long long i = (long long)(v + 4 * u) * C + col;
+        old[u] = Xit[i];
+        nw[u] = act ? Xn[i] + B[i] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const long long i = (long long)(v + 4 * u) * C + col;
+        if (!act) nw[u] = old[u];
+        const double d = nw[u] - old[u];
+        acc += d * d;
+        Xit[i] = nw[u];
+        if (tape) tape[i] = nw[u];
+      }
+    }
+    for (; v < vend; v += 4) {
       long long i = (long long)v * C + col;
       double old = Xit[i];
       double nw = act ? Xn[i] + B[i] : old;
@@ -1906,12 +1927,30 @@ __global__ void k_contact_begin(const double* __restrict__ Xit, double* __restri
   double mx = 0.0;
   if (col < C && iactive[col / 3]) {
     int vend = min(N, (chunk + 1) * kChunk);
-    for (int v = chunk * kChunk + threadIdx.y; v < vend; v += 4) {
+    int v = chunk * kChunk + threadIdx.y;
+    for (; v + 12 < vend; v += 16) {  // four rows per trip, loads issued together
+      double x[4], o[4], xn[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const long long i = (long long)(v + 4 * u) * C + col;
+        x[u] = Xit[i];
+        o[u] = Ori[i];
+        xn[u] = Xn[i];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const long long i = (long long)(v + 4 * u) * C + col;
+        mx = fmax(mx, fabs(x[u] - o[u]));
+        Ori[i] = x[u];
+        Dn[i] = x[u] - xn[u];  // ori - x_n, read by the neighbours' Jacobi update
+      }
+    }
+    for (; v < vend; v += 4) {
       long long i = (long long)v * C + col;
       double x = Xit[i];
       mx = fmax(mx, fabs(x - Ori[i]));
       Ori[i] = x;
-      Dn[i] = x - Xn[i];  // ori - x_n, read by the neighbours' Jacobi update
+      Dn[i] = x - Xn[i];
     }
   }
   red[threadIdx.y][threadIdx.x] = mx;
