@@ -209,27 +209,31 @@ __global__ void k_drag_gather(const double* __restrict__ GXf, const double* __re
 }
 
 // Active-rollout list: alist[0..*acount) = the rollouts r with active[r] != 0,
-// ascending (one block; a block-wide scan per chunk of 1024 flags).
+// ascending.
 __global__ __launch_bounds__(1024) void k_compact(const int* __restrict__ active, int R, int* __restrict__ alist,
                                                   int* __restrict__ acount) {
-  __shared__ int sc[1024];
+  // one block; per 1024 rollouts: a wave ballot gives each listed rollout its
+  // rank in its wave, the waves' counts (in wave order) its offset
+  __shared__ int wsum[16];
   __shared__ int base;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   if (threadIdx.x == 0) base = 0;
   __syncthreads();
   for (int c0 = 0; c0 < R; c0 += 1024) {
-    int i = c0 + threadIdx.x;
-    int f = (i < R && active[i] != 0) ? 1 : 0;
-    sc[threadIdx.x] = f;
+    const int i = c0 + threadIdx.x;
+    const bool f = i < R && active[i] != 0;
+    const unsigned long long m = __ballot(f);
+    if (lane == 0) wsum[wave] = __popcll(m);
     __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-      int v = threadIdx.x >= off ? sc[threadIdx.x - off] : 0;
-      __syncthreads();
-      sc[threadIdx.x] += v;
-      __syncthreads();
+    int off = base + __popcll(m & ((1ull << lane) - 1ull));
+    for (int w = 0; w < wave; ++w) off += wsum[w];
+    if (f) alist[off] = i;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int t = 0;
+      for (int w = 0; w < 16; ++w) t += wsum[w];
+      base += t;
     }
-    if (f) alist[base + sc[threadIdx.x] - 1] = i;
-    __syncthreads();
-    if (threadIdx.x == 1023) base += sc[1023];
     __syncthreads();
   }
   if (threadIdx.x == 0) *acount = base;
