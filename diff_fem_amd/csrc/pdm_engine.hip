@@ -2412,6 +2412,12 @@ struct pdm_scene_s {
   DevBuf<double> ckX, ckV;      // checkpoints
   DevBuf<double> tY, tXn, tVn;  // segment tape (tVn: step-start velocities, drag only)
   DevBuf<int> tIters;           // per step of segment: R
+  // per forward call: PD iterations, contact passes and Jacobi iterations per
+  // (step, rollout); the reduced E gradient of the backward. Kept between
+  // calls: a free inside a call would synchronise the device, stalling other
+  // scenes' streams.
+  DevBuf<int> it_all, np_all, ci_all;
+  DevBuf<double> gEd;
   DevBuf<double> act_dev;       // activations of the recorded forward
   bool have_tape = false;
   int tape_steps = 0;
@@ -3416,7 +3422,9 @@ int forward_impl(pdm_scene s, int steps, const double* x0, const double* v0, con
     s->act_null = act == nullptr;
   }
   if (s->iters_host.size() < (size_t)s->R * steps) s->iters_host.resize((size_t)s->R * steps);
-  DevBuf<int> it_all, np_all, ci_all;
+  DevBuf<int>& it_all = s->it_all;
+  DevBuf<int>& np_all = s->np_all;
+  DevBuf<int>& ci_all = s->ci_all;
   if (it_all.alloc((size_t)s->R * std::max(steps, 1))) return 1;
   if (s->ncv > 0 && (np_all.alloc((size_t)s->R * std::max(steps, 1)) || ci_all.alloc((size_t)s->R * std::max(steps, 1))))
     return 1;
@@ -3688,7 +3696,7 @@ int backward_impl(pdm_scene s, const double* gxT, const double* gvT, double* gx0
   if (gv0)
     hipLaunchKernelGGL(k_engine_to_caller, dim3(cdiv(nc, kThreads)), dim3(kThreads), 0, s->stream, s->gV.p, gv0,
                        s->perm.p, N, R);
-  DevBuf<double> gEd;
+  DevBuf<double>& gEd = s->gEd;
   if (gEd.alloc(S.T)) return 1;
   hipLaunchKernelGGL(k_reduce_rows, dim3(cdiv(S.T, kThreads)), dim3(kThreads), 0, s->stream, s->gEpart.p,
                      s->tet_order.p, gEd.p, S.T, R);
@@ -3704,7 +3712,9 @@ int backward_impl(pdm_scene s, const double* gxT, const double* gvT, double* gx0
     // mMu = E/(1+nu_t): dL/dnu (one nu shifted for every tet) = -sum_t E_t dL/dE_t / (1+nu_t)
     double val = 0;
     for (int t = 0; t < S.T; ++t) val -= S.E[t] * gEh[S.tet_order[t]] / (1.0 + S.nu_fwd_t[t]);
-    HIPX(hipMemcpy(gnu, &val, 8, hipMemcpyHostToDevice));
+    // on the scene's stream: a null-stream copy would wait for every other stream
+    HIPX(hipMemcpyAsync(gnu, &val, 8, hipMemcpyHostToDevice, s->stream));
+    HIPX(hipStreamSynchronize(s->stream));
   }
   float ms = 0;
   HIPX(hipEventElapsedTime(&ms, s->ev0, s->ev1));
