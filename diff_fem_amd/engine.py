@@ -269,7 +269,11 @@ class Scene:
         gv0 = torch.empty_like(gx0)
         gE = torch.empty(self.T, dtype=torch.float64, device=gx.device)
         gnu = torch.empty(1, dtype=torch.float64, device=gx.device)
-        gact = torch.zeros((R, max(self._steps, 1), max(self.n_groups, 1)), dtype=torch.float64, device=gx.device)
+        # filled in full by the library when the scene has muscle groups; torch.empty
+        # launches nothing on torch's stream (a fill there would wait for other
+        # scenes' streams)
+        alloc = torch.empty if self.n_groups else torch.zeros
+        gact = alloc((R, max(self._steps, 1), max(self.n_groups, 1)), dtype=torch.float64, device=gx.device)
         check(_lib.load().pdm_backward(self.h, _vp(gx.contiguous()), _vp(gv.contiguous()), _vp(gx0), _vp(gv0),
                                        _vp(gE), _vp(gnu), _vp(gact) if self.n_groups else None))
         return dict(x0=gx0, v0=gv0, E=gE, nu=gnu, act=gact[:, :self._steps, :self.n_groups])
