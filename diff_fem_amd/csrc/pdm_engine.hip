@@ -1389,58 +1389,79 @@ __global__ void k_contact_reset(int* __restrict__ win_st, int* __restrict__ win_
 constexpr int kSelfImp = 8;    // impacts kept per (vertex, rollout) in a step
 constexpr int kSelfMem = 16;   // vertices of one self-contact group
 
-__device__ inline double sgn_d(double x) { return x < 0 ? -1.0 : 1.0; }
-// ArcSim's root finders (collision.cpp:151-170, 280-326)
-__device__ int solve_quadratic_d(double a, double b, double c, double x[2]) {
-  double d = b * b - 4 * a * c;
-  if (d < 0) {
-    x[0] = -b / (2 * a);
-    return 0;
-  }
-  double q = -(b + sgn_d(b) * sqrt(d)) / 2;
-  int i = 0;
-  if (fabs(a) > 1e-12 * fabs(q)) x[i++] = q / a;
-  if (fabs(q) > 1e-12 * fabs(c)) x[i++] = c / q;
-  if (i == 2 && x[0] > x[1]) {
-    double t = x[0];
-    x[0] = x[1];
-    x[1] = t;
-  }
-  return i;
-}
-__device__ double newtons_method_d(double a, double b, double c, double d, double x0, int init_dir) {
-  if (init_dir != 0) {
-    double y0 = d + x0 * (c + x0 * (b + x0 * a)), ddy0 = 2 * b + x0 * (6 * a);
-    x0 += init_dir * sqrt(fabs(2 * y0 / ddy0));
-  }
-  for (int iter = 0; iter < 100; iter++) {
-    double y = d + x0 * (c + x0 * (b + x0 * a));
-    double dy = c + x0 * (2 * b + x0 * 3 * a);
-    if (dy == 0) return x0;
-    double x1 = x0 - y / dy;
-    if (fabs(x0 - x1) < 1e-6) return x0;
-    x0 = x1;
-  }
-  return x0;
-}
-__device__ int solve_cubic_d(double a, double b, double c, double d, double x[3]) {
-  double xc[2] = {0.0, 0.0};
-  int ncrit = solve_quadratic_d(3 * a, 2 * b, c, xc);
-  if (ncrit == 0) {
-    x[0] = newtons_method_d(a, b, c, d, xc[0], 0);
+// Real roots in [0, 1] of f(t) = a3 t^3 + a2 t^2 + a1 t + a0, ascending: the
+// times at which a moving vertex is coplanar with a moving face. [0, 1] is
+// cut at the stationary points of f inside it (roots of f' by the
+// cancellation-free quadratic formula), so f is monotone on every piece. A
+// piece whose end values differ in sign holds one root, found by bisection
+// down to an interval of width 2^-40 (< 1e-12) and reported as its midpoint;
+// a piece end where f vanishes is a root itself (reported once). An
+// identically zero f is coplanar throughout: its earliest root t = 0.
+__device__ int cubic_roots01(double a3, double a2, double a1, double a0, double t[3]) {
+  auto f = [&](double x) { return ((a3 * x + a2) * x + a1) * x + a0; };
+  if (a3 == 0.0 && a2 == 0.0 && a1 == 0.0 && a0 == 0.0) {
+    t[0] = 0.0;
     return 1;
-  } else if (ncrit == 1) {
-    return solve_quadratic_d(b, c, d, x);
   }
-  double yc[2] = {d + xc[0] * (c + xc[0] * (b + xc[0] * a)), d + xc[1] * (c + xc[1] * (b + xc[1] * a))};
-  int i = 0;
-  if (yc[0] * a >= 0) x[i++] = newtons_method_d(a, b, c, d, xc[0], -1);
-  if (yc[0] * yc[1] <= 0) {
-    int closer = fabs(yc[0]) < fabs(yc[1]) ? 0 : 1;
-    x[i++] = newtons_method_d(a, b, c, d, xc[closer], closer == 0 ? 1 : -1);
+  double cut[4];
+  int nc = 0;
+  cut[nc++] = 0.0;
+  {
+    // f'(t) = A t^2 + B t + c
+    const double A = 3.0 * a3, B = 2.0 * a2, c = a1;
+    double s0 = 2.0, s1 = 2.0;  // stationary points (2: none)
+    if (A == 0.0) {
+      if (B != 0.0) s0 = -c / B;
+    } else {
+      const double disc = B * B - 4.0 * A * c;
+      if (disc > 0.0) {
+        const double q = -0.5 * (B + copysign(sqrt(disc), B));
+        s0 = q / A;
+        if (q != 0.0) s1 = c / q;
+        if (s1 < s0) {
+          double tmp = s0;
+          s0 = s1;
+          s1 = tmp;
+        }
+      }
+    }
+    if (s0 > 0.0 && s0 < 1.0) cut[nc++] = s0;
+    if (s1 > 0.0 && s1 < 1.0 && s1 != s0) cut[nc++] = s1;
   }
-  if (yc[1] * a <= 0) x[i++] = newtons_method_d(a, b, c, d, xc[1], 1);
-  return i;
+  cut[nc++] = 1.0;
+  int n = 0;
+  double fl = f(cut[0]);
+  for (int k = 0; k + 1 < nc; ++k) {
+    double lo = cut[k], hi = cut[k + 1];
+    const double fh = f(hi);
+    const double flo = fl;
+    fl = fh;
+    double r;
+    if (flo == 0.0) {
+      r = lo;
+    } else if (fh == 0.0) {
+      r = hi;
+    } else if ((flo < 0.0) != (fh < 0.0)) {
+      const bool neg_lo = flo < 0.0;
+      for (int it = 0; it < 40; ++it) {
+        const double mid = 0.5 * (lo + hi);
+        const double fm = f(mid);
+        if (fm == 0.0) {
+          lo = hi = mid;
+          break;
+        }
+        if ((fm < 0.0) == neg_lo)
+          lo = mid;
+        else
+          hi = mid;
+      }
+      r = 0.5 * (lo + hi);
+    } else {
+      continue;
+    }
+    if (n == 0 || r != t[n - 1]) t[n++] = r;
+  }
+  return n;
 }
 
 // Moving vertex (p0 -> p1) against a moving face (y0[k] -> y1[k]) with
@@ -1481,7 +1502,7 @@ __device__ bool vf_moving_d(const double* p0, const double* p1, const double* co
   double a2 = stp3(x[0], v[1], v[2]) + stp3(v[0], x[1], v[2]) + stp3(v[0], v[1], x[2]);
   double a3 = stp3(v[0], v[1], v[2]);
   double ts[4];
-  int nsol = solve_cubic_d(a3, a2, a1, a0, ts);
+  int nsol = cubic_roots01(a3, a2, a1, a0, ts);
   ts[nsol++] = 1;
   double c[3];
   {

@@ -70,7 +70,7 @@ def lib():
         L.orc_set_underwater.argtypes = [C.c_void_p, C.c_int]
         L.orc_set_tris.argtypes = [C.c_void_p, C.c_int, _dp]
         L.orc_set_self_collision.argtypes = [C.c_void_p, C.c_int, C.c_double]
-        L.orc_solve_cubic.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, _dp]
+        L.orc_cubic_roots01.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, _dp]
         L.orc_vf_moving.argtypes = [_dp, _dp, _dp, _dp, _dp, _dp, _dp]
         L.orc_self_group.argtypes = [_dp, _dp, _dp, C.c_int, _dp, _dp, C.c_double, _dp, _dp]
         L.orc_detect_tri.argtypes = [_dp, _dp, _dp, _dp]
@@ -317,9 +317,10 @@ def volume_projection_jac(S):
     return D, J, r
 
 
-def solve_cubic(a, b, c, d):
+def cubic_roots01(a, b, c, d):
+    """Real roots in [0, 1] of a t^3 + b t^2 + c t + d, ascending (to 1e-12)."""
     x = np.zeros(3)
-    n = lib().orc_solve_cubic(float(a), float(b), float(c), float(d), x)
+    n = lib().orc_cubic_roots01(float(a), float(b), float(c), float(d), x)
     return x[:n]
 
 

@@ -843,56 +843,64 @@ bool detect_tri(const double* tri, const double* x0, const double* x1, double* n
 }
 
 // ---------------------------------------------------------------- self contact
-// ArcSim's polynomial root finders (collision.cpp:151-170, 280-326), restated:
-// the roots in [lo, hi] order of a x^3 + b x^2 + c x + d used by the moving
-// vertex-face test.
-static double sgn_d(double x) { return x < 0 ? -1.0 : 1.0; }
-static int solve_quadratic_as(double a, double b, double c, double x[2]) {
-  double d = b * b - 4 * a * c;
-  if (d < 0) {
-    x[0] = -b / (2 * a);
-    return 0;
-  }
-  double q = -(b + sgn_d(b) * std::sqrt(d)) / 2;
-  int i = 0;
-  if (std::fabs(a) > 1e-12 * std::fabs(q)) x[i++] = q / a;
-  if (std::fabs(q) > 1e-12 * std::fabs(c)) x[i++] = c / q;
-  if (i == 2 && x[0] > x[1]) std::swap(x[0], x[1]);
-  return i;
-}
-static double newtons_method_as(double a, double b, double c, double d, double x0, int init_dir) {
-  if (init_dir != 0) {
-    double y0 = d + x0 * (c + x0 * (b + x0 * a)), ddy0 = 2 * b + x0 * (6 * a);
-    x0 += init_dir * std::sqrt(std::fabs(2 * y0 / ddy0));
-  }
-  for (int iter = 0; iter < 100; iter++) {
-    double y = d + x0 * (c + x0 * (b + x0 * a));
-    double dy = c + x0 * (2 * b + x0 * 3 * a);
-    if (dy == 0) return x0;
-    double x1 = x0 - y / dy;
-    if (std::fabs(x0 - x1) < 1e-6) return x0;
-    x0 = x1;
-  }
-  return x0;
-}
-static int solve_cubic_as(double a, double b, double c, double d, double x[3]) {
-  double xc[2] = {0.0, 0.0};
-  int ncrit = solve_quadratic_as(3 * a, 2 * b, c, xc);
-  if (ncrit == 0) {
-    x[0] = newtons_method_as(a, b, c, d, xc[0], 0);
+// Coplanarity times of the moving vertex-face test (collision.cpp:211-278
+// takes the roots of this cubic in [0, 1] in ascending order): the real roots
+// in [0, 1] of f(t) = a3 t^3 + a2 t^2 + a1 t + a0. f is monotone between its
+// stationary points; each monotone piece of [0, 1] with a sign change holds
+// one root, bracketed and bisected to a width of 2^-40 (< 1e-12, the stated
+// tolerance), reported as the bracket's midpoint. Zeros at piece ends are
+// roots (reported once); f identically zero gives t = 0.
+static int cubic_roots01(double a3, double a2, double a1, double a0, double t[3]) {
+  auto f = [&](double x) { return ((a3 * x + a2) * x + a1) * x + a0; };
+  if (a3 == 0.0 && a2 == 0.0 && a1 == 0.0 && a0 == 0.0) {
+    t[0] = 0.0;
     return 1;
-  } else if (ncrit == 1) {
-    return solve_quadratic_as(b, c, d, x);
   }
-  double yc[2] = {d + xc[0] * (c + xc[0] * (b + xc[0] * a)), d + xc[1] * (c + xc[1] * (b + xc[1] * a))};
-  int i = 0;
-  if (yc[0] * a >= 0) x[i++] = newtons_method_as(a, b, c, d, xc[0], -1);
-  if (yc[0] * yc[1] <= 0) {
-    int closer = std::fabs(yc[0]) < std::fabs(yc[1]) ? 0 : 1;
-    x[i++] = newtons_method_as(a, b, c, d, xc[closer], closer == 0 ? 1 : -1);
+  std::vector<double> cut{0.0};
+  std::vector<double> st;  // stationary points: roots of 3 a3 t^2 + 2 a2 t + a1
+  const double A = 3.0 * a3, B = 2.0 * a2, c = a1;
+  if (A == 0.0) {
+    if (B != 0.0) st.push_back(-c / B);
+  } else {
+    const double disc = B * B - 4.0 * A * c;
+    if (disc > 0.0) {
+      const double q = -0.5 * (B + std::copysign(std::sqrt(disc), B));
+      st.push_back(q / A);
+      if (q != 0.0) st.push_back(c / q);
+    }
   }
-  if (yc[1] * a <= 0) x[i++] = newtons_method_as(a, b, c, d, xc[1], 1);
-  return i;
+  std::sort(st.begin(), st.end());
+  for (double x : st)
+    if (x > 0.0 && x < 1.0 && x != cut.back()) cut.push_back(x);
+  cut.push_back(1.0);
+  int n = 0;
+  for (size_t k = 0; k + 1 < cut.size(); ++k) {
+    double lo = cut[k], hi = cut[k + 1];
+    const double flo = f(lo), fhi = f(hi);
+    double r;
+    if (flo == 0.0) {
+      r = lo;
+    } else if (fhi == 0.0) {
+      r = hi;
+    } else if ((flo < 0.0) != (fhi < 0.0)) {
+      for (int it = 0; it < 40; ++it) {
+        const double mid = 0.5 * (lo + hi), fm = f(mid);
+        if (fm == 0.0) {
+          lo = hi = mid;
+          break;
+        }
+        if ((fm < 0.0) == (flo < 0.0))
+          lo = mid;
+        else
+          hi = mid;
+      }
+      r = 0.5 * (lo + hi);
+    } else {
+      continue;
+    }
+    if (n == 0 || r != t[n - 1]) t[n++] = r;
+  }
+  return n;
 }
 
 // ArcSim's kDOP-18 box test (single precision) between the swept vertex and
@@ -949,7 +957,7 @@ bool vf_moving(const double* p0, const double* p1, const double* const* y0, cons
   double a2 = triple(x[0], v[1], v[2]) + triple(v[0], x[1], v[2]) + triple(v[0], v[1], x[2]);
   double a3 = triple(v[0], v[1], v[2]);
   double ts[10];
-  int nsol = solve_cubic_as(a3, a2, a1, a0, ts);
+  int nsol = cubic_roots01(a3, a2, a1, a0, ts);
   ts[nsol++] = 1;
   for (int i = 0; i < nsol; i++) {
     double t = ts[i];
@@ -2063,7 +2071,7 @@ void orc_set_self_collision(void* p, int on, double mu) {
     s->sc_face.push_back(cf);
   }
 }
-int orc_solve_cubic(double a, double b, double c, double d, double* x) { return solve_cubic_as(a, b, c, d, x); }
+int orc_cubic_roots01(double a, double b, double c, double d, double* x) { return cubic_roots01(a, b, c, d, x); }
 int orc_vf_moving(const double* p0, const double* p1, const double* y0, const double* y1, double* t, double* d,
                   double* n) {
   const double* a0[3] = {y0, y0 + 3, y0 + 6};

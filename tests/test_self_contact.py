@@ -2,7 +2,7 @@
 body's own surface faces; collisionBrutal.h:281-406, 643-700; World.h:1246-1315,
 1446-1600), on the CPU.
 
-Pins: ArcSim's cubic solver against numpy's roots; the moving vertex-face test
+Pins: the coplanarity-time root finder against numpy's roots; the moving vertex-face test
 is Galilean invariant and agrees with the static-face test; one group's
 response balances its normal reactions (and all of them when everything
 sticks), gives a stuck vertex the face's common velocity and a sliding one
@@ -18,20 +18,38 @@ import pytest
 from diff_fem_amd import scene
 
 
-def test_cubic_solver_matches_numpy(oracle_mod):
+def test_cubic_roots01_matches_numpy(oracle_mod):
+    """The coplanarity-time root finder returns exactly the real roots of the
+    cubic that lie in [0, 1], ascending, each within its 1e-12 tolerance:
+    random cubics with roots inside and outside [0, 1] against numpy's
+    companion-matrix roots, plus the degenerate leading terms."""
     rng = np.random.default_rng(2)
-    for _ in range(200):
-        r = np.sort(rng.uniform(0, 1, 3))
-        if np.min(np.diff(r)) < 0.05:
+    checked = 0
+    for _ in range(400):
+        r = np.sort(rng.uniform(-0.5, 1.5, 3))
+        if np.min(np.diff(r)) < 0.05 or np.min(np.abs(r)) < 1e-3 or np.min(np.abs(r - 1)) < 1e-3:
             continue
-        a = rng.uniform(0.5, 3) * rng.choice([-1, 1])
+        a = rng.uniform(0.5, 3) * rng.choice([-1, 1]) * 10.0 ** rng.uniform(-9, 0)
         co = a * np.poly(r)
-        x = oracle_mod.solve_cubic(*co)
-        assert x.size == 3
-        np.testing.assert_allclose(x, r, atol=1e-6)
+        x = oracle_mod.cubic_roots01(*co)
+        want = r[(r >= 0) & (r <= 1)]
+        assert x.size == want.size, (r, x)
+        np.testing.assert_allclose(x, want, rtol=0, atol=2e-12)
+        checked += 1
+    assert checked > 100
+    # complex pair plus one real root in [0, 1]: (t - 0.3)(t^2 + 1)
+    np.testing.assert_allclose(oracle_mod.cubic_roots01(1.0, -0.3, 1.0, -0.3), [0.3], atol=1e-12)
     # leading coefficient zero: the quadratic's roots
-    x = oracle_mod.solve_cubic(0.0, 2.0, -3.0, 1.0)
-    np.testing.assert_allclose(np.sort(x), [0.5, 1.0], atol=1e-12)
+    np.testing.assert_allclose(oracle_mod.cubic_roots01(0.0, 2.0, -3.0, 1.0), [0.5, 1.0], atol=1e-12)
+    # linear (a static face: t = -a0 / a1)
+    np.testing.assert_allclose(oracle_mod.cubic_roots01(0.0, 0.0, -4.0, 1.0), [0.25], atol=1e-12)
+    # a double root at a stationary point, (t - 0.5)^2 (t - 2), reported once
+    np.testing.assert_allclose(oracle_mod.cubic_roots01(1.0, -3.0, 2.25, -0.5), [0.5], atol=1e-12)
+    # roots exactly at the ends of [0, 1]: t (t - 1) (t - 3)
+    np.testing.assert_allclose(oracle_mod.cubic_roots01(1.0, -4.0, 3.0, 0.0), [0.0, 1.0], atol=0)
+    # no root in [0, 1]; coplanar throughout (earliest time 0)
+    assert oracle_mod.cubic_roots01(1.0, 0.0, 0.0, 1.0).size == 0
+    np.testing.assert_array_equal(oracle_mod.cubic_roots01(0.0, 0.0, 0.0, 0.0), [0.0])
 
 
 def _face():
@@ -53,9 +71,9 @@ def test_vf_moving_is_galilean_invariant(oracle_mod):
         if h1:
             hits += 1
             # (y - d) - y differs from -d by rounding: the cubic and quadratic
-            # terms are rounding noise instead of zero, Newton starts elsewhere,
-            # and its 1e-6 stopping rule returns the previous iterate
-            assert abs(t1 - t2) < 1e-6 and abs(d1 - d2) < 1e-8
+            # terms are rounding noise instead of zero, which moves the root
+            # by far less than this bound
+            assert abs(t1 - t2) < 1e-9 and abs(d1 - d2) < 1e-8
             np.testing.assert_allclose(n1, n2, atol=1e-12)
     assert hits > 10
 
