@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -2322,6 +2323,43 @@ __global__ void k_cadj_post(const double* __restrict__ OB, const double* __restr
 }
 
 // ------------------------------------------------------------------ engine
+}  // namespace
+
+// Working memory of pdm_solve_noalloc: one fixed array per GPU, held until the
+// process ends. Host code must not pass `scratch` itself to a kernel (on the
+// host it names a stand-in, not device memory); it uses the device address
+// that noalloc_scratch() obtains once per device with hipGetSymbolAddress.
+static __device__ __align__(256) double scratch[PDM_NOALLOC_SCRATCH_BYTES / sizeof(double)];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (looked up on the first
+// call for that device, with that device current, then kept).
+int noalloc_scratch(double** out) {
+  int dev = -1;
+  HIPX(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) {
+    g_err = "device index out of range for the no-allocation scratch";
+    return 1;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr) {
+      g_err = std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e);
+      return 1;
+    }
+    g_scratch_ptr[dev] = (double*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
 template <typename X>
 struct DevBuf {
   X* p = nullptr;
@@ -2859,9 +2897,9 @@ int ensure_batch_buffers(pdm_scene s) {
 constexpr size_t kLdsSoft = 64 * 1024, kLdsMax = 160 * 1024;
 size_t level_lds(const pdm_scene_s* s, int l, int Ct) { return (size_t)s->lev_rows[l] * Ct * 8; }
 
-pdm_scene_s::LevelCfg level_default(const pdm_scene_s* s, int l) {
+pdm_scene_s::LevelCfg level_default(const pdm_scene_s* s, int l, int C) {
   int cnt = s->lev_ptr[l + 1] - s->lev_ptr[l];
-  int ncol16 = cdiv(s->C, 16);
+  int ncol16 = cdiv(C, 16);
   int nq = 4;
   while (nq > 1 && level_lds(s, l, 16 * nq) > kLdsSoft) nq /= 2;
   while (nq > 1 && (long long)cnt * cdiv(ncol16, nq) < 512) nq /= 2;
@@ -2870,14 +2908,16 @@ pdm_scene_s::LevelCfg level_default(const pdm_scene_s* s, int l) {
   size_t lds = level_lds(s, l, c.Ct);
   int per_cu = (int)std::max<size_t>(1, std::min<size_t>(8, kLdsMax / std::max<size_t>(lds, 1)));
   int waves = std::min(16, std::max(4, 16 / per_cu));
-  if ((long long)cnt * cdiv(s->C, c.Ct) < 256) waves = 16;
+  if ((long long)cnt * cdiv(C, c.Ct) < 256) waves = 16;
   c.nthr = 64 * waves;
   return c;
 }
+pdm_scene_s::LevelCfg level_default(const pdm_scene_s* s, int l) { return level_default(s, l, s->C); }
 
-// One level of one sweep over the columns [cbase, cbase + cw) on stream st.
-int launch_level(pdm_scene s, int pass, int l, const pdm_scene_s::LevelCfg& c, double* Bp, const int* any, int cbase,
-                 int cw, hipStream_t st) {
+// One level of one sweep over the columns [cbase, cbase + cw) of C on stream
+// st, with the update blocks in Up.
+int launch_level_into(pdm_scene s, int pass, int l, const pdm_scene_s::LevelCfg& c, int C, double* Bp, double* Up,
+                      const int* any, int cbase, int cw, hipStream_t st) {
   const FactorView fv = s->view();
   int cnt = s->lev_ptr[l + 1] - s->lev_ptr[l];
   size_t lds = level_lds(s, l, c.Ct);
@@ -2894,11 +2934,15 @@ int launch_level(pdm_scene s, int pass, int l, const pdm_scene_s::LevelCfg& c, d
   }
   if (pass == 0)
     hipLaunchKernelGGL(k_fwd_level, dim3(cnt * ntile), dim3(c.nthr), lds, st, fv, s->f_lev.p + s->lev_ptr[l], ntile,
-                       lg, s->C, Bp, s->U.p, any, cbase, cw);
+                       lg, C, Bp, Up, any, cbase, cw);
   else
     hipLaunchKernelGGL(k_bwd_level, dim3(cnt * ntile), dim3(c.nthr), lds, st, fv, s->f_lev.p + s->lev_ptr[l], ntile,
-                       lg, s->C, Bp, any, cbase, cw);
+                       lg, C, Bp, any, cbase, cw);
   return 0;
+}
+int launch_level(pdm_scene s, int pass, int l, const pdm_scene_s::LevelCfg& c, double* Bp, const int* any, int cbase,
+                 int cw, hipStream_t st) {
+  return launch_level_into(s, pass, l, c, s->C, Bp, s->U.p, any, cbase, cw, st);
 }
 
 // Subtree kernels of cut level `cut` over the columns [cbase, cbase + cw).
@@ -4069,6 +4113,41 @@ int pdm_solve_host(pdm_scene s, const double* b, double* x) {
                      s->perm.p, s->S.N, s->R);
   HIPX(hipStreamSynchronize(s->stream));
   HIPX(hipMemcpy(x, db.p, n * 8, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int pdm_solve_noalloc(pdm_scene s, int R, const double* b, double* x) {
+  if (!s || !b || !x || R < 1) {
+    g_err = "invalid argument";
+    return 1;
+  }
+  const int N = s->S.N;
+  const unsigned long long C = 3ull * (unsigned long long)R;
+  const unsigned long long urows = (unsigned long long)std::max<long long>(1, s->F.uptr.back());
+  auto up256 = [](unsigned long long v) { return (v + 255ull) & ~255ull; };
+  const unsigned long long bbytes = up256((unsigned long long)N * C * 8ull), ubytes = up256(urows * C * 8ull);
+  if (bbytes + ubytes > PDM_NOALLOC_SCRATCH_BYTES) {
+    g_err = "pdm_solve_noalloc: " + std::to_string(R) + " rollouts need " + std::to_string(bbytes + ubytes) +
+            " bytes of scratch, above PDM_NOALLOC_SCRATCH_BYTES (" + std::to_string(PDM_NOALLOC_SCRATCH_BYTES) + ")";
+    return 1;
+  }
+  double* base = nullptr;
+  if (noalloc_scratch(&base)) return 1;
+  double* Bp = base;
+  double* Up = (double*)((char*)base + bbytes);
+  if (set_level_attrs()) return 1;
+  const int Ci = (int)C;
+  const size_t n = (size_t)N * C;
+  hipLaunchKernelGGL(k_caller_to_engine, dim3(cdiv(n, kThreads)), dim3(kThreads), 0, s->stream, b, Bp, s->perm.p, N,
+                     R);
+  const int nlev = (int)s->lev_ptr.size() - 1;
+  for (int l = 0; l < nlev; ++l)
+    if (launch_level_into(s, 0, l, level_default(s, l, Ci), Ci, Bp, Up, nullptr, 0, Ci, s->stream)) return 1;
+  for (int l = nlev - 1; l >= 0; --l)
+    if (launch_level_into(s, 1, l, level_default(s, l, Ci), Ci, Bp, Up, nullptr, 0, Ci, s->stream)) return 1;
+  hipLaunchKernelGGL(k_engine_to_caller, dim3(cdiv(n, kThreads)), dim3(kThreads), 0, s->stream, Bp, x, s->perm.p, N,
+                     R);
+  HIPX(hipGetLastError());
   return 0;
 }
 

@@ -211,6 +211,25 @@ int pdm_set_solve_plan(pdm_scene s, int subtree_cut, int n_chains);
 /* Timing of the last forward/backward (ms, HIP events on the library stream). */
 int pdm_get_timing(pdm_scene s, double* fwd_ms, double* bwd_ms, double* factor_ms);
 
+/* Largest device scratch pdm_solve_noalloc uses: the engine-layout right-hand
+ * sides (N x 3R doubles) plus the multifrontal update blocks (U_rows x 3R
+ * doubles), each rounded up to 256 B. */
+#define PDM_NOALLOC_SCRATCH_BYTES (64ull << 20)
+
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the solve of pdm_solve_host, (M/h^2 + L) x = b for R
+ * rollouts, on device buffers b and x (R*3N doubles each, caller layout; they
+ * may alias), enqueued on the scene's stream without waiting for it. Its
+ * working memory is one fixed array per GPU of PDM_NOALLOC_SCRATCH_BYTES,
+ * held until the process ends; R is free of the scene's batch but must keep
+ * (N + U_rows) * 3R * 8 B within that size (else an error). It uses the
+ * default solve launch shapes (results equal the tuned solve's: shapes only
+ * partition the work). Every call on one GPU shares that array, so calls
+ * that overlap in time on one GPU (other streams, other threads) race; the
+ * scene's device must be current. */
+int pdm_solve_noalloc(pdm_scene s, int R, const double* b, double* x);
+
 #ifdef __cplusplus
 }
 #endif

@@ -18,6 +18,8 @@ def test_library_exports_header_symbols():
     assert decl and decl == set(_lib.SYMBOLS)
     for name in decl:
         assert hasattr(L, name)
+    m = re.search(r"#define PDM_NOALLOC_SCRATCH_BYTES \((\d+)ull << (\d+)\)", open(HDR).read())
+    assert m and int(m.group(1)) << int(m.group(2)) == _lib.NOALLOC_SCRATCH_BYTES
 
 
 def test_create_without_device_fails_loudly():
