@@ -30,7 +30,11 @@
 //     control flow (iteration counts) frozen as executed.
 //
 // Parity pins live in tests/ (closed forms, numpy SVD, brute-force 3N assembly,
-// dense solves, invariants, central finite differences).
+// dense solves, invariants, central finite differences; the muscle target, the
+// muscle tet selection, the contact Jacobi loop and the regularisation ladder
+// by hand arithmetic, closed forms of their rules and independent computations
+// in tests/test_oracle_pins.py). Parity unpinned: sphere-contact gradients
+// against finite differences (the normal's derivative is frozen, as executed).
 #include <algorithm>
 #include <array>
 #include <cmath>
