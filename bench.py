@@ -44,28 +44,84 @@ def build_workload(args):
     return d
 
 
-def cpu_baseline(d, steps, nthreads):
-    """Oracle (C++ FP64 restatement, single thread) on a bounded sample: one
-    rollout, `steps` steps forward + adjoint."""
+def host_cpu_info():
+    """nproc, the CPU model (lscpu) and the threads this process may use."""
+    import platform
+    import subprocess
+    model = platform.processor() or ""
+    try:
+        for line in subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    try:
+        nproc = int(subprocess.run(["nproc"], capture_output=True, text=True, timeout=20).stdout.strip())
+    except (OSError, subprocess.SubprocessError, ValueError):
+        nproc = os.cpu_count() or 1
+    return dict(nproc=nproc, model=model)
+
+
+def cpu_baseline(d, x_s0, v_s0, acts, s0, steps, threads):
+    """Oracle (C++ FP64 restatement) timed on a bounded sample of the workload:
+    steps [s0, s0 + steps) of the rollouts, starting from their GPU FP64 state
+    at step s0 (the body has reached the floor, so the sample includes contact
+    work), forward + reverse sweep. Single-threaded on one rollout, and OpenMP
+    over `threads` rollouts on that many cores. Setup (mesh, muscle selection,
+    factorisation) is excluded."""
     from oracle import oracle as O
-    from diff_fem_amd import scene
-    o = O.OracleScene(d)
-    acts = scene.activations_sinusoid(4, steps, d.n_groups, 1, d.h)[0]
-    x0 = d.x_rest.ravel().copy()
-    v0 = np.zeros_like(x0)
     t0 = time.perf_counter()
-    xT, vT, its = o.rollout(x0, v0, steps, act=acts, record=True)
-    g, _ = o.backward(np.ones_like(xT), np.zeros_like(vT))
-    dt = time.perf_counter() - t0
-    tet_it = float(d.T * its[:, 0].sum())
-    return dict(value=tet_it / dt, unit="tet-iterations/s", cores=1, kind="port",
-                sample="oracle FP64 C++, 1 rollout x %d steps fwd+adjoint of %s (%.1f s)" % (steps, d.name, dt))
+    o = O.OracleScene(d)
+    setup_s = time.perf_counter() - t0
+    n3 = 3 * d.N
+    res = {}
+    for name, R in (("single", 1), ("multi", threads)):
+        xs = np.ascontiguousarray(np.tile(x_s0, (R, 1)))
+        vs = np.ascontiguousarray(np.tile(v_s0, (R, 1)))
+        a = np.ascontiguousarray(acts[:R, s0:s0 + steps])
+        gx = np.full((R, n3), 1.0 / n3)
+        t0 = time.perf_counter()
+        _, _, its = o.fwd_adj_batch(xs, vs, steps, gx, np.zeros_like(gx), act=a, nthreads=R)
+        dt = time.perf_counter() - t0
+        res[name] = (float(d.T) * float(its.sum()) / dt, dt, R)
+    v1, dt1, _ = res["single"]
+    vm, dtm, Rm = res["multi"]
+    info = host_cpu_info()
+    smp = "oracle FP64 C++, steps %d-%d (ground contact) of the %s rollouts, forward + reverse sweep" % (
+        s0, s0 + steps - 1, d.name)
+    return dict(value=vm, unit="tet-iterations/s", cores=Rm, kind="port",
+                sample=smp + ": %d rollouts over %d threads (%.1f s)" % (Rm, Rm, dtm),
+                single_thread={"value": v1, "cores": 1, "sample": smp + ": 1 rollout (%.1f s)" % dt1},
+                nproc=info["nproc"], cpu_model=info["model"], setup_s=setup_s)
+
+
+def oracle_check(d, x0, v0, acts, steps, gx, gv, rows):
+    """FP64 oracle forward + reverse sweep of the listed rollouts at the full
+    horizon (OpenMP over them): positions and gradients to compare with."""
+    from oracle import oracle as O
+    o = O.OracleScene(d)
+    R = len(rows)
+    xs = np.ascontiguousarray(x0[rows])
+    vs = np.ascontiguousarray(v0[rows])
+    a = np.ascontiguousarray(acts[rows])
+    xT, _ = o.rollout_batch(xs, vs, steps, act=a, nthreads=R)
+    gx0, _, _ = o.fwd_adj_batch(xs, vs, steps, np.ascontiguousarray(gx[rows]), np.ascontiguousarray(gv[rows]),
+                                act=a, nthreads=R)
+    return xT, gx0
+
+
+def rel_rows(a, b):
+    """Per row: max |a - b| / max |b|."""
+    a = np.asarray(a).reshape(a.shape[0], -1)
+    b = np.asarray(b).reshape(b.shape[0], -1)
+    return np.abs(a - b).max(1) / np.maximum(np.abs(b).max(1), 1e-300)
 
 
 def solve_traffic(rhs, nnz_lower):
-    """HBM bytes per solve from the committed counter summary (FETCH_SIZE and
-    WRITE_SIZE runs, scaled by a copy kernel of known size), when it was
-    measured on this same factor and right-hand-side count; else None."""
+    """(HBM bytes per solve, source) from the committed counter summary
+    (FETCH_SIZE and WRITE_SIZE runs, scaled by a copy kernel of known size),
+    when it was measured on this same factor and right-hand-side count; else
+    None. It is not measured in this run: the source names the counter run."""
     path = os.path.join(ROOT, "profiles", "solve_c4_R256_hbm_traffic.md")
     if not os.path.exists(path):
         return None
@@ -76,7 +132,8 @@ def solve_traffic(rhs, nnz_lower):
             vals[k.strip()] = v.strip().split()[0]
     try:
         if int(vals["rhs"]) == rhs and int(vals["nnz_lower"]) == nnz_lower:
-            return float(vals["hbm_bytes_per_solve"])
+            return float(vals["hbm_bytes_per_solve"]), "profiles/solve_c4_R256_hbm_traffic.md (%s)" % (
+                vals.get("measured_by", "rocprofv3 FETCH_SIZE/WRITE_SIZE counter runs"))
     except (KeyError, ValueError):
         pass
     return None
@@ -100,7 +157,11 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=0,
                     help="sim steps of the plain FP64 comparator (0: same as --sim-steps)")
     ap.add_argument("--contact", type=int, default=1)
-    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--cpu-steps", type=int, default=24, help="steps of the CPU baseline sample")
+    ap.add_argument("--cpu-from", type=int, default=14,
+                    help="first step of the CPU sample (the dropped body reaches the floor at about step 14)")
+    ap.add_argument("--cpu-threads", type=int, default=16, help="threads of the multi-core CPU baseline (at most nproc)")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 batch pass (throughput and error figures)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--graph", type=int, default=1,
@@ -130,9 +191,9 @@ def main():
     Rp = R // parts  # rollouts (and 3 Rp right-hand sides) of the first part's solves
     log("setup: T=%d N=%d R=%d (%d part(s)) steps=%d" % (d.T, d.N, R, parts, S))
     t0 = time.perf_counter()
-    flags = engine.PDM_HOST_EARLY_EXIT | engine.PDM_TIME_SOLVES
-    if args.graph:
-        flags |= engine.PDM_USE_GRAPH
+    # graph replays cannot carry per-solve events: with graphs the solves are
+    # timed in one extra eager pass after the timed ones
+    flags = engine.PDM_HOST_EARLY_EXIT | (engine.PDM_USE_GRAPH if args.graph else engine.PDM_TIME_SOLVES)
     if args.solve_only:  # one part's solve: the shape the pipeline launches
         sc = engine.Scene(d, R=Rp, precision=engine.PDM_MIXED, flags=flags, checkpoint_every=args.ckpt)
         parts_list = [sc]
@@ -187,12 +248,14 @@ def main():
         free_b, total_b = torch.cuda.mem_get_info(dev)
         log("warmup %d done; device memory in use %.1f of %.1f GB" % (w, (total_b - free_b) / 1e9, total_b / 1e9))
     sc.solve_timing()  # restart the per-solve event counters
+    g0 = sc.graph_launches()
     n_solve, solve_ms = 0, 0.0
     times, tet_its, fwd_ms, bwd_ms = [], [], [], []
+    g = None
     for k in range(args.steps):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        one_pass()
+        g = one_pass()
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         its = sc.iterations()
@@ -209,6 +272,7 @@ def main():
             k, dt, tm["forward_ms"], tm["backward_ms"], its.mean(), cps.mean(), cjt.mean()))
     total_t = sum(times)
     value = sum(tet_its) / total_t
+    replays = sc.graph_launches() - g0
     out = {
         "metric": "tet-iterations/sec fwd+adjoint, batched mixed vs plain FP64 on GPU at stated err",
         "value": value, "unit": "tet-iterations/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -218,18 +282,33 @@ def main():
                                 if args.workload == "c4" else "c5_packed_64_meshes_R%d_%dsteps_fwd+adjoint" % (R, S)),
                    "tets": d.T, "vertices": d.N, "rollouts": R, "sim_steps": S, "max_iter": d.max_iter,
                    "checkpoint_every": args.ckpt, "contact": bool(d.contact), "nnz_factor": info["nnz_factor"],
-                   "hipgraph": bool(args.graph), "sub_batches": parts,
+                   "hipgraph": replays > 0, "graph_replays_per_pass": replays / max(args.steps, 1),
+                   "sub_batches": parts,
                    "forward_ms": float(np.mean(fwd_ms)), "adjoint_ms": float(np.mean(bwd_ms))},
     }
+    if args.graph:
+        # Solves inside a replayed graph cannot be bracketed by events: one
+        # more pass, eager with the same concurrency, times every solve.
+        for p in parts_list:
+            p.set_batch(p.R, engine.PDM_MIXED, engine.PDM_HOST_EARLY_EXIT | engine.PDM_TIME_SOLVES, args.ckpt)
+        sc.solve_timing()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        one_pass()
+        torch.cuda.synchronize()
+        t_eager = time.perf_counter() - t0
+        n_solve, solve_ms = sc.solve_timing()
+        out["config"]["eager_pass_s"] = t_eager
+        log("instrumented eager pass: %.3fs, %d solves" % (t_eager, n_solve))
     # Roofline of the dominant kernel: the supernodal triangular solve
     # (k_fwd_level + k_bwd_level over all levels, one part's 3R/parts
-    # right-hand sides per launch sequence), the largest share of kernel time.
-    # Its mean duration comes from HIP events around every solve of the timed
-    # passes, on each part's stream (so it includes the other part's kernels
-    # sharing the GPU meanwhile). At 3R/parts = 384 it is FP64-MFMA bound by
-    # arithmetic intensity: algorithmic flops = 2 sweeps x 2 x nnz(L, lower
-    # trapezoid) x RHS; algorithmic bytes = factor read twice + right-hand
-    # sides read and written once per sweep.
+    # right-hand sides per launch sequence). Its mean duration comes from HIP
+    # events around every solve of a forward+adjoint pass, on each part's
+    # stream (so it includes the other part's kernels sharing the GPU
+    # meanwhile). At 3R/parts = 384 it is FP64-MFMA bound by arithmetic
+    # intensity: algorithmic flops = 2 sweeps x 2 x nnz(L, lower trapezoid) x
+    # RHS; algorithmic bytes = factor read twice + right-hand sides read and
+    # written once per sweep.
     ms_live = solve_ms / max(n_solve, 1)
     bytes_alg = 2.0 * info["nnz_lower"] * 8 + 2.0 * 2 * 3 * Rp * d.N * 8
     if flops / bytes_alg >= 78.6e12 / 8.0e12:  # above the FP64 ridge: matrix-core bound
@@ -238,7 +317,8 @@ def main():
     else:  # few right-hand sides: the factor stream bounds it
         achieved = bytes_alg / (ms_live * 1e-3) / 1e9
         rl = {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s", "frac": achieved / 8000.0}
-    rl.update({"traffic": solve_traffic(3 * Rp, info["nnz_lower"]),
+    tr = solve_traffic(3 * Rp, info["nnz_lower"])
+    rl.update({"traffic": tr[0] if tr else None, "traffic_source": tr[1] if tr else None,
                "kernel": "supernodal solve k_fwd_level+k_bwd_level, %d RHS" % (3 * Rp),
                "column_chains": p0.solve_chains,
                "ms_per_solve": ms_live, "solves_timed": n_solve,
@@ -248,8 +328,64 @@ def main():
     rl["frac_back_to_back"] = flops / (rl["ms_per_solve_back_to_back"] * 1e-3) / 1e12 / 78.6 \
         if rl["bound"] == "mfma" else bytes_alg / (rl["ms_per_solve_back_to_back"] * 1e-3) / 1e9 / 8000.0
     out["roofline"] = rl
+    # Whole-pass HBM fraction of the metric's own definition: algorithmic
+    # bytes per local/global iteration for R rollouts (local step with FP32
+    # local data, s = 4, plus both triangular sweeps), per executed
+    # tet-iteration, counted for the forward and again for the adjoint, plus
+    # the checkpoint traffic; over the timed pass time and 8 TB/s.
+    if args.workload == "c4":
+        s_b = 4.0
+        b_local = d.T * (16 + 10 * s_b) + R * 3 * d.N * s_b + R * 3 * d.N * 8
+        b_solve = 2.0 * info["nnz_lower"] * 8 + 2.0 * R * 3 * d.N * 8
+        per_tet_it = (b_local + b_solve) / (R * d.T)
+        b_pass = 2.0 * per_tet_it * (sum(tet_its) / len(tet_its)) + (S / args.ckpt) * 2.0 * R * 3 * d.N * 8
+        t_pass = total_t / args.steps
+        out["pass_roofline"] = {"bound": "hbm", "achieved": b_pass / t_pass / 1e9, "peak": 8000.0, "unit": "GB/s",
+                                "frac": b_pass / t_pass / 8.0e12, "bytes_alg_per_pass": b_pass,
+                                "bytes_alg_per_tet_iteration": per_tet_it}
+    tet_its_pass = sum(tet_its) / len(tet_its)
+    x_mixed = xT.cpu().numpy()
+    g_mixed = {k: g[k].cpu().numpy() for k in ("x0", "v0", "act", "E")}
+    if args.workload == "c4" and not args.no_fp64:
+        # The same batch in FP64 everywhere (same scene, buffers and flags):
+        # its throughput, and the reference the mixed errors are measured on.
+        log("FP64 batch (R = %d) ..." % R)
+        fl64 = engine.PDM_HOST_EARLY_EXIT | (engine.PDM_USE_GRAPH if args.graph else 0)
+        for p in parts_list:
+            p.set_batch(p.R, engine.PDM_FP64, fl64, args.ckpt)
+        one_pass()  # warm-up: first launches of the FP64 kernels
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        g64 = one_pass()
+        torch.cuda.synchronize()
+        t64 = time.perf_counter() - t0
+        its64 = sc.iterations()
+        x64 = xT.cpu().numpy()
+        g64 = {k: g64[k].cpu().numpy() for k in ("x0", "v0", "act", "E")}
+        ep = rel_rows(x_mixed, x64)
+        eg = np.maximum.reduce([rel_rows(g_mixed[k], g64[k]) for k in ("x0", "v0", "act")])
+        eE = float(np.abs(g_mixed["E"] - g64["E"]).max() / np.abs(g64["E"]).max())
+        out["fp64_batch"] = {"value": float(d.T) * float(its64.sum()) / t64, "unit": "tet-iterations/s",
+                             "rollouts": R, "s_per_pass": t64}
+        out["err"] = {"vs": "FP64 GPU batch, same inputs, all rollouts, full horizon",
+                      "pos_max": float(ep.max()), "grad_max": float(eg.max()), "grad_E": eE,
+                      "rollouts_pos_above_1e-5": int((ep > 1e-5).sum()),
+                      "rollouts_grad_above_1e-4": int((eg > 1e-4).sum()),
+                      "same_iterations": bool(np.array_equal(its, its64)), "grads": "x0, v0, activations per rollout"}
+        if not args.no_cpu:
+            rows = [0, R - 1]
+            log("oracle check of rollouts %s at %d steps ..." % (rows, S))
+            xo, go = oracle_check(d, x0.cpu().numpy(), v0.cpu().numpy(), acts.cpu().numpy(), S,
+                                  gx.cpu().numpy(), gv.cpu().numpy(), rows)
+            out["err"]["oracle"] = {"rollouts": rows,
+                                    "fp64_pos": float(rel_rows(x64[rows], xo).max()),
+                                    "fp64_grad_x0": float(rel_rows(g64["x0"][rows], go).max()),
+                                    "mixed_pos": float(rel_rows(x_mixed[rows], xo).max()),
+                                    "mixed_grad_x0": float(rel_rows(g_mixed["x0"][rows], go).max())}
+        log("errors: %s" % (out["err"],))
     sc.close()
     del sc
+    x_s0 = v_s0 = None
     if not args.no_plain:
         log("plain FP64 comparator (R = 1) ...")
         ps = args.plain_steps or S
@@ -258,22 +394,40 @@ def main():
         px0 = x0[:1].contiguous()
         pv0 = v0[:1].contiguous()
         pa = acts[:1, :ps].contiguous()
+        pg = gx[:1].contiguous()
+        pgv = gv[:1].contiguous()
+
+        def plain_pass():
+            pl.forward_device(px0, pv0, ps, act=pa, record=True)
+            pl.backward_device(pg, pgv)
+        plain_pass()  # warm-up: buffers, solve autotuning, first launches
         torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        pl.forward_device(px0, pv0, ps, act=pa, record=True)
-        pl.backward_device(gx[:1].contiguous(), gv[:1].contiguous())
-        torch.cuda.synchronize()
-        dt = time.perf_counter() - t0
-        pv = d.T * float(pl.iterations().sum()) / dt
+        pt, pit = [], []
+        for _ in range(2):
+            t0 = time.perf_counter()
+            plain_pass()
+            torch.cuda.synchronize()
+            pt.append(time.perf_counter() - t0)
+            pit.append(d.T * float(pl.iterations().sum()))
+        pv = sum(pit) / sum(pt)
         out["plain_fp64"] = {"value": pv, "unit": "tet-iterations/s", "rollouts": 1, "sim_steps": ps,
-                             "host_early_exit": True}
+                             "host_early_exit": True, "passes_timed": len(pt), "warmup": 1}
         out["speedup_vs_plain_fp64"] = value / pv
+        if "fp64_batch" in out:
+            out["fp64_batch"]["speedup_vs_plain_fp64"] = out["fp64_batch"]["value"] / pv
+        if args.workload == "c4" and not args.no_cpu:
+            # state at the start of the CPU sample (after the drop onto the floor)
+            xs, vs = torch.empty_like(px0), torch.empty_like(pv0)
+            pl.forward_device(px0, pv0, args.cpu_from, act=acts[:1, :args.cpu_from].contiguous(), xT=xs, vT=vs)
+            torch.cuda.synchronize()
+            x_s0, v_s0 = xs.cpu().numpy()[0], vs.cpu().numpy()[0]
         pl.close()
-    if not args.no_cpu and args.workload == "c4":
-        # (the packed 64-mesh world is out of reach of the single-threaded
-        # oracle: its muscle selection scans every tet for every segment)
-        log("CPU baseline (oracle, 1 thread) ...")
-        out["cpu_baseline"] = cpu_baseline(d, args.cpu_steps, 1)
+    if not args.no_cpu and args.workload == "c4" and x_s0 is not None:
+        # (the packed 64-mesh world is out of reach of the oracle: its muscle
+        # selection scans every tet for every segment)
+        threads = min(host_cpu_info()["nproc"], args.cpu_threads)
+        log("CPU baseline (oracle, 1 and %d threads) ..." % threads)
+        out["cpu_baseline"] = cpu_baseline(d, x_s0, v_s0, acts.cpu().numpy(), args.cpu_from, args.cpu_steps, threads)
     print(json.dumps(out), flush=True)
 
 

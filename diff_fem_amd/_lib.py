@@ -59,7 +59,7 @@ SYMBOLS = [
     "pdm_set_stream", "pdm_set_material", "pdm_set_batch", "pdm_forward", "pdm_forward_host", "pdm_backward",
     "pdm_backward_host", "pdm_get_iterations", "pdm_solve_host", "pdm_force_host", "pdm_get_timing",
     "pdm_host_analyse", "pdm_host_solve", "pdm_get_contact_stats", "pdm_time_solve", "pdm_get_solve_timing",
-    "pdm_get_solve_shapes", "pdm_set_solve_plan", "pdm_solve_noalloc",
+    "pdm_get_solve_shapes", "pdm_set_solve_plan", "pdm_solve_noalloc", "pdm_get_graph_launches",
 ]
 
 # PDM_NOALLOC_SCRATCH_BYTES of include/pdm.h
@@ -98,6 +98,7 @@ def load():
     L.pdm_get_contact_stats.argtypes = [vp, _ip, _ip]
     L.pdm_solve_host.argtypes = [vp, vp, vp]
     L.pdm_solve_noalloc.argtypes = [vp, C.c_int, vp, vp]
+    L.pdm_get_graph_launches.argtypes = [vp, C.POINTER(C.c_longlong)]
     L.pdm_force_host.argtypes = [vp, vp, vp, vp]
     L.pdm_get_timing.argtypes = [vp, _dp, _dp, _dp]
     L.pdm_time_solve.argtypes = [vp, C.c_int, _dp]

@@ -716,8 +716,9 @@ __global__ void k_set_int(int* __restrict__ p, int v) {
 constexpr int kChunk = 64;
 __global__ void k_update(const double* __restrict__ Xn, const double* __restrict__ B, double* __restrict__ Xit,
                          const int* __restrict__ active, double* __restrict__ part, double* __restrict__ tape, int N,
-                         int C, int* __restrict__ any) {
+                         int C, int* __restrict__ any, const int* __restrict__ slot, long long sstride) {
   __shared__ double red[4][64];
+  if (tape && slot) tape += (long long)(*slot) * sstride;
   // any: "some rollout still iterates", rebuilt by k_err_final right after
   if (any && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 && threadIdx.y == 0) *any = 0;
   int col = blockIdx.x * 64 + threadIdx.x;
@@ -847,8 +848,11 @@ __global__ void k_attach_rhs(const double* __restrict__ attw, const double* __re
 // End of step: v = damping (x+ - x)/h ; x = x+ (World.h:757-766, 708).
 // Device copy of FP64 arrays, two values per thread (the runtime's blit
 // kernel moves these state-sized buffers at a fraction of HBM speed).
+// slot (optional): dst is advanced by *slot * sstride doubles (the tape slot of
+// a step, read on the device so that a captured graph can serve every step).
 __global__ __launch_bounds__(256) void k_copy_f64(double* __restrict__ dst, const double* __restrict__ src,
-                                                  long long n) {
+                                                  long long n, const int* __restrict__ slot, long long sstride) {
+  if (slot) dst += (long long)(*slot) * sstride;
   long long i = 2 * ((long long)blockIdx.x * blockDim.x + threadIdx.x);
   if (i + 1 < n) {
     *reinterpret_cast<double2*>(dst + i) = *reinterpret_cast<const double2*>(src + i);
@@ -857,7 +861,8 @@ __global__ __launch_bounds__(256) void k_copy_f64(double* __restrict__ dst, cons
   }
 }
 __global__ __launch_bounds__(256) void k_copy_f64_1(double* __restrict__ dst, const double* __restrict__ src,
-                                                    long long n) {
+                                                    long long n, const int* __restrict__ slot, long long sstride) {
+  if (slot) dst += (long long)(*slot) * sstride;
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) dst[i] = src[i];
 }
@@ -2537,7 +2542,8 @@ struct pdm_scene_s {
     return 0;
   }
   // PDM_USE_GRAPH: captured step block and its device-side controls
-  hipGraphExec_t gexec = nullptr;
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};  // untaped / taped step block
+  long long graph_launches = 0;                  // step-block replays since the scene was made
   const double* g_act = nullptr;
   int g_act_steps = -1;
   bool capturing = false;
@@ -3221,14 +3227,16 @@ void launch_local(pdm_scene s, const double* Xsrc, const double* act, int act_st
 // rollout continues. Rollouts finish with x = ori_x.
 // FP64 device-to-device copy on the scene's stream (two values per thread when
 // both pointers are 16-byte aligned).
-int dcopy(pdm_scene s, double* dst, const double* src, size_t n) {
+// dst (+ *slot * sstride doubles when slot is given) <- src, n doubles.
+int dcopy(pdm_scene s, double* dst, const double* src, size_t n, const int* slot = nullptr, long long sstride = 0) {
   if (n == 0) return 0;
-  bool al = (((uintptr_t)dst | (uintptr_t)src) & 15) == 0;
+  bool al = (((uintptr_t)dst | (uintptr_t)src) & 15) == 0 && (sstride & 1) == 0;
   if (al)
     hipLaunchKernelGGL(k_copy_f64, dim3(cdiv((long long)(n + 1) / 2, 256)), dim3(256), 0, s->stream, dst, src,
-                       (long long)n);
+                       (long long)n, slot, sstride);
   else
-    hipLaunchKernelGGL(k_copy_f64_1, dim3(cdiv((long long)n, 256)), dim3(256), 0, s->stream, dst, src, (long long)n);
+    hipLaunchKernelGGL(k_copy_f64_1, dim3(cdiv((long long)n, 256)), dim3(256), 0, s->stream, dst, src, (long long)n,
+                       slot, sstride);
   HIPX(hipGetLastError());
   return 0;
 }
@@ -3354,11 +3362,14 @@ int contact_phase(pdm_scene s, const double* act, int act_steps, int step, Conta
 // step_dev non-null: device-side control for graph capture (no host reads;
 // activations indexed by the device step counter; the gather and the solve
 // return at once when no rollout iterates any more).
+// slot_dev (graph capture of a taped step): tapeY, tapeXn and tapeVn are the
+// tapes' slot-0 addresses and each write goes to the slot *slot_dev.
 int launch_pd_block(pdm_scene s, const double* act, int act_steps, int step, double* tapeY, double* tapeXn,
-                    double* tapeVn, bool host_exit, int* any, const int* step_dev) {
+                    double* tapeVn, bool host_exit, int* any, const int* step_dev, const int* slot_dev = nullptr) {
   HostScene& S = s->S;
   int N = S.N, C = s->C;
   size_t nc = s->NC();
+  const long long K1nc = (long long)(S.max_iter + 1) * (long long)nc;
   int nchunk = cdiv(N, kChunk);
   dim3 ugrid(cdiv(C, 64), nchunk), ublock(64, 4);
   hipLaunchKernelGGL(k_spmv, dim3(cdiv(nc, kThreads)), dim3(kThreads), 0, s->stream, s->Lptr.p, s->Lcol.p, s->Lval.p,
@@ -3369,9 +3380,9 @@ int launch_pd_block(pdm_scene s, const double* act, int act_steps, int step, dou
   hipLaunchKernelGGL(k_step_prep, dim3(cdiv(nc, kThreads)), dim3(kThreads), 0, s->stream, s->X.p, s->V.p, s->Lx.p,
                      s->mass.p, s->attw.p, s->xrest.p, s->Xn.p, s->Cc.p, s->Xit.p, N, C, S.h, S.g[0], S.g[1], S.g[2],
                      S.gravity_on, s->nfaces > 0 ? s->FF.p : nullptr, s->dvfptr.p, s->dvfq.p);
-  if (tapeXn && dcopy(s, tapeXn, s->Xn.p, nc)) return 1;
-  if (tapeVn && dcopy(s, tapeVn, s->V.p, nc)) return 1;
-  if (tapeY && dcopy(s, tapeY, s->Xit.p, nc)) return 1;
+  if (tapeXn && dcopy(s, tapeXn, s->Xn.p, nc, slot_dev, (long long)nc)) return 1;
+  if (tapeVn && dcopy(s, tapeVn, s->V.p, nc, slot_dev, (long long)nc)) return 1;
+  if (tapeY && dcopy(s, tapeY, s->Xit.p, nc, slot_dev, K1nc)) return 1;
   hipLaunchKernelGGL(k_sqsum_chunks, ugrid, ublock, 0, s->stream, s->Xit.p, s->part.p, N, C, any);
   hipLaunchKernelGGL(k_err_init, dim3(cdiv(s->R, 4)), dim3(256), 0, s->stream, s->part.p, nchunk, s->R, N,
                      s->active.p, s->iters_dev.p, any);
@@ -3390,7 +3401,7 @@ int launch_pd_block(pdm_scene s, const double* act, int act_steps, int step, dou
       launch_local<double>(s, s->Xit.p, act, act_steps, step, s->active.p, step_dev, any);
     if (launch_solve(s, s->B.p, any)) return 1;
     hipLaunchKernelGGL(k_update, ugrid, ublock, 0, s->stream, s->Xn.p, s->B.p, s->Xit.p, s->active.p, s->part.p,
-                       tapeY ? tapeY + (size_t)(it + 1) * nc : nullptr, N, C, any);
+                       tapeY ? tapeY + (size_t)(it + 1) * nc : nullptr, N, C, any, slot_dev, K1nc);
     hipLaunchKernelGGL(k_err_final, dim3(cdiv(s->R, 4)), dim3(256), 0, s->stream, s->part.p, nchunk, s->R, N,
                        s->active.p, s->iters_dev.p, s->err.p, any);
   }
@@ -3399,20 +3410,26 @@ int launch_pd_block(pdm_scene s, const double* act, int act_steps, int step, dou
 }
 
 void drop_step_graph(pdm_scene s) {
-  if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
-  s->gexec = nullptr;
+  for (int k = 0; k < 2; ++k) {
+    if (s->gexec[k]) (void)hipGraphExecDestroy(s->gexec[k]);
+    s->gexec[k] = nullptr;
+  }
 }
 
 // PDM_USE_GRAPH: the prep + PD block of a step, captured once per forward call
 // (buffers and the activation pointer are fixed within it) and replayed every
-// step after the device step counter is set.
-int ensure_step_graph(pdm_scene s, const double* act, int act_steps) {
-  if (s->gexec && s->g_act == act && s->g_act_steps == act_steps) return 0;
-  drop_step_graph(s);
-  if (s->any_dev.alloc(1) || s->step_dev.alloc(1)) return 1;
+// step after the device step counter is set. Graph 1 also writes the step's
+// tapes, at the slot held by the device slot counter.
+int ensure_step_graph(pdm_scene s, const double* act, int act_steps, bool taped) {
+  hipGraphExec_t& gx = s->gexec[taped ? 1 : 0];
+  if (gx && s->g_act == act && s->g_act_steps == act_steps) return 0;
+  if (s->g_act != act || s->g_act_steps != act_steps) drop_step_graph(s);
+  if (s->any_dev.alloc(1) || s->step_dev.alloc(2)) return 1;
   HIPX(hipStreamBeginCapture(s->stream, hipStreamCaptureModeRelaxed));
   s->capturing = true;
-  int rc = launch_pd_block(s, act, act_steps, 0, nullptr, nullptr, nullptr, false, s->any_dev.p, s->step_dev.p);
+  int rc = taped ? launch_pd_block(s, act, act_steps, 0, s->tY.p, s->tXn.p, s->nfaces > 0 ? s->tVn.p : nullptr, false,
+                                   s->any_dev.p, s->step_dev.p, s->step_dev.p + 1)
+                 : launch_pd_block(s, act, act_steps, 0, nullptr, nullptr, nullptr, false, s->any_dev.p, s->step_dev.p);
   s->capturing = false;
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(s->stream, &g);
@@ -3421,10 +3438,10 @@ int ensure_step_graph(pdm_scene s, const double* act, int act_steps) {
     if (!rc) g_err = std::string("graph capture failed: ") + hipGetErrorString(e);
     return 1;
   }
-  e = hipGraphInstantiate(&s->gexec, g, nullptr, nullptr, 0);
+  e = hipGraphInstantiate(&gx, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   if (e != hipSuccess) {
-    s->gexec = nullptr;
+    gx = nullptr;
     g_err = std::string("graph instantiation failed: ") + hipGetErrorString(e);
     return 1;
   }
@@ -3433,18 +3450,27 @@ int ensure_step_graph(pdm_scene s, const double* act, int act_steps) {
   return 0;
 }
 
+__global__ void k_set_int2(int* __restrict__ p, int v0, int v1) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    p[0] = v0;
+    p[1] = v1;
+  }
+}
+
 // One time step on device. tapeY: optional (max_iter+1) x NC buffer for the
-// iterates; tIters: optional R ints receiving the executed iterations.
+// iterates, slot `slot` of the segment tapes (tY, tXn, tVn) when they are
+// given; tIters: optional R ints receiving the executed iterations.
 int run_step(pdm_scene s, const double* act, int act_steps, int step, double* tapeY, double* tapeXn, double* tapeVn,
-             int* tIters, int* tNpass = nullptr, int* tCiters = nullptr, ContactTape* ctp = nullptr) {
+             int* tIters, int* tNpass = nullptr, int* tCiters = nullptr, ContactTape* ctp = nullptr, int slot = 0) {
   HostScene& S = s->S;
   size_t nc = s->NC();
   bool host_exit = (s->flags & PDM_HOST_EARLY_EXIT) != 0;
-  bool graph = (s->flags & PDM_USE_GRAPH) && !tapeY && !tapeXn && !tapeVn;
-  if (graph) {
-    if (ensure_step_graph(s, act, act_steps)) return 1;
-    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, s->stream, s->step_dev.p, step);
-    HIPX(hipGraphLaunch(s->gexec, s->stream));
+  const bool taped = tapeY != nullptr;
+  if (s->flags & PDM_USE_GRAPH) {
+    if (ensure_step_graph(s, act, act_steps, taped)) return 1;
+    hipLaunchKernelGGL(k_set_int2, dim3(1), dim3(64), 0, s->stream, s->step_dev.p, step, slot);
+    HIPX(hipGraphLaunch(s->gexec[taped ? 1 : 0], s->stream));
+    ++s->graph_launches;
   } else if (launch_pd_block(s, act, act_steps, step, tapeY, tapeXn, tapeVn, host_exit, nullptr, nullptr)) {
     return 1;
   }
@@ -3510,7 +3536,7 @@ int forward_impl(pdm_scene s, int steps, const double* x0, const double* v0, con
       int q = n - last_s0;
       if (run_step(s, act, steps, n, s->tY.p + (size_t)q * K1 * nc, s->tXn.p + (size_t)q * nc,
                    s->nfaces > 0 ? s->tVn.p + (size_t)q * nc : nullptr, s->tIters.p + (size_t)q * s->R, np, ci,
-                   s->ncv > 0 ? s->ctape[q].get() : nullptr))
+                   s->ncv > 0 ? s->ctape[q].get() : nullptr, q))
         return 1;
       HIPX(hipMemcpyAsync(it_all.p + (size_t)n * s->R, s->tIters.p + (size_t)q * s->R, s->R * sizeof(int),
                           hipMemcpyDeviceToDevice, s->stream));
@@ -3706,7 +3732,7 @@ int backward_impl(pdm_scene s, const double* gxT, const double* gvT, double* gx0
       int q = n - s0;
       if (run_step(s, act, steps, n, s->tY.p + (size_t)q * K1 * nc, s->tXn.p + (size_t)q * nc,
                    s->nfaces > 0 ? s->tVn.p + (size_t)q * nc : nullptr, s->tIters.p + (size_t)q * R, nullptr, nullptr,
-                   s->ncv > 0 ? s->ctape[q].get() : nullptr))
+                   s->ncv > 0 ? s->ctape[q].get() : nullptr, q))
         return 1;
     }
     HIPX(hipMemcpyAsync(it_h.data(), s->tIters.p, (size_t)(s1 - s0) * R * sizeof(int), hipMemcpyDeviceToHost,
@@ -4254,6 +4280,15 @@ int pdm_get_solve_timing(pdm_scene s, long long* n_solves, double* total_ms) {
   *total_ms = s->solve_ms;
   s->n_solves = 0;
   s->solve_ms = 0.0;
+  return 0;
+}
+
+int pdm_get_graph_launches(pdm_scene s, long long* n) {
+  if (!s || !n) {
+    g_err = "invalid argument";
+    return 1;
+  }
+  *n = s->graph_launches;
   return 0;
 }
 

@@ -238,6 +238,12 @@ class Scene:
         check(_lib.load().pdm_get_solve_timing(self.h, C.byref(n), C.byref(ms)))
         return n.value, ms.value
 
+    def graph_launches(self) -> int:
+        """Step-block hipGraph replays since the scene was made."""
+        n = C.c_longlong()
+        check(_lib.load().pdm_get_graph_launches(self.h, C.byref(n)))
+        return n.value
+
     def timing(self):
         f, b, fa = C.c_double(), C.c_double(), C.c_double()
         check(_lib.load().pdm_get_timing(self.h, C.byref(f), C.byref(b), C.byref(fa)))
@@ -368,6 +374,9 @@ class SplitScene:
     def contact_stats(self):
         st = [p.contact_stats() for p in self.parts]
         return np.concatenate([a for a, _ in st]), np.concatenate([b for _, b in st])
+
+    def graph_launches(self) -> int:
+        return sum(p.graph_launches() for p in self.parts)
 
     def timing(self):
         t = [p.timing() for p in self.parts]

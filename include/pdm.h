@@ -114,11 +114,14 @@ enum {
   PDM_HOST_EARLY_EXIT = 1, /* read the convergence test back each iteration and
                               stop launching (the reference's per-iteration break);
                               without it converged rollouts are masked on device */
-  PDM_USE_GRAPH = 2,       /* forward: capture the step start + PD iterations once per
-                              call as a hipGraph and replay it every step, with the
-                              early exit decided on the device (converged rollouts
-                              masked, gather and solves skipped once none iterates);
-                              the contact phase and the adjoint stay eager */
+  PDM_USE_GRAPH = 2,       /* forward (and the adjoint's recompute): capture the step
+                              start + PD iterations once per call as a hipGraph (one for
+                              taped steps, whose tape slot is read from a device
+                              counter, one for untaped steps) and replay it every step,
+                              with the early exit decided on the device (converged
+                              rollouts masked, gather and solves skipped once none
+                              iterates); the contact phase and the adjoint's reverse
+                              sweep stay eager */
   PDM_TIME_SOLVES = 4,     /* bracket every triangular solve with HIP events on the
                               library stream (read with pdm_get_solve_timing) */
   PDM_NO_AUTOTUNE = 8      /* keep the default solve launch shapes instead of timing
@@ -210,6 +213,10 @@ int pdm_set_solve_plan(pdm_scene s, int subtree_cut, int n_chains);
 
 /* Timing of the last forward/backward (ms, HIP events on the library stream). */
 int pdm_get_timing(pdm_scene s, double* fwd_ms, double* bwd_ms, double* factor_ms);
+
+/* Step-block graph replays (PDM_USE_GRAPH) since the scene was created, taped
+ * and untaped steps together. No counterpart in the reference (measurement). */
+int pdm_get_graph_launches(pdm_scene s, long long* n);
 
 /* Largest device scratch pdm_solve_noalloc uses: the engine-layout right-hand
  * sides (N x 3R doubles) plus the multifrontal update blocks (U_rows x 3R

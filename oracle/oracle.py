@@ -64,6 +64,8 @@ def lib():
         L.orc_rollout.argtypes = [C.c_void_p, C.c_int, _dp, _dp, C.c_void_p, _dp, _dp, C.c_void_p, C.c_int]
         L.orc_rollout_batch.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, C.c_void_p, _dp, _dp, C.c_int]
         L.orc_backward.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, C.c_void_p]
+        L.orc_fwd_adj_batch.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, C.c_void_p, _dp, _dp, _dp, _dp,
+                                        C.c_void_p, C.c_int]
         L.orc_corotate_project.argtypes = [_dp, _dp, _dp]
         L.orc_corotate_vjp.argtypes = [_dp, C.c_double, C.c_double, _dp, _dp]
         L.orc_svd3.argtypes = [_dp, _dp, _dp, _dp]
@@ -258,6 +260,23 @@ class OracleScene:
         if rc:
             raise RuntimeError(lib().orc_last_error().decode())
         return xT, vT
+
+    def fwd_adj_batch(self, x0, v0, steps, gx, gv, act=None, nthreads=1):
+        """R rollouts forward + reverse sweep, OpenMP over rollouts.
+        Returns (gx0 (R, 3N), gE (T,) summed over rollouts, iters (R, steps))."""
+        x0 = np.ascontiguousarray(x0, np.float64)
+        R = x0.shape[0]
+        v0 = np.ascontiguousarray(v0, np.float64)
+        gx = np.ascontiguousarray(gx, np.float64)
+        gv = np.ascontiguousarray(gv, np.float64)
+        a = None if act is None else np.ascontiguousarray(act, np.float64).reshape(R, steps, self.n_groups)
+        gx0 = np.zeros_like(x0)
+        gE = np.zeros(self.T)
+        its = np.zeros((R, steps), np.int32)
+        rc = lib().orc_fwd_adj_batch(self.h, R, steps, x0, v0, _ptr(a), gx, gv, gx0, gE, _ptr(its), int(nthreads))
+        if rc:
+            raise RuntimeError(lib().orc_last_error().decode())
+        return gx0, gE, its
 
     def backward(self, gx, gv):
         """Gradient of <gx,x_T>+<gv,v_T> over the last recorded rollout.

@@ -2338,6 +2338,60 @@ int orc_rollout_batch(void* p, int R, int steps, const double* x0, const double*
   return 0;
 }
 
+// Forward + reverse sweep of R independent rollouts (shared factor), OpenMP
+// over rollouts: the multi-core CPU baseline. Each rollout records its own
+// tape, then seeds the reverse sweep with gxT/gvT (R x 3N) and writes
+// gx0 (R x 3N); iters (R x steps, optional) receives the executed PD
+// iterations. Per-tet E and activation gradients are accumulated per rollout
+// and summed into gE (T) after the loop, in rollout order.
+int orc_fwd_adj_batch(void* p, int R, int steps, const double* x0, const double* v0, const double* act,
+                      const double* gxT, const double* gvT, double* gx0, double* gE, int* iters, int nthreads) {
+  Scene* s = (Scene*)p;
+  if (!s->factored && !factorize(*s)) return 1;
+  const int n3 = 3 * s->N;
+  const size_t T = s->tets.size();
+  std::vector<std::vector<double>> gEr(R, std::vector<double>(gE ? T : 0, 0.0));
+  int bad = 0;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : bad)
+#endif
+  for (int r = 0; r < R; ++r) {
+    std::vector<double> x(x0 + (size_t)r * n3, x0 + (size_t)(r + 1) * n3), v(v0 + (size_t)r * n3, v0 + (size_t)(r + 1) * n3);
+    std::vector<StepTape> tape(steps);
+    for (int n = 0; n < steps; ++n) {
+      int a = 0, b = 0, c = 0;
+      step(*s, x.data(), v.data(), act ? act + ((size_t)r * steps + n) * s->n_groups : nullptr, &tape[n], &a, &b, &c);
+      if (iters) iters[(size_t)r * steps + n] = a;
+    }
+    std::vector<double> gx(gxT + (size_t)r * n3, gxT + (size_t)(r + 1) * n3), gv(gvT + (size_t)r * n3, gvT + (size_t)(r + 1) * n3);
+    std::vector<double> gact((size_t)std::max(s->n_groups, 1), 0.0);
+    double gnu = 0.0;
+    for (int n = steps - 1; n >= 0; --n) {
+      if (tape[n].self_active) {
+        bad += 1;
+        break;
+      }
+      std::vector<double> gxn(n3, 0.0), gvn(n3, 0.0);
+      step_backward(*s, tape[n], gx.data(), gv.data(), gxn.data(), gvn.data(), gE ? gEr[r].data() : nullptr, &gnu,
+                    gact.data());
+      gx.swap(gxn);
+      gv.swap(gvn);
+    }
+    std::copy(gx.begin(), gx.end(), gx0 + (size_t)r * n3);
+  }
+  if (bad) {
+    g_err = "the reverse sweep does not cover self contact";
+    return 1;
+  }
+  if (gE) {
+    std::fill(gE, gE + T, 0.0);
+    for (int r = 0; r < R; ++r)
+      for (size_t t = 0; t < T; ++t) gE[t] += gEr[r][t];
+  }
+  return 0;
+}
+
 int orc_max_threads() {
 #ifdef _OPENMP
   return omp_get_max_threads();

@@ -221,32 +221,40 @@ def test_contact_gradient_matches_oracle(eng, oracle_mod, sphere):
     assert rel(g["E"], gE) < 1e-9
 
 
-@pytest.mark.parametrize("which", ["muscle", "contact"])
-def test_graph_replay_matches_eager(eng, which):
+@pytest.mark.parametrize("which,ckpt,prec", [("muscle", 20, 0), ("contact", 20, 0), ("contact", 25, 0),
+                                              ("contact", 25, 1)])
+def test_graph_replay_matches_eager(eng, which, ckpt, prec):
     """PDM_USE_GRAPH replays one captured step block per step (device-side
-    early exit). Every replay must leave the same state as the eager path:
-    positions, velocities and per-step iteration counts are compared after
-    two forward calls (the second re-captures), and the adjoint that follows
-    a graph forward must equal the eager one."""
+    early exit): untaped steps one graph, taped steps another whose tape slot
+    comes from a device counter. Every replay must leave the same state as the
+    eager path: positions, velocities and per-step iteration counts are
+    compared after two forward calls (the second re-captures), and the adjoint
+    that follows a graph forward (which replays the taped graph for the
+    recomputed segments) must equal the eager one. ckpt = steps tapes the
+    whole rollout in the forward pass, as the benchmark does."""
     d = small_muscle_scene() if which == "muscle" else contact_scene()
     R, steps = 3, 25
     rng = np.random.default_rng(17)
     x0 = np.tile(d.x_rest.ravel(), (R, 1))
     v0 = 0.02 * rng.standard_normal((R, 3 * d.N))
     act = rng.uniform(0, 1, (R, steps, d.n_groups)) if d.n_groups else None
-    eager = eng.Scene(d, R=R, flags=0)
-    graph = eng.Scene(d, R=R, flags=eng.PDM_USE_GRAPH)
+    eager = eng.Scene(d, R=R, precision=prec, flags=eng.PDM_HOST_EARLY_EXIT, checkpoint_every=ckpt)
+    graph = eng.Scene(d, R=R, precision=prec, flags=eng.PDM_USE_GRAPH | eng.PDM_HOST_EARLY_EXIT,
+                      checkpoint_every=ckpt)
     xa, va = eager.forward_host(x0, v0, steps, act=act, record=True)
     for call in range(2):
         xb, vb = graph.forward_host(x0, v0, steps, act=act, record=True)
         assert np.array_equal(xa, xb) and np.array_equal(va, vb), call
         assert np.array_equal(eager.iterations(), graph.iterations()), call
+    assert graph.graph_launches() == 2 * steps and eager.graph_launches() == 0
     gx = rng.standard_normal((R, 3 * d.N))
     gv = 0.01 * rng.standard_normal((R, 3 * d.N))
     ga = eager.backward_host(gx, gv)
     gb = graph.backward_host(gx, gv)
     for k in ("x0", "v0", "E"):
         assert np.array_equal(ga[k], gb[k]), k
+    # the reverse sweep recomputes every segment but the last from its checkpoint
+    assert graph.graph_launches() == 2 * steps + (steps - 1) // ckpt * ckpt
 
 
 def packed_scene():

@@ -476,3 +476,35 @@ def test_packed_rollout_equals_separate_rollouts(oracle_mod):
         assert np.abs(x - (xT[off:off + n3] - shift)).max() < 1e-12
         assert np.abs(v - vT[off:off + n3]).max() < 1e-10
         off += n3
+
+
+def test_fwd_adj_batch_equals_single_rollouts(oracle_mod):
+    """The OpenMP forward + reverse sweep over rollouts (the multi-core CPU
+    baseline) gives each rollout's gradient of the single-rollout sweep, and
+    the per-tet E gradient summed over rollouts."""
+    x, t = scene.box_grid(4, 2, 2)
+    x = x.copy()
+    x[:, 2] += 0.01
+    s0, s1, g, ng = scene.straight_fibres(2, 1, x.min(0), x.max(0), nseg=8)
+    d = scene.SceneDesc(x_rest=x, tets=t, total_mass=2.0, E=np.full(t.shape[0], 1e5), nu_fwd=0.4,
+                        nu_ctor=np.full(t.shape[0], 0.4), seg_start=s0, seg_end=s1, seg_group=g, n_groups=ng,
+                        h=0.01, max_iter=10, gravity=(0, 0, -9.81), contact=True, max_iter1=30, mu=0.5,
+                        planes=np.array([[0, 0, 0, 0, 0, 1.0]]))
+    o = oracle_mod.OracleScene(d)
+    R, steps = 3, 8
+    rng = np.random.default_rng(4)
+    x0 = np.tile(x.ravel(), (R, 1))
+    v0 = 0.05 * rng.standard_normal((R, 3 * d.N))
+    act = rng.uniform(0, 1, (R, steps, ng))
+    gx = rng.standard_normal((R, 3 * d.N))
+    gv = rng.standard_normal((R, 3 * d.N))
+    gx0, gE, its = o.fwd_adj_batch(x0, v0, steps, gx, gv, act=act, nthreads=3)
+    gE_sum = np.zeros(d.T)
+    for r in range(R):
+        _, _, it1 = o.rollout(x0[r], v0[r], steps, act=act[r], record=True)
+        g1, rc = o.backward(gx[r], gv[r])
+        assert rc == 0
+        np.testing.assert_array_equal(its[r], it1[:, 0])
+        assert np.abs(gx0[r] - g1["x0"]).max() <= 1e-12 * np.abs(g1["x0"]).max()
+        gE_sum += g1["E"]
+    assert np.abs(gE - gE_sum).max() <= 1e-12 * np.abs(gE_sum).max()
