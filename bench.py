@@ -7,8 +7,14 @@ R = 256 rollouts sharing one Cholesky factor (run as two concurrent
 sub-batches of 128 on their own streams), seeded sinusoidal activations,
 100 steps x 10 iterations, forward + adjoint with the whole rollout taped by
 the forward pass (about 232 GB of the 309 GB of HBM, so the adjoint recomputes
-nothing; --ckpt k keeps checkpoints every k steps and recomputes segments),
-FP32 local step with FP64 factor/solves/adjoint accumulation.
+nothing; --ckpt k keeps checkpoints every k steps and recomputes segments).
+
+Precision (--precision): mixed_adjoint (default) runs the forward in FP64, so
+the trajectory, its iteration counts and contact branches are the FP64 path's,
+and the reverse sweep's local-step VJP in FP32; mixed runs the local step in
+FP32 in both directions. Factor, solves, contact and accumulation are FP64 in
+both. The line reports the errors of both forms against an FP64 pass of the
+same batch (every rollout, full horizon) and the all-FP32 form's throughput.
 
 One bench "step" = one full forward + adjoint pass over the batch (inputs
 already resident in HBM). value = sum over rollouts of T * executed PD
@@ -146,6 +152,9 @@ def main():
                     help="c4: 40x12x12 muscle box, R rollouts; c5: 64 packed uneven meshes (one world), R = 1")
     ap.add_argument("--steps", type=int, default=2, help="timed forward+adjoint passes")
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--precision", choices=["mixed", "mixed_adjoint"], default="mixed_adjoint",
+                    help="mixed: FP32 local step forward and in the reverse sweep; mixed_adjoint: FP64 forward "
+                         "(the FP64 path's trajectory and branches), FP32 local-step VJP in the reverse sweep")
     ap.add_argument("--rollouts", type=int, default=256)
     ap.add_argument("--parts", type=int, default=2,
                     help="independent sub-batches run concurrently on their own streams (engine.SplitScene); "
@@ -182,6 +191,7 @@ def main():
     from diff_fem_amd import engine, scene
 
     d = build_workload(args)
+    prec = engine.PDM_MIXED if args.precision == "mixed" else engine.PDM_MIXED_ADJOINT
     R, S = args.rollouts, args.sim_steps
     if args.ckpt <= 0:
         args.ckpt = S
@@ -195,14 +205,14 @@ def main():
     # timed in one extra eager pass after the timed ones
     flags = engine.PDM_HOST_EARLY_EXIT | (engine.PDM_USE_GRAPH if args.graph else engine.PDM_TIME_SOLVES)
     if args.solve_only:  # one part's solve: the shape the pipeline launches
-        sc = engine.Scene(d, R=Rp, precision=engine.PDM_MIXED, flags=flags, checkpoint_every=args.ckpt)
+        sc = engine.Scene(d, R=Rp, precision=prec, flags=flags, checkpoint_every=args.ckpt)
         parts_list = [sc]
     elif parts > 1:
-        sc = engine.SplitScene(d, R=R, parts=parts, precision=engine.PDM_MIXED, flags=flags,
+        sc = engine.SplitScene(d, R=R, parts=parts, precision=prec, flags=flags,
                                checkpoint_every=args.ckpt)
         parts_list = sc.parts
     else:
-        sc = engine.Scene(d, R=R, precision=engine.PDM_MIXED, flags=flags, checkpoint_every=args.ckpt)
+        sc = engine.Scene(d, R=R, precision=prec, flags=flags, checkpoint_every=args.ckpt)
         parts_list = [sc]
     p0 = parts_list[0]
     info = sc.info()
@@ -277,7 +287,8 @@ def main():
         "metric": "tet-iterations/sec fwd+adjoint, batched mixed vs plain FP64 on GPU at stated err",
         "value": value, "unit": "tet-iterations/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": 1000.0 * total_t / args.steps, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "fp32-local/fp64-solve", "data": "synthetic (seeded box mesh, seeded activations)",
+        "vs_baseline": None,
+        "dtype": "fp32-local/fp64-solve" if prec == engine.PDM_MIXED else "fp64-forward/fp32-adjoint-local/fp64-solve", "data": "synthetic (seeded box mesh, seeded activations)",
         "config": {"workload": ("c4_muscle_40x12x12_R%d_%dsteps_fwd+adjoint%s" % (R, S, "" if d.contact else "_nocontact")
                                 if args.workload == "c4" else "c5_packed_64_meshes_R%d_%dsteps_fwd+adjoint" % (R, S)),
                    "tets": d.T, "vertices": d.N, "rollouts": R, "sim_steps": S, "max_iter": d.max_iter,
@@ -290,7 +301,7 @@ def main():
         # Solves inside a replayed graph cannot be bracketed by events: one
         # more pass, eager with the same concurrency, times every solve.
         for p in parts_list:
-            p.set_batch(p.R, engine.PDM_MIXED, engine.PDM_HOST_EARLY_EXIT | engine.PDM_TIME_SOLVES, args.ckpt)
+            p.set_batch(p.R, prec, engine.PDM_HOST_EARLY_EXIT | engine.PDM_TIME_SOLVES, args.ckpt)
         sc.solve_timing()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -334,55 +345,72 @@ def main():
     # tet-iteration, counted for the forward and again for the adjoint, plus
     # the checkpoint traffic; over the timed pass time and 8 TB/s.
     if args.workload == "c4":
-        s_b = 4.0
-        b_local = d.T * (16 + 10 * s_b) + R * 3 * d.N * s_b + R * 3 * d.N * 8
-        b_solve = 2.0 * info["nnz_lower"] * 8 + 2.0 * R * 3 * d.N * 8
-        per_tet_it = (b_local + b_solve) / (R * d.T)
-        b_pass = 2.0 * per_tet_it * (sum(tet_its) / len(tet_its)) + (S / args.ckpt) * 2.0 * R * 3 * d.N * 8
+        def per_tet_it(s_b):
+            b_local = d.T * (16 + 10 * s_b) + R * 3 * d.N * s_b + R * 3 * d.N * 8
+            b_solve = 2.0 * info["nnz_lower"] * 8 + 2.0 * R * 3 * d.N * 8
+            return (b_local + b_solve) / (R * d.T)
+        s_fwd = 4.0 if prec == engine.PDM_MIXED else 8.0
+        b_pass = (per_tet_it(s_fwd) + per_tet_it(4.0)) * (sum(tet_its) / len(tet_its)) \
+            + (S / args.ckpt) * 2.0 * R * 3 * d.N * 8
         t_pass = total_t / args.steps
         out["pass_roofline"] = {"bound": "hbm", "achieved": b_pass / t_pass / 1e9, "peak": 8000.0, "unit": "GB/s",
                                 "frac": b_pass / t_pass / 8.0e12, "bytes_alg_per_pass": b_pass,
-                                "bytes_alg_per_tet_iteration": per_tet_it}
+                                "bytes_alg_per_tet_iteration": 0.5 * (per_tet_it(s_fwd) + per_tet_it(4.0))}
     tet_its_pass = sum(tet_its) / len(tet_its)
     x_mixed = xT.cpu().numpy()
     g_mixed = {k: g[k].cpu().numpy() for k in ("x0", "v0", "act", "E")}
     if args.workload == "c4" and not args.no_fp64:
-        # The same batch in FP64 everywhere (same scene, buffers and flags):
-        # its throughput, and the reference the mixed errors are measured on.
-        log("FP64 batch (R = %d) ..." % R)
-        fl64 = engine.PDM_HOST_EARLY_EXIT | (engine.PDM_USE_GRAPH if args.graph else 0)
-        for p in parts_list:
-            p.set_batch(p.R, engine.PDM_FP64, fl64, args.ckpt)
-        one_pass()  # warm-up: first launches of the FP64 kernels
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        g64 = one_pass()
-        torch.cuda.synchronize()
-        t64 = time.perf_counter() - t0
-        its64 = sc.iterations()
-        x64 = xT.cpu().numpy()
-        g64 = {k: g64[k].cpu().numpy() for k in ("x0", "v0", "act", "E")}
-        ep = rel_rows(x_mixed, x64)
-        eg = np.maximum.reduce([rel_rows(g_mixed[k], g64[k]) for k in ("x0", "v0", "act")])
-        eE = float(np.abs(g_mixed["E"] - g64["E"]).max() / np.abs(g64["E"]).max())
-        out["fp64_batch"] = {"value": float(d.T) * float(its64.sum()) / t64, "unit": "tet-iterations/s",
-                             "rollouts": R, "s_per_pass": t64}
+        def batch_pass(p_, label):
+            """Warm-up + one timed pass of the whole batch at precision p_
+            (same scene, buffers and flags); returns its result."""
+            log("%s batch (R = %d) ..." % (label, R))
+            fl = engine.PDM_HOST_EARLY_EXIT | (engine.PDM_USE_GRAPH if args.graph else 0)
+            for p in parts_list:
+                p.set_batch(p.R, p_, fl, args.ckpt)
+            one_pass()  # warm-up: first launches of this precision's kernels
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            gg = one_pass()
+            torch.cuda.synchronize()
+            tp = time.perf_counter() - t0
+            it = sc.iterations()
+            return dict(t=tp, its=it, value=float(d.T) * float(it.sum()) / tp, x=xT.cpu().numpy(),
+                        g={k: gg[k].cpu().numpy() for k in ("x0", "v0", "act", "E")})
+
+        def errors(x_m, g_m, its_m, ref):
+            ep = rel_rows(x_m, ref["x"])
+            eg = np.maximum.reduce([rel_rows(g_m[k], ref["g"][k]) for k in ("x0", "v0", "act")])
+            return {"pos_max": float(ep.max()), "grad_max": float(eg.max()),
+                    "grad_E": float(np.abs(g_m["E"] - ref["g"]["E"]).max() / np.abs(ref["g"]["E"]).max()),
+                    "rollouts_pos_above_1e-5": int((ep > 1e-5).sum()),
+                    "rollouts_grad_above_1e-4": int((eg > 1e-4).sum()),
+                    "same_iterations": bool(np.array_equal(its_m, ref["its"]))}
+        # The same batch in FP64 everywhere: its throughput, and the reference
+        # the mixed errors are measured on (all rollouts, full horizon).
+        ref = batch_pass(engine.PDM_FP64, "FP64")
+        out["fp64_batch"] = {"value": ref["value"], "unit": "tet-iterations/s", "rollouts": R, "s_per_pass": ref["t"]}
         out["err"] = {"vs": "FP64 GPU batch, same inputs, all rollouts, full horizon",
-                      "pos_max": float(ep.max()), "grad_max": float(eg.max()), "grad_E": eE,
-                      "rollouts_pos_above_1e-5": int((ep > 1e-5).sum()),
-                      "rollouts_grad_above_1e-4": int((eg > 1e-4).sum()),
-                      "same_iterations": bool(np.array_equal(its, its64)), "grads": "x0, v0, activations per rollout"}
+                      "grads": "x0, v0, activations per rollout; E summed over rollouts"}
+        out["err"].update(errors(x_mixed, g_mixed, its, ref))
+        # the other mixed form, for comparison
+        alt = engine.PDM_MIXED if prec == engine.PDM_MIXED_ADJOINT else engine.PDM_MIXED_ADJOINT
+        oth = batch_pass(alt, "alternative mixed")
+        out["other_mixed"] = {"precision": "mixed" if alt == engine.PDM_MIXED else "mixed_adjoint",
+                              "dtype": "fp32-local/fp64-solve" if alt == engine.PDM_MIXED
+                              else "fp64-forward/fp32-adjoint-local/fp64-solve",
+                              "value": oth["value"], "unit": "tet-iterations/s", "s_per_pass": oth["t"],
+                              "err": errors(oth["x"], oth["g"], oth["its"], ref)}
         if not args.no_cpu:
             rows = [0, R - 1]
             log("oracle check of rollouts %s at %d steps ..." % (rows, S))
             xo, go = oracle_check(d, x0.cpu().numpy(), v0.cpu().numpy(), acts.cpu().numpy(), S,
                                   gx.cpu().numpy(), gv.cpu().numpy(), rows)
             out["err"]["oracle"] = {"rollouts": rows,
-                                    "fp64_pos": float(rel_rows(x64[rows], xo).max()),
-                                    "fp64_grad_x0": float(rel_rows(g64["x0"][rows], go).max()),
+                                    "fp64_pos": float(rel_rows(ref["x"][rows], xo).max()),
+                                    "fp64_grad_x0": float(rel_rows(ref["g"]["x0"][rows], go).max()),
                                     "mixed_pos": float(rel_rows(x_mixed[rows], xo).max()),
                                     "mixed_grad_x0": float(rel_rows(g_mixed["x0"][rows], go).max())}
-        log("errors: %s" % (out["err"],))
+        log("errors: %s; other mixed form: %s" % (out["err"], out["other_mixed"]))
     sc.close()
     del sc
     x_s0 = v_s0 = None

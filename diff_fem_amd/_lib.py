@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(HERE, "libpdm.so")
 
 PDM_FP64 = 0
 PDM_MIXED = 1
+PDM_MIXED_ADJOINT = 2
 PDM_HOST_EARLY_EXIT = 1
 PDM_USE_GRAPH = 2
 PDM_TIME_SOLVES = 4

@@ -2867,6 +2867,8 @@ int ensure_batch_buffers(pdm_scene s) {
                         s->GWf.alloc((size_t)s->nfaces * s->C)))
     return 1;
   if (s->U.alloc(std::max<long long>(1, s->F.uptr.back()) * (size_t)s->C)) return 1;
+  // forward slots are FP32 only in PDM_MIXED; the FP32 reverse sweep of
+  // PDM_MIXED_ADJOINT writes FP32 slots into the same (larger) buffer
   size_t esz = s->precision == PDM_MIXED ? sizeof(float) : sizeof(double);
   if (s->corner.alloc((size_t)s->Cn.nslot * s->C * esz)) return 1;
   if (s->part.alloc((size_t)cdiv(N, kChunk) * s->C)) return 1;
@@ -3655,7 +3657,7 @@ int contact_reverse(pdm_scene s, ContactTape& tp, const double* Xn, const double
                        s->alist.p, s->acount.p, s->Zb.p, s->Zs.p, s->cCD.p, N, R);
     hipLaunchKernelGGL((R % 64 == 0 ? k_cadj_cz<true> : k_cadj_cz<false>), dim3(cdiv(nc, kThreads)), dim3(kThreads), 0, s->stream, s->Lptr.p, s->Lcol.p,
                        s->Lval.p, s->Zb.p, s->Xb.p, s->cOB.p, s->alist.p, s->acount.p, N, C);
-    int rc = s->precision == PDM_MIXED
+    int rc = s->precision != PDM_FP64
                  ? vjp_iteration<float>(s, nullptr, ori, ori, act, act_steps, step, s->cCD.p, s->cOB.p, Xn)
                  : vjp_iteration<double>(s, nullptr, ori, ori, act, act_steps, step, s->cCD.p, s->cOB.p, Xn);
     if (rc) return 1;
@@ -3757,7 +3759,7 @@ int backward_impl(pdm_scene s, const double* gxT, const double* gvT, double* gx0
         if (launch_solve(s, s->Zb.p)) return 1;
         hipLaunchKernelGGL(k_adj_post_solve, dim3(cdiv(nc, kThreads)), dim3(kThreads), 0, s->stream, s->Zb.p,
                            s->Yb.p, s->Xb.p, s->Zs.p, s->ractive.p, (long long)nc, C);
-        int rc = s->precision == PDM_MIXED
+        int rc = s->precision != PDM_FP64
                      ? vjp_iteration<float>(s, s->ractive.p, Y + (size_t)k * nc, Y + (size_t)(k + 1) * nc, act,
                                             steps, n)
                      : vjp_iteration<double>(s, s->ractive.p, Y + (size_t)k * nc, Y + (size_t)(k + 1) * nc, act,
@@ -4028,7 +4030,8 @@ int pdm_set_material(pdm_scene s, const double* E, double nu_fwd) {
 }
 
 int pdm_set_batch(pdm_scene s, int R, int precision, int flags, int checkpoint_every) {
-  if (!s || R < 1 || (precision != PDM_FP64 && precision != PDM_MIXED) || checkpoint_every < 1) {
+  if (!s || R < 1 || (precision != PDM_FP64 && precision != PDM_MIXED && precision != PDM_MIXED_ADJOINT) ||
+      checkpoint_every < 1) {
     g_err = "invalid batch arguments";
     return 1;
   }

@@ -11,8 +11,8 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import (PDM_FP64, PDM_HOST_EARLY_EXIT, PDM_MIXED, PDM_NO_AUTOTUNE, PDM_TIME_SOLVES,  # noqa: F401
-                   PDM_USE_GRAPH, check)
+from ._lib import (PDM_FP64, PDM_HOST_EARLY_EXIT, PDM_MIXED, PDM_MIXED_ADJOINT, PDM_NO_AUTOTUNE,  # noqa: F401
+                   PDM_TIME_SOLVES, PDM_USE_GRAPH, check)
 
 
 def _dptr(a):

@@ -101,13 +101,20 @@ typedef struct pdm_info {
   double reg_used;           /* regularisation applied by the ladder (0 if none) */
   int n_contact_vertices;
   int batch;                 /* rollouts R */
-  int precision;             /* PDM_FP64 or PDM_MIXED */
+  int precision;             /* PDM_FP64, PDM_MIXED or PDM_MIXED_ADJOINT */
   int n_drag_faces;          /* surface faces carrying drag (0 unless underwater) */
   int n_obstacle_tris;       /* static obstacle triangles */
   int n_self_faces;          /* surface faces tested for self contact */
 } pdm_info;
 
-enum { PDM_FP64 = 0, PDM_MIXED = 1 };
+/* Arithmetic of pdm_set_batch:
+ *   PDM_FP64           FP64 everywhere;
+ *   PDM_MIXED          FP32 local step (forward and its VJP in the reverse
+ *                      sweep), FP64 factor, solves, contact and accumulation;
+ *   PDM_MIXED_ADJOINT  FP64 forward (the trajectory, iteration counts and
+ *                      contact branches are the FP64 path's), FP32 local-step
+ *                      VJP in the reverse sweep, FP64 elsewhere. */
+enum { PDM_FP64 = 0, PDM_MIXED = 1, PDM_MIXED_ADJOINT = 2 };
 
 /* Flags of pdm_set_batch. */
 enum {

@@ -391,7 +391,7 @@ def _drop_batch_scene():
                            max_iter1=10, planes=np.array([[0, 0, 0, 0, 0, 1.0]]))
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "mixed_adjoint"])
 def test_batch64_drop_matches_oracle(eng, oracle_mod, precision):
     """R = 64 (the batched layout of the benchmark: one wave per tet for 64
     listed rollouts, wave-uniform vertex rows) with the benchmark's flags
@@ -407,11 +407,15 @@ def test_batch64_drop_matches_oracle(eng, oracle_mod, precision):
     then belongs to the neighbouring piece. Such a rollout may miss the 1e-4
     bar only if the oracle's own FP64 gradient jumps by a comparable amount
     when the start velocity changes by 1e-4 relative (a kink next to the
-    trajectory); at most two of the 64 may, each within 1e-2."""
+    trajectory); at most two of the 64 may, each within 1e-2.
+
+    Mixed adjoint (FP64 forward, FP32 local-step VJP): the forward is the FP64
+    path's, so positions and iteration counts hold the FP64 bar and every
+    rollout's gradient the 1e-4 bar, without exception."""
     d = _drop_batch_scene()
     R, steps = 64, 12
-    prec = eng.PDM_FP64 if precision == "fp64" else eng.PDM_MIXED
-    tol_x, tol_g = (1e-9, 1e-9) if precision == "fp64" else (1e-5, 1e-4)
+    prec = {"fp64": eng.PDM_FP64, "mixed": eng.PDM_MIXED, "mixed_adjoint": eng.PDM_MIXED_ADJOINT}[precision]
+    tol_x, tol_g = {"fp64": (1e-9, 1e-9), "mixed": (1e-5, 1e-4), "mixed_adjoint": (1e-9, 1e-4)}[precision]
     s = eng.Scene(d, R=R, precision=prec, flags=eng.PDM_HOST_EARLY_EXIT | eng.PDM_USE_GRAPH,
                   checkpoint_every=steps)
     o = oracle_mod.OracleScene(d)
@@ -436,7 +440,7 @@ def test_batch64_drop_matches_oracle(eng, oracle_mod, precision):
     kinked = []
     for r in range(R):
         xo, vo, io = o.rollout(x0[r], v0[r], steps, act=act[r], record=True)
-        if precision == "fp64":
+        if precision != "mixed":
             assert np.array_equal(its[r], io[:, 0]) and np.array_equal(jac[r], io[:, 1]), r
         assert rel(xT[r], xo) < tol_x, r
         go, rc = o.backward(gx[r], gv[r])
@@ -450,6 +454,45 @@ def test_batch64_drop_matches_oracle(eng, oracle_mod, precision):
     _assert_kinks(o, kinked, x0, v0, act, gx, gv, steps)
     # the per-tet E gradient sums over rollouts, kinked ones included
     assert rel(g["E"], gE_sum) < (tol_g if not kinked else 1e-2)
+
+
+def test_mixed_adjoint_full_horizon_all_rollouts(eng, oracle_mod):
+    """60 steps of a muscle box bouncing on a friction floor, 64 rollouts:
+    the mixed-adjoint forward equals the FP64 forward bitwise (positions and
+    iteration counts), and every rollout's gradient (x0, v0, activations) is
+    within 1e-4 of the FP64 gradient; two rollouts also against the oracle."""
+    d = _drop_batch_scene()
+    R, steps = 64, 60
+    rng = np.random.default_rng(66)
+    x0 = np.tile(d.x_rest.ravel(), (R, 1))
+    v0 = np.zeros((R, 3 * d.N))
+    for r in range(R):
+        u = np.array([rng.uniform(-0.3, 0.3), rng.uniform(-0.3, 0.3), rng.uniform(-1.2, 1.0)])
+        v0[r] = np.tile(u, d.N) + 0.02 * rng.standard_normal(3 * d.N)
+    act = rng.uniform(0, 1, (R, steps, d.n_groups))
+    gx = rng.standard_normal((R, 3 * d.N))
+    gv = 0.01 * rng.standard_normal((R, 3 * d.N))
+    res = {}
+    for name, prec in (("fp64", eng.PDM_FP64), ("ma", eng.PDM_MIXED_ADJOINT)):
+        s = eng.Scene(d, R=R, precision=prec, flags=eng.PDM_HOST_EARLY_EXIT | eng.PDM_USE_GRAPH,
+                      checkpoint_every=20)
+        xT, vT = s.forward_host(x0, v0, steps, act=act, record=True)
+        res[name] = (xT, vT, s.iterations(), s.contact_stats(), s.backward_host(gx, gv))
+        s.close()
+    (x64, v64, i64, c64, g64), (xma, vma, ima, cma, gma) = res["fp64"], res["ma"]
+    assert np.array_equal(x64, xma) and np.array_equal(v64, vma)
+    assert np.array_equal(i64, ima) and np.array_equal(c64[1], cma[1])
+    assert c64[1].sum() > 0
+    for r in range(R):
+        err = max(rel(gma[k][r], g64[k][r]) for k in ("x0", "v0", "act"))
+        assert err < 1e-4, (r, err)
+    for r in (0, R - 1):
+        xo, _, _ = oracle_mod.OracleScene(d).rollout(x0[r], v0[r], steps, act=act[r], record=False)
+        assert rel(xma[r], xo) < 1e-9
+    o = oracle_mod.OracleScene(d)
+    o.rollout(x0[0], v0[0], steps, act=act[0], record=True)
+    go, rc = o.backward(gx[0], gv[0])
+    assert rc == 0 and rel(gma["x0"][0], go["x0"]) < 1e-4 and rel(gma["act"][0], go["act"]) < 1e-4
 
 
 def test_split_scene_matches_one_scene(eng):
@@ -571,7 +614,7 @@ def test_config4_full_size_matches_oracle(eng, oracle_mod, precision):
     kinked = []
     for r in check:
         xo, vo, io = o.rollout(x0[r], v0[r], steps, act=act[r], record=True)
-        if precision == "fp64":
+        if precision != "mixed":
             assert np.array_equal(its[r], io[:, 0]) and np.array_equal(jac[r], io[:, 1]), r
         assert rel(xT[r], xo) < tol_x, r
         go, rc = o.backward(gx[r], gv[r])
