@@ -8,13 +8,28 @@ namespace pdm {
 
 template <typename T>
 struct Num;
-// FP64: IEEE division and square root (the plain path is held to 1e-9).
+// FP64: reciprocal and inverse square root from the hardware estimates
+// (v_rcp_f64, v_rsq_f64) refined by two Newton steps, to about 1 ulp (not
+// always correctly rounded; the plain path is held to 1e-9); IEEE square root.
 template <>
 struct Num<double> {
-  static __device__ __forceinline__ double rsqrt(double x) { return 1.0 / sqrt(x); }
+  static __device__ __forceinline__ double rcp(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+  }
+  // y <- y + (y / 2) (1 - x y^2), twice
+  static __device__ __forceinline__ double rsqrt(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    double e = fma(-x * y, y, 1.0);
+    y = fma(0.5 * y, e, y);
+    e = fma(-x * y, y, 1.0);
+    return fma(0.5 * y, e, y);
+  }
   static __device__ __forceinline__ double sqrt_(double x) { return sqrt(x); }
-  static __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
-  static __device__ __forceinline__ double div(double a, double b) { return a / b; }
+  static __device__ __forceinline__ double div(double a, double b) { return a * rcp(b); }
   static constexpr int kSweeps = 6;
   static constexpr double kTiny = 1e-300;
   static constexpr double kRel = 1e-15;
@@ -35,17 +50,20 @@ struct Num<float> {
 
 // Jacobi rotation (c, s) that zeroes a_pq of the symmetric 2x2 block
 // [[a_pp, a_pq], [a_pq, a_qq]]: t = tan(theta) = sign(tau) / (|tau| + sqrt(1 + tau^2)),
-// tau = (a_qq - a_pp) / (2 a_pq). The FP32 form multiplies through by |2 a_pq|
-// (t = sign(tau) |e| / (|d| + sqrt(d^2 + e^2)), d = a_qq - a_pp, e = 2 a_pq),
+// tau = (a_qq - a_pp) / (2 a_pq), multiplied through by |2 a_pq|:
+// t = sign(tau) |e| / (|d| + sqrt(d^2 + e^2)), d = a_qq - a_pp, e = 2 a_pq,
 // which needs one reciprocal and no branch; a_pq = 0 gives t = 0 (the tiny
-// addend keeps d = e = 0 finite). Returns false when FP64 skips the rotation.
+// addend keeps d = e = 0 finite).
 template <typename T>
 __device__ __forceinline__ bool jacobi_cs(T app, T aqq, T apq, T* t_out, T* c_out, T* s_out) {
   if constexpr (sizeof(T) == 8) {
-    if (!(fabs(apq) > Num<T>::kTiny)) return false;
-    T tau = (aqq - app) / (T(2) * apq);
-    T t = (tau >= 0 ? T(1) : T(-1)) / (fabs(tau) + sqrt(T(1) + tau * tau));
-    T c = T(1) / sqrt(T(1) + t * t);
+    T d = aqq - app, e = T(2) * apq;
+    T ae = fabs(e);
+    T num = d * e >= T(0) ? ae : -ae;
+    T h2 = d * d + e * e;
+    T hyp = h2 * Num<T>::rsqrt(h2 + T(1e-300));
+    T t = num * Num<T>::rcp(fabs(d) + hyp + T(1e-300));
+    T c = Num<T>::rsqrt(T(1) + t * t);
     *t_out = t;
     *c_out = c;
     *s_out = t * c;
@@ -134,8 +152,16 @@ __device__ __forceinline__ void svd3(const T* F, T* U, T* S, T* V) {
     }
     };
   if constexpr (sizeof(T) == 8) {
+    // three sweeps, then up to three more while the off-diagonal part of
+    // F^T F is above 1e-15 of its trace (cyclic Jacobi converges
+    // quadratically: one sweep from there reaches rounding level)
 #pragma unroll
-    for (int sw = 0; sw < Num<T>::kSweeps; ++sw) sweep();
+    for (int sw = 0; sw < 3; ++sw) sweep();
+    for (int sw = 3; sw < Num<T>::kSweeps; ++sw) {
+      T off = fabs(a01) + fabs(a02) + fabs(a12);
+      if (!(off > T(1e-15) * (a00 + a11 + a22))) break;
+      sweep();
+    }
   } else {
     // FP32: three sweeps, a fourth only while the off-diagonal part of F^T F
     // is above 1e-7 of its trace (rare for the strains met here)
