@@ -29,7 +29,30 @@ def build(verbose=False):
     cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", OUT + ".tmp", *objs]
     subprocess.run(cmd, check=True)
     os.replace(OUT + ".tmp", OUT)
+    build_pybind(verbose)
     return OUT
+
+
+def pybind_module_path():
+    import sysconfig
+    return os.path.join(HERE, "pybind", "pdm_pybind" + sysconfig.get_config_var("EXT_SUFFIX"))
+
+
+def build_pybind(verbose=False):
+    """The reference-side pybind11 binding over the C ABI (pybind/pdm_pybind.cc),
+    linked against libpdm.so (host C++ only, no GPU code of its own)."""
+    import sysconfig
+    import pybind11
+    out = pybind_module_path()
+    cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-shared", "-fPIC",
+           os.path.join(HERE, "pybind", "pdm_pybind.cc"), "-I", os.path.join(os.path.dirname(HERE), "include"),
+           "-I", sysconfig.get_paths()["include"], "-I", pybind11.get_include(), "-L", HERE, "-lpdm",
+           "-Wl,-rpath,$ORIGIN/..", "-o", out + ".tmp"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(out + ".tmp", out)
+    return out
 
 
 if __name__ == "__main__":

@@ -2402,15 +2402,20 @@ struct DevBuf {
 };
 
 // Grow a device buffer to `count` elements keeping its first `keep` elements.
+// The old buffer goes to `grave` and is freed at the end of the call (a free
+// here would wait for the whole device, stalling other scenes' streams; the
+// copy out of it is ordered on st before any later use).
 template <typename X>
-int grow_keep(DevBuf<X>& b, size_t count, size_t keep, hipStream_t st) {
+int grow_keep(DevBuf<X>& b, size_t count, size_t keep, hipStream_t st, std::vector<void*>& grave) {
   if (count <= b.n && b.p) return 0;
   DevBuf<X> nb;
   if (nb.alloc(count)) return 1;
   if (keep && b.p) HIPX(hipMemcpyAsync(nb.p, b.p, keep * sizeof(X), hipMemcpyDeviceToDevice, st));
-  HIPX(hipStreamSynchronize(st));
   std::swap(b.p, nb.p);
   std::swap(b.n, nb.n);
+  if (nb.p) grave.push_back((void*)nb.p);
+  nb.p = nullptr;
+  nb.n = 0;
   return 0;
 }
 
@@ -2544,6 +2549,11 @@ struct pdm_scene_s {
   // PDM_USE_GRAPH: captured step block and its device-side controls
   hipGraphExec_t gexec[2] = {nullptr, nullptr};  // untaped / taped step block
   long long graph_launches = 0;                  // step-block replays since the scene was made
+  std::vector<void*> grave;                      // device buffers replaced during a call, freed at its end
+  void flush_grave() {
+    for (void* p : grave) (void)hipFree(p);
+    grave.clear();
+  }
   const double* g_act = nullptr;
   int g_act_steps = -1;
   bool capturing = false;
@@ -3269,7 +3279,7 @@ int contact_phase(pdm_scene s, const double* act, int act_steps, int step, Conta
     tp->P = 0;
     tp->first.clear();
     tp->pass_of.clear();
-    if (grow_keep(tp->xit, nc, 0, s->stream)) return 1;
+    if (grow_keep(tp->xit, nc, 0, s->stream, s->grave)) return 1;
     if (dcopy(s, tp->xit.p, s->Xit.p, nc)) return 1;
   }
   for (int pass = 0; pass <= S.outer_cap; ++pass) {
@@ -3306,10 +3316,10 @@ int contact_phase(pdm_scene s, const double* act, int act_steps, int step, Conta
       // store of x_it fills the later slots for every rollout that reads them
       if (tp->G > 0 && !self && dcopy(s, tp->xit.p + (size_t)tp->G * nc, s->Xit.p, nc)) return 1;
       size_t g1 = (size_t)tp->G + S.max_iter1;
-      if (grow_keep(tp->xit, (g1 + 1) * nc, (tp->G + 1) * nc, s->stream) ||
-          grow_keep(tp->rc, g1 * cvc, tp->G * cvc, s->stream) ||
-          grow_keep(tp->nrm, (size_t)(tp->P + 1) * cvc, (size_t)tp->P * cvc, s->stream) ||
-          grow_keep(tp->exec, g1 * R, (size_t)tp->G * R, s->stream))
+      if (grow_keep(tp->xit, (g1 + 1) * nc, (tp->G + 1) * nc, s->stream, s->grave) ||
+          grow_keep(tp->rc, g1 * cvc, tp->G * cvc, s->stream, s->grave) ||
+          grow_keep(tp->nrm, (size_t)(tp->P + 1) * cvc, (size_t)tp->P * cvc, s->stream, s->grave) ||
+          grow_keep(tp->exec, g1 * R, (size_t)tp->G * R, s->stream, s->grave))
         return 1;
     }
     for (int i = 0; i < S.max_iter1; ++i) {
@@ -3352,7 +3362,7 @@ int contact_phase(pdm_scene s, const double* act, int act_steps, int step, Conta
     if (tp) tp->P += 1;
   }
   if (tp) {
-    if (grow_keep(tp->flast, R, 0, s->stream)) return 1;
+    if (grow_keep(tp->flast, R, 0, s->stream, s->grave)) return 1;
     HIPX(hipMemcpyAsync(tp->flast.p, s->clast.p, R * sizeof(int), hipMemcpyDeviceToDevice, s->stream));
   }
   if (dcopy(s, s->Xit.p, s->Ori.p, nc)) return 1;
@@ -3580,6 +3590,7 @@ int forward_impl(pdm_scene s, int steps, const double* x0, const double* v0, con
     s->have_tape = true;
     s->tape_steps = steps;
   }
+  s->flush_grave();  // the stream was synchronised above
   return 0;
 }
 
@@ -3641,7 +3652,7 @@ int contact_reverse(pdm_scene s, ContactTape& tp, const double* Xn, const double
   inj.assign((size_t)G * R, 0);
   for (int g = 0; g < G; ++g)
     for (int r = 0; r < R; ++r) inj[(size_t)g * R + r] = (g == glast[r] && !fl[r]) ? 1 : 0;
-  if (s->cinjG.alloc(inj.size())) return 1;
+  if (grow_keep(s->cinjG, inj.size(), 0, s->stream, s->grave)) return 1;
   DevBuf<int>& dinj = s->cinjG;
   HIPX(hipMemcpyAsync(dinj.p, inj.data(), inj.size() * sizeof(int), hipMemcpyHostToDevice, s->stream));
   for (int g = G - 1; g >= 0; --g) {
@@ -3812,6 +3823,10 @@ int backward_impl(pdm_scene s, const double* gxT, const double* gvT, double* gx0
   float ms = 0;
   HIPX(hipEventElapsedTime(&ms, s->ev0, s->ev1));
   s->t_bwd = ms;
+  if (!s->grave.empty()) {
+    HIPX(hipStreamSynchronize(s->stream));
+    s->flush_grave();
+  }
   return 0;
 }
 
@@ -3955,6 +3970,7 @@ void pdm_scene_destroy(pdm_scene s) {
   }
   if (s->ev_fork) (void)hipEventDestroy(s->ev_fork);
   drop_step_graph(s);
+  s->flush_grave();
   if (s->ev0) (void)hipEventDestroy(s->ev0);
   if (s->ev1) (void)hipEventDestroy(s->ev1);
   if (s->own_stream && s->stream) (void)hipStreamDestroy(s->stream);

@@ -111,6 +111,9 @@ class Scene:
         check(L.pdm_scene_create(C.byref(d), C.byref(h)))
         self.h = h
         self._steps = 0
+        # Every forward, material update or batch change replaces (or drops)
+        # the library's one tape; tape_id names the tape a backward would use.
+        self.tape_id = 0
         self.set_batch(R, precision, flags, checkpoint_every)
 
     def close(self):
@@ -128,12 +131,14 @@ class Scene:
     def set_batch(self, R: int, precision: int = PDM_FP64, flags: int = PDM_HOST_EARLY_EXIT,
                   checkpoint_every: int = 20):
         check(_lib.load().pdm_set_batch(self.h, int(R), int(precision), int(flags), int(checkpoint_every)))
+        self.tape_id += 1
         self.R = int(R)
         self.precision = int(precision)
         self.flags = int(flags)
 
     def set_material(self, E=None, nu_fwd=None):
         L = _lib.load()
+        self.tape_id += 1
         Ea = None
         if E is not None:
             Ea = np.ascontiguousarray(np.broadcast_to(np.asarray(E, np.float64), (self.T,)))
@@ -160,6 +165,7 @@ class Scene:
                                                      (R, steps, self.n_groups)))
         xT = np.zeros((R, n3))
         vT = np.zeros((R, n3))
+        self.tape_id += 1
         check(_lib.load().pdm_forward_host(self.h, int(steps), _vp(x0), _vp(v0), _vp(a), _vp(xT), _vp(vT),
                                            int(record)))
         self._steps = steps
@@ -263,6 +269,7 @@ class Scene:
             xT = torch.empty_like(x0)
         if vT is None:
             vT = torch.empty_like(v0)
+        self.tape_id += 1
         check(_lib.load().pdm_forward(self.h, int(steps), _vp(x0), _vp(v0), _vp(act), _vp(xT), _vp(vT),
                                       int(record)))
         self._steps = steps
@@ -333,6 +340,10 @@ class SplitScene:
     def close(self):
         for p in self.parts:
             p.close()
+
+    @property
+    def tape_id(self):
+        return tuple(p.tape_id for p in self.parts)
 
     def set_material(self, E=None, nu_fwd=None):
         for p in self.parts:

@@ -12,7 +12,7 @@ reference that this engine does not keep):
   * backward calls raise instead of ``exit(0)`` when no forward was recorded
     (the reference exits in its FORWARD_ONLY build, pydifem.cc:105-109, 291-295);
   * OBJ frames are not written per step (pydifem.cc:226-239); ``SaveObj`` writes
-    one on request;
+    them on request, as the reference's Environment::SaveObj does;
   * the config path is resolved against ``PDM_DATA_ROOT`` (or the path as
     given) instead of the compile-time SOFTCON_DIR (Environment.h:90).
 Arithmetic runs in FP64 on the GPU; results are Python lists of floats, as the
@@ -27,16 +27,84 @@ from . import actuation, engine, scene
 __version__ = "dev"
 
 
+class EigenVector(np.ndarray):
+    """Eigen::Matrix<Scalar, Dynamic, 1> as the binding exposes it
+    (pydifem.cc:354-356): ``rows()``, ``cols()``; FP64 numpy underneath."""
+
+    def __new__(cls, a=()):
+        return np.array(a, dtype=np.float64).ravel().view(cls)
+
+    def rows(self):
+        return int(self.size)
+
+    def cols(self):
+        return 1
+
+
+class EigenMatrix(np.ndarray):
+    """Eigen::Matrix<Scalar, Dynamic, Dynamic> (pydifem.cc:351-353)."""
+
+    def __new__(cls, a=((),)):
+        return np.array(a, dtype=np.float64, ndmin=2).view(cls)
+
+    def rows(self):
+        return int(self.shape[0])
+
+    def cols(self):
+        return int(self.shape[1])
+
+
+class Scalar(float):
+    """The scalar type of the build (double in the FORWARD_ONLY build,
+    pydifem.cc:14)."""
+
+
+class Utils:
+    """DoubleUtils as the binding exposes it (pydifem.cc:381-385;
+    sim/diff/tiny_double_utils.h)."""
+
+    @staticmethod
+    def getDouble(v):
+        return float(v)
+
+    @staticmethod
+    def scalar_from_double(v):
+        return float(v)
+
+    @staticmethod
+    def zero():
+        return 0.0
+
+    @staticmethod
+    def fraction(num, denom):
+        return float(int(num)) / float(int(denom))
+
+
+def _ev_prop(name):
+    def get(self):
+        return getattr(self, "_" + name)
+
+    def put(self, v):
+        setattr(self, "_" + name, EigenVector(v))
+    return property(get, put)
+
+
 class World:
-    """Subset of FEM::World exposed by the reference binding (pydifem.cc:357-365)."""
+    """Subset of FEM::World exposed by the reference binding (pydifem.cc:357-365);
+    its state vectors are EigenVectors (assignments are converted)."""
+    mX = _ev_prop("mX")
+    mV = _ev_prop("mV")
+    mJointQ = _ev_prop("mJointQ")
+    mJointQd = _ev_prop("mJointQd")
+    mJointTorque = _ev_prop("mJointTorque")
 
     def __init__(self, env):
         self._env = env
-        self.mX = env._desc.x_rest.ravel().copy()
-        self.mV = np.zeros_like(self.mX)
-        self.mJointQ = np.zeros(0)
-        self.mJointQd = np.zeros(0)
-        self.mJointTorque = np.zeros(0)
+        self.mX = env._desc.x_rest.ravel()
+        self.mV = np.zeros(self.mX.size)
+        self.mJointQ = ()
+        self.mJointQd = ()
+        self.mJointTorque = ()
 
     def GetTimeStep(self):
         return self._env._desc.h
@@ -47,6 +115,8 @@ class Dfobj:
     binding (pydifem.cc:367-373), plus its phase (Dfobj::mPhase) and actuator.
     ``mActions`` has 4 entries per muscle for actuator type 2
     (Dfobj.h:335-383), none otherwise."""
+
+    mActions = _ev_prop("mActions")
 
     def __init__(self, env, info: scene.DfobjInfo):
         self._env = env
@@ -61,7 +131,7 @@ class Dfobj:
 
     def GetActivationLevelsAggregate(self):
         """The muscles' levels, 50 per muscle (Dfobj.h:689-707)."""
-        return self._env._act_state[self._groups].copy()
+        return EigenVector(self._env._act_state[self._groups])
 
     def SetActivationLevelsAggregate(self, a):
         """Sets the muscles' levels (Dfobj.h:709-720); the levels the
@@ -71,6 +141,14 @@ class Dfobj:
         if a.size != n:
             raise ValueError("expected %d activation levels" % n)
         self._env._act_state[self._groups] = a
+
+    def _contours(self):
+        """This mesh's contour faces in its own vertex numbering (cached)."""
+        if getattr(self, "_cont", None) is None:
+            i, d = self._info, self._env._desc
+            tets = d.tets[i.tet_offset:i.tet_offset + i.n_tets] - i.vertex_offset
+            self._cont = scene.contour_faces(d.x_rest[i.vertex_offset:i.vertex_offset + i.n_vertices], tets)
+        return self._cont
 
     def _set_actions(self, actions):
         """Dfobj::SetActions for a muscle actuator (Dfobj.h:316-331): new
@@ -111,14 +189,22 @@ class Environment:
         self.mPhase = 0
         self._recorded = None
 
-    def SaveObj(self, filename: str):
-        """Write the current positions as an OBJ of vertices and tets."""
-        x = self.mSoftWorld.mX.reshape(-1, 3)
-        with open(filename, "w") as f:
-            for p in x:
-                f.write("v %f %f %f\n" % tuple(p))
-            for t in self._desc.tets:
-                f.write("t %d %d %d %d\n" % tuple(t))
+    def SaveObj(self, filename: str, x):
+        """Environment::SaveObj (Environment.h:296-326): for every Dfobj i,
+        `<filename>_DfObj<i>.obj` with its vertices taken from consecutive
+        segments of x (mNumUnknown values each) as `v` lines and its contour
+        faces (1-based, outward) as `f` lines."""
+        x = np.asarray(x, np.float64).ravel()
+        need = sum(d.mNumUnknown for d in self.mDfobjs)
+        if x.size < need:
+            raise ValueError("SaveObj needs %d values of x, got %d" % (need, x.size))
+        off = 0
+        for i, d in enumerate(self.mDfobjs):
+            seg = x[off:off + d.mNumUnknown].reshape(-1, 3)
+            with open("%s_DfObj%d.obj" % (filename, i), "w") as f:
+                f.writelines("v %06f %06f %06f\n" % (p[0], p[1], p[2]) for p in seg)
+                f.writelines("f %d %d %d\n" % (t[0] + 1, t[1] + 1, t[2] + 1) for t in d._contours())
+            off += d.mNumUnknown
 
     def Reset(self):
         """Environment::Reset (Environment.h:388-399): muscle levels and the
@@ -269,4 +355,4 @@ def vector2double(ev):
 
 
 def double2vector(v):
-    return np.asarray(v, np.float64).ravel().copy()
+    return EigenVector(v)

@@ -234,6 +234,28 @@ def activations_sinusoid(seed: int, steps: int, n_groups: int, R: int, h: float)
     return 0.5 * (1.0 + np.sin(2 * math.pi * f * t + ph))
 
 
+def contour_faces(x_rest: np.ndarray, tets: np.ndarray) -> np.ndarray:
+    """Surface triangles of a tet mesh as Dfobj::Initialize builds mContours
+    (Dfobj.h:497-585): each tet contributes the faces (s0,s1,s2), (s0,s1,s3),
+    (s0,s2,s3), (s1,s2,s3) of its sorted vertex ids, in tet order; a face no
+    other tet shares is kept and turned outward (q0 and q1 swap when
+    n = (q1 - q0) x (q2 - q1) points towards its tet's centre). Unshared faces
+    are found by sorting the 4T face keys instead of the all-pairs scan."""
+    x = np.asarray(x_rest, np.float64).reshape(-1, 3)
+    t = np.asarray(tets, np.int64).reshape(-1, 4)
+    st = np.sort(t, axis=1)
+    tri = st[:, [[0, 1, 2], [0, 1, 3], [0, 2, 3], [1, 2, 3]]].reshape(-1, 3)
+    _, inv, cnt = np.unique(tri, axis=0, return_inverse=True, return_counts=True)
+    keep = np.flatnonzero(cnt[inv.ravel()] == 1)
+    f = tri[keep].copy()
+    centre = x[t[keep // 4]].mean(axis=1)
+    q0, q1, q2 = x[f[:, 0]], x[f[:, 1]], x[f[:, 2]]
+    n = np.cross(q1 - q0, q2 - q1)
+    flip = np.einsum("ij,ij->i", centre - q0, n) > 0.0
+    f[flip, 0], f[flip, 1] = f[flip, 1], f[flip, 0].copy()
+    return f.astype(np.int32)
+
+
 def load_obj_tet(path: str, scale: float = 0.05):
     """``v x y z`` / ``t i0 i1 i2 i3`` (0-based) tet OBJ, vertices scaled by
     ``scale`` (OBJMesh.h:40-97; Dfobj.h:163-190 default Scale 0.05). Lines of

@@ -495,6 +495,33 @@ def test_mixed_adjoint_full_horizon_all_rollouts(eng, oracle_mod):
     assert rc == 0 and rel(gma["x0"][0], go["x0"]) < 1e-4 and rel(gma["act"][0], go["act"]) < 1e-4
 
 
+def test_rollout_backward_refuses_a_replaced_tape(eng):
+    """Two rollout nodes on one scene: the later node's backward runs first
+    and matches a fresh single-node run; the earlier node's tape is gone, so
+    its backward raises instead of returning the later rollout's gradients."""
+    import torch
+    from diff_fem_amd import api_diff
+    d = small_muscle_scene()
+    R, steps = 2, 6
+    s = eng.Scene(d, R=R)
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(5)
+    x0 = torch.tensor(np.tile(d.x_rest.ravel(), (R, 1)), device=dev, requires_grad=True)
+    v0 = torch.tensor(0.01 * rng.standard_normal((R, 3 * d.N)), device=dev, requires_grad=True)
+    E = torch.tensor(d.E, dtype=torch.float64)
+    act1 = torch.tensor(rng.uniform(0, 1, (R, steps, d.n_groups)), device=dev)
+    act2 = torch.tensor(rng.uniform(0, 1, (R, steps, d.n_groups)), device=dev)
+    xa, _ = api_diff.rollout(x0, v0, E, act1, s, steps)
+    xb, _ = api_diff.rollout(x0, v0, E, act2, s, steps)
+    with pytest.raises(RuntimeError, match="tape is gone"):
+        (xa.sum() + xb.sum()).backward()
+    # one node per forward works
+    x0.grad = None
+    xb, _ = api_diff.rollout(x0, v0, E, act2, s, steps)
+    xb.sum().backward()
+    assert torch.isfinite(x0.grad).all() and x0.grad.abs().max() > 0
+
+
 def test_split_scene_matches_one_scene(eng):
     """engine.SplitScene (two sub-batches, each on its own stream and host
     thread) against one Scene on the same batch with contact, graphs and a

@@ -276,3 +276,68 @@ def test_self_contact_batch(oracle_mod, precision):
     assert jac.sum() > 0
     with pytest.raises(RuntimeError, match="self contact"):
         s.backward_host(np.ones((R, 3 * d.N)), np.zeros((R, 3 * d.N)))
+
+
+def test_pybind_stub_forward_backward_matches_oracle(oracle_mod):
+    """The reference-side pybind11 binding (diff_fem_amd/pybind/pdm_pybind.cc)
+    over the C ABI: forward_all / backward_all of the 12x4x4 cantilever
+    against the oracle at the FP64 bar, dtau[0] the E gradient summed over
+    tets, dtau[1] the nu gradient."""
+    import importlib.util
+    from diff_fem_amd import _lib, build
+    _lib.load()
+    spec = importlib.util.spec_from_file_location("pdm_pybind", build.pybind_module_path())
+    m = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(m)
+    d = scene.config1()
+    steps = 10
+    env = m.Environment(steps, d.x_rest.ravel().tolist(), d.tets.ravel().tolist(), d.total_mass, d.h, d.max_iter,
+                        list(d.gravity), d.attach_idx.tolist(), d.attach_w, d.damping)
+    q0 = d.x_rest.ravel().tolist()
+    qd0 = [0.0] * len(q0)
+    tau = [1.2e5, 0.35]
+    qT, qdT = m.forward_all(q0, qd0, tau, env)
+    o = oracle_mod.OracleScene(d)
+    o.set_material(np.full(d.T, tau[0]), tau[1])
+    xo, vo, _ = o.rollout(np.asarray(q0), np.asarray(qd0), steps, record=True)
+    assert rel(qT, xo) < 1e-9 and rel(qdT, vo) < 1e-9
+    rng = np.random.default_rng(12)
+    gx, gv = rng.standard_normal(len(q0)), rng.standard_normal(len(q0))
+    dq, dqd, dtau = m.backward_all(q0, qd0, tau, gx.tolist(), gv.tolist(), env)
+    go, _ = o.backward(gx, gv)
+    assert rel(dq, go["x0"]) < 1e-9 and rel(dqd, go["v0"]) < 1e-9
+    assert abs(dtau[0] - go["E"].sum()) < 1e-9 * abs(go["E"].sum())
+    assert abs(dtau[1] - go["nu"]) < 1e-9 * abs(go["nu"])
+    with pytest.raises(RuntimeError):
+        m.backward_all(q0, qd0, tau, gx.tolist(), gv.tolist(), env)
+
+
+def test_octopus_demo_calls_run_unchanged(tmp_path):
+    """The calls of the reference's python/demo_octopus.py, unchanged: the
+    initial state from the world, sim_act steps with actions every 6 steps,
+    env.SaveObj(name, pd.double2vector(q.detach().numpy())) after each step,
+    the loss gradient by backward, env.mPhase reset."""
+    import torch
+    from diff_fem_amd import api_diff, pydifem as pd
+    steps = 8
+    env = pd.init_env(steps, "/data/octopus/octopus.json", ROOT)
+    f32 = torch.float32
+    init_x = torch.tensor(pd.vector2double(env.mSoftWorld.mX), dtype=f32, requires_grad=True)
+    init_v = torch.tensor(pd.vector2double(env.mSoftWorld.mV), dtype=f32, requires_grad=True)
+    init_jq = torch.tensor(pd.vector2double(env.mSoftWorld.mJointQ), dtype=f32, requires_grad=True)
+    init_tau = torch.tensor([0] * 16 * 4 * steps, dtype=f32, requires_grad=True)
+    taus = torch.reshape(init_tau, [steps, -1])
+    q, qd, jq, jqd = init_x, init_v, init_jq, init_jq
+    for i in range(steps):
+        q, qd, jq, jqd = api_diff.sim_act(q, qd, jq, jqd, taus[i], env, i % 6 == 0)
+        env.SaveObj(str(tmp_path / ("000_%03d" % i)), pd.double2vector(q.detach().numpy()))
+    target = torch.tensor([0.4, 0.8, 0.4], dtype=f32, requires_grad=True)
+    loss = (torch.reshape(q, [-1, 3]).mean(0) - target).norm()
+    loss.backward()
+    env.mPhase = 0
+    assert init_tau.grad.dtype == f32 and init_x.grad.dtype == f32
+    assert torch.isfinite(init_tau.grad).all() and init_tau.grad.abs().max() > 0
+    lines = open(tmp_path / "000_007_DfObj0.obj").read().splitlines()
+    nv = sum(1 for line in lines if line.startswith("v "))
+    nf = sum(1 for line in lines if line.startswith("f "))
+    assert nv == env._desc.N and nf == scene.contour_faces(env._desc.x_rest, env._desc.tets).shape[0]
