@@ -43,6 +43,7 @@ thread_local std::string g_err;
 constexpr double kEps = 5e-7;  // World.h:214
 constexpr int kThreads = 256;
 constexpr int kMaxSplit = 4;  // column chains of the split solve
+constexpr int kWslItem = 6144;  // doubles of W one grouped level-kernel block stages in LDS (48 KB)
 
 inline int cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
 
@@ -705,6 +706,290 @@ __global__ __launch_bounds__(1024) void k_bwd_subtree(FactorView f, const int2* 
     bwd_supernode(f, s, col0, ct, lg, C, B, sh);
     __syncthreads();
   }
+}
+
+// ----------------------------------------------------------------------------
+// Level kernels with one load phase. A block owns (supernode, a contiguous
+// range of its 16-row output tiles, one column tile). All it reads from global
+// memory it requests at once: the front rows it needs, the children's update
+// rows that add into them (pulled through per-row lists, in child order) and
+// the slice of W its tiles multiply with; then the MFMA chains run from LDS
+// and the results go straight to global memory. The arithmetic (values, their
+// order of summation, the MFMA chain over k) is that of fwd_supernode /
+// bwd_supernode, so results are bitwise the same; what changes is the number
+// of dependent global round trips (two instead of one per child plus one per
+// eight MFMA operand batches) and that a large supernode is spread over
+// several blocks. The forward writes y into Yb (not in place), so that blocks
+// of one supernode never overwrite rows another block still reads; the
+// backward reads its own rows' y from Yb and the rows below from B (final
+// x of the ancestors) and writes x into B.
+struct PullView {
+  const long long* pbase;  // per supernode: offset of its m x maxp slot table in ptab
+  const int* maxp;         // per supernode: slots per front row (its most contributions to one row)
+  const int* ptab;         // per front row, maxp global update-block rows in child order (-1: none)
+};
+constexpr int kPullFast = 4;  // slots whose loads are issued together; more go one by one
+
+// value of front row r (< m) at column c: own b (rows < w) plus the pulls.
+// The slot indices are read with the own values, then the update rows of up
+// to kPullFast slots at once (two dependent round trips); the sum runs in
+// slot (child) order.
+__device__ __forceinline__ void load_rows_pulled(const double* __restrict__ B, const double* __restrict__ U,
+                                                 const int* __restrict__ ptab, int maxp, int c0, int w, int r0,
+                                                 int nrows, int m, int lg, int ct, int C, int col0,
+                                                 double* __restrict__ T, TileIdx ti) {
+  const int Ct = 1 << lg;
+  const bool cok = ti.c < ct;
+  const int np = min(maxp, kPullFast);
+  for (int rb = ti.r0; rb < nrows; rb += 4 * ti.rs) {
+    double v[4];
+    int q[4][kPullFast];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int j = rb + u * ti.rs;
+      const int r = r0 + j;
+      const bool ok = cok && j < nrows && r < m;
+      v[u] = ok && r < w ? B[(long long)(c0 + r) * C + col0 + ti.c] : 0.0;
+#pragma unroll
+      for (int k = 0; k < kPullFast; ++k) q[u][k] = ok && k < np ? ptab[(long long)r * maxp + k] : -1;
+    }
+    double a[4][kPullFast];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int k = 0; k < kPullFast; ++k) a[u][k] = q[u][k] >= 0 ? U[(long long)q[u][k] * C + col0 + ti.c] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+#pragma unroll
+      for (int k = 0; k < kPullFast; ++k)
+        if (q[u][k] >= 0) v[u] += a[u][k];
+      const int j = rb + u * ti.rs;
+      const int r = r0 + j;
+      if (cok && j < nrows && r < m)
+        for (int k = kPullFast; k < maxp; ++k) {
+          const int qq = ptab[(long long)r * maxp + k];
+          if (qq >= 0) v[u] += U[(long long)qq * C + col0 + ti.c];
+        }
+      if (j < nrows) T[j * Ct + ti.c] = v[u];
+    }
+  }
+}
+
+// Forward tiles [t0, t1) of supernode s: tiles below nrt = wpad / 16 are the
+// y rows (inv(L_ss) T_top, written to Yb), the others update rows (written to
+// U). LDS: T_top (wpad x Ct), the group's update rows (16 per update tile x
+// Ct), then W per tile as [k][16] (k < wpad).
+__device__ __forceinline__ void fwd_group(const FactorView& f, const PullView& p, int s, int t0, int t1, int col0,
+                                          int ct, int lg, int C, const double* __restrict__ B, double* __restrict__ Yb,
+                                          double* __restrict__ U, double* __restrict__ sh) {
+  const int Ct = 1 << lg;
+  const int c0 = f.first[s];
+  const int w = f.first[s + 1] - c0;
+  const int m = (int)(f.rowptr[s + 1] - f.rowptr[s]);
+  const int wpad = (w + 15) & ~15;
+  const int nrt = wpad >> 4;
+  const int u0 = max(t0, nrt) - nrt;  // first update tile of the group
+  const int nupd = t1 - max(t0, nrt);
+  const int ntg = t1 - t0;
+  double* Ttop = sh;
+  double* Tupd = sh + wpad * Ct;
+  double* Wsl = Tupd + (nupd > 0 ? nupd : 0) * 16 * Ct;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int lane = tid & 63, wave = tid >> 6, nwave = nt >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int nq = Ct >> 4;
+  const TileIdx ti = tile_idx(tid, nt, lg);
+  const int* ptab = p.ptab + p.pbase[s];
+  const int maxp = p.maxp[s];
+  load_rows_pulled(B, U, ptab, maxp, c0, w, 0, wpad, w, lg, ct, C, col0, Ttop, ti);
+  if (nupd > 0) load_rows_pulled(B, U, ptab, maxp, c0, w, w + 16 * u0, 16 * nupd, m, lg, ct, C, col0, Tupd, ti);
+  {
+    // W slice: tile g rows (y tile: rt + r, update tile: w + 16 (t - nrt) + r), k < wpad
+    const double* Ws = f.W + f.lptr[s];
+    const int tot = ntg * wpad * 16;
+    for (int e0 = tid; e0 < tot; e0 += 8 * nt) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int e = e0 + u * nt;
+        double val = 0.0;
+        if (e < tot) {
+          const int r = e & 15, k = (e >> 4) % wpad, g = (e >> 4) / wpad;
+          const int t = t0 + g;
+          const int row = t < nrt ? 16 * t + r : w + 16 * (t - nrt) + r;
+          const bool ok = k < w && (t < nrt ? row < w : row < m);
+          val = ok ? Ws[(long long)k * m + row] : 0.0;
+        }
+        v[u] = val;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int e = e0 + u * nt;
+        if (e < tot) Wsl[e] = v[u];
+      }
+    }
+  }
+  __syncthreads();
+  for (int tq = wave; tq < ntg * nq; tq += nwave) {
+    const int g = tq / nq, q = tq % nq;
+    const int t = t0 + g;
+    const double* pa = Wsl + (size_t)g * wpad * 16 + lr;
+    const double* pb = Ttop + q * 16 + lr;
+    const int col = q * 16 + lr;
+    if (t < nrt) {
+      const int rt = 16 * t;
+      v4d acc = {0.0, 0.0, 0.0, 0.0};
+      acc = mfma_chain(acc, 0, min(w, rt + 16), lk, [&](int k) { return pa[k * 16]; },
+                       [&](int k) { return pb[k * Ct]; });
+      if (col < ct) {
+        double* Yo = Yb + col0 + col;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int r = rt + 4 * i + lk;
+          if (r < w) Yo[(long long)(c0 + r) * C] = acc[i];
+        }
+      }
+    } else {
+      const int ru = 16 * (t - nrt);  // update row offset (rows w + ru ...)
+      const double* pu = Tupd + (size_t)(ru - 16 * u0) * Ct + q * 16 + lr;
+      v4d acc;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int a = ru + 4 * i + lk;
+        acc[i] = w + a < m ? pu[(4 * i + lk) * Ct] : 0.0;
+      }
+      acc = mfma_chain(acc, 0, w, lk, [&](int k) { return pa[k * 16]; }, [&](int k) { return pb[k * Ct]; });
+      if (col < ct) {
+        double* Us = U + f.uptr[s] * C + col0 + col;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int a = ru + 4 * i + lk;
+          if (w + a < m) Us[(long long)a * C] = acc[i];
+        }
+      }
+    }
+  }
+}
+
+// Backward tiles [t0, t1) of supernode s (output rows 16 t ..., < w):
+// x = W^T [y_s; x_below]. LDS: the front (mpad x Ct: y_s from Yb, the rows
+// below from B), then W per tile as [k - rt][16] for k in [rt, m).
+__device__ __forceinline__ void bwd_group(const FactorView& f, int s, int t0, int t1, int col0, int ct, int lg, int C,
+                                          double* __restrict__ B, const double* __restrict__ Yb,
+                                          double* __restrict__ sh) {
+  const int Ct = 1 << lg;
+  const int c0 = f.first[s];
+  const int w = f.first[s + 1] - c0;
+  const long long rp = f.rowptr[s];
+  const int m = (int)(f.rowptr[s + 1] - rp);
+  const int mpad = (m + 15) & ~15;
+  const int* rows = f.rows + rp;
+  const double* WRs = f.WR + f.lptr[s];
+  double* T = sh;
+  double* Wsl = sh + mpad * Ct;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int lane = tid & 63, wave = tid >> 6, nwave = nt >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int nq = Ct >> 4;
+  const TileIdx ti = tile_idx(tid, nt, lg);
+  {
+    const bool cok = ti.c < ct;
+    for (int rb = ti.r0; rb < mpad; rb += 8 * ti.rs) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int r = rb + u * ti.rs;
+        const bool ok = cok && r < m;
+        const double* src = r < w ? Yb : B;
+        v[u] = ok ? src[(long long)rows[r] * C + col0 + ti.c] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int r = rb + u * ti.rs;
+        if (r < mpad) T[r * Ct + ti.c] = v[u];
+      }
+    }
+  }
+  // W slice offsets: tile g starts after the (m - 16 (t0 + j)) * 16 values of the tiles before it
+  const int rt0 = 16 * t0;
+  const int tot = [&]() {
+    int n = 0;
+    for (int t = t0; t < t1; ++t) n += (m - 16 * t) * 16;
+    return n;
+  }();
+  for (int e0 = tid; e0 < tot; e0 += 8 * nt) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = e0 + u * nt;
+      double val = 0.0;
+      if (e < tot) {
+        // locate tile g: offsets sum over j < g of (m - rt0 - 16 j) * 16
+        int g = 0, base = 0;
+        while (base + (m - rt0 - 16 * g) * 16 <= e) {
+          base += (m - rt0 - 16 * g) * 16;
+          ++g;
+        }
+        const int rt = rt0 + 16 * g;
+        const int kk = (e - base) >> 4, cc = (e - base) & 15;
+        const int col = rt + cc;
+        val = col < w ? WRs[(long long)(rt + kk) * w + col] : 0.0;
+      }
+      v[u] = val;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = e0 + u * nt;
+      if (e < tot) Wsl[e] = v[u];
+    }
+  }
+  __syncthreads();
+  const int ntg = t1 - t0;
+  for (int tq = wave; tq < ntg * nq; tq += nwave) {
+    const int g = tq / nq, q = tq % nq;
+    const int rt = rt0 + 16 * g;
+    int base = 0;
+    for (int j = 0; j < g; ++j) base += (m - rt0 - 16 * j) * 16;
+    const double* pa = Wsl + base + lr;  // pa[(k - rt) * 16] = W^T(rt + lr, k) for k >= rt
+    const double* pb = T + q * 16 + lr;
+    v4d acc = {0.0, 0.0, 0.0, 0.0};
+    acc = mfma_chain(acc, rt, m, lk, [&](int k) { return pa[(k - rt) * 16]; }, [&](int k) { return pb[k * Ct]; });
+    const int col = q * 16 + lr;
+    if (col < ct) {
+      double* Bo = B + col0 + col;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int r = rt + 4 * i + lk;
+        if (r < w) Bo[(long long)(c0 + r) * C] = acc[i];
+      }
+    }
+  }
+}
+
+// items: (supernode, first tile, end tile, 0) per block row; block = item x column tile
+__global__ __launch_bounds__(1024) void k_fwd_level_g(FactorView f, PullView p, const int4* __restrict__ items,
+                                                     int ntile, int lg, int C, const double* __restrict__ B,
+                                                     double* __restrict__ Yb, double* __restrict__ U,
+                                                     const int* __restrict__ any, int cbase, int cw) {
+  if (any && *any == 0) return;
+  extern __shared__ double sh[];
+  const int Ct = 1 << lg;
+  const int lb = xcd_block();
+  const int4 it = items[lb / ntile];
+  const int col0 = cbase + (lb % ntile) * Ct;
+  fwd_group(f, p, it.x, it.y, it.z, col0, min(Ct, cbase + cw - col0), lg, C, B, Yb, U, sh);
+}
+
+__global__ __launch_bounds__(1024) void k_bwd_level_g(FactorView f, const int4* __restrict__ items, int ntile, int lg,
+                                                     int C, double* __restrict__ B, const double* __restrict__ Yb,
+                                                     const int* __restrict__ any, int cbase, int cw) {
+  if (any && *any == 0) return;
+  extern __shared__ double sh[];
+  const int Ct = 1 << lg;
+  const int lb = xcd_block();
+  const int4 it = items[lb / ntile];
+  const int col0 = cbase + (lb % ntile) * Ct;
+  bwd_group(f, it.x, it.y, it.z, col0, min(Ct, cbase + cw - col0), lg, C, B, Yb, sh);
 }
 
 __global__ void k_set_int(int* __restrict__ p, int v) {
@@ -2402,20 +2687,19 @@ struct DevBuf {
 };
 
 // Grow a device buffer to `count` elements keeping its first `keep` elements.
-// The old buffer goes to `grave` and is freed at the end of the call (a free
-// here would wait for the whole device, stalling other scenes' streams; the
-// copy out of it is ordered on st before any later use).
+// The old buffer is freed at once, after the copy out of it: keeping it to
+// the end of the call would hold both generations of a whole-rollout contact
+// tape at the same time. Growth happens only when a step needs more contact
+// iterations than any step before it (in practice the first call of a batch).
 template <typename X>
-int grow_keep(DevBuf<X>& b, size_t count, size_t keep, hipStream_t st, std::vector<void*>& grave) {
+int grow_keep(DevBuf<X>& b, size_t count, size_t keep, hipStream_t st) {
   if (count <= b.n && b.p) return 0;
   DevBuf<X> nb;
   if (nb.alloc(count)) return 1;
   if (keep && b.p) HIPX(hipMemcpyAsync(nb.p, b.p, keep * sizeof(X), hipMemcpyDeviceToDevice, st));
+  HIPX(hipStreamSynchronize(st));
   std::swap(b.p, nb.p);
   std::swap(b.n, nb.n);
-  if (nb.p) grave.push_back((void*)nb.p);
-  nb.p = nullptr;
-  nb.n = 0;
   return 0;
 }
 
@@ -2464,6 +2748,13 @@ struct pdm_scene_s {
   DevBuf<long long> f_rowptr, f_lptr, f_relptr, f_uptr;
   DevBuf<double> f_W, f_WR;
   std::vector<int> lev_ptr, lev_rows;  // lev_rows: max (mpad + wpad) over the level's supernodes
+  // grouped level kernels: per sweep (0 fwd, 1 bwd) the work items of every
+  // level (supernode, tile range) at item_ptr[pass][l], and per level the
+  // largest front rows and W-slice doubles of one item (their LDS need)
+  DevBuf<long long> f_pbase;
+  DevBuf<int> f_maxp, f_ptab;
+  DevBuf<int4> f_items[2];
+  std::vector<int> item_ptr[2], item_front[2], item_wsl[2];
   // solve launch shape per level and sweep (autotuned per right-hand-side count)
   struct LevelCfg {
     int Ct = 16, nthr = 256;
@@ -2472,7 +2763,7 @@ struct pdm_scene_s {
   int tuned_C = -1;
   bool autotune = true;
   // dynamic
-  DevBuf<double> X, V, Xn, Cc, Xit, B, Lx, U, part, err;
+  DevBuf<double> X, V, Xn, Cc, Xit, B, Lx, U, Ys, part, err;  // Ys: y of the grouped forward sweep
   DevBuf<char> corner;
   DevBuf<int> active, iters_dev;
   std::vector<int> iters_host;  // R*steps of the last forward
@@ -2549,11 +2840,6 @@ struct pdm_scene_s {
   // PDM_USE_GRAPH: captured step block and its device-side controls
   hipGraphExec_t gexec[2] = {nullptr, nullptr};  // untaped / taped step block
   long long graph_launches = 0;                  // step-block replays since the scene was made
-  std::vector<void*> grave;                      // device buffers replaced during a call, freed at its end
-  void flush_grave() {
-    for (void* p : grave) (void)hipFree(p);
-    grave.clear();
-  }
   const double* g_act = nullptr;
   int g_act_steps = -1;
   bool capturing = false;
@@ -2564,6 +2850,7 @@ struct pdm_scene_s {
   long long n_solves = 0;
   double solve_ms = 0.0;
 
+  PullView pview() const { return PullView{f_pbase.p, f_maxp.p, f_ptab.p}; }
   FactorView view() const {
     return FactorView{f_W.p, f_WR.p, f_first.p, f_rowptr.p, f_rows.p, f_lptr.p, f_child_ptr.p, f_child.p,
                       f_relptr.p, f_rel.p, f_uptr.p};
@@ -2721,6 +3008,69 @@ int upload_factor(pdm_scene s) {
     }
   if (old_ptr != s->lev_ptr || old_rows != s->lev_rows) s->tuned_C = -1;  // level structure changed
   {
+    // pull tables: for every front row of every supernode, the update rows of
+    // its children that add into it, in child order, padded with -1 to the
+    // supernode's largest count per row
+    int ns = (int)F.first.size() - 1;
+    std::vector<long long> pbase(ns + 1, 0);
+    std::vector<int> maxp(ns, 0), ptab;
+    std::vector<std::vector<int>> per;
+    for (int sn = 0; sn < ns; ++sn) {
+      int m = (int)(F.rowptr[sn + 1] - F.rowptr[sn]);
+      per.assign(m, {});
+      for (int q = F.child_ptr[sn]; q < F.child_ptr[sn + 1]; ++q) {
+        int ch = F.child[q];
+        int wc = F.first[ch + 1] - F.first[ch];
+        int mbc = (int)(F.rowptr[ch + 1] - F.rowptr[ch]) - wc;
+        for (int a = 0; a < mbc; ++a) per[F.rel[F.relptr[ch] + a]].push_back((int)(F.uptr[ch] + a));
+      }
+      int mp = 0;
+      for (int r = 0; r < m; ++r) mp = std::max(mp, (int)per[r].size());
+      maxp[sn] = mp;
+      pbase[sn] = (long long)ptab.size();
+      for (int r = 0; r < m; ++r)
+        for (int k = 0; k < mp; ++k) ptab.push_back(k < (int)per[r].size() ? per[r][k] : -1);
+    }
+    pbase[ns] = (long long)ptab.size();
+    if (s->f_pbase.upload(pbase.data(), pbase.size()) || s->f_maxp.upload(maxp.data(), std::max<size_t>(1, maxp.size())) ||
+        s->f_ptab.upload(ptab.data(), std::max<size_t>(1, ptab.size())))
+      return 1;
+    // work items: consecutive 16-row output tiles of one supernode, as many as
+    // keep the W slice within kWslItem doubles (at least one tile)
+    for (int pass = 0; pass < 2; ++pass) {
+      std::vector<int4> items;
+      s->item_ptr[pass].assign(nlev + 1, 0);
+      s->item_front[pass].assign(nlev, 16);
+      s->item_wsl[pass].assign(nlev, 0);
+      for (int l = 0; l < nlev; ++l) {
+        for (int q = F.level_ptr[l]; q < F.level_ptr[l + 1]; ++q) {
+          int sn = F.level_sn[q];
+          int m = (int)(F.rowptr[sn + 1] - F.rowptr[sn]), w = F.first[sn + 1] - F.first[sn];
+          int wpad = (w + 15) & ~15, mpad = (m + 15) & ~15, nrt = wpad / 16, nbt = (m - w + 15) / 16;
+          int ntl = pass == 0 ? nrt + nbt : nrt;
+          int t = 0;
+          while (t < ntl) {
+            int t1 = t, wsl = 0;
+            while (t1 < ntl) {
+              int per_tile = pass == 0 ? wpad * 16 : (m - 16 * t1) * 16;
+              if (t1 > t && wsl + per_tile > kWslItem) break;
+              wsl += per_tile;
+              ++t1;
+            }
+            items.push_back(make_int4(sn, t, t1, 0));
+            int nupd = pass == 0 ? std::max(0, t1 - std::max(t, nrt)) : 0;
+            int front = pass == 0 ? wpad + 16 * nupd : mpad;
+            s->item_front[pass][l] = std::max(s->item_front[pass][l], front);
+            s->item_wsl[pass][l] = std::max(s->item_wsl[pass][l], wsl);
+            t = t1;
+          }
+        }
+        s->item_ptr[pass][l + 1] = (int)items.size();
+      }
+      if (s->f_items[pass].upload(items.data(), std::max<size_t>(1, items.size()))) return 1;
+    }
+  }
+  {
     // subtrees for every cut level c: roots of height < c whose parent has
     // height >= c (or none); a subtree is the postorder range [first, root]
     int ns = (int)F.first.size() - 1;
@@ -2877,6 +3227,7 @@ int ensure_batch_buffers(pdm_scene s) {
                         s->GWf.alloc((size_t)s->nfaces * s->C)))
     return 1;
   if (s->U.alloc(std::max<long long>(1, s->F.uptr.back()) * (size_t)s->C)) return 1;
+  if (s->Ys.alloc(nc)) return 1;
   // forward slots are FP32 only in PDM_MIXED; the FP32 reverse sweep of
   // PDM_MIXED_ADJOINT writes FP32 slots into the same (larger) buffer
   size_t esz = s->precision == PDM_MIXED ? sizeof(float) : sizeof(double);
@@ -2932,6 +3283,27 @@ pdm_scene_s::LevelCfg level_default(const pdm_scene_s* s, int l, int C) {
 }
 pdm_scene_s::LevelCfg level_default(const pdm_scene_s* s, int l) { return level_default(s, l, s->C); }
 
+// The grouped level kernels (k_fwd_level_g, k_bwd_level_g): LDS = the item's
+// front rows x Ct plus its W slice; the default shape follows level_default.
+size_t level_lds_g(const pdm_scene_s* s, int pass, int l, int Ct) {
+  return ((size_t)s->item_front[pass][l] * Ct + (size_t)s->item_wsl[pass][l]) * 8;
+}
+pdm_scene_s::LevelCfg level_default_g(const pdm_scene_s* s, int pass, int l) {
+  int cnt = s->item_ptr[pass][l + 1] - s->item_ptr[pass][l];
+  int ncol16 = cdiv(s->C, 16);
+  int nq = 4;
+  while (nq > 1 && level_lds_g(s, pass, l, 16 * nq) > kLdsSoft) nq /= 2;
+  while (nq > 1 && (long long)cnt * cdiv(ncol16, nq) < 512) nq /= 2;
+  pdm_scene_s::LevelCfg c;
+  c.Ct = 16 * nq;
+  size_t lds = level_lds_g(s, pass, l, c.Ct);
+  int per_cu = (int)std::max<size_t>(1, std::min<size_t>(8, kLdsMax / std::max<size_t>(lds, 1)));
+  int waves = std::min(16, std::max(4, 16 / per_cu));
+  if ((long long)cnt * cdiv(s->C, c.Ct) < 256) waves = 16;
+  c.nthr = 64 * waves;
+  return c;
+}
+
 // One level of one sweep over the columns [cbase, cbase + cw) of C on stream
 // st, with the update blocks in Up.
 int launch_level_into(pdm_scene s, int pass, int l, const pdm_scene_s::LevelCfg& c, int C, double* Bp, double* Up,
@@ -2958,9 +3330,32 @@ int launch_level_into(pdm_scene s, int pass, int l, const pdm_scene_s::LevelCfg&
                        lg, C, Bp, any, cbase, cw);
   return 0;
 }
+// One level of one sweep with the grouped kernels (the solve's own path; y
+// goes through the scene's Yb).
 int launch_level(pdm_scene s, int pass, int l, const pdm_scene_s::LevelCfg& c, double* Bp, const int* any, int cbase,
                  int cw, hipStream_t st) {
-  return launch_level_into(s, pass, l, c, s->C, Bp, s->U.p, any, cbase, cw, st);
+  const FactorView fv = s->view();
+  int cnt = s->item_ptr[pass][l + 1] - s->item_ptr[pass][l];
+  size_t lds = level_lds_g(s, pass, l, c.Ct);
+  if (lds > kLdsMax) {
+    g_err = "supernode front exceeds the LDS budget";
+    return 1;
+  }
+  int ntile = cdiv(cw, c.Ct);
+  int lg = 0;
+  while ((1 << lg) < c.Ct) ++lg;
+  if ((1 << lg) != c.Ct || c.nthr % c.Ct != 0) {
+    g_err = "invalid solve launch shape";
+    return 1;
+  }
+  if (cnt == 0 || ntile == 0) return 0;
+  if (pass == 0)
+    hipLaunchKernelGGL(k_fwd_level_g, dim3(cnt * ntile), dim3(c.nthr), lds, st, fv, s->pview(),
+                       s->f_items[0].p + s->item_ptr[0][l], ntile, lg, s->C, Bp, s->Ys.p, s->U.p, any, cbase, cw);
+  else
+    hipLaunchKernelGGL(k_bwd_level_g, dim3(cnt * ntile), dim3(c.nthr), lds, st, fv,
+                       s->f_items[1].p + s->item_ptr[1][l], ntile, lg, s->C, Bp, s->Ys.p, any, cbase, cw);
+  return 0;
 }
 
 // Subtree kernels of cut level `cut` over the columns [cbase, cbase + cw).
@@ -3021,6 +3416,8 @@ int set_level_attrs() {
     HIPX(hipFuncSetAttribute((const void*)k_bwd_level, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsMax));
     HIPX(hipFuncSetAttribute((const void*)k_fwd_subtree, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsMax));
     HIPX(hipFuncSetAttribute((const void*)k_bwd_subtree, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsMax));
+    HIPX(hipFuncSetAttribute((const void*)k_fwd_level_g, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsMax));
+    HIPX(hipFuncSetAttribute((const void*)k_bwd_level_g, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsMax));
     attr_set = true;
   }
   return 0;
@@ -3037,7 +3434,10 @@ int autotune_levels(pdm_scene s) {
   int nlev = (int)s->lev_ptr.size() - 1;
   s->cfg_fwd.resize(nlev);
   s->cfg_bwd.resize(nlev);
-  for (int l = 0; l < nlev; ++l) s->cfg_fwd[l] = s->cfg_bwd[l] = level_default(s, l);
+  for (int l = 0; l < nlev; ++l) {
+    s->cfg_fwd[l] = level_default_g(s, 0, l);
+    s->cfg_bwd[l] = level_default_g(s, 1, l);
+  }
   if (set_level_attrs()) return 1;
   s->cut = 0;
   s->nsplit = 1;
@@ -3048,6 +3448,7 @@ int autotune_levels(pdm_scene s) {
   if (set_level_attrs()) return 1;
   HIPX(hipMemsetAsync(s->B.p, 0, s->NC() * 8, s->stream));
   HIPX(hipMemsetAsync(s->U.p, 0, std::max<long long>(1, s->F.uptr.back()) * (size_t)s->C * 8, s->stream));
+  HIPX(hipMemsetAsync(s->Ys.p, 0, s->NC() * 8, s->stream));
   hipEvent_t e0, e1;
   HIPX(hipEventCreate(&e0));
   HIPX(hipEventCreate(&e1));
@@ -3061,7 +3462,7 @@ int autotune_levels(pdm_scene s) {
           pdm_scene_s::LevelCfg c;
           c.Ct = 16 * nq;
           c.nthr = nthr;
-          if (level_lds(s, l, c.Ct) > kLdsMax) continue;
+          if (level_lds_g(s, pass, l, c.Ct) > kLdsMax) continue;
           if (launch_level(s, pass, l, c, s->B.p, nullptr, 0, s->C, s->stream)) return 1;  // warm-up
           HIPX(hipEventRecord(e0, s->stream));
           for (int r = 0; r < reps; ++r)
@@ -3279,7 +3680,7 @@ int contact_phase(pdm_scene s, const double* act, int act_steps, int step, Conta
     tp->P = 0;
     tp->first.clear();
     tp->pass_of.clear();
-    if (grow_keep(tp->xit, nc, 0, s->stream, s->grave)) return 1;
+    if (grow_keep(tp->xit, nc, 0, s->stream)) return 1;
     if (dcopy(s, tp->xit.p, s->Xit.p, nc)) return 1;
   }
   for (int pass = 0; pass <= S.outer_cap; ++pass) {
@@ -3316,10 +3717,10 @@ int contact_phase(pdm_scene s, const double* act, int act_steps, int step, Conta
       // store of x_it fills the later slots for every rollout that reads them
       if (tp->G > 0 && !self && dcopy(s, tp->xit.p + (size_t)tp->G * nc, s->Xit.p, nc)) return 1;
       size_t g1 = (size_t)tp->G + S.max_iter1;
-      if (grow_keep(tp->xit, (g1 + 1) * nc, (tp->G + 1) * nc, s->stream, s->grave) ||
-          grow_keep(tp->rc, g1 * cvc, tp->G * cvc, s->stream, s->grave) ||
-          grow_keep(tp->nrm, (size_t)(tp->P + 1) * cvc, (size_t)tp->P * cvc, s->stream, s->grave) ||
-          grow_keep(tp->exec, g1 * R, (size_t)tp->G * R, s->stream, s->grave))
+      if (grow_keep(tp->xit, (g1 + 1) * nc, (tp->G + 1) * nc, s->stream) ||
+          grow_keep(tp->rc, g1 * cvc, tp->G * cvc, s->stream) ||
+          grow_keep(tp->nrm, (size_t)(tp->P + 1) * cvc, (size_t)tp->P * cvc, s->stream) ||
+          grow_keep(tp->exec, g1 * R, (size_t)tp->G * R, s->stream))
         return 1;
     }
     for (int i = 0; i < S.max_iter1; ++i) {
@@ -3362,7 +3763,7 @@ int contact_phase(pdm_scene s, const double* act, int act_steps, int step, Conta
     if (tp) tp->P += 1;
   }
   if (tp) {
-    if (grow_keep(tp->flast, R, 0, s->stream, s->grave)) return 1;
+    if (grow_keep(tp->flast, R, 0, s->stream)) return 1;
     HIPX(hipMemcpyAsync(tp->flast.p, s->clast.p, R * sizeof(int), hipMemcpyDeviceToDevice, s->stream));
   }
   if (dcopy(s, s->Xit.p, s->Ori.p, nc)) return 1;
@@ -3590,7 +3991,6 @@ int forward_impl(pdm_scene s, int steps, const double* x0, const double* v0, con
     s->have_tape = true;
     s->tape_steps = steps;
   }
-  s->flush_grave();  // the stream was synchronised above
   return 0;
 }
 
@@ -3652,7 +4052,7 @@ int contact_reverse(pdm_scene s, ContactTape& tp, const double* Xn, const double
   inj.assign((size_t)G * R, 0);
   for (int g = 0; g < G; ++g)
     for (int r = 0; r < R; ++r) inj[(size_t)g * R + r] = (g == glast[r] && !fl[r]) ? 1 : 0;
-  if (grow_keep(s->cinjG, inj.size(), 0, s->stream, s->grave)) return 1;
+  if (s->cinjG.alloc(inj.size())) return 1;
   DevBuf<int>& dinj = s->cinjG;
   HIPX(hipMemcpyAsync(dinj.p, inj.data(), inj.size() * sizeof(int), hipMemcpyHostToDevice, s->stream));
   for (int g = G - 1; g >= 0; --g) {
@@ -3823,10 +4223,6 @@ int backward_impl(pdm_scene s, const double* gxT, const double* gvT, double* gx0
   float ms = 0;
   HIPX(hipEventElapsedTime(&ms, s->ev0, s->ev1));
   s->t_bwd = ms;
-  if (!s->grave.empty()) {
-    HIPX(hipStreamSynchronize(s->stream));
-    s->flush_grave();
-  }
   return 0;
 }
 
@@ -3970,7 +4366,6 @@ void pdm_scene_destroy(pdm_scene s) {
   }
   if (s->ev_fork) (void)hipEventDestroy(s->ev_fork);
   drop_step_graph(s);
-  s->flush_grave();
   if (s->ev0) (void)hipEventDestroy(s->ev0);
   if (s->ev1) (void)hipEventDestroy(s->ev1);
   if (s->own_stream && s->stream) (void)hipStreamDestroy(s->stream);
