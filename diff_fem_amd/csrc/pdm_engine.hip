@@ -2673,10 +2673,11 @@ struct DevBuf {
     n = count;
     return 0;
   }
+  // h may be null (an empty host vector padded to one element): zero-filled
   int upload(const X* h, size_t count) {
     if (alloc(count)) return 1;
     if (count) {
-      hipError_t e = hipMemcpy(p, h, count * sizeof(X), hipMemcpyHostToDevice);
+      hipError_t e = h ? hipMemcpy(p, h, count * sizeof(X), hipMemcpyHostToDevice) : hipMemset(p, 0, count * sizeof(X));
       if (e != hipSuccess) {
         g_err = std::string("hipMemcpy failed: ") + hipGetErrorString(e);
         return 1;
