@@ -1402,7 +1402,10 @@ __device__ bool detect_plane_d(const double* pl, const double* x0, const double*
   return true;
 }
 
-__device__ bool detect_sphere_d(const double* sp, const double* x0, const double* x1, double* n) {
+// t_out: the contact time (a crossing root of |x0 + t (x1 - x0) - c| = r, or 1
+// for end-of-step proximity); root_out: whether t is such a root.
+__device__ bool detect_sphere_d(const double* sp, const double* x0, const double* x1, double* n, double* t_out,
+                                int* root_out) {
   double d[3], e[3];
   for (int r = 0; r < 3; ++r) {
     d[r] = x1[r] - x0[r];
@@ -1437,6 +1440,8 @@ __device__ bool detect_sphere_d(const double* sp, const double* x0, const double
       double vn = (p[0] * d[0] + p[1] * d[1] + p[2] * d[2]) / len;
       double sg = vn > 0 ? -1.0 : 1.0;
       for (int r = 0; r < 3; ++r) n[r] = sg * p[r] / len;
+      *t_out = t;
+      *root_out = k < nt - 1;
       return true;
     }
   }
@@ -1573,7 +1578,8 @@ __global__ void k_contact_detect(const int* __restrict__ cvtx, int ncv, const do
                                  const int* __restrict__ bvh_b, const int* __restrict__ bvh_tri,
                                  int* __restrict__ win_st, int* __restrict__ win_o, double* __restrict__ win_n,
                                  int* __restrict__ nseen, int* __restrict__ seen, int pass,
-                                 int* __restrict__ cactive, int* __restrict__ fresh) {
+                                 int* __restrict__ cactive, int* __restrict__ fresh, int* __restrict__ win_sph,
+                                 double* __restrict__ win_t, double* __restrict__ win_x1) {
   long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (idx >= (long long)ncv * R) return;
   int ci = (int)(idx / R), r = (int)(idx % R);
@@ -1586,12 +1592,20 @@ __global__ void k_contact_detect(const int* __restrict__ cvtx, int ncv, const do
   int wst = win_st[idx], wo = win_o[idx], ns = nseen[idx];
   double wn[3] = {win_n[3 * idx], win_n[3 * idx + 1], win_n[3 * idx + 2]};
   bool any = false, over = false;
-  auto record = [&](int o, const double* n) {
+  // sphere data of the winner's latest detection (win_sph = 4 (pass * nsph +
+  // sphere) + 2 root + 1, or 0 for another obstacle): the reverse sweep
+  // differentiates its normal
+  int wsph = win_sph ? win_sph[idx] : 0;
+  double wt = 0.0;
+  auto record = [&](int o, const double* n, int sinfo, double t) {
     any = true;
     for (int k = 0; k < ns; ++k)
       if (seen[(long long)k * ncv * R + idx] == o) {
-        if (o == wo)
+        if (o == wo) {
           for (int c = 0; c < 3; ++c) wn[c] = n[c];
+          wsph = sinfo;
+          wt = t;
+        }
         return;
       }
     if (ns == kSeen) {
@@ -1605,13 +1619,18 @@ __global__ void k_contact_detect(const int* __restrict__ cvtx, int ncv, const do
       wst = st;
       wo = o;
       for (int c = 0; c < 3; ++c) wn[c] = n[c];
+      wsph = sinfo;
+      wt = t;
     }
   };
   double n[3];
   for (int o = 0; o < np; ++o)
-    if (detect_plane_d(planes + 6 * o, x0, x1, n)) record(o, n);
-  for (int o = 0; o < nsph; ++o)
-    if (detect_sphere_d(spheres + 4 * o, x0, x1, n)) record(np + o, n);
+    if (detect_plane_d(planes + 6 * o, x0, x1, n)) record(o, n, 0, 0.0);
+  for (int o = 0; o < nsph; ++o) {
+    double t;
+    int root;
+    if (detect_sphere_d(spheres + 4 * o, x0, x1, n, &t, &root)) record(np + o, n, 4 * (pass * nsph + o) + 2 * root + 1, t);
+  }
   if (ntri > 0) {
     // swept box of the vertex, widened by the thickness and a margin for the
     // single-precision box test, against the BVH's double boxes
@@ -1639,7 +1658,7 @@ __global__ void k_contact_detect(const int* __restrict__ cvtx, int ncv, const do
       }
       for (int q = -1 - a, qe = -1 - a + bvh_b[k]; q < qe; ++q) {
         int t = bvh_tri[q];
-        if (detect_tri_d(tris + 9 * (long long)t, x0, x1, n)) record(np + nsph + t, n);
+        if (detect_tri_d(tris + 9 * (long long)t, x0, x1, n)) record(np + nsph + t, n, 0, 0.0);
       }
     }
   }
@@ -1649,18 +1668,25 @@ __global__ void k_contact_detect(const int* __restrict__ cvtx, int ncv, const do
   win_n[3 * idx] = wn[0];
   win_n[3 * idx + 1] = wn[1];
   win_n[3 * idx + 2] = wn[2];
+  if (win_sph && win_sph[idx] != wsph) {  // a new sphere detection won: keep where it read
+    win_sph[idx] = wsph;
+    win_t[idx] = wt;
+    for (int c = 0; c < 3; ++c) win_x1[3 * idx + c] = x1[c];
+  }
   if (any) atomicOr(&fresh[r], 1);
   if (over) atomicOr(&cactive[R], 1);
 }
 
 __global__ void k_contact_reset(int* __restrict__ win_st, int* __restrict__ win_o, int* __restrict__ nseen, long long n,
                                 int* __restrict__ cactive, int* __restrict__ cfirst, int* __restrict__ fresh,
-                                int* __restrict__ clast, int* __restrict__ npass, int* __restrict__ citers, int R) {
+                                int* __restrict__ clast, int* __restrict__ npass, int* __restrict__ citers, int R,
+                                int* __restrict__ win_sph) {
   long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (idx < n) {
     win_st[idx] = -1;
     win_o[idx] = -1;
     nseen[idx] = 0;
+    if (win_sph) win_sph[idx] = 0;
   }
   if (idx < R) {
     cactive[idx] = 1;
@@ -1671,6 +1697,23 @@ __global__ void k_contact_reset(int* __restrict__ win_st, int* __restrict__ win_
     citers[idx] = 0;
   }
   if (idx == 0) cactive[R] = 0;
+}
+
+// Sphere data of the winning normals at the start of a pass (0 where the
+// winner is no sphere), and per rollout whether the detection read x_it after
+// the last executed iteration (clast = 1) or that iteration's input.
+__global__ void k_contact_tape_sph(const int* __restrict__ win_st, const int* __restrict__ win_sph,
+                                   const double* __restrict__ win_t, const double* __restrict__ win_x1,
+                                   const int* __restrict__ clast, long long ni, int R, int* __restrict__ tSi,
+                                   double* __restrict__ tSt, double* __restrict__ tSx1, int* __restrict__ tDl) {
+  long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx < ni) {
+    const int si = win_st[idx] >= 0 ? win_sph[idx] : 0;
+    tSi[idx] = si;
+    tSt[idx] = si ? win_t[idx] : 0.0;
+    for (int c = 0; c < 3; ++c) tSx1[3 * idx + c] = si ? win_x1[3 * idx + c] : 0.0;
+  }
+  if (idx < R) tDl[idx] = clast[idx];
 }
 
 // ------------------------------------------------------------ self contact
@@ -2503,6 +2546,67 @@ __device__ void cone_vjp(const double* n, double mu, const double* f_in, const d
   for (int c = 0; c < 3; ++c) out[c] = e0[c] * g0 + t[c] * g1 + e2[c] * g2;
 }
 
+// Adjoint of the cone projection with respect to its unit normal n (branch
+// frozen): take-off (f) and stick (-f) replies do not depend on n; the sliding
+// reply out = -phi n + mu phi e (phi = n.f, e = f_t/|f_t|, f_t = f - phi n) gives
+// nbar = f (-ob.n + mu ob.e - g.n) - phi (ob + g), g = mu phi (ob - e (e.ob))/|f_t|.
+__device__ void cone_nvjp(const double* n, double mu, const double* f, const double* ob, double* nbar) {
+  nbar[0] = nbar[1] = nbar[2] = 0.0;
+  const double phi = n[0] * f[0] + n[1] * f[1] + n[2] * f[2];
+  if (phi > 0.0) return;
+  double ft[3];
+  for (int r = 0; r < 3; ++r) ft[r] = f[r] - phi * n[r];
+  const double tau = sqrt(ft[0] * ft[0] + ft[1] * ft[1] + ft[2] * ft[2]);
+  if (!(tau > mu * (-phi))) return;
+  const double e[3] = {ft[0] / tau, ft[1] / tau, ft[2] / tau};
+  const double obn = ob[0] * n[0] + ob[1] * n[1] + ob[2] * n[2];
+  const double obe = ob[0] * e[0] + ob[1] * e[1] + ob[2] * e[2];
+  const double sc = mu * phi / tau;
+  const double coef = -obn + mu * obe - sc * obn;
+  for (int r = 0; r < 3; ++r) nbar[r] = f[r] * coef - phi * (ob[r] + sc * (ob[r] - e[r] * obe));
+}
+
+// Adjoint of a sphere normal n = sg p/|p|, p = x0 + t (x1 - x0) - c, to x0 and
+// x1: t = 1 is fixed; a crossing root of |p(t)| = r moves as dt = -p.q / p.d
+// (d = x1 - x0, q = (1-t) dx0 + t dx1), so q's adjoint is w = pbar - p (pbar.d)/(p.d).
+__device__ void sphere_normal_vjp(const double* c, const double* x0, const double* x1, double t, int root,
+                                  const double* n, const double* nbar, double* x0bar, double* x1bar) {
+  double p[3], d[3];
+  for (int r = 0; r < 3; ++r) {
+    d[r] = x1[r] - x0[r];
+    p[r] = x0[r] + t * d[r] - c[r];
+  }
+  const double len = sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+  const double sg = (n[0] * p[0] + n[1] * p[1] + n[2] * p[2]) > 0.0 ? 1.0 : -1.0;
+  const double nn = n[0] * nbar[0] + n[1] * nbar[1] + n[2] * nbar[2];
+  double pb[3];
+  for (int r = 0; r < 3; ++r) pb[r] = sg * (nbar[r] - n[r] * nn) / len;
+  if (!root) {
+    for (int r = 0; r < 3; ++r) x1bar[r] = pb[r];
+    return;
+  }
+  const double pd = p[0] * d[0] + p[1] * d[1] + p[2] * d[2];
+  const double bd = pb[0] * d[0] + pb[1] * d[1] + pb[2] * d[2];
+  for (int r = 0; r < 3; ++r) {
+    const double w = pb[r] - p[r] * bd / pd;
+    x0bar[r] = (1.0 - t) * w;
+    x1bar[r] = t * w;
+  }
+}
+
+// Adds the adjoint of the positions a detection pass read (NB, per contact
+// vertex and rollout) into the running x_it adjoint of the rollouts whose
+// detection read x_it (want = 1) or the last iteration's input (want = 0).
+__global__ void k_cadj_nb(double* __restrict__ XB, const double* __restrict__ NB, const int* __restrict__ dl, int want,
+                          const int* __restrict__ cvtx, long long ni, int R) {
+  long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx >= ni) return;
+  const int ci = (int)(idx / R), r = (int)(idx % R);
+  if (dl[r] != want) return;
+  const long long base = (long long)cvtx[ci] * 3 * R + 3 * r;
+  for (int c = 0; c < 3; ++c) XB[base + c] += NB[3 * idx + c];
+}
+
 __global__ void k_cadj_init(double* __restrict__ XB, const double* __restrict__ Yb, const int* __restrict__ flast,
                             long long n, int C) {
   long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
@@ -2520,7 +2624,10 @@ __global__ void k_cadj_pre(const double* __restrict__ XB, double* __restrict__ P
                            const double* __restrict__ Kdiag, const int* __restrict__ cv_of_v,
                            const double* __restrict__ tRC, const double* __restrict__ tN, double mu, double alpha,
                            double h, int first, const int* __restrict__ alist, const int* __restrict__ acount,
-                           double* __restrict__ Zb, double* __restrict__ Zs, double* __restrict__ CD, int N, int R) {
+                           double* __restrict__ Zb, double* __restrict__ Zs, double* __restrict__ CD, int N, int R,
+                           const double* __restrict__ spheres, int nsph, const int* __restrict__ tSi,
+                           const double* __restrict__ tSt, const double* __restrict__ tSx1, double* __restrict__ NB,
+                           long long nbstride) {
   long long idx = (long long)xcd_block() * blockDim.x + threadIdx.x;
   if (idx >= (long long)N * R) return;
   int v = (int)(idx / R);
@@ -2543,6 +2650,22 @@ __global__ void k_cadj_pre(const double* __restrict__ XB, double* __restrict__ P
   if (ci >= 0) {
     long long tb = (long long)ci * C + 3 * r;
     cone_vjp(tN + tb, mu, tRC + tb, q, rcb);
+    const long long wi = (long long)ci * R + r;
+    const int si = tSi ? tSi[wi] : 0;
+    if (si > 0) {  // sphere normal: it moves with x_n and the position its detection read
+      double nb[3];
+      cone_nvjp(tN + tb, mu, tRC + tb, q, nb);
+      if (nb[0] != 0.0 || nb[1] != 0.0 || nb[2] != 0.0) {
+        const int k = (si - 1) >> 2, o = k % nsph, ps = k / nsph;
+        double x0b[3] = {0.0, 0.0, 0.0}, x1b[3];
+        sphere_normal_vjp(spheres + 4 * o, Xn + base, tSx1 + 3 * wi, tSt[wi], ((si - 1) >> 1) & 1, tN + tb, nb, x0b,
+                          x1b);
+        for (int c = 0; c < 3; ++c) {
+          Xb[base + c] += x0b[c];
+          NB[ps * nbstride + tb + c] += x1b[c];
+        }
+      }
+    }
   }
   for (int c = 0; c < 3; ++c) {
     if (first) {
@@ -2719,6 +2842,11 @@ struct ContactTape {
   DevBuf<double> nrm;          // P x ncv x C: the winning normals of each pass
   DevBuf<int> exec;            // G x R
   DevBuf<int> flast;           // R: result is x_it of the last executed iteration
+  // with spheres, per pass: the winners' sphere data (code, contact time,
+  // position the detection read; ncv x R, 3 ncv x R) and per rollout whether
+  // that position was x_it (1) or the last iteration's input (0)
+  DevBuf<int> sinfo, dl;
+  DevBuf<double> st, sx1;
 };
 
 template <typename X>
@@ -2801,7 +2929,9 @@ struct pdm_scene_s {
   DevBuf<double> d_tris, d_bvh_box;
   DevBuf<int> d_bvh_a, d_bvh_b, d_bvh_tri;
   std::vector<std::unique_ptr<ContactTape>> ctape;  // per step of the segment being reversed
-  DevBuf<double> cXB, cPB, cOB, cGf, cCD;
+  DevBuf<double> cXB, cPB, cOB, cGf, cCD, cNB;
+  DevBuf<int> win_sph;
+  DevBuf<double> win_t, win_x1;
   DevBuf<int> cinjG;          // contact reverse: injection flags per (iteration, rollout)
   std::vector<int> cinj_h;
   DevBuf<double> Kdiag, d_planes, d_spheres, Ori, Dn, Prev, win_n;
@@ -3243,6 +3373,7 @@ int ensure_batch_buffers(pdm_scene s) {
         s->iactive.alloc(s->R) || s->cfirst.alloc(s->R) || s->fresh.alloc(s->R) || s->npass.alloc(s->R) ||
         s->citers.alloc(s->R) || s->clast.alloc(s->R) || s->cinj.alloc(s->R))
       return 1;
+    if (!s->S.spheres.empty() && (s->win_sph.alloc(ni) || s->win_t.alloc(ni) || s->win_x1.alloc(3 * ni))) return 1;
   }
   if (s->ncs > 0) {
     size_t na = (size_t)s->ncs * s->R, nf = (size_t)s->nsf * s->R;
@@ -3666,7 +3797,7 @@ int contact_phase(pdm_scene s, const double* act, int act_steps, int step, Conta
   if (dcopy(s, s->Ori.p, s->Xit.p, nc)) return 1;
   hipLaunchKernelGGL(k_contact_reset, dim3(cdiv(std::max<long long>(ni, R), kThreads)), dim3(kThreads), 0, s->stream,
                      s->win_st.p, s->win_o.p, s->nseen.p, ni, s->cactive.p, s->cfirst.p, s->fresh.p, s->clast.p,
-                     s->npass.p, s->citers.p, R);
+                     s->npass.p, s->citers.p, R, nsph > 0 ? s->win_sph.p : nullptr);
   const bool self = s->ncs > 0;
   if (self) {
     long long na = (long long)s->ncs * R;
@@ -3688,7 +3819,8 @@ int contact_phase(pdm_scene s, const double* act, int act_steps, int step, Conta
     hipLaunchKernelGGL(k_contact_detect, dim3(cdiv((long long)s->ncv * R, kThreads)), dim3(kThreads), 0, s->stream,
                        s->cvtx.p, s->ncv, s->Xn.p, s->Ori.p, R, s->d_planes.p, np, s->d_spheres.p, nsph, s->d_tris.p,
                        s->ntri, s->d_bvh_box.p, s->d_bvh_a.p, s->d_bvh_b.p, s->d_bvh_tri.p, s->win_st.p, s->win_o.p,
-                       s->win_n.p, s->nseen.p, s->seen.p, pass, s->cactive.p, s->fresh.p);
+                       s->win_n.p, s->nseen.p, s->seen.p, pass, s->cactive.p, s->fresh.p,
+                       nsph > 0 ? s->win_sph.p : nullptr, s->win_t.p, s->win_x1.p);
     if (self) {
       long long na = (long long)s->ncs * R, nf = (long long)s->nsf * R;
       hipLaunchKernelGGL(k_self_detect, dim3(cdiv(na, kThreads)), dim3(kThreads), 0, s->stream, s->d_sc_vtx.p,
@@ -3723,6 +3855,16 @@ int contact_phase(pdm_scene s, const double* act, int act_steps, int step, Conta
           grow_keep(tp->nrm, (size_t)(tp->P + 1) * cvc, (size_t)tp->P * cvc, s->stream) ||
           grow_keep(tp->exec, g1 * R, (size_t)tp->G * R, s->stream))
         return 1;
+      if (nsph > 0) {
+        size_t P = (size_t)tp->P;
+        if (grow_keep(tp->sinfo, (P + 1) * ni, P * ni, s->stream) || grow_keep(tp->st, (P + 1) * ni, P * ni, s->stream) ||
+            grow_keep(tp->sx1, (P + 1) * 3 * ni, P * 3 * ni, s->stream) ||
+            grow_keep(tp->dl, (P + 1) * R, P * R, s->stream))
+          return 1;
+        hipLaunchKernelGGL(k_contact_tape_sph, dim3(cdiv(std::max<long long>(ni, R), kThreads)), dim3(kThreads), 0,
+                           s->stream, s->win_st.p, s->win_sph.p, s->win_t.p, s->win_x1.p, s->clast.p, ni, R,
+                           tp->sinfo.p + P * ni, tp->st.p + P * ni, tp->sx1.p + P * 3 * ni, tp->dl.p + P * R);
+      }
     }
     for (int i = 0; i < S.max_iter1; ++i) {
       hipLaunchKernelGGL(k_contact_begin, ugrid, ublock, 0, s->stream, s->Xit.p, s->Ori.p, s->iactive.p, s->part.p, N,
@@ -4028,8 +4170,11 @@ int vjp_iteration(pdm_scene s, const int* active, const double* Yk, const double
 }
 
 
-// Reverse of the taped contact phase of one step (branches and contact
-// normals frozen). In: Yb = adjoint of the step's final positions. Out: Yb =
+int contact_reverse_iter(pdm_scene s, ContactTape& tp, int g, const double* Xn, const double* act, int act_steps,
+                         int step, const int* dinj, bool sph, int nsph);
+
+// Reverse of the taped contact phase of one step (branches frozen; sphere
+// normals differentiated, other normals constant). In: Yb = adjoint of the step's final positions. Out: Yb =
 // adjoint of the PD result; Xb, Zs, gEpart, gMpart accumulate.
 int contact_reverse(pdm_scene s, ContactTape& tp, const double* Xn, const double* act, int act_steps, int step) {
   HostScene& S = s->S;
@@ -4056,17 +4201,56 @@ int contact_reverse(pdm_scene s, ContactTape& tp, const double* Xn, const double
   if (s->cinjG.alloc(inj.size())) return 1;
   DevBuf<int>& dinj = s->cinjG;
   HIPX(hipMemcpyAsync(dinj.p, inj.data(), inj.size() * sizeof(int), hipMemcpyHostToDevice, s->stream));
+  // sphere normals: adjoints of the positions each detection pass read, added
+  // into the x_it adjoint where the reverse sweep reaches those positions
+  const int nsph = (int)(S.spheres.size() / 4);
+  const bool sph = nsph > 0 && tp.P > 0;
+  const long long ni = (long long)s->ncv * R;
+  if (sph) {
+    if (s->cNB.alloc((size_t)tp.P * cvc)) return 1;
+    HIPX(hipMemsetAsync(s->cNB.p, 0, (size_t)tp.P * cvc * 8, s->stream));
+  }
+  auto add_nb = [&](int pass, int want) {
+    hipLaunchKernelGGL(k_cadj_nb, dim3(cdiv(ni, kThreads)), dim3(kThreads), 0, s->stream, s->cXB.p,
+                       s->cNB.p + (size_t)pass * cvc, tp.dl.p + (size_t)pass * R, want, s->cvtx.p, ni, R);
+  };
+  int pend_input = -1;  // pass whose input-reading rollouts get their NB after the next iteration
   for (int g = G - 1; g >= 0; --g) {
     bool any = false;
     for (int r = 0; r < R; ++r) any |= ex[(size_t)g * R + r] != 0;
-    if (!any) continue;
+    if (any && contact_reverse_iter(s, tp, g, Xn, act, act_steps, step, dinj.p, sph, nsph)) return 1;
+    if (sph) {
+      if (pend_input >= 0) add_nb(pend_input, 0);
+      pend_input = -1;
+      if (tp.first[g]) {
+        add_nb(tp.pass_of[g], 1);
+        pend_input = tp.pass_of[g];
+      }
+    }
+  }
+  if (dcopy(s, s->Yb.p, s->cXB.p, nc)) return 1;
+  HIPX(hipGetLastError());
+  return 0;
+}
+
+// One executed contact iteration g of the reverse sweep.
+int contact_reverse_iter(pdm_scene s, ContactTape& tp, int g, const double* Xn, const double* act, int act_steps,
+                         int step, const int* dinj, bool sph, int nsph) {
+  HostScene& S = s->S;
+  int N = S.N, C = s->C, R = s->R;
+  size_t nc = s->NC(), cvc = (size_t)s->ncv * C;
+  const long long ni = (long long)s->ncv * R;
+  const size_t pg = (size_t)tp.pass_of[g];
+  {
     const double* ori = tp.xit.p + (size_t)g * nc;
     hipLaunchKernelGGL(k_compact, dim3(1), dim3(1024), 0, s->stream, tp.exec.p + (size_t)g * R, R, s->alist.p,
                        s->acount.p);
     hipLaunchKernelGGL((R % 64 == 0 ? k_cadj_pre<true> : k_cadj_pre<false>), dim3(cdiv((long long)N * R, kThreads)), dim3(kThreads), 0, s->stream, s->cXB.p,
                        s->cPB.p, s->Xb.p, tp.xit.p + (size_t)(g + 1) * nc, Xn, s->Kdiag.p, s->cv_of_v.p,
-                       tp.rc.p + (size_t)g * cvc, tp.nrm.p + (size_t)tp.pass_of[g] * cvc, S.mu, S.alpha, S.h, (int)tp.first[g],
-                       s->alist.p, s->acount.p, s->Zb.p, s->Zs.p, s->cCD.p, N, R);
+                       tp.rc.p + (size_t)g * cvc, tp.nrm.p + pg * cvc, S.mu, S.alpha, S.h, (int)tp.first[g],
+                       s->alist.p, s->acount.p, s->Zb.p, s->Zs.p, s->cCD.p, N, R, s->d_spheres.p, nsph,
+                       sph ? tp.sinfo.p + pg * ni : nullptr, sph ? tp.st.p + pg * ni : nullptr,
+                       sph ? tp.sx1.p + pg * 3 * ni : nullptr, s->cNB.p, (long long)cvc);
     hipLaunchKernelGGL((R % 64 == 0 ? k_cadj_cz<true> : k_cadj_cz<false>), dim3(cdiv(nc, kThreads)), dim3(kThreads), 0, s->stream, s->Lptr.p, s->Lcol.p,
                        s->Lval.p, s->Zb.p, s->Xb.p, s->cOB.p, s->alist.p, s->acount.p, N, C);
     int rc = s->precision != PDM_FP64
@@ -4074,10 +4258,8 @@ int contact_reverse(pdm_scene s, ContactTape& tp, const double* Xn, const double
                  : vjp_iteration<double>(s, nullptr, ori, ori, act, act_steps, step, s->cCD.p, s->cOB.p, Xn);
     if (rc) return 1;
     hipLaunchKernelGGL(k_cadj_post, dim3(cdiv(nc, kThreads)), dim3(kThreads), 0, s->stream, s->cOB.p, s->cGf.p,
-                       dinj.p + (size_t)g * R, s->alist.p, s->acount.p, s->cXB.p, (long long)nc, C);
+                       dinj + (size_t)g * R, s->alist.p, s->acount.p, s->cXB.p, (long long)nc, C);
   }
-  if (dcopy(s, s->Yb.p, s->cXB.p, nc)) return 1;
-  HIPX(hipGetLastError());
   return 0;
 }
 

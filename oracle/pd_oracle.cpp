@@ -33,8 +33,9 @@
 // dense solves, invariants, central finite differences; the muscle target, the
 // muscle tet selection, the contact Jacobi loop and the regularisation ladder
 // by hand arithmetic, closed forms of their rules and independent computations
-// in tests/test_oracle_pins.py). Parity unpinned: sphere-contact gradients
-// against finite differences (the normal's derivative is frozen, as executed).
+// in tests/test_oracle_pins.py; sphere-contact gradients, whose normal moves
+// with the detected positions, against central differences in
+// tests/test_oracle.py).
 #include <algorithm>
 #include <array>
 #include <cmath>
@@ -340,7 +341,15 @@ struct Muscle {
 struct Plane { double p[3], n[3]; };
 struct Sphere { double c[3], r; };
 
-struct Impact { int vtx; int obs; double n[3]; };
+// sph >= 0: sphere obstacle, normal n = sg p/|p| with p = x0 + t (x1 - x0) - c,
+// x1 the contact-phase position P_det (P_0 = PD result, P_{j+1} = output of
+// taped contact iteration j), t a crossing root of |p| = r (root) or 1.
+struct Impact {
+  int vtx; int obs; double n[3];
+  int sph = -1, det = 0;
+  double t = 1.0;
+  bool root = false;
+};
 
 // Skyline Cholesky of the scalar K_s in RCM order.
 struct Skyline {
@@ -411,6 +420,9 @@ struct ContactIter {
   bool first = false;
   std::vector<int> cv;        // vertices with an impact
   std::vector<double> B, f;   // 9 and 3 per cv
+  std::vector<int> sph, det;  // per cv: sphere index (-1: other obstacle), detection position
+  std::vector<double> st;     // per cv: sphere contact time t
+  std::vector<char> sroot;    // per cv: t is a crossing root
 };
 
 struct StepTape {
@@ -697,7 +709,8 @@ bool detect_plane(const Plane& pl, const double* x0, const double* x1, double* n
   return true;
 }
 
-bool detect_sphere(const Sphere& sp, const double* x0, const double* x1, double* n) {
+bool detect_sphere(const Sphere& sp, const double* x0, const double* x1, double* n, double* t_out = nullptr,
+                   bool* root_out = nullptr) {
   double d[3], e[3];
   for (int r = 0; r < 3; ++r) {
     d[r] = x1[r] - x0[r];
@@ -732,6 +745,8 @@ bool detect_sphere(const Sphere& sp, const double* x0, const double* x1, double*
       double vn = nf[0] * d[0] + nf[1] * d[1] + nf[2] * d[2];
       double sg = vn > 0 ? -1.0 : 1.0;
       for (int r = 0; r < 3; ++r) n[r] = sg * nf[r];
+      if (t_out) *t_out = t;
+      if (root_out) *root_out = k < nt - 1;
       return true;
     }
   }
@@ -1381,6 +1396,10 @@ void step(Scene& s, double* x, double* v, const double* act, StepTape* tape, int
     bool last_begin = true;
     while (true) {
       std::vector<Impact> fresh;
+      // position read by this detection: the last taped output, or the input
+      // of the last taped iteration when the pass ran out of iterations
+      const int det_k = tape ? (int)tape->citers.size() - (last_begin ? 0 : 1) : 0;
+      const int nplanes = (int)s.planes.size();
       for (int ci = 0; ci < (int)s.contact_vtx.size(); ++ci) {
         int vtx = s.contact_vtx[ci];
         double nrm[3];
@@ -1390,7 +1409,16 @@ void step(Scene& s, double* x, double* v, const double* act, StepTape* tape, int
           ++ob;
         }
         for (const Sphere& sp : s.spheres) {
-          if (detect_sphere(sp, &xn[3 * vtx], &ori_x[3 * vtx], nrm)) fresh.push_back({vtx, ob, {nrm[0], nrm[1], nrm[2]}});
+          double tt = 1.0;
+          bool rt = false;
+          if (detect_sphere(sp, &xn[3 * vtx], &ori_x[3 * vtx], nrm, &tt, &rt)) {
+            Impact im{vtx, ob, {nrm[0], nrm[1], nrm[2]}};
+            im.sph = ob - nplanes;
+            im.det = det_k;
+            im.t = tt;
+            im.root = rt;
+            fresh.push_back(im);
+          }
           ++ob;
         }
         for (const auto& tr : s.tris) {
@@ -1567,6 +1595,11 @@ void step(Scene& s, double* x, double* v, const double* act, StepTape* tape, int
             ci.cv.push_back(kv.first);
             ci.B.insert(ci.B.end(), &Bs[9 * kv.second], &Bs[9 * kv.second] + 9);
             ci.f.insert(ci.f.end(), &orig[3 * kv.first], &orig[3 * kv.first] + 3);
+            const Impact& w = impacts[kv.second];
+            ci.sph.push_back(w.sph);
+            ci.det.push_back(w.det);
+            ci.st.push_back(w.t);
+            ci.sroot.push_back(w.root ? 1 : 0);
           }
           tape->citers.push_back(std::move(ci));
         }
@@ -1614,6 +1647,56 @@ void project_contact_vjp(const double* B, double mu, const double* f_in, const d
   for (int r = 0; r < 3; ++r) out[r] = B[r * 3 + 0] * fb[0] + B[r * 3 + 1] * fb[1] + B[r * 3 + 2] * fb[2];
 }
 
+// Adjoint of the cone projection with respect to its unit normal n (branch
+// frozen). The take-off reply f and the stick reply -f do not depend on n; the
+// sliding reply is out = -phi n + mu phi e with phi = n.f, e = f_t/|f_t|,
+// f_t = f - phi n, whose differential in n gives
+// nbar = f (-ob.n + mu ob.e - g.n) - phi (ob + g), g = mu phi (ob - e (e.ob))/|f_t|.
+void project_contact_nvjp(const double* n, double mu, const double* f, const double* ob, double* nbar) {
+  nbar[0] = nbar[1] = nbar[2] = 0.0;
+  const double phi = n[0] * f[0] + n[1] * f[1] + n[2] * f[2];
+  if (phi > 0.0) return;
+  double ft[3];
+  for (int r = 0; r < 3; ++r) ft[r] = f[r] - phi * n[r];
+  const double tau = std::sqrt(ft[0] * ft[0] + ft[1] * ft[1] + ft[2] * ft[2]);
+  if (!(tau > mu * (-phi))) return;
+  double e[3] = {ft[0] / tau, ft[1] / tau, ft[2] / tau};
+  const double obn = ob[0] * n[0] + ob[1] * n[1] + ob[2] * n[2];
+  const double obe = ob[0] * e[0] + ob[1] * e[1] + ob[2] * e[2];
+  const double sc = mu * phi / tau;
+  const double coef = -obn + mu * obe - sc * obn;
+  for (int r = 0; r < 3; ++r) nbar[r] = f[r] * coef - phi * (ob[r] + sc * (ob[r] - e[r] * obe));
+}
+
+// Adjoint of a sphere contact normal n = sg p/|p|, p = x0 + t (x1 - x0) - c, to
+// the two positions: t = 1 moves with neither, a crossing root t of
+// |p(t)| = r moves as dt = -p.q / p.d (d = x1 - x0, q = (1-t) dx0 + t dx1), so
+// the adjoint of q is w = pbar - p (pbar.d)/(p.d).
+void sphere_normal_vjp(const double* c, const double* x0, const double* x1, double t, bool root, const double* n,
+                       const double* nbar, double* x0bar, double* x1bar) {
+  double p[3], d[3];
+  for (int r = 0; r < 3; ++r) {
+    d[r] = x1[r] - x0[r];
+    p[r] = x0[r] + t * d[r] - c[r];
+  }
+  const double len = std::sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+  const double sg = (n[0] * p[0] + n[1] * p[1] + n[2] * p[2]) > 0.0 ? 1.0 : -1.0;
+  const double nn = n[0] * nbar[0] + n[1] * nbar[1] + n[2] * nbar[2];
+  double pb[3];
+  for (int r = 0; r < 3; ++r) pb[r] = sg * (nbar[r] - n[r] * nn) / len;
+  if (!root) {
+    for (int r = 0; r < 3; ++r) x1bar[r] += pb[r];
+    return;
+  }
+  const double pd = p[0] * d[0] + p[1] * d[1] + p[2] * d[2];
+  const double bd = pb[0] * d[0] + pb[1] * d[1] + pb[2] * d[2];
+  for (int r = 0; r < 3; ++r) {
+    const double w = pb[r] - p[r] * bd / pd;
+    x0bar[r] += (1.0 - t) * w;
+    x1bar[r] += t * w;
+  }
+}
+
 // Reverse sweep of the taped contact iterations. In: ybar = adjoint of the
 // step's final positions. Out: ybar = adjoint of the PD result y; xbar, zsum
 // and the parameter gradients accumulate.
@@ -1625,9 +1708,13 @@ void contact_backward(Scene& s, const StepTape& tp, std::vector<double>& ybar, s
   const double* act = tp.act.empty() ? nullptr : tp.act.data();
   int J = (int)tp.citers.size();
   std::vector<double> gfin = ybar, XB(n3, 0.0), PB(n3, 0.0), rcb(n3), z(n3), coefD(N), ob(n3);
+  // adjoints of the contact-phase positions P_k read by sphere detections
+  std::vector<std::vector<double>> pend(J + 1);
   if (tp.final_latest) XB = gfin;
   for (int j = J - 1; j >= 0; --j) {
     const ContactIter& it = tp.citers[j];
+    if (!pend[j + 1].empty())
+      for (int k = 0; k < n3; ++k) XB[k] += pend[j + 1][k];
     for (int k = 0; k < n3; ++k) xbar[k] += XB[k];
     for (int v = 0; v < N; ++v) {
       double cd = 0;
@@ -1643,6 +1730,18 @@ void contact_backward(Scene& s, const StepTape& tp, std::vector<double>& ybar, s
       double qb[3] = {al * PB[3 * v], al * PB[3 * v + 1], al * PB[3 * v + 2]}, o[3];
       project_contact_vjp(&it.B[9 * q], s.mu, &it.f[3 * q], qb, o);
       for (int c = 0; c < 3; ++c) rcb[3 * v + c] = o[c];
+      if (it.sph[q] >= 0) {  // the normal moves with the positions it was detected on
+        const double* B = &it.B[9 * q];
+        double n[3] = {B[0], B[3], B[6]}, nb[3];
+        project_contact_nvjp(n, s.mu, &it.f[3 * q], qb, nb);
+        if (nb[0] != 0.0 || nb[1] != 0.0 || nb[2] != 0.0) {
+          const int k = it.det[q];
+          const std::vector<double>& Pk = k < J ? tp.citers[k].ori : tp.citers[J - 1].xit;
+          if (pend[k].empty()) pend[k].assign(n3, 0.0);
+          sphere_normal_vjp(s.spheres[it.sph[q]].c, &xn[3 * v], &Pk[3 * v], it.st[q], it.sroot[q] != 0, n, nb,
+                            &xbar[3 * v], &pend[k][3 * v]);
+        }
+      }
     }
     if (it.first) {
       for (int k = 0; k < n3; ++k) {
@@ -1704,6 +1803,8 @@ void contact_backward(Scene& s, const StepTape& tp, std::vector<double>& ybar, s
     if (j == J - 1 && !tp.final_latest)
       for (int k = 0; k < n3; ++k) XB[k] += gfin[k];
   }
+  if (!pend[0].empty())
+    for (int k = 0; k < n3; ++k) XB[k] += pend[0][k];
   ybar = XB;
 }
 
@@ -2403,7 +2504,8 @@ int orc_max_threads() {
 // Gradient of <gx, x_T> + <gv, v_T> w.r.t. x0, v0, per-tet E, nu (the
 // forward value entering mMu) and activations (steps x n_groups), over the tape
 // of the last recorded rollout; contact iterations are reversed with their
-// branches (take-off / stick / slide) and contact normals frozen.
+// branches (take-off / stick / slide) frozen and sphere normals differentiated
+// through the positions they were detected on.
 int orc_backward(void* p, const double* gxT, const double* gvT, double* gx0, double* gv0, double* gE, double* gnu,
                  double* gact) {
   Scene* s = (Scene*)p;

@@ -195,8 +195,8 @@ def test_contact_forward_matches_oracle(eng, oracle_mod, sphere):
 
 @pytest.mark.parametrize("sphere", [False, True])
 def test_contact_gradient_matches_oracle(eng, oracle_mod, sphere):
-    """Adjoint through the contact phase (branches and normals frozen, as in
-    the oracle's reverse sweep) at the FP64 bar."""
+    """Adjoint through the contact phase (branches frozen, sphere normals
+    differentiated, as in the oracle's reverse sweep) at the FP64 bar."""
     d = contact_scene(sphere=sphere)
     R = 2
     s = eng.Scene(d, R=R, flags=0, checkpoint_every=5)
@@ -218,6 +218,41 @@ def test_contact_gradient_matches_oracle(eng, oracle_mod, sphere):
         assert rel(g["x0"][r], go["x0"]) < 1e-9
         assert rel(g["v0"][r], go["v0"]) < 1e-9
         gE += go["E"]
+    assert rel(g["E"], gE) < 1e-9
+
+
+@pytest.mark.parametrize("max_iter1", [10, 3])
+def test_sphere_normal_gradient_matches_oracle(eng, oracle_mod, max_iter1):
+    """Sphere contact normals move with the positions their detection read
+    (start-of-step positions and a contact iterate, through the crossing
+    time); the oracle's reverse sweep, pinned against central differences,
+    carries that derivative. A tumbling bar on a small sphere, two rollouts
+    whose contact passes end differently (out of iterations or converged):
+    FP64 bar on positions, iteration counts and gradients."""
+    from contact_scenes import sphere_tumble_scene
+    d, v = sphere_tumble_scene(max_iter1=max_iter1)
+    R = 2
+    s = eng.Scene(d, R=R, flags=0, checkpoint_every=6)
+    o = oracle_mod.OracleScene(d)
+    rng = np.random.default_rng(21)
+    x0 = np.tile(d.x_rest.ravel(), (R, 1))
+    v0 = np.stack([v, 0.8 * v])
+    steps = 12
+    xT, vT = s.forward_host(x0, v0, steps, record=True)
+    passes, jit = s.contact_stats()
+    gx = rng.standard_normal((R, 3 * d.N))
+    gv = 0.01 * rng.standard_normal((R, 3 * d.N))
+    g = s.backward_host(gx, gv)
+    gE = np.zeros(d.T)
+    for r in range(R):
+        xo, vo, io = o.rollout(x0[r], v0[r], steps, record=True)
+        assert np.array_equal(passes[r], io[:, 2]) and np.array_equal(jit[r], io[:, 1])
+        assert rel(xT[r], xo) < 1e-9
+        go, _ = o.backward(gx[r], gv[r])
+        assert rel(g["x0"][r], go["x0"]) < 1e-9, r
+        assert rel(g["v0"][r], go["v0"]) < 1e-9, r
+        gE += go["E"]
+    assert passes.max() >= 2
     assert rel(g["E"], gE) < 1e-9
 
 

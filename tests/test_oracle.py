@@ -8,6 +8,8 @@ import pytest
 
 from diff_fem_amd import scene
 
+from contact_scenes import sphere_tumble_scene
+
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
 
 
@@ -399,6 +401,83 @@ def test_contact_gradient_vs_central_fd_plane(oracle_mod):
         fd = (loss(dv=dv) - loss(dv=-dv)) / 2
         assert abs(fd - g["v0"] @ dv) < 1e-5 * abs(fd)
         s.set_material(np.full(s.T, 8e4), 0.4)
+
+
+@pytest.mark.parametrize("mu,sphere", [(0.5, (0.12, 0.05, -0.19, 0.2)), (0.2, (0.12, 0.05, -0.19, 0.2)),
+                                       (0.5, (0.02, 0.03, -0.28, 0.3))])
+def test_contact_gradient_vs_central_fd_sphere(oracle_mod, mu, sphere):
+    """Sphere obstacles: the contact normal moves with the positions it was
+    detected on (through the crossing time, or the end-of-step position), and
+    the reverse sweep carries that derivative back to the start-of-step
+    positions and to the contact iterate the detection read. Both normal
+    branches occur in these drops; freezing the normal, dropping either branch,
+    or sending the root branch's adjoint to the wrong position misses the
+    central differences by 1e-4 to 9e-2."""
+    x, t = scene.box_grid(5, 2, 2, origin=(0, 0, 0.015))
+    d = scene.SceneDesc(x_rest=x, tets=t, total_mass=2.0, E=np.full(t.shape[0], 8e4), h=0.01, max_iter=10,
+                        gravity=(0, 0, -9.81), contact=True, max_iter1=10, mu=mu,
+                        planes=np.zeros((0, 6)), spheres=np.array([sphere]))
+    s = oracle_mod.OracleScene(d)
+    x0 = x.ravel().copy()
+    v0 = np.zeros_like(x0)
+    v0[0::3] = 0.3
+    steps = 14
+    rng = np.random.default_rng(12)
+    gx = rng.standard_normal(x0.shape)
+    gv = 0.01 * rng.standard_normal(x0.shape)
+    _, _, its = s.rollout(x0, v0, steps, record=True)
+    assert its[:, 1].sum() > 0 and its[:, 2].sum() > 0
+    g, _ = s.backward(gx, gv)
+
+    def loss(E=8e4, dx=0.0, dv=0.0):
+        s.set_material(np.full(s.T, E), 0.4)
+        xT, vT, its1 = s.rollout(x0 + dx, v0 + dv, steps)
+        assert np.array_equal(its1, its)
+        return gx @ xT + gv @ vT
+
+    hE = 8e4 * 1e-6
+    fd = (loss(E=8e4 + hE) - loss(E=8e4 - hE)) / (2 * hE)
+    assert abs(fd - g["E"].sum()) < 1e-5 * abs(fd)
+    dx = 1e-7 * rng.standard_normal(x0.shape)
+    fd = (loss(dx=dx) - loss(dx=-dx)) / 2
+    assert abs(fd - g["x0"] @ dx) < 1e-5 * abs(fd)
+    dv = 1e-6 * rng.standard_normal(x0.shape)
+    fd = (loss(dv=dv) - loss(dv=-dv)) / 2
+    assert abs(fd - g["v0"] @ dv) < 1e-5 * abs(fd)
+
+
+def test_contact_gradient_vs_central_fd_sphere_later_pass(oracle_mod):
+    """Sphere normals detected in later contact passes: their adjoint goes to
+    the position that detection read (here the input of the pass's last
+    iteration). Reading the output instead, or sending every normal's adjoint
+    to the PD result, misses the central differences by 4e-4 to 5e-1."""
+    d, v0 = sphere_tumble_scene()
+    s = oracle_mod.OracleScene(d)
+    x0 = d.x_rest.ravel().copy()
+    steps = 12
+    rng = np.random.default_rng(12)
+    gx = rng.standard_normal(x0.shape)
+    gv = 0.01 * rng.standard_normal(x0.shape)
+    _, _, its = s.rollout(x0, v0, steps, record=True)
+    assert its[:, 2].max() >= 2  # a step with more than one detection pass
+    g, _ = s.backward(gx, gv)
+
+    def loss(E=8e4, dx=0.0, dv=0.0):
+        s.set_material(np.full(s.T, E), 0.4)
+        xT, vT, its1 = s.rollout(x0 + dx, v0 + dv, steps)
+        assert np.array_equal(its1, its)
+        return gx @ xT + gv @ vT
+
+    hE = 8e4 * 1e-6
+    fd = (loss(E=8e4 + hE) - loss(E=8e4 - hE)) / (2 * hE)
+    assert abs(fd - g["E"].sum()) < 1e-5 * abs(fd)
+    for _ in range(2):
+        dx = 1e-7 * rng.standard_normal(x0.shape)
+        fd = (loss(dx=dx) - loss(dx=-dx)) / 2
+        assert abs(fd - g["x0"] @ dx) < 1e-5 * abs(fd)
+        dv = 1e-6 * rng.standard_normal(x0.shape)
+        fd = (loss(dv=dv) - loss(dv=-dv)) / 2
+        assert abs(fd - g["v0"] @ dv) < 1e-5 * abs(fd)
 
 
 def test_contact_per_tet_E_gradient_vs_fd(oracle_mod):
