@@ -61,6 +61,7 @@ SYMBOLS = [
     "pdm_backward_host", "pdm_get_iterations", "pdm_solve_host", "pdm_force_host", "pdm_get_timing",
     "pdm_host_analyse", "pdm_host_solve", "pdm_get_contact_stats", "pdm_time_solve", "pdm_get_solve_timing",
     "pdm_get_solve_shapes", "pdm_set_solve_plan", "pdm_solve_noalloc", "pdm_get_graph_launches",
+    "pdm_refactor", "pdm_get_factor_w",
 ]
 
 # PDM_NOALLOC_SCRATCH_BYTES of include/pdm.h
@@ -90,6 +91,8 @@ def load():
     L.pdm_scene_info.argtypes = [vp, C.POINTER(InfoC)]
     L.pdm_set_stream.argtypes = [vp, vp]
     L.pdm_set_material.argtypes = [vp, vp, C.c_double]
+    L.pdm_refactor.argtypes = [vp, C.c_int]
+    L.pdm_get_factor_w.argtypes = [vp, vp, C.c_longlong]
     L.pdm_set_batch.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
     for name in ("pdm_forward", "pdm_forward_host"):
         getattr(L, name).argtypes = [vp, C.c_int, vp, vp, vp, vp, vp, C.c_int]

@@ -24,6 +24,7 @@
 #include "../../include/pdm.h"
 #include "pdm_internal.h"
 #include "pdm_kernels.h"
+#include "pdm_factor_kernels.h"
 
 using namespace pdm;
 
@@ -2876,6 +2877,18 @@ struct pdm_scene_s {
   DevBuf<int> f_first, f_rows, f_child_ptr, f_child, f_rel, f_lev;
   DevBuf<long long> f_rowptr, f_lptr, f_relptr, f_uptr;
   DevBuf<double> f_W, f_WR;
+  // device re-factorisation plan, built once from the symbolic analysis: the
+  // contributions of every stored L entry, the front position of every CSR
+  // entry and diagonal, update-block offsets, and per level the extend and
+  // update work items (fc_ext_ptr / fc_upd_ptr)
+  bool fac_plan = false;
+  DevBuf<long long> fc_cptr, fc_amap, fc_dpos, fc_upos;
+  DevBuf<int> fc_ctet, fc_fail;
+  DevBuf<double> fc_cval, fc_L, fc_U;
+  DevBuf<int2> fc_ext;
+  DevBuf<int4> fc_upd;
+  std::vector<int> fc_ext_ptr, fc_upd_ptr;
+  long long fc_usize = 0;
   std::vector<int> lev_ptr, lev_rows;  // lev_rows: max (mpad + wpad) over the level's supernodes
   // grouped level kernels: per sweep (0 fwd, 1 bwd) the work items of every
   // level (supernode, tile range) at item_ptr[pass][l], and per level the
@@ -3077,7 +3090,9 @@ int upload_material(pdm_scene s) {
   return 0;
 }
 
-int upload_factor(pdm_scene s) {
+// Structure of the factor (and with host_w the W blocks from the host
+// factor's L and inv(L_ss); otherwise W is only allocated).
+int upload_factor(pdm_scene s, bool host_w) {
   Factor& F = s->F;
   if (s->f_first.upload(F.first.data(), F.first.size())) return 1;
   if (s->f_rows.upload(F.rows.data(), F.rows.size())) return 1;
@@ -3093,7 +3108,9 @@ int upload_factor(pdm_scene s) {
   if (s->f_relptr.upload(tmp.data(), tmp.size())) return 1;
   tmp.assign(F.uptr.begin(), F.uptr.end());
   if (s->f_uptr.upload(tmp.data(), tmp.size())) return 1;
-  {
+  if (!host_w) {
+    if (s->f_W.alloc(F.nnz) || s->f_WR.alloc(F.nnz)) return 1;
+  } else {
     // W = [inv(L_ss); -L_below inv(L_ss)] per supernode, column-major and
     // row-major (coalesced operand loads in both sweeps)
     int ns = (int)F.first.size() - 1;
@@ -3225,14 +3242,241 @@ int upload_factor(pdm_scene s) {
 
 int host_factor(pdm_scene s);
 
-// Re-factorisation after a material change, then upload.
-int refactor(pdm_scene s) {
+// Re-factorisation on the host CPU (multifrontal factor and W precompute in
+// host code, then uploaded).
+int refactor_host(pdm_scene s) {
   auto t0 = std::chrono::steady_clock::now();
   if (host_factor(s)) return 1;
-  if (upload_material(s) || upload_factor(s)) return 1;
+  if (upload_material(s) || upload_factor(s, true)) return 1;
+  s->fac_plan = false;  // the structure upload above replaced the device plan's inputs
   s->t_factor = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return 0;
 }
+
+// Device re-factorisation plan (once per scene; the structure never changes).
+int build_fac_plan(pdm_scene s) {
+#pragma clang fp contract(off)
+  HostScene& S = s->S;
+  const Csr& A = s->Ls;
+  Factor& F = s->F;
+  const int N = S.N, ns = F.ns;
+  const long long nnzA = (long long)A.col.size();
+  auto find_q = [&](int i, int j) -> long long {
+    const int* b = A.col.data() + A.ptr[i];
+    const int* e = A.col.data() + A.ptr[i + 1];
+    const int* f = std::lower_bound(b, e, j);
+    return (f != e && *f == j) ? (long long)(f - A.col.data()) : -1;
+  };
+  // contributions of each stored entry, in the host assembly's order (tets
+  // ascending, muscles, attachments, the structural 0 of the diagonal)
+  size_t ncon = 16 * (size_t)S.T + 16 * (size_t)S.M + S.att_idx.size() + (size_t)N;
+  std::vector<long long> cq;
+  std::vector<int> ct;
+  std::vector<double> cv;
+  cq.reserve(ncon);
+  ct.reserve(ncon);
+  cv.reserve(ncon);
+  for (int t = 0; t < S.T; ++t) {
+    const double* G = &S.tG[12 * (size_t)t];
+    for (int a = 0; a < 4; ++a)
+      for (int b = 0; b < 4; ++b) {
+        double gg = G[3 * a] * G[3 * b] + G[3 * a + 1] * G[3 * b + 1] + G[3 * a + 2] * G[3 * b + 2];
+        cq.push_back(find_q(S.tv[4 * t + a], S.tv[4 * t + b]));
+        ct.push_back(t);
+        cv.push_back(gg);
+      }
+  }
+  for (int m = 0; m < S.M; ++m)
+    for (int a = 0; a < 4; ++a)
+      for (int b = 0; b < 4; ++b) {
+        cq.push_back(find_q(S.mv[4 * m + a], S.mv[4 * m + b]));
+        ct.push_back(-1);
+        cv.push_back(S.mw[m] * S.mg[4 * m + a] * S.mg[4 * m + b]);
+      }
+  for (size_t k = 0; k < S.att_idx.size(); ++k) {
+    cq.push_back(find_q(S.att_idx[k], S.att_idx[k]));
+    ct.push_back(-1);
+    cv.push_back(S.att_w[k]);
+  }
+  for (int i = 0; i < N; ++i) {
+    cq.push_back(find_q(i, i));
+    ct.push_back(-1);
+    cv.push_back(0.0);
+  }
+  std::vector<long long> cptr(nnzA + 1, 0);
+  for (long long q : cq) {
+    if (q < 0) {
+      g_err = "device factor plan: a contribution has no stored entry";
+      return 1;
+    }
+    cptr[q + 1]++;
+  }
+  for (long long q = 0; q < nnzA; ++q) cptr[q + 1] += cptr[q];
+  std::vector<int> ct2(cq.size());
+  std::vector<double> cv2(cq.size());
+  {
+    std::vector<long long> fill(cptr.begin(), cptr.end() - 1);
+    for (size_t k = 0; k < cq.size(); ++k) {  // stable: keeps the order above within an entry
+      long long d = fill[cq[k]]++;
+      ct2[d] = ct[k];
+      cv2[d] = cv[k];
+    }
+  }
+  std::vector<long long>().swap(cq);
+  std::vector<int>().swap(ct);
+  std::vector<double>().swap(cv);
+  // the lists re-sum, in k_fac_assemble_L's arithmetic, to the host assembly's
+  // values bit for bit (same terms, same order, same roundings)
+  for (long long q = 0; q < nnzA; ++q) {
+    double v = 0.0;
+    for (long long c = cptr[q]; c < cptr[q + 1]; ++c) {
+      const int t = ct2[c];
+      if (t >= 0)
+        v += (S.wr[t] + S.wv[t]) * cv2[c];
+      else
+        v += cv2[c];
+    }
+    if (std::memcmp(&v, &A.val[q], sizeof(double)) != 0) {
+      g_err = "device factor plan: contribution lists do not reproduce the assembled matrix";
+      return 1;
+    }
+  }
+  // front positions of A's lower entries and of the diagonals; update blocks
+  std::vector<long long> amap(nnzA, -1), dpos(N), upos(ns + 1, 0);
+  for (int sn = 0; sn < ns; ++sn) {
+    const int c0 = F.first[sn], w = F.first[sn + 1] - c0;
+    const int m = (int)(F.rowptr[sn + 1] - F.rowptr[sn]);
+    const int* rows = &F.rows[F.rowptr[sn]];
+    for (int j = 0; j < w; ++j) {
+      const int col = c0 + j;
+      dpos[col] = F.lptr[sn] + (long long)j * m + j;
+      for (int q = A.ptr[col]; q < A.ptr[col + 1]; ++q) {
+        const int r = A.col[q];
+        if (r < col) continue;
+        const int k = (int)(std::lower_bound(rows, rows + m, r) - rows);
+        if (k >= m || rows[k] != r) {
+          g_err = "device factor plan: an entry of A is outside its supernode's rows";
+          return 1;
+        }
+        amap[q] = F.lptr[sn] + (long long)j * m + k;
+      }
+    }
+    upos[sn + 1] = upos[sn] + (long long)(m - w) * (m - w);
+  }
+  s->fc_usize = upos[ns];
+  // per level: extend items (supernode, parent row tile) of supernodes with
+  // children; update items (supernode, kind, row tile, column tile)
+  const int nlev = (int)F.level_ptr.size() - 1;
+  std::vector<int2> ext;
+  std::vector<int4> upd;
+  s->fc_ext_ptr.assign(nlev + 1, 0);
+  s->fc_upd_ptr.assign(nlev + 1, 0);
+  const int T = pdmfac::kTile;
+  for (int l = 0; l < nlev; ++l) {
+    for (int q = F.level_ptr[l]; q < F.level_ptr[l + 1]; ++q) {
+      const int sn = F.level_sn[q];
+      const int w = F.first[sn + 1] - F.first[sn], m = (int)(F.rowptr[sn + 1] - F.rowptr[sn]), mb = m - w;
+      if (F.child_ptr[sn + 1] > F.child_ptr[sn])
+        for (int t = 0; t < cdiv(m, T); ++t) ext.push_back(make_int2(sn, t));
+      for (int ti = 0; ti < cdiv(mb, T); ++ti) {
+        for (int tj = 0; tj < cdiv(w, T); ++tj) upd.push_back(make_int4(sn, 0, ti, tj));
+        for (int tj = 0; tj <= ti; ++tj) upd.push_back(make_int4(sn, 1, ti, tj));
+      }
+    }
+    s->fc_ext_ptr[l + 1] = (int)ext.size();
+    s->fc_upd_ptr[l + 1] = (int)upd.size();
+  }
+  if (s->fc_cptr.upload(cptr.data(), cptr.size()) || s->fc_ctet.upload(ct2.data(), std::max<size_t>(1, ct2.size())) ||
+      s->fc_cval.upload(cv2.data(), std::max<size_t>(1, cv2.size())) || s->fc_amap.upload(amap.data(), std::max<size_t>(1, amap.size())) ||
+      s->fc_dpos.upload(dpos.data(), dpos.size()) || s->fc_upos.upload(upos.data(), upos.size()) ||
+      s->fc_ext.upload(ext.data(), std::max<size_t>(1, ext.size())) ||
+      s->fc_upd.upload(upd.data(), std::max<size_t>(1, upd.size())) || s->fc_fail.alloc(1) ||
+      s->fc_L.alloc(std::max<long long>(1, F.nnz)) || s->fc_U.alloc(std::max<long long>(1, s->fc_usize)))
+    return 1;
+  if (s->Lptr.upload(A.ptr.data(), A.ptr.size()) || s->Lcol.upload(A.col.data(), A.col.size()) ||
+      s->Lval.alloc(std::max<long long>(1, nnzA)) || s->Kdiag.alloc(N))
+    return 1;
+  return 0;
+}
+
+// Per-tet weights and the E-gradient factors (host, O(T)), uploaded.
+int upload_weights(pdm_scene s) {
+  HostScene& S = s->S;
+  update_weights(S);
+  if (s->wr.upload(S.wr.data(), S.T) || s->wv.upload(S.wv.data(), S.T)) return 1;
+  std::vector<double> der(S.T), dev(S.T);
+  for (int t = 0; t < S.T; ++t) {
+    der[t] = S.tvol[t] / (1.0 + S.nu_fwd_t[t]);
+    dev[t] = S.tvol[t] * S.nu_ctor[t] / (1.0 + S.nu_fwd_t[t]);
+  }
+  if (s->dEr.upload(der.data(), S.T) || s->dEv.upload(dev.data(), S.T)) return 1;
+  return 0;
+}
+
+// Re-factorisation on the device: L's values and diag(K_s) from the weights,
+// then the multifrontal factor level by level and W, with the regularisation
+// ladder of World::FactorizeLDLT (World.h:1212-1232): reg = 1e-5, 1e-4, ...
+// while a pivot is not positive. One host synchronisation per rung.
+int refactor_device(pdm_scene s) {
+  auto t0 = std::chrono::steady_clock::now();
+  HostScene& S = s->S;
+  Factor& F = s->F;
+  if (!s->fac_plan) {
+    if (upload_factor(s, false) || build_fac_plan(s)) return 1;
+    s->fac_plan = true;
+  }
+  if (upload_weights(s)) return 1;
+  const int N = S.N;
+  const long long nnzA = (long long)s->Ls.col.size();
+  const hipStream_t st = s->stream;
+  hipLaunchKernelGGL(pdmfac::k_fac_assemble_L, dim3(cdiv(nnzA, 256)), dim3(256), 0, st, s->fc_cptr.p, s->fc_ctet.p,
+                     s->fc_cval.p, s->wr.p, s->wv.p, nnzA, s->Lval.p);
+  hipLaunchKernelGGL(pdmfac::k_fac_kdiag, dim3(cdiv(N, 256)), dim3(256), 0, st, s->Lptr.p, s->Lcol.p, s->Lval.p,
+                     s->mass.p, S.h, N, s->Kdiag.p);
+  const int nlev = (int)F.level_ptr.size() - 1;
+  double reg = 0.0, reg_step = 1e-6;
+  for (int tries = 0;; ++tries) {
+    HIPX(hipMemsetAsync(s->fc_L.p, 0, (size_t)F.nnz * sizeof(double), st));
+    if (s->fc_usize > 0) HIPX(hipMemsetAsync(s->fc_U.p, 0, (size_t)s->fc_usize * sizeof(double), st));
+    HIPX(hipMemsetAsync(s->fc_fail.p, 0, sizeof(int), st));
+    hipLaunchKernelGGL(pdmfac::k_fac_scatter, dim3(cdiv(nnzA, 256)), dim3(256), 0, st, s->Lval.p, s->fc_amap.p,
+                       nnzA, s->fc_L.p);
+    hipLaunchKernelGGL(pdmfac::k_fac_diag, dim3(cdiv(N, 256)), dim3(256), 0, st, s->fc_dpos.p, s->mass.p, S.h, reg,
+                       N, s->fc_L.p);
+    for (int l = 0; l < nlev; ++l) {
+      const int ne = s->fc_ext_ptr[l + 1] - s->fc_ext_ptr[l];
+      if (ne > 0)
+        hipLaunchKernelGGL(pdmfac::k_fac_extend, dim3(ne), dim3(256), 0, st, s->fc_ext.p + s->fc_ext_ptr[l],
+                           s->f_first.p, s->f_rowptr.p, s->f_lptr.p, s->fc_upos.p, s->f_child_ptr.p, s->f_child.p,
+                           s->f_rel.p, s->f_relptr.p, s->fc_L.p, s->fc_U.p);
+      const int nsn = F.level_ptr[l + 1] - F.level_ptr[l];
+      hipLaunchKernelGGL(pdmfac::k_fac_panel, dim3(nsn), dim3(256), 0, st, s->f_lev.p + F.level_ptr[l], s->f_first.p,
+                         s->f_rowptr.p, s->f_lptr.p, s->fc_L.p, s->f_W.p, s->f_WR.p, s->fc_fail.p);
+      const int nu = s->fc_upd_ptr[l + 1] - s->fc_upd_ptr[l];
+      if (nu > 0)
+        hipLaunchKernelGGL(pdmfac::k_fac_update, dim3(nu), dim3(256), 0, st, s->fc_upd.p + s->fc_upd_ptr[l],
+                           s->f_first.p, s->f_rowptr.p, s->f_lptr.p, s->fc_upos.p, s->fc_L.p, s->fc_U.p, s->f_W.p,
+                           s->f_WR.p);
+    }
+    HIPX(hipGetLastError());
+    int fail = 0;
+    HIPX(hipMemcpyAsync(&fail, s->fc_fail.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPX(hipStreamSynchronize(st));
+    if (!fail) break;
+    if (tries == 40) {
+      g_err = "factorisation failed: non-positive pivot";
+      return 1;
+    }
+    reg_step *= 10;
+    reg = reg_step;
+  }
+  F.reg_used = reg;
+  s->t_factor = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return 0;
+}
+
+// Re-factorisation after a material change (on the device).
+int refactor(pdm_scene s) { return refactor_device(s); }
 
 
 // Scene construction shared by pdm_scene_create and the host-only diagnostics.
@@ -4621,6 +4865,26 @@ int pdm_set_material(pdm_scene s, const double* E, double nu_fwd) {
   }
   s->have_tape = false;
   return refactor(s);
+}
+
+int pdm_refactor(pdm_scene s, int on_host) {
+  if (!s) {
+    g_err = "null scene";
+    return 1;
+  }
+  HIPX(hipStreamSynchronize(s->stream));
+  s->have_tape = false;
+  return on_host ? refactor_host(s) : refactor_device(s);
+}
+
+int pdm_get_factor_w(pdm_scene s, double* W, long long n) {
+  if (!s || !W || n != (long long)s->F.nnz) {
+    g_err = "pdm_get_factor_w: n must equal nnz_factor";
+    return 1;
+  }
+  HIPX(hipMemcpyAsync(W, s->f_W.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  HIPX(hipStreamSynchronize(s->stream));
+  return 0;
 }
 
 int pdm_set_batch(pdm_scene s, int R, int precision, int flags, int checkpoint_every) {

@@ -146,6 +146,20 @@ class Scene:
         nu = float("nan") if nu_fwd is None else float(nu_fwd)
         check(L.pdm_set_material(self.h, _vp(Ea), nu))
 
+    def refactor(self, on_host: bool = False):
+        """Re-factorise for the current material: on the device (what
+        set_material does) or, with on_host, in host code (a cross-check)."""
+        self.tape_id += 1
+        check(_lib.load().pdm_refactor(self.h, 1 if on_host else 0))
+
+    def factor_w(self) -> np.ndarray:
+        """W = [inv(L_ss); -L_below inv(L_ss)] of every supernode, column-major
+        blocks in factor order (nnz_factor values)."""
+        n = self.info()["nnz_factor"]
+        W = np.empty(n)
+        check(_lib.load().pdm_get_factor_w(self.h, _vp(W), n))
+        return W
+
     def set_stream(self, stream_ptr):
         check(_lib.load().pdm_set_stream(self.h, C.c_void_p(stream_ptr) if stream_ptr else None))
 

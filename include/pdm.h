@@ -145,10 +145,24 @@ int pdm_scene_info(pdm_scene s, pdm_info* info);
 /* Bind a HIP stream (hipStream_t as void*; NULL = the library's own stream). */
 int pdm_set_stream(pdm_scene s, void* stream);
 
-/* Material update + numeric re-factorisation. E: T values (caller tet order)
- * or NULL to keep; nu_fwd as forward_all's tau[1]. */
-/* nu_fwd = NaN keeps the current per-tet nu (from the scene description). */
+/* Material update + numeric re-factorisation on the device. E: T values
+ * (caller tet order) or NULL to keep; nu_fwd as forward_all's tau[1];
+ * nu_fwd = NaN keeps the current per-tet nu (from the scene description). */
 int pdm_set_material(pdm_scene s, const double* E, double nu_fwd);
+
+/* Numeric re-factorisation of K_s = M/h^2 + L for the current material, with
+ * the regularisation ladder of World::FactorizeLDLT (World.h:1212-1232;
+ * replaces World::PreComputation's refactor, World.h:1089-1139, that
+ * forward_all runs on every call, pydifem.cc:207-221). on_host = 0: on the
+ * device, as pdm_scene_create and pdm_set_material do; 1: the same
+ * factorisation in host code, uploaded (a cross-check of the device path).
+ * The time lands in pdm_get_timing's factor_ms. */
+int pdm_refactor(pdm_scene s, int on_host);
+
+/* Copy the solve operand W = [inv(L_ss); -L_below inv(L_ss)] of every
+ * supernode (column-major m x w blocks in factor order; n = nnz_factor).
+ * No counterpart in the reference (a diagnostic of the factorisation). */
+int pdm_get_factor_w(pdm_scene s, double* W, long long n);
 
 /* Rollout batch R, arithmetic precision (PDM_FP64 everywhere, or PDM_MIXED:
  * FP32 local step, FP64 factor/solves/accumulation), flags, and the
