@@ -2887,7 +2887,7 @@ struct pdm_scene_s {
   DevBuf<double> fc_cval, fc_L, fc_U;
   DevBuf<int2> fc_ext;
   DevBuf<int4> fc_upd;
-  std::vector<int> fc_ext_ptr, fc_upd_ptr;
+  std::vector<int> fc_ext_ptr, fc_upd_ptr, fc_lev_maxm;
   long long fc_usize = 0;
   std::vector<int> lev_ptr, lev_rows;  // lev_rows: max (mpad + wpad) over the level's supernodes
   // grouped level kernels: per sweep (0 fwd, 1 bwd) the work items of every
@@ -3371,11 +3371,13 @@ int build_fac_plan(pdm_scene s) {
   std::vector<int4> upd;
   s->fc_ext_ptr.assign(nlev + 1, 0);
   s->fc_upd_ptr.assign(nlev + 1, 0);
+  s->fc_lev_maxm.assign(nlev, 1);
   const int T = pdmfac::kTile;
   for (int l = 0; l < nlev; ++l) {
     for (int q = F.level_ptr[l]; q < F.level_ptr[l + 1]; ++q) {
       const int sn = F.level_sn[q];
       const int w = F.first[sn + 1] - F.first[sn], m = (int)(F.rowptr[sn + 1] - F.rowptr[sn]), mb = m - w;
+      s->fc_lev_maxm[l] = std::max(s->fc_lev_maxm[l], m);
       if (F.child_ptr[sn + 1] > F.child_ptr[sn])
         for (int t = 0; t < cdiv(m, T); ++t) ext.push_back(make_int2(sn, t));
       for (int ti = 0; ti < cdiv(mb, T); ++ti) {
@@ -3434,6 +3436,25 @@ int refactor_device(pdm_scene s) {
   hipLaunchKernelGGL(pdmfac::k_fac_kdiag, dim3(cdiv(N, 256)), dim3(256), 0, st, s->Lptr.p, s->Lcol.p, s->Lval.p,
                      s->mass.p, S.h, N, s->Kdiag.p);
   const int nlev = (int)F.level_ptr.size() - 1;
+  // panel block width per level: 16 columns, fewer where 16 x front rows of
+  // doubles would not fit the 160 KiB of LDS
+  constexpr size_t kPanelLds = 160 * 1024 - 256;
+  static bool panel_attr = false;
+  if (!panel_attr) {
+    HIPX(hipFuncSetAttribute((const void*)pdmfac::k_fac_panel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)kPanelLds));
+    panel_attr = true;
+  }
+  std::vector<int> nbl(nlev);
+  for (int l = 0; l < nlev; ++l) {
+    int nb = 16;
+    while (nb > 1 && (size_t)nb * s->fc_lev_maxm[l] * sizeof(double) > kPanelLds) nb /= 2;
+    if ((size_t)nb * s->fc_lev_maxm[l] * sizeof(double) > kPanelLds) {
+      g_err = "device factorisation: a front has more rows than one block column fits in LDS";
+      return 1;
+    }
+    nbl[l] = nb;
+  }
   double reg = 0.0, reg_step = 1e-6;
   for (int tries = 0;; ++tries) {
     HIPX(hipMemsetAsync(s->fc_L.p, 0, (size_t)F.nnz * sizeof(double), st));
@@ -3450,8 +3471,9 @@ int refactor_device(pdm_scene s) {
                            s->f_first.p, s->f_rowptr.p, s->f_lptr.p, s->fc_upos.p, s->f_child_ptr.p, s->f_child.p,
                            s->f_rel.p, s->f_relptr.p, s->fc_L.p, s->fc_U.p);
       const int nsn = F.level_ptr[l + 1] - F.level_ptr[l];
-      hipLaunchKernelGGL(pdmfac::k_fac_panel, dim3(nsn), dim3(256), 0, st, s->f_lev.p + F.level_ptr[l], s->f_first.p,
-                         s->f_rowptr.p, s->f_lptr.p, s->fc_L.p, s->f_W.p, s->f_WR.p, s->fc_fail.p);
+      hipLaunchKernelGGL(pdmfac::k_fac_panel, dim3(nsn), dim3(256), (size_t)nbl[l] * s->fc_lev_maxm[l] * sizeof(double),
+                         st, s->f_lev.p + F.level_ptr[l], s->f_first.p, s->f_rowptr.p, s->f_lptr.p, s->fc_L.p, s->f_W.p,
+                         s->f_WR.p, s->fc_fail.p, nbl[l]);
       const int nu = s->fc_upd_ptr[l + 1] - s->fc_upd_ptr[l];
       if (nu > 0)
         hipLaunchKernelGGL(pdmfac::k_fac_update, dim3(nu), dim3(256), 0, st, s->fc_upd.p + s->fc_upd_ptr[l],

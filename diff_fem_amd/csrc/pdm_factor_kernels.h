@@ -135,55 +135,118 @@ __global__ __launch_bounds__(256) void k_fac_extend(const int2* __restrict__ ite
 }
 
 // Partial Cholesky of supernode s's w columns (m x w, column-major, in place),
-// right-looking: column j takes its pivot sqrt(d) and is scaled by 1/sqrt(d),
-// then updates the later columns' rows at and below their diagonal, so every
-// entry receives its updates in column order. A pivot that is not positive sets
-// *fail (the host then retries with the next regularisation). Then inv(L_ss)
-// by forward substitution, one column per thread, into WR's top w rows
-// (row-major, zero above the diagonal) and Wc's.
+// right-looking, so every entry receives its updates in column order: column j
+// takes its pivot sqrt(d) and is scaled by 1/sqrt(d), then updates the later
+// columns' rows at and below their diagonal. Blocked by nb <= 16 columns: a
+// block column (rows j0..m-1) is factored in LDS (dynamic, nb * m doubles,
+// rows contiguous), written back, and updates the trailing columns from LDS. A
+// pivot that is not positive sets *fail (the host then retries with the next
+// regularisation). Then inv(L_ss) by forward substitution, thread k owning
+// column k, by blocks of nb rows whose L rows are staged in LDS, into WR's top
+// w rows (row-major, zero above the diagonal) and Wc's.
 __global__ __launch_bounds__(256) void k_fac_panel(const int* __restrict__ sns, const int* __restrict__ first,
                                                    const long long* __restrict__ rowptr,
                                                    const long long* __restrict__ lptr, double* __restrict__ Lf,
                                                    double* __restrict__ Wc, double* __restrict__ WR,
-                                                   int* __restrict__ fail) {
+                                                   int* __restrict__ fail, int nb) {
+  extern __shared__ double P[];
+  __shared__ double sh_inv;
   const int s = sns[blockIdx.x];
   const int w = first[s + 1] - first[s], m = (int)(rowptr[s + 1] - rowptr[s]);
   double* L = Lf + lptr[s];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  __shared__ double sh_inv;
-  for (int j = 0; j < w; ++j) {
-    double* cj = L + (long long)j * m;
-    if (tid == 0) {
-      double d = cj[j];
-      if (!(d > 0.0)) {
-        *fail = 1;
-        d = 1.0;
-      }
-      d = sqrt(d);
-      cj[j] = d;
-      sh_inv = 1.0 / d;
+  for (int j0 = 0; j0 < w; j0 += nb) {
+    const int bw = min(nb, w - j0), mr = m - j0;
+    for (int e = tid; e < bw * mr; e += 256) {
+      const int kk = e / mr, rr = e - kk * mr;
+      P[kk * mr + rr] = L[(long long)(j0 + kk) * m + j0 + rr];
     }
     __syncthreads();
-    const double inv = sh_inv;
-    for (int i = j + 1 + tid; i < m; i += 256) cj[i] *= inv;
-    __syncthreads();
-    for (int jp = j + 1 + wave; jp < w; jp += 4) {
-      const double f = cj[jp];
+    for (int jj = 0; jj < bw; ++jj) {
+      double* cj = P + jj * mr;
+      if (tid == 0) {
+        double d = cj[jj];
+        if (!(d > 0.0)) {
+          *fail = 1;
+          d = 1.0;
+        }
+        d = sqrt(d);
+        cj[jj] = d;
+        sh_inv = 1.0 / d;
+      }
+      __syncthreads();
+      const double inv = sh_inv;
+      for (int i = jj + 1 + tid; i < mr; i += 256) cj[i] *= inv;
+      __syncthreads();
+      for (int jp = jj + 1 + wave; jp < bw; jp += 4) {
+        const double f = cj[jp];
+        double* cp = P + jp * mr;
+        for (int i = jp + lane; i < mr; i += 64) cp[i] -= f * cj[i];
+      }
+      __syncthreads();
+    }
+    for (int e = tid; e < bw * mr; e += 256) {
+      const int kk = e / mr, rr = e - kk * mr;
+      L[(long long)(j0 + kk) * m + j0 + rr] = P[kk * mr + rr];
+    }
+    for (int jp = j0 + bw + wave; jp < w; jp += 4) {
+      double f[16];
+#pragma unroll
+      for (int kk = 0; kk < 16; ++kk) f[kk] = kk < bw ? P[kk * mr + (jp - j0)] : 0.0;
       double* cp = L + (long long)jp * m;
-      for (int i = jp + lane; i < m; i += 64) cp[i] -= f * cj[i];
+      for (int i = jp + lane; i < m; i += 64) {
+        double acc = cp[i];
+#pragma unroll
+        for (int kk = 0; kk < 16; ++kk)
+          if (kk < bw) acc -= f[kk] * P[kk * mr + (i - j0)];
+        cp[i] = acc;
+      }
     }
     __syncthreads();
   }
   double* D = WR + lptr[s];
-  double* Wt = Wc + lptr[s];
-  for (int k = tid; k < w; k += 256) {
-    for (int r = 0; r < k; ++r) D[(long long)r * w + k] = 0.0;
-    for (int r = k; r < w; ++r) {
-      double acc = r == k ? 1.0 : 0.0;
-      for (int q = k; q < r; ++q) acc -= L[(long long)q * m + r] * D[(long long)q * w + k];
-      D[(long long)r * w + k] = acc / L[(long long)r * m + r];
+  for (int r0 = 0; r0 < w; r0 += nb) {
+    const int bh = min(nb, w - r0), ncol = r0 + bh;
+    for (int e = tid; e < bh * ncol; e += 256) {  // Q[rr][q] = L[r0 + rr][q]
+      const int q = e / bh, rr = e - q * bh;
+      P[rr * ncol + q] = L[(long long)q * m + r0 + rr];
     }
-    for (int r = 0; r < w; ++r) Wt[(long long)k * m + r] = D[(long long)r * w + k];
+    __syncthreads();
+    for (int k = tid; k < w; k += 256) {
+      double acc[16], dblk[16];
+#pragma unroll
+      for (int rr = 0; rr < 16; ++rr) {
+        acc[rr] = (r0 + rr == k) ? 1.0 : 0.0;
+        dblk[rr] = 0.0;
+      }
+#pragma unroll 4
+      for (int q = k; q < r0; ++q) {
+        const double dq = D[(long long)q * w + k];
+#pragma unroll
+        for (int rr = 0; rr < 16; ++rr)
+          if (rr < bh) acc[rr] -= P[rr * ncol + q] * dq;
+      }
+#pragma unroll
+      for (int rr = 0; rr < 16; ++rr) {
+        if (rr >= bh) continue;
+        const int r = r0 + rr;
+        if (r < k) {
+          D[(long long)r * w + k] = 0.0;
+          continue;
+        }
+#pragma unroll
+        for (int qq = 0; qq < 16; ++qq)
+          if (qq < rr && r0 + qq >= k) acc[rr] -= P[rr * ncol + r0 + qq] * dblk[qq];
+        dblk[rr] = acc[rr] / P[rr * ncol + r];
+        D[(long long)r * w + k] = dblk[rr];
+      }
+    }
+    __syncthreads();
+  }
+  double* Wt = Wc + lptr[s];
+  for (int e = tid; e < w * w; e += 256) {
+    const int k = e / w, r = e - k * w;
+    Wt[(long long)k * m + r] = D[(long long)r * w + k];
   }
 }
 

@@ -72,6 +72,7 @@ def lib():
         L.orc_volume_projection.argtypes = [_dp, _dp]
         L.orc_volume_projection_jac.argtypes = [_dp, _dp, _dp, _dp]
         L.orc_set_underwater.argtypes = [C.c_void_p, C.c_int]
+        L.orc_self_normals.argtypes = [C.c_void_p, C.c_int]
         L.orc_set_tris.argtypes = [C.c_void_p, C.c_int, _dp]
         L.orc_set_self_collision.argtypes = [C.c_void_p, C.c_int, C.c_double]
         L.orc_cubic_roots01.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, _dp]
@@ -230,6 +231,13 @@ class OracleScene:
         a = None if act is None else np.ascontiguousarray(act, dtype=np.float64)
         lib().orc_force_terms(self.h, np.ascontiguousarray(x, np.float64).ravel(), _ptr(a), Jd, Lx)
         return Jd, Lx
+
+    def self_normals(self, mode):
+        """Self-contact impact normals: 0 computed, 1 recorded by the next
+        rollout, 2 replayed from that recording (central differences of the
+        derivative with the normals held, as the reference's tape holds them)."""
+        if lib().orc_self_normals(self.h, int(mode)):
+            raise RuntimeError(lib().orc_last_error().decode())
 
     def rollout(self, x0, v0, steps, act=None, record=False):
         """x0, v0: (N,3) or (3N,). act: (steps, n_groups) or None.

@@ -423,6 +423,18 @@ struct ContactIter {
   std::vector<int> sph, det;  // per cv: sphere index (-1: other obstacle), detection position
   std::vector<double> st;     // per cv: sphere contact time t
   std::vector<char> sroot;    // per cv: t is a crossing root
+  // self contact: whether a group acted, each group as executed (face corners,
+  // basis, approaching members, final stick flags) and the forces it read
+  // (rhs + obstacle reactions, 3N)
+  struct Group {
+    int fv[3];
+    double B[9];
+    std::vector<int> vl;
+    std::vector<char> stick;
+  };
+  bool grp = false;
+  std::vector<Group> groups;
+  std::vector<double> F;
 };
 
 struct StepTape {
@@ -433,7 +445,7 @@ struct StepTape {
   bool contact_active = false;
   std::vector<ContactIter> citers;         // executed contact iterations
   bool final_latest = true;                // result = x_it of the last iteration (else its input)
-  bool self_active = false;                // a self-contact group acted (no reverse for it)
+  bool self_on = false;                    // self contact handled (forces carried between iterations)
 };
 
 struct Scene {
@@ -464,6 +476,13 @@ struct Scene {
   std::vector<int> sc_vtx;                    // contact index -> vertex
   std::vector<std::array<int, 3>> sc_face;    // surface triangles, contact indices
   int outer_cap = 16;  // detection passes with a Jacobi loop (the reference has no cap)
+  // Self-contact impact normals held at recorded values (a checker's switch:
+  // the reference computes them from plain doubles, so its taped gradient
+  // treats them as constants; central differences with the normals replayed
+  // measure that derivative). 0 off, 1 record, 2 replay; keyed by (step of
+  // orc_rollout, detection pass, vertex, face corners).
+  int sn_mode = 0, cur_step = 0;
+  std::map<std::array<int, 6>, std::array<double, 3>> sn_tab;
   std::vector<Plane> planes;
   std::vector<Sphere> spheres;
   std::vector<std::array<double, 9>> tris;  // static obstacle triangles
@@ -1394,6 +1413,7 @@ void step(Scene& s, double* x, double* v, const double* act, StepTape* tape, int
     std::vector<double> self_cur(n3, 0.0), self_next(n3, 0.0), con_next(n3, 0.0);
     std::vector<double> xit = y, oold(n3, 0.0), prev(n3, 0.0), rc(n3);
     bool last_begin = true;
+    if (tape && selfon) tape->self_on = true;
     while (true) {
       std::vector<Impact> fresh;
       // position read by this detection: the last taped output, or the input
@@ -1451,6 +1471,16 @@ void step(Scene& s, double* x, double* v, const double* act, StepTape* tape, int
             if (vf_moving(&xn[3 * va], &ori_x[3 * va], y0, y1, im)) {
               im.a = a;
               for (int k = 0; k < 3; ++k) im.f[k] = f[k];
+              if (s.sn_mode) {
+                const std::array<int, 6> key = {s.cur_step, outer_n, a, f[0], f[1], f[2]};
+                if (s.sn_mode == 1) {
+                  s.sn_tab[key] = {im.n[0], im.n[1], im.n[2]};
+                } else {
+                  auto it = s.sn_tab.find(key);
+                  if (it != s.sn_tab.end())
+                    for (int k = 0; k < 3; ++k) im.n[k] = it->second[k];
+                }
+              }
               sfresh.push_back(im);
             }
           }
@@ -1490,6 +1520,13 @@ void step(Scene& s, double* x, double* v, const double* act, StepTape* tape, int
         for (int k = 0; k < n3; ++k) rc[k] *= s.h;
         if (i == 0) prev = rc;
         std::vector<double> orig = rc;
+        // the obstacle projection's input: rhs plus the previous handling's
+        // self-contact forces
+        std::vector<double> finall = orig;
+        if (selfon)
+          for (int k = 0; k < n3; ++k) finall[k] += self_cur[k];
+        std::vector<ContactIter::Group> gtape;
+        std::vector<double> tape_F;
         const bool have_self = selfon && !groups.empty();
         if (!impacts.empty() || have_self) {  // CollisionHandling, World.h:1317-1600
           for (size_t k = 0; k < impacts.size(); ++k) {
@@ -1556,6 +1593,14 @@ void step(Scene& s, double* x, double* v, const double* act, StepTape* tape, int
                   }
                 if (!changed) break;
               }
+              if (tape) {
+                ContactIter::Group gt;
+                for (int q = 0; q < 3; ++q) gt.fv[q] = fv[q];
+                std::copy(B, B + 9, gt.B);
+                gt.vl = vl;
+                gt.stick = stick;
+                gtape.push_back(std::move(gt));
+              }
               double r0w[3], bf0[3], v0[3];
               toworld(r0, r0w);
               toworld(f0, bf0);
@@ -1569,8 +1614,11 @@ void step(Scene& s, double* x, double* v, const double* act, StepTape* tape, int
                 for (int r = 0; r < 3; ++r) self_next[3 * vl[k] + r] += w[r];
               }
             }
+            if (tape) {
+              tape_F.resize(n3);
+              for (int k = 0; k < n3; ++k) tape_F[k] = orig[k] + con_next[k];
+            }
             for (int k = 0; k < n3; ++k) rc[k] += con_next[k] + self_next[k];
-            if (tape) tape->self_active = true;
           }
           if (selfon) {
             self_cur.swap(self_next);
@@ -1589,12 +1637,17 @@ void step(Scene& s, double* x, double* v, const double* act, StepTape* tape, int
           ci.ori = ori_x;
           ci.xit = xit;
           ci.first = i == 0;
+          if (have_self) {
+            ci.grp = true;
+            ci.groups.swap(gtape);
+            ci.F.swap(tape_F);
+          }
           std::map<int, size_t> win;  // last impact of each vertex wins
           for (size_t k = 0; k < impacts.size(); ++k) win[impacts[k].vtx] = k;
           for (auto& kv : win) {
             ci.cv.push_back(kv.first);
             ci.B.insert(ci.B.end(), &Bs[9 * kv.second], &Bs[9 * kv.second] + 9);
-            ci.f.insert(ci.f.end(), &orig[3 * kv.first], &orig[3 * kv.first] + 3);
+            ci.f.insert(ci.f.end(), &finall[3 * kv.first], &finall[3 * kv.first] + 3);
             const Impact& w = impacts[kv.second];
             ci.sph.push_back(w.sph);
             ci.det.push_back(w.det);
@@ -1697,6 +1750,129 @@ void sphere_normal_vjp(const double* c, const double* x0, const double* x1, doub
   }
 }
 
+// Dual numbers (value and one directional derivative): the self-contact
+// group's Jacobian by forward differentiation, one input direction at a time.
+struct Dual {
+  double v = 0.0, d = 0.0;
+  Dual() = default;
+  Dual(double a, double b = 0.0) : v(a), d(b) {}
+};
+inline Dual operator+(Dual a, Dual b) { return Dual(a.v + b.v, a.d + b.d); }
+inline Dual operator-(Dual a, Dual b) { return Dual(a.v - b.v, a.d - b.d); }
+inline Dual operator*(Dual a, Dual b) { return Dual(a.v * b.v, a.d * b.v + a.v * b.d); }
+inline Dual operator/(Dual a, Dual b) { return Dual(a.v / b.v, (a.d * b.v - a.v * b.d) / (b.v * b.v)); }
+inline Dual sqrt(Dual a) {
+  const double r = std::sqrt(a.v);
+  return Dual(r, a.d / (2.0 * r));
+}
+inline double val(double a) { return a; }
+inline double val(Dual a) { return a.v; }
+
+// One self-contact group of a handling (World.h:1451-1530) as executed: the
+// members that approached (in order) and the final stick flags of
+// do_solve_friction (World.h:1246-1315) are given. F: forces (rhs + obstacle
+// reactions) of the three face corners, then of the members, 3 each; m: their
+// masses (diagonal of K_s). out: the changes the group adds to the
+// self-contact forces, same layout. B: the frozen basis [n t b] (row-major).
+template <typename T>
+void group_apply(const double* B, int nv, const T* F, const T* m, const char* stick, double mu, T* out) {
+  using std::sqrt;
+  auto toloc = [&](const T* w, T* l) {
+    for (int c = 0; c < 3; ++c) l[c] = T(B[0 * 3 + c]) * w[0] + T(B[1 * 3 + c]) * w[1] + T(B[2 * 3 + c]) * w[2];
+  };
+  auto toworld = [&](const T* l, T* w) {
+    for (int r = 0; r < 3; ++r) w[r] = T(B[r * 3 + 0]) * l[0] + T(B[r * 3 + 1]) * l[1] + T(B[r * 3 + 2]) * l[2];
+  };
+  T f0w[3] = {T(0.0), T(0.0), T(0.0)}, f0[3];
+  for (int q = 0; q < 3; ++q)
+    for (int r = 0; r < 3; ++r) f0w[r] = f0w[r] + F[3 * q + r];
+  toloc(f0w, f0);
+  const T m0 = m[0] + m[1] + m[2];
+  std::vector<std::array<T, 3>> fs(nv), rs(nv);
+  std::vector<T> ms(nv);
+  T sm(0.0), smv(0.0);
+  for (int k = 0; k < nv; ++k) {
+    ms[k] = m[3 + k];
+    toloc(&F[3 * (3 + k)], fs[k].data());
+    sm = sm + ms[k];
+    smv = smv + fs[k][0];
+  }
+  T r0[3] = {T(0.0), T(0.0), T(0.0)};
+  for (int k = 0; k < nv; ++k) {
+    T fn = (f0[0] + smv) * ms[k] / (m0 + sm) - fs[k][0];
+    rs[k] = {fn, T(0.0), T(0.0)};
+    r0[0] = r0[0] - fn;
+  }
+  T pst[3] = {f0[0], f0[1], f0[2]}, mst = m0, psl[3] = {T(0.0), T(0.0), T(0.0)}, msl(0.0), fnl(0.0);
+  for (int k = 0; k < nv; ++k) {
+    if (stick[k]) {
+      for (int r = 0; r < 3; ++r) pst[r] = pst[r] + fs[k][r];
+      mst = mst + ms[k];
+    } else {
+      for (int r = 0; r < 3; ++r) psl[r] = psl[r] + fs[k][r];
+      msl = msl + ms[k];
+      fnl = fnl + rs[k][0];
+    }
+  }
+  T v[3] = {pst[0] / mst, pst[1] / mst, pst[2] / mst};
+  r0[1] = v[1] * m0 - f0[1];
+  r0[2] = v[2] * m0 - f0[2];
+  for (int k = 0; k < nv; ++k)
+    if (stick[k]) {
+      rs[k][1] = v[1] * ms[k] - fs[k][1];
+      rs[k][2] = v[2] * ms[k] - fs[k][2];
+    }
+  if (val(msl) > 0) {
+    T rel[3];
+    for (int r = 0; r < 3; ++r) rel[r] = psl[r] / msl - pst[r] / mst;
+    T tl = sqrt(rel[1] * rel[1] + rel[2] * rel[2]);
+    for (int r = 0; r < 3; ++r) rel[r] = rel[r] / tl;
+    T mult = T(mu) * fnl / (msl + mst);
+    r0[1] = r0[1] + rel[1] * mult * m0;
+    r0[2] = r0[2] + rel[2] * mult * m0;
+    for (int k = 0; k < nv; ++k)
+      if (stick[k]) {
+        rs[k][1] = rs[k][1] + rel[1] * mult * ms[k];
+        rs[k][2] = rs[k][2] + rel[2] * mult * ms[k];
+      }
+    T av[3] = {(f0[0] + r0[0]) / m0, (f0[1] + r0[1]) / m0, (f0[2] + r0[2]) / m0};
+    for (int k = 0; k < nv; ++k)
+      if (!stick[k]) {
+        T rv[3];
+        for (int r = 0; r < 3; ++r) rv[r] = fs[k][r] / ms[k] - av[r];
+        T t2 = sqrt(rv[1] * rv[1] + rv[2] * rv[2]);
+        for (int r = 0; r < 3; ++r) rv[r] = rv[r] / t2;
+        T mm = T(0.0) - T(mu) * rs[k][0] * ms[k] / (ms[k] + mst);
+        rs[k][1] = mm * rv[1];
+        rs[k][2] = mm * rv[2];
+      }
+  }
+  T r0w[3], bf0[3];
+  toworld(r0, r0w);
+  toworld(f0, bf0);
+  for (int q = 0; q < 3; ++q)
+    for (int r = 0; r < 3; ++r) out[3 * q + r] = (bf0[r] + r0w[r]) / m0 * m[q] - F[3 * q + r];
+  for (int k = 0; k < nv; ++k) toworld(rs[k].data(), &out[3 * (3 + k)]);
+}
+
+// Vector-Jacobian product of group_apply: Fb, mb += J^T ob.
+void group_vjp(const double* B, int nv, const double* F, const double* m, const char* stick, double mu,
+               const double* ob, double* Fb, double* mb) {
+  const int nf = 3 * (3 + nv), nm = 3 + nv;
+  std::vector<Dual> Fd(nf), md(nm), od(nf);
+  for (int i = 0; i < nf + nm; ++i) {
+    for (int k = 0; k < nf; ++k) Fd[k] = Dual(F[k], k == i ? 1.0 : 0.0);
+    for (int k = 0; k < nm; ++k) md[k] = Dual(m[k], nf + k == i ? 1.0 : 0.0);
+    group_apply<Dual>(B, nv, Fd.data(), md.data(), stick, mu, od.data());
+    double g = 0.0;
+    for (int o = 0; o < nf; ++o) g += ob[o] * od[o].d;
+    if (i < nf)
+      Fb[i] += g;
+    else
+      mb[i - nf] += g;
+  }
+}
+
 // Reverse sweep of the taped contact iterations. In: ybar = adjoint of the
 // step's final positions. Out: ybar = adjoint of the PD result y; xbar, zsum
 // and the parameter gradients accumulate.
@@ -1710,6 +1886,9 @@ void contact_backward(Scene& s, const StepTape& tp, std::vector<double>& ybar, s
   std::vector<double> gfin = ybar, XB(n3, 0.0), PB(n3, 0.0), rcb(n3), z(n3), coefD(N), ob(n3);
   // adjoints of the contact-phase positions P_k read by sphere detections
   std::vector<std::vector<double>> pend(J + 1);
+  // with self contact: adjoint of the self-contact forces entering the
+  // iteration being reversed (the previous handling's, carried over)
+  std::vector<double> SCB(n3, 0.0);
   if (tp.final_latest) XB = gfin;
   for (int j = J - 1; j >= 0; --j) {
     const ContactIter& it = tp.citers[j];
@@ -1725,7 +1904,66 @@ void contact_backward(Scene& s, const StepTape& tp, std::vector<double>& ybar, s
       coefD[v] = cd;
     }
     for (int k = 0; k < n3; ++k) rcb[k] = al * PB[k];
-    for (size_t q = 0; q < it.cv.size(); ++q) {
+    if (tp.self_on) {
+      // rhs = forces + obstacle reactions + self-contact forces while a group
+      // acts (World.h:1585-1588), else the obstacle reaction replaces the
+      // forces; the obstacle projection reads forces + the carried self-contact
+      // forces; group membership, stick flags and bases frozen as executed
+      std::vector<double> Q = rcb, cnb, scb_new(n3, 0.0);
+      if (it.grp) {
+        cnb = Q;
+        std::vector<double> Fb(n3, 0.0);
+        for (const ContactIter::Group& g : it.groups) {
+          const int nv = (int)g.vl.size();
+          std::vector<int> vs(g.fv, g.fv + 3);
+          vs.insert(vs.end(), g.vl.begin(), g.vl.end());
+          std::vector<double> Fl(3 * vs.size()), ml(vs.size()), ob(3 * vs.size()), Fbl(3 * vs.size(), 0.0),
+              mbl(vs.size(), 0.0);
+          for (size_t a = 0; a < vs.size(); ++a) {
+            ml[a] = s.Kdiag[vs[a]];
+            for (int c = 0; c < 3; ++c) {
+              Fl[3 * a + c] = it.F[3 * vs[a] + c];
+              ob[3 * a + c] = Q[3 * vs[a] + c] + SCB[3 * vs[a] + c];
+            }
+          }
+          group_vjp(g.B, nv, Fl.data(), ml.data(), g.stick.data(), s.self_mu, ob.data(), Fbl.data(), mbl.data());
+          for (size_t a = 0; a < vs.size(); ++a) {
+            coefD[vs[a]] += mbl[a];
+            for (int c = 0; c < 3; ++c) Fb[3 * vs[a] + c] += Fbl[3 * a + c];
+          }
+        }
+        for (int k = 0; k < n3; ++k) {
+          cnb[k] += Fb[k];
+          rcb[k] += Fb[k];
+        }
+      }
+      for (size_t q = 0; q < it.cv.size(); ++q) {
+        const int v = it.cv[q];
+        const double* B = &it.B[9 * q];
+        const double* fin = &it.f[3 * q];
+        if (B[0] * fin[0] + B[3] * fin[1] + B[6] * fin[2] > 0.0) continue;  // take-off: no reaction
+        const double* qb = it.grp ? &cnb[3 * v] : &Q[3 * v];
+        double o[3];
+        project_contact_vjp(B, s.mu, fin, qb, o);
+        for (int c = 0; c < 3; ++c) {
+          rcb[3 * v + c] = it.grp ? rcb[3 * v + c] + o[c] : o[c];
+          scb_new[3 * v + c] = o[c];
+        }
+        if (it.sph[q] >= 0) {
+          double n[3] = {B[0], B[3], B[6]}, nb[3];
+          project_contact_nvjp(n, s.mu, fin, qb, nb);
+          if (nb[0] != 0.0 || nb[1] != 0.0 || nb[2] != 0.0) {
+            const int k = it.det[q];
+            const std::vector<double>& Pk = k < J ? tp.citers[k].ori : tp.citers[J - 1].xit;
+            if (pend[k].empty()) pend[k].assign(n3, 0.0);
+            sphere_normal_vjp(s.spheres[it.sph[q]].c, &xn[3 * v], &Pk[3 * v], it.st[q], it.sroot[q] != 0, n, nb,
+                              &xbar[3 * v], &pend[k][3 * v]);
+          }
+        }
+      }
+      SCB.swap(scb_new);
+    }
+    for (size_t q = 0; q < (tp.self_on ? 0 : it.cv.size()); ++q) {
       int v = it.cv[q];
       double qb[3] = {al * PB[3 * v], al * PB[3 * v + 1], al * PB[3 * v + 2]}, o[3];
       project_contact_vjp(&it.B[9 * q], s.mu, &it.f[3 * q], qb, o);
@@ -2402,8 +2640,10 @@ int orc_rollout(void* p, int steps, const double* x0, const double* v0, const do
   std::vector<double> x(x0, x0 + n3), v(v0, v0 + n3);
   if (record) s->tape.assign(steps, StepTape());
   else s->tape.clear();
+  if (s->sn_mode == 1) s->sn_tab.clear();
   for (int n = 0; n < steps; ++n) {
     int a = 0, b = 0, c = 0;
+    s->cur_step = n;
     step(*s, x.data(), v.data(), act ? act + (size_t)n * s->n_groups : nullptr, record ? &s->tape[n] : nullptr, &a,
          &b, &c);
     if (iters) {
@@ -2417,11 +2657,27 @@ int orc_rollout(void* p, int steps, const double* x0, const double* v0, const do
   return 0;
 }
 
+// Self-contact normals: 0 computed, 1 recorded by the next orc_rollout, 2
+// replayed from the last recording (single rollouts only).
+int orc_self_normals(void* p, int mode) {
+  Scene* s = (Scene*)p;
+  if (mode < 0 || mode > 2) {
+    g_err = "mode must be 0, 1 or 2";
+    return 1;
+  }
+  s->sn_mode = mode;
+  return 0;
+}
+
 // Several independent rollouts (shared factor), OpenMP over rollouts when
 // built with it. x0,v0,xT,vT: R x 3N; act: R x steps x n_groups (or null).
 int orc_rollout_batch(void* p, int R, int steps, const double* x0, const double* v0, const double* act, double* xT,
                       double* vT, int nthreads) {
   Scene* s = (Scene*)p;
+  if (s->sn_mode) {
+    g_err = "self-contact normal recording/replay is for single rollouts";
+    return 1;
+  }
   if (!s->factored && !factorize(*s)) return 1;
   int n3 = 3 * s->N;
 #ifdef _OPENMP
@@ -2448,14 +2704,17 @@ int orc_rollout_batch(void* p, int R, int steps, const double* x0, const double*
 int orc_fwd_adj_batch(void* p, int R, int steps, const double* x0, const double* v0, const double* act,
                       const double* gxT, const double* gvT, double* gx0, double* gE, int* iters, int nthreads) {
   Scene* s = (Scene*)p;
+  if (s->sn_mode) {
+    g_err = "self-contact normal recording/replay is for single rollouts";
+    return 1;
+  }
   if (!s->factored && !factorize(*s)) return 1;
   const int n3 = 3 * s->N;
   const size_t T = s->tets.size();
   std::vector<std::vector<double>> gEr(R, std::vector<double>(gE ? T : 0, 0.0));
-  int bad = 0;
 #ifdef _OPENMP
   if (nthreads > 0) omp_set_num_threads(nthreads);
-#pragma omp parallel for schedule(dynamic, 1) reduction(+ : bad)
+#pragma omp parallel for schedule(dynamic, 1)
 #endif
   for (int r = 0; r < R; ++r) {
     std::vector<double> x(x0 + (size_t)r * n3, x0 + (size_t)(r + 1) * n3), v(v0 + (size_t)r * n3, v0 + (size_t)(r + 1) * n3);
@@ -2469,10 +2728,6 @@ int orc_fwd_adj_batch(void* p, int R, int steps, const double* x0, const double*
     std::vector<double> gact((size_t)std::max(s->n_groups, 1), 0.0);
     double gnu = 0.0;
     for (int n = steps - 1; n >= 0; --n) {
-      if (tape[n].self_active) {
-        bad += 1;
-        break;
-      }
       std::vector<double> gxn(n3, 0.0), gvn(n3, 0.0);
       step_backward(*s, tape[n], gx.data(), gv.data(), gxn.data(), gvn.data(), gE ? gEr[r].data() : nullptr, &gnu,
                     gact.data());
@@ -2480,10 +2735,6 @@ int orc_fwd_adj_batch(void* p, int R, int steps, const double* x0, const double*
       gv.swap(gvn);
     }
     std::copy(gx.begin(), gx.end(), gx0 + (size_t)r * n3);
-  }
-  if (bad) {
-    g_err = "the reverse sweep does not cover self contact";
-    return 1;
   }
   if (gE) {
     std::fill(gE, gE + T, 0.0);
@@ -2511,11 +2762,6 @@ int orc_backward(void* p, const double* gxT, const double* gvT, double* gx0, dou
   Scene* s = (Scene*)p;
   int n3 = 3 * s->N;
   int steps = (int)s->tape.size();
-  for (const StepTape& tp : s->tape)
-    if (tp.self_active) {
-      g_err = "the reverse sweep does not cover self contact";
-      return 1;
-    }
   std::vector<double> gx(gxT, gxT + n3), gv(gvT, gvT + n3);
   size_t T = s->tets.size();
   if (gE) std::fill(gE, gE + T, 0.0);

@@ -18,3 +18,19 @@ def sphere_tumble_scene(mu=0.5, max_iter1=10):
     v0[0::3] = -0.056 - 19.72 * X[:, 2]
     v0[2::3] = -1.452 + 19.72 * X[:, 0]
     return d, v0
+
+
+def side_push_scene(mu=0.5, self_mu=0.5, gap=0.004):
+    """Two 2x2x1 boxes on a friction floor, the second sliding into the first's
+    side: the face it hits has corners on the floor, so obstacle reactions and
+    self-contact groups act on the same vertices. Returns the scene and the
+    vertex count of the first box."""
+    xa, ta = scene.box_grid(2, 2, 1, origin=(0, 0, 0.0015))
+    xb, tb = scene.box_grid(2, 2, 1, origin=(0.1 + gap, 0.02, 0.0015))
+    da = scene.SceneDesc(x_rest=xa, tets=ta, total_mass=1.0, E=np.full(ta.shape[0], 1e5))
+    db = scene.SceneDesc(x_rest=xb, tets=tb, total_mass=0.5, E=np.full(tb.shape[0], 1e5))
+    d = scene.pack_scenes([da, db], translate=False)
+    d.h, d.max_iter, d.gravity, d.contact, d.max_iter1, d.mu = 0.01, 10, (0.0, 0.0, -9.81), True, 10, mu
+    d.planes = np.array([[0, 0, 0, 0, 0, 1.0]])
+    d.self_collision, d.self_mu = True, self_mu
+    return d, xa.shape[0]

@@ -169,8 +169,85 @@ def test_stacked_boxes_do_not_pass_through(oracle_mod):
         x, v, its = o.rollout(d.x_rest, v0, 25, record=True)
         res[on] = x.reshape(-1, 3)[na:, 2].min()
         g, rc = o.backward(np.ones(3 * d.N), np.zeros(3 * d.N))
-        assert (rc != 0) == on                   # no reverse sweep through self contact
+        assert rc == 0 and np.all(np.isfinite(g["x0"]))
     assert res[False] < -0.2 and res[True] > 0.03
+
+
+def _fd_check(o, d, x0, v0, steps, gx, gv, rng, hx, bars):
+    """Central differences of <gx, x_T> + <gv, v_T> with the self-contact
+    normals replayed from the recorded rollout (the reference computes them
+    from plain doubles, so its taped gradient holds them constant), against
+    the reverse sweep: E along all tets and a random per-tet direction (step
+    1e-4 relative), x0 (step hx) and v0 (1e-5), executed iterations equal."""
+    o.self_normals(1)
+    _, _, its = o.rollout(x0, v0, steps, record=True)
+    assert its[:, 1].sum() > 0
+    o.self_normals(2)
+    g, rc = o.backward(gx, gv)
+    assert rc == 0
+    E0 = d.E.copy()
+
+    def loss(E=E0, dx=0.0, dv=0.0):
+        o.set_material(E, 0.4)
+        xT, vT, its1 = o.rollout(x0 + dx, v0 + dv, steps)
+        assert np.array_equal(its1, its)
+        return gx @ xT + gv @ vT
+
+    errs = {}
+    for key, dirE in (("E", E0), ("E_rand", rng.uniform(-1, 1, d.T) * E0)):
+        fd = (loss(E=E0 + 1e-4 * dirE) - loss(E=E0 - 1e-4 * dirE)) / 2e-4
+        errs[key] = abs(fd - g["E"] @ dirE) / abs(fd)
+    o.set_material(E0, 0.4)
+    for key, h in (("x0", hx), ("v0", 1e-5)):
+        dd = h * rng.standard_normal(x0.shape)
+        kw, kw2 = ({"dx": dd}, {"dx": -dd}) if key == "x0" else ({"dv": dd}, {"dv": -dd})
+        fd = (loss(**kw) - loss(**kw2)) / 2
+        errs[key] = abs(fd - g[key] @ dd) / abs(fd)
+    o.self_normals(0)
+    for key, bar in bars.items():
+        assert errs[key] < bar, (key, errs[key])
+    return errs
+
+
+@pytest.mark.parametrize("steps,vz", [(25, -0.6), (16, -1.0)])
+def test_self_contact_gradient_vs_central_fd_stacked(oracle_mod, steps, vz):
+    """Reverse sweep through self contact (World.h:1246-1315, 1446-1586):
+    group membership, stick/slide flags and bases frozen as executed, masses
+    (diag of K_s) and forces taped. A box dropped onto a pinned one. Measured
+    agreement 3e-8 to 1.5e-5 (the E-directions are limited by the differences'
+    roundoff); dropping the masses' adjoint misses E by 1e-3 to 4e-1, dropping
+    the group forces' adjoint misses every direction by a factor 2 or more."""
+    d, na = two_boxes(True)
+    o = oracle_mod.OracleScene(d)
+    x0 = d.x_rest.ravel().copy()
+    v0 = np.zeros(3 * d.N)
+    v0[3 * na:] = np.tile([0.15, 0.05, vz], d.N - na)
+    rng = np.random.default_rng(3)
+    gx = np.zeros(3 * d.N)
+    gv = np.zeros(3 * d.N)
+    gx[3 * na:] = rng.standard_normal(3 * (d.N - na))
+    gv[3 * na:] = 0.01 * rng.standard_normal(3 * (d.N - na))
+    _fd_check(o, d, x0, v0, steps, gx, gv, rng, 1e-6, {"E": 5e-5, "E_rand": 5e-5, "x0": 1e-5, "v0": 5e-5})
+
+
+@pytest.mark.parametrize("mu,self_mu,steps,vx", [(0.5, 0.5, 16, -0.8), (0.3, 0.2, 14, -1.0)])
+def test_self_contact_with_floor_gradient_vs_central_fd(oracle_mod, mu, self_mu, steps, vx):
+    """Self contact and floor contact on the same vertices: the floor's
+    reaction reads the forces plus the previous handling's self-contact
+    forces, and the groups read the forces plus the floor's reactions. Measured
+    agreement 5e-11 to 2e-8; carrying no adjoint through the previous
+    self-contact forces, or reversing the floor's reaction with the rhs
+    adjoint instead of its own, misses by 0.1 to 3."""
+    from contact_scenes import side_push_scene
+    d, na = side_push_scene(mu=mu, self_mu=self_mu)
+    o = oracle_mod.OracleScene(d)
+    x0 = d.x_rest.ravel().copy()
+    v0 = np.zeros(3 * d.N)
+    v0[3 * na:] = np.tile([vx, 0.05, -0.1], d.N - na)
+    rng = np.random.default_rng(3)
+    gx = rng.standard_normal(3 * d.N)
+    gv = 0.01 * rng.standard_normal(3 * d.N)
+    _fd_check(o, d, x0, v0, steps, gx, gv, rng, 1e-7, {"E": 1e-5, "E_rand": 1e-5, "x0": 1e-5, "v0": 1e-5})
 
 
 def test_loader_reads_self_collision(tmp_path):
