@@ -25,6 +25,8 @@
 #include "pdm_internal.h"
 #include "pdm_kernels.h"
 #include "pdm_factor_kernels.h"
+#define PDM_HD __host__ __device__
+#include "pdm_self_adj.h"
 
 using namespace pdm;
 
@@ -2043,7 +2045,7 @@ __global__ void k_self_group(const int* __restrict__ sc_vtx, const int* __restri
                              const int* __restrict__ cface, const double* __restrict__ imp_n,
                              const double* __restrict__ Fo, const double* __restrict__ CN,
                              const double* __restrict__ Kdiag, double mu, double* __restrict__ GF,
-                             double* __restrict__ GV) {
+                             double* __restrict__ GV, int* __restrict__ tFlags = nullptr) {
   long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (idx >= (long long)nsf * R) return;
   int f = (int)(idx / R), r = (int)(idx % R);
@@ -2183,6 +2185,11 @@ __global__ void k_self_group(const int* __restrict__ sc_vtx, const int* __restri
       }
     if (!changed) break;
   }
+  if (tFlags) {  // the reverse sweep's frozen decisions: members that approached, final stick flags
+    int fl = 0;
+    for (int k = 0; k < nv; ++k) fl |= (1 << vl[k]) | (stick[k] ? (1 << (16 + vl[k])) : 0);
+    tFlags[idx] = fl;
+  }
   double r0w[3], bf0[3], v0[3];
   toworld(r0, r0w);
   toworld(f0, bf0);
@@ -2258,6 +2265,49 @@ __global__ void k_self_finish(const int* __restrict__ scf_ptr, const int* __rest
     Prev[base + c] = p;
     Xit[base + c] = Xn[base + c] + (p / h) / kd;
   }
+}
+
+// Self-contact groups of a pass for the reverse sweep: counts, leaders,
+// members (-1 past the count), the leader's impact normal, each contact
+// vertex's own group face (-1: none) and the groups per rollout.
+__global__ void k_self_tape_pass(int ncs, int nsf, int R, const int* __restrict__ gcount, const int* __restrict__ glead,
+                                 const int* __restrict__ gmem, const int* __restrict__ cface,
+                                 const int* __restrict__ imp_f, const double* __restrict__ imp_n,
+                                 const int* __restrict__ ngrp, int* __restrict__ tCount, int* __restrict__ tLead,
+                                 int* __restrict__ tMem, double* __restrict__ tNrm, int* __restrict__ tOwn,
+                                 int* __restrict__ tNgrp) {
+  const long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const long long NA = (long long)ncs * R, NF = (long long)nsf * R;
+  if (idx < NF) {
+    const int cnt = gcount[idx];
+    tCount[idx] = cnt;
+    tLead[idx] = glead[idx];
+    for (int k = 0; k < kSelfMem; ++k) tMem[(long long)k * NF + idx] = k < cnt ? gmem[(long long)k * NF + idx] : -1;
+    double n[3] = {0.0, 0.0, 0.0};
+    if (cnt > 0) {
+      const long long li = (long long)glead[idx] * R + idx % R;
+      const int ls = cface[li];
+      if (ls >= 0)
+        for (int c = 0; c < 3; ++c) n[c] = imp_n[(long long)(3 * ls + c) * NA + li];
+    }
+    for (int c = 0; c < 3; ++c) tNrm[3 * idx + c] = n[c];
+  }
+  if (idx < NA) {
+    const int slot = cface[idx];
+    tOwn[idx] = slot >= 0 ? imp_f[(long long)slot * NA + idx] : -1;
+  }
+  if (idx < R) tNgrp[idx] = ngrp[idx];
+}
+
+// The forces the groups of an iteration read (rhs + obstacle reactions) at the
+// surface vertices, for the reverse sweep.
+__global__ void k_self_tape_F(const int* __restrict__ sc_vtx, int ncs, int C, const double* __restrict__ Fo,
+                              const double* __restrict__ CN, double* __restrict__ tF) {
+  const long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx >= (long long)ncs * C) return;
+  const int a = (int)(idx / C), col = (int)(idx % C);
+  const long long o = (long long)sc_vtx[a] * C + col;
+  tF[idx] = Fo[o] + CN[o];
 }
 
 // Rollouts with no fresh impact leave the contact phase; the ones that stay
@@ -2431,11 +2481,9 @@ __global__ void k_contact_jacobi(const int* __restrict__ Lptr, const int* __rest
     }
   }
   int ci = cv_of_v[v];
-  if (ci >= 0 && tRC) {
+  if (ci >= 0 && tRC) {  // the obstacle projection's input (with self contact: + the carried forces)
     long long tb = (long long)ci * C + 3 * r;
-    tRC[tb] = rc[0];
-    tRC[tb + 1] = rc[1];
-    tRC[tb + 2] = rc[2];
+    for (int c = 0; c < 3; ++c) tRC[tb + c] = self ? rc[c] + self_cur[base + c] : rc[c];
     if (tN) {  // once per pass: the winners do not change within a pass
       tN[tb] = 0.0;
       tN[tb + 1] = 0.0;
@@ -2736,6 +2784,239 @@ __global__ void k_cadj_post(const double* __restrict__ OB, const double* __restr
   XB[o] = OB[o] + (inj[r] ? Gf[o] : 0.0);
 }
 
+// ---------------------------------------- contact-phase adjoint, self contact
+// With self contact on (World.h:1373-1430, 1446-1600) an iteration's rhs is
+// forces + obstacle reactions + self-contact forces while the rollout has a
+// group, else the obstacle reaction replaces the forces; the obstacle
+// projection reads forces + the previous handling's self-contact forces
+// (carried, adjoint SCB). Group members, stick flags and bases are frozen as
+// executed; masses (diag of K_s) and forces are differentiated.
+
+// First part of one iteration's reverse: QB = alpha * adjoint of prev (the rhs
+// adjoint), QF = the first iteration's (1 - alpha) share that goes to the
+// forces, the running prev adjoint, x_n's adjoint and the K_s-diagonal
+// coefficient of the Jacobi update.
+template <bool UNI>
+__global__ void k_sadj_pre(const double* __restrict__ XB, double* __restrict__ PB, double* __restrict__ Xb,
+                           const double* __restrict__ Xit1, const double* __restrict__ Xn,
+                           const double* __restrict__ Kdiag, double alpha, double h, int first,
+                           const int* __restrict__ alist, const int* __restrict__ acount, double* __restrict__ QB,
+                           double* __restrict__ QF, double* __restrict__ CD, int N, int R) {
+  long long idx = (long long)xcd_block() * blockDim.x + threadIdx.x;
+  if (idx >= (long long)N * R) return;
+  int v = (int)(idx / R);
+  const int j = (int)(idx % R);
+  if (UNI) v = __builtin_amdgcn_readfirstlane(v);
+  if (j >= *acount) return;
+  const int r = alist[j];
+  const long long base = (long long)v * 3 * R + 3 * r;
+  const double kd = Kdiag[v];
+  double cd = 0.0;
+  for (int c = 0; c < 3; ++c) {
+    const double xb = XB[base + c];
+    Xb[base + c] += xb;
+    const double pb = PB[base + c] + xb / (kd * h);
+    cd -= xb * (Xit1[base + c] - Xn[base + c]) / kd;
+    QB[base + c] = alpha * pb;
+    QF[base + c] = first ? (1.0 - alpha) * pb : 0.0;
+    PB[base + c] = first ? 0.0 : (1.0 - alpha) * pb;
+  }
+  CD[(long long)v * R + r] = cd;
+}
+
+// Reverse of one group (face f, rollout r) that acted in this iteration:
+// output adjoints QB + SCB at its vertices; input adjoints into per-group
+// slots (face corners: SFb/SFm; members: SVb/SVm, zero for members that
+// receded), summed per vertex by k_sadj_finish.
+__global__ void k_sadj_group(const int* __restrict__ sc_vtx, const int* __restrict__ sc_face, int ncs, int nsf, int R,
+                             const int* __restrict__ exec, const int* __restrict__ tNgrp,
+                             const int* __restrict__ tCount, const int* __restrict__ tMem,
+                             const double* __restrict__ tNrm, const int* __restrict__ tFlags,
+                             const double* __restrict__ tF, const double* __restrict__ Kdiag, double mu,
+                             const double* __restrict__ QB, const double* __restrict__ SCB, double* __restrict__ SFb,
+                             double* __restrict__ SFm, double* __restrict__ SVb, double* __restrict__ SVm) {
+  const long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx >= (long long)nsf * R) return;
+  const int f = (int)(idx / R), r = (int)(idx % R);
+  if (!exec[r] || tNgrp[r] == 0) return;
+  const int cnt = min(tCount[idx], kSelfMem);
+  if (cnt == 0) return;
+  const long long C = 3LL * R, NF = (long long)nsf * R;
+  int mem[kSelfMem];
+  for (int k = 0; k < cnt; ++k) mem[k] = tMem[(long long)k * NF + idx];
+  for (int i = 1; i < cnt; ++i)  // ascending contact index, as the forward orders them
+    for (int j = i; j > 0 && mem[j - 1] > mem[j]; --j) {
+      int t = mem[j];
+      mem[j] = mem[j - 1];
+      mem[j - 1] = t;
+    }
+  const double* n = tNrm + 3 * idx;
+  double t[3] = {0.0, n[2], -n[1]};
+  double tn = sqrt(t[1] * t[1] + t[2] * t[2]);
+  if (tn < 1e-15) {
+    t[0] = -n[2]; t[1] = 0.0; t[2] = n[0];
+    tn = sqrt(t[0] * t[0] + t[2] * t[2]);
+  }
+  t[0] /= tn; t[1] /= tn; t[2] /= tn;
+  double b[3] = {n[1] * t[2] - n[2] * t[1], n[2] * t[0] - n[0] * t[2], n[0] * t[1] - n[1] * t[0]};
+  const double bn = sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+  const double nn = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+  double E[3][3];
+  for (int c = 0; c < 3; ++c) {
+    E[0][c] = n[c] / nn;
+    E[1][c] = t[c];
+    E[2][c] = b[c] / bn;
+  }
+  const int fl = tFlags[idx];
+  int fa[3], fv[3];
+  double Ff[3][3], mf[3], GFb[3][3];
+  for (int q = 0; q < 3; ++q) {
+    fa[q] = sc_face[3 * f + q];
+    fv[q] = sc_vtx[fa[q]];
+    mf[q] = Kdiag[fv[q]];
+    for (int c = 0; c < 3; ++c) {
+      Ff[q][c] = tF[(long long)fa[q] * C + 3 * r + c];
+      const long long o = (long long)fv[q] * C + 3 * r + c;
+      GFb[q][c] = QB[o] + SCB[o];
+    }
+  }
+  int nv = 0, who[kSelfMem];
+  double Fm[kSelfMem][3], mm[kSelfMem], GVb[kSelfMem][3];
+  bool stick[kSelfMem];
+  for (int k = 0; k < cnt; ++k) {
+    if (!((fl >> k) & 1)) continue;
+    const int vk = sc_vtx[mem[k]];
+    who[nv] = k;
+    mm[nv] = Kdiag[vk];
+    stick[nv] = (fl >> (16 + k)) & 1;
+    for (int c = 0; c < 3; ++c) {
+      Fm[nv][c] = tF[(long long)mem[k] * C + 3 * r + c];
+      const long long o = (long long)vk * C + 3 * r + c;
+      GVb[nv][c] = QB[o] + SCB[o];
+    }
+    ++nv;
+  }
+  double Ffb[3][3] = {}, Fmb[kSelfMem][3] = {}, mfb[3] = {0.0, 0.0, 0.0}, mmb[kSelfMem] = {};
+  pdmself::group_reverse(E, nv, mf, mm, Ff, Fm, stick, mu, GFb, GVb, Ffb, Fmb, mfb, mmb);
+  for (int q = 0; q < 3; ++q) {
+    for (int c = 0; c < 3; ++c) SFb[(long long)(3 * f + q) * C + 3 * r + c] = Ffb[q][c];
+    SFm[(long long)(3 * f + q) * R + r] = mfb[q];
+  }
+  for (int k = 0; k < cnt; ++k) {
+    for (int c = 0; c < 3; ++c) SVb[(long long)mem[k] * C + 3 * r + c] = 0.0;
+    SVm[(long long)mem[k] * R + r] = 0.0;
+  }
+  for (int kk = 0; kk < nv; ++kk) {
+    const int a = mem[who[kk]];
+    for (int c = 0; c < 3; ++c) SVb[(long long)a * C + 3 * r + c] = Fmb[kk][c];
+    SVm[(long long)a * R + r] = mmb[kk];
+  }
+}
+
+// Second part of one iteration's reverse, per (vertex, listed rollout): the
+// groups' shares summed in the forward's order (ascending leader), the
+// obstacle projection's reverse (its output adjoint: the rhs adjoint plus the
+// groups' while the rollout has a group), the forces' adjoint z = h * (...)
+// for the element VJP, and the carried forces' adjoint SCB.
+__global__ void k_sadj_finish(const int* __restrict__ scf_ptr, const int* __restrict__ scf_q,
+                              const int* __restrict__ sc_of_v, int R, const int* __restrict__ tNgrp,
+                              const int* __restrict__ tCount, const int* __restrict__ tLead,
+                              const int* __restrict__ tOwn, const double* __restrict__ SFb,
+                              const double* __restrict__ SFm, const double* __restrict__ SVb,
+                              const double* __restrict__ SVm, const double* __restrict__ QB,
+                              const double* __restrict__ QF, const int* __restrict__ cv_of_v,
+                              const double* __restrict__ tRC, const double* __restrict__ tN, double mu, double h,
+                              const double* __restrict__ Xn, const double* __restrict__ spheres, int nsph,
+                              const int* __restrict__ tSi, const double* __restrict__ tSt,
+                              const double* __restrict__ tSx1, double* __restrict__ NB, long long nbstride,
+                              const int* __restrict__ alist, const int* __restrict__ acount,
+                              double* __restrict__ Xb, double* __restrict__ CD, double* __restrict__ SCB,
+                              double* __restrict__ Zb, double* __restrict__ Zs, int N) {
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)N * R) return;
+  const int v = (int)(idx / R), j = (int)(idx % R);
+  if (j >= *acount) return;
+  const int r = alist[j];
+  const long long C = 3LL * R, base = (long long)v * C + 3 * r;
+  const bool grp = tNgrp[r] > 0;
+  double og[3] = {QB[base], QB[base + 1], QB[base + 2]};
+  if (grp) {
+    double sf[3] = {0.0, 0.0, 0.0}, sm = 0.0;
+    int own_lead = 0x7fffffff;
+    const int a = sc_of_v[v];
+    if (a >= 0) {
+      const int of = tOwn[(long long)a * R + r];
+      if (of >= 0) own_lead = tLead[(long long)of * R + r];
+    }
+    for (int last = -1;;) {
+      int best = 0x7fffffff, bq = -1;
+      for (int q = scf_ptr[v]; q < scf_ptr[v + 1]; ++q) {
+        const long long fr = (long long)(scf_q[q] / 3) * R + r;
+        if (tCount[fr] == 0) continue;
+        const int L = tLead[fr];
+        if (L > last && L < best) {
+          best = L;
+          bq = q;
+        }
+      }
+      if (own_lead > last && own_lead < best) {
+        best = own_lead;
+        bq = -2;
+      }
+      if (best == 0x7fffffff) break;
+      if (bq == -2) {
+        for (int c = 0; c < 3; ++c) sf[c] += SVb[(long long)a * C + 3 * r + c];
+        sm += SVm[(long long)a * R + r];
+      } else {
+        for (int c = 0; c < 3; ++c) sf[c] += SFb[(long long)scf_q[bq] * C + 3 * r + c];
+        sm += SFm[(long long)scf_q[bq] * R + r];
+      }
+      last = best;
+    }
+    for (int c = 0; c < 3; ++c) og[c] += sf[c];
+    CD[(long long)v * R + r] += sm;
+  }
+  double scb[3] = {0.0, 0.0, 0.0};
+  const int ci = cv_of_v[v];
+  if (ci >= 0) {
+    const long long tb = (long long)ci * C + 3 * r;
+    const double* n = tN + tb;
+    const double* fin = tRC + tb;
+    const double nn = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    if (nn > 0.0 && !((n[0] * fin[0] + n[1] * fin[1] + n[2] * fin[2]) / nn > 0.0)) {
+      const double qb[3] = {og[0], og[1], og[2]};  // grp: rhs + groups' adjoint; else the rhs adjoint
+      double o[3];
+      cone_vjp(n, mu, fin, qb, o);
+      for (int c = 0; c < 3; ++c) {
+        og[c] = grp ? og[c] + o[c] : o[c];
+        scb[c] = o[c];
+      }
+      const long long wi = (long long)ci * R + r;
+      const int si = tSi ? tSi[wi] : 0;
+      if (si > 0) {
+        double nb[3];
+        cone_nvjp(n, mu, fin, qb, nb);
+        if (nb[0] != 0.0 || nb[1] != 0.0 || nb[2] != 0.0) {
+          const int k = (si - 1) >> 2, os = k % nsph, ps = k / nsph;
+          double x0b[3] = {0.0, 0.0, 0.0}, x1b[3];
+          sphere_normal_vjp(spheres + 4 * os, Xn + base, tSx1 + 3 * wi, tSt[wi], ((si - 1) >> 1) & 1, n, nb, x0b,
+                            x1b);
+          for (int c = 0; c < 3; ++c) {
+            Xb[base + c] += x0b[c];
+            NB[ps * nbstride + tb + c] += x1b[c];
+          }
+        }
+      }
+    }
+  }
+  for (int c = 0; c < 3; ++c) {
+    const double z = h * (og[c] + QF[base + c]);
+    Zb[base + c] = z;
+    Zs[base + c] += z;
+    SCB[base + c] = scb[c];
+  }
+}
+
 // ------------------------------------------------------------------ engine
 }  // namespace
 
@@ -2848,6 +3129,12 @@ struct ContactTape {
   // that position was x_it (1) or the last iteration's input (0)
   DevBuf<int> sinfo, dl;
   DevBuf<double> st, sx1;
+  // with self contact, per pass: group counts, leaders, members, leader
+  // normals, each contact vertex's own group face, groups per rollout; per
+  // iteration: the forces the groups read (surface vertices) and each group's
+  // frozen decisions (approached / stick bits)
+  DevBuf<int> gCount, gLead, gMem, gOwn, gNgrp, gFlags;
+  DevBuf<double> gNrm, gF;
 };
 
 template <typename X>
@@ -2943,6 +3230,7 @@ struct pdm_scene_s {
   DevBuf<int> d_bvh_a, d_bvh_b, d_bvh_tri;
   std::vector<std::unique_ptr<ContactTape>> ctape;  // per step of the segment being reversed
   DevBuf<double> cXB, cPB, cOB, cGf, cCD, cNB;
+  DevBuf<double> cQB, cQF, cSCB, sSFb, sSFm, sSVb, sSVm;  // self-contact reverse
   DevBuf<int> win_sph;
   DevBuf<double> win_t, win_x1;
   DevBuf<int> cinjG;          // contact reverse: injection flags per (iteration, rollout)
@@ -4121,6 +4409,23 @@ int contact_phase(pdm_scene s, const double* act, int act_steps, int step, Conta
           grow_keep(tp->nrm, (size_t)(tp->P + 1) * cvc, (size_t)tp->P * cvc, s->stream) ||
           grow_keep(tp->exec, g1 * R, (size_t)tp->G * R, s->stream))
         return 1;
+      if (self) {
+        const size_t P = (size_t)tp->P, NF = (size_t)s->nsf * R, NA = (size_t)s->ncs * R;
+        const size_t NFc = (size_t)s->ncs * C;
+        if (grow_keep(tp->gCount, (P + 1) * NF, P * NF, s->stream) ||
+            grow_keep(tp->gLead, (P + 1) * NF, P * NF, s->stream) ||
+            grow_keep(tp->gMem, (P + 1) * kSelfMem * NF, P * kSelfMem * NF, s->stream) ||
+            grow_keep(tp->gNrm, (P + 1) * 3 * NF, P * 3 * NF, s->stream) ||
+            grow_keep(tp->gOwn, (P + 1) * NA, P * NA, s->stream) || grow_keep(tp->gNgrp, (P + 1) * R, P * R, s->stream) ||
+            grow_keep(tp->gF, g1 * NFc, (size_t)tp->G * NFc, s->stream) ||
+            grow_keep(tp->gFlags, g1 * NF, (size_t)tp->G * NF, s->stream))
+          return 1;
+        hipLaunchKernelGGL(k_self_tape_pass, dim3(cdiv((long long)std::max(std::max(NF, NA), (size_t)R), kThreads)),
+                           dim3(kThreads), 0, s->stream, s->ncs, s->nsf, R, s->s_gcount.p, s->s_glead.p, s->s_gmem.p,
+                           s->s_cface.p, s->s_impf.p, s->s_impn.p, s->s_ngrp.p, tp->gCount.p + P * NF,
+                           tp->gLead.p + P * NF, tp->gMem.p + P * kSelfMem * NF, tp->gNrm.p + P * 3 * NF,
+                           tp->gOwn.p + P * NA, tp->gNgrp.p + P * R);
+      }
       if (nsph > 0) {
         size_t P = (size_t)tp->P;
         if (grow_keep(tp->sinfo, (P + 1) * ni, P * ni, s->stream) || grow_keep(tp->st, (P + 1) * ni, P * ni, s->stream) ||
@@ -4151,10 +4456,13 @@ int contact_phase(pdm_scene s, const double* act, int act_steps, int step, Conta
                          self ? s->s_Fo.p : nullptr, self ? s->s_CN.p : nullptr, self ? s->s_RC.p : nullptr,
                          self ? s->s_ngrp.p : nullptr);
       if (self) {
+        if (tp)
+          hipLaunchKernelGGL(k_self_tape_F, dim3(cdiv((long long)s->ncs * C, kThreads)), dim3(kThreads), 0, s->stream,
+                             s->d_sc_vtx.p, s->ncs, C, s->s_Fo.p, s->s_CN.p, tp->gF.p + (size_t)tp->G * s->ncs * C);
         hipLaunchKernelGGL(k_self_group, dim3(cdiv((long long)s->nsf * R, kThreads)), dim3(kThreads), 0, s->stream,
                            s->d_sc_vtx.p, s->d_sc_face.p, s->ncs, s->nsf, R, s->iactive.p, s->s_gcount.p,
                            s->s_glead.p, s->s_gmem.p, s->s_cface.p, s->s_impn.p, s->s_Fo.p, s->s_CN.p, s->Kdiag.p,
-                           S.self_mu, s->s_GF.p, s->s_GV.p);
+                           S.self_mu, s->s_GF.p, s->s_GV.p, tp ? tp->gFlags.p + (size_t)tp->G * s->nsf * R : nullptr);
         hipLaunchKernelGGL(k_self_finish, dim3(cdiv((long long)N * R, kThreads)), dim3(kThreads), 0, s->stream,
                            s->d_scf_ptr.p, s->d_scf_q.p, s->d_sc_of_v.p, s->ncs, R, s->s_gcount.p, s->s_glead.p,
                            s->s_impf.p, s->s_cface.p, s->s_ngrp.p,
@@ -4480,6 +4788,7 @@ int contact_reverse(pdm_scene s, ContactTape& tp, const double* Xn, const double
     hipLaunchKernelGGL(k_cadj_nb, dim3(cdiv(ni, kThreads)), dim3(kThreads), 0, s->stream, s->cXB.p,
                        s->cNB.p + (size_t)pass * cvc, tp.dl.p + (size_t)pass * R, want, s->cvtx.p, ni, R);
   };
+  if (s->ncs > 0) HIPX(hipMemsetAsync(s->cSCB.p, 0, nc * sizeof(double), s->stream));
   int pend_input = -1;  // pass whose input-reading rollouts get their NB after the next iteration
   for (int g = G - 1; g >= 0; --g) {
     bool any = false;
@@ -4511,6 +4820,25 @@ int contact_reverse_iter(pdm_scene s, ContactTape& tp, int g, const double* Xn, 
     const double* ori = tp.xit.p + (size_t)g * nc;
     hipLaunchKernelGGL(k_compact, dim3(1), dim3(1024), 0, s->stream, tp.exec.p + (size_t)g * R, R, s->alist.p,
                        s->acount.p);
+    if (s->ncs > 0) {
+      const size_t NF = (size_t)s->nsf * R, NA = (size_t)s->ncs * R;
+      hipLaunchKernelGGL((R % 64 == 0 ? k_sadj_pre<true> : k_sadj_pre<false>), dim3(cdiv((long long)N * R, kThreads)),
+                         dim3(kThreads), 0, s->stream, s->cXB.p, s->cPB.p, s->Xb.p,
+                         tp.xit.p + (size_t)(g + 1) * nc, Xn, s->Kdiag.p, S.alpha, S.h, (int)tp.first[g], s->alist.p,
+                         s->acount.p, s->cQB.p, s->cQF.p, s->cCD.p, N, R);
+      hipLaunchKernelGGL(k_sadj_group, dim3(cdiv((long long)NF, kThreads)), dim3(kThreads), 0, s->stream,
+                         s->d_sc_vtx.p, s->d_sc_face.p, s->ncs, s->nsf, R, tp.exec.p + (size_t)g * R,
+                         tp.gNgrp.p + pg * R, tp.gCount.p + pg * NF, tp.gMem.p + pg * kSelfMem * NF,
+                         tp.gNrm.p + pg * 3 * NF, tp.gFlags.p + (size_t)g * NF, tp.gF.p + (size_t)g * s->ncs * C,
+                         s->Kdiag.p, S.self_mu, s->cQB.p, s->cSCB.p, s->sSFb.p, s->sSFm.p, s->sSVb.p, s->sSVm.p);
+      hipLaunchKernelGGL(k_sadj_finish, dim3(cdiv((long long)N * R, kThreads)), dim3(kThreads), 0, s->stream,
+                         s->d_scf_ptr.p, s->d_scf_q.p, s->d_sc_of_v.p, R, tp.gNgrp.p + pg * R, tp.gCount.p + pg * NF,
+                         tp.gLead.p + pg * NF, tp.gOwn.p + pg * NA, s->sSFb.p, s->sSFm.p, s->sSVb.p, s->sSVm.p,
+                         s->cQB.p, s->cQF.p, s->cv_of_v.p, tp.rc.p + (size_t)g * cvc, tp.nrm.p + pg * cvc, S.mu, S.h,
+                         Xn, s->d_spheres.p, nsph, sph ? tp.sinfo.p + pg * ni : nullptr,
+                         sph ? tp.st.p + pg * ni : nullptr, sph ? tp.sx1.p + pg * 3 * ni : nullptr, s->cNB.p,
+                         (long long)cvc, s->alist.p, s->acount.p, s->Xb.p, s->cCD.p, s->cSCB.p, s->Zb.p, s->Zs.p, N);
+    } else
     hipLaunchKernelGGL((R % 64 == 0 ? k_cadj_pre<true> : k_cadj_pre<false>), dim3(cdiv((long long)N * R, kThreads)), dim3(kThreads), 0, s->stream, s->cXB.p,
                        s->cPB.p, s->Xb.p, tp.xit.p + (size_t)(g + 1) * nc, Xn, s->Kdiag.p, s->cv_of_v.p,
                        tp.rc.p + (size_t)g * cvc, tp.nrm.p + pg * cvc, S.mu, S.alpha, S.h, (int)tp.first[g],
@@ -4550,10 +4878,6 @@ int backward_impl(pdm_scene s, const double* gxT, const double* gvT, double* gx0
     g_err = "backward needs a recorded forward (record=1)";
     return 1;
   }
-  if (s->ncs > 0) {
-    g_err = "the reverse sweep does not cover self contact";
-    return 1;
-  }
   HostScene& S = s->S;
   int N = S.N, C = s->C, R = s->R, steps = s->tape_steps;
   size_t nc = s->NC();
@@ -4566,6 +4890,10 @@ int backward_impl(pdm_scene s, const double* gxT, const double* gvT, double* gx0
   if (S.M > 0 && s->gMpart.alloc((size_t)S.M * R)) return 1;
   if (s->ncv > 0) {
     if (s->cXB.alloc(nc) || s->cPB.alloc(nc) || s->cOB.alloc(nc) || s->cGf.alloc(nc) || s->cCD.alloc((size_t)N * R))
+      return 1;
+    if (s->ncs > 0 && (s->cQB.alloc(nc) || s->cQF.alloc(nc) || s->cSCB.alloc(nc) ||
+                       s->sSFb.alloc((size_t)3 * s->nsf * C) || s->sSFm.alloc((size_t)3 * s->nsf * R) ||
+                       s->sSVb.alloc((size_t)s->ncs * C) || s->sSVm.alloc((size_t)s->ncs * R)))
       return 1;
   }
   size_t nact = (size_t)R * steps * std::max(S.n_groups, 1);

@@ -242,16 +242,20 @@ def _two_boxes(mu=0.5):
     return d, xa.shape[0]
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "mixed_adjoint"])
 def test_self_contact_batch(oracle_mod, precision):
     """Self contact between two bodies of one world: a box dropped onto a
     pinned box, three rollouts with different velocities. Forward against the
-    oracle (same contact passes and Jacobi iterations in FP64); the reverse
-    sweep is refused."""
+    oracle (same contact passes and Jacobi iterations where the forward is
+    FP64), and the reverse sweep through the self-contact groups (members,
+    stick flags and bases frozen as executed, masses and forces
+    differentiated): x0, v0 per rollout and E summed over rollouts at the FP64
+    bar, at 1e-4 with the FP32 reverse local step (mixed-adjoint); with the
+    all-FP32 local step the gradient is checked to be finite."""
     from diff_fem_amd import engine as eng
     d, na = _two_boxes()
-    prec = eng.PDM_FP64 if precision == "fp64" else eng.PDM_MIXED
-    tol = 1e-9 if precision == "fp64" else 1e-5
+    prec = {"fp64": eng.PDM_FP64, "mixed": eng.PDM_MIXED, "mixed_adjoint": eng.PDM_MIXED_ADJOINT}[precision]
+    tol = 1e-5 if precision == "mixed" else 1e-9
     R, steps = 3, 20
     s = eng.Scene(d, R=R, precision=prec, flags=0, checkpoint_every=8)
     assert s.info()["n_self_faces"] > 0
@@ -262,20 +266,68 @@ def test_self_contact_batch(oracle_mod, precision):
         v0[r, 3 * na:] = np.tile(u, d.N - na)
     xT, vT = s.forward_host(x0, v0, steps, record=True)
     passes, jac = s.contact_stats()
+    rng = np.random.default_rng(41)
+    gx = rng.standard_normal((R, 3 * d.N))
+    gv = 0.01 * rng.standard_normal((R, 3 * d.N))
+    g = s.backward_host(gx, gv)
+    gE = np.zeros(d.T)
     for r in range(R):
-        xo, vo, its = o.rollout(x0[r], v0[r], steps)
-        if precision == "fp64":
+        xo, vo, its = o.rollout(x0[r], v0[r], steps, record=True)
+        if precision != "mixed":
             assert np.array_equal(jac[r], its[:, 1]), r
         # velocities v = (x_{n+1} - x_n)/h of a box come to rest: their error is
         # measured against the rollout's speed scale (max of start and end)
         vscale = max(np.abs(vo).max(), np.abs(v0[r]).max())
         assert rel(xT[r], xo) < tol, r
-        if precision == "fp64":            # the mixed bar is stated on positions
+        if precision != "mixed":            # the mixed bar is stated on positions
             assert np.abs(vT[r] - vo).max() / vscale < tol, r
         assert xo.reshape(-1, 3)[na:, 2].min() > 0.03           # the top box rests on the bottom one
+        go, rc = o.backward(gx[r], gv[r])
+        assert rc == 0
+        gE += go["E"]
+        if precision == "mixed":
+            assert np.all(np.isfinite(g["x0"][r])) and np.all(np.isfinite(g["v0"][r]))
+            continue
+        gtol = 1e-9 if precision == "fp64" else 1e-4
+        assert rel(g["x0"][r], go["x0"]) < gtol, r
+        assert rel(g["v0"][r], go["v0"]) < gtol, r
     assert jac.sum() > 0
-    with pytest.raises(RuntimeError, match="self contact"):
-        s.backward_host(np.ones((R, 3 * d.N)), np.zeros((R, 3 * d.N)))
+    if precision != "mixed":
+        assert rel(g["E"], gE) < (1e-9 if precision == "fp64" else 1e-4)
+
+
+def test_self_contact_with_floor_gradient(oracle_mod):
+    """Self contact and floor contact on the same vertices (a box pushed into
+    another's side on a friction floor): the floor reaction reads the forces
+    plus the carried self-contact forces, the groups read the forces plus the
+    floor reactions. Two rollouts, FP64 bar on positions and gradients."""
+    from diff_fem_amd import engine as eng
+    from contact_scenes import side_push_scene
+    d, na = side_push_scene(mu=0.3, self_mu=0.2)
+    R, steps = 2, 14
+    s = eng.Scene(d, R=R, flags=0, checkpoint_every=5)
+    o = oracle_mod.OracleScene(d)
+    x0 = np.tile(d.x_rest.ravel(), (R, 1))
+    v0 = np.zeros((R, 3 * d.N))
+    v0[0, 3 * na:] = np.tile([-1.0, 0.05, -0.1], d.N - na)
+    v0[1, 3 * na:] = np.tile([-0.8, -0.05, -0.1], d.N - na)
+    xT, vT = s.forward_host(x0, v0, steps, record=True)
+    passes, jac = s.contact_stats()
+    rng = np.random.default_rng(42)
+    gx = rng.standard_normal((R, 3 * d.N))
+    gv = 0.01 * rng.standard_normal((R, 3 * d.N))
+    g = s.backward_host(gx, gv)
+    gE = np.zeros(d.T)
+    for r in range(R):
+        xo, vo, its = o.rollout(x0[r], v0[r], steps, record=True)
+        assert np.array_equal(jac[r], its[:, 1]) and np.array_equal(passes[r], its[:, 2]), r
+        assert rel(xT[r], xo) < 1e-9, r
+        go, rc = o.backward(gx[r], gv[r])
+        assert rc == 0
+        assert rel(g["x0"][r], go["x0"]) < 1e-9, r
+        assert rel(g["v0"][r], go["v0"]) < 1e-9, r
+        gE += go["E"]
+    assert rel(g["E"], gE) < 1e-9
 
 
 def test_pybind_stub_forward_backward_matches_oracle(oracle_mod):
