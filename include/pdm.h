@@ -86,8 +86,9 @@ typedef struct pdm_scene_desc {
   int n_tris;
   const double* tris;
   /* "Self-collision" (World.h:1246-1315, 1446-1600): surface vertices against
-   * the body's own surface faces, with "World Friction" self_mu. Forward
-   * only: pdm_backward refuses scenes that enable it. */
+   * the body's own surface faces, with "World Friction" self_mu. The reverse
+   * sweep holds group membership, stick/slide flags and impact normals as
+   * executed (the reference computes them from plain doubles). */
   int self_collision;
   double self_mu;
 } pdm_scene_desc;
