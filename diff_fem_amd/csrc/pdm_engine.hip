@@ -47,6 +47,9 @@ constexpr double kEps = 5e-7;  // World.h:214
 constexpr int kThreads = 256;
 constexpr int kMaxSplit = 4;  // column chains of the split solve
 constexpr int kWslItem = 6144;  // doubles of W one grouped level-kernel block stages in LDS (48 KB)
+// largest top-separator block (rows) per elimination tree solved as one dense
+// product; 0 turns the dense blocks off
+constexpr int kTopCapDefault = 1536;
 
 inline int cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
 
@@ -994,6 +997,8 @@ __global__ __launch_bounds__(1024) void k_bwd_level_g(FactorView f, const int4* 
   const int col0 = cbase + (lb % ntile) * Ct;
   bwd_group(f, it.x, it.y, it.z, col0, min(Ct, cbase + cw - col0), lg, C, B, Yb, sh);
 }
+
+#include "pdm_top.h"
 
 __global__ void k_set_int(int* __restrict__ p, int v) {
   if (blockIdx.x == 0 && threadIdx.x == 0) *p = v;
@@ -3282,6 +3287,26 @@ struct pdm_scene_s {
   long long n_solves = 0;
   double solve_ms = 0.0;
 
+  // dense separator blocks (pdm_top.h): per tree with a top, its padded
+  // offset a0, rows n and padded rows np (a multiple of 32), the offset of its
+  // np x np block; per padded top row its engine row (-1: padding) and the
+  // update rows that add into it; the elimination waves of the factor-time
+  // inverse; the product's work items (block, 32-row tile)
+  int top_cap = kTopCapDefault;
+  bool top_on = false;
+  std::vector<char> is_top;
+  int top_rows = 0, top_maxp = 0;
+  std::vector<int> top_a0, top_n, top_np;
+  std::vector<long long> top_zoff;
+  long long top_zsize = 0, top_tsize = 0;
+  std::vector<int> top_wave_ptr, top_wave_sn, top_wave_blk;
+  DevBuf<int> t_row, t_slot, t_a0, t_np, t_frow;
+  DevBuf<long long> t_zoff, t_toff;
+  DevBuf<int4> t_wave;
+  DevBuf<int2> t_items;
+  int t_nitems = 0;
+  DevBuf<double> tZ, tM, tT, TF;  // Z, Linv, elimination scratch, fronts [top_rows][C]
+
   PullView pview() const { return PullView{f_pbase.p, f_maxp.p, f_ptab.p}; }
   FactorView view() const {
     return FactorView{f_W.p, f_WR.p, f_first.p, f_rowptr.p, f_rows.p, f_lptr.p, f_child_ptr.p, f_child.p,
@@ -3380,6 +3405,9 @@ int upload_material(pdm_scene s) {
 
 // Structure of the factor (and with host_w the W blocks from the host
 // factor's L and inv(L_ss); otherwise W is only allocated).
+int plan_solve(pdm_scene s);
+int build_top_z(pdm_scene s);
+
 int upload_factor(pdm_scene s, bool host_w) {
   Factor& F = s->F;
   if (s->f_first.upload(F.first.data(), F.first.size())) return 1;
@@ -3443,6 +3471,206 @@ int upload_factor(pdm_scene s, bool host_w) {
       s->lev_rows[l] = std::max(s->lev_rows[l], ((m + 15) & ~15) + ((w + 15) & ~15));
     }
   if (old_ptr != s->lev_ptr || old_rows != s->lev_rows) s->tuned_C = -1;  // level structure changed
+  return plan_solve(s);
+}
+
+// Top-separator blocks of every elimination tree (pdm_top.h): per tree the
+// largest upward-closed set of whole heights whose supernodes own at most
+// top_cap rows. Host tables and their device copies; Z itself is formed by
+// build_top_z once W is on the device.
+int plan_top(pdm_scene s) {
+  Factor& F = s->F;
+  const int ns = (int)F.first.size() - 1;
+  const int nlev = (int)F.level_ptr.size() - 1;
+  std::vector<int> height(ns, 0), tree(ns, -1);
+  for (int l = 0; l < nlev; ++l)
+    for (int q = F.level_ptr[l]; q < F.level_ptr[l + 1]; ++q) height[F.level_sn[q]] = l;
+  for (int sn = ns - 1; sn >= 0; --sn) tree[sn] = F.parent[sn] < 0 ? sn : tree[F.parent[sn]];
+  // rows per (tree, height)
+  std::vector<int> cutof(ns, -1);  // per root: lowest height in its top set (-1: none)
+  {
+    std::vector<std::vector<long long>> wh(ns);
+    for (int sn = 0; sn < ns; ++sn) {
+      auto& v = wh[tree[sn]];
+      if ((int)v.size() <= height[sn]) v.resize(height[sn] + 1, 0);
+      v[height[sn]] += F.first[sn + 1] - F.first[sn];
+    }
+    for (int rt = 0; rt < ns; ++rt) {
+      if (F.parent[rt] >= 0 || s->top_cap <= 0) continue;
+      long long acc = 0;
+      for (int h = (int)wh[rt].size() - 1; h >= 0; --h) {
+        acc += wh[rt][h];
+        if (acc > s->top_cap) break;
+        cutof[rt] = h;
+      }
+    }
+  }
+  s->is_top.assign(ns, 0);
+  for (int sn = 0; sn < ns; ++sn) s->is_top[sn] = cutof[tree[sn]] >= 0 && height[sn] >= cutof[tree[sn]];
+  s->top_a0.clear();
+  s->top_n.clear();
+  s->top_np.clear();
+  s->top_zoff.clear();
+  std::vector<int> blk(ns, -1);  // per root: its block
+  std::vector<int> tmap(F.n, -1), plocal(ns, -1);
+  int rows = 0;
+  long long zsz = 0;
+  for (int rt = 0; rt < ns; ++rt) {
+    if (cutof[rt] < 0) continue;
+    blk[rt] = (int)s->top_a0.size();
+    s->top_a0.push_back(rows);
+    s->top_zoff.push_back(zsz);
+    s->top_n.push_back(0);
+    s->top_np.push_back(0);
+  }
+  // local rows: top supernodes of a tree in ascending (post)order, so the
+  // local numbering is ascending engine order and L restricted to it is lower
+  // triangular
+  for (int sn = 0; sn < ns; ++sn)
+    if (s->is_top[sn]) {
+      const int b = blk[tree[sn]];
+      plocal[sn] = s->top_n[b];
+      s->top_n[b] += F.first[sn + 1] - F.first[sn];
+    }
+  for (size_t b = 0; b < s->top_a0.size(); ++b) {
+    s->top_np[b] = (s->top_n[b] + 31) & ~31;
+    s->top_a0[b] = rows;
+    s->top_zoff[b] = zsz;
+    rows += s->top_np[b];
+    zsz += (long long)s->top_np[b] * s->top_np[b];
+  }
+  s->top_on = !s->top_a0.empty();
+  s->top_rows = rows;
+  s->top_zsize = zsz;
+  if (!s->top_on) return 0;
+  std::vector<int> trow(rows, -1);
+  for (int sn = 0; sn < ns; ++sn)
+    if (s->is_top[sn]) {
+      const int b = blk[tree[sn]];
+      for (int j = F.first[sn]; j < F.first[sn + 1]; ++j) {
+        const int a = s->top_a0[b] + plocal[sn] + (j - F.first[sn]);
+        tmap[j] = a;
+        trow[a] = j;
+      }
+    }
+  // fronts: the update rows of the top supernodes' children below the cut
+  std::vector<std::vector<int>> per(rows);
+  for (int sn = 0; sn < ns; ++sn) {
+    if (!s->is_top[sn]) continue;
+    for (int q = F.child_ptr[sn]; q < F.child_ptr[sn + 1]; ++q) {
+      const int ch = F.child[q];
+      if (s->is_top[ch]) continue;
+      const int wc = F.first[ch + 1] - F.first[ch];
+      const int mbc = (int)(F.rowptr[ch + 1] - F.rowptr[ch]) - wc;
+      for (int a = 0; a < mbc; ++a) {
+        const int g = tmap[F.rows[F.rowptr[ch] + wc + a]];
+        if (g < 0) {
+          g_err = "top separator plan: an update row lies outside the top rows";
+          return 1;
+        }
+        per[g].push_back((int)(F.uptr[ch] + a));
+      }
+    }
+  }
+  int mp = 0;
+  for (int a = 0; a < rows; ++a) mp = std::max(mp, (int)per[a].size());
+  s->top_maxp = mp;
+  std::vector<int> slot((size_t)rows * std::max(mp, 1), -1);
+  for (int a = 0; a < rows; ++a)
+    for (size_t k = 0; k < per[a].size(); ++k) slot[(size_t)a * mp + k] = per[a][k];
+  // elimination waves: wave k holds the k-th top supernode of every tree;
+  // per item its front rows in local numbering and its scratch offset
+  std::vector<std::vector<int>> order(s->top_a0.size());
+  for (int sn = 0; sn < ns; ++sn)
+    if (s->is_top[sn]) order[blk[tree[sn]]].push_back(sn);
+  size_t nw = 0;
+  for (auto& o : order) nw = std::max(nw, o.size());
+  std::vector<int4> wave;
+  std::vector<long long> toff;
+  std::vector<int> frow;
+  s->top_wave_ptr.assign(1, 0);
+  s->top_wave_sn.clear();
+  s->top_wave_blk.clear();
+  long long tsz = 0;
+  for (size_t k = 0; k < nw; ++k) {
+    for (size_t b = 0; b < order.size(); ++b) {
+      if (k >= order[b].size()) continue;
+      const int sn = order[b][k];
+      const int m = (int)(F.rowptr[sn + 1] - F.rowptr[sn]);
+      wave.push_back(make_int4(sn, (int)b, plocal[sn], (int)frow.size()));
+      s->top_wave_sn.push_back(sn);
+      s->top_wave_blk.push_back((int)b);
+      for (int i = 0; i < m; ++i) {
+        const int g = tmap[F.rows[F.rowptr[sn] + i]];
+        if (g < 0) {
+          g_err = "top separator plan: a front row lies outside the top rows";
+          return 1;
+        }
+        frow.push_back(g - s->top_a0[b]);
+      }
+      toff.push_back(tsz);
+      tsz += (long long)(F.first[sn + 1] - F.first[sn]) * s->top_np[b];
+    }
+    s->top_wave_ptr.push_back((int)wave.size());
+  }
+  s->top_tsize = tsz;
+  std::vector<int2> items;
+  for (size_t b = 0; b < s->top_a0.size(); ++b)
+    for (int t = 0; t < s->top_np[b] / 32; ++t) items.push_back(make_int2((int)b, t));
+  s->t_nitems = (int)items.size();
+  if (s->t_row.upload(trow.data(), trow.size()) || s->t_slot.upload(slot.data(), slot.size()) ||
+      s->t_a0.upload(s->top_a0.data(), s->top_a0.size()) || s->t_np.upload(s->top_np.data(), s->top_np.size()) ||
+      s->t_zoff.upload(s->top_zoff.data(), s->top_zoff.size()) || s->t_frow.upload(frow.data(), frow.size()) ||
+      s->t_toff.upload(toff.data(), toff.size()) || s->t_wave.upload(wave.data(), wave.size()) ||
+      s->t_items.upload(items.data(), items.size()))
+    return 1;
+  if (s->tZ.alloc(zsz) || s->tM.alloc(zsz) || s->tT.alloc(std::max<long long>(1, tsz))) return 1;
+  return 0;
+}
+
+// Z = (L_t L_t^T)^{-1} of every top block from the device factor (W): the
+// identity, the elimination waves in postorder, then the Gram product.
+int build_top_z(pdm_scene s) {
+  if (!s->top_on) return 0;
+  const hipStream_t st = s->stream;
+  const int nb = (int)s->top_a0.size();
+  for (int b = 0; b < nb; ++b) {
+    const int np = s->top_np[b];
+    hipLaunchKernelGGL(k_top_eye, dim3(cdiv((long long)np * np, 256)), dim3(256), 0, st, s->tM.p + s->top_zoff[b], np);
+  }
+  const FactorView fv = s->view();
+  const Factor& F = s->F;
+  for (size_t k = 0; k + 1 < s->top_wave_ptr.size(); ++k) {
+    const int i0 = s->top_wave_ptr[k], ni = s->top_wave_ptr[k + 1] - i0;
+    // grid: the wave's largest w * np (copy) and 16 x 16 tiles (elimination)
+    long long maxe = 0, maxt = 0;
+    for (int i = i0; i < i0 + ni; ++i) {
+      const int sn = s->top_wave_sn[i], b = s->top_wave_blk[i];
+      const int w = F.first[sn + 1] - F.first[sn];
+      const int m = (int)(F.rowptr[sn + 1] - F.rowptr[sn]);
+      maxe = std::max(maxe, (long long)w * s->top_np[b]);
+      maxt = std::max(maxt, (long long)((m + 15) / 16) * (s->top_np[b] / 16));
+    }
+    hipLaunchKernelGGL(k_top_copy, dim3(cdiv(maxe, 256), ni), dim3(256), 0, st, s->t_wave.p + i0, s->t_toff.p + i0,
+                       s->f_first.p, s->t_np.p, s->t_zoff.p, s->tM.p, s->tT.p);
+    hipLaunchKernelGGL(k_top_elim, dim3((unsigned)maxt, ni), dim3(64), 0, st, fv, s->t_wave.p + i0, s->t_toff.p + i0,
+                       s->t_frow.p, s->t_np.p, s->t_zoff.p, s->tT.p, s->tM.p);
+  }
+  for (int b = 0; b < nb; ++b) {
+    const int np = s->top_np[b];
+    hipLaunchKernelGGL(k_top_gram, dim3((np / 16) * (np / 16)), dim3(64), 0, st, s->tM.p + s->top_zoff[b],
+                       s->tZ.p + s->top_zoff[b], np);
+  }
+  HIPX(hipGetLastError());
+  return 0;
+}
+
+// Pull tables, the grouped level kernels' work items and the subtree ranges
+// (all without the top-separator supernodes when dense blocks are on).
+int plan_solve(pdm_scene s) {
+  Factor& F = s->F;
+  const int nlev = (int)F.level_ptr.size() - 1;
+  if (plan_top(s)) return 1;
   {
     // pull tables: for every front row of every supernode, the update rows of
     // its children that add into it, in child order, padded with -1 to the
@@ -3481,6 +3709,7 @@ int upload_factor(pdm_scene s, bool host_w) {
       for (int l = 0; l < nlev; ++l) {
         for (int q = F.level_ptr[l]; q < F.level_ptr[l + 1]; ++q) {
           int sn = F.level_sn[q];
+          if (s->is_top[sn]) continue;
           int m = (int)(F.rowptr[sn + 1] - F.rowptr[sn]), w = F.first[sn + 1] - F.first[sn];
           int wpad = (w + 15) & ~15, mpad = (m + 15) & ~15, nrt = wpad / 16, nbt = (m - w + 15) / 16;
           int ntl = pass == 0 ? nrt + nbt : nrt;
@@ -3521,7 +3750,8 @@ int upload_factor(pdm_scene s, bool host_w) {
     for (int c = 1; c <= nlev; ++c)
       for (int sn = 0; sn < ns; ++sn) {
         int par = F.parent[sn];
-        if (height[sn] < c && (par < 0 || height[par] >= c)) s->sub_ranges[c].push_back(make_int2(fd[sn], sn));
+        if (s->is_top[sn] || height[sn] >= c) continue;
+        if (par < 0 || height[par] >= c || s->is_top[par]) s->sub_ranges[c].push_back(make_int2(fd[sn], sn));
       }
     s->d_sub_cut = -1;
   }
@@ -3535,7 +3765,7 @@ int host_factor(pdm_scene s);
 int refactor_host(pdm_scene s) {
   auto t0 = std::chrono::steady_clock::now();
   if (host_factor(s)) return 1;
-  if (upload_material(s) || upload_factor(s, true)) return 1;
+  if (upload_material(s) || upload_factor(s, true) || build_top_z(s)) return 1;
   s->fac_plan = false;  // the structure upload above replaced the device plan's inputs
   s->t_factor = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return 0;
@@ -3781,6 +4011,8 @@ int refactor_device(pdm_scene s) {
     reg = reg_step;
   }
   F.reg_used = reg;
+  if (build_top_z(s)) return 1;
+  HIPX(hipStreamSynchronize(st));
   s->t_factor = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return 0;
 }
@@ -3913,6 +4145,9 @@ int ensure_batch_buffers(pdm_scene s) {
     return 1;
   if (s->U.alloc(std::max<long long>(1, s->F.uptr.back()) * (size_t)s->C)) return 1;
   if (s->Ys.alloc(nc)) return 1;
+  // fronts of the top blocks; 32 doubles of tail padding keep the product's
+  // reads of columns past a chain's end inside the buffer
+  if (s->top_on && s->TF.alloc((size_t)s->top_rows * s->C + 32)) return 1;
   // forward slots are FP32 only in PDM_MIXED; the FP32 reverse sweep of
   // PDM_MIXED_ADJOINT writes FP32 slots into the same (larger) buffer
   size_t esz = s->precision == PDM_MIXED ? sizeof(float) : sizeof(double);
@@ -4041,6 +4276,22 @@ int launch_level(pdm_scene s, int pass, int l, const pdm_scene_s::LevelCfg& c, d
   else
     hipLaunchKernelGGL(k_bwd_level_g, dim3(cnt * ntile), dim3(c.nthr), lds, st, fv,
                        s->f_items[1].p + s->item_ptr[1][l], ntile, lg, s->C, Bp, s->Ys.p, any, cbase, cw);
+  return 0;
+}
+
+// Dense top-separator blocks over the columns [cbase, cbase + cw): fronts,
+// then x_t = Z f_t into B (between the sweeps' level kernels).
+int launch_top(pdm_scene s, double* Bp, const int* any, int cbase, int cw, hipStream_t st) {
+  if (cw <= 0) return 0;
+  if (s->TF.n < (size_t)s->top_rows * s->C + 32) {
+    g_err = "top separator fronts not allocated for this batch";
+    return 1;
+  }
+  hipLaunchKernelGGL(k_top_gather, dim3(cdiv((long long)s->top_rows * cw, 256)), dim3(256), 0, st, s->t_row.p,
+                     s->t_slot.p, s->top_maxp, s->top_rows, Bp, s->U.p, s->TF.p, s->C, cbase, cw, any);
+  const int ncolt = cdiv(cw, 32);
+  hipLaunchKernelGGL(k_top_apply, dim3(s->t_nitems * ncolt), dim3(256), 0, st, s->tZ.p, s->t_zoff.p, s->t_a0.p,
+                     s->t_np.p, s->t_items.p, ncolt, s->TF.p, s->t_row.p, Bp, s->C, cbase, cw, any);
   return 0;
 }
 
@@ -4285,6 +4536,7 @@ int launch_solve_levels(pdm_scene s, double* Bp, const int* any) {
     if (cut > 0 && launch_subtrees(s, 0, cut, s->cfg_sub, Bp, any, cb[k], cwid[k], st)) return 1;
     for (int l = cut; l < nlev; ++l)
       if (launch_level(s, 0, l, s->cfg_fwd[l], Bp, any, cb[k], cwid[k], st)) return 1;
+    if (s->top_on && launch_top(s, Bp, any, cb[k], cwid[k], st)) return 1;
     for (int l = nlev - 1; l >= cut; --l)
       if (launch_level(s, 1, l, s->cfg_bwd[l], Bp, any, cb[k], cwid[k], st)) return 1;
     if (cut > 0 && launch_subtrees(s, 1, cut, s->cfg_sub, Bp, any, cb[k], cwid[k], st)) return 1;
@@ -5178,6 +5430,8 @@ int pdm_scene_info(pdm_scene s, pdm_info* info) {
   info->n_drag_faces = s->nfaces;
   info->n_obstacle_tris = s->ntri;
   info->n_self_faces = s->nsf;
+  info->top_rows = s->top_on ? s->top_rows : 0;
+  info->top_blocks = s->top_on ? (int)s->top_a0.size() : 0;
   info->batch = s->R;
   info->precision = s->precision;
   return 0;
@@ -5478,6 +5732,20 @@ int pdm_set_solve_plan(pdm_scene s, int subtree_cut, int n_chains) {
   HIPX(hipStreamSynchronize(s->stream));
   if (set_cut(s, subtree_cut)) return 1;
   s->nsplit = n_chains;
+  return 0;
+}
+
+int pdm_set_dense_top(pdm_scene s, int max_rows) {
+  if (!s || max_rows < 0) {
+    g_err = "pdm_set_dense_top: invalid argument";
+    return 1;
+  }
+  HIPX(hipStreamSynchronize(s->stream));
+  s->top_cap = max_rows;
+  if (plan_solve(s) || build_top_z(s)) return 1;
+  HIPX(hipStreamSynchronize(s->stream));
+  s->tuned_C = -1;  // launch shapes and subtree ranges are re-chosen on the next call
+  s->have_tape = false;
   return 0;
 }
 

@@ -251,6 +251,11 @@ class Scene:
         chains); results do not depend on it."""
         check(_lib.load().pdm_set_solve_plan(self.h, int(subtree_cut), int(n_chains)))
 
+    def set_dense_top(self, max_rows: int):
+        """Top-separator blocks solved as one dense product (rows per
+        elimination tree at most max_rows; 0: off, level sweeps throughout)."""
+        check(_lib.load().pdm_set_dense_top(self.h, int(max_rows)))
+
     def solve_timing(self):
         """(number of solves, summed ms) since the last call; needs the
         PDM_TIME_SOLVES flag (HIP events around each solve)."""
@@ -362,6 +367,10 @@ class SplitScene:
     def set_material(self, E=None, nu_fwd=None):
         for p in self.parts:
             p.set_material(E, nu_fwd)
+
+    def set_dense_top(self, max_rows: int):
+        for p in self.parts:
+            p.set_dense_top(max_rows)
 
     def forward_device(self, x0, v0, steps: int, act=None, xT=None, vT=None, record: bool = False):
         import torch

@@ -106,6 +106,8 @@ typedef struct pdm_info {
   int n_drag_faces;          /* surface faces carrying drag (0 unless underwater) */
   int n_obstacle_tris;       /* static obstacle triangles */
   int n_self_faces;          /* surface faces tested for self contact */
+  int top_rows;              /* rows of the dense top-separator blocks (padded; 0: none) */
+  int top_blocks;            /* elimination trees with a dense top-separator block */
 } pdm_info;
 
 /* Arithmetic of pdm_set_batch:
@@ -233,6 +235,16 @@ int pdm_get_solve_shapes(pdm_scene s, int cap, int* n_levels, int* ct_fwd, int* 
  * plan. No counterpart in the reference (launch tuning). */
 int pdm_set_solve_plan(pdm_scene s, int subtree_cut, int n_chains);
 
+/* Dense top-separator blocks of the solve: in every elimination tree, the
+ * supernodes of the top heights that own at most max_rows rows together (the
+ * largest such set) are solved as one dense product x_t = Z f_t with
+ * Z = (L_t L_t^T)^{-1} formed on the device at every factorisation, in place
+ * of their levels in both sweeps (the other supernodes run as before and see
+ * the same values up to rounding). max_rows = 0 turns the blocks off; the
+ * default is 1536. Re-plans the solve (launch shapes are re-chosen on the next
+ * call). No counterpart in the reference (it solves with Eigen's LDL^T). */
+int pdm_set_dense_top(pdm_scene s, int max_rows);
+
 /* Timing of the last forward/backward (ms, HIP events on the library stream). */
 int pdm_get_timing(pdm_scene s, double* fwd_ms, double* bwd_ms, double* factor_ms);
 
@@ -253,8 +265,10 @@ int pdm_get_graph_launches(pdm_scene s, long long* n);
  * working memory is one fixed array per GPU of PDM_NOALLOC_SCRATCH_BYTES,
  * held until the process ends; R is free of the scene's batch but must keep
  * (N + U_rows) * 3R * 8 B within that size (else an error). It uses the
- * default solve launch shapes (results equal the tuned solve's: shapes only
- * partition the work). Every call on one GPU shares that array, so calls
+ * default solve launch shapes and the level sweeps over all supernodes,
+ * without the dense top-separator blocks, so it agrees with pdm_solve_host to
+ * rounding.
+ * Every call on one GPU shares that array, so calls
  * that overlap in time on one GPU (other streams, other threads) race; the
  * scene's device must be current. */
 int pdm_solve_noalloc(pdm_scene s, int R, const double* b, double* x);
