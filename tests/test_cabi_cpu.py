@@ -41,8 +41,8 @@ def test_invalid_mesh_is_rejected():
 
 
 def test_host_factor_solve_matches_oracle(oracle_mod):
-    for d in (scene.config1(), scene.load_json_scene(os.path.join(os.path.dirname(__file__), "golden",
-                                                                   "bridgesus.json"), data_root="")):
+    scenes_root = os.path.join(os.path.dirname(__file__), "golden", "scenes")
+    for d in (scene.config1(), scene.load_json_scene("/data/bridgesus/bridgesus.json", scenes_root)):
         o = oracle_mod.OracleScene(d)
         b = np.random.default_rng(0).standard_normal((2, 3 * d.N))
         x = engine.host_solve(d, b)

@@ -4,6 +4,8 @@ Error measure: relative max-norm, ||a - b||_inf / ||b||_inf.
 Bars: 1e-9 for the plain FP64 path (positions and gradients), 1e-5 on
 positions and 1e-4 on gradients for the mixed path (FP32 local step).
 """
+import copy
+
 import numpy as np
 import pytest
 
@@ -60,6 +62,44 @@ def test_solve_shapes_do_not_change_results(eng):
         for chains in (1, 2, 3):
             s1.set_solve_plan(cut, chains)
             assert np.array_equal(s1.solve_host(b), x2), (cut, chains)
+
+
+@pytest.mark.parametrize("which", ["config2", "packed"])
+def test_dense_top_blocks_match_oracle(eng, oracle_mod, which):
+    """The top-separator blocks solved as one dense product per elimination
+    tree (one tree; four trees of a packed world) against the level sweeps
+    throughout (max_rows 0) and the oracle's LDL^T solve, with several block
+    sizes, and again after a material change re-forms Z on the device."""
+    d = scene.config2() if which == "config2" else packed_scene()
+    R = 3
+    s = eng.Scene(d, R=R)
+    o = oracle_mod.OracleScene(d)
+    rng = np.random.default_rng(7)
+    b = rng.standard_normal((R, 3 * d.N))
+    info = s.info()
+    assert info["top_rows"] > 0 and info["top_blocks"] == (1 if which == "config2" else 4)
+    assert info["top_rows"] % 32 == 0
+    x = s.solve_host(b)
+    for r in range(R):
+        assert rel(x[r], o.solve3(b[r])) < 1e-11
+    s.set_dense_top(0)
+    assert s.info()["top_rows"] == 0
+    x0 = s.solve_host(b)
+    assert rel(x0, x) < 1e-12
+    for cap in (200, 700, 100000):
+        s.set_dense_top(cap)
+        assert rel(s.solve_host(b), x) < 1e-12, cap
+    # new material: the device factorisation forms Z again
+    s.set_dense_top(1536)
+    E2 = d.E * rng.uniform(0.5, 2.0, d.E.shape)
+    s.set_material(E2)
+    d2 = copy.copy(d)
+    d2.E = E2
+    o2 = oracle_mod.OracleScene(d2)
+    x2 = s.solve_host(b)
+    for r in range(R):
+        assert rel(x2[r], o2.solve3(b[r])) < 1e-11
+    s.close()
 
 
 def test_force_matches_oracle(eng, oracle_mod):

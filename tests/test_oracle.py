@@ -357,7 +357,7 @@ def test_contact_rollout_keeps_body_above_ground(oracle_mod):
 
 
 def test_bridgesus_fixture_runs(oracle_mod):
-    d = scene.load_json_scene(os.path.join(GOLD, "bridgesus.json"), data_root="")
+    d = scene.load_json_scene("/data/bridgesus/bridgesus.json", os.path.join(GOLD, "scenes"))
     assert d.N == 212 and d.T == 668
     s = oracle_mod.OracleScene(d)
     x0 = d.x_rest.ravel().copy()
