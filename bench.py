@@ -123,21 +123,25 @@ def rel_rows(a, b):
     return np.abs(a - b).max(1) / np.maximum(np.abs(b).max(1), 1e-300)
 
 
-def solve_traffic(rhs, nnz_lower):
+def solve_traffic(rhs, nnz_lower, top_rows):
     """(HBM bytes per solve, source) from the committed counter summary
     (FETCH_SIZE and WRITE_SIZE runs, scaled by a copy kernel of known size),
-    when it was measured on this same factor and right-hand-side count; else
-    None. It is not measured in this run: the source names the counter run."""
+    when it was measured on this same factor, dense top block and
+    right-hand-side count; else None. It is not measured in this run: the
+    source names the counter run."""
     path = os.path.join(ROOT, "profiles", "solve_c4_R256_hbm_traffic.md")
     if not os.path.exists(path):
         return None
-    vals = {}
+    vals, section = {}, False
     for line in open(path):
-        if line.startswith("- ") and ":" in line:
+        if line.startswith("Values read by bench.py"):
+            section = True
+        elif section and line.startswith("- ") and ":" in line:
             k, v = line[2:].split(":", 1)
-            vals[k.strip()] = v.strip().split()[0]
+            if v.strip():
+                vals[k.strip()] = v.strip() if k.strip() == "measured_by" else v.strip().split()[0]
     try:
-        if int(vals["rhs"]) == rhs and int(vals["nnz_lower"]) == nnz_lower:
+        if int(vals["rhs"]) == rhs and int(vals["nnz_lower"]) == nnz_lower and int(vals["top_rows"]) == top_rows:
             return float(vals["hbm_bytes_per_solve"]), "profiles/solve_c4_R256_hbm_traffic.md (%s)" % (
                 vals.get("measured_by", "rocprofv3 FETCH_SIZE/WRITE_SIZE counter runs"))
     except (KeyError, ValueError):
@@ -328,7 +332,7 @@ def main():
     else:  # few right-hand sides: the factor stream bounds it
         achieved = bytes_alg / (ms_live * 1e-3) / 1e9
         rl = {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s", "frac": achieved / 8000.0}
-    tr = solve_traffic(3 * Rp, info["nnz_lower"])
+    tr = solve_traffic(3 * Rp, info["nnz_lower"], info["top_rows"])
     rl.update({"traffic": tr[0] if tr else None, "traffic_source": tr[1] if tr else None,
                "kernel": "supernodal solve k_fwd_level+k_bwd_level, %d RHS" % (3 * Rp),
                "column_chains": p0.solve_chains,
