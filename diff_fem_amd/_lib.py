@@ -51,6 +51,7 @@ class InfoC(C.Structure):
         ("nnz_factor", C.c_longlong), ("nnz_lower", C.c_longlong), ("factor_flops", C.c_double), ("reg_used", C.c_double),
         ("n_contact_vertices", C.c_int), ("batch", C.c_int), ("precision", C.c_int), ("n_drag_faces", C.c_int),
         ("n_obstacle_tris", C.c_int), ("n_self_faces", C.c_int), ("top_rows", C.c_int), ("top_blocks", C.c_int),
+        ("leaf_blocks", C.c_int),
     ]
 
 
@@ -61,7 +62,7 @@ SYMBOLS = [
     "pdm_backward_host", "pdm_get_iterations", "pdm_solve_host", "pdm_force_host", "pdm_get_timing",
     "pdm_host_analyse", "pdm_host_solve", "pdm_get_contact_stats", "pdm_time_solve", "pdm_get_solve_timing",
     "pdm_get_solve_shapes", "pdm_set_solve_plan", "pdm_solve_noalloc", "pdm_get_graph_launches",
-    "pdm_refactor", "pdm_get_factor_w", "pdm_set_dense_top",
+    "pdm_refactor", "pdm_get_factor_w", "pdm_set_dense_top", "pdm_set_dense_leaves",
 ]
 
 # PDM_NOALLOC_SCRATCH_BYTES of include/pdm.h
@@ -94,6 +95,7 @@ def load():
     L.pdm_refactor.argtypes = [vp, C.c_int]
     L.pdm_get_factor_w.argtypes = [vp, vp, C.c_longlong]
     L.pdm_set_dense_top.argtypes = [vp, C.c_int]
+    L.pdm_set_dense_leaves.argtypes = [vp, C.c_int]
     L.pdm_set_batch.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
     for name in ("pdm_forward", "pdm_forward_host"):
         getattr(L, name).argtypes = [vp, C.c_int, vp, vp, vp, vp, vp, C.c_int]

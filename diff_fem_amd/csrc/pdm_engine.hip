@@ -50,6 +50,9 @@ constexpr int kWslItem = 6144;  // doubles of W one grouped level-kernel block s
 // largest top-separator block (rows) per elimination tree solved as one dense
 // product; 0 turns the dense blocks off
 constexpr int kTopCapDefault = 1536;
+// largest leaf subtree (rows) below the top separators solved as dense
+// products; 0 turns the dense leaf blocks off
+constexpr int kLeafCapDefault = 512;
 
 inline int cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
 
@@ -628,7 +631,8 @@ __global__ __launch_bounds__(1024) void k_fwd_level(FactorView f, const int* __r
 // and are read back by the same block.
 __global__ __launch_bounds__(1024) void k_fwd_subtree(FactorView f, const int2* __restrict__ sub, int ntile, int lg,
                                                      int C, double* __restrict__ B, double* __restrict__ U,
-                                                     const int* __restrict__ any, int cbase, int cw) {
+                                                     const int* __restrict__ any, int cbase, int cw,
+                                                     const char* __restrict__ skip) {
   if (any && *any == 0) return;
   extern __shared__ double sh[];
   const int Ct = 1 << lg;
@@ -637,6 +641,7 @@ __global__ __launch_bounds__(1024) void k_fwd_subtree(FactorView f, const int2* 
   const int col0 = cbase + (lb % ntile) * Ct;
   const int ct = min(Ct, cbase + cw - col0);
   for (int s = rg.x; s <= rg.y; ++s) {
+    if (skip && skip[s]) continue;  // a dense leaf (its root's update block is already in U)
     fwd_supernode(f, s, col0, ct, lg, C, B, U, sh);
     __syncthreads();
   }
@@ -700,7 +705,7 @@ __global__ __launch_bounds__(1024) void k_bwd_level(FactorView f, const int* __r
 // postorder), after the levels above them.
 __global__ __launch_bounds__(1024) void k_bwd_subtree(FactorView f, const int2* __restrict__ sub, int ntile, int lg,
                                                      int C, double* __restrict__ B, const int* __restrict__ any,
-                                                     int cbase, int cw) {
+                                                     int cbase, int cw, const char* __restrict__ skip) {
   if (any && *any == 0) return;
   extern __shared__ double sh[];
   const int Ct = 1 << lg;
@@ -709,6 +714,7 @@ __global__ __launch_bounds__(1024) void k_bwd_subtree(FactorView f, const int2* 
   const int col0 = cbase + (lb % ntile) * Ct;
   const int ct = min(Ct, cbase + cw - col0);
   for (int s = rg.y; s >= rg.x; --s) {
+    if (skip && skip[s]) continue;  // a dense leaf (solved after the sweeps)
     bwd_supernode(f, s, col0, ct, lg, C, B, sh);
     __syncthreads();
   }
@@ -3306,6 +3312,27 @@ struct pdm_scene_s {
   DevBuf<int2> t_items;
   int t_nitems = 0;
   DevBuf<double> tZ, tM, tT, TF;  // Z, Linv, elimination scratch, fronts [top_rows][C]
+  // dense leaf subtrees (pdm_top.h): per leaf its root, own rows (j0, nsub,
+  // padded np), above rows (nab, padded nabp), offsets of its operators and
+  // of its above rows in XA; the elimination waves; the products' work items
+  int leaf_cap = kLeafCapDefault;
+  bool leaf_on = false;
+  std::vector<char> is_leaf;
+  std::vector<int> leaf_np, leaf_nabp, leaf_nsub, leaf_nab, leaf_j0;
+  std::vector<long long> leaf_moff, leaf_poff, leaf_zoff, leaf_uoff, leaf_xaoff;
+  long long leaf_msize = 0, leaf_psize = 0, leaf_zsize = 0, leaf_tsize = 0;
+  int leaf_xrows = 0;
+  std::vector<int> leaf_wave_ptr, leaf_wave_sn, leaf_wave_blk;
+  DevBuf<int> l_np, l_nabp, l_nsub, l_nab, l_j0, l_frow, l_xrow;
+  DevBuf<long long> l_moff, l_poff, l_uoff, l_xaoff, l_toff;
+  DevBuf<int4> l_wave;
+  DevBuf<int2> l_items[2];
+  int l_nitems[2] = {0, 0};
+  DevBuf<char> d_skip;             // per supernode: inside a dense leaf (the subtree kernels skip it)
+  DevBuf<double> lM, lZ, lP, lT, XA;  // [Linv; G], Zs, forward operators, scratch, above rows [xrows][C]
+  LeafView lview() const {
+    return LeafView{lP.p, lM.p, l_poff.p, l_moff.p, l_np.p, l_nabp.p, l_nsub.p, l_nab.p, l_j0.p, l_uoff.p, l_xaoff.p};
+  }
 
   PullView pview() const { return PullView{f_pbase.p, f_maxp.p, f_ptab.p}; }
   FactorView view() const {
@@ -3407,6 +3434,7 @@ int upload_material(pdm_scene s) {
 // factor's L and inv(L_ss); otherwise W is only allocated).
 int plan_solve(pdm_scene s);
 int build_top_z(pdm_scene s);
+int build_leaf_ops(pdm_scene s);
 
 int upload_factor(pdm_scene s, bool host_w) {
   Factor& F = s->F;
@@ -3636,7 +3664,8 @@ int build_top_z(pdm_scene s) {
   const int nb = (int)s->top_a0.size();
   for (int b = 0; b < nb; ++b) {
     const int np = s->top_np[b];
-    hipLaunchKernelGGL(k_top_eye, dim3(cdiv((long long)np * np, 256)), dim3(256), 0, st, s->tM.p + s->top_zoff[b], np);
+    hipLaunchKernelGGL(k_top_eye, dim3(cdiv((long long)np * np, 256)), dim3(256), 0, st, s->tM.p + s->top_zoff[b], np,
+                       np);
   }
   const FactorView fv = s->view();
   const Factor& F = s->F;
@@ -3665,12 +3694,182 @@ int build_top_z(pdm_scene s) {
   return 0;
 }
 
+// The dense leaf operators from the device factor (W): M = [I; 0] through
+// each leaf's elimination steps (postorder) gives [Linv; G]; the Gram product
+// of Linv gives Zs = K_s^{-1}; the forward operator is packed as [Zs | G^T].
+int build_leaf_ops(pdm_scene s) {
+  if (!s->leaf_on) return 0;
+  const hipStream_t st = s->stream;
+  const int nl = (int)s->leaf_np.size();
+  for (int l = 0; l < nl; ++l) {
+    const int np = s->leaf_np[l], rows = np + s->leaf_nabp[l];
+    hipLaunchKernelGGL(k_top_eye, dim3(cdiv((long long)rows * np, 256)), dim3(256), 0, st, s->lM.p + s->leaf_moff[l],
+                       np, rows);
+  }
+  const FactorView fv = s->view();
+  const Factor& F = s->F;
+  for (size_t k = 0; k + 1 < s->leaf_wave_ptr.size(); ++k) {
+    const int i0 = s->leaf_wave_ptr[k], ni = s->leaf_wave_ptr[k + 1] - i0;
+    long long maxe = 0, maxt = 0;
+    for (int i = i0; i < i0 + ni; ++i) {
+      const int sn = s->leaf_wave_sn[i], l = s->leaf_wave_blk[i];
+      const int w = F.first[sn + 1] - F.first[sn];
+      const int m = (int)(F.rowptr[sn + 1] - F.rowptr[sn]);
+      maxe = std::max(maxe, (long long)w * s->leaf_np[l]);
+      maxt = std::max(maxt, (long long)((m + 15) / 16) * (s->leaf_np[l] / 16));
+    }
+    if (ni > 65535) {
+      g_err = "leaf plan: more than 65535 leaves in one elimination wave";
+      return 1;
+    }
+    hipLaunchKernelGGL(k_top_copy, dim3(cdiv(maxe, 256), ni), dim3(256), 0, st, s->l_wave.p + i0, s->l_toff.p + i0,
+                       s->f_first.p, s->l_np.p, s->l_moff.p, s->lM.p, s->lT.p);
+    hipLaunchKernelGGL(k_top_elim, dim3((unsigned)maxt, ni), dim3(64), 0, st, fv, s->l_wave.p + i0, s->l_toff.p + i0,
+                       s->l_frow.p, s->l_np.p, s->l_moff.p, s->lT.p, s->lM.p);
+  }
+  for (int l = 0; l < nl; ++l) {
+    const int np = s->leaf_np[l], nabp = s->leaf_nabp[l];
+    hipLaunchKernelGGL(k_top_gram, dim3((np / 16) * (np / 16)), dim3(64), 0, st, s->lM.p + s->leaf_moff[l],
+                       s->lZ.p + s->leaf_zoff[l], np);
+    hipLaunchKernelGGL(k_leaf_pack, dim3(cdiv((long long)np * (np + nabp), 256)), dim3(256), 0, st,
+                       s->lZ.p + s->leaf_zoff[l], s->lM.p + s->leaf_moff[l], s->lP.p + s->leaf_poff[l], np, nabp);
+  }
+  HIPX(hipGetLastError());
+  return 0;
+}
+
+// Dense leaf subtrees (pdm_top.h): the maximal subtrees below the top
+// separators whose supernodes own at most leaf_cap rows together; host tables
+// and their device copies (the operators are formed by build_top_z).
+int plan_leaf(pdm_scene s) {
+  Factor& F = s->F;
+  const int ns = (int)F.first.size() - 1;
+  std::vector<long long> rows(ns, 0);
+  std::vector<int> fd(ns);
+  for (int sn = 0; sn < ns; ++sn) {
+    rows[sn] += F.first[sn + 1] - F.first[sn];
+    fd[sn] = sn;
+    for (int q = F.child_ptr[sn]; q < F.child_ptr[sn + 1]; ++q) fd[sn] = std::min(fd[sn], fd[F.child[q]]);
+    if (F.parent[sn] >= 0) rows[F.parent[sn]] += rows[sn];
+  }
+  s->is_leaf.assign(ns, 0);
+  s->leaf_np.clear(); s->leaf_nabp.clear(); s->leaf_nsub.clear(); s->leaf_nab.clear(); s->leaf_j0.clear();
+  s->leaf_moff.clear(); s->leaf_poff.clear(); s->leaf_zoff.clear(); s->leaf_uoff.clear(); s->leaf_xaoff.clear();
+  s->leaf_wave_ptr.assign(1, 0);
+  s->leaf_wave_sn.clear();
+  s->leaf_wave_blk.clear();
+  std::vector<int> roots;
+  if (s->leaf_cap > 0)
+    for (int sn = 0; sn < ns; ++sn) {
+      if (s->is_top[sn] || rows[sn] > s->leaf_cap) continue;
+      const int par = F.parent[sn];
+      if (par >= 0 && !s->is_top[par] && rows[par] <= s->leaf_cap) continue;  // inside a larger leaf
+      roots.push_back(sn);
+    }
+  long long msz = 0, psz = 0, zsz = 0, tsz = 0;
+  int xrows = 0;
+  std::vector<int> xrow, frow;
+  std::vector<int4> wave;
+  std::vector<long long> toff;
+  std::vector<std::vector<int>> order(roots.size());
+  for (size_t l = 0; l < roots.size(); ++l) {
+    const int rt = roots[l], a0 = fd[rt];
+    const int j0 = F.first[a0], nsub = F.first[rt + 1] - j0;
+    const int wr = F.first[rt + 1] - F.first[rt];
+    const int nab = (int)(F.rowptr[rt + 1] - F.rowptr[rt]) - wr;
+    const int np = (nsub + 31) & ~31, nabp = (nab + 31) & ~31;
+    s->leaf_np.push_back(np);
+    s->leaf_nabp.push_back(nabp);
+    s->leaf_nsub.push_back(nsub);
+    s->leaf_nab.push_back(nab);
+    s->leaf_j0.push_back(j0);
+    s->leaf_moff.push_back(msz);
+    s->leaf_poff.push_back(psz);
+    s->leaf_zoff.push_back(zsz);
+    s->leaf_uoff.push_back(F.uptr[rt]);
+    s->leaf_xaoff.push_back(xrows);
+    msz += (long long)(np + nabp) * np;
+    psz += (long long)np * (np + nabp);
+    zsz += (long long)np * np;
+    for (int i = 0; i < nabp; ++i) xrow.push_back(i < nab ? F.rows[F.rowptr[rt] + wr + i] : -1);
+    xrows += nabp;
+    for (int sn = a0; sn <= rt; ++sn) {
+      s->is_leaf[sn] = 1;
+      order[l].push_back(sn);
+    }
+  }
+  s->leaf_on = !roots.empty();
+  s->leaf_msize = msz;
+  s->leaf_psize = psz;
+  s->leaf_zsize = zsz;
+  s->leaf_xrows = xrows;
+  if (!s->leaf_on) return 0;
+  // elimination waves: wave k holds the k-th supernode (postorder) of every
+  // leaf; per item its front rows in the leaf's local numbering (own rows
+  // first, then the above rows after the padded own rows)
+  size_t nw = 0;
+  for (auto& o : order) nw = std::max(nw, o.size());
+  for (size_t k = 0; k < nw; ++k) {
+    for (size_t l = 0; l < order.size(); ++l) {
+      if (k >= order[l].size()) continue;
+      const int sn = order[l][k], rt = roots[l];
+      const int j0 = s->leaf_j0[l], nsub = s->leaf_nsub[l], np = s->leaf_np[l];
+      const int wr = F.first[rt + 1] - F.first[rt];
+      const int m = (int)(F.rowptr[sn + 1] - F.rowptr[sn]);
+      wave.push_back(make_int4(sn, (int)l, F.first[sn] - j0, (int)frow.size()));
+      s->leaf_wave_sn.push_back(sn);
+      s->leaf_wave_blk.push_back((int)l);
+      for (int i = 0; i < m; ++i) {
+        const int row = F.rows[F.rowptr[sn] + i];
+        int lr;
+        if (row >= j0 && row < j0 + nsub) {
+          lr = row - j0;
+        } else {
+          const int* b = F.rows.data() + F.rowptr[rt] + wr;
+          const int* e = b + s->leaf_nab[l];
+          const int* p = std::lower_bound(b, e, row);
+          if (p == e || *p != row) {
+            g_err = "leaf plan: a front row lies outside the leaf and its root's rows";
+            return 1;
+          }
+          lr = np + (int)(p - b);
+        }
+        frow.push_back(lr);
+      }
+      toff.push_back(tsz);
+      tsz += (long long)(F.first[sn + 1] - F.first[sn]) * np;
+    }
+    s->leaf_wave_ptr.push_back((int)wave.size());
+  }
+  s->leaf_tsize = tsz;
+  std::vector<int2> items[2];
+  for (size_t l = 0; l < roots.size(); ++l) {
+    for (int t = 0; t < (s->leaf_np[l] + s->leaf_nabp[l]) / 32; ++t) items[0].push_back(make_int2((int)l, t));
+    for (int t = 0; t < s->leaf_np[l] / 32; ++t) items[1].push_back(make_int2((int)l, t));
+  }
+  s->l_nitems[0] = (int)items[0].size();
+  s->l_nitems[1] = (int)items[1].size();
+  std::vector<char> skip(s->is_leaf.begin(), s->is_leaf.end());
+  if (s->l_np.upload(s->leaf_np.data(), s->leaf_np.size()) || s->l_nabp.upload(s->leaf_nabp.data(), s->leaf_nabp.size()) ||
+      s->l_nsub.upload(s->leaf_nsub.data(), s->leaf_nsub.size()) || s->l_nab.upload(s->leaf_nab.data(), s->leaf_nab.size()) ||
+      s->l_j0.upload(s->leaf_j0.data(), s->leaf_j0.size()) || s->l_moff.upload(s->leaf_moff.data(), s->leaf_moff.size()) ||
+      s->l_poff.upload(s->leaf_poff.data(), s->leaf_poff.size()) || s->l_uoff.upload(s->leaf_uoff.data(), s->leaf_uoff.size()) ||
+      s->l_xaoff.upload(s->leaf_xaoff.data(), s->leaf_xaoff.size()) || s->l_frow.upload(frow.data(), frow.size()) ||
+      s->l_toff.upload(toff.data(), toff.size()) || s->l_wave.upload(wave.data(), wave.size()) ||
+      s->l_xrow.upload(xrow.data(), xrow.size()) || s->l_items[0].upload(items[0].data(), items[0].size()) ||
+      s->l_items[1].upload(items[1].data(), items[1].size()) || s->d_skip.upload(skip.data(), skip.size()))
+    return 1;
+  if (s->lM.alloc(msz) || s->lZ.alloc(zsz) || s->lP.alloc(psz) || s->lT.alloc(std::max<long long>(1, tsz))) return 1;
+  return 0;
+}
+
 // Pull tables, the grouped level kernels' work items and the subtree ranges
-// (all without the top-separator supernodes when dense blocks are on).
+// (all without the top-separator and dense-leaf supernodes when dense blocks
+// are on).
 int plan_solve(pdm_scene s) {
   Factor& F = s->F;
   const int nlev = (int)F.level_ptr.size() - 1;
-  if (plan_top(s)) return 1;
+  if (plan_top(s) || plan_leaf(s)) return 1;
   {
     // pull tables: for every front row of every supernode, the update rows of
     // its children that add into it, in child order, padded with -1 to the
@@ -3709,7 +3908,7 @@ int plan_solve(pdm_scene s) {
       for (int l = 0; l < nlev; ++l) {
         for (int q = F.level_ptr[l]; q < F.level_ptr[l + 1]; ++q) {
           int sn = F.level_sn[q];
-          if (s->is_top[sn]) continue;
+          if (s->is_top[sn] || s->is_leaf[sn]) continue;
           int m = (int)(F.rowptr[sn + 1] - F.rowptr[sn]), w = F.first[sn + 1] - F.first[sn];
           int wpad = (w + 15) & ~15, mpad = (m + 15) & ~15, nrt = wpad / 16, nbt = (m - w + 15) / 16;
           int ntl = pass == 0 ? nrt + nbt : nrt;
@@ -3750,7 +3949,7 @@ int plan_solve(pdm_scene s) {
     for (int c = 1; c <= nlev; ++c)
       for (int sn = 0; sn < ns; ++sn) {
         int par = F.parent[sn];
-        if (s->is_top[sn] || height[sn] >= c) continue;
+        if (s->is_top[sn] || s->is_leaf[sn] || height[sn] >= c) continue;
         if (par < 0 || height[par] >= c || s->is_top[par]) s->sub_ranges[c].push_back(make_int2(fd[sn], sn));
       }
     s->d_sub_cut = -1;
@@ -3765,7 +3964,7 @@ int host_factor(pdm_scene s);
 int refactor_host(pdm_scene s) {
   auto t0 = std::chrono::steady_clock::now();
   if (host_factor(s)) return 1;
-  if (upload_material(s) || upload_factor(s, true) || build_top_z(s)) return 1;
+  if (upload_material(s) || upload_factor(s, true) || build_top_z(s) || build_leaf_ops(s)) return 1;
   s->fac_plan = false;  // the structure upload above replaced the device plan's inputs
   s->t_factor = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return 0;
@@ -4011,7 +4210,7 @@ int refactor_device(pdm_scene s) {
     reg = reg_step;
   }
   F.reg_used = reg;
-  if (build_top_z(s)) return 1;
+  if (build_top_z(s) || build_leaf_ops(s)) return 1;
   HIPX(hipStreamSynchronize(st));
   s->t_factor = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return 0;
@@ -4148,6 +4347,7 @@ int ensure_batch_buffers(pdm_scene s) {
   // fronts of the top blocks; 32 doubles of tail padding keep the product's
   // reads of columns past a chain's end inside the buffer
   if (s->top_on && s->TF.alloc((size_t)s->top_rows * s->C + 32)) return 1;
+  if (s->leaf_on && s->XA.alloc((size_t)s->leaf_xrows * s->C + 32)) return 1;
   // forward slots are FP32 only in PDM_MIXED; the FP32 reverse sweep of
   // PDM_MIXED_ADJOINT writes FP32 slots into the same (larger) buffer
   size_t esz = s->precision == PDM_MIXED ? sizeof(float) : sizeof(double);
@@ -4295,6 +4495,33 @@ int launch_top(pdm_scene s, double* Bp, const int* any, int cbase, int cw, hipSt
   return 0;
 }
 
+// Dense leaf subtrees over the columns [cbase, cbase + cw): pass 0 before the
+// forward sweep (z into Ys, the roots' update blocks into U), pass 1 after the
+// backward sweep (the above rows compacted, then x = z + G^T x_above into B).
+int launch_leaf(pdm_scene s, int pass, double* Bp, const int* any, int cbase, int cw, hipStream_t st) {
+  if (cw <= 0) return 0;
+  const int ncolt = cdiv(cw, 32);
+  const LeafView lv = s->lview();
+  if (pass == 0) {
+    if (s->l_nitems[0] > 0)
+      hipLaunchKernelGGL(k_leaf_apply<true>, dim3(s->l_nitems[0] * ncolt), dim3(256), 0, st, lv, s->l_items[0].p,
+                         ncolt, Bp, s->XA.p, s->Ys.p, s->U.p, Bp, s->C, cbase, cw, any);
+    return 0;
+  }
+  if (s->XA.n < (size_t)s->leaf_xrows * s->C + 32) {
+    g_err = "leaf above rows not allocated for this batch";
+    return 1;
+  }
+  // (leaves that are whole trees have no rows above them)
+  if (s->leaf_xrows > 0)
+    hipLaunchKernelGGL(k_leaf_gather, dim3(cdiv((long long)s->leaf_xrows * cw, 256)), dim3(256), 0, st, s->l_xrow.p,
+                       s->leaf_xrows, Bp, s->XA.p, s->C, cbase, cw, any);
+  if (s->l_nitems[1] > 0)
+    hipLaunchKernelGGL(k_leaf_apply<false>, dim3(s->l_nitems[1] * ncolt), dim3(256), 0, st, lv, s->l_items[1].p,
+                       ncolt, Bp, s->XA.p, s->Ys.p, s->U.p, Bp, s->C, cbase, cw, any);
+  return 0;
+}
+
 // Subtree kernels of cut level `cut` over the columns [cbase, cbase + cw).
 int launch_subtrees(pdm_scene s, int pass, int cut, const pdm_scene_s::LevelCfg& c, double* Bp, const int* any,
                     int cbase, int cw, hipStream_t st) {
@@ -4313,10 +4540,10 @@ int launch_subtrees(pdm_scene s, int pass, int cut, const pdm_scene_s::LevelCfg&
   if (nsub == 0) return 0;
   if (pass == 0)
     hipLaunchKernelGGL(k_fwd_subtree, dim3(nsub * ntile), dim3(c.nthr), lds, st, fv, s->d_sub.p, ntile, lg, s->C, Bp,
-                       s->U.p, any, cbase, cw);
+                       s->U.p, any, cbase, cw, s->leaf_on ? s->d_skip.p : nullptr);
   else
     hipLaunchKernelGGL(k_bwd_subtree, dim3(nsub * ntile), dim3(c.nthr), lds, st, fv, s->d_sub.p, ntile, lg, s->C, Bp,
-                       any, cbase, cw);
+                       any, cbase, cw, s->leaf_on ? s->d_skip.p : nullptr);
   return 0;
 }
 
@@ -4533,6 +4760,7 @@ int launch_solve_levels(pdm_scene s, double* Bp, const int* any) {
   const int cut = s->cut;
   for (int k = 0; k < ns; ++k) {
     hipStream_t st = k == 0 ? s->stream : s->aux[k - 1];
+    if (s->leaf_on && launch_leaf(s, 0, Bp, any, cb[k], cwid[k], st)) return 1;
     if (cut > 0 && launch_subtrees(s, 0, cut, s->cfg_sub, Bp, any, cb[k], cwid[k], st)) return 1;
     for (int l = cut; l < nlev; ++l)
       if (launch_level(s, 0, l, s->cfg_fwd[l], Bp, any, cb[k], cwid[k], st)) return 1;
@@ -4540,6 +4768,7 @@ int launch_solve_levels(pdm_scene s, double* Bp, const int* any) {
     for (int l = nlev - 1; l >= cut; --l)
       if (launch_level(s, 1, l, s->cfg_bwd[l], Bp, any, cb[k], cwid[k], st)) return 1;
     if (cut > 0 && launch_subtrees(s, 1, cut, s->cfg_sub, Bp, any, cb[k], cwid[k], st)) return 1;
+    if (s->leaf_on && launch_leaf(s, 1, Bp, any, cb[k], cwid[k], st)) return 1;
   }
   if (ns > 1)
     for (int k = 1; k < ns; ++k) {
@@ -5432,6 +5661,7 @@ int pdm_scene_info(pdm_scene s, pdm_info* info) {
   info->n_self_faces = s->nsf;
   info->top_rows = s->top_on ? s->top_rows : 0;
   info->top_blocks = s->top_on ? (int)s->top_a0.size() : 0;
+  info->leaf_blocks = s->leaf_on ? (int)s->leaf_np.size() : 0;
   info->batch = s->R;
   info->precision = s->precision;
   return 0;
@@ -5742,9 +5972,23 @@ int pdm_set_dense_top(pdm_scene s, int max_rows) {
   }
   HIPX(hipStreamSynchronize(s->stream));
   s->top_cap = max_rows;
-  if (plan_solve(s) || build_top_z(s)) return 1;
+  if (plan_solve(s) || build_top_z(s) || build_leaf_ops(s)) return 1;
   HIPX(hipStreamSynchronize(s->stream));
   s->tuned_C = -1;  // launch shapes and subtree ranges are re-chosen on the next call
+  s->have_tape = false;
+  return 0;
+}
+
+int pdm_set_dense_leaves(pdm_scene s, int max_rows) {
+  if (!s || max_rows < 0) {
+    g_err = "pdm_set_dense_leaves: invalid argument";
+    return 1;
+  }
+  HIPX(hipStreamSynchronize(s->stream));
+  s->leaf_cap = max_rows;
+  if (plan_solve(s) || build_top_z(s) || build_leaf_ops(s)) return 1;
+  HIPX(hipStreamSynchronize(s->stream));
+  s->tuned_C = -1;
   s->have_tape = false;
   return 0;
 }

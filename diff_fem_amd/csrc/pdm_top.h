@@ -17,10 +17,10 @@
 // then the Gram product. The block of tree r is np_r x np_r (n_r padded to a
 // multiple of 32, zero padding), row-major.
 
-// Identity of one np x np block.
-__global__ void k_top_eye(double* __restrict__ M, int np) {
+// Identity of one block of np columns, zero in the rows past np (rows x np).
+__global__ void k_top_eye(double* __restrict__ M, int np, int rows) {
   const long long e = blockIdx.x * 256LL + threadIdx.x;
-  if (e >= (long long)np * np) return;
+  if (e >= (long long)rows * np) return;
   M[e] = (e / np == e % np) ? 1.0 : 0.0;
 }
 
@@ -240,5 +240,144 @@ __global__ __launch_bounds__(256) void k_top_apply(const double* __restrict__ Z,
     const int col = c0 + cc;
     const int er = trow[a0 + row0 + rr];
     if (er >= 0 && col < cend) B[(long long)er * C + col] = v;
+  }
+}
+
+// ----------------------------------------------------------------------------
+// Dense leaf subtrees. A leaf subtree (a subtree below the top separators
+// whose supernodes own at most a capped number of rows, nsub, in the
+// contiguous engine rows [j0, j0 + nsub)) touches only its own rows and the
+// nab rows of its root's update block ("above" rows). With L_s its factor
+// block, Linv = inv(L_s) and G = -L_{above,s} Linv (what its elimination does
+// to the above rows), both sweeps over the subtree reduce to
+//   forward:  z = K_s^{-1} b_s (K_s = L_s L_s^T), u = G b_s   (u: the root's
+//             update block, for the parent's front);
+//   backward: x_s = z + G^T x_above.
+// Formed at factorisation like the top blocks: M = [I; 0] ((np + nabp) x np,
+// the above rows after the padded own rows) through the subtree's elimination
+// steps gives [Linv; G]; the Gram product gives Zs = Linv^T Linv; the forward
+// operator is stored k-major as P[k][a] = [Zs | G^T] (np x (np + nabp)), the
+// backward one is G itself (rows np .. np + nabp of M, k-major already).
+struct LeafView {
+  const double* P;           // forward operators (k-major)
+  const double* M;           // [Linv; G] per leaf (the backward reads its G rows)
+  const long long* poff;     // per leaf: offset of P
+  const long long* moff;     // per leaf: offset of M
+  const int* np_of;          // padded own rows (multiple of 32)
+  const int* nabp_of;        // padded above rows (multiple of 32)
+  const int* nsub_of;        // own rows
+  const int* nab_of;         // above rows
+  const int* j0_of;          // first own engine row
+  const long long* uoff;     // per leaf: first update-block row of its root
+  const long long* xaoff;    // per leaf: first row of its above rows in XA
+};
+
+// P[k][a] of one leaf: a < np: Zs[k][a]; a >= np: G[a - np][k] (= M[np + a - np][k]).
+__global__ void k_leaf_pack(const double* __restrict__ Zs, const double* __restrict__ M, double* __restrict__ P,
+                            int np, int nabp) {
+  const int w = np + nabp;
+  const long long e = blockIdx.x * 256LL + threadIdx.x;
+  if (e >= (long long)np * w) return;
+  const int k = (int)(e / w), a = (int)(e % w);
+  P[e] = a < np ? Zs[(long long)k * np + a] : M[(long long)(np + a - np) * np + k];
+}
+
+// The above rows of every leaf, compacted: XA[xaoff + i][c] = B[rows[i]][c]
+// (rows -1: padding, zero), columns [cbase, cbase + cw).
+__global__ __launch_bounds__(256) void k_leaf_gather(const int* __restrict__ xrow, int nrows,
+                                                     const double* __restrict__ B, double* __restrict__ XA, int C,
+                                                     int cbase, int cw, const int* __restrict__ any) {
+  if (any && *any == 0) return;
+  const long long idx = (long long)xcd_block() * 256 + threadIdx.x;
+  if (idx >= (long long)nrows * cw) return;
+  const int a = (int)(idx / cw), c = cbase + (int)(idx % cw);
+  const int r = xrow[a];
+  XA[(long long)a * C + c] = r >= 0 ? B[(long long)r * C + c] : 0.0;
+}
+
+// Leaf products. FWD: out = P^T b (K = np; b rows j0 + k, clamped to the own
+// rows: P's padded rows are zero); rows a < nsub -> Yb (z), np <= a < np + nab
+// -> U at the root's update block (u). !FWD: out = z + G^T xa (K = nabp; xa
+// from XA, zero padded), rows a < nsub -> B (x). Block = (leaf, 32 output rows,
+// 32 columns), four waves each a quarter of k, quarters added in a fixed order
+// (as k_top_apply).
+template <bool FWD>
+__global__ __launch_bounds__(256) void k_leaf_apply(LeafView L, const int2* __restrict__ items, int ncolt,
+                                                    const double* __restrict__ Bsrc, const double* __restrict__ XA,
+                                                    double* __restrict__ Yb, double* __restrict__ U,
+                                                    double* __restrict__ B, int C, int cbase, int cw,
+                                                    const int* __restrict__ any) {
+  if (any && *any == 0) return;
+  __shared__ double red[4][16 * 64];
+  const int lb = xcd_block();
+  const int2 it = items[lb / ncolt];
+  const int l = it.x, row0 = 32 * it.y;
+  const int c0 = cbase + 32 * (lb % ncolt);
+  const int np = L.np_of[l], nabp = L.nabp_of[l], nsub = L.nsub_of[l], j0 = L.j0_of[l];
+  const int K = FWD ? np : nabp;
+  const int lda = FWD ? np + nabp : np;
+  const double* A = FWD ? L.P + L.poff[l] : L.M + L.moff[l] + (long long)np * np;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, lr = lane & 15, lk = lane >> 4;
+  const int kq = K >> 2;  // K is a multiple of 32
+  const int kb = wave * kq, ke = kb + kq;
+  const int cend = cbase + cw;
+  // columns past the end read a valid column (their outputs are not written)
+  const int colA = min(c0 + lr, C - 1), colB = min(c0 + 16 + lr, C - 1);
+  const double* xa = XA + L.xaoff[l] * C;
+  v4d acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+  for (int k = kb; k < ke; k += 4 * kTopBatch) {
+    double a[2][kTopBatch], b[2][kTopBatch];
+#pragma unroll
+    for (int u = 0; u < kTopBatch; ++u) {
+      const int kk = k + 4 * u + lk;
+      const bool in = kk < ke;
+      const int kc = in ? kk : ke - 1;
+      const double* ar = A + (long long)kc * lda + row0 + lr;
+      a[0][u] = ar[0];
+      a[1][u] = ar[16];
+      const double* br = FWD ? Bsrc + (long long)(j0 + min(kc, nsub - 1)) * C : xa + (long long)kc * C;
+      const double x = br[colA], y = br[colB];
+      b[0][u] = in ? x : 0.0;
+      b[1][u] = in ? y : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kTopBatch; ++u)
+      if (k + 4 * u < ke) {
+        acc[0][0] = mfma16(a[0][u], b[0][u], acc[0][0]);
+        acc[0][1] = mfma16(a[0][u], b[1][u], acc[0][1]);
+        acc[1][0] = mfma16(a[1][u], b[0][u], acc[1][0]);
+        acc[1][1] = mfma16(a[1][u], b[1][u], acc[1][1]);
+      }
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) red[wave][((i * 2 + j) * 4 + q) * 64 + lane] = acc[i][j][q];
+  __syncthreads();
+  const int nab = L.nab_of[l];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int e = tid + 256 * u;
+    const int rr = e >> 5, cc = e & 31;
+    const int rl = rr & 15;
+    const int idx = (((rr >> 4) * 2 + (cc >> 4)) * 4 + (rl >> 2)) * 64 + (rl & 3) * 16 + (cc & 15);
+    const double v = ((red[0][idx] + red[1][idx]) + red[2][idx]) + red[3][idx];
+    const int col = c0 + cc;
+    if (col >= cend) continue;
+    const int a = row0 + rr;
+    if (FWD) {
+      if (a < nsub)
+        Yb[(long long)(j0 + a) * C + col] = v;
+      else if (a >= np && a - np < nab)
+        U[(L.uoff[l] + (a - np)) * C + col] = v;
+    } else if (a < nsub) {
+      B[(long long)(j0 + a) * C + col] = Yb[(long long)(j0 + a) * C + col] + v;
+    }
   }
 }

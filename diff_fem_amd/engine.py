@@ -256,6 +256,11 @@ class Scene:
         elimination tree at most max_rows; 0: off, level sweeps throughout)."""
         check(_lib.load().pdm_set_dense_top(self.h, int(max_rows)))
 
+    def set_dense_leaves(self, max_rows: int):
+        """Leaf subtrees of at most max_rows rows solved as dense products
+        (0: off)."""
+        check(_lib.load().pdm_set_dense_leaves(self.h, int(max_rows)))
+
     def solve_timing(self):
         """(number of solves, summed ms) since the last call; needs the
         PDM_TIME_SOLVES flag (HIP events around each solve)."""
@@ -371,6 +376,10 @@ class SplitScene:
     def set_dense_top(self, max_rows: int):
         for p in self.parts:
             p.set_dense_top(max_rows)
+
+    def set_dense_leaves(self, max_rows: int):
+        for p in self.parts:
+            p.set_dense_leaves(max_rows)
 
     def forward_device(self, x0, v0, steps: int, act=None, xT=None, vT=None, record: bool = False):
         import torch

@@ -108,6 +108,7 @@ typedef struct pdm_info {
   int n_self_faces;          /* surface faces tested for self contact */
   int top_rows;              /* rows of the dense top-separator blocks (padded; 0: none) */
   int top_blocks;            /* elimination trees with a dense top-separator block */
+  int leaf_blocks;           /* leaf subtrees solved as dense products */
 } pdm_info;
 
 /* Arithmetic of pdm_set_batch:
@@ -244,6 +245,15 @@ int pdm_set_solve_plan(pdm_scene s, int subtree_cut, int n_chains);
  * default is 1536. Re-plans the solve (launch shapes are re-chosen on the next
  * call). No counterpart in the reference (it solves with Eigen's LDL^T). */
 int pdm_set_dense_top(pdm_scene s, int max_rows);
+
+/* Dense leaf subtrees of the solve: below the top-separator blocks, every
+ * maximal subtree whose supernodes own at most max_rows rows together is
+ * solved as two dense products, z = K_s^{-1} b_s with its root's update block
+ * u = G b_s before the sweeps and x_s = z + G^T x_above after them (K_s^{-1}
+ * and G formed on the device at every factorisation). max_rows = 0 turns them
+ * off; the default is 512. Re-plans the solve. No counterpart in the
+ * reference. */
+int pdm_set_dense_leaves(pdm_scene s, int max_rows);
 
 /* Timing of the last forward/backward (ms, HIP events on the library stream). */
 int pdm_get_timing(pdm_scene s, double* fwd_ms, double* bwd_ms, double* factor_ms);

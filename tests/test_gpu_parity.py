@@ -65,11 +65,13 @@ def test_solve_shapes_do_not_change_results(eng):
 
 
 @pytest.mark.parametrize("which", ["config2", "packed"])
-def test_dense_top_blocks_match_oracle(eng, oracle_mod, which):
+def test_dense_blocks_match_oracle(eng, oracle_mod, which):
     """The top-separator blocks solved as one dense product per elimination
-    tree (one tree; four trees of a packed world) against the level sweeps
-    throughout (max_rows 0) and the oracle's LDL^T solve, with several block
-    sizes, and again after a material change re-forms Z on the device."""
+    tree (one tree; four trees of a packed world) and the dense leaf subtrees
+    below them, against the level sweeps throughout (max_rows 0) and the
+    oracle's LDL^T solve, with several block sizes (leaves from none to whole
+    trees), and again after a material change re-forms the dense operators
+    on the device."""
     d = scene.config2() if which == "config2" else packed_scene()
     R = 3
     s = eng.Scene(d, R=R)
@@ -89,6 +91,14 @@ def test_dense_top_blocks_match_oracle(eng, oracle_mod, which):
     for cap in (200, 700, 100000):
         s.set_dense_top(cap)
         assert rel(s.solve_host(b), x) < 1e-12, cap
+    # dense leaf subtrees below (small) top blocks: off, small, default, whole trees
+    s.set_dense_top(200)
+    for leaves in (0, 128, 512, 100000):
+        s.set_dense_leaves(leaves)
+        assert (s.info()["leaf_blocks"] > 0) == (leaves > 0)
+        assert rel(s.solve_host(b), x) < 1e-12, leaves
+    s.set_dense_top(1536)
+    s.set_dense_leaves(512)
     # new material: the device factorisation forms Z again
     s.set_dense_top(1536)
     E2 = d.E * rng.uniform(0.5, 2.0, d.E.shape)
