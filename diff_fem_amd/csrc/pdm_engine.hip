@@ -3103,6 +3103,17 @@ struct DevBuf {
   }
 };
 
+// A right-hand-side buffer of the solve ([N][C] doubles): 32 more rows and 32
+// doubles, zeroed, so that the dense leaf products may read past the last
+// leaf's own rows (they meet zero operator rows there) and past the last
+// column without leaving the buffer or meeting a non-finite value.
+int alloc_rhs(DevBuf<double>& b, size_t nc, int C, hipStream_t st) {
+  const size_t tail = 32 * (size_t)C + 32;
+  if (b.alloc(nc + tail)) return 1;
+  HIPX(hipMemsetAsync(b.p + nc, 0, tail * sizeof(double), st));
+  return 0;
+}
+
 // Grow a device buffer to `count` elements keeping its first `keep` elements.
 // The old buffer is freed at once, after the copy out of it: keeping it to
 // the end of the call would hold both generations of a whole-rollout contact
@@ -4336,7 +4347,8 @@ int ensure_batch_buffers(pdm_scene s) {
     g_err = "tets x rollouts must stay below 2^31 (element kernels index items in 32 bits)";
     return 1;
   }
-  if (s->X.alloc(nc) || s->V.alloc(nc) || s->Xn.alloc(nc) || s->Cc.alloc(nc) || s->Xit.alloc(nc) || s->B.alloc(nc) ||
+  if (s->X.alloc(nc) || s->V.alloc(nc) || s->Xn.alloc(nc) || s->Cc.alloc(nc) || s->Xit.alloc(nc) ||
+      alloc_rhs(s->B, nc, s->C, s->stream) ||
       s->Lx.alloc(nc))
     return 1;
   if (s->nfaces > 0 && (s->FF.alloc((size_t)s->nfaces * s->C) || s->GXf.alloc((size_t)3 * s->nfaces * s->C) ||
@@ -5366,7 +5378,8 @@ int backward_impl(pdm_scene s, const double* gxT, const double* gvT, double* gx0
   int seg = std::min(s->ckpt, std::max(steps, 1));
   if (ensure_tapes(s, steps)) return 1;
   if (s->gX.alloc(nc) || s->gV.alloc(nc) || s->Yb.alloc(nc) || s->Xb.alloc(nc) || s->Zs.alloc(nc) ||
-      s->Zb.alloc(nc) || s->LZ.alloc(nc) || s->gEpart.alloc((size_t)S.T * R) || s->ractive.alloc(R))
+      alloc_rhs(s->Zb, nc, C, s->stream) || s->LZ.alloc(nc) || s->gEpart.alloc((size_t)S.T * R) ||
+      s->ractive.alloc(R))
     return 1;
   if (S.M > 0 && s->gMpart.alloc((size_t)S.M * R)) return 1;
   if (s->ncv > 0) {

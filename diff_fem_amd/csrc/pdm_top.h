@@ -128,15 +128,15 @@ __global__ __launch_bounds__(256) void k_top_gather(const int* __restrict__ trow
 // flight while the products of the current one run. The quarters are added
 // through LDS in a fixed order, so the result does not depend on the launch
 // (columns, chains).
-constexpr int kTopBatch = 8;  // k-steps (4 k each) of operands per load batch
+constexpr int kTopBatch = 4;  // k-steps (4 k each) of operands per load batch
 struct TopOps {
   double a[2][kTopBatch], b[2][kTopBatch];
 };
 // Loads of one full batch: byte offsets zo / fo of the batch's first k-step
 // from uniform bases, advancing by zs / fs per k-step (32-bit offsets from a
-// scalar base). Columns past the chain's end are read too (F is padded so
-// every such read stays inside it); they only feed output columns that are
-// not written.
+// scalar base). Columns past the chain's end are read too (the buffers are
+// padded so every such read stays inside them); they only feed output columns
+// that are not written.
 __device__ __forceinline__ double ldo(const char* __restrict__ base, unsigned off) {
   return *(const double*)(base + off);
 }
@@ -159,43 +159,20 @@ __device__ __forceinline__ void top_mul(const TopOps& o, v4d (&acc)[2][2]) {
     acc[1][1] = mfma16(o.a[1][u], o.b[1][u], acc[1][1]);
   }
 }
-__global__ __launch_bounds__(256) void k_top_apply(const double* __restrict__ Z, const long long* __restrict__ zoff,
-                                                   const int* __restrict__ a0_of, const int* __restrict__ np_of,
-                                                   const int2* __restrict__ items, int ncolt,
-                                                   const double* __restrict__ F, const int* __restrict__ trow,
-                                                   double* __restrict__ B, int C, int cbase, int cw,
-                                                   const int* __restrict__ any) {
-  if (any && *any == 0) return;
-  __shared__ double red[4][16 * 64];
-  const int lb = xcd_block();
-  const int2 it = items[lb / ncolt];
-  const int r = it.x, row0 = 32 * it.y;
-  const int c0 = cbase + 32 * (lb % ncolt);
-  const int np = np_of[r], a0 = a0_of[r];
-  const double* Zr = Z + zoff[r];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, lr = lane & 15, lk = lane >> 4;
-  const int kq = np >> 2;  // np is a multiple of 32: quarters are multiples of 8
-  const int kb = wave * kq, ke = kb + kq;
-  const int cend = cbase + cw;
-  const char* zb = (const char*)Zr;
-  const char* fb = (const char*)(F + (long long)a0 * C);
-  // per k-step 4 k: Z advances 4 rows of np, F 4 rows of C
-  const unsigned zs = 32u * (unsigned)np, fs = 32u * (unsigned)C;
-  const unsigned zo0 = 8u * ((unsigned)(kb + lk) * (unsigned)np + (unsigned)(row0 + lr));
-  const unsigned fo0 = 8u * ((unsigned)(kb + lk) * (unsigned)C + (unsigned)(c0 + lr));
-  v4d acc[2][2];
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
-  constexpr int KB = 4 * kTopBatch;  // k per batch
-  const int nb = (ke - kb) / KB;     // full batches, then single k-steps
+
+// One wave's part of a dense product tile: acc[i][j] += sum over its nk
+// k-steps (4 k each) of A(k, rows 16 i + lr) B(k, columns 16 j + lr), with A
+// and B read at byte offsets zo0 / fo0 (its first k-step, lanes at their k
+// row lk and column lr) advancing zs / fs per k-step. Batch j + 1's loads are
+// issued before batch j's products; scheduling barriers keep the compiler
+// from sinking loads down to their uses (which would serialise load latency
+// and products again). The last even step reloads batch nb - 1 in place of
+// one past the end.
+__device__ __forceinline__ void dense_tile(v4d (&acc)[2][2], const char* __restrict__ zb, unsigned zo0, unsigned zs,
+                                           const char* __restrict__ fb, unsigned fo0, unsigned fs, int nk) {
+  const int nb = nk / kTopBatch;  // full batches, then single k-steps
   const unsigned zbs = kTopBatch * zs, fbs = kTopBatch * fs;
   if (nb > 0) {
-    // batch j + 1's loads are issued before batch j's products; scheduling
-    // barriers keep the compiler from sinking loads down to their uses (which
-    // would serialise load latency and products again). The last even step
-    // reloads batch nb - 1 in place of one past the end.
     TopOps o0, o1;
     top_load(o0, zb, fb, zo0, fo0, zs, fs);
     int j = 0;
@@ -212,7 +189,7 @@ __global__ __launch_bounds__(256) void k_top_apply(const double* __restrict__ Z,
     }
     if (j < nb) top_mul(o0, acc);  // odd count: the last batch is in o0
   }
-  for (int t = nb * kTopBatch; t < (ke - kb) >> 2; ++t) {
+  for (int t = nb * kTopBatch; t < nk; ++t) {
     const unsigned zo = zo0 + (unsigned)t * zs, fo = fo0 + (unsigned)t * fs;
     const double a0v = ldo(zb, zo), a1v = ldo(zb, zo + 128);
     const double b0v = ldo(fb, fo), b1v = ldo(fb, fo + 128);
@@ -221,6 +198,13 @@ __global__ __launch_bounds__(256) void k_top_apply(const double* __restrict__ Z,
     acc[1][0] = mfma16(a1v, b0v, acc[1][0]);
     acc[1][1] = mfma16(a1v, b1v, acc[1][1]);
   }
+}
+
+// The four waves' partial 32 x 32 tiles added in a fixed order through LDS;
+// out(rr, cc, v) is called for the 1,024 outputs, four per thread.
+template <typename Out>
+__device__ __forceinline__ void dense_reduce(const v4d (&acc)[2][2], double (*red)[16 * 64], Out out) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -236,11 +220,47 @@ __global__ __launch_bounds__(256) void k_top_apply(const double* __restrict__ Z,
     const int rr = e >> 5, cc = e & 31;
     const int rl = rr & 15;
     const int idx = (((rr >> 4) * 2 + (cc >> 4)) * 4 + (rl >> 2)) * 64 + (rl & 3) * 16 + (cc & 15);
-    const double v = ((red[0][idx] + red[1][idx]) + red[2][idx]) + red[3][idx];
+    out(rr, cc, ((red[0][idx] + red[1][idx]) + red[2][idx]) + red[3][idx]);
+  }
+}
+
+// Solve, part 2: x_t = Z f_t, written to the top rows of B. Block = (tree
+// block r, 32 output rows, 32 columns), 4 waves; wave q sums the q-th quarter
+// of k (np_r / 4 each) for the whole 32 x 32 output as 2 x 2 MFMA tiles with
+// operands from global memory (Z read as Z[k][a], its symmetric twin, so both
+// operands are 16 consecutive doubles per k). The quarters are added in a
+// fixed order, so the result does not depend on the launch (columns, chains).
+__global__ __launch_bounds__(256) void k_top_apply(const double* __restrict__ Z, const long long* __restrict__ zoff,
+                                                   const int* __restrict__ a0_of, const int* __restrict__ np_of,
+                                                   const int2* __restrict__ items, int ncolt,
+                                                   const double* __restrict__ F, const int* __restrict__ trow,
+                                                   double* __restrict__ B, int C, int cbase, int cw,
+                                                   const int* __restrict__ any) {
+  if (any && *any == 0) return;
+  __shared__ double red[4][16 * 64];
+  const int lb = xcd_block();
+  const int2 it = items[lb / ncolt];
+  const int r = it.x, row0 = 32 * it.y;
+  const int c0 = cbase + 32 * (lb % ncolt);
+  const int np = np_of[r], a0 = a0_of[r];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, lr = lane & 15, lk = lane >> 4;
+  const int kq = np >> 2;  // np is a multiple of 32: quarters are multiples of 8
+  const int kb = wave * kq;
+  const int cend = cbase + cw;
+  // per k-step 4 k: Z advances 4 rows of np, F 4 rows of C
+  v4d acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+  dense_tile(acc, (const char*)(Z + zoff[r]), 8u * ((unsigned)(kb + lk) * (unsigned)np + (unsigned)(row0 + lr)),
+             32u * (unsigned)np, (const char*)(F + (long long)a0 * C),
+             8u * ((unsigned)(kb + lk) * (unsigned)C + (unsigned)(c0 + lr)), 32u * (unsigned)C, kq >> 2);
+  dense_reduce(acc, red, [&](int rr, int cc, double v) {
     const int col = c0 + cc;
     const int er = trow[a0 + row0 + rr];
     if (er >= 0 && col < cend) B[(long long)er * C + col] = v;
-  }
+  });
 }
 
 // ----------------------------------------------------------------------------
@@ -295,12 +315,13 @@ __global__ __launch_bounds__(256) void k_leaf_gather(const int* __restrict__ xro
   XA[(long long)a * C + c] = r >= 0 ? B[(long long)r * C + c] : 0.0;
 }
 
-// Leaf products. FWD: out = P^T b (K = np; b rows j0 + k, clamped to the own
-// rows: P's padded rows are zero); rows a < nsub -> Yb (z), np <= a < np + nab
-// -> U at the root's update block (u). !FWD: out = z + G^T xa (K = nabp; xa
-// from XA, zero padded), rows a < nsub -> B (x). Block = (leaf, 32 output rows,
-// 32 columns), four waves each a quarter of k, quarters added in a fixed order
-// (as k_top_apply).
+// Leaf products. FWD: out = P^T b (K = np; b rows j0 + k: rows past the own
+// ones meet zero rows of P, and the right-hand-side buffers carry 32 zero rows
+// past N for the last leaf); rows a < nsub -> Yb (z), np <= a < np + nab -> U
+// at the root's update block (u). !FWD: out = z + G^T xa (K = nabp; xa from
+// XA, zero padded), rows a < nsub -> B (x). Block = (leaf, 32 output rows,
+// 32 columns), four waves each a quarter of k, quarters added in a fixed
+// order (dense_tile, dense_reduce).
 template <bool FWD>
 __global__ __launch_bounds__(256) void k_leaf_apply(LeafView L, const int2* __restrict__ items, int ncolt,
                                                     const double* __restrict__ Bsrc, const double* __restrict__ XA,
@@ -317,59 +338,23 @@ __global__ __launch_bounds__(256) void k_leaf_apply(LeafView L, const int2* __re
   const int K = FWD ? np : nabp;
   const int lda = FWD ? np + nabp : np;
   const double* A = FWD ? L.P + L.poff[l] : L.M + L.moff[l] + (long long)np * np;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, lr = lane & 15, lk = lane >> 4;
+  const double* Bop = FWD ? Bsrc + (long long)j0 * C : XA + L.xaoff[l] * C;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, lr = lane & 15, lk = lane >> 4;
   const int kq = K >> 2;  // K is a multiple of 32
-  const int kb = wave * kq, ke = kb + kq;
+  const int kb = wave * kq;
   const int cend = cbase + cw;
-  // columns past the end read a valid column (their outputs are not written)
-  const int colA = min(c0 + lr, C - 1), colB = min(c0 + 16 + lr, C - 1);
-  const double* xa = XA + L.xaoff[l] * C;
   v4d acc[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
-  for (int k = kb; k < ke; k += 4 * kTopBatch) {
-    double a[2][kTopBatch], b[2][kTopBatch];
-#pragma unroll
-    for (int u = 0; u < kTopBatch; ++u) {
-      const int kk = k + 4 * u + lk;
-      const bool in = kk < ke;
-      const int kc = in ? kk : ke - 1;
-      const double* ar = A + (long long)kc * lda + row0 + lr;
-      a[0][u] = ar[0];
-      a[1][u] = ar[16];
-      const double* br = FWD ? Bsrc + (long long)(j0 + min(kc, nsub - 1)) * C : xa + (long long)kc * C;
-      const double x = br[colA], y = br[colB];
-      b[0][u] = in ? x : 0.0;
-      b[1][u] = in ? y : 0.0;
-    }
-#pragma unroll
-    for (int u = 0; u < kTopBatch; ++u)
-      if (k + 4 * u < ke) {
-        acc[0][0] = mfma16(a[0][u], b[0][u], acc[0][0]);
-        acc[0][1] = mfma16(a[0][u], b[1][u], acc[0][1]);
-        acc[1][0] = mfma16(a[1][u], b[0][u], acc[1][0]);
-        acc[1][1] = mfma16(a[1][u], b[1][u], acc[1][1]);
-      }
-  }
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-#pragma unroll
-      for (int q = 0; q < 4; ++q) red[wave][((i * 2 + j) * 4 + q) * 64 + lane] = acc[i][j][q];
-  __syncthreads();
+  dense_tile(acc, (const char*)A, 8u * ((unsigned)(kb + lk) * (unsigned)lda + (unsigned)(row0 + lr)),
+             32u * (unsigned)lda, (const char*)Bop, 8u * ((unsigned)(kb + lk) * (unsigned)C + (unsigned)(c0 + lr)),
+             32u * (unsigned)C, kq >> 2);
   const int nab = L.nab_of[l];
-#pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    const int e = tid + 256 * u;
-    const int rr = e >> 5, cc = e & 31;
-    const int rl = rr & 15;
-    const int idx = (((rr >> 4) * 2 + (cc >> 4)) * 4 + (rl >> 2)) * 64 + (rl & 3) * 16 + (cc & 15);
-    const double v = ((red[0][idx] + red[1][idx]) + red[2][idx]) + red[3][idx];
+  dense_reduce(acc, red, [&](int rr, int cc, double v) {
     const int col = c0 + cc;
-    if (col >= cend) continue;
+    if (col >= cend) return;
     const int a = row0 + rr;
     if (FWD) {
       if (a < nsub)
@@ -379,5 +364,5 @@ __global__ __launch_bounds__(256) void k_leaf_apply(LeafView L, const int2* __re
     } else if (a < nsub) {
       B[(long long)(j0 + a) * C + col] = Yb[(long long)(j0 + a) * C + col] + v;
     }
-  }
+  });
 }
