@@ -101,6 +101,59 @@ def cpu_baseline(d, x_s0, v_s0, acts, s0, steps, threads):
                 nproc=info["nproc"], cpu_model=info["model"], setup_s=setup_s)
 
 
+def cpu_baseline_packed(d, meshes, x_s0, v_s0, acts, s0, steps, threads, n_meshes):
+    """Oracle timed on a bounded sample of the packed-mesh world. Its bodies
+    share no constraint and each keeps its own obstacles, so the sample is
+    the first `n_meshes` bodies, each as a scene of its own (moved back to
+    where its mesh was built, which changes nothing but the coordinates),
+    steps [s0, s0 + steps) from the world's GPU FP64 state at step s0,
+    forward + reverse sweep; one body per thread on `threads` threads. Setup
+    (mesh, muscle selection, factorisation) is excluded; each body's
+    iterations are its own (the world iterates until every body converges)."""
+    from concurrent.futures import ThreadPoolExecutor
+    from oracle import oracle as O
+    jobs = []
+    nv = ng = 0
+    for b, m in enumerate(meshes):
+        if b < n_meshes:
+            off = d.x_rest[nv] - m.x_rest[0]
+            xs = (x_s0.reshape(-1, 3)[nv:nv + m.N] - off).reshape(1, -1)
+            vs = v_s0.reshape(-1, 3)[nv:nv + m.N].reshape(1, -1)
+            a = acts[:1, s0:s0 + steps, ng:ng + m.n_groups] if m.seg_group.size else None
+            jobs.append((m, np.ascontiguousarray(xs), np.ascontiguousarray(vs),
+                         None if a is None else np.ascontiguousarray(a)))
+        nv += m.N
+        if m.seg_group.size:
+            ng += m.n_groups
+    t0 = time.perf_counter()
+    scenes = [O.OracleScene(m) for m, _, _, _ in jobs]
+    setup_s = time.perf_counter() - t0
+
+    def run(k):
+        m, xs, vs, a = jobs[k]
+        n3 = 3 * m.N
+        gx = np.full((1, n3), 1.0 / n3)
+        t1 = time.perf_counter()
+        _, _, its = scenes[k].fwd_adj_batch(xs, vs, steps, gx, np.zeros_like(gx), act=a, nthreads=1)
+        dt = time.perf_counter() - t1
+        log("  CPU sample: body %d (%d tets) done in %.1f s" % (k, m.T, dt))
+        return float(m.T) * float(its.sum()), dt
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        res = list(ex.map(run, range(len(jobs))))
+    wall = time.perf_counter() - t0
+    work = sum(w for w, _ in res)
+    info = host_cpu_info()
+    smp = ("oracle FP64 C++, bodies 0-%d of the packed world (%d tets of %d), each a scene of its own, "
+           "steps %d-%d (sphere and ground contact), forward + reverse sweep, one body per thread (%.1f s)"
+           % (len(jobs) - 1, sum(m.T for m, _, _, _ in jobs), d.T, s0, s0 + steps - 1, wall))
+    return dict(value=work / wall, unit="tet-iterations/s", cores=min(threads, len(jobs)), kind="port", sample=smp,
+                per_thread={"value": work / sum(t for _, t in res), "cores": 1,
+                            "sample": "the same work over the summed per-body thread times"},
+                nproc=info["nproc"], cpu_model=info["model"], setup_s=setup_s)
+
+
 def oracle_check(d, x0, v0, acts, steps, gx, gv, rows):
     """FP64 oracle forward + reverse sweep of the listed rollouts at the full
     horizon (OpenMP over them): positions and gradients to compare with."""
@@ -123,11 +176,11 @@ def rel_rows(a, b):
     return np.abs(a - b).max(1) / np.maximum(np.abs(b).max(1), 1e-300)
 
 
-def solve_traffic(rhs, nnz_lower, top_rows):
+def solve_traffic(rhs, nnz_lower, top_rows, leaf_blocks):
     """(HBM bytes per solve, source) from the committed counter summary
     (FETCH_SIZE and WRITE_SIZE runs, scaled by a copy kernel of known size),
-    when it was measured on this same factor, dense top block and
-    right-hand-side count; else None. It is not measured in this run: the
+    when it was measured on this same factor, dense top block, dense leaf
+    subtrees and right-hand-side count; else None. It is not measured in this run: the
     source names the counter run."""
     path = os.path.join(ROOT, "profiles", "solve_c4_R256_hbm_traffic.md")
     if not os.path.exists(path):
@@ -141,7 +194,8 @@ def solve_traffic(rhs, nnz_lower, top_rows):
             if v.strip():
                 vals[k.strip()] = v.strip() if k.strip() == "measured_by" else v.strip().split()[0]
     try:
-        if int(vals["rhs"]) == rhs and int(vals["nnz_lower"]) == nnz_lower and int(vals["top_rows"]) == top_rows:
+        if (int(vals["rhs"]) == rhs and int(vals["nnz_lower"]) == nnz_lower and int(vals["top_rows"]) == top_rows
+                and int(vals["leaf_blocks"]) == leaf_blocks):
             return float(vals["hbm_bytes_per_solve"]), "profiles/solve_c4_R256_hbm_traffic.md (%s)" % (
                 vals.get("measured_by", "rocprofv3 FETCH_SIZE/WRITE_SIZE counter runs"))
     except (KeyError, ValueError):
@@ -174,6 +228,10 @@ def main():
     ap.add_argument("--cpu-from", type=int, default=14,
                     help="first step of the CPU sample (the dropped body reaches the floor at about step 14)")
     ap.add_argument("--cpu-threads", type=int, default=16, help="threads of the multi-core CPU baseline (at most nproc)")
+    ap.add_argument("--cpu-bodies", type=int, default=16,
+                    help="packed-mesh workload: bodies in the CPU baseline sample (one per thread)")
+    ap.add_argument("--cpu-steps-c5", type=int, default=10,
+                    help="packed-mesh workload: steps of the CPU baseline sample")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 batch pass (throughput and error figures)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
@@ -332,9 +390,9 @@ def main():
     else:  # few right-hand sides: the factor stream bounds it
         achieved = bytes_alg / (ms_live * 1e-3) / 1e9
         rl = {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s", "frac": achieved / 8000.0}
-    tr = solve_traffic(3 * Rp, info["nnz_lower"], info["top_rows"])
+    tr = solve_traffic(3 * Rp, info["nnz_lower"], info["top_rows"], info["leaf_blocks"])
     rl.update({"traffic": tr[0] if tr else None, "traffic_source": tr[1] if tr else None,
-               "kernel": "supernodal solve k_fwd_level+k_bwd_level, %d RHS" % (3 * Rp),
+               "kernel": "supernodal solve (dense leaf subtrees, level sweeps, dense top separators), %d RHS" % (3 * Rp),
                "column_chains": p0.solve_chains,
                "ms_per_solve": ms_live, "solves_timed": n_solve,
                "ms_per_solve_back_to_back": p0.time_solve(20), "flops_per_solve": flops,
@@ -428,9 +486,17 @@ def main():
     out["factor"] = {"ms": f_dev, "wall_ms": f_wall, "where": "device (per-level extend-add, panel, update kernels)",
                      "host_code_ms": f_host, "flops": info["factor_flops"],
                      "gflops_per_s": info["factor_flops"] / (f_dev * 1e6)}
+    x_s0 = v_s0 = None
+    if args.workload == "c5" and not args.no_cpu:
+        # the world's state at the start of the CPU sample (FP64 forward)
+        for p in parts_list:
+            p.set_batch(p.R, engine.PDM_FP64, engine.PDM_HOST_EARLY_EXIT, args.ckpt)
+        xs, vs = torch.empty_like(x0), torch.empty_like(v0)
+        sc.forward_device(x0, v0, args.cpu_from, act=acts[:, :args.cpu_from].contiguous(), xT=xs, vT=vs)
+        torch.cuda.synchronize()
+        x_s0, v_s0 = xs.cpu().numpy()[0], vs.cpu().numpy()[0]
     sc.close()
     del sc
-    x_s0 = v_s0 = None
     if not args.no_plain:
         log("plain FP64 comparator (R = 1) ...")
         ps = args.plain_steps or S
@@ -467,12 +533,20 @@ def main():
             torch.cuda.synchronize()
             x_s0, v_s0 = xs.cpu().numpy()[0], vs.cpu().numpy()[0]
         pl.close()
-    if not args.no_cpu and args.workload == "c4" and x_s0 is not None:
-        # (the packed 64-mesh world is out of reach of the oracle: its muscle
-        # selection scans every tet for every segment)
+    if not args.no_cpu and x_s0 is not None:
         threads = min(host_cpu_info()["nproc"], args.cpu_threads)
-        log("CPU baseline (oracle, 1 and %d threads) ..." % threads)
-        out["cpu_baseline"] = cpu_baseline(d, x_s0, v_s0, acts.cpu().numpy(), args.cpu_from, args.cpu_steps, threads)
+        if args.workload == "c4":
+            log("CPU baseline (oracle, 1 and %d threads) ..." % threads)
+            out["cpu_baseline"] = cpu_baseline(d, x_s0, v_s0, acts.cpu().numpy(), args.cpu_from, args.cpu_steps,
+                                               threads)
+        else:
+            # the whole world in one oracle scene would spend minutes in its
+            # muscle selection (every tet against every segment): bodies apart
+            log("CPU baseline (oracle, first %d bodies on %d threads, %d steps) ..." % (
+                args.cpu_bodies, threads, args.cpu_steps_c5))
+            out["cpu_baseline"] = cpu_baseline_packed(d, scene.config5_meshes(seed=5), x_s0, v_s0,
+                                                      acts.cpu().numpy(), args.cpu_from, args.cpu_steps_c5, threads,
+                                                      args.cpu_bodies)
     print(json.dumps(out), flush=True)
 
 
