@@ -217,7 +217,8 @@ def main():
     ap.add_argument("--parts", type=int, default=2,
                     help="independent sub-batches run concurrently on their own streams (engine.SplitScene); "
                          "one part's latency-bound solve levels overlap the other's element kernels")
-    ap.add_argument("--sim-steps", type=int, default=100)
+    ap.add_argument("--sim-steps", type=int, default=0,
+                    help="simulated steps per pass (0: 100 for c4, 150 for c5)")
     ap.add_argument("--ckpt", type=int, default=0,
                     help="checkpoint interval; 0: tape the whole rollout in the forward pass (no recompute in the "
                          "adjoint; about 2.3 GB of HBM per step for the 40x12x12 box at R = 256)")
@@ -254,6 +255,8 @@ def main():
 
     d = build_workload(args)
     prec = engine.PDM_MIXED if args.precision == "mixed" else engine.PDM_MIXED_ADJOINT
+    if args.sim_steps <= 0:
+        args.sim_steps = 150 if args.workload == "c5" else 100
     R, S = args.rollouts, args.sim_steps
     if args.ckpt <= 0:
         args.ckpt = S
