@@ -220,17 +220,17 @@ __global__ void k_drag_gather(const double* __restrict__ GXf, const double* __re
 }
 
 // Active-rollout list: alist[0..*acount) = the rollouts r with active[r] != 0,
-// ascending.
+// ascending. One block of blockDim.x (a multiple of 64, at most 1024) threads.
 __global__ __launch_bounds__(1024) void k_compact(const int* __restrict__ active, int R, int* __restrict__ alist,
                                                   int* __restrict__ acount) {
-  // one block; per 1024 rollouts: a wave ballot gives each listed rollout its
+  // per blockDim.x rollouts: a wave ballot gives each listed rollout its
   // rank in its wave, the waves' counts (in wave order) its offset
   __shared__ int wsum[16];
   __shared__ int base;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
   if (threadIdx.x == 0) base = 0;
   __syncthreads();
-  for (int c0 = 0; c0 < R; c0 += 1024) {
+  for (int c0 = 0; c0 < R; c0 += blockDim.x) {
     const int i = c0 + threadIdx.x;
     const bool f = i < R && active[i] != 0;
     const unsigned long long m = __ballot(f);
@@ -242,12 +242,19 @@ __global__ __launch_bounds__(1024) void k_compact(const int* __restrict__ active
     __syncthreads();
     if (threadIdx.x == 0) {
       int t = 0;
-      for (int w = 0; w < 16; ++w) t += wsum[w];
+      for (int w = 0; w < nw; ++w) t += wsum[w];
       base += t;
     }
     __syncthreads();
   }
   if (threadIdx.x == 0) *acount = base;
+}
+
+// The list's launch: as many threads as rollouts (rounded up to a wave), at
+// most 1024.
+inline void launch_compact(const int* active, int R, int* alist, int* acount, hipStream_t st) {
+  const int threads = std::min(1024, std::max(64, (R + 63) / 64 * 64));
+  hipLaunchKernelGGL(k_compact, dim3(1), dim3(threads), 0, st, active, R, alist, acount);
 }
 
 // Active-rollout indexing of the element kernels. Work items are (tet t,
@@ -4799,7 +4806,7 @@ void launch_local(pdm_scene s, const double* Xsrc, const double* act, int act_st
   long long nt = (long long)S.T * s->R;
   const int G = S.n_groups;
   // element work only for the rollouts that still iterate
-  hipLaunchKernelGGL(k_compact, dim3(1), dim3(1024), 0, s->stream, mask, s->R, s->alist.p, s->acount.p);
+  launch_compact(mask, s->R, s->alist.p, s->acount.p, s->stream);
   // wave-uniform variant for kDenseBelow or more listed rollouts, dense one for fewer
   const bool uni = s->R % 64 == 0;
   if (uni)
@@ -5213,7 +5220,7 @@ int vjp_iteration(pdm_scene s, const int* active, const double* Yk, const double
   long long nt = (long long)S.T * s->R;
   // active null: the rollout list is already compacted
   if (active)
-    hipLaunchKernelGGL(k_compact, dim3(1), dim3(1024), 0, s->stream, active, s->R, s->alist.p, s->acount.p);
+    launch_compact(active, s->R, s->alist.p, s->acount.p, s->stream);
   const bool uni = s->R % 64 == 0;
   if (uni)
     hipLaunchKernelGGL((k_vjp_tet<T, true>), dim3(cdiv(nt, kThreads)), dim3(kThreads), 0, s->stream, Yk, Yk1,
@@ -5311,8 +5318,7 @@ int contact_reverse_iter(pdm_scene s, ContactTape& tp, int g, const double* Xn, 
   const size_t pg = (size_t)tp.pass_of[g];
   {
     const double* ori = tp.xit.p + (size_t)g * nc;
-    hipLaunchKernelGGL(k_compact, dim3(1), dim3(1024), 0, s->stream, tp.exec.p + (size_t)g * R, R, s->alist.p,
-                       s->acount.p);
+    launch_compact(tp.exec.p + (size_t)g * R, R, s->alist.p, s->acount.p, s->stream);
     if (s->ncs > 0) {
       const size_t NF = (size_t)s->nsf * R, NA = (size_t)s->ncs * R;
       hipLaunchKernelGGL((R % 64 == 0 ? k_sadj_pre<true> : k_sadj_pre<false>), dim3(cdiv((long long)N * R, kThreads)),
