@@ -371,14 +371,17 @@ __global__ __launch_bounds__(256) void k_local_tet(const double* __restrict__ X,
 // Segmented gather over the listed rollouts only: thread (vertex, jj) sums
 // column 3 alist[jj/3] + jj%3 over the vertex's slot range (four partial sums,
 // fixed order), so that a wave reads contiguous slot entries. MODE 0:
-// B = Cc + sum. MODE 1: B = sum. MODE 2: B += sum. Columns of unlisted
-// rollouts are left as they are. UNI: 3R is a multiple of 64 (one vertex per
-// wave, its slot range by scalar loads).
+// B = Cc + sum, plus (Gf non-null) inj[r] ? Gf : 0 (the contact reverse
+// sweep's end of iteration). MODE 1: B = sum. MODE 2: B += sum. Columns of
+// unlisted rollouts are left as they are. UNI: 3R is a multiple of 64 (one
+// vertex per wave, its slot range by scalar loads).
 template <typename T, int MODE, bool UNI>
 __global__ __launch_bounds__(256) void k_gather_list(const double* __restrict__ Cc, const T* __restrict__ corner,
                                                      const int* __restrict__ vptr, const int* __restrict__ alist,
                                                      const int* __restrict__ acount, double* __restrict__ B, int N,
-                                                     int R, const int* __restrict__ any = nullptr) {
+                                                     int R, const int* __restrict__ any = nullptr,
+                                                     const double* __restrict__ Gf = nullptr,
+                                                     const int* __restrict__ inj = nullptr) {
   if (any && *any == 0) return;
   const long long C = 3LL * R;
   long long idx = (long long)xcd_block() * blockDim.x + threadIdx.x;
@@ -419,7 +422,7 @@ __global__ __launch_bounds__(256) void k_gather_list(const double* __restrict__ 
   for (; q < qe; ++q, p += C) s0 += (double)p[0];
   const double sum = (s0 + s1) + (s2 + s3);
   if (MODE == 0)
-    B[o] = c0 + sum;
+    B[o] = Gf ? (c0 + sum) + (inj[alist[r3]] ? Gf[o] : 0.0) : c0 + sum;
   else if (MODE == 1)
     B[o] = sum;
   else
@@ -2787,21 +2790,6 @@ __global__ void k_cadj_cz(const int* __restrict__ Lptr, const int* __restrict__ 
   OB[o] = -sum;
 }
 
-// x_it_{g-1} adjoint = ori adjoint (+ the final-position adjoint for rollouts
-// whose result is this iteration's input).
-__global__ void k_cadj_post(const double* __restrict__ OB, const double* __restrict__ Gf, const int* __restrict__ inj,
-                            const int* __restrict__ alist, const int* __restrict__ acount, double* __restrict__ XB,
-                            long long n, int C) {
-  long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (idx >= n) return;
-  const long long v = idx / C;
-  const int jj = (int)(idx - v * C);
-  if (jj >= 3 * *acount) return;
-  const int r = alist[jj / 3];
-  const long long o = v * C + 3 * r + jj % 3;
-  XB[o] = OB[o] + (inj[r] ? Gf[o] : 0.0);
-}
-
 // ---------------------------------------- contact-phase adjoint, self contact
 // With self contact on (World.h:1373-1430, 1446-1600) an iteration's rhs is
 // forces + obstacle reactions + self-contact forces while the rollout has a
@@ -4817,7 +4805,7 @@ void launch_local(pdm_scene s, const double* Xsrc, const double* act, int act_st
                      Xsrc, s->tv.p, s->tG.p, s->wr.p, s->wv.p, s->tslot.p, s->alist.p, s->acount.p, corner, S.T, s->R,
                      s->mptr.p, s->mg.p, s->mw.p, s->mgroup.p, S.M, act, act_steps * G, step * G, step_dev, G);
   hipLaunchKernelGGL((uni ? k_gather_list<T, 0, true> : k_gather_list<T, 0, false>), dim3(cdiv((long long)S.N * 3 * s->R, kThreads)), dim3(kThreads), 0, s->stream,
-                     s->Cc.p, corner, s->vptr.p, s->alist.p, s->acount.p, s->B.p, S.N, s->R, any);
+                     s->Cc.p, corner, s->vptr.p, s->alist.p, s->acount.p, s->B.p, S.N, s->R, any, nullptr, nullptr);
 }
 
 
@@ -5214,7 +5202,8 @@ int forward_impl(pdm_scene s, int steps, const double* x0, const double* v0, con
 template <typename T>
 int vjp_iteration(pdm_scene s, const int* active, const double* Yk, const double* Yk1, const double* act,
                   int act_steps, int step, const double* CDp = nullptr, double* out_add = nullptr,
-                  const double* Xn = nullptr) {
+                  const double* Xn = nullptr, double* post_out = nullptr, const double* post_add = nullptr,
+                  const int* post_inj = nullptr) {
   HostScene& S = s->S;
   T* corner = (T*)s->corner.p;
   long long nt = (long long)S.T * s->R;
@@ -5234,12 +5223,16 @@ int vjp_iteration(pdm_scene s, const int* active, const double* Yk, const double
                      s->mw.p, s->mgroup.p, S.M, act, act_steps * S.n_groups, step * S.n_groups,
                      S.M > 0 ? s->gMpart.p : nullptr);
   dim3 gg(cdiv((long long)S.N * 3 * s->R, kThreads));
-  if (out_add)
+  if (out_add && post_out)  // post_out = out_add + sum + (post_inj[r] ? post_add : 0)
+    hipLaunchKernelGGL((uni ? k_gather_list<T, 0, true> : k_gather_list<T, 0, false>), gg, dim3(kThreads), 0,
+                       s->stream, out_add, corner, s->vptr.p, s->alist.p, s->acount.p, post_out, S.N, s->R, nullptr,
+                       post_add, post_inj);
+  else if (out_add)
     hipLaunchKernelGGL((uni ? k_gather_list<T, 2, true> : k_gather_list<T, 2, false>), gg, dim3(kThreads), 0, s->stream, nullptr, corner, s->vptr.p,
-                       s->alist.p, s->acount.p, out_add, S.N, s->R, nullptr);
+                       s->alist.p, s->acount.p, out_add, S.N, s->R, nullptr, nullptr, nullptr);
   else
     hipLaunchKernelGGL((uni ? k_gather_list<T, 1, true> : k_gather_list<T, 1, false>), gg, dim3(kThreads), 0, s->stream, nullptr, corner, s->vptr.p,
-                       s->alist.p, s->acount.p, s->Yb.p, S.N, s->R, nullptr);
+                       s->alist.p, s->acount.p, s->Yb.p, S.N, s->R, nullptr, nullptr, nullptr);
   return 0;
 }
 
@@ -5346,12 +5339,14 @@ int contact_reverse_iter(pdm_scene s, ContactTape& tp, int g, const double* Xn, 
                        sph ? tp.sx1.p + pg * 3 * ni : nullptr, s->cNB.p, (long long)cvc);
     hipLaunchKernelGGL((R % 64 == 0 ? k_cadj_cz<true> : k_cadj_cz<false>), dim3(cdiv(nc, kThreads)), dim3(kThreads), 0, s->stream, s->Lptr.p, s->Lcol.p,
                        s->Lval.p, s->Zb.p, s->Xb.p, s->cOB.p, s->alist.p, s->acount.p, N, C);
+    // x_it_{g-1} adjoint = ori adjoint (+ the final-position adjoint for
+    // rollouts whose result is this iteration's input), written by the gather
     int rc = s->precision != PDM_FP64
-                 ? vjp_iteration<float>(s, nullptr, ori, ori, act, act_steps, step, s->cCD.p, s->cOB.p, Xn)
-                 : vjp_iteration<double>(s, nullptr, ori, ori, act, act_steps, step, s->cCD.p, s->cOB.p, Xn);
+                 ? vjp_iteration<float>(s, nullptr, ori, ori, act, act_steps, step, s->cCD.p, s->cOB.p, Xn,
+                                        s->cXB.p, s->cGf.p, dinj + (size_t)g * R)
+                 : vjp_iteration<double>(s, nullptr, ori, ori, act, act_steps, step, s->cCD.p, s->cOB.p, Xn,
+                                         s->cXB.p, s->cGf.p, dinj + (size_t)g * R);
     if (rc) return 1;
-    hipLaunchKernelGGL(k_cadj_post, dim3(cdiv(nc, kThreads)), dim3(kThreads), 0, s->stream, s->cOB.p, s->cGf.p,
-                       dinj + (size_t)g * R, s->alist.p, s->acount.p, s->cXB.p, (long long)nc, C);
   }
   return 0;
 }
