@@ -2434,7 +2434,8 @@ __global__ void k_contact_jacobi(const int* __restrict__ Lptr, const int* __rest
                                  double* __restrict__ tX, double* __restrict__ self_cur = nullptr,
                                  double* __restrict__ Fo = nullptr,
                                  double* __restrict__ CN = nullptr, double* __restrict__ RC = nullptr,
-                                 const int* __restrict__ ngrp = nullptr) {
+                                 const int* __restrict__ ngrp = nullptr, const double* __restrict__ Cc = nullptr,
+                                 const double* __restrict__ corner = nullptr, const int* __restrict__ vptr = nullptr) {
   // self_cur non-null: self contact is on (World.h:1373-1430 with
   // m_handle_self_collision): the obstacle reaction is computed from the
   // forces plus the previous self-contact forces and kept in CN; it replaces
@@ -2449,7 +2450,36 @@ __global__ void k_contact_jacobi(const int* __restrict__ Lptr, const int* __rest
   const int r = alist[j];
   long long C = 3LL * R;
   long long base = (long long)v * C + 3 * r;
-  double rc[3] = {B[base], B[base + 1], B[base + 2]};
+  double rc[3];
+  if (corner) {
+    // c + J d gathered here from the FP64 corner slots (the gather's sums:
+    // slot k of the range to partial sum k mod 4 while four remain, the rest
+    // to the first, then (s0 + s1) + (s2 + s3))
+    double sp[3][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};
+    int q = vptr[v];
+    const int qe = vptr[v + 1];
+    const double* p = corner + (long long)q * C + 3 * r;
+    for (; q + 4 <= qe; q += 4, p += 4 * C) {
+      double x[4][3];
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) x[k][c] = p[k * C + c];
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) sp[c][k] += x[k][c];
+    }
+    for (; q < qe; ++q, p += C)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) sp[c][0] += p[c];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) rc[c] = Cc[base + c] + ((sp[c][0] + sp[c][1]) + (sp[c][2] + sp[c][3]));
+  } else {
+    rc[0] = B[base];
+    rc[1] = B[base + 1];
+    rc[2] = B[base + 2];
+  }
   {
     // neighbours in row order; the loads of four are issued together
     int q = Lptr[v];
@@ -4788,7 +4818,7 @@ int launch_solve_levels(pdm_scene s, double* Bp, const int* any) {
 
 template <typename T>
 void launch_local(pdm_scene s, const double* Xsrc, const double* act, int act_steps, int step, const int* mask,
-                  const int* step_dev = nullptr, const int* any = nullptr) {
+                  const int* step_dev = nullptr, const int* any = nullptr, bool gather = true) {
   HostScene& S = s->S;
   T* corner = (T*)s->corner.p;
   long long nt = (long long)S.T * s->R;
@@ -4804,8 +4834,10 @@ void launch_local(pdm_scene s, const double* Xsrc, const double* act, int act_st
   hipLaunchKernelGGL((k_local_tet<T, false>), dim3(element_blocks(S.T, uni ? kDenseBelow - 1 : s->R)), dim3(kThreads), 0, s->stream,
                      Xsrc, s->tv.p, s->tG.p, s->wr.p, s->wv.p, s->tslot.p, s->alist.p, s->acount.p, corner, S.T, s->R,
                      s->mptr.p, s->mg.p, s->mw.p, s->mgroup.p, S.M, act, act_steps * G, step * G, step_dev, G);
-  hipLaunchKernelGGL((uni ? k_gather_list<T, 0, true> : k_gather_list<T, 0, false>), dim3(cdiv((long long)S.N * 3 * s->R, kThreads)), dim3(kThreads), 0, s->stream,
-                     s->Cc.p, corner, s->vptr.p, s->alist.p, s->acount.p, s->B.p, S.N, s->R, any, nullptr, nullptr);
+  if (gather)
+    hipLaunchKernelGGL((uni ? k_gather_list<T, 0, true> : k_gather_list<T, 0, false>),
+                       dim3(cdiv((long long)S.N * 3 * s->R, kThreads)), dim3(kThreads), 0, s->stream, s->Cc.p, corner,
+                       s->vptr.p, s->alist.p, s->acount.p, s->B.p, S.N, s->R, any, nullptr, nullptr);
 }
 
 
@@ -4931,10 +4963,12 @@ int contact_phase(pdm_scene s, const double* act, int act_steps, int step, Conta
       hipLaunchKernelGGL(k_contact_err, dim3(cdiv(R, 4)), dim3(256), 0, s->stream, s->part.p, nchunk, R,
                          s->iactive.p, s->cfirst.p, s->citers.p, s->clast.p,
                          tp ? tp->exec.p + (size_t)tp->G * R : nullptr);
-      if (s->precision == PDM_MIXED)
-        launch_local<float>(s, s->Ori.p, act, act_steps, step, s->iactive.p);
+      // FP64 corner slots: the Jacobi kernel gathers them itself
+      const bool fused = s->precision != PDM_MIXED;
+      if (fused)
+        launch_local<double>(s, s->Ori.p, act, act_steps, step, s->iactive.p, nullptr, nullptr, false);
       else
-        launch_local<double>(s, s->Ori.p, act, act_steps, step, s->iactive.p);
+        launch_local<float>(s, s->Ori.p, act, act_steps, step, s->iactive.p);
       hipLaunchKernelGGL((R % 64 == 0 ? k_contact_jacobi<true> : k_contact_jacobi<false>), dim3(cdiv((long long)N * R, kThreads)), dim3(kThreads), 0, s->stream,
                          s->Lptr.p, s->Lcol.p, s->Lval.p, s->B.p, s->Dn.p, s->Xn.p, s->Kdiag.p, s->cv_of_v.p,
                          s->win_st.p, s->win_n.p, S.mu, S.alpha, S.h, i == 0 ? 1 : 0, s->alist.p,
@@ -4942,7 +4976,8 @@ int contact_phase(pdm_scene s, const double* act, int act_steps, int step, Conta
                          tp && i == 0 ? tp->nrm.p + (size_t)tp->P * cvc : nullptr,
                          tp && !self ? tp->xit.p + (size_t)(tp->G + 1) * nc : nullptr, self ? s->s_cur.p : nullptr,
                          self ? s->s_Fo.p : nullptr, self ? s->s_CN.p : nullptr, self ? s->s_RC.p : nullptr,
-                         self ? s->s_ngrp.p : nullptr);
+                         self ? s->s_ngrp.p : nullptr, fused ? s->Cc.p : nullptr,
+                         fused ? (const double*)s->corner.p : nullptr, fused ? s->vptr.p : nullptr);
       if (self) {
         if (tp)
           hipLaunchKernelGGL(k_self_tape_F, dim3(cdiv((long long)s->ncs * C, kThreads)), dim3(kThreads), 0, s->stream,
