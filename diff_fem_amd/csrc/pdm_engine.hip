@@ -371,17 +371,14 @@ __global__ __launch_bounds__(256) void k_local_tet(const double* __restrict__ X,
 // Segmented gather over the listed rollouts only: thread (vertex, jj) sums
 // column 3 alist[jj/3] + jj%3 over the vertex's slot range (four partial sums,
 // fixed order), so that a wave reads contiguous slot entries. MODE 0:
-// B = Cc + sum, plus (Gf non-null) inj[r] ? Gf : 0 (the contact reverse
-// sweep's end of iteration). MODE 1: B = sum. MODE 2: B += sum. Columns of
-// unlisted rollouts are left as they are. UNI: 3R is a multiple of 64 (one
-// vertex per wave, its slot range by scalar loads).
+// B = Cc + sum. MODE 1: B = sum. Columns of unlisted rollouts are left as
+// they are. UNI: 3R is a multiple of 64 (one vertex per
+// wave, its slot range by scalar loads).
 template <typename T, int MODE, bool UNI>
 __global__ __launch_bounds__(256) void k_gather_list(const double* __restrict__ Cc, const T* __restrict__ corner,
                                                      const int* __restrict__ vptr, const int* __restrict__ alist,
                                                      const int* __restrict__ acount, double* __restrict__ B, int N,
-                                                     int R, const int* __restrict__ any = nullptr,
-                                                     const double* __restrict__ Gf = nullptr,
-                                                     const int* __restrict__ inj = nullptr) {
+                                                     int R, const int* __restrict__ any = nullptr) {
   if (any && *any == 0) return;
   const long long C = 3LL * R;
   long long idx = (long long)xcd_block() * blockDim.x + threadIdx.x;
@@ -421,12 +418,7 @@ __global__ __launch_bounds__(256) void k_gather_list(const double* __restrict__ 
   }
   for (; q < qe; ++q, p += C) s0 += (double)p[0];
   const double sum = (s0 + s1) + (s2 + s3);
-  if (MODE == 0)
-    B[o] = Gf ? (c0 + sum) + (inj[alist[r3]] ? Gf[o] : 0.0) : c0 + sum;
-  else if (MODE == 1)
-    B[o] = sum;
-  else
-    B[o] += sum;
+  B[o] = MODE == 0 ? c0 + sum : sum;
 }
 
 // Per supernode s (w columns, m rows, the first w its own) the sweeps use
@@ -2781,43 +2773,69 @@ __global__ void k_cadj_pre(const double* __restrict__ XB, double* __restrict__ P
   CD[(long long)v * R + r] = cd;
 }
 
-// rc = h (c + J d(ori) - C (ori - x_n)): Xb += C z, OB = -C z (C = offdiag of
-// the scalar matrix without masses). UNI: C is a multiple of 64 (one vertex
-// per wave).
-template <bool UNI>
-__global__ void k_cadj_cz(const int* __restrict__ Lptr, const int* __restrict__ Lcol, const double* __restrict__ Lval,
-                          const double* __restrict__ Zb, double* __restrict__ Xb, double* __restrict__ OB,
-                          const int* __restrict__ alist, const int* __restrict__ acount, int N, int C) {
+// End of one executed contact iteration's reverse, per (vertex, column) of
+// the listed rollouts. rc = h (c + J d(ori) - C (ori - x_n)) with C = offdiag
+// of the scalar matrix without masses gives s = (C z)_v: Xb += s. The adjoint
+// of x_it_{g-1} (= ori) is XB = (-s + the VJP's corner sums at v) + the
+// final-position adjoint Gf for rollouts whose result is this iteration's
+// input. The neighbour sum has the row order, loads four at a time; the
+// corner sums the gather's partial-sum order. UNI: 3R is a multiple of 64 (one
+// vertex per wave).
+template <typename T, bool UNI>
+__global__ __launch_bounds__(256) void k_cadj_gather(const int* __restrict__ Lptr, const int* __restrict__ Lcol,
+                                                     const double* __restrict__ Lval, const double* __restrict__ Zb,
+                                                     double* __restrict__ Xb, const T* __restrict__ corner,
+                                                     const int* __restrict__ vptr, const double* __restrict__ Gf,
+                                                     const int* __restrict__ inj, const int* __restrict__ alist,
+                                                     const int* __restrict__ acount, double* __restrict__ XB, int N,
+                                                     int R) {
+  const long long C = 3LL * R;
   long long idx = (long long)xcd_block() * blockDim.x + threadIdx.x;
   if (idx >= (long long)N * C) return;
   int v = (int)(idx / C);
-  const int jj = (int)(idx % C);
+  const int jj = (int)(idx - (long long)v * C);
   if (UNI) v = __builtin_amdgcn_readfirstlane(v);
   if (jj >= 3 * *acount) return;
-  const int j = 3 * alist[jj / 3] + jj % 3;
-  double sum = 0.0;
-  int q = Lptr[v];
-  const int qe = Lptr[v + 1];
-  for (; q + 4 <= qe; q += 4) {
-    int u[4];
-    double w[4], z[4];
+  const int r = alist[jj / 3];
+  const int col = 3 * r + jj % 3;
+  const long long o = (long long)v * C + col;
+  double cz = 0.0;
+  {
+    int q = Lptr[v];
+    const int qe = Lptr[v + 1];
+    for (; q + 4 <= qe; q += 4) {
+      int u[4];
+      double w[4], z[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      u[k] = Lcol[q + k];
-      w[k] = Lval[q + k];
-      z[k] = Zb[(long long)u[k] * C + j];
+      for (int k = 0; k < 4; ++k) {
+        u[k] = Lcol[q + k];
+        w[k] = Lval[q + k];
+        z[k] = Zb[(long long)u[k] * C + col];
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (u[k] != v) cz += w[k] * z[k];
     }
+    for (; q < qe; ++q) {
+      int u = Lcol[q];
+      if (u != v) cz += Lval[q] * Zb[(long long)u * C + col];
+    }
+  }
+  Xb[o] += cz;
+  double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+  int q = vptr[v], qe = vptr[v + 1];
+  const T* p = corner + (long long)q * C + col;
+  for (; q + 4 <= qe; q += 4, p += 4 * C) {
+    double x[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
-      if (u[k] != v) sum += w[k] * z[k];
+    for (int u = 0; u < 4; ++u) x[u] = (double)p[u * C];
+    s0 += x[0];
+    s1 += x[1];
+    s2 += x[2];
+    s3 += x[3];
   }
-  for (; q < qe; ++q) {
-    int u = Lcol[q];
-    if (u != v) sum += Lval[q] * Zb[(long long)u * C + j];
-  }
-  const long long o = (long long)v * C + j;
-  Xb[o] += sum;
-  OB[o] = -sum;
+  for (; q < qe; ++q, p += C) s0 += (double)p[0];
+  XB[o] = (-cz + ((s0 + s1) + (s2 + s3))) + (inj[r] ? Gf[o] : 0.0);
 }
 
 // ---------------------------------------- contact-phase adjoint, self contact
@@ -3276,7 +3294,7 @@ struct pdm_scene_s {
   DevBuf<double> d_tris, d_bvh_box;
   DevBuf<int> d_bvh_a, d_bvh_b, d_bvh_tri;
   std::vector<std::unique_ptr<ContactTape>> ctape;  // per step of the segment being reversed
-  DevBuf<double> cXB, cPB, cOB, cGf, cCD, cNB;
+  DevBuf<double> cXB, cPB, cGf, cCD, cNB;
   DevBuf<double> cQB, cQF, cSCB, sSFb, sSFm, sSVb, sSVm;  // self-contact reverse
   DevBuf<int> win_sph;
   DevBuf<double> win_t, win_x1;
@@ -4837,7 +4855,7 @@ void launch_local(pdm_scene s, const double* Xsrc, const double* act, int act_st
   if (gather)
     hipLaunchKernelGGL((uni ? k_gather_list<T, 0, true> : k_gather_list<T, 0, false>),
                        dim3(cdiv((long long)S.N * 3 * s->R, kThreads)), dim3(kThreads), 0, s->stream, s->Cc.p, corner,
-                       s->vptr.p, s->alist.p, s->acount.p, s->B.p, S.N, s->R, any, nullptr, nullptr);
+                       s->vptr.p, s->alist.p, s->acount.p, s->B.p, S.N, s->R, any);
 }
 
 
@@ -5236,9 +5254,8 @@ int forward_impl(pdm_scene s, int steps, const double* x0, const double* v0, con
 
 template <typename T>
 int vjp_iteration(pdm_scene s, const int* active, const double* Yk, const double* Yk1, const double* act,
-                  int act_steps, int step, const double* CDp = nullptr, double* out_add = nullptr,
-                  const double* Xn = nullptr, double* post_out = nullptr, const double* post_add = nullptr,
-                  const int* post_inj = nullptr) {
+                  int act_steps, int step, const double* CDp = nullptr, const double* Xn = nullptr,
+                  bool gather = true) {
   HostScene& S = s->S;
   T* corner = (T*)s->corner.p;
   long long nt = (long long)S.T * s->R;
@@ -5257,17 +5274,10 @@ int vjp_iteration(pdm_scene s, const int* active, const double* Yk, const double
                      s->acount.p, corner, s->gEpart.p, S.T, s->R, CDp ? Xn : nullptr, CDp, s->mptr.p, s->mg.p,
                      s->mw.p, s->mgroup.p, S.M, act, act_steps * S.n_groups, step * S.n_groups,
                      S.M > 0 ? s->gMpart.p : nullptr);
-  dim3 gg(cdiv((long long)S.N * 3 * s->R, kThreads));
-  if (out_add && post_out)  // post_out = out_add + sum + (post_inj[r] ? post_add : 0)
-    hipLaunchKernelGGL((uni ? k_gather_list<T, 0, true> : k_gather_list<T, 0, false>), gg, dim3(kThreads), 0,
-                       s->stream, out_add, corner, s->vptr.p, s->alist.p, s->acount.p, post_out, S.N, s->R, nullptr,
-                       post_add, post_inj);
-  else if (out_add)
-    hipLaunchKernelGGL((uni ? k_gather_list<T, 2, true> : k_gather_list<T, 2, false>), gg, dim3(kThreads), 0, s->stream, nullptr, corner, s->vptr.p,
-                       s->alist.p, s->acount.p, out_add, S.N, s->R, nullptr, nullptr, nullptr);
-  else
-    hipLaunchKernelGGL((uni ? k_gather_list<T, 1, true> : k_gather_list<T, 1, false>), gg, dim3(kThreads), 0, s->stream, nullptr, corner, s->vptr.p,
-                       s->alist.p, s->acount.p, s->Yb.p, S.N, s->R, nullptr, nullptr, nullptr);
+  if (!gather) return 0;  // the caller gathers the corner slots itself
+  hipLaunchKernelGGL((uni ? k_gather_list<T, 1, true> : k_gather_list<T, 1, false>),
+                     dim3(cdiv((long long)S.N * 3 * s->R, kThreads)), dim3(kThreads), 0, s->stream, nullptr, corner,
+                     s->vptr.p, s->alist.p, s->acount.p, s->Yb.p, S.N, s->R, nullptr);
   return 0;
 }
 
@@ -5372,16 +5382,21 @@ int contact_reverse_iter(pdm_scene s, ContactTape& tp, int g, const double* Xn, 
                        s->alist.p, s->acount.p, s->Zb.p, s->Zs.p, s->cCD.p, N, R, s->d_spheres.p, nsph,
                        sph ? tp.sinfo.p + pg * ni : nullptr, sph ? tp.st.p + pg * ni : nullptr,
                        sph ? tp.sx1.p + pg * 3 * ni : nullptr, s->cNB.p, (long long)cvc);
-    hipLaunchKernelGGL((R % 64 == 0 ? k_cadj_cz<true> : k_cadj_cz<false>), dim3(cdiv(nc, kThreads)), dim3(kThreads), 0, s->stream, s->Lptr.p, s->Lcol.p,
-                       s->Lval.p, s->Zb.p, s->Xb.p, s->cOB.p, s->alist.p, s->acount.p, N, C);
-    // x_it_{g-1} adjoint = ori adjoint (+ the final-position adjoint for
-    // rollouts whose result is this iteration's input), written by the gather
-    int rc = s->precision != PDM_FP64
-                 ? vjp_iteration<float>(s, nullptr, ori, ori, act, act_steps, step, s->cCD.p, s->cOB.p, Xn,
-                                        s->cXB.p, s->cGf.p, dinj + (size_t)g * R)
-                 : vjp_iteration<double>(s, nullptr, ori, ori, act, act_steps, step, s->cCD.p, s->cOB.p, Xn,
-                                         s->cXB.p, s->cGf.p, dinj + (size_t)g * R);
-    if (rc) return 1;
+    // local-step VJP into the corner slots, then the iteration's end (C z,
+    // the corner sums, x_it_{g-1}'s adjoint) in one kernel
+    const dim3 gg(cdiv(nc, kThreads));
+    const bool uni = R % 64 == 0;
+    if (s->precision != PDM_FP64) {
+      if (vjp_iteration<float>(s, nullptr, ori, ori, act, act_steps, step, s->cCD.p, Xn, false)) return 1;
+      hipLaunchKernelGGL((uni ? k_cadj_gather<float, true> : k_cadj_gather<float, false>), gg, dim3(kThreads), 0,
+                         s->stream, s->Lptr.p, s->Lcol.p, s->Lval.p, s->Zb.p, s->Xb.p, (const float*)s->corner.p,
+                         s->vptr.p, s->cGf.p, dinj + (size_t)g * R, s->alist.p, s->acount.p, s->cXB.p, N, R);
+    } else {
+      if (vjp_iteration<double>(s, nullptr, ori, ori, act, act_steps, step, s->cCD.p, Xn, false)) return 1;
+      hipLaunchKernelGGL((uni ? k_cadj_gather<double, true> : k_cadj_gather<double, false>), gg, dim3(kThreads), 0,
+                         s->stream, s->Lptr.p, s->Lcol.p, s->Lval.p, s->Zb.p, s->Xb.p, (const double*)s->corner.p,
+                         s->vptr.p, s->cGf.p, dinj + (size_t)g * R, s->alist.p, s->acount.p, s->cXB.p, N, R);
+    }
   }
   return 0;
 }
@@ -5419,7 +5434,7 @@ int backward_impl(pdm_scene s, const double* gxT, const double* gvT, double* gx0
     return 1;
   if (S.M > 0 && s->gMpart.alloc((size_t)S.M * R)) return 1;
   if (s->ncv > 0) {
-    if (s->cXB.alloc(nc) || s->cPB.alloc(nc) || s->cOB.alloc(nc) || s->cGf.alloc(nc) || s->cCD.alloc((size_t)N * R))
+    if (s->cXB.alloc(nc) || s->cPB.alloc(nc) || s->cGf.alloc(nc) || s->cCD.alloc((size_t)N * R))
       return 1;
     if (s->ncs > 0 && (s->cQB.alloc(nc) || s->cQF.alloc(nc) || s->cSCB.alloc(nc) ||
                        s->sSFb.alloc((size_t)3 * s->nsf * C) || s->sSFm.alloc((size_t)3 * s->nsf * R) ||
