@@ -187,38 +187,32 @@ __device__ __forceinline__ void svd3(const T* F, T* U, T* S, T* V) {
     T off = fabs(a01) + fabs(a02) + fabs(a12);
     if (off > T(1e-7) * (a00 + a11 + a22)) sweep();
   }
-  // B = F V, columns sorted by norm (descending)
+  // Columns of V in the descending order of F^T F's eigenvalues (the
+  // converged diagonal, a_ii = |F v_i|^2 up to rounding), so that the columns
+  // of B = F V come out sorted by norm: only V's nine entries are swapped.
+  auto swapv = [&](int i, int j) {
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      T t = v[r * 3 + i];
+      v[r * 3 + i] = v[r * 3 + j];
+      v[r * 3 + j] = t;
+    }
+  };
+  if (a00 < a11) {
+    swapv(0, 1);
+    T t = a00; a00 = a11; a11 = t;
+  }
+  if (a00 < a22) {
+    swapv(0, 2);
+    T t = a00; a00 = a22; a22 = t;
+  }
+  if (a11 < a22) swapv(1, 2);
   T b[9];
 #pragma unroll
   for (int r = 0; r < 3; ++r)
 #pragma unroll
     for (int c = 0; c < 3; ++c) b[r * 3 + c] = F[r * 3 + 0] * v[0 * 3 + c] + F[r * 3 + 1] * v[1 * 3 + c] + F[r * 3 + 2] * v[2 * 3 + c];
-  T n0 = b[0] * b[0] + b[3] * b[3] + b[6] * b[6];
-  T n1 = b[1] * b[1] + b[4] * b[4] + b[7] * b[7];
-  T n2 = b[2] * b[2] + b[5] * b[5] + b[8] * b[8];
-  auto swapc = [&](int i, int j) {
-#pragma unroll
-    for (int r = 0; r < 3; ++r) {
-      T t = b[r * 3 + i];
-      b[r * 3 + i] = b[r * 3 + j];
-      b[r * 3 + j] = t;
-      t = v[r * 3 + i];
-      v[r * 3 + i] = v[r * 3 + j];
-      v[r * 3 + j] = t;
-    }
-  };
-  if (n0 < n1) {
-    swapc(0, 1);
-    T t = n0; n0 = n1; n1 = t;
-  }
-  if (n0 < n2) {
-    swapc(0, 2);
-    T t = n0; n0 = n2; n2 = t;
-  }
-  if (n1 < n2) {
-    swapc(1, 2);
-    T t = n1; n1 = n2; n2 = t;
-  }
+  const T n0 = b[0] * b[0] + b[3] * b[3] + b[6] * b[6];
   // Gram-Schmidt QR of b
   T s0 = Num<T>::sqrt_(n0);
   T u0[3], u1[3], u2[3];
