@@ -301,6 +301,24 @@ __device__ __forceinline__ bool element_item(const int* __restrict__ acount, int
   return true;
 }
 
+// The wave-uniform item of this thread with the wave's idle lanes kept
+// (*live = false, j clamped to 0): false only for waves without work.
+__device__ __forceinline__ bool element_wave(const int* __restrict__ acount, int Tn, int* t, int* j, bool* live) {
+  const int A = *acount;
+  if (A < kDenseBelow) return false;
+  const unsigned P = (unsigned)(A + 63) & ~63u;
+  const unsigned n = (unsigned)Tn * P;
+  const int b = xcd_block_n((int)((n + blockDim.x - 1) / blockDim.x));
+  if (b < 0) return false;
+  const unsigned idx = (unsigned)b * blockDim.x + threadIdx.x;
+  if (idx >= n) return false;  // n is a multiple of 64: whole waves
+  *t = __builtin_amdgcn_readfirstlane((int)(idx / P));
+  const int jj = (int)(idx - (unsigned)*t * P);
+  *live = jj < A;
+  *j = *live ? jj : jj & ~63;  // idle lanes take their wave's first (listed) item
+  return true;
+}
+
 template <typename T, bool UNI>
 __global__ __launch_bounds__(256) void k_local_tet(const double* __restrict__ X, const int* __restrict__ tv,
                                                    const double* __restrict__ tG, const double* __restrict__ wr,
@@ -1231,9 +1249,18 @@ __global__ __launch_bounds__(256, (sizeof(T) == 4 && UNI) ? 4 : 1) void k_vjp_te
                                                  const int* __restrict__ mptr, const double* __restrict__ mg,
                                                  const double* __restrict__ mw, const int* __restrict__ mgroup, int Mn,
                                                  const double* __restrict__ act, int act_stride_r, int act_off,
-                                                 double* __restrict__ gMpart) {
+                                                 double* __restrict__ gMpart, double* __restrict__ gEw) {
+  // UNI: the whole wave stays (its idle lanes compute on the wave's first
+  // rollout and write nothing) so that the E-gradient terms of its 64
+  // rollouts are summed across the wave and added once per (tet, wave) into
+  // gEw; otherwise per (tet, rollout) into gEpart
   int t, j;
-  if (!element_item<UNI>(acount, Tn, R, &t, &j)) return;
+  bool live = true;
+  if (UNI) {
+    if (!element_wave(acount, Tn, &t, &j, &live)) return;
+  } else if (!element_item<false>(acount, Tn, R, &t, &j)) {
+    return;
+  }
   const int r = alist[j];
   int C = 3 * R;
   int v[4] = {tv[t], tv[Tn + t], tv[2 * Tn + t], tv[3 * Tn + t]};
@@ -1308,7 +1335,13 @@ __global__ __launch_bounds__(256, (sizeof(T) == 4 && UNI) ? 4 : 1) void k_vjp_te
     gr = (double)zmd - fzf1 + extra;
     gv = (double)zmdv - fzf1 + extra;
   }
-  gEpart[(long long)t * R + r] += gr * dEr[t] + gv * dEv[t];
+  const double ge = gr * dEr[t] + gv * dEv[t];
+  if (UNI) {
+    const double sw = wave_sum(live ? ge : 0.0);  // fixed-order butterfly over the wave
+    if ((threadIdx.x & 63) == 0) gEw[(long long)t * (R >> 6) + (j >> 6)] += sw;
+  } else {
+    gEpart[(long long)t * R + r] += ge;
+  }
   T f[12];
 #pragma unroll
   for (int k = 0; k < 4; ++k)
@@ -1331,13 +1364,14 @@ __global__ __launch_bounds__(256, (sizeof(T) == 4 && UNI) ? 4 : 1) void k_vjp_te
       for (int k = 1; k < 4; ++k) ff[c] += (Yk[(long long)v[k] * C + 3 * r + c] - b) * g[k];
     }
     double wm = mw[m];
-    gMpart[(long long)m * R + r] += -wm * (fz[0] * ff[0] + fz[1] * ff[1] + fz[2] * ff[2]);
+    if (live) gMpart[(long long)m * R + r] += -wm * (fz[0] * ff[0] + fz[1] * ff[1] + fz[2] * ff[2]);
     double sc = wm * (1.0 - a);
 #pragma unroll
     for (int k = 0; k < 4; ++k)
 #pragma unroll
       for (int c = 0; c < 3; ++c) f[3 * k + c] += (T)(sc * fz[c] * g[k]);
   }
+  if (!live) return;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     T* o = corner + (long long)tslot[k * Tn + t] * C + 3 * r;
@@ -1381,12 +1415,14 @@ __global__ void k_active_rev(const int* __restrict__ iters, int k, int R, int* _
   if (r < R) active[r] = k < iters[r] ? 1 : 0;
 }
 
-__global__ void k_reduce_rows(const double* __restrict__ part, const int* __restrict__ tet_order, double* __restrict__ out,
-                              int Tn, int R) {
+// gE of tet t (caller order): its per-rollout partials, then its per-wave ones.
+__global__ void k_reduce_rows(const double* __restrict__ part, const double* __restrict__ wpart,
+                              const int* __restrict__ tet_order, double* __restrict__ out, int Tn, int R, int W) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= Tn) return;
   double s = 0.0;
   for (int r = 0; r < R; ++r) s += part[(long long)t * R + r];
+  for (int w = 0; w < W; ++w) s += wpart[(long long)t * W + w];
   out[tet_order[t]] = s;
 }
 
@@ -3304,6 +3340,7 @@ struct pdm_scene_s {
   std::vector<int> npass_host, citers_host;  // R*steps of the last forward
   // adjoint buffers
   DevBuf<double> gX, gV, Yb, Xb, Zs, Zb, LZ, gEpart, gMpart, gact;
+  DevBuf<double> gEw;  // E-gradient sums per (tet, wave of 64 listed rollouts) of the wave-uniform VJP
   DevBuf<int> ractive;
   double t_fwd = 0, t_bwd = 0, t_factor = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -5268,12 +5305,12 @@ int vjp_iteration(pdm_scene s, const int* active, const double* Yk, const double
                        s->Zb.p, s->tv.p, s->tG.p, s->wr.p, s->wv.p, s->dEr.p, s->dEv.p, s->tslot.p, s->alist.p,
                        s->acount.p, corner, s->gEpart.p, S.T, s->R, CDp ? Xn : nullptr, CDp, s->mptr.p, s->mg.p,
                        s->mw.p, s->mgroup.p, S.M, act, act_steps * S.n_groups, step * S.n_groups,
-                       S.M > 0 ? s->gMpart.p : nullptr);
+                       S.M > 0 ? s->gMpart.p : nullptr, s->gEw.p);
   hipLaunchKernelGGL((k_vjp_tet<T, false>), dim3(element_blocks(S.T, uni ? kDenseBelow - 1 : s->R)), dim3(kThreads), 0, s->stream,
                      Yk, Yk1, s->Zb.p, s->tv.p, s->tG.p, s->wr.p, s->wv.p, s->dEr.p, s->dEv.p, s->tslot.p, s->alist.p,
                      s->acount.p, corner, s->gEpart.p, S.T, s->R, CDp ? Xn : nullptr, CDp, s->mptr.p, s->mg.p,
                      s->mw.p, s->mgroup.p, S.M, act, act_steps * S.n_groups, step * S.n_groups,
-                     S.M > 0 ? s->gMpart.p : nullptr);
+                     S.M > 0 ? s->gMpart.p : nullptr, s->gEw.p);
   if (!gather) return 0;  // the caller gathers the corner slots itself
   hipLaunchKernelGGL((uni ? k_gather_list<T, 1, true> : k_gather_list<T, 1, false>),
                      dim3(cdiv((long long)S.N * 3 * s->R, kThreads)), dim3(kThreads), 0, s->stream, nullptr, corner,
@@ -5430,6 +5467,7 @@ int backward_impl(pdm_scene s, const double* gxT, const double* gvT, double* gx0
   if (ensure_tapes(s, steps)) return 1;
   if (s->gX.alloc(nc) || s->gV.alloc(nc) || s->Yb.alloc(nc) || s->Xb.alloc(nc) || s->Zs.alloc(nc) ||
       alloc_rhs(s->Zb, nc, C, s->stream) || s->LZ.alloc(nc) || s->gEpart.alloc((size_t)S.T * R) ||
+      s->gEw.alloc((size_t)S.T * std::max(1, R / 64)) ||
       s->ractive.alloc(R))
     return 1;
   if (S.M > 0 && s->gMpart.alloc((size_t)S.M * R)) return 1;
@@ -5445,6 +5483,7 @@ int backward_impl(pdm_scene s, const double* gxT, const double* gvT, double* gx0
   if (s->gact.alloc(nact)) return 1;
   HIPX(hipEventRecord(s->ev0, s->stream));
   HIPX(hipMemsetAsync(s->gEpart.p, 0, (size_t)S.T * R * 8, s->stream));
+  HIPX(hipMemsetAsync(s->gEw.p, 0, (size_t)S.T * std::max(1, R / 64) * 8, s->stream));
   HIPX(hipMemsetAsync(s->gact.p, 0, nact * 8, s->stream));
   hipLaunchKernelGGL(k_caller_to_engine, dim3(cdiv(nc, kThreads)), dim3(kThreads), 0, s->stream, gxT, s->gX.p,
                      s->perm.p, N, R);
@@ -5524,8 +5563,8 @@ int backward_impl(pdm_scene s, const double* gxT, const double* gvT, double* gx0
                        s->perm.p, N, R);
   DevBuf<double>& gEd = s->gEd;
   if (gEd.alloc(S.T)) return 1;
-  hipLaunchKernelGGL(k_reduce_rows, dim3(cdiv(S.T, kThreads)), dim3(kThreads), 0, s->stream, s->gEpart.p,
-                     s->tet_order.p, gEd.p, S.T, R);
+  hipLaunchKernelGGL(k_reduce_rows, dim3(cdiv(S.T, kThreads)), dim3(kThreads), 0, s->stream, s->gEpart.p, s->gEw.p,
+                     s->tet_order.p, gEd.p, S.T, R, std::max(1, R / 64));
   if (gE) HIPX(hipMemcpyAsync(gE, gEd.p, (size_t)S.T * 8, hipMemcpyDeviceToDevice, s->stream));
   if (gact && S.n_groups > 0)
     HIPX(hipMemcpyAsync(gact, s->gact.p, (size_t)R * steps * S.n_groups * 8, hipMemcpyDeviceToDevice, s->stream));
