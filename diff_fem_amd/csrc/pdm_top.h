@@ -128,7 +128,7 @@ __global__ __launch_bounds__(256) void k_top_gather(const int* __restrict__ trow
 // flight while the products of the current one run. The quarters are added
 // through LDS in a fixed order, so the result does not depend on the launch
 // (columns, chains).
-constexpr int kTopBatch = 4;  // k-steps (4 k each) of operands per load batch
+constexpr int kTopBatch = 2;  // k-steps (4 k each) of operands per load batch
 struct TopOps {
   double a[2][kTopBatch], b[2][kTopBatch];
 };
