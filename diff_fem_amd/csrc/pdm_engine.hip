@@ -319,21 +319,15 @@ __device__ __forceinline__ bool element_wave(const int* __restrict__ acount, int
   return true;
 }
 
-template <typename T, bool UNI>
-__global__ __launch_bounds__(256) void k_local_tet(const double* __restrict__ X, const int* __restrict__ tv,
-                                                   const double* __restrict__ tG, const double* __restrict__ wr,
-                                                   const double* __restrict__ wv, const int* __restrict__ tslot,
-                                                   const int* __restrict__ alist, const int* __restrict__ acount,
-                                                   T* __restrict__ corner, int Tn,
-                                                   int R, const int* __restrict__ mptr, const double* __restrict__ mg,
-                                                   const double* __restrict__ mw, const int* __restrict__ mgroup,
-                                                   int Mn, const double* __restrict__ act, int act_stride_r,
-                                                   int act_off, const int* __restrict__ step_dev, int ngroups) {
-  int t, j;
-  if (!element_item<UNI>(acount, Tn, R, &t, &j)) return;
-  const int r = alist[j];
-  if (step_dev) act_off = *step_dev * ngroups;
-  int C = 3 * R;
+// Corner forces of tet t for rollout r (the local step's projection P, then
+// f_k = P g_k, plus the tet's muscles), in the tet's corner order.
+template <typename T>
+__device__ __forceinline__ void tet_forces(const double* __restrict__ X, const int* __restrict__ tv,
+                                           const double* __restrict__ tG, const double* __restrict__ wr,
+                                           const double* __restrict__ wv, int t, int Tn, int r, int C,
+                                           const int* __restrict__ mptr, const double* __restrict__ mg,
+                                           const double* __restrict__ mw, const int* __restrict__ mgroup, int Mn,
+                                           const double* __restrict__ act, int act_stride_r, int act_off, T* f) {
   int v0 = tv[t], v1 = tv[Tn + t], v2 = tv[2 * Tn + t], v3 = tv[3 * Tn + t];
   const double* x0 = X + (long long)v0 * C + 3 * r;
   const double* x1 = X + (long long)v1 * C + 3 * r;
@@ -356,7 +350,6 @@ __global__ __launch_bounds__(256) void k_local_tet(const double* __restrict__ X,
       F[rr * 3 + j] = ds[rr * 3 + 0] * G[3 + j] + ds[rr * 3 + 1] * G[6 + j] + ds[rr * 3 + 2] * G[9 + j];
   T P[9];
   corotate_P<T>(F, (T)wr[t], (T)wv[t], P);
-  T f[12];
 #pragma unroll
   for (int k = 0; k < 4; ++k)
 #pragma unroll
@@ -378,12 +371,125 @@ __global__ __launch_bounds__(256) void k_local_tet(const double* __restrict__ X,
 #pragma unroll
       for (int c = 0; c < 3; ++c) f[3 * k + c] += sc * ff[c] * g[k];
   }
+}
+
+// Dense element variant: thread (tet, listed rollout), corner forces into the
+// tet's corner slots.
+template <typename T>
+__global__ __launch_bounds__(256) void k_local_tet(const double* __restrict__ X, const int* __restrict__ tv,
+                                                   const double* __restrict__ tG, const double* __restrict__ wr,
+                                                   const double* __restrict__ wv, const int* __restrict__ tslot,
+                                                   const int* __restrict__ alist, const int* __restrict__ acount,
+                                                   T* __restrict__ corner, int Tn,
+                                                   int R, const int* __restrict__ mptr, const double* __restrict__ mg,
+                                                   const double* __restrict__ mw, const int* __restrict__ mgroup,
+                                                   int Mn, const double* __restrict__ act, int act_stride_r,
+                                                   int act_off, const int* __restrict__ step_dev, int ngroups) {
+  int t, j;
+  if (!element_item<false>(acount, Tn, R, &t, &j)) return;
+  const int r = alist[j];
+  if (step_dev) act_off = *step_dev * ngroups;
+  const int C = 3 * R;
+  T f[12];
+  tet_forces<T>(X, tv, tG, wr, wv, t, Tn, r, C, mptr, mg, mw, mgroup, Mn, act, act_stride_r, act_off, f);
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     T* o = corner + (long long)tslot[k * Tn + t] * C + 3 * r;
 #pragma unroll
     for (int rr = 0; rr < 3; ++rr) o[rr] = f[3 * k + rr];
   }
+}
+
+// a += f, never contracted with the product that formed f: the dense variant
+// stores f and k_strip_fold adds the stored value, so both must add the same
+// rounded f.
+template <typename T>
+__device__ __forceinline__ void strip_add(T* a, const T* f) {
+#pragma clang fp contract(off)
+  a[0] = a[0] + f[0];
+  a[1] = a[1] + f[1];
+  a[2] = a[2] + f[2];
+}
+
+// Wave-uniform element variant (kDenseBelow or more listed rollouts, R a
+// multiple of 64): wave (strip, 64 listed rollouts) walks the strip's tets and
+// adds each corner force into the register slot its code names (uniform, so
+// the slot choice is a scalar branch); after each tet the slots of its flush
+// mask are written as strip sums and cleared.
+template <typename T>
+__global__ __launch_bounds__(256) void k_local_strip(const double* __restrict__ X, const int* __restrict__ tv,
+                                                     const double* __restrict__ tG, const double* __restrict__ wr,
+                                                     const double* __restrict__ wv, const int* __restrict__ stet,
+                                                     const int* __restrict__ scode, const int* __restrict__ sfbase,
+                                                     const int* __restrict__ ssum_of, const int* __restrict__ alist,
+                                                     const int* __restrict__ acount, T* __restrict__ ssum, int Tn,
+                                                     int nstrip, int R, const int* __restrict__ mptr,
+                                                     const double* __restrict__ mg, const double* __restrict__ mw,
+                                                     const int* __restrict__ mgroup, int Mn,
+                                                     const double* __restrict__ act, int act_stride_r, int act_off,
+                                                     const int* __restrict__ step_dev, int ngroups) {
+  int s, j;
+  if (!element_item<true>(acount, nstrip, R, &s, &j)) return;
+  const int r = alist[j];
+  if (step_dev) act_off = *step_dev * ngroups;
+  const int C = 3 * R;
+  T acc[kStripSlots][3];
+#pragma unroll
+  for (int q = 0; q < kStripSlots; ++q) acc[q][0] = acc[q][1] = acc[q][2] = (T)0;
+  int fl = sfbase[s];
+  for (int i = 0; i < kStripLen; ++i) {
+    const int t = __builtin_amdgcn_readfirstlane(stet[s * kStripLen + i]);
+    if (t < 0) break;
+    const int code = __builtin_amdgcn_readfirstlane(scode[s * kStripLen + i]);
+    T f[12];
+    tet_forces<T>(X, tv, tG, wr, wv, t, Tn, r, C, mptr, mg, mw, mgroup, Mn, act, act_stride_r, act_off, f);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int sl = (code >> (3 * k)) & 7;
+      // the empty asm keeps each add behind its (uniform) branch: speculated,
+      // the compiler would add every corner into every slot and select
+#pragma unroll
+      for (int q = 0; q < kStripSlots; ++q)
+        if (sl == q) {
+          asm volatile("");
+          strip_add<T>(acc[q], f + 3 * k);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < kStripSlots; ++q)
+      if ((code >> (12 + q)) & 1) {
+        const int p = __builtin_amdgcn_readfirstlane(ssum_of[fl++]);
+        T* o = ssum + (long long)p * C + 3 * r;
+        o[0] = acc[q][0];
+        o[1] = acc[q][1];
+        o[2] = acc[q][2];
+        acc[q][0] = acc[q][1] = acc[q][2] = (T)0;
+      }
+  }
+}
+
+// Strip sums from the dense variant's corner slots, each the sum of its
+// corner slots in the strip kernel's order, starting from zero as the strip
+// kernel's slots do. Runs exactly when the dense variant does, on a
+// wave-uniform scene.
+template <typename T>
+__global__ __launch_bounds__(256) void k_strip_fold(const T* __restrict__ corner, const int* __restrict__ sfptr,
+                                                    const int* __restrict__ sfq, const int* __restrict__ alist,
+                                                    const int* __restrict__ acount, T* __restrict__ ssum, int nsum,
+                                                    int R) {
+  const int A = *acount;
+  if (A >= kDenseBelow || A == 0) return;
+  const long long idx = (long long)xcd_block() * blockDim.x + threadIdx.x;
+  const long long n = (long long)nsum * 3 * A;
+  if (idx >= n) return;
+  const int p = (int)(idx / (3 * A));
+  const int jj = (int)(idx - (long long)p * 3 * A);
+  const int r3 = jj / 3;
+  const long long C = 3LL * R;
+  const int col = 3 * alist[r3] + (jj - 3 * r3);
+  T a = (T)0;
+  for (int k = sfptr[p], ke = sfptr[p + 1]; k < ke; ++k) a = a + corner[(long long)sfq[k] * C + col];
+  ssum[(long long)p * C + col] = a;
 }
 
 // Segmented gather over the listed rollouts only: thread (vertex, jj) sums
@@ -3253,6 +3359,7 @@ struct pdm_scene_s {
   Csr Ls;
   Factor F;
   Corners Cn;
+  Strips Sp;
   hipStream_t stream = nullptr;
   bool own_stream = false;
   int R = 1, C = 3, precision = PDM_FP64, flags = PDM_HOST_EARLY_EXIT, ckpt = 20;
@@ -3295,6 +3402,9 @@ struct pdm_scene_s {
   // dynamic
   DevBuf<double> X, V, Xn, Cc, Xit, B, Lx, U, Ys, part, err;  // Ys: y of the grouped forward sweep
   DevBuf<char> corner;
+  // strip sums of the forward local step (wave-uniform scenes, R % 64 == 0)
+  DevBuf<char> ssum;
+  DevBuf<int> stet, scode, sfbase, ssum_of, ssptr, sfptr, sfq;
   DevBuf<int> active, iters_dev;
   std::vector<int> iters_host;  // R*steps of the last forward
   int last_steps = 0;
@@ -3431,6 +3541,10 @@ struct pdm_scene_s {
                       f_relptr.p, f_rel.p, f_uptr.p};
   }
   size_t NC() const { return (size_t)S.N * C; }
+  // the forward local step's per-vertex sums: strip sums on wave-uniform
+  // scenes, corner slots otherwise (with their vertex ranges)
+  const void* fsum() const { return R % 64 == 0 ? (const void*)ssum.p : (const void*)corner.p; }
+  const int* fsum_ptr() const { return R % 64 == 0 ? ssptr.p : vptr.p; }
 };
 
 namespace {
@@ -3443,6 +3557,12 @@ int upload_static(pdm_scene s) {
   if (s->tG.upload(soa(S.tG, S.T, 12).data(), 12 * (size_t)S.T)) return 1;
   if (s->tslot.upload(soa(s->Cn.tet_slot, S.T, 4).data(), 4 * (size_t)S.T)) return 1;
   if (s->vptr.upload(s->Cn.vptr.data(), S.N + 1)) return 1;
+  const Strips& P = s->Sp;
+  if (s->stet.upload(P.tet.data(), P.tet.size()) || s->scode.upload(P.code.data(), P.code.size()) ||
+      s->sfbase.upload(P.fbase.data(), P.fbase.size()) || s->ssum_of.upload(P.sum_of.data(), P.sum_of.size()) ||
+      s->ssptr.upload(P.ssptr.data(), P.ssptr.size()) || s->sfptr.upload(P.fptr.data(), P.fptr.size()) ||
+      s->sfq.upload(P.fq.data(), P.fq.size()))
+    return 1;
   if (s->mass.upload(S.mass.data(), S.N)) return 1;
   std::vector<double> aw(S.N, 0.0);
   for (size_t k = 0; k < S.att_idx.size(); ++k) aw[S.att_idx[k]] += S.att_w[k];
@@ -4388,6 +4508,7 @@ int host_setup(const pdm_scene_desc* d, pdm_scene s) {
     return 1;
   }
   build_corners(S, s->Cn);
+  build_strips(S, s->Cn, s->Sp);
   return 0;
 }
 
@@ -4444,6 +4565,7 @@ int ensure_batch_buffers(pdm_scene s) {
   // PDM_MIXED_ADJOINT writes FP32 slots into the same (larger) buffer
   size_t esz = s->precision == PDM_MIXED ? sizeof(float) : sizeof(double);
   if (s->corner.alloc((size_t)s->Cn.nslot * s->C * esz)) return 1;
+  if (s->R % 64 == 0 && s->ssum.alloc((size_t)std::max(1, s->Sp.nsum) * s->C * esz)) return 1;
   if (s->part.alloc((size_t)cdiv(N, kChunk) * s->C)) return 1;
   if (s->err.alloc(s->R) || s->active.alloc(s->R) || s->iters_dev.alloc(s->R)) return 1;
   if (s->alist.alloc(s->R) || s->acount.alloc(1)) return 1;
@@ -4883,16 +5005,21 @@ void launch_local(pdm_scene s, const double* Xsrc, const double* act, int act_st
   // wave-uniform variant for kDenseBelow or more listed rollouts, dense one for fewer
   const bool uni = s->R % 64 == 0;
   if (uni)
-    hipLaunchKernelGGL((k_local_tet<T, true>), dim3(cdiv(nt, kThreads)), dim3(kThreads), 0, s->stream, Xsrc, s->tv.p,
-                       s->tG.p, s->wr.p, s->wv.p, s->tslot.p, s->alist.p, s->acount.p, corner, S.T, s->R, s->mptr.p,
+    hipLaunchKernelGGL((k_local_strip<T>), dim3(cdiv((long long)s->Sp.nstrip * s->R, kThreads)), dim3(kThreads), 0,
+                       s->stream, Xsrc, s->tv.p, s->tG.p, s->wr.p, s->wv.p, s->stet.p, s->scode.p, s->sfbase.p,
+                       s->ssum_of.p, s->alist.p, s->acount.p, (T*)s->ssum.p, S.T, s->Sp.nstrip, s->R, s->mptr.p,
                        s->mg.p, s->mw.p, s->mgroup.p, S.M, act, act_steps * G, step * G, step_dev, G);
-  hipLaunchKernelGGL((k_local_tet<T, false>), dim3(element_blocks(S.T, uni ? kDenseBelow - 1 : s->R)), dim3(kThreads), 0, s->stream,
+  hipLaunchKernelGGL((k_local_tet<T>), dim3(element_blocks(S.T, uni ? kDenseBelow - 1 : s->R)), dim3(kThreads), 0, s->stream,
                      Xsrc, s->tv.p, s->tG.p, s->wr.p, s->wv.p, s->tslot.p, s->alist.p, s->acount.p, corner, S.T, s->R,
                      s->mptr.p, s->mg.p, s->mw.p, s->mgroup.p, S.M, act, act_steps * G, step * G, step_dev, G);
+  if (uni)
+    hipLaunchKernelGGL((k_strip_fold<T>), dim3(cdiv((long long)s->Sp.nsum * 3 * (kDenseBelow - 1), kThreads)),
+                       dim3(kThreads), 0, s->stream, corner, s->sfptr.p, s->sfq.p, s->alist.p, s->acount.p,
+                       (T*)s->ssum.p, s->Sp.nsum, s->R);
   if (gather)
     hipLaunchKernelGGL((uni ? k_gather_list<T, 0, true> : k_gather_list<T, 0, false>),
-                       dim3(cdiv((long long)S.N * 3 * s->R, kThreads)), dim3(kThreads), 0, s->stream, s->Cc.p, corner,
-                       s->vptr.p, s->alist.p, s->acount.p, s->B.p, S.N, s->R, any);
+                       dim3(cdiv((long long)S.N * 3 * s->R, kThreads)), dim3(kThreads), 0, s->stream, s->Cc.p,
+                       (const T*)s->fsum(), s->fsum_ptr(), s->alist.p, s->acount.p, s->B.p, S.N, s->R, any);
 }
 
 
@@ -5032,7 +5159,7 @@ int contact_phase(pdm_scene s, const double* act, int act_steps, int step, Conta
                          tp && !self ? tp->xit.p + (size_t)(tp->G + 1) * nc : nullptr, self ? s->s_cur.p : nullptr,
                          self ? s->s_Fo.p : nullptr, self ? s->s_CN.p : nullptr, self ? s->s_RC.p : nullptr,
                          self ? s->s_ngrp.p : nullptr, fused ? s->Cc.p : nullptr,
-                         fused ? (const double*)s->corner.p : nullptr, fused ? s->vptr.p : nullptr);
+                         fused ? (const double*)s->fsum() : nullptr, fused ? s->fsum_ptr() : nullptr);
       if (self) {
         if (tp)
           hipLaunchKernelGGL(k_self_tape_F, dim3(cdiv((long long)s->ncs * C, kThreads)), dim3(kThreads), 0, s->stream,

@@ -106,6 +106,30 @@ struct Corners {
   std::vector<int> tet_slot;       // 4T: slot of (tet, corner)
 };
 
+// Tet strips of the wave-uniform local step. A wave walks the kStripLen tets
+// of a strip (face neighbours where the mesh allows) and keeps the running
+// corner-force sums of kStripSlots vertices in registers; when a vertex leaves
+// them (no later use in the strip, or evicted for a vertex needed sooner) its
+// sum is written once, as one "strip sum". A vertex's strip sums are stored
+// contiguously (vptr-like ssptr), so the per-vertex gathers read them as they
+// read corner slots, about a quarter as many on a structured mesh. The
+// dense element kernel writes corner slots instead and k_strip_fold forms the
+// same sums from them in the same order, so a vertex's force does not depend
+// on which variant ran.
+constexpr int kStripLen = 12;
+constexpr int kStripSlots = 6;
+struct Strips {
+  int nstrip = 0, nsum = 0;
+  std::vector<int> tet;     // nstrip * kStripLen tet ids, -1 past a short strip's end
+  std::vector<int> code;    // per position: the corners' slots (3 bits each) | flush mask << 12
+  std::vector<int> fbase;   // nstrip: first entry of each strip's flushes in sum_of
+  std::vector<int> sum_of;  // per flush, in strip order (position, then slot): its strip sum
+  std::vector<int> ssptr;   // N + 1: strip sums of each vertex
+  std::vector<int> fptr;    // nsum + 1: corner slots summed into each strip sum
+  std::vector<int> fq;      // 4T corner slots, in accumulation order
+};
+void build_strips(const HostScene& S, const Corners& C, Strips& P);
+
 // Build the engine scene from caller arrays (see pdm.h for the meaning).
 bool build_scene(HostScene& S, int N, const double* xrest, int T, const int* tets, int mass_same, double total_mass,
                  std::string& err);

@@ -824,6 +824,147 @@ void build_corners(const HostScene& S, Corners& C) {
     for (int k = 0; k < 4; ++k) C.tet_slot[4 * t + k] = fill[S.tv[4 * t + k]]++;
 }
 
+void build_strips(const HostScene& S, const Corners& C, Strips& P) {
+  const int T = S.T, N = S.N, L = kStripLen, K = kStripSlots;
+  // vertex -> tets
+  std::vector<int> vtp(N + 1, 0), vtl(4 * (size_t)T);
+  for (int q = 0; q < 4 * T; ++q) vtp[S.tv[q] + 1]++;
+  for (int i = 0; i < N; ++i) vtp[i + 1] += vtp[i];
+  {
+    std::vector<int> fill(vtp.begin(), vtp.end() - 1);
+    for (int q = 0; q < 4 * T; ++q) vtl[fill[S.tv[q]]++] = q / 4;
+  }
+  // Greedy strips: start at the first unused tet (engine order), then extend
+  // to the unused tet sharing the most vertices with the last two tets
+  // (lowest id on ties; the next unused tet when no neighbour is left).
+  std::vector<char> used(T, 0);
+  std::vector<int> score(T, 0), touched;
+  P.tet.clear();
+  int next0 = 0;
+  while (true) {
+    while (next0 < T && used[next0]) ++next0;
+    if (next0 >= T) break;
+    int cur = next0;
+    for (int i = 0; i < L; ++i) {
+      used[cur] = 1;
+      P.tet.push_back(cur);
+      if (i + 1 == L) break;
+      int prev = i > 0 ? P.tet[P.tet.size() - 2] : -1;
+      int best = -1;
+      for (int k = 0; k < 4; ++k) {
+        int v = S.tv[4 * cur + k];
+        for (int a = vtp[v]; a < vtp[v + 1]; ++a) {
+          int u = vtl[a];
+          if (used[u]) continue;
+          if (score[u] == 0) touched.push_back(u);
+          score[u] += 2;  // shares v with the current tet
+        }
+      }
+      if (prev >= 0)
+        for (int u : touched)
+          for (int k = 0; k < 4; ++k)
+            for (int m = 0; m < 4; ++m)
+              if (S.tv[4 * u + k] == S.tv[4 * prev + m]) score[u] += 1;
+      for (int u : touched)
+        if (best < 0 || score[u] > score[best] || (score[u] == score[best] && u < best)) best = u;
+      for (int u : touched) score[u] = 0;
+      touched.clear();
+      if (best < 0) {
+        while (next0 < T && used[next0]) ++next0;
+        if (next0 >= T) break;
+        best = next0;
+      }
+      cur = best;
+    }
+    while (P.tet.size() % L) P.tet.push_back(-1);
+  }
+  P.nstrip = (int)(P.tet.size() / L);
+  P.code.assign(P.tet.size(), 0);
+  P.fbase.assign(P.nstrip, 0);
+  // Register slots per strip (farthest next use is evicted); a flush ends the
+  // sum of a vertex in its slot. Flushes are recorded as (vertex, order).
+  struct Flush {
+    int v, pos, slot;
+    std::vector<int> q;
+  };
+  std::vector<Flush> fl;
+  fl.reserve(T);
+  for (int s = 0; s < P.nstrip; ++s) {
+    P.fbase[s] = (int)fl.size();
+    const int* st = &P.tet[(size_t)s * L];
+    int len = 0;
+    while (len < L && st[len] >= 0) ++len;
+    int sv[kStripSlots];
+    std::vector<int> sq[kStripSlots];
+    for (int k = 0; k < K; ++k) sv[k] = -1;
+    auto next_use = [&](int v, int from) {
+      for (int i = from; i < len; ++i)
+        for (int k = 0; k < 4; ++k)
+          if (S.tv[4 * st[i] + k] == v) return i;
+      return L + 1;
+    };
+    auto flush = [&](int pos, int k) {
+      P.code[(size_t)s * L + pos] |= 1 << (12 + k);
+      fl.push_back(Flush{sv[k], pos, k, std::move(sq[k])});
+      sq[k].clear();
+      sv[k] = -1;
+    };
+    for (int i = 0; i < len; ++i) {
+      const int t = st[i];
+      int slot[4];
+      for (int c = 0; c < 4; ++c) {
+        const int v = S.tv[4 * t + c];
+        int k = 0;
+        while (k < K && sv[k] != v) ++k;
+        if (k == K) {
+          k = 0;
+          while (k < K && sv[k] >= 0) ++k;
+        }
+        if (k == K) {  // evict the vertex used farthest ahead (never by this tet)
+          int far = -1, fu = -1;
+          for (int e = 0; e < K; ++e) {
+            int u = next_use(sv[e], i);
+            if (u > fu) fu = u, far = e;
+          }
+          flush(i - 1, far);  // i > 0: a slot is only full after a tet
+          k = far;
+        }
+        sv[k] = v;
+        slot[c] = k;
+        sq[k].push_back(C.tet_slot[4 * t + c]);
+      }
+      P.code[(size_t)s * L + i] |= slot[0] | slot[1] << 3 | slot[2] << 6 | slot[3] << 9;
+      for (int k = 0; k < K; ++k)
+        if (sv[k] >= 0 && next_use(sv[k], i + 1) > L) flush(i, k);
+    }
+    // the kernel flushes position by position, slots in ascending order
+    std::stable_sort(fl.begin() + P.fbase[s], fl.end(), [](const Flush& a, const Flush& b) {
+      return a.pos < b.pos || (a.pos == b.pos && a.slot < b.slot);
+    });
+  }
+  // strip sums numbered vertex by vertex (flush order within a vertex)
+  const int nf = (int)fl.size();
+  P.nsum = nf;
+  P.ssptr.assign(N + 1, 0);
+  for (const Flush& f : fl) P.ssptr[f.v + 1]++;
+  for (int i = 0; i < N; ++i) P.ssptr[i + 1] += P.ssptr[i];
+  std::vector<int> fill(P.ssptr.begin(), P.ssptr.end() - 1);
+  P.sum_of.assign(nf, 0);
+  std::vector<int> owner(nf);
+  for (int e = 0; e < nf; ++e) {
+    P.sum_of[e] = fill[fl[e].v]++;
+    owner[P.sum_of[e]] = e;
+  }
+  P.fptr.assign(nf + 1, 0);
+  P.fq.clear();
+  P.fq.reserve(4 * (size_t)T);
+  for (int p = 0; p < nf; ++p) {
+    const Flush& f = fl[owner[p]];
+    P.fq.insert(P.fq.end(), f.q.begin(), f.q.end());
+    P.fptr[p + 1] = (int)P.fq.size();
+  }
+}
+
 // Per-segment tet selection of Muscle::Initialize (Muscle.h:109-215):
 // tets the segment crosses, tets whose centre lies within exp(-d/0.005) > 0.2,
 // plus the tets containing the segment's start and end (added again).

@@ -541,6 +541,45 @@ def test_batch64_drop_matches_oracle(eng, oracle_mod, precision):
     assert rel(g["E"], gE_sum) < (tol_g if not kinked else 1e-2)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_element_variants_agree_bitwise(eng, precision):
+    """A rollout's forward does not depend on which element kernels ran for
+    it. With 24 or more rollouts iterating, the strip kernel adds the corner
+    forces into per-vertex strip sums held in registers; with fewer, the dense
+    kernel writes corner slots and k_strip_fold sums them in the same order.
+    In the 64-rollout drop each rollout passes through both; 64 copies of one
+    rollout keep all 64 iterating together (strip kernel only). States and
+    PD and contact iteration counts must be bitwise equal."""
+    d = _drop_batch_scene()
+    R, steps = 64, 12
+    prec = {"fp64": eng.PDM_FP64, "mixed": eng.PDM_MIXED}[precision]
+    flags = eng.PDM_HOST_EARLY_EXIT | eng.PDM_USE_GRAPH
+    rng = np.random.default_rng(65)
+    x0 = np.tile(d.x_rest.ravel(), (R, 1))
+    v0 = np.zeros((R, 3 * d.N))
+    for r in range(R):
+        u = np.array([rng.uniform(-0.3, 0.3), rng.uniform(-0.3, 0.3), rng.uniform(-1.2, 1.0)])
+        v0[r] = np.tile(u, d.N) + 0.02 * rng.standard_normal(3 * d.N)
+    act = rng.uniform(0, 1, (R, steps, d.n_groups))
+    sa = eng.Scene(d, R=R, precision=prec, flags=flags)
+    xa, va = sa.forward_host(x0, v0, steps, act=act)
+    its = sa.iterations()
+    jac = sa.contact_stats()[1]
+    sa.close()
+    live = np.array([[(its[:, n] > k).sum() for k in range(d.max_iter)] for n in range(steps)])
+    dense = np.array([sum(int((live[n, :its[r, n]] < 24).sum()) for n in range(steps)) for r in range(R)])
+    rs = int(np.argmax(dense))
+    assert dense[rs] > 0
+    sb = eng.Scene(d, R=R, precision=prec, flags=flags)
+    xb, vb = sb.forward_host(np.tile(x0[rs], (R, 1)), np.tile(v0[rs], (R, 1)), steps, act=np.tile(act[rs], (R, 1, 1)))
+    itb = sb.iterations()
+    jb = sb.contact_stats()[1]
+    sb.close()
+    assert np.array_equal(itb[0], its[rs]) and np.array_equal(jb[0], jac[rs])
+    assert np.array_equal(xb[0], xa[rs]) and np.array_equal(vb[0], va[rs])
+    assert np.array_equal(xb, np.tile(xb[0], (R, 1)))
+
+
 def test_mixed_adjoint_full_horizon_all_rollouts(eng, oracle_mod):
     """60 steps of a muscle box bouncing on a friction floor, 64 rollouts:
     the mixed-adjoint forward equals the FP64 forward bitwise (positions and
