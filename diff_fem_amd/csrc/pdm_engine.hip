@@ -267,38 +267,55 @@ inline void launch_compact(const int* active, int R, int* alist, int* acount, hi
 //     item = t A + j, lanes span several tets. Per item it costs about three
 //     times the wave-uniform variant, so it pays only while most lanes of the
 //     wave-uniform one would idle.
-// A launch of the variant that does not apply exits after one load.
+// A launch of the variant that does not apply exits after one load. On
+// wave-uniform scenes the element kernels walk tet strips (k_local_strip)
+// or tets (k_vjp_tet<., true>) with their items numbered per strip or tet.
 constexpr int kDenseBelow = 24;
-// Blocks of a launch: the wave-uniform variant covers T P <= T R items, the
-// dense one T (kDenseBelow - 1) items, or T R when it runs alone.
-inline long long element_blocks(long long Tn, int amax) { return (Tn * amax + 255) / 256; }
 
-// Item of this thread: (tet t, listed rollout j); false for idle lanes and for
-// the variant that does not apply. 32-bit: the host keeps T R below 2^31.
-template <bool UNI>
-__device__ __forceinline__ bool element_item(const int* __restrict__ acount, int Tn, int R, int* t, int* j) {
+// Wave-uniform item of this thread: (unit u = tet or strip, listed rollout
+// j); false for idle lanes and below kDenseBelow listed rollouts. 32-bit: the
+// host keeps T R below 2^31.
+__device__ __forceinline__ bool uniform_item(const int* __restrict__ acount, int Un, int* u, int* j) {
   const int A = *acount;
-  if (UNI) {
-    if (A < kDenseBelow) return false;
-    const unsigned P = (unsigned)(A + 63) & ~63u;
-    const unsigned n = (unsigned)Tn * P;
-    const int b = xcd_block_n((int)((n + blockDim.x - 1) / blockDim.x));
-    if (b < 0) return false;
-    const unsigned idx = (unsigned)b * blockDim.x + threadIdx.x;
-    if (idx >= n) return false;
-    *t = __builtin_amdgcn_readfirstlane((int)(idx / P));
-    *j = (int)(idx - (unsigned)*t * P);
-    return *j < A;
-  }
-  if ((R % 64 == 0 && A >= kDenseBelow) || A == 0) return false;
-  const unsigned n = (unsigned)Tn * (unsigned)A;
+  if (A < kDenseBelow) return false;
+  const unsigned P = (unsigned)(A + 63) & ~63u;
+  const unsigned n = (unsigned)Un * P;
   const int b = xcd_block_n((int)((n + blockDim.x - 1) / blockDim.x));
   if (b < 0) return false;
   const unsigned idx = (unsigned)b * blockDim.x + threadIdx.x;
   if (idx >= n) return false;
+  *u = __builtin_amdgcn_readfirstlane((int)(idx / P));
+  *j = (int)(idx - (unsigned)*u * P);
+  return *j < A;
+}
+
+// Dense item of virtual block vb (a grid smaller than the work loops over
+// virtual blocks vb = blockIdx.x + m gridDim.x; gridDim.x is a multiple of 8 or
+// covers the work, so vb stays on the hardware block's XCD): 1 with (t, j),
+// 0 for a thread past the work in a block that has some, -1 past the work or
+// when the dense variant does not apply.
+__device__ __forceinline__ int dense_item(const int* __restrict__ acount, int Tn, int R, int vb, int* t, int* j) {
+  const int A = *acount;
+  if ((R % 64 == 0 && A >= kDenseBelow) || A == 0) return -1;
+  const unsigned n = (unsigned)Tn * (unsigned)A;
+  const int nb = (int)((n + blockDim.x - 1) / blockDim.x);
+  if (vb >= nb) return -1;
+  const int q = nb >> 3, rem = nb & 7, k = vb & 7;
+  const int b = k * q + min(k, rem) + (vb >> 3);
+  const unsigned idx = (unsigned)b * blockDim.x + threadIdx.x;
+  if (idx >= n) return 0;
   *t = (int)(idx / (unsigned)A);
   *j = (int)(idx - (unsigned)*t * (unsigned)A);
-  return true;
+  return 1;
+}
+
+// Blocks of a dense element launch: on wave-uniform scenes the dense variant
+// runs only below kDenseBelow listed rollouts and its launch is mostly empty,
+// so the grid is capped (a multiple of 8) and the blocks loop.
+constexpr int kDenseGridCap = 1024;
+inline int dense_blocks(long long Tn, int R) {
+  const long long nb = R % 64 == 0 ? (Tn * (kDenseBelow - 1) + 255) / 256 : (Tn * R + 255) / 256;
+  return R % 64 == 0 ? (int)std::min<long long>(nb, kDenseGridCap) : (int)nb;
 }
 
 // The wave-uniform item of this thread with the wave's idle lanes kept
@@ -385,18 +402,22 @@ __global__ __launch_bounds__(256) void k_local_tet(const double* __restrict__ X,
                                                    const double* __restrict__ mw, const int* __restrict__ mgroup,
                                                    int Mn, const double* __restrict__ act, int act_stride_r,
                                                    int act_off, const int* __restrict__ step_dev, int ngroups) {
-  int t, j;
-  if (!element_item<false>(acount, Tn, R, &t, &j)) return;
-  const int r = alist[j];
   if (step_dev) act_off = *step_dev * ngroups;
   const int C = 3 * R;
-  T f[12];
-  tet_forces<T>(X, tv, tG, wr, wv, t, Tn, r, C, mptr, mg, mw, mgroup, Mn, act, act_stride_r, act_off, f);
+  for (int vb = blockIdx.x;; vb += gridDim.x) {
+    int t, j;
+    const int st = dense_item(acount, Tn, R, vb, &t, &j);
+    if (st < 0) return;
+    if (st == 0) continue;
+    const int r = alist[j];
+    T f[12];
+    tet_forces<T>(X, tv, tG, wr, wv, t, Tn, r, C, mptr, mg, mw, mgroup, Mn, act, act_stride_r, act_off, f);
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    T* o = corner + (long long)tslot[k * Tn + t] * C + 3 * r;
+    for (int k = 0; k < 4; ++k) {
+      T* o = corner + (long long)tslot[k * Tn + t] * C + 3 * r;
 #pragma unroll
-    for (int rr = 0; rr < 3; ++rr) o[rr] = f[3 * k + rr];
+      for (int rr = 0; rr < 3; ++rr) o[rr] = f[3 * k + rr];
+    }
   }
 }
 
@@ -429,7 +450,7 @@ __global__ __launch_bounds__(256) void k_local_strip(const double* __restrict__ 
                                                      const double* __restrict__ act, int act_stride_r, int act_off,
                                                      const int* __restrict__ step_dev, int ngroups) {
   int s, j;
-  if (!element_item<true>(acount, nstrip, R, &s, &j)) return;
+  if (!uniform_item(acount, nstrip, &s, &j)) return;
   const int r = alist[j];
   if (step_dev) act_off = *step_dev * ngroups;
   const int C = 3 * R;
@@ -479,17 +500,18 @@ __global__ __launch_bounds__(256) void k_strip_fold(const T* __restrict__ corner
                                                     int R) {
   const int A = *acount;
   if (A >= kDenseBelow || A == 0) return;
-  const long long idx = (long long)xcd_block() * blockDim.x + threadIdx.x;
   const long long n = (long long)nsum * 3 * A;
-  if (idx >= n) return;
-  const int p = (int)(idx / (3 * A));
-  const int jj = (int)(idx - (long long)p * 3 * A);
-  const int r3 = jj / 3;
   const long long C = 3LL * R;
-  const int col = 3 * alist[r3] + (jj - 3 * r3);
-  T a = (T)0;
-  for (int k = sfptr[p], ke = sfptr[p + 1]; k < ke; ++k) a = a + corner[(long long)sfq[k] * C + col];
-  ssum[(long long)p * C + col] = a;
+  for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
+       idx += (long long)gridDim.x * blockDim.x) {
+    const int p = (int)(idx / (3 * A));
+    const int jj = (int)(idx - (long long)p * 3 * A);
+    const int r3 = jj / 3;
+    const int col = 3 * alist[r3] + (jj - 3 * r3);
+    T a = (T)0;
+    for (int k = sfptr[p], ke = sfptr[p + 1]; k < ke; ++k) a = a + corner[(long long)sfq[k] * C + col];
+    ssum[(long long)p * C + col] = a;
+  }
 }
 
 // Segmented gather over the listed rollouts only: thread (vertex, jj) sums
@@ -1364,7 +1386,7 @@ __global__ __launch_bounds__(256, (sizeof(T) == 4 && UNI) ? 4 : 1) void k_vjp_te
   bool live = true;
   if (UNI) {
     if (!element_wave(acount, Tn, &t, &j, &live)) return;
-  } else if (!element_item<false>(acount, Tn, R, &t, &j)) {
+  } else if (dense_item(acount, Tn, R, blockIdx.x, &t, &j) <= 0) {
     return;
   }
   const int r = alist[j];
@@ -2586,9 +2608,10 @@ __global__ void k_contact_jacobi(const int* __restrict__ Lptr, const int* __rest
   long long base = (long long)v * C + 3 * r;
   double rc[3];
   if (corner) {
-    // c + J d gathered here from the FP64 corner slots (the gather's sums:
-    // slot k of the range to partial sum k mod 4 while four remain, the rest
-    // to the first, then (s0 + s1) + (s2 + s3))
+    // c + J d gathered here from the local step's FP64 per-vertex sums
+    // (strip sums or corner slots; the gather's order: entry k of the range
+    // to partial sum k mod 4 while four remain, the rest to the first, then
+    // (s0 + s1) + (s2 + s3))
     double sp[3][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};
     int q = vptr[v];
     const int qe = vptr[v + 1];
@@ -5009,11 +5032,12 @@ void launch_local(pdm_scene s, const double* Xsrc, const double* act, int act_st
                        s->stream, Xsrc, s->tv.p, s->tG.p, s->wr.p, s->wv.p, s->stet.p, s->scode.p, s->sfbase.p,
                        s->ssum_of.p, s->alist.p, s->acount.p, (T*)s->ssum.p, S.T, s->Sp.nstrip, s->R, s->mptr.p,
                        s->mg.p, s->mw.p, s->mgroup.p, S.M, act, act_steps * G, step * G, step_dev, G);
-  hipLaunchKernelGGL((k_local_tet<T>), dim3(element_blocks(S.T, uni ? kDenseBelow - 1 : s->R)), dim3(kThreads), 0, s->stream,
+  hipLaunchKernelGGL((k_local_tet<T>), dim3(dense_blocks(S.T, s->R)), dim3(kThreads), 0, s->stream,
                      Xsrc, s->tv.p, s->tG.p, s->wr.p, s->wv.p, s->tslot.p, s->alist.p, s->acount.p, corner, S.T, s->R,
                      s->mptr.p, s->mg.p, s->mw.p, s->mgroup.p, S.M, act, act_steps * G, step * G, step_dev, G);
   if (uni)
-    hipLaunchKernelGGL((k_strip_fold<T>), dim3(cdiv((long long)s->Sp.nsum * 3 * (kDenseBelow - 1), kThreads)),
+    hipLaunchKernelGGL((k_strip_fold<T>),
+                       dim3(std::min(cdiv((long long)s->Sp.nsum * 3 * (kDenseBelow - 1), kThreads), kDenseGridCap)),
                        dim3(kThreads), 0, s->stream, corner, s->sfptr.p, s->sfq.p, s->alist.p, s->acount.p,
                        (T*)s->ssum.p, s->Sp.nsum, s->R);
   if (gather)
@@ -5433,7 +5457,8 @@ int vjp_iteration(pdm_scene s, const int* active, const double* Yk, const double
                        s->acount.p, corner, s->gEpart.p, S.T, s->R, CDp ? Xn : nullptr, CDp, s->mptr.p, s->mg.p,
                        s->mw.p, s->mgroup.p, S.M, act, act_steps * S.n_groups, step * S.n_groups,
                        S.M > 0 ? s->gMpart.p : nullptr, s->gEw.p);
-  hipLaunchKernelGGL((k_vjp_tet<T, false>), dim3(element_blocks(S.T, uni ? kDenseBelow - 1 : s->R)), dim3(kThreads), 0, s->stream,
+  hipLaunchKernelGGL((k_vjp_tet<T, false>), dim3(cdiv((long long)S.T * (uni ? kDenseBelow - 1 : s->R), kThreads)),
+                     dim3(kThreads), 0, s->stream,
                      Yk, Yk1, s->Zb.p, s->tv.p, s->tG.p, s->wr.p, s->wv.p, s->dEr.p, s->dEv.p, s->tslot.p, s->alist.p,
                      s->acount.p, corner, s->gEpart.p, S.T, s->R, CDp ? Xn : nullptr, CDp, s->mptr.p, s->mg.p,
                      s->mw.p, s->mgroup.p, S.M, act, act_steps * S.n_groups, step * S.n_groups,

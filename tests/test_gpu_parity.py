@@ -541,18 +541,20 @@ def test_batch64_drop_matches_oracle(eng, oracle_mod, precision):
     assert rel(g["E"], gE_sum) < (tol_g if not kinked else 1e-2)
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "mixed_adjoint"])
 def test_element_variants_agree_bitwise(eng, precision):
     """A rollout's forward does not depend on which element kernels ran for
     it. With 24 or more rollouts iterating, the strip kernel adds the corner
     forces into per-vertex strip sums held in registers; with fewer, the dense
     kernel writes corner slots and k_strip_fold sums them in the same order.
     In the 64-rollout drop each rollout passes through both; 64 copies of one
-    rollout keep all 64 iterating together (strip kernel only). States and
-    PD and contact iteration counts must be bitwise equal."""
+    rollout keep all 64 iterating together (strip kernel only). The reverse
+    sweep's wave-uniform and dense VJP kernels must agree in the same way.
+    States, PD and contact iteration counts and the rollout's gradients (x0,
+    v0, activations) must be bitwise equal."""
     d = _drop_batch_scene()
     R, steps = 64, 12
-    prec = {"fp64": eng.PDM_FP64, "mixed": eng.PDM_MIXED}[precision]
+    prec = {"fp64": eng.PDM_FP64, "mixed": eng.PDM_MIXED, "mixed_adjoint": eng.PDM_MIXED_ADJOINT}[precision]
     flags = eng.PDM_HOST_EARLY_EXIT | eng.PDM_USE_GRAPH
     rng = np.random.default_rng(65)
     x0 = np.tile(d.x_rest.ravel(), (R, 1))
@@ -561,23 +563,30 @@ def test_element_variants_agree_bitwise(eng, precision):
         u = np.array([rng.uniform(-0.3, 0.3), rng.uniform(-0.3, 0.3), rng.uniform(-1.2, 1.0)])
         v0[r] = np.tile(u, d.N) + 0.02 * rng.standard_normal(3 * d.N)
     act = rng.uniform(0, 1, (R, steps, d.n_groups))
-    sa = eng.Scene(d, R=R, precision=prec, flags=flags)
-    xa, va = sa.forward_host(x0, v0, steps, act=act)
+    gx = rng.standard_normal((R, 3 * d.N))
+    gv = 0.01 * rng.standard_normal((R, 3 * d.N))
+    sa = eng.Scene(d, R=R, precision=prec, flags=flags, checkpoint_every=steps)
+    xa, va = sa.forward_host(x0, v0, steps, act=act, record=True)
     its = sa.iterations()
     jac = sa.contact_stats()[1]
+    ga = sa.backward_host(gx, gv)
     sa.close()
     live = np.array([[(its[:, n] > k).sum() for k in range(d.max_iter)] for n in range(steps)])
     dense = np.array([sum(int((live[n, :its[r, n]] < 24).sum()) for n in range(steps)) for r in range(R)])
     rs = int(np.argmax(dense))
     assert dense[rs] > 0
-    sb = eng.Scene(d, R=R, precision=prec, flags=flags)
-    xb, vb = sb.forward_host(np.tile(x0[rs], (R, 1)), np.tile(v0[rs], (R, 1)), steps, act=np.tile(act[rs], (R, 1, 1)))
+    sb = eng.Scene(d, R=R, precision=prec, flags=flags, checkpoint_every=steps)
+    xb, vb = sb.forward_host(np.tile(x0[rs], (R, 1)), np.tile(v0[rs], (R, 1)), steps, act=np.tile(act[rs], (R, 1, 1)),
+                             record=True)
     itb = sb.iterations()
     jb = sb.contact_stats()[1]
+    gb = sb.backward_host(np.tile(gx[rs], (R, 1)), np.tile(gv[rs], (R, 1)))
     sb.close()
     assert np.array_equal(itb[0], its[rs]) and np.array_equal(jb[0], jac[rs])
     assert np.array_equal(xb[0], xa[rs]) and np.array_equal(vb[0], va[rs])
     assert np.array_equal(xb, np.tile(xb[0], (R, 1)))
+    for k in ("x0", "v0", "act"):
+        assert np.array_equal(gb[k][0], ga[k][rs]), k
 
 
 def test_mixed_adjoint_full_horizon_all_rollouts(eng, oracle_mod):
