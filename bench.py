@@ -517,7 +517,7 @@ def main():
         plain_pass()  # warm-up: buffers, solve autotuning, first launches
         torch.cuda.synchronize()
         pt, pit = [], []
-        for _ in range(2):
+        for _ in range(6):  # a pass is about 0.2 s and its host round trips jitter
             t0 = time.perf_counter()
             plain_pass()
             torch.cuda.synchronize()
@@ -525,7 +525,8 @@ def main():
             pit.append(d.T * float(pl.iterations().sum()))
         pv = sum(pit) / sum(pt)
         out["plain_fp64"] = {"value": pv, "unit": "tet-iterations/s", "rollouts": 1, "sim_steps": ps,
-                             "host_early_exit": True, "passes_timed": len(pt), "warmup": 1}
+                             "host_early_exit": True, "passes_timed": len(pt), "warmup": 1,
+                             "pass_s_min": min(pt), "pass_s_max": max(pt)}
         out["speedup_vs_plain_fp64"] = value / pv
         if "fp64_batch" in out:
             out["fp64_batch"]["speedup_vs_plain_fp64"] = out["fp64_batch"]["value"] / pv

@@ -391,8 +391,9 @@ __device__ __forceinline__ void tet_forces(const double* __restrict__ X, const i
 }
 
 // Dense element variant: thread (tet, listed rollout), corner forces into the
-// tet's corner slots.
-template <typename T>
+// tet's corner slots. LOOP: a capped grid whose blocks loop over the virtual
+// blocks (wave-uniform scenes); otherwise one item per thread.
+template <typename T, bool LOOP>
 __global__ __launch_bounds__(256) void k_local_tet(const double* __restrict__ X, const int* __restrict__ tv,
                                                    const double* __restrict__ tG, const double* __restrict__ wr,
                                                    const double* __restrict__ wv, const int* __restrict__ tslot,
@@ -407,7 +408,7 @@ __global__ __launch_bounds__(256) void k_local_tet(const double* __restrict__ X,
   for (int vb = blockIdx.x;; vb += gridDim.x) {
     int t, j;
     const int st = dense_item(acount, Tn, R, vb, &t, &j);
-    if (st < 0) return;
+    if (st < 0 || (!LOOP && st == 0)) return;
     if (st == 0) continue;
     const int r = alist[j];
     T f[12];
@@ -418,6 +419,7 @@ __global__ __launch_bounds__(256) void k_local_tet(const double* __restrict__ X,
 #pragma unroll
       for (int rr = 0; rr < 3; ++rr) o[rr] = f[3 * k + rr];
     }
+    if (!LOOP) return;
   }
 }
 
@@ -5032,7 +5034,7 @@ void launch_local(pdm_scene s, const double* Xsrc, const double* act, int act_st
                        s->stream, Xsrc, s->tv.p, s->tG.p, s->wr.p, s->wv.p, s->stet.p, s->scode.p, s->sfbase.p,
                        s->ssum_of.p, s->alist.p, s->acount.p, (T*)s->ssum.p, S.T, s->Sp.nstrip, s->R, s->mptr.p,
                        s->mg.p, s->mw.p, s->mgroup.p, S.M, act, act_steps * G, step * G, step_dev, G);
-  hipLaunchKernelGGL((k_local_tet<T>), dim3(dense_blocks(S.T, s->R)), dim3(kThreads), 0, s->stream,
+  hipLaunchKernelGGL((uni ? k_local_tet<T, true> : k_local_tet<T, false>), dim3(dense_blocks(S.T, s->R)), dim3(kThreads), 0, s->stream,
                      Xsrc, s->tv.p, s->tG.p, s->wr.p, s->wv.p, s->tslot.p, s->alist.p, s->acount.p, corner, S.T, s->R,
                      s->mptr.p, s->mg.p, s->mw.p, s->mgroup.p, S.M, act, act_steps * G, step * G, step_dev, G);
   if (uni)
