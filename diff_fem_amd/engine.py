@@ -334,6 +334,12 @@ class SplitScene:
         self.offsets = np.concatenate([[0], np.cumsum(sizes)]).astype(int)
         self.parts = [Scene(desc, R=r, precision=precision, flags=flags, checkpoint_every=checkpoint_every)
                       for r in sizes]
+        # Batch buffers and the solve autotuning now, one part after the other:
+        # their set-up copies and fills go through the null stream, which must
+        # not happen in one part's thread while another part's blocking stream
+        # captures its step graph.
+        for p in self.parts:
+            p.solve_shapes()
         self.R, self.N, self.T, self.n_groups = int(R), int(desc.N), int(desc.T), int(desc.n_groups)
         self._steps = 0
 
@@ -372,14 +378,17 @@ class SplitScene:
     def set_material(self, E=None, nu_fwd=None):
         for p in self.parts:
             p.set_material(E, nu_fwd)
+            p.solve_shapes()  # any re-planning here, not in the parts' threads
 
     def set_dense_top(self, max_rows: int):
         for p in self.parts:
             p.set_dense_top(max_rows)
+            p.solve_shapes()
 
     def set_dense_leaves(self, max_rows: int):
         for p in self.parts:
             p.set_dense_leaves(max_rows)
+            p.solve_shapes()
 
     def forward_device(self, x0, v0, steps: int, act=None, xT=None, vT=None, record: bool = False):
         import torch
